@@ -1,0 +1,281 @@
+// extern "C" entry points of libat3d_hip.so (declared in include/at3d_hip.h).
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <new>
+#include <string>
+
+#include "../../include/at3d_hip.h"
+#include "host_plan.h"
+#include "solver_state.h"
+
+struct at3d_plan {
+  at3d::Plan p;
+};
+struct at3d_state {
+  at3d::State s;
+};
+
+namespace {
+void set_err(char* err, size_t errlen, const std::string& msg) {
+  if (!err || errlen == 0) return;
+  size_t n = msg.size() < errlen - 1 ? msg.size() : errlen - 1;
+  memcpy(err, msg.data(), n);
+  err[n] = 0;
+}
+
+template <typename T>
+int copy_out(const std::vector<T>& v, void* dst, size_t nbytes, char* err, size_t errlen) {
+  const size_t need = v.size() * sizeof(T);
+  if (nbytes < need) {
+    set_err(err, errlen, "destination buffer too small: need " + std::to_string(need) + " bytes");
+    return AT3D_ERR_ARG;
+  }
+  if (need) memcpy(dst, v.data(), need);
+  return AT3D_OK;
+}
+}  // namespace
+
+extern "C" {
+
+const char* at3d_version(void) { return "at3d_hip 0.1 (gfx950)"; }
+
+int at3d_plan_create(const at3d_scene_desc* desc, at3d_plan** out, char* err, size_t errlen) {
+  if (!desc || !out) {
+    set_err(err, errlen, "null argument");
+    return AT3D_ERR_ARG;
+  }
+  *out = nullptr;
+  at3d_plan* h = new (std::nothrow) at3d_plan;
+  if (!h) {
+    set_err(err, errlen, "out of host memory");
+    return AT3D_ERR_MEMORY;
+  }
+  std::string msg;
+  int rc;
+  try {
+    rc = at3d::build_plan(*desc, h->p, msg);
+  } catch (const std::bad_alloc&) {
+    rc = AT3D_ERR_MEMORY;
+    msg = "out of host memory building the plan";
+  }
+  if (rc) {
+    set_err(err, errlen, msg);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return AT3D_OK;
+}
+
+void at3d_plan_destroy(at3d_plan* plan) { delete plan; }
+
+int64_t at3d_plan_size(const at3d_plan* h, const char* key) {
+  if (!h || !key) return -1;
+  const at3d::Plan& p = h->p;
+  const std::string k(key);
+  if (k == "npts") return p.npts;
+  if (k == "ncells") return p.ncells;
+  if (k == "nx1") return p.nx1;
+  if (k == "ny1") return p.ny1;
+  if (k == "nz") return p.nz;
+  if (k == "nlm") return p.nlm;
+  if (k == "ml") return p.ml;
+  if (k == "mm") return p.mm;
+  if (k == "nmu") return p.nmu;
+  if (k == "nang") return p.nang;
+  if (k == "nphi0max") return p.nphi0max;
+  if (k == "bcflag") return p.bcflag;
+  if (k == "ipflag") return p.ipflag;
+  if (k == "ntop") return (int64_t)p.bctop.size();
+  if (k == "nbot") return (int64_t)p.bcbot.size();
+  if (k == "nlevels_max") return p.nlevels_max;
+  if (k == "npart") return p.npart;
+  if (k == "numphase") return p.numphase;
+  if (k == "nleg") return p.nleg;
+  if (k == "nlegp") return p.nlegp;
+  if (k == "nscatangle") return p.nscatangle;
+  if (k == "nslow") {
+    int64_t s = 0;
+    for (int o = 0; o < 8; o++) s += p.oct[o].nslow;
+    return s;
+  }
+  return -1;
+}
+
+int at3d_plan_get(const at3d_plan* h, const char* name, void* dst, size_t nbytes, char* err,
+                  size_t errlen) {
+  if (!h || !name || (!dst && nbytes)) {
+    set_err(err, errlen, "null argument");
+    return AT3D_ERR_ARG;
+  }
+  const at3d::Plan& p = h->p;
+  const std::string k(name);
+  if (k == "gridpos") return copy_out(p.gridpos, dst, nbytes, err, errlen);
+  if (k == "gridptr") return copy_out(p.gridptr, dst, nbytes, err, errlen);
+  if (k == "neighptr") return copy_out(p.neighptr, dst, nbytes, err, errlen);
+  if (k == "cellflags") return copy_out(p.cellflags, dst, nbytes, err, errlen);
+  if (k == "xgrid") return copy_out(p.xgrid, dst, nbytes, err, errlen);
+  if (k == "ygrid") return copy_out(p.ygrid, dst, nbytes, err, errlen);
+  if (k == "zgrid") return copy_out(p.zgrid, dst, nbytes, err, errlen);
+  if (k == "mu") return copy_out(p.mu, dst, nbytes, err, errlen);
+  if (k == "wtmu") return copy_out(p.wtmu, dst, nbytes, err, errlen);
+  if (k == "nphi0") return copy_out(p.nphi0, dst, nbytes, err, errlen);
+  if (k == "phi") return copy_out(p.phi, dst, nbytes, err, errlen);
+  if (k == "wtdo") return copy_out(p.wtdo, dst, nbytes, err, errlen);
+  if (k == "ymat") return copy_out(p.ymat, dst, nbytes, err, errlen);
+  if (k == "wmat") return copy_out(p.wmat, dst, nbytes, err, errlen);
+  if (k == "ylmsun") return copy_out(p.ylmsun, dst, nbytes, err, errlen);
+  if (k == "total_ext") return copy_out(p.total_ext, dst, nbytes, err, errlen);
+  if (k == "extinct") return copy_out(p.extinct, dst, nbytes, err, errlen);
+  if (k == "albedo") return copy_out(p.albedo, dst, nbytes, err, errlen);
+  if (k == "dirflux") return copy_out(p.dirflux, dst, nbytes, err, errlen);
+  if (k == "radiance0") return copy_out(p.radiance0, dst, nbytes, err, errlen);
+  if (k == "chi") return copy_out(p.chi, dst, nbytes, err, errlen);
+  if (k == "alb") return copy_out(p.alb, dst, nbytes, err, errlen);
+  if (k == "bcptr_top") return copy_out(p.bctop, dst, nbytes, err, errlen);
+  if (k == "bcptr_bot") return copy_out(p.bcbot, dst, nbytes, err, errlen);
+  if (k == "sweepord" || k == "level") {
+    std::vector<int32_t> all;
+    all.reserve((size_t)8 * p.npts);
+    for (int o = 0; o < 8; o++) {
+      const std::vector<int32_t>& v = (k == "sweepord") ? p.oct[o].sweepord : p.oct[o].level;
+      all.insert(all.end(), v.begin(), v.end());
+    }
+    return copy_out(all, dst, nbytes, err, errlen);
+  }
+  set_err(err, errlen, "unknown plan array '" + k + "'");
+  return AT3D_ERR_ARG;
+}
+
+int at3d_state_create(const at3d_plan* plan, at3d_state** out, char* err, size_t errlen) {
+  if (!plan || !out) {
+    set_err(err, errlen, "null argument");
+    return AT3D_ERR_ARG;
+  }
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    set_err(err, errlen, "no HIP device available");
+    return AT3D_ERR_DEVICE;
+  }
+  at3d_state* h = new (std::nothrow) at3d_state;
+  if (!h) {
+    set_err(err, errlen, "out of host memory");
+    return AT3D_ERR_MEMORY;
+  }
+  std::string msg;
+  int rc;
+  try {
+    rc = at3d::state_create(plan->p, h->s, msg);
+  } catch (const std::bad_alloc&) {
+    rc = AT3D_ERR_MEMORY;
+    msg = "out of host memory creating the device state";
+  }
+  if (rc) {
+    at3d::state_free(h->s);
+    delete h;
+    set_err(err, errlen, msg);
+    return rc;
+  }
+  *out = h;
+  return AT3D_OK;
+}
+
+void at3d_state_destroy(at3d_state* st) {
+  if (!st) return;
+  at3d::state_free(st->s);
+  delete st;
+}
+
+int at3d_init_solution(at3d_state* st, char* err, size_t errlen) {
+  if (!st) {
+    set_err(err, errlen, "null state");
+    return AT3D_ERR_ARG;
+  }
+  std::string msg;
+  int rc = at3d::state_init_solution(st->s, msg);
+  if (rc) set_err(err, errlen, msg);
+  return rc;
+}
+
+int at3d_solve(at3d_state* st, int32_t maxiter, int32_t verbose, float* solcrit_out,
+               int32_t* iters_out, char* err, size_t errlen) {
+  if (!st) {
+    set_err(err, errlen, "null state");
+    return AT3D_ERR_ARG;
+  }
+  std::string msg;
+  int rc = at3d::state_solve(st->s, maxiter, verbose, msg);
+  if (solcrit_out) *solcrit_out = st->s.solcrit;
+  if (iters_out) *iters_out = st->s.iters;
+  if (rc) set_err(err, errlen, msg);
+  return rc;
+}
+
+int at3d_state_get(at3d_state* st, const char* name, void* dst, size_t nbytes, char* err,
+                   size_t errlen) {
+  if (!st || !name || !dst) {
+    set_err(err, errlen, "null argument");
+    return AT3D_ERR_ARG;
+  }
+  at3d::State& s = st->s;
+  const std::string k(name);
+  const size_t npts = s.npts;
+  auto get2d = [&](const float* src, size_t rows, size_t cols, size_t ld) -> int {
+    if (nbytes < rows * cols * sizeof(float)) {
+      set_err(err, errlen, "destination buffer too small");
+      return AT3D_ERR_ARG;
+    }
+    hipError_t e = hipMemcpy2D(dst, cols * sizeof(float), src, ld * sizeof(float),
+                               cols * sizeof(float), rows, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      set_err(err, errlen, hipGetErrorString(e));
+      return AT3D_ERR_DEVICE;
+    }
+    return AT3D_OK;
+  };
+  auto get1d = [&](const void* src, size_t bytes) -> int {
+    if (nbytes < bytes) {
+      set_err(err, errlen, "destination buffer too small");
+      return AT3D_ERR_ARG;
+    }
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      set_err(err, errlen, hipGetErrorString(e));
+      return AT3D_ERR_DEVICE;
+    }
+    return AT3D_OK;
+  };
+  if (hipStreamSynchronize(s.stream) != hipSuccess) {
+    set_err(err, errlen, "stream synchronize failed");
+    return AT3D_ERR_DEVICE;
+  }
+  if (k == "source") return get2d(s.S, npts, s.nlm, s.ld);
+  if (k == "radiance") return get2d(s.R, npts, s.nlm, s.ld);
+  if (k == "delsource") return get2d(s.DS, npts, s.nlm, s.ld);
+  if (k == "do_radiance") return get2d(s.dorad, s.nang, npts, s.nptsp);
+  if (k == "do_source") return get2d(s.dosrc, s.nang, npts, s.nptsp);
+  if (k == "fluxes") return get1d(s.fluxes, 2 * npts * sizeof(float));
+  if (k == "dirflux") return get1d(s.dirflux, npts * sizeof(float));
+  if (k == "ns") return get1d(s.ns, npts * sizeof(int));
+  if (k == "nr") return get1d(s.nr, npts * sizeof(int));
+  set_err(err, errlen, "unknown state array '" + k + "'");
+  return AT3D_ERR_ARG;
+}
+
+double at3d_state_timing(const at3d_state* st, const char* key) {
+  if (!st || !key) return -1.0;
+  const at3d::State& s = st->s;
+  const std::string k(key);
+  if (k == "truncation") return s.tsum[0];
+  if (k == "sh_to_do") return s.tsum[1];
+  if (k == "sweep") return s.tsum[2];
+  if (k == "do_to_sh") return s.tsum[3];
+  if (k == "source") return s.tsum[4];
+  if (k == "total") return s.tsum[0] + s.tsum[1] + s.tsum[2] + s.tsum[3] + s.tsum[4];
+  if (k == "iters") return s.iters_last;
+  return -1.0;
+}
+
+}  // extern "C"

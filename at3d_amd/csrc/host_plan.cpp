@@ -1,0 +1,1300 @@
+// Host setup of one SHDOM solve.
+//
+// Each function restates one piece of the reference's Fortran setup path and
+// cites the routine it follows.  Arithmetic keeps the reference's mixture of
+// single-precision (REAL) and double-precision (DOUBLE PRECISION) steps where
+// that decides rounding of stored values, so the GPU starts from the same
+// initial state as the reference.
+#include "host_plan.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+
+namespace at3d {
+namespace {
+
+inline bool btest(int v, int bit) { return (v >> bit) & 1; }
+
+// ---------------------------------------------------------------------------
+// Base grid: NEW_GRIDS (shdomsub2.f:16-81) with GRIDTYPE 'P' and
+// INIT_CELL_STRUCTURE (shdomsub2.f:84-309) for a base grid without a tree.
+// ---------------------------------------------------------------------------
+void make_grid(Plan& p) {
+  const at3d_scene_desc& d = p.d;
+  p.xgrid.resize(p.nx + 1);
+  p.ygrid.resize(p.ny + 1);
+  for (int ix = 0; ix <= p.nx; ix++) {
+    float num = d.delx * (float)d.npx;
+    p.xgrid[ix] = 0.0f + (float)ix * num / (float)p.nx;
+  }
+  for (int iy = 0; iy <= p.ny; iy++) {
+    float num = d.dely * (float)d.npy;
+    p.ygrid[iy] = 0.0f + (float)iy * num / (float)p.ny;
+  }
+  p.xgrid.resize(p.nx1);
+  p.ygrid.resize(p.ny1);
+  p.zgrid = p.zlevels;
+  p.npts = p.nx1 * p.ny1 * p.nz;
+  p.gridpos.resize((size_t)p.npts * 3);
+  for (int ix = 0; ix < p.nx1; ix++)
+    for (int iy = 0; iy < p.ny1; iy++)
+      for (int iz = 0; iz < p.nz; iz++) {
+        size_t i = iz + (size_t)p.nz * (iy + (size_t)p.ny1 * ix);
+        p.gridpos[i * 3 + 0] = p.xgrid[ix];
+        p.gridpos[i * 3 + 1] = p.ygrid[iy];
+        p.gridpos[i * 3 + 2] = p.zgrid[iz];
+      }
+  const int bc = p.bcflag, ip = p.ipflag;
+  int nxc = p.nx, nyc = p.ny;
+  if (btest(bc, 0)) nxc = p.nx + 1;
+  if (btest(bc, 2) && !btest(ip, 0)) nxc = p.nx - 1;
+  if (btest(bc, 1)) nyc = p.ny + 1;
+  if (btest(bc, 3) && !btest(ip, 1)) nyc = p.ny - 1;
+  p.nxc = nxc;
+  p.nyc = nyc;
+  const int nzc = p.nz - 1;
+  p.ncells = nxc * nyc * nzc;
+  p.gridptr.assign((size_t)p.ncells * 8, 0);
+  p.neighptr.assign((size_t)p.ncells * 6, -1);
+  p.cellflags.assign(p.ncells, 0);
+  const int NZ = p.nz, NY1 = p.ny1;
+  for (int ix = 1; ix <= nxc; ix++) {
+    int ix0, ix1;
+    if (btest(ip, 0)) {
+      ix0 = ix1 = ix;
+    } else if (btest(bc, 0)) {
+      ix0 = std::max(1, ix - 1);
+      ix1 = std::min(p.nx, ix);
+    } else {
+      ix0 = ix;
+      ix1 = ix + 1;
+    }
+    for (int iy = 1; iy <= nyc; iy++) {
+      int iy0, iy1;
+      if (btest(ip, 1)) {
+        iy0 = iy1 = iy;
+      } else if (btest(bc, 1)) {
+        iy0 = std::max(1, iy - 1);
+        iy1 = std::min(p.ny, iy);
+      } else {
+        iy0 = iy;
+        iy1 = iy + 1;
+      }
+      for (int iz = 1; iz <= nzc; iz++) {
+        const int I = iz + nzc * (iy - 1) + nzc * nyc * (ix - 1);  // 1-based cell
+        int* gp = &p.gridptr[(size_t)(I - 1) * 8];
+        auto pt = [&](int z, int y, int x) { return (z + NZ * (y - 1) + NZ * NY1 * (x - 1)) - 1; };
+        gp[0] = pt(iz, iy0, ix0);
+        gp[1] = pt(iz, iy0, ix1);
+        gp[2] = pt(iz, iy1, ix0);
+        gp[3] = pt(iz, iy1, ix1);
+        gp[4] = pt(iz + 1, iy0, ix0);
+        gp[5] = pt(iz + 1, iy0, ix1);
+        gp[6] = pt(iz + 1, iy1, ix0);
+        gp[7] = pt(iz + 1, iy1, ix1);
+        int8_t fl = 0;
+        int n[6];  // 1-based neighbour (0 = none), as in the reference
+        if (btest(ip, 0)) {
+          n[0] = I;
+          n[1] = I;
+          fl |= 1;
+        } else if (btest(bc, 0)) {
+          if (ix == 1) {
+            n[0] = I;
+            n[1] = I + nzc * nyc;
+            fl |= 1;
+          } else if (ix == nxc) {
+            n[0] = I - nzc * nyc;
+            n[1] = I;
+            fl |= 1;
+          } else {
+            n[0] = I - nzc * nyc;
+            n[1] = I + nzc * nyc;
+          }
+        } else if (btest(bc, 2)) {
+          n[0] = (ix == 1) ? 0 : I - nzc * nyc;
+          n[1] = (ix == nxc) ? 0 : I + nzc * nyc;
+        } else {
+          n[0] = (ix == 1) ? I + (nxc - 1) * nzc * nyc : I - nzc * nyc;
+          n[1] = (ix == p.nx) ? I - (nxc - 1) * nzc * nyc : I + nzc * nyc;
+        }
+        if (btest(ip, 1)) {
+          n[2] = I;
+          n[3] = I;
+          fl |= 2;
+        } else if (btest(bc, 1)) {
+          if (iy == 1) {
+            n[2] = I;
+            n[3] = I + nzc;
+            fl |= 2;
+          } else if (iy == nyc) {
+            n[2] = I - nzc;
+            n[3] = I;
+            fl |= 2;
+          } else {
+            n[2] = I - nzc;
+            n[3] = I + nzc;
+          }
+        } else if (btest(bc, 3)) {
+          n[2] = (iy == 1) ? 0 : I - nzc;
+          n[3] = (iy == nyc) ? 0 : I + nzc;
+        } else {
+          n[2] = (iy == 1) ? I + (nyc - 1) * nzc : I - nzc;
+          n[3] = (iy == p.ny) ? I - (nyc - 1) * nzc : I + nzc;
+        }
+        n[4] = (iz == 1) ? 0 : I - 1;
+        n[5] = (iz == nzc) ? 0 : I + 1;
+        for (int f = 0; f < 6; f++) p.neighptr[(size_t)(I - 1) * 6 + f] = n[f] - 1;
+        p.cellflags[I - 1] = fl;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Gauss-Legendre quadrature (GAUSQUADS shdomsub2.f:4874-4910) and the reduced
+// Gaussian ordinate set (MAKE_ANGLE_SET shdomsub2.f:1061-1144).
+// ---------------------------------------------------------------------------
+void gausquads(int n, std::vector<float>& xa, std::vector<float>& wt) {
+  const int k = n / 2;
+  xa.assign(n, 0.f);
+  wt.assign(n, 0.f);
+  for (int j = 1; j <= k; j++) {
+    float arg = 3.141592654f * ((float)j - 0.25f) / ((float)n + 0.5f);
+    double x = (double)cosf(arg), xp, pl, pl1, pl2, dpl;
+    int it = 0;
+    do {
+      pl1 = 1;
+      pl = x;
+      for (int l = 2; l <= n; l++) {
+        pl2 = pl1;
+        pl1 = pl;
+        pl = ((2 * l - 1) * x * pl1 - (l - 1) * pl2) / l;
+      }
+      dpl = n * (x * pl - pl1) / (x * x - 1);
+      xp = x;
+      x = xp - pl / dpl;
+      it++;
+    } while (fabs(x - xp) > 3.0e-7 && it < 10);
+    xa[j - 1] = (float)(-x);
+    xa[j - 1 + k] = (float)x;
+    wt[j - 1] = (float)(2.0 / ((1.0 - x * x) * dpl * dpl));
+    wt[j - 1 + k] = wt[j - 1];
+  }
+}
+
+// Smallest FFT-friendly azimuth count >= n used by the reduced-Gauss set: the
+// values of the reference's GOODNFFT table (MAKE_ANGLE_SET shdomsub2.f:1090-1114).
+int good_nfft(int n) {
+  if (n <= 14) return n;
+  static const int good[] = {16, 18, 20, 24, 25, 27, 30, 32, 36, 40, 45, 48, 50, 54, 60, 64, 72,
+                             80, 81, 90, 96, 100, 108, 120, 128, 144, 160, 180, 192, 200, 216, 240, 256};
+  for (int g : good)
+    if (g >= n) return g;
+  return -1;
+}
+
+int make_angle_set(Plan& p, std::string& err) {
+  const int nmu = p.nmu;
+  if (p.d.angle_set != 1 && p.d.angle_set != 2) {
+    err = "angle_set must be 1 (gaussian) or 2 (reduced gaussian)";
+    return AT3D_ERR_ARG;
+  }
+  if (p.nphi0max > 256 && p.d.angle_set == 2) {
+    err = "MAKE_ANGLE_SET: exceeded GOODNFFT";
+    return AT3D_ERR_ARG;
+  }
+  gausquads(nmu, p.mu, p.wtmu);
+  p.nphi0.assign(nmu, 0);
+  p.phi.assign((size_t)nmu * p.nphi0max, 0.f);
+  p.wtdo.assign((size_t)nmu * p.nphi0max, 0.f);
+  p.nang = 0;
+  for (int j = 0; j < nmu; j++) {
+    int n0;
+    if (p.d.angle_set == 1) {
+      n0 = p.nphi0max;
+    } else {
+      n0 = (int)(0.9f + (float)p.nphi0max * sqrtf(1.0f - p.mu[j] * p.mu[j]));
+      n0 = std::min(p.nphi0max, good_nfft(n0));
+    }
+    p.nphi0[j] = n0;
+    float delphi = 2.0f * acosf(-1.0f) / (float)n0;
+    for (int k = 0; k < n0; k++) {
+      p.phi[(size_t)j * p.nphi0max + k] = (float)k * delphi;
+      p.wtdo[(size_t)j * p.nphi0max + k] = delphi * p.wtmu[j];
+    }
+    p.nang += n0;
+  }
+  return AT3D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Normalised Wigner d^l_{m0} (WIGNERFCT02P2M_NORMALIZED DM0 branch,
+// shdomsub2.f:4363-4400, used by PLMALL) and the unpolarised variant
+// (WIGNERFCT_DM0, shdomsub2.f:4542-4578, used by YLMALL_UNPOL).
+// ---------------------------------------------------------------------------
+double dmm1_n0(double x, int m, int m1) {  // DMM1_N0 shdomsub2.f:4452-4487
+  if (m == m1) return pow((1.0 + x) / 2.0, m);
+  double cmm1 = (m1 > m) ? 1.0 : ((m - m1) % 2 ? -1.0 : 1.0);
+  int maxm = std::max(m, m1), minm = std::min(m, m1);
+  double prod = 1.0;
+  for (int pp = 1; pp <= maxm - minm; pp++) prod *= sqrt((double)(m + m1 + pp) / (double)pp);
+  double f = sqrt((1.0 - x) / 2.0);
+  double r = cmm1 * prod * pow(f, maxm - minm);
+  f = sqrt((1.0 + x) / 2.0);
+  return r * pow(f, maxm + minm);
+}
+
+void wigner_dm0_plm(double x, int nrank, int m, std::vector<double>& dm0) {
+  dm0.assign(nrank + 1, 0.0);
+  if (m > nrank) return;
+  if (m == 0) {
+    dm0[0] = 1.0;
+    if (nrank >= 1) dm0[1] = x;
+    for (int n = 1; n <= nrank - 1; n++) {
+      double f1 = (double)(2 * n + 1) * x / (double)(n + 1);
+      double f2 = (double)n / (double)(n + 1);
+      dm0[n + 1] = f1 * dm0[n] - f2 * dm0[n - 1];
+    }
+  } else {
+    dm0[m] = dmm1_n0(x, m, 0);
+    for (int n = m; n <= nrank - 1; n++) {
+      double f1 = (double)(n * (n + 1)) * x;
+      f1 = f1 / sqrt((double)((n + 1) * (n + 1) - m * m));
+      f1 = f1 / (double)(n + 1);
+      f1 = f1 * (double)(2 * n + 1) / (double)n;
+      double f2 = sqrt((double)(n * n - m * m)) * (double)n;
+      f2 = f2 / sqrt((double)((n + 1) * (n + 1) - m * m));
+      f2 = f2 / (double)(n + 1);
+      f2 = f2 * (double)(n + 1) / (double)n;
+      dm0[n + 1] = f1 * dm0[n] - f2 * dm0[n - 1];
+    }
+  }
+  for (int n = 0; n <= nrank; n++) dm0[n] = sqrt(n + 0.5) * dm0[n];
+}
+
+void wigner_dm0_ylm(double x, int nrank, int m, std::vector<double>& dm0) {
+  dm0.assign(nrank + 1, 0.0);
+  if (m > nrank) return;
+  if (m == 0) {
+    dm0[0] = 1.0;
+    if (nrank >= 1) dm0[1] = x;
+    for (int n = 1; n <= nrank - 1; n++)
+      dm0[n + 1] = ((2 * n + 1) * x * dm0[n] - n * dm0[n - 1]) / (n + 1);
+  } else {
+    double cm = (m % 2) ? -1.0 : 1.0, prod = 1.0;
+    for (int pp = 1; pp <= m; pp++) prod *= sqrt((m + pp) / (double)pp);
+    dm0[m] = cm * prod * pow(0.5 * sqrt((1.0 - x) * (1.0 + x)), m);
+    for (int n = m; n <= nrank - 1; n++)
+      dm0[n + 1] = ((2 * n + 1) * x * dm0[n] - sqrt((double)(n * n - m * m)) * dm0[n - 1]) /
+                   sqrt((double)((n + 1) * (n + 1) - m * m));
+  }
+  for (int n = 0; n <= nrank; n++) dm0[n] = sqrt(n + 0.5) * dm0[n];
+}
+
+inline int sh_index(int l, int m, int mm) {  // 0-based J for (l,m)
+  return (l <= mm) ? l * (l + 1) + m : (2 * mm + 1) * l - mm * mm + m;
+}
+
+// YLMALL_UNPOL (shdomsub2.f:4490-4539): real generalised SH of one direction.
+void ylm_unpol(float mu, float phi, int ml, int mm, float* yr) {
+  const double fct = 1.0 / sqrt(2.0 * acos(-1.0));
+  std::vector<double> dm0;
+  for (int m = 0; m <= mm; m++) {
+    wigner_dm0_ylm((double)mu, ml, m, dm0);
+    double cosm = 1.0, sinm = 0.0;
+    if (m > 0) {
+      cosm = (double)cosf((float)m * phi);
+      sinm = (double)sinf((float)m * phi);
+    }
+    for (int l = m; l <= ml; l++) {
+      double v = fct * dm0[l];
+      yr[sh_index(l, m, mm)] = (float)((cosm - sinm) * v);
+      yr[sh_index(l, -m, mm)] = (float)((cosm + sinm) * v);
+    }
+  }
+}
+
+// Dense SH<->DO transform matrices.  SH_TO_DO_UNPOL (shdomsub1.f:2928-3036)
+// sums P_lm(mu) over l per m (CMU1 = PLMALL, MAKE_SH_DO_COEF shdomsub2.f:1149-1220)
+// and then synthesises the azimuth with cos/sin of m*phi (FFT or DFT, both
+// identical in exact arithmetic) for |m| <= NPHI0/2-1.  DO_TO_SH_UNPOL
+// (shdomsub1.f:3164-3254) is the weighted transpose with weight
+// WTMU * (2 pi / NPHI0).  Both are written here as one [nang][nlm] matrix each.
+void make_sh_tables(Plan& p) {
+  const int ml = p.ml, mm = p.mm, nlm = p.nlm;
+  p.lofj.assign(nlm, 0);
+  p.mofj.assign(nlm, 0);
+  for (int l = 0, j = 0; l <= ml; l++)
+    for (int m = -std::min(l, mm); m <= std::min(l, mm); m++, j++) {
+      p.lofj[j] = l;
+      p.mofj[j] = m;
+    }
+  const double fct = 1.0 / sqrt(2.0 * acos(-1.0));
+  p.ymat.assign((size_t)p.nang * nlm, 0.f);
+  p.wmat.assign((size_t)p.nang * nlm, 0.f);
+  std::vector<float> plm(nlm);
+  std::vector<double> dm0;
+  int k = 0;
+  for (int imu = 0; imu < p.nmu; imu++) {
+    for (int m = 0; m <= mm; m++) {
+      wigner_dm0_plm((double)p.mu[imu], ml, m, dm0);
+      for (int l = m; l <= ml; l++) {
+        float v = (float)(fct * dm0[l]);
+        plm[sh_index(l, m, mm)] = v;
+        plm[sh_index(l, -m, mm)] = v;
+      }
+    }
+    const int n0 = p.nphi0[imu];
+    const int me = std::max(0, std::min(n0 / 2 - 1, mm));
+    const double dphi = 2.0 * acos(-1.0) / n0;
+    const float wmu = p.wtmu[imu];
+    for (int ip = 0; ip < n0; ip++, k++) {
+      const double ph = ip * dphi;
+      for (int j = 0; j < nlm; j++) {
+        const int m = p.mofj[j], am = std::abs(m);
+        if (am > me) continue;
+        double t;
+        if (m == 0)
+          t = 1.0;
+        else if (m > 0)
+          t = cos(am * ph) - sin(am * ph);
+        else
+          t = cos(am * ph) + sin(am * ph);
+        p.ymat[(size_t)k * nlm + j] = (float)((double)plm[j] * t);
+        p.wmat[(size_t)k * nlm + j] = (float)((double)plm[j] * (double)wmu * dphi * t);
+      }
+    }
+  }
+  p.ylmsun.assign(nlm, 0.f);
+  ylm_unpol(p.d.solarmu, p.d.solaraz, ml, mm, p.ylmsun.data());
+}
+
+// ---------------------------------------------------------------------------
+// Property transfer: TRILIN_INTERP_PROP (shdom90.f90:17-346, 'ON' method),
+// INTERP_GRID (shdomsub2.f:313-387) and PREPARE_PROP (shdomsub2.f:479-608).
+// ---------------------------------------------------------------------------
+void interp_optics(Plan& p) {
+  const at3d_scene_desc& d = p.d;
+  const int npx = d.npx, npy = d.npy, npz = d.npz, npg = npx * npy * npz;
+  const int npts = p.npts, npart = p.npart;
+  p.extmin = (double)(1.0e-5f / ((p.zlevels[npz - 1] - p.zlevels[0]) / (float)npz));
+  p.scatmin = (double)0.1f * p.extmin;
+  p.extinct.assign((size_t)npart * npts, 0.f);
+  p.albedo.assign((size_t)npart * npts, 0.f);
+  p.iphase.assign((size_t)npart * npts * 8, 1);
+  p.phaseinterpwt.assign((size_t)npart * npts * 8, 0.f);
+  p.total_ext.assign(npts, 0.f);
+  for (int ipa = 0; ipa < npart; ipa++) {
+    const float* E = &p.extinctp[(size_t)ipa * npg];
+    const float* A = &p.albedop[(size_t)ipa * npg];
+    const float* PW = &p.phasewtp[(size_t)ipa * npg];
+    const int32_t* IP = &p.iphasep[(size_t)ipa * npg];
+    for (int ip = 0; ip < npts; ip++) {
+      const float X = p.gridpos[ip * 3], Y = p.gridpos[ip * 3 + 1], Z = p.gridpos[ip * 3 + 2];
+      int il = 0, iu = npz;
+      while (iu - il > 1) {
+        int im = (iu + il) / 2;
+        if (Z >= p.zlevels[im - 1]) il = im; else iu = im;
+      }
+      const int iz = std::max(il, 1);
+      double w = (double)(Z - p.zlevels[iz - 1]) / (double)(p.zlevels[iz] - p.zlevels[iz - 1]);
+      w = std::max(std::min(w, 1.0), 0.0);
+      int ix = (int)((X - 0.0f) / d.delx) + 1;
+      if (fabsf(X - 0.0f - (float)npx * d.delx) < 0.01f * d.delx) ix = npx;
+      int ixp = ix % npx + 1;
+      double u = (double)(X - 0.0f - d.delx * (float)(ix - 1)) / (double)d.delx;
+      u = std::max(std::min(u, 1.0), 0.0);
+      if (u < 1.0e-5) u = 0.0;
+      if (u > 1.0 - 1.0e-5) u = 1.0;
+      int iy = (int)((Y - 0.0f) / d.dely) + 1;
+      if (fabsf(Y - 0.0f - (float)npy * d.dely) < 0.01f * d.dely) iy = npy;
+      int iyp = iy % npy + 1;
+      double v = (double)(Y - 0.0f - d.dely * (float)(iy - 1)) / (double)d.dely;
+      v = std::max(std::min(v, 1.0), 0.0);
+      if (v < 1.0e-5) v = 0.0;
+      if (v > 1.0 - 1.0e-5) v = 1.0;
+      const double F[8] = {(1 - u) * (1 - v) * (1 - w), u * (1 - v) * (1 - w), (1 - u) * v * (1 - w),
+                           u * v * (1 - w),             (1 - u) * (1 - v) * w, u * (1 - v) * w,
+                           (1 - u) * v * w,             u * v * w};
+      int I[8];
+      I[0] = iz + npz * (iy - 1) + npz * npy * (ix - 1) - 1;
+      I[1] = iz + npz * (iy - 1) + npz * npy * (ixp - 1) - 1;
+      I[2] = iz + npz * (iyp - 1) + npz * npy * (ix - 1) - 1;
+      I[3] = iz + npz * (iyp - 1) + npz * npy * (ixp - 1) - 1;
+      for (int q = 0; q < 4; q++) I[q + 4] = I[q] + 1;
+      double ext = 0.0, scat[8], scatter = 0.0;
+      for (int q = 0; q < 8; q++) ext += F[q] * (double)E[I[q]];
+      for (int q = 0; q < 8; q++) {
+        scat[q] = F[q] * (double)E[I[q]] * (double)A[I[q]];
+        scatter += scat[q];
+      }
+      const float extf = (float)ext;
+      float albf = ((double)extf > p.extmin) ? (float)(scatter / (double)extf)
+                                             : (float)(scatter / p.extmin);
+      int iph[8];
+      float wt[8];
+      for (int q = 0; q < 8; q++) {
+        iph[q] = IP[I[q]];
+        wt[q] = (scatter >= p.scatmin) ? (float)((double)PW[I[q]] * scat[q] / scatter)
+                                       : (float)((double)PW[I[q]] * scat[q] / p.scatmin);
+      }
+      for (int q = 0; q < 8; q++)
+        for (int q2 = q + 1; q2 < 8; q2++)
+          if (iph[q] == iph[q2]) {
+            wt[q] += wt[q2];
+            wt[q2] = 0.f;
+          }
+      int ord[8];
+      std::iota(ord, ord + 8, 0);
+      std::stable_sort(ord, ord + 8, [&](int a, int b) { return wt[a] > wt[b]; });
+      const size_t base = ((size_t)ipa * npts + ip) * 8;
+      for (int q = 0; q < 8; q++) {
+        p.iphase[base + q] = iph[ord[q]];
+        p.phaseinterpwt[base + q] = wt[ord[q]];
+      }
+      p.extinct[(size_t)ipa * npts + ip] = extf;
+      p.albedo[(size_t)ipa * npts + ip] = albf;
+      p.total_ext[ip] = p.total_ext[ip] + extf;
+    }
+  }
+  // Tabulated phase functions divided by (2l+1) (TRILIN_INTERP_PROP init branch).
+  const int nl = p.nleg + 1;
+  p.legen.assign((size_t)p.numphase * nl, 0.f);
+  for (int iph = 0; iph < p.numphase; iph++)
+    for (int l = 0; l < nl; l++)
+      p.legen[(size_t)iph * nl + l] = p.legenp[(size_t)iph * (p.nlegp + 1) + l] / (float)(2 * l + 1);
+  // PREPARE_PROP, 'N' interpolation: subtract f from l<=ML, scale ext/albedo.
+  const int ML = p.ml;
+  const float phasemax = 0.999f;
+  if (p.d.deltam) {
+    for (int iph = 0; iph < p.numphase; iph++) {
+      float f = p.legen[(size_t)iph * nl + ML + 1];
+      for (int l = 0; l <= ML; l++) p.legen[(size_t)iph * nl + l] = p.legen[(size_t)iph * nl + l] - f;
+    }
+    for (int ip = 0; ip < npts; ip++) {
+      float s = 0.f;
+      for (int ipa = 0; ipa < npart; ipa++) s += p.extinct[(size_t)ipa * npts + ip];
+      float t = p.total_ext[ip] - s;
+      p.total_ext[ip] = t < 0.f ? 0.f : t;
+    }
+    for (int ipa = 0; ipa < npart; ipa++)
+      for (int ip = 0; ip < npts; ip++) {
+        const size_t base = ((size_t)ipa * npts + ip) * 8;
+        float f;
+        if (p.phaseinterpwt[base] >= phasemax) {
+          f = p.legen[(size_t)(p.iphase[base] - 1) * nl + ML + 1];
+        } else {
+          f = 0.f;
+          for (int q = 0; q < 8; q++)
+            f = f + p.legen[(size_t)(p.iphase[base + q] - 1) * nl + ML + 1] * p.phaseinterpwt[base + q];
+        }
+        float& ext = p.extinct[(size_t)ipa * npts + ip];
+        float& alb = p.albedo[(size_t)ipa * npts + ip];
+        float alb0 = alb;
+        ext = (1.0f - alb0 * f) * ext;
+        alb = (1.0f - f) * alb0 / (1.0f - alb0 * f);
+        p.total_ext[ip] = p.total_ext[ip] + ext;
+      }
+  }
+  p.albmax = 0.f;
+  for (float a : p.albedo) p.albmax = std::max(p.albmax, a);
+}
+
+// Mixed phase function LEGENT1 of one species at one point ('N' branch of
+// COMPUTE_SOURCE, shdomsub1.f:1088-1112): returns coefficients 0..nleg.
+void mixed_legen(const Plan& p, int ipa, int ip, std::vector<float>& out) {
+  const int nl = p.nleg + 1;
+  const size_t base = ((size_t)ipa * p.npts + ip) * 8;
+  out.assign(nl, 0.f);
+  if (p.phaseinterpwt[base] >= 0.999f) {
+    const float* L = &p.legen[(size_t)(p.iphase[base] - 1) * nl];
+    for (int l = 0; l < nl; l++) out[l] = L[l];
+  } else {
+    for (int q = 0; q < 8; q++) {
+      const float w = p.phaseinterpwt[base + q];
+      if (w <= 1e-5f) continue;
+      const float* L = &p.legen[(size_t)(p.iphase[base + q] - 1) * nl];
+      for (int l = 0; l < nl; l++) out[l] = out[l] + L[l] * w;
+    }
+  }
+}
+
+// Per-point constants of the source computation: the scattering albedo and
+// mixed, delta-M renormalised Legendre coefficients (COMPUTE_SOURCE NEWMETHOD,
+// shdomsub1.f:1075-1140) and the radiance-truncation weights
+// (RADIANCE_TRUNCATION, shdomsub1.f:1665-1735).
+void source_constants(Plan& p) {
+  const int ML = p.ml, npts = p.npts, npart = p.npart, nl = p.nleg + 1;
+  p.chi.assign((size_t)npts * (ML + 1), 0.f);
+  p.alb.assign(npts, 0.f);
+  p.tc.assign((size_t)npts * (ML + 1), 0.f);
+  std::vector<float> l1, legent(nl);
+  for (int i = 0; i < npts; i++) {
+    const float ext = p.total_ext[i];
+    double alb = 0.0;
+    std::fill(legent.begin(), legent.end(), 0.f);
+    for (int ipa = 0; ipa < npart; ipa++) {
+      const double scat = (double)(p.extinct[(size_t)ipa * npts + i] * p.albedo[(size_t)ipa * npts + i]);
+      alb += scat;
+      mixed_legen(p, ipa, i, l1);
+      if (p.d.deltam) {
+        const float f = l1[ML + 1];
+        for (int l = 0; l <= ML; l++) l1[l] = l1[l] / (1 - f);
+      }
+      for (int l = 0; l < nl; l++) legent[l] = (float)((double)legent[l] + scat * (double)l1[l]);
+    }
+    if (alb > 1e-10) {
+      for (int l = 0; l < nl; l++) legent[l] = (float)((double)legent[l] / alb);
+    } else {
+      for (int l = 0; l < nl; l++) legent[l] = legent[l] / (float)npart;
+    }
+    alb = (ext > 1e-10f) ? alb / (double)ext : 0.0;
+    legent[0] = 1.0f;
+    for (int l = 0; l <= ML; l++) p.chi[(size_t)i * (ML + 1) + l] = legent[l];
+    p.alb[i] = (float)alb;
+    // truncation weights sum_ipa W*ALBEDO*LEGENT*F
+    for (int l = 1; l <= ML; l++) {
+      float rad = 0.f;
+      for (int ipa = 0; ipa < npart; ipa++) {
+        const float W = (ext == 0.0f) ? 1.0f : p.extinct[(size_t)ipa * npts + i] / ext;
+        if (W == 0.0f) continue;
+        const size_t base = ((size_t)ipa * npts + i) * 8;
+        float legt;
+        if (p.phaseinterpwt[base] >= 0.999f) {
+          legt = p.legen[(size_t)(p.iphase[base] - 1) * nl + l];
+        } else {
+          legt = 0.f;
+          for (int q = 0; q < 8; q++) {
+            if (p.phaseinterpwt[base + q] <= 1e-5f) continue;
+            legt = legt + p.legen[(size_t)(p.iphase[base + q] - 1) * nl + l] * p.phaseinterpwt[base + q];
+          }
+        }
+        if (p.d.deltam) {
+          float F;
+          if (p.phaseinterpwt[base] >= 0.999f) {
+            F = p.legen[(size_t)(p.iphase[base] - 1) * nl + ML + 1];
+          } else {
+            // The reference indexes LEGEN(Q,ML+1,IPHASE(Q)) here (first index Q,
+            // shdomsub1.f:1688-1689); with NSTLEG=1 that is coefficient ML+Q.
+            F = 0.f;
+            const size_t total = p.legen.size();
+            for (int q = 0; q < 8; q++) {
+              if (p.phaseinterpwt[base + q] <= 1e-5f) continue;
+              size_t flat = (size_t)(p.iphase[base + q] - 1) * nl + ML + 1 + q;
+              float c = flat < total ? p.legen[flat] : 0.f;
+              F = F + c * p.phaseinterpwt[base + q];
+            }
+          }
+          legt = legt * (1.0f / (1 - F));
+        }
+        rad = rad + W * p.albedo[(size_t)ipa * npts + i] * legt;
+      }
+      p.tc[(size_t)i * (ML + 1) + l] = rad;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Direct solar beam through the property grid: DIRECT_BEAM_PROP
+// (shdom90.f90:352-867) called by MAKE_DIRECT (shdomsub2.f:393-473).
+// ---------------------------------------------------------------------------
+struct DirectBeam {
+  std::vector<float> extdirp;
+  double cx, cy, cz, cxinv, cyinv, czinv, epss, epsz, xdomain, ydomain, delxd, delyd;
+  int di, dj, dk, ipdirect;
+};
+
+void direct_beam_init(const Plan& p, DirectBeam& b) {
+  const at3d_scene_desc& d = p.d;
+  const int npx = d.npx, npy = d.npy, npz = d.npz, npg = npx * npy * npz;
+  const int ML = p.ml, npart = p.npart;
+  b.extdirp.assign(npg, 0.f);
+  for (int ip = 0; ip < npg; ip++) {
+    float acc = 0.f;
+    for (int ipa = 0; ipa < npart; ipa++) {
+      double ext = p.extinctp[(size_t)ipa * npg + ip];
+      double alb = p.albedop[(size_t)ipa * npg + ip];
+      if (d.deltam) {
+        const int L = ML + 1;
+        int iph = p.iphasep[(size_t)ipa * npg + ip];
+        float term = p.phasewtp[(size_t)ipa * npg + ip] * p.legenp[(size_t)(iph - 1) * (p.nlegp + 1) + L];
+        term = term / (float)(2 * L + 1);
+        double F = 0.0 + (double)term;
+        ext = (1.0 - alb * F) * ext;
+      }
+      acc = (float)((double)acc + ext);
+    }
+    b.extdirp[ip] = acc;
+  }
+  b.ipdirect = p.ipflag;
+  if (btest(p.ipflag, 2)) b.ipdirect = 0;
+  const double sunmu = -(double)d.solarmu;
+  const double sunaz = (double)(d.solaraz + acosf(-1.0f));
+  b.cx = sqrt(1.0 - sunmu * sunmu) * cos(sunaz);
+  b.cy = sqrt(1.0 - sunmu * sunmu) * sin(sunaz);
+  b.cz = fabs(sunmu);
+  if (fabs(b.cx) > 1.0e-6) b.cxinv = 1.0 / b.cx; else { b.cx = 0.0; b.cxinv = 1.0e20; }
+  if (fabs(b.cy) > 1.0e-6) b.cyinv = 1.0 / b.cy; else { b.cy = 0.0; b.cyinv = 1.0e20; }
+  if (fabs(b.cz) > 1.0e-6) b.czinv = 1.0 / b.cz; else { b.cz = 0.0; b.czinv = 1.0e20; }
+  b.di = b.cx >= 0 ? 1 : -1;
+  b.dj = b.cy >= 0 ? 1 : -1;
+  b.dk = b.cz >= 0 ? 1 : -1;
+  const float zspan = p.zlevels[npz - 1] - p.zlevels[0];
+  b.epsz = (double)(1.0e-6f * zspan);
+  b.epss = (double)(1.0e-3f * zspan / (float)npz);
+  if (!btest(b.ipdirect, 0)) b.epss = std::max(b.epss, 1.0e-4 * (double)d.delx);
+  if (!btest(b.ipdirect, 1)) b.epss = std::max(b.epss, 1.0e-4 * (double)d.dely);
+  b.delxd = d.delx;
+  b.delyd = d.dely;
+  b.epss = std::max(std::max((double)0.001f * b.delxd, (double)0.001f * b.delyd), b.epss);
+  b.xdomain = b.delxd * npx;
+  if (btest(p.bcflag, 2)) b.xdomain = b.delxd * (npx - 1);
+  b.ydomain = b.delyd * npy;
+  if (btest(p.bcflag, 3)) b.ydomain = b.delyd * (npy - 1);
+}
+
+// Returns false when the reference would STOP.
+bool direct_beam_point(const Plan& p, const DirectBeam& b, float xi, float yi, float zi,
+                       float& dirflux) {
+  const at3d_scene_desc& d = p.d;
+  const int npx = d.npx, npy = d.npy, npz = d.npz, bc = p.bcflag;
+  const float* ZL = p.zlevels.data();
+  double z = zi, x = (double)(xi - 0.0f), y = (double)(yi - 0.0f);
+  int il = 0, iu = npz;
+  while (iu - il > 1) {
+    int im = (iu + il) / 2;
+    if (z >= (double)ZL[im - 1]) il = im; else iu = im;
+  }
+  int k = std::max(il, 1);
+  int i = (int)(x / b.delxd) + 1;
+  if (i > npx && fabs(x - b.xdomain) < 0.001 * b.delxd) i = npx;
+  if (i < 1 || i > npx) return false;
+  int j = (int)(y / b.delyd) + 1;
+  if (j > npy && fabs(y - b.ydomain) < 0.001 * b.delyd) j = npy;
+  if (j < 1 || j > npy) return false;
+  double xe = x, ye = y, ze = z, xp = xe, yp = ye, zp = ze;
+  bool constx = btest(b.ipdirect, 0), consty = btest(b.ipdirect, 1);
+  if (b.cx == 0.0) constx = true;
+  if (b.cy == 0.0) consty = true;
+  if (btest(bc, 0) && (fabs(x) < (double)0.01f * b.delxd || fabs(x - (npx - 1) * b.delxd) < (double)0.01f * b.delxd))
+    constx = true;
+  if (btest(bc, 1) && (fabs(y) < (double)0.01f * b.delyd || fabs(y - (npy - 1) * b.delyd) < (double)0.01f * b.delyd))
+    consty = true;
+  double path = 0.0;
+  const float* E = b.extdirp.data();
+  int guard = 0;
+  while (fabs(ze - (double)ZL[npz - 1]) > b.epsz) {
+    if (++guard > 100000000) return false;
+    int ip1 = i + 1;
+    if (i == npx) ip1 = (btest(bc, 0) || btest(bc, 2)) ? npx : 1;
+    int jp1 = j + 1;
+    if (j == npy) jp1 = (btest(bc, 1) || btest(bc, 3)) ? npy : 1;
+    const double x0 = b.delxd * (i - 1), x1 = x0 + b.delxd;
+    const double y0 = b.delyd * (j - 1), y1 = y0 + b.delyd;
+    const double z0 = ZL[k - 1], z1 = ZL[k];
+    if (i < 1 || i > npx || j < 1 || j > npy || k < 1 || k >= npz) return false;
+    const int I1 = k + npz * (j - 1) + npz * npy * (i - 1) - 1;
+    const int I2 = k + npz * (j - 1) + npz * npy * (ip1 - 1) - 1;
+    const int I3 = k + npz * (jp1 - 1) + npz * npy * (i - 1) - 1;
+    const int I4 = k + npz * (jp1 - 1) + npz * npy * (ip1 - 1) - 1;
+    const double E1 = E[I1], E2 = E[I2], E3 = E[I3], E4 = E[I4];
+    const double E5 = E[I1 + 1], E6 = E[I2 + 1], E7 = E[I3 + 1], E8 = E[I4 + 1];
+    double sox, soy, soz;
+    if (constx) sox = 1.0e30;
+    else if (b.cx > 0.0) { sox = (x1 - xe) * b.cxinv; xp = x1; }
+    else { sox = (x0 - xe) * b.cxinv; xp = x0; }
+    if (consty) soy = 1.0e30;
+    else if (b.cy > 0.0) { soy = (y1 - ye) * b.cyinv; yp = y1; }
+    else { soy = (y0 - ye) * b.cyinv; yp = y0; }
+    if (b.cz > 0.0) { soz = (z1 - ze) * b.czinv; zp = z1; }
+    else if (b.cz < 0.0) { soz = (z0 - ze) * b.czinv; zp = z0; }
+    else soz = 1.0e30;
+    double xoffs = 0.0, yoffs = 0.0, so;
+    if (soz <= sox && soz <= soy) {
+      so = soz;
+      if (!constx) xp = xe + so * b.cx;
+      if (!consty) yp = ye + so * b.cy;
+      k += b.dk;
+    } else if (sox <= soy) {
+      so = sox;
+      if (!consty) yp = ye + so * b.cy;
+      zp = ze + so * b.cz;
+      i += b.di;
+      if (i == 0) {
+        if (btest(bc, 0)) { i = 1; constx = true; }
+        else if (btest(bc, 2)) return false;
+        else { i = npx; xoffs = b.xdomain; }
+      } else if (i >= npx && btest(bc, 2)) {
+        return false;
+      } else if (i == npx + 1) {
+        if (btest(bc, 0)) { i = npx; constx = true; }
+        else { i = 1; xoffs = -b.xdomain; }
+      }
+    } else {
+      so = soy;
+      if (!constx) xp = xe + so * b.cx;
+      zp = ze + so * b.cz;
+      j += b.dj;
+      if (j == 0) {
+        if (btest(bc, 1)) { j = 1; consty = true; }
+        else if (btest(bc, 3)) return false;
+        else { j = npy; yoffs = b.ydomain; }
+      } else if (j >= npy && btest(bc, 3)) {
+        return false;
+      } else if (j == npy + 1) {
+        if (btest(bc, 1)) { j = npy; consty = true; }
+        else { j = 1; yoffs = -b.ydomain; }
+      }
+    }
+    if (so < -b.epss) return false;
+    so = std::max(so, 0.0);
+    const double ax = 1.0 / (x1 - x0), ay = 1.0 / (y1 - y0), az = 1.0 / (z1 - z0);
+    const double u0 = (xe - x0) * ax, v0 = (ye - y0) * ay, w0 = (ze - z0) * az;
+    const double u1 = (xp - x0) * ax, v1 = (yp - y0) * ay, w1 = (zp - z0) * az;
+    const double u0m = 1.0 - u0, v0m = 1.0 - v0, w0m = 1.0 - w0;
+    const double du = u1 - u0, dv = v1 - v0, dw = w1 - w0;
+    double uv = u0 * v0, umv = u0m * v0, uvm = u0 * v0m, umvm = u0m * v0m;
+    double uw = u0 * w0, umw = u0m * w0, uwm = u0 * w0m, umwm = u0m * w0m;
+    double vw = v0 * w0, vmw = v0m * w0, vwm = v0 * w0m, vmwm = v0m * w0m;
+    const double A = (E1 * u0m + E2 * u0) * vmwm + (E3 * u0m + E4 * u0) * vwm +
+                     (E5 * u0m + E6 * u0) * vmw + (E7 * u0m + E8 * u0) * vw;
+    const double B1 = -du * vmwm - dv * umwm - dw * umvm;
+    const double B2 = du * vmwm - dv * uwm - dw * uvm;
+    const double B3 = -du * vwm + dv * umwm - dw * umv;
+    const double B4 = du * vwm + dv * uwm - dw * uv;
+    const double B5 = -du * vmw - dv * umw + dw * umvm;
+    const double B6 = du * vmw - dv * uw + dw * uvm;
+    const double B7 = -du * vw + dv * umw + dw * umv;
+    const double B8 = du * vw + dv * uw + dw * uv;
+    const double B = B1 * E1 + B2 * E2 + B3 * E3 + B4 * E4 + B5 * E5 + B6 * E6 + B7 * E7 + B8 * E8;
+    vw = dv * dw;
+    const double vwu = vw * u0, vwum = vw * u0m;
+    uw = du * dw;
+    const double uwv = uw * v0, uwvm = uw * v0m;
+    uv = du * dv;
+    const double uvw = uv * w0, uvwm = uv * w0m;
+    const double C1 = +vwum + uwvm + uvwm, C2 = +vwu - uwvm - uvwm, C3 = -vwum + uwv - uvwm;
+    const double C4 = -vwu - uwv + uvwm, C5 = -vwum - uwvm + uvw, C6 = -vwu + uwvm - uvw;
+    const double C7 = +vwum - uwv - uvw, C8 = +vwu + uwv + uvw;
+    const double C = C1 * E1 + C2 * E2 + C3 * E3 + C4 * E4 + C5 * E5 + C6 * E6 + C7 * E7 + C8 * E8;
+    const double D = du * dv * dw * (E2 + E3 + E5 + E8 - E1 - E4 - E6 - E7);
+    path = path + so * (A + 0.5 * B + 0.3333333333333333 * C + 0.25 * D);
+    xe = xp + xoffs;
+    ye = yp + yoffs;
+    ze = zp;
+  }
+  dirflux = (float)((double)d.solarflux * exp(-path));
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Eddington two-stream initial radiance: INIT_RADIANCE (shdomsub2.f:614-760),
+// EDDRTF (shdomsub2.f:765-993) solar branch and TRIDIAG (shdomsub2.f:997-1054).
+// ---------------------------------------------------------------------------
+bool tridiag(int n, std::vector<double>& lower, std::vector<double>& diag,
+             std::vector<double>& upper, std::vector<double>& rhs) {
+  // 1-based arrays of size n+1
+  if (n == 1) {
+    if (diag[1] == 0.0) return false;
+    rhs[1] = rhs[1] / diag[1];
+  }
+  lower[1] = diag[1];
+  diag[1] = upper[1];
+  upper[1] = 0.0;
+  upper[n] = 0.0;
+  for (int k = 1; k <= n - 1; k++) {
+    if (fabs(lower[k + 1]) >= fabs(lower[k])) {
+      std::swap(lower[k + 1], lower[k]);
+      std::swap(diag[k + 1], diag[k]);
+      std::swap(upper[k + 1], upper[k]);
+      std::swap(rhs[k + 1], rhs[k]);
+    }
+    if (lower[k] == 0.0) return false;
+    double t = -lower[k + 1] / lower[k];
+    lower[k + 1] = diag[k + 1] + t * diag[k];
+    diag[k + 1] = upper[k + 1] + t * upper[k];
+    upper[k + 1] = 0.0;
+    rhs[k + 1] = rhs[k + 1] + t * rhs[k];
+  }
+  if (lower[n] == 0.0) return false;
+  rhs[n] = rhs[n] / lower[n];
+  rhs[n - 1] = (rhs[n - 1] - diag[n - 1] * rhs[n]) / lower[n - 1];
+  for (int kb = 1; kb <= n - 2; kb++) {
+    int k = n - 2 - kb + 1;
+    rhs[k] = (rhs[k] - diag[k] * rhs[k + 1] - upper[k] * rhs[k + 2]) / lower[k];
+  }
+  return true;
+}
+
+bool eddrtf_solar(int nlayer, const std::vector<float>& od, const std::vector<float>& alb,
+                  const std::vector<float>& asym, float solarflux, float solarmu, float gndemis,
+                  float skyrad, std::vector<float>& fluxes /*3*(nlayer+1)*/) {
+  const int N = 2 * nlayer + 2;
+  std::vector<double> lower(N + 1), upper(N + 1), diag(N + 1), rhs(N + 1);
+  const double PI = (double)3.1415926535f;
+  const double mu0 = fabs((double)solarmu);
+  double tau = 0.0;
+  int I = 2;
+  fluxes.assign(3 * (nlayer + 1), 0.f);
+  for (int L = 1; L <= nlayer; L++) {
+    double deltau = od[L - 1], trans, reflect, sourcep, sourcem;
+    if (deltau < 0.0) return false;
+    if (deltau == 0.0) {
+      trans = 1.0;
+      reflect = 0.0;
+      sourcep = 0.0;
+      sourcem = 0.0;
+    } else {
+      const double omega = alb[L - 1], g = asym[L - 1];
+      const double R = (1.0 - omega * (4.0 - 3.0 * g)) / 4.0;
+      const double T = (7.0 - omega * (4.0 + 3.0 * g)) / 4.0;
+      const double lambda = sqrt(3.0 * (1.0 - omega) * (1.0 - omega * g));
+      double D, X1 = 0, X2 = 0, exlp = 0, exlm = 0;
+      if (lambda == 0.0) {
+        D = 1.0 / (1.0 + T * deltau);
+        trans = D;
+        reflect = -R * deltau * D;
+      } else {
+        X1 = -R;
+        X2 = lambda + T;
+        exlp = exp(std::min(lambda * deltau, 75.0));
+        exlm = 1.0 / exlp;
+        trans = 2. * lambda / (X2 * exlp + (lambda - T) * exlm);
+        reflect = X1 * (exlp - exlm) * trans / (2. * lambda);
+        D = 1.0 / (X2 * X2 * exlp - X1 * X1 * exlm);
+      }
+      fluxes[3 * (L - 1) + 2] = (float)((double)solarflux * exp(-tau / mu0));
+      const double ds = 1.0 / (lambda * lambda - 1.0 / (mu0 * mu0));
+      const double b1 = 0.5 * omega * ((double)solarflux / mu0) * exp(-tau / mu0) * ds;
+      const double b2 = 0.5 * omega * ((double)solarflux / mu0) * exp(-(tau + deltau) / mu0) * ds;
+      const double solpp = 1.0 + 1.5 * g * mu0, solpm = -1.0 + 1.5 * g * mu0;
+      const double radp1p = ((T + 1.0 / mu0) * solpp + R * solpm) * b1;
+      const double radp2m = ((-T + 1.0 / mu0) * solpm - R * solpp) * b2;
+      const double radp2p = ((T + 1.0 / mu0) * solpp + R * solpm) * b2;
+      const double radp1m = ((-T + 1.0 / mu0) * solpm - R * solpp) * b1;
+      if (lambda == 0.0) {
+        const double A = (R * deltau * radp1p - radp2m) * D;
+        const double B = -(R * radp1p + T * radp2m) * D;
+        sourcep = (B - T * (A + B * deltau)) / R + radp2p;
+        sourcem = A + radp1m;
+      } else {
+        const double cp = (X1 * exlm * radp1p - X2 * radp2m) * D;
+        const double cm = (-X2 * exlp * radp1p + X1 * radp2m) * D;
+        sourcep = X1 * cp * exlp + X2 * cm * exlm + radp2p;
+        sourcem = X2 * cp + X1 * cm + radp1m;
+      }
+      tau = tau + deltau;
+    }
+    diag[I] = -reflect;
+    diag[I + 1] = -reflect;
+    lower[I] = 1.0;
+    lower[I + 1] = -trans;
+    upper[I] = -trans;
+    upper[I + 1] = 1.0;
+    rhs[I] = sourcem;
+    rhs[I + 1] = sourcep;
+    I += 2;
+  }
+  fluxes[3 * nlayer + 2] = (float)((double)solarflux * exp(-tau / mu0));
+  double gndflux = (1.0 - (double)gndemis) * (double)solarflux * exp(-tau / mu0);
+  const double skyflux = PI * (double)skyrad;
+  gndflux = gndflux + 0.0;  // no surface emission source
+  rhs[1] = skyflux;
+  diag[1] = 0.0;
+  upper[1] = 1.0;
+  diag[N] = -(1.0 - (double)gndemis);
+  lower[N] = 1.0;
+  rhs[N] = gndflux;
+  if (!tridiag(N, lower, diag, upper, rhs)) return false;
+  for (int L = 1, i = 1; L <= nlayer + 1; L++, i += 2) {
+    fluxes[3 * (L - 1) + 0] = (float)rhs[i];
+    fluxes[3 * (L - 1) + 1] = (float)rhs[i + 1];
+  }
+  return true;
+}
+
+bool init_radiance(Plan& p, std::string& err) {
+  const int nz = p.nz, nxy = p.nx1 * p.ny1, nlayer = nz - 1, npart = p.npart, ML = p.ml;
+  const int nl = p.nleg + 1;
+  const float PI = acosf(-1.0f);
+  const float C0 = sqrtf(1.0f / PI), C1 = sqrtf(3.0f / (4 * PI));
+  // sky radiance flux over the downwelling ordinates (INIT_SOLUTION shdomsub1.f:355-361)
+  float skyradalb = 0.f;
+  for (int imu = 0; imu < p.nmu / 2; imu++)
+    for (int ip = 0; ip < p.nphi0[imu]; ip++)
+      skyradalb = skyradalb + fabsf(p.mu[imu]) * p.wtdo[(size_t)imu * p.nphi0max + ip] * p.d.skyrad;
+  std::vector<float> od(nz), albs(nz), asym(nz), fl;
+  p.radiance0.assign((size_t)p.npts * 4, 0.f);
+  auto wt1 = [&](int ipa, int ip, int q) { return p.phaseinterpwt[((size_t)ipa * p.npts + ip) * 8 + q]; };
+  auto iph = [&](int ipa, int ip, int q) { return p.iphase[((size_t)ipa * p.npts + ip) * 8 + q]; };
+  for (int I = 0; I < nxy; I++) {
+    for (int iz = 1; iz <= nlayer; iz++) {
+      const int L = nz - iz;
+      const int ip0 = (iz - 1) + nz * I, ip1 = iz + nz * I;
+      const float ext0 = p.total_ext[ip0], ext1 = p.total_ext[ip1];
+      float scat0 = 0.f, scat1 = 0.f;
+      for (int ipa = 0; ipa < npart; ipa++) {
+        scat0 += p.albedo[(size_t)ipa * p.npts + ip0] * p.extinct[(size_t)ipa * p.npts + ip0];
+        scat1 += p.albedo[(size_t)ipa * p.npts + ip1] * p.extinct[(size_t)ipa * p.npts + ip1];
+      }
+      od[L - 1] = (p.zgrid[iz] - p.zgrid[iz - 1]) * (ext0 + ext1) / 2;
+      albs[L - 1] = (ext0 + ext1 > 0.f) ? (scat0 + scat1) / (ext0 + ext1) : 0.f;
+      float g0 = 0.f, g1 = 0.f;
+      for (int ipa = 0; ipa < npart; ipa++) {
+        float le0, le1;
+        // branch on species 1's leading weight, as the reference does
+        if (wt1(0, ip0, 0) >= 0.999f) {
+          le0 = p.legen[(size_t)(iph(ipa, ip0, 0) - 1) * nl + 1];
+        } else {
+          le0 = 0.f;
+          for (int q = 0; q < 8; q++) le0 = le0 + p.legen[(size_t)(iph(ipa, ip0, q) - 1) * nl + 1] * wt1(ipa, ip0, q);
+        }
+        if (wt1(0, ip1, 0) >= 0.999f) {
+          le1 = p.legen[(size_t)(iph(ipa, ip1, 0) - 1) * nl + 1];
+        } else {
+          le1 = 0.f;
+          for (int q = 0; q < 8; q++) le1 = le1 + p.legen[(size_t)(iph(ipa, ip1, q) - 1) * nl + 1] * wt1(ipa, ip1, q);
+        }
+        if (p.d.deltam) {
+          float f0, f1;
+          if (wt1(0, ip0, 0) >= 0.999f) {
+            f0 = p.legen[(size_t)(iph(ipa, ip0, 0) - 1) * nl + ML + 1];
+          } else {
+            f0 = 0.f;
+            for (int q = 0; q < 8; q++) f0 = f0 + p.legen[(size_t)(iph(ipa, ip0, q) - 1) * nl + ML + 1] * wt1(ipa, ip0, q);
+          }
+          if (wt1(0, ip1, 0) >= 0.999f) {
+            f1 = p.legen[(size_t)(iph(ipa, ip1, 0) - 1) * nl + ML + 1];
+          } else {
+            f1 = 0.f;
+            for (int q = 0; q < 8; q++) f1 = f1 + p.legen[(size_t)(iph(ipa, ip1, q) - 1) * nl + ML + 1] * wt1(ipa, ip1, q);
+          }
+          le0 = le0 / (1 - f0);
+          le1 = le1 / (1 - f1);
+        }
+        g0 += p.albedo[(size_t)ipa * p.npts + ip0] * p.extinct[(size_t)ipa * p.npts + ip0] * le0;
+        g1 += p.albedo[(size_t)ipa * p.npts + ip1] * p.extinct[(size_t)ipa * p.npts + ip1] * le1;
+      }
+      asym[L - 1] = (scat0 + scat1 > 0.f) ? (g0 + g1) / (scat0 + scat1) : 0.f;
+    }
+    const float gndemis = 1.0f - p.d.gndalbedo;
+    if (!eddrtf_solar(nlayer, od, albs, asym, p.d.solarflux, p.d.solarmu, gndemis, skyradalb, fl)) {
+      err = "EDDRTF: singular two-stream system";
+      return false;
+    }
+    for (int iz = 1; iz <= nz; iz++) {
+      const int L = nz + 1 - iz;
+      const size_t ip = (size_t)(iz - 1) + (size_t)nz * I;
+      p.radiance0[ip * 4 + 0] = C0 * (fl[3 * (L - 1) + 0] + fl[3 * (L - 1) + 1]);
+      p.radiance0[ip * 4 + 2] = C1 * (fl[3 * (L - 1) + 0] - fl[3 * (L - 1) + 1]);
+    }
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Discrete ordinate trace geometry (BACK_INT_GRID3D_UNPOL shdomsub1.f:3736-3785).
+// ---------------------------------------------------------------------------
+void make_ordinates(Plan& p) {
+  p.ords.clear();
+  const double PI = (double)acosf(-1.0f);  // ACOS(-1.0) of a REAL argument
+  const int c1 = p.gridptr[0 * 8 + 0], c8 = p.gridptr[0 * 8 + 7];
+  const double eps = (double)(1.0e-3f * (p.gridpos[c8 * 3 + 2] - p.gridpos[c1 * 3 + 2]));
+  for (int imu = 0; imu < p.nmu; imu++)
+    for (int ip = 0; ip < p.nphi0[imu]; ip++) {
+      Ordinate o;
+      o.imu = imu;
+      o.iphi = ip;
+      o.mu = p.mu[imu];
+      o.phi = p.phi[(size_t)imu * p.nphi0max + ip];
+      o.wtdo = p.wtdo[(size_t)imu * p.nphi0max + ip];
+      o.down = o.mu < 0.f;
+      const double st = (double)sqrtf(1.0f - o.mu * o.mu);
+      OrdGeom& g = o.g;
+      g.cx = st * cos((double)o.phi + PI);
+      g.cy = st * sin((double)o.phi + PI);
+      g.cz = -(double)o.mu;
+      if (fabs(g.cx) > (double)1.0e-5f) g.cxinv = 1.0 / g.cx; else { g.cx = 0.0; g.cxinv = 1.0e6; }
+      if (fabs(g.cy) > (double)1.0e-5f) g.cyinv = 1.0 / g.cy; else { g.cy = 0.0; g.cyinv = 1.0e6; }
+      g.czinv = 1.0 / g.cz;
+      g.bitx = g.cx < 0.0 ? 1 : 0;
+      g.bity = g.cy < 0.0 ? 1 : 0;
+      g.bitz = g.cz < -(double)1.0e-3f ? 1 : 0;
+      g.ioct = g.bitx + 2 * g.bity + 4 * g.bitz;
+      g.eps = eps;
+      p.ords.push_back(o);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Sweep order (SWEEPING_ORDER shdomsub1.f:3261-3351 with SWEEP_BASE_CELL
+// :4529-4619 for a tree-less base grid) and the dependency levels that let all
+// points of one level be integrated in parallel.
+// ---------------------------------------------------------------------------
+void sweeping_order(const Plan& p, int ioct0, std::vector<int32_t>& ord) {
+  const int ioct = ioct0 + 1, nz = p.nz, nxc = p.nxc, nyc = p.nyc, bc = p.bcflag;
+  int dix, six, eix, diy, siy, eiy, diz, siz, eiz;
+  if (btest(ioct - 1, 0)) {
+    dix = 1;
+    if (btest(bc, 0)) { six = nxc; eix = std::max(nxc - 1, 1); } else { six = 1; eix = nxc; }
+  } else {
+    dix = -1;
+    if (btest(bc, 0)) { six = 1; eix = std::min(2, nxc); } else { six = nxc; eix = 1; }
+  }
+  if (btest(ioct - 1, 1)) {
+    diy = 1;
+    if (btest(bc, 1)) { siy = nyc; eiy = std::max(nyc - 1, 1); } else { siy = 1; eiy = nyc; }
+  } else {
+    diy = -1;
+    if (btest(bc, 1)) { siy = 1; eiy = std::min(2, nyc); } else { siy = nyc; eiy = 1; }
+  }
+  if (btest(ioct - 1, 2)) { diz = 1; siz = 1; eiz = nz - 1; } else { diz = -1; siz = nz - 1; eiz = 1; }
+  std::vector<char> visited(p.npts, 0);
+  ord.clear();
+  ord.reserve(p.npts);
+  int ix = six, iy = siy, iz = siz;
+  for (;;) {
+    const int cell = (iz + (nz - 1) * (iy - 1) + (nz - 1) * nyc * (ix - 1)) - 1;
+    for (int k = 0; k < 8; k++) {
+      const int corner = k ^ (7 - ioct0);  // CORNDOG(k+1, IOCT) - 1
+      const int ipt = p.gridptr[(size_t)cell * 8 + corner];
+      if (visited[ipt]) continue;
+      visited[ipt] = 1;
+      ord.push_back((cell << 3) | corner);
+    }
+    if (ix == eix) {
+      if (iy == eiy) {
+        if (iz == eiz) break;
+        iz += diz;
+        iy = siy;
+      } else {
+        iy = (iy + diy + nyc - 1) % nyc + 1;
+      }
+      ix = six;
+    } else {
+      ix = (ix + dix + nxc - 1) % nxc + 1;
+    }
+  }
+}
+
+int build_levels(Plan& p, std::string& err) {
+  const GridTopo g{p.gridpos.data(), p.gridptr.data(), p.neighptr.data(), p.cellflags.data(),
+                   p.npts, p.ncells};
+  std::vector<char> top(p.npts, 0), bot(p.npts, 0);
+  for (int i : p.bctop) top[i] = 1;
+  for (int i : p.bcbot) bot[i] = 1;
+  p.nlevels_max = 0;
+  for (int o = 0; o < 8; o++) {
+    OctantPlan& op = p.oct[o];
+    sweeping_order(p, o, op.sweepord);
+    if ((int)op.sweepord.size() != p.npts) {
+      err = "SWEEPING_ORDER did not visit every grid point";
+      return AT3D_ERR_NUMERIC;
+    }
+    op.rank.assign(p.npts, 0);
+    for (int r = 0; r < p.npts; r++) op.rank[p.gridptr[(size_t)(op.sweepord[r] >> 3) * 8 + (op.sweepord[r] & 7)]] = r;
+    const bool up = (o >> 2) & 1;  // trace-back ray goes -z: upwelling ordinate
+    const std::vector<char>& preset = up ? bot : top;
+    std::vector<const Ordinate*> ords;
+    for (const Ordinate& od : p.ords)
+      if (od.g.ioct == o) ords.push_back(&od);
+    op.level.assign(p.npts, 0);
+    op.nslow = 0;
+    const int bitx = o & 1, bity = (o >> 1) & 1, bitz = (o >> 2) & 1;
+    for (int r = 0; r < p.npts; r++) {
+      const int cell = op.sweepord[r] >> 3, corner = op.sweepord[r] & 7;
+      const int pt = p.gridptr[(size_t)cell * 8 + corner];
+      if (preset[pt]) continue;
+      auto valid = [&](int q) { return preset[q] || op.rank[q] < r; };
+      int lvl = -1;
+      if (corner == o) {
+        // one-cell characteristic for every ordinate of the octant if all far faces are known
+        const int8_t fl = p.cellflags[cell];
+        int faces[3], nf = 0;
+        if (!(fl & 1)) faces[nf++] = 1 - bitx;
+        if (!(fl & 2)) faces[nf++] = 3 - bity;
+        faces[nf++] = 5 - bitz;
+        bool ok = true;
+        int mx = 0;
+        for (int f = 0; f < nf && ok; f++)
+          for (int c = 0; c < 4; c++) {
+            const int q = p.gridptr[(size_t)cell * 8 + face_corner(faces[f], c)];
+            if (!valid(q)) { ok = false; break; }
+            mx = std::max(mx, op.level[q]);
+          }
+        if (ok) lvl = mx + 1;
+      }
+      if (lvl < 0) {
+        op.nslow++;
+        int mx = 0;
+        for (const Ordinate* od : ords) {
+          int icell = cell;
+          double xe = p.gridpos[pt * 3], ye = p.gridpos[pt * 3 + 1], ze = p.gridpos[pt * 3 + 2];
+          bool done = false;
+          for (int step = 0; step <= p.ncells && !done; step++) {
+            TraceStep t;
+            if (!trace_step(g, od->g, icell, xe, ye, ze, t)) {
+              err = "BACK_INT_GRID: negative path length (SO<0)";
+              return AT3D_ERR_NUMERIC;
+            }
+            const bool vf = valid(t.i[0]) && valid(t.i[1]) && valid(t.i[2]) && valid(t.i[3]);
+            if (t.next < 0 || vf) {
+              if (!vf) {
+                err = "BACK_INT_GRID: reached a boundary face without valid radiances";
+                return AT3D_ERR_NUMERIC;
+              }
+              for (int c = 0; c < 4; c++) mx = std::max(mx, op.level[t.i[c]]);
+              done = true;
+            } else {
+              icell = t.next;
+            }
+          }
+          if (!done) {
+            err = "BACK_INT_GRID: characteristic did not terminate";
+            return AT3D_ERR_NUMERIC;
+          }
+        }
+        lvl = mx + 1;
+      }
+      op.level[pt] = lvl;
+    }
+    // level lists (level >= 1), ordered by (level, rank)
+    int nl = 0;
+    for (int i = 0; i < p.npts; i++) nl = std::max(nl, op.level[i]);
+    op.nlevels = nl;
+    p.nlevels_max = std::max(p.nlevels_max, nl);
+    std::vector<int32_t> cnt(nl + 2, 0);
+    for (int r = 0; r < p.npts; r++) {
+      const int pt = p.gridptr[(size_t)(op.sweepord[r] >> 3) * 8 + (op.sweepord[r] & 7)];
+      if (op.level[pt] > 0) cnt[op.level[pt]]++;
+    }
+    op.lvl_off.assign(nl + 1, 0);  // lvl_off[L-1] .. lvl_off[L] for level L
+    for (int L = 1; L <= nl; L++) op.lvl_off[L] = op.lvl_off[L - 1] + cnt[L];
+    op.lvl_pts.assign(op.lvl_off[nl], 0);
+    op.lvl_cell.assign(op.lvl_off[nl], 0);
+    std::vector<int32_t> fill(op.lvl_off.begin(), op.lvl_off.end());
+    for (int r = 0; r < p.npts; r++) {
+      const int cell = op.sweepord[r] >> 3;
+      const int pt = p.gridptr[(size_t)cell * 8 + (op.sweepord[r] & 7)];
+      const int L = op.level[pt];
+      if (L <= 0) continue;
+      const int pos = fill[L - 1]++;
+      op.lvl_pts[pos] = pt;
+      op.lvl_cell[pos] = cell;
+    }
+  }
+  return AT3D_OK;
+}
+
+}  // namespace
+
+int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err) {
+  p.d = desc;
+  const at3d_scene_desc& d = desc;
+  if (d.npx < 2 || d.npy < 2 || d.npz < 2) {
+    err = "grid must be at least 2x2x2 (1D/2D independent-pixel grids are not supported)";
+    return AT3D_ERR_ARG;
+  }
+  if (d.splitacc > 0.0f) {
+    err = "adaptive cell splitting (split_accuracy > 0) is not supported";
+    return AT3D_ERR_ARG;
+  }
+  if (d.transmin != 1.0f) {
+    err = "transmin must be 1.0 (one-cell short characteristics)";
+    return AT3D_ERR_ARG;
+  }
+  if (d.ip_flag != 0) {
+    err = "independent pixel modes are not supported (ip_flag must be 0)";
+    return AT3D_ERR_ARG;
+  }
+  if (!(d.solarmu < 0.0f && d.solarmu >= -1.0f)) {
+    err = "solarmu must be in [-1, 0)";
+    return AT3D_ERR_ARG;
+  }
+  if (d.npart < 1 || d.numphase < 1 || d.nlegp < 0) {
+    err = "need at least one species and one phase function";
+    return AT3D_ERR_ARG;
+  }
+  const int npg = d.npx * d.npy * d.npz;
+  p.zlevels.assign(d.zlevels, d.zlevels + d.npz);
+  for (int i = 1; i < d.npz; i++)
+    if (!(p.zlevels[i] > p.zlevels[i - 1])) {
+      err = "zlevels must increase";
+      return AT3D_ERR_ARG;
+    }
+  p.npart = d.npart;
+  p.extinctp.assign(d.extinctp, d.extinctp + (size_t)npg * d.npart);
+  p.albedop.assign(d.albedop, d.albedop + (size_t)npg * d.npart);
+  p.iphasep.assign(d.iphasep, d.iphasep + (size_t)npg * d.npart);
+  p.phasewtp.assign(d.phasewtp, d.phasewtp + (size_t)npg * d.npart);
+  for (int v : p.iphasep)
+    if (v < 1 || v > d.numphase) {
+      err = "Phase function indices are out of bounds.";
+      return AT3D_ERR_ARG;
+    }
+  // solver.py:1970-1990 and 2190-2200
+  p.nmu = std::max(2, 2 * ((d.nmu + 1) / 2));
+  p.nphi = std::max(1, d.nphi);
+  p.ml = p.nmu - 1;
+  p.mm = std::max(0, p.nphi / 2 - 1);
+  p.nlm = (2 * p.mm + 1) * (p.ml + 1) - p.mm * (p.mm + 1);
+  if (p.nlm < 4) {
+    err = "Insufficient Spherical Harmonics (NLM < 4)";
+    return AT3D_ERR_ARG;
+  }
+  p.nphi0max = p.nphi;
+  p.nleg = d.deltam ? p.ml + 1 : p.ml;
+  p.numphase = d.numphase;
+  // pad the phase table to nleg (solver.py:2385-2395)
+  const int nlegp_in = d.nlegp;
+  p.nlegp = std::max(nlegp_in, p.nleg);
+  p.nscatangle = std::max(36, std::min(721, 2 * p.nlegp));
+  p.legenp.assign((size_t)p.numphase * (p.nlegp + 1), 0.f);
+  for (int iph = 0; iph < p.numphase; iph++)
+    for (int l = 0; l <= nlegp_in; l++)
+      p.legenp[(size_t)iph * (p.nlegp + 1) + l] = d.legenp[(size_t)iph * (nlegp_in + 1) + l];
+  // _setup_grid (solver.py:2036-2168)
+  p.nx = d.npx;
+  p.ny = d.npy;
+  p.nz = d.npz;
+  p.ipflag = d.ip_flag;
+  p.bcflag = d.open_bc ? 3 : 0;
+  p.nx1 = p.nx + 1;
+  p.ny1 = p.ny + 1;
+  if ((p.bcflag & 5) || (p.ipflag & 1)) p.nx1 -= 1;
+  if ((p.bcflag & 7) || (p.ipflag & 2)) p.ny1 -= 1;
+  make_grid(p);
+  int rc = make_angle_set(p, err);
+  if (rc) return rc;
+  make_sh_tables(p);
+  interp_optics(p);
+  source_constants(p);
+  // MAKE_DIRECT
+  DirectBeam beam;
+  direct_beam_init(p, beam);
+  p.dirflux.assign(p.npts, 0.f);
+  for (int ip = 0; ip < p.npts; ip++)
+    if (!direct_beam_point(p, beam, p.gridpos[ip * 3], p.gridpos[ip * 3 + 1], p.gridpos[ip * 3 + 2],
+                           p.dirflux[ip])) {
+      err = "DIRECT_BEAM_PROP: ray left the property grid";
+      return AT3D_ERR_NUMERIC;
+    }
+  if (!init_radiance(p, err)) return AT3D_ERR_NUMERIC;
+  // BOUNDARY_PNTS (shdomsub1.f:2173-2215)
+  p.bctop.clear();
+  p.bcbot.clear();
+  for (int i = 0; i < p.npts; i++) {
+    if (p.gridpos[i * 3 + 2] >= p.zgrid[p.nz - 1]) p.bctop.push_back(i);
+    if (p.gridpos[i * 3 + 2] <= p.zgrid[0]) p.bcbot.push_back(i);
+  }
+  make_ordinates(p);
+  rc = build_levels(p, err);
+  if (rc) return rc;
+  return AT3D_OK;
+}
+
+}  // namespace at3d

@@ -1,0 +1,68 @@
+// Host-side setup of one SHDOM solve (CPU only).  See host_plan.cpp.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/at3d_hip.h"
+#include "shdom_geom.h"
+
+namespace at3d {
+
+struct Ordinate {
+  int imu, iphi;      // 0-based zenith / azimuth index
+  float mu, phi;
+  float wtdo;
+  OrdGeom g;          // trace-back geometry
+  bool down;
+};
+
+struct OctantPlan {
+  std::vector<int32_t> sweepord;   // [npts] (cell<<3)|corner in sweep order
+  std::vector<int32_t> rank;       // [npts] position of each point in sweepord
+  std::vector<int32_t> level;      // [npts] dependency level (0 = boundary preset)
+  std::vector<int32_t> lvl_pts;    // non-preset points sorted by (level, rank)
+  std::vector<int32_t> lvl_cell;   // their start cell
+  std::vector<int32_t> lvl_off;    // [nlevels+1] offsets into lvl_pts (level 1..)
+  int nlevels = 0;
+  int nslow = 0;                   // points needing multi-cell traces
+};
+
+struct Plan {
+  at3d_scene_desc d;               // copy of scalar parameters (pointers cleared)
+  // sizes
+  int nx, ny, nz, nx1, ny1, nxc, nyc, bcflag, ipflag, npts, ncells;
+  int nmu, nphi, nphi0max, nang, ml, mm, nlm, nleg, nlegp, numphase, npart, nscatangle;
+  // grid
+  std::vector<float> xgrid, ygrid, zgrid, zlevels, gridpos;
+  std::vector<int32_t> gridptr, neighptr;
+  std::vector<int8_t> cellflags;
+  // property grid (kept for the direct beam and the render phase table)
+  std::vector<float> extinctp, albedop, phasewtp, legenp;
+  std::vector<int32_t> iphasep;
+  // angles
+  std::vector<float> mu, wtmu, phi, wtdo;
+  std::vector<int32_t> nphi0;
+  std::vector<Ordinate> ords;
+  std::vector<int32_t> lofj, mofj;
+  std::vector<float> ymat, wmat, ylmsun;   // [nang][nlm], [nang][nlm], [nlm]
+  // optics on the RTE grid
+  std::vector<float> extinct, albedo, total_ext;  // [npart][npts], [npts]
+  std::vector<int32_t> iphase;                    // [npart][npts][8]
+  std::vector<float> phaseinterpwt;               // [npart][npts][8]
+  std::vector<float> legen;                       // [numphase][nleg+1] (post delta-M prep)
+  std::vector<float> chi, alb, tc;                // [npts][ml+1], [npts], [npts][ml+1]
+  float albmax = 0.f;
+  double extmin = 0, scatmin = 0;
+  // solution setup
+  std::vector<float> dirflux, radiance0;          // [npts], [npts][4]
+  std::vector<int32_t> bctop, bcbot;
+  OctantPlan oct[8];
+  int nlevels_max = 0;
+};
+
+// Builds the plan; returns AT3D_OK or an error status with a message.
+int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err);
+
+}  // namespace at3d

@@ -1,0 +1,268 @@
+"""The SHDOM radiative-transfer solver object on one MI355X.
+
+``RTE`` mirrors the reference's ``at3d.solver.RTE`` (at3d/solver.py:143-2951)
+for the scalar (NSTOKES=1) solar path on a fixed base grid: the constructor
+prepares the grid, optics and discrete ordinates, ``solve`` runs the
+source-function iteration on the GPU, ``integrate_to_sensor`` renders rays.
+All numerics run in libat3d_hip.so behind the C ABI of include/at3d_hip.h;
+this module only marshals arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import warnings
+from collections import OrderedDict
+
+import numpy as np
+
+from . import _lib
+from .configuration import get_config
+from .medium import Scatterer
+
+
+class RTE:
+    """Radiative transfer solver (at3d/solver.py:143)."""
+
+    def __init__(self, numerical_params, medium, source, surface, num_stokes=1, name=None,
+                 atmosphere=None):
+        if num_stokes != 1:
+            raise ValueError("only num_stokes=1 is supported by this engine")
+        if atmosphere is not None:
+            raise ValueError("atmosphere (thermal / gas absorption) is not supported")
+        self._nstokes = 1
+        self.numerical_params = dict(get_config()) if numerical_params is None else dict(numerical_params)
+        self.source = dict(source)
+        self.surface = dict(surface)
+        if self.source.get("srctype", "S") != "S":
+            raise ValueError("only solar sources are supported")
+        if self.surface.get("sfctype", "FL") != "FL":
+            raise ValueError("only fixed Lambertian surfaces are supported")
+        self.medium = self._setup_medium(medium)
+        self.wavelength = float(self.source["wavelength"])
+        self._name = "Radiance {:1.3f} micron".format(self.wavelength) if name is None else name
+        self._solcrit = None
+        self._iters = 0
+        self._state = None
+        self._plan = None
+        self._keep = []          # arrays referenced by the C descriptor
+        self._build_plan()
+
+    # -- setup -----------------------------------------------------------------
+    @staticmethod
+    def _setup_medium(medium):
+        if isinstance(medium, Scatterer):
+            md = OrderedDict(scatterer_000=medium)
+        elif isinstance(medium, dict):
+            md = OrderedDict(medium)
+        else:
+            md = OrderedDict(("scatterer_{:03d}".format(i), v) for i, v in enumerate(medium))
+        first = next(iter(md.values()))
+        for k, v in md.items():
+            if not isinstance(v, Scatterer):
+                raise TypeError(f"medium['{k}'] must be an at3d_amd.medium.Scatterer")
+            if v.shape != first.shape or not np.allclose(v.z, first.z) or \
+                    v.delx != first.delx or v.dely != first.dely:
+                raise ValueError("all scatterers must be on the same grid")
+        return md
+
+    def _build_plan(self):
+        npar = self.numerical_params
+        scs = list(self.medium.values())
+        npx, npy, npz = scs[0].shape
+        # concatenated Legendre table (solver.py:2340-2395)
+        maxleg = max(s.legcoef.shape[0] for s in scs)
+        tables, offset = [], 0
+        iphasep = []
+        for s in scs:
+            t = np.pad(s.legcoef, ((0, maxleg - s.legcoef.shape[0]), (0, 0)))
+            tables.append(t)
+            iphasep.append(np.asarray(s.table_index, np.int32) + offset)
+            offset += t.shape[1]
+        table = np.concatenate(tables, axis=1)            # (nleg+1, numphase)
+        legenp = _lib.as_f32(table.T)                     # [numphase][nlegp+1]
+        ext = _lib.as_f32(np.stack([s.extinction for s in scs]))
+        alb = _lib.as_f32(np.stack([s.ssalb for s in scs]))
+        iph = _lib.as_i32(np.stack(iphasep))
+        pw = _lib.as_f32(np.stack([s.phase_weights for s in scs]))
+        zl = _lib.as_f32(scs[0].z)
+        bcx, bcy = npar["x_boundary_condition"], npar["y_boundary_condition"]
+        if bcx != bcy:
+            raise ValueError("x and y boundary conditions must match (both 'open' or 'periodic')")
+        d = _lib.SceneDesc()
+        d.npx, d.npy, d.npz = npx, npy, npz
+        d.delx, d.dely = float(scs[0].delx), float(scs[0].dely)
+        d.zlevels = zl.ctypes.data_as(_lib._f32p)
+        d.npart = len(scs)
+        d.extinctp = ext.ctypes.data_as(_lib._f32p)
+        d.albedop = alb.ctypes.data_as(_lib._f32p)
+        d.iphasep = iph.ctypes.data_as(_lib._i32p)
+        d.phasewtp = pw.ctypes.data_as(_lib._f32p)
+        d.numphase = table.shape[1]
+        d.nlegp = table.shape[0] - 1
+        d.legenp = legenp.ctypes.data_as(_lib._f32p)
+        src, sfc = self.source, self.surface
+        d.solarflux, d.solarmu, d.solaraz = src["solarflux"], src["solarmu"], src["solaraz"]
+        d.skyrad, d.gndalbedo, d.wavelength = src.get("skyrad", 0.0), sfc["gndalbedo"], self.wavelength
+        d.nmu, d.nphi = int(npar["num_mu_bins"]), int(npar["num_phi_bins"])
+        d.open_bc = 1 if bcx == "open" else 0
+        d.ip_flag = int(npar["ip_flag"])
+        d.deltam = int(bool(npar["deltam"]))
+        d.accelflag = int(bool(npar["acceleration_flag"]))
+        d.high_order_radiance = int(bool(npar["high_order_radiance"]))
+        d.iterfixsh = int(npar["iterfixsh"])
+        d.angle_set = int(npar["angle_set"])
+        d.shacc = float(npar["spherical_harmonics_accuracy"])
+        d.solacc = float(npar["solution_accuracy"])
+        d.splitacc = float(npar["split_accuracy"])
+        d.transmin = float(npar["transmin"])
+        d.tautol = float(npar["tautol"])
+        d.transcut = float(npar["transcut"])
+        self._keep = [ext, alb, iph, pw, zl, legenp]
+        self._desc = d
+        L = _lib.lib()
+        h = C.c_void_p()
+        err = _lib.errbuf()
+        _lib.check(L.at3d_plan_create(C.byref(d), C.byref(h), err, len(err)), err)
+        self._plan = h
+        self._npts = int(L.at3d_plan_size(h, b"npts"))
+        self._nlm = int(L.at3d_plan_size(h, b"nlm"))
+        self._nang = int(L.at3d_plan_size(h, b"nang"))
+        self._ml = int(L.at3d_plan_size(h, b"ml"))
+        self._mm = int(L.at3d_plan_size(h, b"mm"))
+
+    # -- plan inspection ---------------------------------------------------------
+    def plan_size(self, key: str) -> int:
+        return int(_lib.lib().at3d_plan_size(self._plan, key.encode()))
+
+    _PLAN_SHAPES = {
+        "gridpos": ("f4", lambda s: (s._npts, 3)), "dirflux": ("f4", lambda s: (s._npts,)),
+        "total_ext": ("f4", lambda s: (s._npts,)), "radiance0": ("f4", lambda s: (s._npts, 4)),
+        "ymat": ("f4", lambda s: (s._nang, s._nlm)), "wmat": ("f4", lambda s: (s._nang, s._nlm)),
+        "ylmsun": ("f4", lambda s: (s._nlm,)), "mu": ("f4", lambda s: (s.plan_size("nmu"),)),
+        "wtmu": ("f4", lambda s: (s.plan_size("nmu"),)), "nphi0": ("i4", lambda s: (s.plan_size("nmu"),)),
+        "phi": ("f4", lambda s: (s.plan_size("nmu"), s.plan_size("nphi0max"))),
+        "wtdo": ("f4", lambda s: (s.plan_size("nmu"), s.plan_size("nphi0max"))),
+        "gridptr": ("i4", lambda s: (s.plan_size("ncells"), 8)),
+        "neighptr": ("i4", lambda s: (s.plan_size("ncells"), 6)),
+        "cellflags": ("i1", lambda s: (s.plan_size("ncells"),)),
+        "sweepord": ("i4", lambda s: (8, s._npts)), "level": ("i4", lambda s: (8, s._npts)),
+        "chi": ("f4", lambda s: (s._npts, s._ml + 1)), "alb": ("f4", lambda s: (s._npts,)),
+        "extinct": ("f4", lambda s: (s.plan_size("npart"), s._npts)),
+        "albedo": ("f4", lambda s: (s.plan_size("npart"), s._npts)),
+        "bcptr_top": ("i4", lambda s: (s.plan_size("ntop"),)),
+        "bcptr_bot": ("i4", lambda s: (s.plan_size("nbot"),)),
+        "xgrid": ("f4", lambda s: (s.plan_size("nx1"),)), "ygrid": ("f4", lambda s: (s.plan_size("ny1"),)),
+        "zgrid": ("f4", lambda s: (s.plan_size("nz"),)),
+    }
+
+    def plan_array(self, name: str) -> np.ndarray:
+        dt, shp = self._PLAN_SHAPES[name]
+        out = np.zeros(shp(self), dtype=dt)
+        err = _lib.errbuf()
+        _lib.check(_lib.lib().at3d_plan_get(self._plan, name.encode(), out.ctypes.data,
+                                            out.nbytes, err, len(err)), err)
+        return out
+
+    # -- solve -------------------------------------------------------------------
+    def _ensure_state(self):
+        if self._state is None:
+            h = C.c_void_p()
+            err = _lib.errbuf()
+            _lib.check(_lib.lib().at3d_state_create(self._plan, C.byref(h), err, len(err)), err)
+            self._state = h
+
+    def solve(self, maxiter, init_solution=True, setup_grid=True, verbose=True, solve=True):
+        """Source-function iterations (solver.py:279-631)."""
+        if not isinstance(verbose, bool):
+            raise TypeError("`verbose` should be a boolean.")
+        if not isinstance(init_solution, bool):
+            raise TypeError("`init_solution` should be a boolean.")
+        self._ensure_state()
+        L = _lib.lib()
+        err = _lib.errbuf()
+        if init_solution or self._solcrit is None:
+            _lib.check(L.at3d_init_solution(self._state, err, len(err)), err)
+            self._solcrit, self._iters = 1.0, 0
+        if not solve:
+            return
+        sc, it = C.c_float(), C.c_int32()
+        _lib.check(L.at3d_solve(self._state, int(maxiter), int(verbose), C.byref(sc), C.byref(it),
+                                err, len(err)), err)
+        self._solcrit, self._iters = float(sc.value), int(it.value)
+        if not self.check_solved(verbose=False):
+            warnings.warn("solution did not converge to solution_accuracy={} in maxiter={} "
+                          "(SOLCRIT={})".format(self.solution_accuracy, maxiter, self._solcrit))
+
+    def check_solved(self, verbose=True):
+        """True once SOLCRIT <= solution_accuracy (solver.py:1012-1035)."""
+        ok = self._solcrit is not None and self._solcrit <= self.solution_accuracy
+        if not ok and verbose:
+            warnings.warn("RTE is not solved to the requested accuracy")
+        return ok
+
+    @property
+    def solution_accuracy(self):
+        return float(self.numerical_params["solution_accuracy"])
+
+    @property
+    def num_iterations(self):
+        return self._iters
+
+    @property
+    def solcrit(self):
+        return self._solcrit
+
+    def state_array(self, name: str) -> np.ndarray:
+        """Copy a device array: source, radiance, delsource [npts, nlm]; fluxes [2, npts];
+        dirflux [npts]; ns, nr [npts]; do_radiance, do_source [nang, npts]."""
+        if self._state is None:
+            raise RuntimeError("solve() has not been called")
+        shapes = {"source": ((self._npts, self._nlm), np.float32),
+                  "radiance": ((self._npts, self._nlm), np.float32),
+                  "delsource": ((self._npts, self._nlm), np.float32),
+                  "fluxes": ((2, self._npts), np.float32), "dirflux": ((self._npts,), np.float32),
+                  "ns": ((self._npts,), np.int32), "nr": ((self._npts,), np.int32),
+                  "do_radiance": ((self._nang, self._npts), np.float32),
+                  "do_source": ((self._nang, self._npts), np.float32)}
+        shp, dt = shapes[name]
+        out = np.zeros(shp, dt)
+        err = _lib.errbuf()
+        _lib.check(_lib.lib().at3d_state_get(self._state, name.encode(), out.ctypes.data,
+                                             out.nbytes, err, len(err)), err)
+        return out
+
+    def timing(self, key: str) -> float:
+        if self._state is None:
+            return -1.0
+        return float(_lib.lib().at3d_state_timing(self._state, key.encode()))
+
+    @property
+    def _source(self):
+        return self.state_array("source")
+
+    @property
+    def _radiance(self):
+        return self.state_array("radiance")
+
+    @property
+    def _fluxes(self):
+        return self.state_array("fluxes")
+
+    @property
+    def _dirflux(self):
+        return self.plan_array("dirflux")
+
+    def release(self):
+        """Free device memory (the reference's _release_big_arrays, solver.py:2529)."""
+        if self._state is not None:
+            _lib.lib().at3d_state_destroy(self._state)
+            self._state = None
+
+    def __del__(self):
+        try:
+            self.release()
+            if self._plan is not None:
+                _lib.lib().at3d_plan_destroy(self._plan)
+                self._plan = None
+        except Exception:
+            pass

@@ -1,0 +1,128 @@
+"""Seeded synthetic media for the benchmark and parity workloads.
+
+Every generator returns plain numpy arrays (property grid = RTE base grid) so
+that the same inputs can drive this engine and any checker.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class Workload:
+    name: str
+    delx: float
+    dely: float
+    z: np.ndarray
+    species: list = field(default_factory=list)   # list of (ext, ssalb, legcoef[nleg+1, ntab])
+    bc: str = "open"
+    nmu: int = 16
+    nphi: int = 32
+    solacc: float = 1e-4
+    maxiter: int = 100
+    solarmu: float = -0.5
+    solaraz_deg: float = 0.0
+    gndalbedo: float = 0.05
+    wavelength: float = 0.86
+    deltam: bool = True
+
+    @property
+    def shape(self):
+        return self.species[0][0].shape
+
+
+def hg_legendre(g: float, nleg: int) -> np.ndarray:
+    """Henyey-Greenstein Legendre coefficients (2l+1) g^l, l = 0..nleg, as a one-table column."""
+    l = np.arange(nleg + 1, dtype=np.float64)
+    return ((2 * l + 1) * g ** l)[:, None]
+
+
+def lognormal_blob(n=16, dx=0.05, dz=0.04, radius=0.3, mean=10.0, sigma=0.5, seed=0,
+                   ssalb=0.999, g=0.85, nleg=7, bc="periodic", nmu=8, nphi=16, solacc=1e-5):
+    """Config-1 medium: lognormal extinction (mean `mean`) inside a sphere of
+    radius `radius` km about the domain centre, zero outside; HG phase
+    function with `nleg`+1 Legendre terms."""
+    rng = np.random.default_rng(seed)
+    x = np.arange(n) * dx
+    z = (np.arange(n) * dz).astype(np.float32)
+    X, Y, Z = np.meshgrid(x, x, z, indexing="ij")
+    c = np.array([x[-1] / 2 + dx / 2 if bc == "periodic" else x[-1] / 2, x[-1] / 2, z[-1] / 2])
+    if bc == "periodic":
+        c[1] = c[0]
+    r = np.sqrt((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2)
+    inside = r < radius
+    ext = np.zeros((n, n, n), np.float32)
+    ext[inside] = mean * np.exp(sigma * rng.standard_normal(inside.sum()) - 0.5 * sigma ** 2)
+    alb = np.full(ext.shape, ssalb, np.float32)
+    return Workload("lognormal_blob", dx, dx, z, [(ext, alb, hg_legendre(g, nleg))], bc=bc,
+                    nmu=nmu, nphi=nphi, solacc=solacc)
+
+
+def gaussian_field(shape, slope=-5.0 / 3.0, seed=0):
+    """Isotropic Gaussian random field with power spectrum |k|^slope (normalised to
+    zero mean, unit variance): white noise filtered by |k|^(slope/2) in Fourier space."""
+    rng = np.random.default_rng(seed)
+    noise = rng.standard_normal(shape)
+    ks = np.meshgrid(*[np.fft.fftfreq(s) for s in shape], indexing="ij")
+    k = np.sqrt(sum(kk ** 2 for kk in ks))
+    k[(0,) * len(shape)] = np.inf
+    f = np.real(np.fft.ifftn(np.fft.fftn(noise) * k ** (slope / 2.0)))
+    return (f - f.mean()) / f.std()
+
+
+def cloud_field(nx=64, ny=64, nz=40, dx=0.02, dz=0.025, z0=0.0, seed=0, cover=0.4,
+                ext_scale=30.0, cloud_base=0.5, cloud_top=1.2, rayleigh=0.01,
+                ssalb=0.9995, g=0.86, nleg_table=199, nmu=16, nphi=32, solacc=1e-4):
+    """Config-2 medium: a spectral-slope -5/3 Gaussian field thresholded at
+    (1-cover) quantile, extinction ext_scale * field**1.5 (field shifted to start
+    at 0 above the threshold) between cloud_base and cloud_top km, plus a
+    Rayleigh species of 0.01/km with albedo 1.  The 40 levels at 0.025 km span
+    0.975 km, so the vertical grid is placed at z0 + k*dz with z0 = 0.35 km, which
+    puts the 0.5-1.2 km cloud layer inside the domain."""
+    z = (z0 + np.arange(nz) * dz).astype(np.float32)
+    f = gaussian_field((nx, ny, nz), seed=seed)
+    thr = np.quantile(f, 1.0 - cover)
+    cl = np.clip(f - thr, 0.0, None)
+    cl = cl / max(cl.max(), 1e-12)
+    layer = (z >= cloud_base) & (z <= cloud_top)
+    ext = (ext_scale * cl ** 1.5 * layer[None, None, :]).astype(np.float32)
+    alb = np.full(ext.shape, ssalb, np.float32)
+    l = np.arange(nleg_table + 1, dtype=np.float64)
+    cloud_leg = (np.exp(-l / 30.0) * (2 * l + 1) * g ** l)
+    cloud_leg[0] = 1.0
+    rext = np.full(ext.shape, rayleigh, np.float32)
+    ralb = np.ones(ext.shape, np.float32)
+    rleg = np.zeros(3)
+    rleg[0], rleg[2] = 1.0, 0.5   # Rayleigh phase function: (2l+1) chi_l with chi_2 = 0.1
+    return Workload("cloud_field", dx, dx, z,
+                    [(ext, alb, cloud_leg[:, None]), (rext, ralb, rleg[:, None])],
+                    bc="open", nmu=nmu, nphi=nphi, solacc=solacc)
+
+
+def workload(name: str) -> Workload:
+    if name == "blob16":
+        w = lognormal_blob(n=16, nleg=64)
+    elif name == "blob16_l7":
+        w = lognormal_blob(n=16, nleg=7)
+    elif name == "cloud64":
+        w = cloud_field(z0=0.35)
+    else:
+        raise KeyError(name)
+    return w
+
+
+def make_rte(w: Workload, **overrides):
+    """An RTE for a workload (solar source, fixed Lambertian surface)."""
+    from . import configuration, source, surface
+    from .medium import Scatterer
+    from .solver import RTE
+    cfg = configuration.get_config(num_mu_bins=w.nmu, num_phi_bins=w.nphi,
+                                   x_boundary_condition=w.bc, y_boundary_condition=w.bc,
+                                   solution_accuracy=w.solacc, deltam=w.deltam)
+    cfg.update(overrides)
+    med = {f"species_{i}": Scatterer(e, a, leg, w.delx, w.dely, w.z)
+           for i, (e, a, leg) in enumerate(w.species)}
+    return RTE(cfg, med, source.solar(w.wavelength, w.solarmu, w.solaraz_deg),
+               surface.lambertian(w.gndalbedo))

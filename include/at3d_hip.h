@@ -1,0 +1,134 @@
+/*
+ * at3d_hip.h -- C ABI of the MI355X SHDOM engine (libat3d_hip.so).
+ *
+ * Plain C types only: pointers, sizes, status codes.  Every entry point
+ * returns 0 on success and a nonzero status on failure, with a NUL-terminated
+ * message in the caller's `err` buffer; nothing aborts the process.  Host
+ * arrays are C-ordered as documented per argument.
+ *
+ * The entry points replace the reference's f2py boundary (`at3d.core`, generated
+ * from CloudTomography/AT3D tools/generate_pyf.py:51-149) for the solve / render
+ * path; the reference routine each one stands in for is cited on it.
+ */
+#ifndef AT3D_HIP_H
+#define AT3D_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AT3D_OK 0
+#define AT3D_ERR_ARG 1     /* invalid argument / unsupported option      */
+#define AT3D_ERR_MEMORY 2  /* device or host allocation failed           */
+#define AT3D_ERR_DEVICE 3  /* HIP runtime error                          */
+#define AT3D_ERR_NUMERIC 4 /* the reference would STOP here (bad trace)  */
+
+/* Scene and numerical parameters.  Mirrors what the reference's RTE object
+ * passes to the Fortran core: the property grid (at3d/solver.py:2036-2168,
+ * 2324-2528), source (source.py:14-79), Lambertian surface (surface.py:28-105)
+ * and numerical parameters (default_config.json, solver.py:1960-2034). */
+typedef struct at3d_scene_desc {
+  int32_t npx, npy, npz;   /* property grid = base RTE grid                    */
+  float delx, dely;        /* km                                               */
+  const float* zlevels;    /* [npz] km, increasing                             */
+  int32_t npart;           /* number of scattering species                     */
+  const float* extinctp;   /* [npart][npx][npy][npz]  1/km                      */
+  const float* albedop;    /* [npart][npx][npy][npz]                            */
+  const int32_t* iphasep;  /* [npart][npx][npy][npz] 1-based into legenp table   */
+  const float* phasewtp;   /* [npart][npx][npy][npz] phase weights (num_micro=1) */
+  int32_t numphase;        /* number of tabulated phase functions              */
+  int32_t nlegp;           /* highest Legendre index in legenp                 */
+  const float* legenp;     /* [numphase][nlegp+1], coefficients incl. (2l+1)   */
+  /* source / surface */
+  float solarflux;         /* flux on a horizontal surface                     */
+  float solarmu;           /* cosine of solar zenith, direction of propagation (<0) */
+  float solaraz;           /* radians                                          */
+  float skyrad;            /* isotropic downwelling radiance at the domain top */
+  float gndalbedo;         /* fixed Lambertian surface albedo                  */
+  float wavelength;        /* microns (informational for a solar source)      */
+  /* numerical parameters */
+  int32_t nmu, nphi;       /* num_mu_bins, num_phi_bins                         */
+  int32_t open_bc;         /* 1: open x/y boundaries, 0: periodic              */
+  int32_t ip_flag;         /* independent pixel bits (0 = 3D)                  */
+  int32_t deltam;
+  int32_t accelflag;
+  int32_t high_order_radiance;
+  int32_t iterfixsh;
+  int32_t angle_set;       /* 1 gaussian, 2 reduced gaussian                    */
+  float shacc;             /* spherical_harmonics_accuracy                     */
+  float solacc;            /* solution_accuracy                                */
+  float splitacc;          /* split_accuracy; must be <= 0 (no cell splitting) */
+  float transmin;          /* must be 1.0 (one-cell characteristics)           */
+  double tautol;           /* render sub-step optical depth                   */
+  double transcut;         /* render transmission cutoff                      */
+} at3d_scene_desc;
+
+typedef struct at3d_plan at3d_plan;   /* host-side setup (no GPU needed)     */
+typedef struct at3d_state at3d_state; /* device-resident solver state       */
+
+/* Library version string. */
+const char* at3d_version(void);
+
+/* Host setup of one solve: base grid and cells (NEW_GRIDS / INIT_CELL_STRUCTURE,
+ * shdomsub2.f:16-309), property transfer with delta-M (TRANSFER_PA_TO_GRID,
+ * shdomsub1.f:19-112), discrete ordinates and SH transform tables
+ * (MAKE_ANGLE_SET / MAKE_SH_DO_COEF, shdomsub2.f:1061-1220), the direct beam
+ * (MAKE_DIRECT, shdomsub2.f:393-473), the Eddington initial radiance
+ * (INIT_RADIANCE, shdomsub2.f:614-760), boundary points (BOUNDARY_PNTS,
+ * shdomsub1.f:2173) and the per-octant sweep order (SWEEPING_ORDER,
+ * shdomsub1.f:3261) with its dependency levels.  Runs on the CPU only. */
+int at3d_plan_create(const at3d_scene_desc* desc, at3d_plan** out, char* err, size_t errlen);
+void at3d_plan_destroy(at3d_plan* plan);
+
+/* Integer sizes of a plan: keys "npts","ncells","nx1","ny1","nz","nlm","ml","mm",
+ * "nmu","nang","nphi0max","bcflag","ipflag","ntop","nbot","nlevels_max",
+ * "npart","numphase","nleg","nlegp","nscatangle".  Returns -1 for an unknown key. */
+int64_t at3d_plan_size(const at3d_plan* plan, const char* key);
+
+/* Copy a named plan array into `dst` (capacity `nbytes`).  Names and layouts:
+ *  "gridpos" f32[npts][3]      "gridptr" i32[ncells][8]   "neighptr" i32[ncells][6]
+ *  "cellflags" i8[ncells]      "xgrid" f32[nx1] "ygrid" f32[ny1] "zgrid" f32[nz]
+ *  "mu" f32[nmu] "wtmu" f32[nmu] "nphi0" i32[nmu] "phi" f32[nmu][nphi0max]
+ *  "wtdo" f32[nmu][nphi0max]   "ymat" f32[nang][nlm]  "wmat" f32[nang][nlm]
+ *  "ylmsun" f32[nlm]           "total_ext" f32[npts]  "extinct" f32[npart][npts]
+ *  "albedo" f32[npart][npts]   "dirflux" f32[npts]    "radiance0" f32[npts][4]
+ *  "chi" f32[npts][ml+1]       "alb" f32[npts]        "sweepord" i32[8][npts]
+ *  "level" i32[8][npts]        "bcptr_top" i32[ntop]  "bcptr_bot" i32[nbot]
+ * Sweep order entries are (cell<<3)|corner, 0-based, indexed by the octant
+ * number IOCT-1 of the reference (not its JOCT slot). */
+int at3d_plan_get(const at3d_plan* plan, const char* name, void* dst, size_t nbytes,
+                  char* err, size_t errlen);
+
+/* Device state for a plan on the current HIP device (allocates HBM). */
+int at3d_state_create(const at3d_plan* plan, at3d_state** out, char* err, size_t errlen);
+void at3d_state_destroy(at3d_state* st);
+
+/* (Re)initialise the solution: initial radiance, first source function
+ * (INIT_SOLUTION, shdomsub1.f:113-442). */
+int at3d_init_solution(at3d_state* st, char* err, size_t errlen);
+
+/* Source-function iterations until SOLCRIT <= solacc or `maxiter` total
+ * iterations (SOLUTION_ITERATIONS, shdomsub1.f:445-818).  Continues from the
+ * current state; writes the final criterion and total iteration count. */
+int at3d_solve(at3d_state* st, int32_t maxiter, int32_t verbose, float* solcrit_out,
+               int32_t* iters_out, char* err, size_t errlen);
+
+/* Copy a named device array to host.  Names: "source" f32[npts][nlm],
+ * "radiance" f32[npts][nlm], "delsource" f32[npts][nlm], "fluxes" f32[2][npts],
+ * "dirflux" f32[npts], "ns" i32[npts], "nr" i32[npts], "do_radiance" f32[nang][npts],
+ * "do_source" f32[nang][npts] (max(0,S)*ext). */
+int at3d_state_get(at3d_state* st, const char* name, void* dst, size_t nbytes, char* err,
+                   size_t errlen);
+
+/* Time (ms, HIP events on the state's stream) of each kernel class over the
+ * last at3d_solve call: keys "sh_to_do","sweep","do_to_sh","source","total";
+ * also "iters".  Returns a negative value for unknown keys. */
+double at3d_state_timing(const at3d_state* st, const char* key);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AT3D_HIP_H */

@@ -1,0 +1,546 @@
+"""ctypes driver for the reference SHDOM Fortran core (test oracle only).
+
+TEST INFRASTRUCTURE.  This module is imported only by ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py``, and
+only as the checker or the CPU baseline -- never by the product package.
+
+It loads ``oracle/_ref/libshdomref.so`` (built from the reference's own Fortran
+sources by ``oracle/Makefile``) and replays, call for call, the host sequence
+that the reference's Python solver object performs:
+
+* ``RTE._setup_grid``                  at3d/solver.py:2036-2168  (NEW_GRIDS, INIT_CELL_STRUCTURE)
+* ``RTE._setup_memory``                at3d/solver.py:2170-2322
+* ``RTE._prepare_optical_properties``  at3d/solver.py:2324-2528  (TRANSFER_PA_TO_GRID)
+* ``RTE._init_solution``               at3d/solver.py:2539-2798  (MAKE_ANGLE_SET, MAKE_SH_DO_COEF, INIT_SOLUTION)
+* ``RTE.solve``                        at3d/solver.py:279-631    (SOLUTION_ITERATIONS)
+* ``RTE.integrate_to_sensor``          at3d/solver.py:633-791    (PRECOMPUTE_PHASE_CHECK, RENDER)
+
+Fortran calling convention (LLVM flang): every argument by reference, symbol
+name lower case with a trailing underscore, and one hidden ``size_t`` length per
+CHARACTER argument appended after the explicit arguments, in order.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "_ref", "libshdomref.so")
+_lib = None
+
+
+def available() -> bool:
+    return os.path.exists(LIBPATH)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not available():
+            raise FileNotFoundError(
+                f"{LIBPATH} missing: build it with `make -C oracle` (needs the reference sources)")
+        _lib = C.CDLL(LIBPATH)
+    return _lib
+
+
+# ----------------------------------------------------------------------------
+# argument marshalling
+# ----------------------------------------------------------------------------
+class _Ref:
+    """A scalar passed by reference; ``.value`` reads it back after the call."""
+
+    def __init__(self, ctype, value):
+        self.obj = ctype(value)
+
+    @property
+    def value(self):
+        return self.obj.value
+
+
+def i4(v=0):
+    return _Ref(C.c_int32, int(v))
+
+
+def r4(v=0.0):
+    return _Ref(C.c_float, float(v))
+
+
+def r8(v=0.0):
+    return _Ref(C.c_double, float(v))
+
+
+def lg(v=False):
+    return _Ref(C.c_int32, 1 if v else 0)
+
+
+class _Str:
+    def __init__(self, s: str, n: int | None = None):
+        n = len(s) if n is None else n
+        self.n = n
+        self.buf = C.create_string_buffer(s.encode().ljust(n)[:n], n)
+
+    @property
+    def value(self):
+        return self.buf.raw.decode(errors="replace").strip()
+
+
+def s(v, n=None):
+    return _Str(v, n)
+
+
+def farr(a, dtype):
+    """Fortran-ordered contiguous copy (the object passed by address)."""
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+def fcall(name, *args):
+    fn = getattr(lib(), name.lower() + "_")
+    cargs, lens = [], []
+    for a in args:
+        if isinstance(a, np.ndarray):
+            if not (a.flags.f_contiguous or a.ndim <= 1):
+                raise ValueError(f"{name}: non Fortran-contiguous array")
+            cargs.append(C.c_void_p(a.ctypes.data))
+        elif isinstance(a, _Ref):
+            cargs.append(C.byref(a.obj))
+        elif isinstance(a, _Str):
+            cargs.append(a.buf)
+            lens.append(C.c_size_t(a.n))
+        else:
+            raise TypeError(f"{name}: bad argument type {type(a)}")
+    fn.restype = None
+    fn(*cargs, *lens)
+
+
+def _check(ierr, errmsg, where):
+    if ierr.value != 0:
+        raise RuntimeError(f"{where}: IERR={ierr.value}: {errmsg.value}")
+
+
+# ----------------------------------------------------------------------------
+# scene description (plain numpy; mirrors the xr.Dataset fields the reference reads)
+# ----------------------------------------------------------------------------
+@dataclass
+class Species:
+    extinction: np.ndarray          # (npx, npy, npz)
+    ssalb: np.ndarray               # (npx, npy, npz)
+    legcoef: np.ndarray             # (nleg+1, ntable) scalar I-I coefficients incl. (2l+1)
+    table_index: np.ndarray | None = None   # (npx,npy,npz) 1-based, default 1
+    phase_weights: np.ndarray | None = None
+
+
+@dataclass
+class Scene:
+    delx: float
+    dely: float
+    zlevels: np.ndarray             # (npz,)
+    species: list
+    solarmu: float = -0.5           # direction of propagation (negative)
+    solaraz: float = 0.0            # radians
+    solarflux: float = 1.0
+    skyrad: float = 0.0
+    gndalbedo: float = 0.05
+    gndtemp: float = 298.15
+    wavelength: float = 0.86
+    # numerical parameters (default_config.json names)
+    num_mu_bins: int = 8
+    num_phi_bins: int = 16
+    bc: str = "periodic"            # 'open' or 'periodic' for both x and y
+    deltam: bool = True
+    shacc: float = 0.0
+    accelflag: bool = True
+    solacc: float = 1e-5
+    splitacc: float = 0.0
+    high_order_radiance: bool = False
+    iterfixsh: int = 30
+    tautol: float = 0.1
+    angle_set: int = 2
+    transcut: float = 1e-5
+    transmin: float = 1.0
+    ip_flag: int = 0
+
+    @property
+    def shape(self):
+        return self.species[0].extinction.shape
+
+
+@dataclass
+class RefState:
+    """Everything the reference's RTE object holds after solve()."""
+    sc: Scene
+    d: dict = field(default_factory=dict)
+
+    def __getattr__(self, k):
+        try:
+            return self.__dict__["d"][k]
+        except KeyError as e:
+            raise AttributeError(k) from e
+
+
+def setup(sc: Scene) -> RefState:
+    st = RefState(sc)
+    d = st.d
+    npx, npy, npz = sc.shape
+    # _setup_numerical_params (solver.py:1970-1990)
+    nmu = max(2, 2 * int((sc.num_mu_bins + 1) / 2))
+    nphi = max(1, sc.num_phi_bins)
+    # _setup_grid (solver.py:2036-2168)
+    nx, ny, nz = npx, npy, npz
+    ipflag = sc.ip_flag
+    if nx == 1:
+        ipflag |= 1
+    if ny == 1:
+        ipflag |= 2
+    bcflag = 0
+    if sc.bc == "open" and ipflag in (0, 2, 4, 6):
+        bcflag += 1
+    if sc.bc == "open" and ipflag in (0, 1, 4, 5):
+        bcflag += 2
+    nx1, ny1 = nx + 1, ny + 1
+    if (bcflag & 5) or (ipflag & 1):
+        nx1 -= 1
+    if (bcflag & 7) or (ipflag & 2):
+        ny1 -= 1
+    nbpts = nx1 * ny1 * nz
+    nbcells = (nz - 1) * (nx + (bcflag & 1) - ((bcflag >> 2) & 1)) * \
+        (ny + ((bcflag >> 1) & 1) - ((bcflag >> 3) & 1))
+    maxpg = npx * npy * npz
+    xgrid = np.zeros(nx + 1, np.float32, order="F")
+    ygrid = np.zeros(ny + 1, np.float32, order="F")
+    zgrid = np.zeros(nz, np.float32, order="F")
+    zlev = farr(sc.zlevels, np.float32)
+    fcall("NEW_GRIDS", i4(bcflag), s("P"), i4(npx), i4(npy), i4(npz), i4(nx), i4(ny), i4(nz),
+          r4(0.0), r4(0.0), r4(sc.delx), r4(sc.dely), zlev, xgrid, ygrid, zgrid)
+    xgrid = np.asfortranarray(xgrid[:nx1])
+    ygrid = np.asfortranarray(ygrid[:ny1])
+    # _setup_memory (solver.py:2170-2322) with adapt_grid_factor = 1 (no cell splitting)
+    ml = nmu - 1
+    mm = max(0, int(nphi / 2) - 1)
+    nlm = (2 * mm + 1) * (ml + 1) - mm * (mm + 1)
+    nphi0max = nphi
+    maxig = nbpts
+    maxic = max(int(1.5 * maxig), nbcells)
+    maxiv = max(nlm * maxig, nbpts * 4)
+    maxido = maxig * nphi0max
+    maxnbc = int(maxig * 3 / nz)
+    npts, ncells = i4(), i4()
+    gridpos = np.zeros((3, maxig), np.float32, order="F")
+    gridptr = np.zeros((8, maxic), np.int32, order="F")
+    neighptr = np.zeros((6, maxic), np.int32, order="F")
+    treeptr = np.zeros((2, maxic), np.int32, order="F")
+    cellflags = np.zeros(maxic, np.int16, order="F")
+    fcall("INIT_CELL_STRUCTURE", i4(bcflag), i4(ipflag), i4(nx), i4(ny), i4(nz), i4(nx1), i4(ny1),
+          npts, ncells, xgrid, ygrid, zgrid, gridpos, i4(maxic), i4(maxig),
+          gridptr, neighptr, treeptr, cellflags)
+    npts, ncells = npts.value, ncells.value
+    d.update(nmu=nmu, nphi=nphi, ml=ml, mm=mm, nlm=nlm, nphi0max=nphi0max, nx=nx, ny=ny, nz=nz,
+             nx1=nx1, ny1=ny1, npx=npx, npy=npy, npz=npz, bcflag=bcflag, ipflag=ipflag,
+             nbpts=nbpts, nbcells=nbcells, maxpg=maxpg, maxig=maxig, maxic=maxic, maxiv=maxiv,
+             maxido=maxido, maxnbc=maxnbc, xgrid=xgrid, ygrid=ygrid, zgrid=zgrid, zlevels=zlev,
+             npts=npts, ncells=ncells, gridpos=gridpos, gridptr=gridptr, neighptr=neighptr,
+             treeptr=treeptr, cellflags=cellflags, maxbcrad=2 * maxnbc)
+    _prepare_optical_properties(st)
+    return st
+
+
+def _prepare_optical_properties(st: RefState):
+    """solver.py:2324-2528 for NSTOKES=1."""
+    sc, d = st.sc, st.d
+    maxpg, npart = d["maxpg"], len(sc.species)
+    extinctp = np.zeros((maxpg, npart), np.float32, order="F")
+    albedop = np.zeros((maxpg, npart), np.float32, order="F")
+    maxnmicro = 1
+    iphasep = np.zeros((maxnmicro, maxpg, npart), np.int32, order="F")
+    phasewtp = np.zeros((maxnmicro, maxpg, npart), np.float32, order="F")
+    tables = []
+    for i, sp in enumerate(sc.species):
+        extinctp[:, i] = np.asarray(sp.extinction, np.float32).ravel()
+        albedop[:, i] = np.asarray(sp.ssalb, np.float32).ravel()
+        ti = np.ones(maxpg, np.int32) if sp.table_index is None else \
+            np.asarray(sp.table_index, np.int32).ravel()
+        iphasep[0, :, i] = ti + iphasep.max()
+        pw = np.ones(maxpg, np.float32) if sp.phase_weights is None else \
+            np.asarray(sp.phase_weights, np.float32).ravel()
+        phasewtp[0, :, i] = pw
+        tables.append(np.asarray(sp.legcoef, np.float64))
+    iphasep[iphasep == 0] = 1
+    maxleg = max(t.shape[0] for t in tables)
+    table = np.concatenate([np.pad(t, ((0, maxleg - t.shape[0]), (0, 0))) for t in tables], axis=1)
+    numphase = table.shape[1]
+    nleg = d["ml"] + 1 if sc.deltam else d["ml"]
+    nlegp = max(table.shape[0] - 1, nleg)
+    nscatangle = max(36, min(721, 2 * nlegp))
+    if nlegp + 1 > table.shape[0]:
+        table = np.pad(table, ((0, 1 + nlegp - table.shape[0]), (0, 0)))
+    legenp = farr(table.ravel(order="F"), np.float32)
+    tempp = np.zeros(maxpg, np.float32) + 273.0
+    maxig = d["maxig"]
+    temp = np.zeros(maxig, np.float32, order="F")
+    planck = np.zeros((maxig, npart), np.float32, order="F")
+    extinct = np.zeros((maxig, npart), np.float32, order="F")
+    albedo = np.zeros((maxig, npart), np.float32, order="F")
+    legen = np.zeros((1, nleg + 1, numphase), np.float32, order="F")
+    iphase = np.zeros((8 * maxnmicro, maxig, npart), np.int32, order="F")
+    total_ext = np.zeros(maxig, np.float32, order="F")
+    phaseinterpwt = np.zeros((8 * maxnmicro, maxig, npart), np.float32, order="F")
+    extmin, scatmin, albmax = r8(), r8(), r4()
+    ierr, errmsg = i4(), s("", 600)
+    zckd = np.zeros(1, np.float32)
+    gasabs = np.zeros(1, np.float32)
+    fcall("TRANSFER_PA_TO_GRID", i4(d["ml"]), i4(d["mm"]), i4(maxig), i4(nleg), i4(numphase),
+          lg(sc.deltam), s("S"), s("R"), farr([10000, 10001], np.float32), r4(sc.wavelength),
+          temp, planck, extinct, albedo, legen, iphase, i4(d["npts"]), d["gridpos"],
+          i4(d["npx"]), i4(d["npy"]), i4(d["npz"]), r4(sc.delx), r4(sc.dely), r4(0.0), r4(0.0),
+          d["zlevels"], tempp, extinctp, i4(maxpg), albedop, legenp, iphasep, i4(0), zckd, gasabs,
+          i4(npart), total_ext, extmin, scatmin, albmax, i4(1), s("ON"), ierr, errmsg,
+          phaseinterpwt, r4(0.999), i4(nlegp), phasewtp, i4(maxnmicro))
+    _check(ierr, errmsg, "TRANSFER_PA_TO_GRID")
+    d.update(extinctp=extinctp, albedop=albedop, iphasep=iphasep, phasewtp=phasewtp,
+             legenp=legenp, table=table, numphase=numphase, nleg=nleg, nlegp=nlegp,
+             nscatangle=nscatangle, tempp=tempp, temp=temp, planck=planck, extinct=extinct,
+             albedo=albedo, legen=legen, iphase=iphase, total_ext=total_ext,
+             phaseinterpwt=phaseinterpwt, extmin=extmin.value, scatmin=scatmin.value,
+             albmax=albmax.value, npart=npart, maxnmicro=maxnmicro, zckd=zckd, gasabs=gasabs)
+
+
+def make_angle_set(nmu, nphi, itype, nphi0max):
+    nphi0 = np.zeros(nmu, np.int32)
+    mu = np.zeros(nmu, np.float32)
+    phi = np.zeros((nmu, nphi0max), np.float32, order="F")
+    wtmu = np.zeros(nmu, np.float32)
+    wtdo = np.zeros((nmu, nphi0max), np.float32, order="F")
+    fftflag = np.zeros(nmu, np.int32)
+    nang = i4()
+    fcall("MAKE_ANGLE_SET", i4(nmu), i4(nphi), i4(itype), i4(nphi0max), nphi0, mu, phi, wtmu,
+          wtdo, fftflag, nang)
+    return dict(nphi0=nphi0, mu=mu, phi=phi, wtmu=wtmu, wtdo=wtdo, fftflag=fftflag,
+                nang=nang.value)
+
+
+def make_sh_do_coef(ml, mm, nlm, nmu, nphi0, nphi0max, fftflag, mu, phi, wtmu, wtdo):
+    cmu1 = np.zeros((1, nlm, nmu), np.float32, order="F")
+    cmu2 = np.zeros((1, nmu, nlm), np.float32, order="F")
+    cphi1 = np.zeros((33, 32, nmu), np.float32, order="F")
+    cphi2 = np.zeros((32, 33, nmu), np.float32, order="F")
+    wphisave = np.zeros((3 * nphi0max + 15, nmu), np.float32, order="F")
+    fcall("MAKE_SH_DO_COEF", i4(1), i4(ml), i4(mm), i4(nlm), i4(nmu), nphi0, i4(nphi0max),
+          fftflag, mu, phi, wtmu, wtdo, cmu1, cmu2, cphi1, cphi2, wphisave)
+    return dict(cmu1=cmu1, cmu2=cmu2, cphi1=cphi1, cphi2=cphi2, wphisave=wphisave)
+
+
+def init_solution(st: RefState):
+    """RTE._init_solution (solver.py:2539-2798) for a fixed Lambertian surface, solar source."""
+    sc, d = st.sc, st.d
+    nmu, nphi0max, nstokes = d["nmu"], d["nphi0max"], 1
+    ang = make_angle_set(nmu, d["nphi"], sc.angle_set, nphi0max)
+    d.update(ang)
+    d.update(make_sh_do_coef(d["ml"], d["mm"], d["nlm"], nmu, ang["nphi0"], nphi0max,
+                             ang["fftflag"], ang["mu"], ang["phi"], ang["wtmu"], ang["wtdo"]))
+    nang = ang["nang"]
+    skyrad = np.full((1, nmu // 2, nphi0max), sc.skyrad, np.float32, order="F")
+    surface_rad = np.zeros((nang // 2 + 1, d["nx1"] + 1, d["ny1"] + 1), np.float32, order="F")
+    maxig, maxiv, maxic, maxido = d["maxig"], d["maxiv"], d["maxic"], d["maxido"]
+    maxnbc, maxbcrad, maxpg = d["maxnbc"], d["maxbcrad"], d["maxpg"]
+    source = np.zeros((1, maxiv), np.float32, order="F")
+    ndel = maxiv if sc.accelflag else 1
+    delsource = np.zeros((1, ndel), np.float32, order="F")
+    radiance = np.zeros((1, maxiv + maxig), np.float32, order="F")
+    dirflux = np.zeros(maxig, np.float32, order="F")
+    work1 = np.zeros(8 * maxig, np.int32, order="F")
+    work = np.zeros(maxido, np.float32, order="F")
+    work2_size = max(maxig, maxic)
+    work2 = np.zeros(work2_size, np.float32, order="F")
+    bcrad = np.zeros((1, maxbcrad), np.float32, order="F")
+    fluxes = np.zeros((2, maxig), np.float32, order="F")
+    shptr = np.zeros(maxig + 1, np.int32)
+    rshptr = np.zeros(maxig + 2, np.int32)
+    oshptr = np.zeros(maxig + 1, np.int32)
+    maxsfcpars = 4
+    sfcgridparms = np.zeros(maxsfcpars * maxnbc, np.float32)
+    bcptr = np.zeros((maxnbc, 2), np.int32, order="F")
+    ylmsun = np.zeros((1, d["nlm"]), np.float32, order="F")
+    extdirp = np.zeros(maxpg, np.float32)
+    sfc_brdf_do = np.zeros((1, nmu // 2, nphi0max, 1, nmu // 2, nphi0max), np.float32, order="F")
+    ntoppts, nbotpts = i4(), i4()
+    uniformzlev = r8()
+    cx, cy, cz, cxinv, cyinv, czinv = r8(), r8(), r8(), r8(), r8(), r8()
+    ipdirect, di, dj, dk = i4(), i4(), i4(), i4()
+    epss, epsz, xdomain, ydomain, delxd, delyd = r8(), r8(), r8(), r8(), r8(), r8()
+    deljdot, deljold, deljnew, jnorm = r4(), r4(), r4(), r4()
+    uniform_sfc_brdf = lg(False)
+    ierr, errmsg = i4(), s("", 600)
+    longest = i4()
+    nang_ref = i4(nang)
+    volsrc = np.zeros((1, 1), np.float32, order="F")
+    volsrcgridptr = np.zeros((2, maxig), np.int32, order="F")
+    volsrcshptr = np.zeros(2, np.int32)
+    sfcparms = np.zeros(1, np.float32)
+    fcall("INIT_SOLUTION",
+          i4(d["nx"]), i4(d["ny"]), i4(d["nz"]), i4(d["nx1"]), i4(d["ny1"]), nang_ref,
+          i4(d["ml"]), i4(d["mm"]), i4(maxiv), i4(maxic), i4(maxig), i4(maxido), r4(sc.shacc),
+          i4(2), i4(d["nlm"]), i4(nmu), i4(d["nphi"]), i4(d["nleg"]), i4(d["numphase"]),
+          ang["nphi0"], ang["mu"], ang["phi"], ang["wtdo"], i4(d["bcflag"]), i4(d["ipflag"]),
+          lg(sc.deltam), s("S"), r4(sc.solarflux), r4(sc.solarmu), r4(sc.solaraz), skyrad,
+          s("FL"), r4(sc.gndtemp), r4(sc.gndalbedo), i4(0), i4(0), d["cellflags"],
+          r4(0.0), r4(0.0), i4(1), sfcparms, sfcgridparms,
+          s("R"), farr([10000, 10001], np.float32), r4(sc.wavelength), lg(sc.accelflag),
+          d["xgrid"], d["ygrid"], d["zgrid"], d["temp"], d["planck"], d["extinct"], d["albedo"],
+          d["legen"], d["iphase"], i4(maxnbc), i4(maxbcrad), ntoppts, nbotpts, bcptr,
+          d["neighptr"], i4(d["npts"]), d["gridpos"], i4(d["ncells"]), d["gridptr"],
+          d["treeptr"], rshptr, shptr, oshptr, source, delsource, radiance, fluxes, dirflux,
+          ylmsun, uniformzlev, i4(d["npx"]), i4(d["npy"]), i4(d["npz"]), r4(sc.delx),
+          r4(sc.dely), r4(0.0), r4(0.0), d["zlevels"], d["tempp"], d["extinctp"],
+          i4(d["nbpts"]), d["albedop"], d["legenp"], extdirp, d["iphasep"], i4(0), d["zckd"],
+          d["gasabs"], i4(d["npart"]), d["total_ext"], bcrad, cx, cy, cz, cxinv, cyinv, czinv,
+          ipdirect, di, dj, dk, i4(nphi0max), epss, epsz, xdomain, ydomain, delxd, delyd,
+          deljdot, deljold, deljnew, jnorm, i4(d["nbcells"]), ang["fftflag"], d["cmu1"],
+          d["cmu2"], ang["wtmu"], d["cphi1"], d["cphi2"], d["wphisave"], i4(maxsfcpars),
+          work, work1, work2, i4(1), i4(1), uniform_sfc_brdf, sfc_brdf_do, lg(True), i4(ndel),
+          ierr, errmsg, i4(maxpg), i4(work2_size), d["phaseinterpwt"], r4(0.999), s("ON"),
+          i4(d["nlegp"]), lg(False), i4(d["maxnmicro"]), d["phasewtp"], i4(sc.angle_set),
+          lg(True), longest, r4(0.0), surface_rad, volsrc, volsrcgridptr, volsrcshptr, i4(1))
+    _check(ierr, errmsg, "INIT_SOLUTION")
+    d.update(skyrad=skyrad, surface_rad=surface_rad, source=source, delsource=delsource,
+             radiance=radiance, dirflux=dirflux, work1=work1, work=work, work2=work2,
+             bcrad=bcrad, fluxes=fluxes, shptr=shptr, rshptr=rshptr, oshptr=oshptr,
+             sfcgridparms=sfcgridparms, bcptr=bcptr, ylmsun=ylmsun, extdirp=extdirp,
+             sfc_brdf_do=sfc_brdf_do, ntoppts=ntoppts.value, nbotpts=nbotpts.value,
+             uniformzlev=uniformzlev.value, cx=cx.value, cy=cy.value, cz=cz.value,
+             cxinv=cxinv.value, cyinv=cyinv.value, czinv=czinv.value, ipdirect=ipdirect.value,
+             di=di.value, dj=dj.value, dk=dk.value, epss=epss.value, epsz=epsz.value,
+             xdomain=xdomain.value, ydomain=ydomain.value, delxd=delxd.value,
+             delyd=delyd.value, deljdot=deljdot.value, deljold=deljold.value,
+             deljnew=deljnew.value, jnorm=jnorm.value, ndel=ndel, work2_size=work2_size,
+             volsrc=volsrc, volsrcgridptr=volsrcgridptr, volsrcshptr=volsrcshptr,
+             maxsfcpars=maxsfcpars, sfcparms=sfcparms, oldnpts=0, solcrit=1.0, iters=0,
+             nang=nang)
+
+
+def solution_iterations(st: RefState, maxiter: int, verbose=False):
+    """RTE.solve part 2 (solver.py:447-631) -> SOLUTION_ITERATIONS (shdomsub1.f:445)."""
+    sc, d = st.sc, st.d
+    nmu, nphi0max = d["nmu"], d["nphi0max"]
+    sfcgridrad = np.zeros((d["nang"] // 2 + 1, d["maxnbc"]), np.float32, order="F")
+    solcrit, it = r4(d["solcrit"]), i4(d["iters"])
+    npts, ncells = i4(d["npts"]), i4(d["ncells"])
+    ntoppts, nbotpts = i4(d["ntoppts"]), i4(d["nbotpts"])
+    oldnpts = i4(d["oldnpts"])
+    uniformzlev = r8(d["uniformzlev"])
+    deljdot, deljold = r4(d["deljdot"]), r4(d["deljold"])
+    deljnew, jnorm = r4(d["deljnew"]), r4(d["jnorm"])
+    ierr, errmsg = i4(), s("", 600)
+    comptime, splitcrit = r4(), r4()
+    fcall("SOLUTION_ITERATIONS",
+          i4(1), i4(d["nx"]), i4(d["ny"]), i4(d["nz"]), i4(d["nx1"]), i4(d["ny1"]),
+          i4(d["nang"]), i4(d["ml"]), i4(d["mm"]), i4(2), i4(d["nlm"]), i4(nmu), i4(d["nphi"]),
+          i4(d["nleg"]), i4(1), i4(d["numphase"]), d["nphi0"], d["mu"], d["phi"], d["wtdo"],
+          i4(d["maxiv"]), i4(d["maxic"]), i4(d["maxig"]), i4(d["maxido"]), i4(d["bcflag"]),
+          i4(d["ipflag"]), lg(sc.deltam), s("S"), lg(sc.high_order_radiance),
+          r4(sc.solarflux), r4(sc.solarmu), r4(sc.solaraz), d["skyrad"], s("FL"),
+          r4(sc.gndtemp), r4(sc.gndalbedo), i4(0), i4(0), r4(0.0), r4(0.0), i4(1),
+          d["sfcparms"], d["sfcgridparms"], s("R"), farr([10000, 10001], np.float32),
+          r4(sc.wavelength), lg(sc.accelflag), r4(sc.solacc), i4(maxiter), solcrit, it,
+          r4(sc.splitacc), r4(sc.shacc), d["xgrid"], d["ygrid"], d["zgrid"], d["temp"],
+          d["planck"], d["extinct"], d["albedo"], d["legen"], d["iphase"], i4(d["maxnbc"]),
+          i4(d["maxbcrad"]), ntoppts, nbotpts, d["bcptr"], d["bcrad"], npts, d["gridpos"],
+          ncells, d["gridptr"], d["neighptr"], d["treeptr"], d["cellflags"], d["rshptr"],
+          d["shptr"], d["oshptr"], d["source"], d["delsource"], d["radiance"], d["fluxes"],
+          d["dirflux"], d["ylmsun"], lg(verbose), s("oracle"), uniformzlev, i4(d["npx"]),
+          i4(d["npy"]), i4(d["npz"]), r4(sc.delx), r4(sc.dely), r4(0.0), r4(0.0),
+          d["zlevels"], d["tempp"], d["extinctp"], i4(d["nbpts"]), d["albedop"], d["legenp"],
+          d["extdirp"], d["iphasep"], i4(0), d["zckd"], d["gasabs"], oldnpts, i4(d["npart"]),
+          d["total_ext"], r8(d["extmin"]), r8(d["scatmin"]), r8(d["cx"]), r8(d["cy"]),
+          r8(d["cz"]), r8(d["cxinv"]), r8(d["cyinv"]), r8(d["czinv"]), i4(d["ipdirect"]),
+          i4(d["di"]), i4(d["dj"]), i4(d["dk"]), i4(nphi0max), r8(d["epss"]), r8(d["epsz"]),
+          r8(d["xdomain"]), r8(d["ydomain"]), r8(d["delxd"]), r8(d["delyd"]), r4(d["albmax"]),
+          deljdot, deljold, deljnew, jnorm, d["fftflag"], d["cmu1"], d["cmu2"], d["wtmu"],
+          d["cphi1"], d["cphi2"], d["wphisave"], d["work"], d["work1"], d["work2"],
+          lg(False), d["sfc_brdf_do"], i4(sc.iterfixsh), s("ON"), ierr, errmsg, i4(d["maxpg"]),
+          d["phaseinterpwt"], r4(0.999), i4(d["nlegp"]), i4(d["maxnmicro"]), d["phasewtp"],
+          lg(True), comptime, lg(True), r4(sc.transmin), splitcrit, d["surface_rad"],
+          sfcgridrad, d["volsrc"], d["volsrcgridptr"], d["volsrcshptr"], i4(1))
+    _check(ierr, errmsg, "SOLUTION_ITERATIONS")
+    d.update(solcrit=solcrit.value, iters=it.value, npts=npts.value, ncells=ncells.value,
+             ntoppts=ntoppts.value, nbotpts=nbotpts.value, oldnpts=oldnpts.value,
+             deljdot=deljdot.value, deljold=deljold.value, deljnew=deljnew.value,
+             jnorm=jnorm.value, comptime=comptime.value, sfcgridrad=sfcgridrad)
+    return st
+
+
+def solve(sc: Scene, maxiter: int = 100, verbose=False) -> RefState:
+    st = setup(sc)
+    init_solution(st)
+    solution_iterations(st, maxiter, verbose)
+    return st
+
+
+def precompute_phase(st: RefState):
+    d = st.d
+    numphase, nlegp = d["numphase"], d["nlegp"]
+    legen = farr(d["legenp"].reshape((1, nlegp + 1, numphase), order="F"), np.float32)
+    phasetab = np.zeros((1, numphase, d["nscatangle"]), np.float32, order="F")
+    ierr, errmsg = i4(), s("", 600)
+    fcall("PRECOMPUTE_PHASE_CHECK", i4(d["nscatangle"]), i4(numphase), i4(1), i4(1),
+          i4(d["ml"]), i4(d["nlm"]), i4(1), i4(nlegp), legen, phasetab, lg(st.sc.deltam),
+          lg(True), errmsg, ierr)
+    _check(ierr, errmsg, "PRECOMPUTE_PHASE_CHECK")
+    d["phasetab"] = phasetab
+    return phasetab
+
+
+def render(st: RefState, x, y, z, mu, phi, single_scatter=False, nosurface=False):
+    """RTE.integrate_to_sensor (solver.py:633-791) -> RENDER (shdomsub4.f:93)."""
+    sc, d = st.sc, st.d
+    precompute_phase(st)
+    npix = len(x)
+    camx, camy, camz = farr(x, np.float32), farr(y, np.float32), farr(z, np.float32)
+    cammu, camphi = farr(mu, np.float64), farr(phi, np.float64)
+    stokes = np.zeros((1, npix), np.float32, order="F")
+    bcrad = d["bcrad"].copy(order="F")
+    ierr, errmsg = i4(), s("", 600)
+    npts = d["npts"]
+    fcall("RENDER", i4(1), i4(d["nx"]), i4(d["ny"]), i4(d["nz"]), i4(npts), i4(d["ncells"]),
+          i4(d["ml"]), i4(d["mm"]), i4(2), i4(d["nlm"]), i4(1), i4(d["nleg"]),
+          i4(d["numphase"]), i4(d["nmu"]), i4(d["nphi0max"]), d["nphi0"], d["mu"], d["phi"],
+          d["wtdo"], i4(d["bcflag"]), i4(d["ipflag"]), s("S"), lg(sc.deltam), r4(sc.solarmu),
+          r4(sc.solaraz), s("FL"), i4(1), d["sfcgridparms"], i4(d["maxnbc"]), i4(d["ntoppts"]),
+          i4(d["nbotpts"]), d["bcptr"], bcrad, r4(sc.gndtemp), r4(sc.gndalbedo), d["skyrad"],
+          farr([10000, 10001], np.float32), r4(sc.wavelength), s("R"), d["xgrid"], d["ygrid"],
+          d["zgrid"], d["gridpos"], d["gridptr"], d["neighptr"], d["treeptr"], d["cellflags"],
+          farr(d["extinct"][:npts], np.float32), farr(d["albedo"][:npts], np.float32),
+          d["legen"], farr(d["iphase"][:, :npts], np.int32), d["dirflux"], d["fluxes"],
+          d["shptr"], d["source"], camx, camy, camz, cammu, camphi, i4(npix), i4(d["npart"]),
+          farr(d["total_ext"][:npts], np.float32), stokes, i4(d["nscatangle"]), d["ylmsun"],
+          d["phasetab"], i4(1), ierr, errmsg, s("ON"),
+          farr(d["phaseinterpwt"][:, :npts], np.float32), r4(0.999), i4(d["maxnmicro"]),
+          r8(sc.tautol), lg(nosurface), lg(True), r8(sc.transcut), lg(single_scatter),
+          d["sfcgridrad"], i4(d["nang"]))
+    _check(ierr, errmsg, "RENDER")
+    return stokes[0].copy()
+
+
+# ----------------------------------------------------------------------------
+# dense views of the ragged reference arrays
+# ----------------------------------------------------------------------------
+def dense_sh(ptr, arr, npts, nlm):
+    """Expand a ragged SH array (pointer SHPTR/RSHPTR) to dense (npts, nlm)."""
+    out = np.zeros((npts, nlm), np.float32)
+    a = arr[0]
+    for i in range(npts):
+        lo, hi = ptr[i], ptr[i + 1]
+        out[i, :hi - lo] = a[lo:hi]
+    return out
+
+
+def scene_from_workload(w, **kw) -> Scene:
+    """Reference scene for an at3d_amd.synthetic.Workload (same arrays)."""
+    sp = [Species(e, a, leg) for (e, a, leg) in w.species]
+    args = dict(delx=w.delx, dely=w.dely, zlevels=w.z, species=sp, bc=w.bc,
+                num_mu_bins=w.nmu, num_phi_bins=w.nphi, solacc=w.solacc,
+                solarmu=w.solarmu, solaraz=float(np.deg2rad(w.solaraz_deg)),
+                gndalbedo=w.gndalbedo, wavelength=w.wavelength, deltam=w.deltam)
+    args.update(kw)
+    return Scene(**args)
