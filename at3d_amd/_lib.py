@@ -66,6 +66,9 @@ def lib():
         L.at3d_solve.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_float),
                                  C.POINTER(C.c_int32), C.c_char_p, C.c_size_t]
         L.at3d_state_get.argtypes = [vp, C.c_char_p, vp, C.c_size_t, C.c_char_p, C.c_size_t]
+        L.at3d_state_set_option.argtypes = [vp, C.c_char_p, C.c_int32, C.c_char_p, C.c_size_t]
+        L.at3d_render.argtypes = [vp, C.c_int64, vp, vp, vp, vp, vp, C.c_int32, C.c_int32, vp,
+                                  C.c_char_p, C.c_size_t]
         L.at3d_state_timing.argtypes = [vp, C.c_char_p]
         L.at3d_state_timing.restype = C.c_double
         _lib = L
@@ -84,7 +87,8 @@ def errbuf():
 # symbols every build must export (checked by the CPU test-suite)
 EXPORTS = ["at3d_version", "at3d_plan_create", "at3d_plan_destroy", "at3d_plan_size",
            "at3d_plan_get", "at3d_state_create", "at3d_state_destroy", "at3d_init_solution",
-           "at3d_solve", "at3d_state_get", "at3d_state_timing"]
+           "at3d_solve", "at3d_state_get", "at3d_state_set_option", "at3d_state_timing",
+           "at3d_render"]
 
 
 def as_f32(a):

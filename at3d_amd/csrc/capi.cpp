@@ -254,13 +254,70 @@ int at3d_state_get(at3d_state* st, const char* name, void* dst, size_t nbytes, c
   if (k == "source") return get2d(s.S, npts, s.nlm, s.ld);
   if (k == "radiance") return get2d(s.R, npts, s.nlm, s.ld);
   if (k == "delsource") return get2d(s.DS, npts, s.nlm, s.ld);
-  if (k == "do_radiance") return get2d(s.dorad, s.nang, npts, s.nptsp);
-  if (k == "do_source") return get2d(s.dosrc, s.nang, npts, s.nptsp);
+  if (k == "do_radiance" || k == "do_source") {
+    // device layout [point][column] -> [reference ordinate][point]
+    const size_t nang = s.nang;
+    if (nbytes < nang * npts * sizeof(float)) {
+      set_err(err, errlen, "destination buffer too small");
+      return AT3D_ERR_ARG;
+    }
+    std::vector<float> tmp((size_t)npts * s.ldo);
+    hipError_t e = hipMemcpy(tmp.data(), k == "do_radiance" ? s.dorad : s.dosrc,
+                             tmp.size() * sizeof(float), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      set_err(err, errlen, hipGetErrorString(e));
+      return AT3D_ERR_DEVICE;
+    }
+    float* out = static_cast<float*>(dst);
+    for (size_t k2 = 0; k2 < nang; k2++) {
+      const size_t c = s.col_of_ref[k2];
+      for (size_t pnt = 0; pnt < npts; pnt++) out[k2 * npts + pnt] = tmp[pnt * s.ldo + c];
+    }
+    return AT3D_OK;
+  }
   if (k == "fluxes") return get1d(s.fluxes, 2 * npts * sizeof(float));
   if (k == "dirflux") return get1d(s.dirflux, npts * sizeof(float));
   if (k == "ns") return get1d(s.ns, npts * sizeof(int));
   if (k == "nr") return get1d(s.nr, npts * sizeof(int));
   set_err(err, errlen, "unknown state array '" + k + "'");
+  return AT3D_ERR_ARG;
+}
+
+int at3d_render(at3d_state* st, int64_t nrays, const float* x, const float* y, const float* z,
+                const double* mu, const double* phi, int32_t single_scatter, int32_t nosurface,
+                float* out, char* err, size_t errlen) {
+  if (!st || (nrays > 0 && (!x || !y || !z || !mu || !phi || !out))) {
+    set_err(err, errlen, "null argument");
+    return AT3D_ERR_ARG;
+  }
+  std::string msg;
+  int rc;
+  try {
+    rc = at3d::state_render(st->s, nrays, x, y, z, mu, phi, single_scatter, nosurface, out, msg);
+  } catch (const std::bad_alloc&) {
+    rc = AT3D_ERR_MEMORY;
+    msg = "out of host memory in render";
+  }
+  if (rc) set_err(err, errlen, msg);
+  return rc;
+}
+
+int at3d_state_set_option(at3d_state* st, const char* key, int32_t value, char* err,
+                          size_t errlen) {
+  if (!st || !key) {
+    set_err(err, errlen, "null argument");
+    return AT3D_ERR_ARG;
+  }
+  const std::string k(key);
+  if (k == "sweep_general") {
+    st->s.force_general = value ? 1 : 0;
+    return AT3D_OK;
+  }
+  if (k == "sweep_graph") {
+    st->s.use_graph = value ? 1 : 0;
+    return AT3D_OK;
+  }
+  set_err(err, errlen, "unknown option '" + k + "'");
   return AT3D_ERR_ARG;
 }
 
@@ -273,8 +330,12 @@ double at3d_state_timing(const at3d_state* st, const char* key) {
   if (k == "sweep") return s.tsum[2];
   if (k == "do_to_sh") return s.tsum[3];
   if (k == "source") return s.tsum[4];
-  if (k == "total") return s.tsum[0] + s.tsum[1] + s.tsum[2] + s.tsum[3] + s.tsum[4];
+  if (k == "sweep_levels")
+    return (double)(s.levels_down.size() + s.levels_up.size());
+  if (k == "total") return s.tsum[0] + s.tsum[1] + s.tsum[2] + s.tsum[3] + s.tsum[4] + s.tsum[5];
+  if (k == "sweep_launches") return s.nsweep_launch;
   if (k == "iters") return s.iters_last;
+  if (k == "render") return s.render_ms;
   return -1.0;
 }
 

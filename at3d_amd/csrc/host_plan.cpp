@@ -1103,6 +1103,7 @@ int build_levels(Plan& p, std::string& err) {
     for (const Ordinate& od : p.ords)
       if (od.g.ioct == o) ords.push_back(&od);
     op.level.assign(p.npts, 0);
+    op.simple_pt.assign(p.npts, 0);
     op.nslow = 0;
     const int bitx = o & 1, bity = (o >> 1) & 1, bitz = (o >> 2) & 1;
     for (int r = 0; r < p.npts; r++) {
@@ -1111,6 +1112,7 @@ int build_levels(Plan& p, std::string& err) {
       if (preset[pt]) continue;
       auto valid = [&](int q) { return preset[q] || op.rank[q] < r; };
       int lvl = -1;
+      bool simple = false;
       if (corner == o) {
         // one-cell characteristic for every ordinate of the octant if all far faces are known
         const int8_t fl = p.cellflags[cell];
@@ -1126,7 +1128,10 @@ int build_levels(Plan& p, std::string& err) {
             if (!valid(q)) { ok = false; break; }
             mx = std::max(mx, op.level[q]);
           }
-        if (ok) lvl = mx + 1;
+        if (ok) {
+          lvl = mx + 1;
+          simple = (fl & 3) == 0;  // regular cell: the GPU may use analytic geometry
+        }
       }
       if (lvl < 0) {
         op.nslow++;
@@ -1161,6 +1166,7 @@ int build_levels(Plan& p, std::string& err) {
         lvl = mx + 1;
       }
       op.level[pt] = lvl;
+      if (simple) op.simple_pt[pt] = 1;
     }
     // level lists (level >= 1), ordered by (level, rank)
     int nl = 0;
@@ -1183,14 +1189,96 @@ int build_levels(Plan& p, std::string& err) {
       const int L = op.level[pt];
       if (L <= 0) continue;
       const int pos = fill[L - 1]++;
-      op.lvl_pts[pos] = pt;
+      // one-cell characteristics in regular cells are stored as the point index,
+      // every other point as ~point (traced by the general path)
+      op.lvl_pts[pos] = op.simple_pt[pt] ? pt : ~pt;
       op.lvl_cell[pos] = cell;
     }
   }
   return AT3D_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Rendering constants.  PRECOMPUTE_PHASE_CHECK (shdomsub4.f:2388-2492): the
+// untruncated phase function of every table entry on NSCATANGLE equispaced
+// scattering angles, with the reference's negativity check.  Per point, the
+// TMS terms of COMPUTE_SOURCE_1CELL_UNPOL (shdomsub2.f:3096-3175): the
+// truncated single-scatter weights sum_species DA * LEGENT_l / (1-f) and the
+// (phase table, DA * w / (1-f)) pairs of the exact single-scatter term.
+// ---------------------------------------------------------------------------
+int render_constants(Plan& p, std::string& err) {
+  const int nsa = p.nscatangle, nlegp = p.nlegp, np = p.numphase;
+  p.phasetab.assign((size_t)np * nsa, 0.f);
+  const double PI = acos(-1.0);
+  const double ofourpi = 1.0 / (4.0 * PI);
+  std::vector<double> pl(nlegp + 1);
+  for (int j = 0; j < nsa; j++) {
+    const double x = cos(PI * (double)j / (double)(nsa - 1));
+    // WIGNERFCT(X, NLEG, 0, 0): Legendre polynomials by recurrence
+    pl[0] = 1.0;
+    if (nlegp >= 1) pl[1] = x;
+    for (int n = 1; n <= nlegp - 1; n++) {
+      const double f1 = (double)(2 * n + 1) * x / (double)(n + 1);
+      const double f2 = (double)n / (double)(n + 1);
+      pl[n + 1] = f1 * pl[n] - f2 * pl[n - 1];
+    }
+    for (int iph = 0; iph < np; iph++) {
+      double a1 = 0.0;
+      for (int l = 0; l <= nlegp; l++) {
+        const float unsc = p.legenp[(size_t)iph * (nlegp + 1) + l] / (float)(2 * l + 1);
+        a1 += (2.0 * l + 1.0) * (double)unsc * pl[l];
+      }
+      if (a1 <= 0.0 && p.phasetab_err.empty()) {
+        // reported when rendering, where the reference raises it
+        char buf[200];
+        snprintf(buf, sizeof buf,
+                 "PRECOMPUTE_PHASE_CHECK: negative phase function for tabulated phase function: "
+                 "IPH %d J %d A1 %.10g", iph + 1, j + 1, a1);
+        p.phasetab_err = buf;
+      }
+      p.phasetab[(size_t)iph * nsa + j] = (float)(a1 * ofourpi);
+    }
+  }
+  const int ML = p.ml, npts = p.npts, npart = p.npart;
+  const float secmu0 = (float)(1.0 / fabs((double)p.d.solarmu));
+  p.tms.assign((size_t)npts * (ML + 1), 0.f);
+  p.ss_off.assign(npts + 1, 0);
+  p.ss_iph.clear();
+  p.ss_coef.clear();
+  std::vector<float> legt;
+  for (int ip = 0; ip < npts; ip++) {
+    p.ss_off[ip] = (int)p.ss_iph.size();
+    if (!p.d.deltam) continue;
+    const float ext = p.total_ext[ip];
+    for (int ipa = 0; ipa < npart; ipa++) {
+      const float W = (ext == 0.0f) ? 1.0f : p.extinct[(size_t)ipa * npts + ip] / ext;
+      if (W == 0.0f) continue;
+      mixed_legen(p, ipa, ip, legt);
+      const float f = legt[ML + 1];
+      for (float& v : legt) v = v / (1 - f);
+      const float da = p.albedo[(size_t)ipa * npts + ip] * p.dirflux[ip] * secmu0 * W;
+      for (int l = 0; l <= ML; l++) p.tms[(size_t)ip * (ML + 1) + l] += da * legt[l];
+      const size_t base = ((size_t)ipa * npts + ip) * 8;
+      if (p.phaseinterpwt[base] >= 0.999f) {
+        p.ss_iph.push_back(p.iphase[base] - 1);
+        p.ss_coef.push_back(da / (1 - f));
+      } else {
+        for (int q = 0; q < 8; q++) {
+          const float w = p.phaseinterpwt[base + q];
+          if (w <= 1e-5f) continue;
+          p.ss_iph.push_back(p.iphase[base + q] - 1);
+          p.ss_coef.push_back(da * w / (1 - f));
+        }
+      }
+    }
+  }
+  p.ss_off[npts] = (int)p.ss_iph.size();
+  return AT3D_OK;
+}
+
 }  // namespace
+
+void ylm_direction(float mu, float phi, int ml, int mm, float* yr) { ylm_unpol(mu, phi, ml, mm, yr); }
 
 int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err) {
   p.d = desc;
@@ -1293,6 +1381,8 @@ int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err) {
   }
   make_ordinates(p);
   rc = build_levels(p, err);
+  if (rc) return rc;
+  rc = render_constants(p, err);
   if (rc) return rc;
   return AT3D_OK;
 }

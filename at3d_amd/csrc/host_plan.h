@@ -22,7 +22,9 @@ struct OctantPlan {
   std::vector<int32_t> sweepord;   // [npts] (cell<<3)|corner in sweep order
   std::vector<int32_t> rank;       // [npts] position of each point in sweepord
   std::vector<int32_t> level;      // [npts] dependency level (0 = boundary preset)
-  std::vector<int32_t> lvl_pts;    // non-preset points sorted by (level, rank)
+  std::vector<int8_t> simple_pt;   // [npts] one-cell characteristic in a regular cell
+  std::vector<int32_t> lvl_pts;    // non-preset points sorted by (level, rank);
+                                   // ~point for points that need the general trace
   std::vector<int32_t> lvl_cell;   // their start cell
   std::vector<int32_t> lvl_off;    // [nlevels+1] offsets into lvl_pts (level 1..)
   int nlevels = 0;
@@ -55,6 +57,12 @@ struct Plan {
   std::vector<float> chi, alb, tc;                // [npts][ml+1], [npts], [npts][ml+1]
   float albmax = 0.f;
   double extmin = 0, scatmin = 0;
+  // rendering constants
+  std::vector<float> phasetab;                    // [numphase][nscatangle]
+  std::string phasetab_err;                       // negativity failure, raised by render
+  std::vector<float> tms;                         // [npts][ml+1] sum_species DA * chi'_l
+  std::vector<int32_t> ss_off, ss_iph;            // CSR per point of (phase index, coefficient)
+  std::vector<float> ss_coef;
   // solution setup
   std::vector<float> dirflux, radiance0;          // [npts], [npts][4]
   std::vector<int32_t> bctop, bcbot;
@@ -64,5 +72,9 @@ struct Plan {
 
 // Builds the plan; returns AT3D_OK or an error status with a message.
 int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err);
+
+// Real generalised spherical harmonics of one direction (YLMALL_UNPOL,
+// shdomsub2.f:4490-4539), J-ordered, nlm values.
+void ylm_direction(float mu, float phi, int ml, int mm, float* yr);
 
 }  // namespace at3d

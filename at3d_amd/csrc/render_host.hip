@@ -1,0 +1,200 @@
+// Host orchestration of rendering (RTE.integrate_to_sensor -> RENDER,
+// shdomsub4.f:93-286): per distinct ray direction the spherical harmonics and
+// the single-scatter phase values, then the direction source for every grid
+// point on the GPU and one ray per lane.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "host_plan.h"
+#include "kernels.h"
+#include "solver_state.h"
+
+namespace at3d {
+
+#define HIPCHK(x)                                                              \
+  do {                                                                         \
+    hipError_t e_ = (x);                                                       \
+    if (e_ != hipSuccess) {                                                    \
+      err = std::string(#x) + ": " + hipGetErrorString(e_);                    \
+      return AT3D_ERR_DEVICE;                                                  \
+    }                                                                          \
+  } while (0)
+
+static size_t rup(size_t v, size_t m) { return (v + m - 1) / m * m; }
+
+int state_render(State& s, int64_t nrays, const float* x, const float* y, const float* z,
+                 const double* mu, const double* phi, int single_scatter, int nosurface,
+                 float* out, std::string& err) {
+  const Plan& p = *s.plan;
+  if (nrays <= 0) return AT3D_OK;
+  if (nrays > (int64_t)1 << 30) {
+    err = "too many rays in one call";
+    return AT3D_ERR_ARG;
+  }
+  if (!p.phasetab_err.empty()) {
+    err = p.phasetab_err;
+    return AT3D_ERR_NUMERIC;
+  }
+  const float ztop = p.zgrid[p.nz - 1], zbot = p.zgrid[0];
+  for (int64_t i = 0; i < nrays; i++) {
+    if (z[i] < zbot) {
+      err = "RENDER: Level below domain";
+      return AT3D_ERR_ARG;
+    }
+    if (!(mu[i] >= -1.0 && mu[i] <= 1.0) || mu[i] == 0.0) {
+      err = "ray mu must be in [-1, 1] and nonzero";
+      return AT3D_ERR_ARG;
+    }
+  }
+  // distinct directions
+  std::map<std::pair<double, double>, int> dmap;
+  std::vector<int32_t> dir_of_ray(nrays);
+  std::vector<std::pair<double, double>> dirs;
+  for (int64_t i = 0; i < nrays; i++) {
+    auto key = std::make_pair(mu[i], phi[i]);
+    auto it = dmap.find(key);
+    if (it == dmap.end()) {
+      it = dmap.emplace(key, (int)dirs.size()).first;
+      dirs.push_back(key);
+    }
+    dir_of_ray[i] = it->second;
+  }
+  const int ndir = (int)dirs.size();
+  const int ldd = (int)rup(ndir, 64);
+  if ((int64_t)s.nptsp * ldd >= ((int64_t)1 << 31)) {
+    err = "too many distinct ray directions for the direction-source table";
+    return AT3D_ERR_ARG;
+  }
+  if ((size_t)s.nptsp * ldd * sizeof(float) > ((size_t)48 << 30)) {
+    err = "too many distinct ray directions for the direction-source table";
+    return AT3D_ERR_ARG;
+  }
+  const int ML = p.ml, nlm = p.nlm, np = p.numphase, nsa = p.nscatangle;
+  const bool tms_on = p.d.deltam != 0;
+  std::vector<float> Yd((size_t)ldd * s.ld, 0.f), G((size_t)ndir * (ML + 1), 0.f),
+      ssd((size_t)ndir * np, 0.f), ylm(nlm);
+  const double PI = acos(-1.0);
+  const float solarmu = p.d.solarmu, solaraz = p.d.solaraz;
+  for (int d = 0; d < ndir; d++) {
+    const double mu2 = dirs[d].first, phi2 = dirs[d].second;
+    ylm_direction((float)mu2, (float)phi2, ML, p.mm, ylm.data());
+    for (int j = 0; j < nlm; j++) Yd[(size_t)d * s.ld + j] = ylm[j];
+    std::vector<double> g(ML + 1, 0.0);
+    for (int j = 0; j < nlm; j++) g[p.lofj[j]] += (double)ylm[j] * (double)p.ylmsun[j];
+    for (int l = 0; l <= ML; l++) G[(size_t)d * (ML + 1) + l] = (float)g[l];
+    if (tms_on) {
+      double cosscat = (double)solarmu * mu2 +
+                       sqrt((double)(1.0f - solarmu * solarmu) * (1.0 - mu2 * mu2)) *
+                           cos((double)solaraz - phi2);
+      cosscat = std::max(std::min(1.0, cosscat), -1.0);
+      float f = (float)((nsa - 1) * (acos(cosscat) / PI) + 1);
+      const int j = std::min(nsa - 1, (int)f);
+      f = f - (float)j;
+      for (int iph = 0; iph < np; iph++)
+        ssd[(size_t)d * np + iph] =
+            (1 - f) * p.phasetab[(size_t)iph * nsa + j - 1] + f * p.phasetab[(size_t)iph * nsa + j];
+    }
+  }
+  std::vector<float> topval(nrays);
+  for (int64_t i = 0; i < nrays; i++) topval[i] = (mu[i] < 0.0) ? p.d.skyrad : 0.0f;
+  // device buffers in one block
+  const size_t nr = (size_t)nrays;
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    const size_t o = off;
+    off = rup(off + bytes, 256);
+    return o;
+  };
+  const size_t o_yd = carve(Yd.size() * 4), o_g = carve(G.size() * 4), o_ss = carve(ssd.size() * 4 + 4);
+  const size_t o_raw = carve((size_t)s.nptsp * ldd * 4);
+  const size_t o_x = carve(nr * 4), o_y = carve(nr * 4), o_z = carve(nr * 4);
+  const size_t o_mu = carve(nr * 8), o_phi = carve(nr * 8), o_dir = carve(nr * 4);
+  const size_t o_top = carve(nr * 4), o_out = carve(nr * 4);
+  if (off > s.rbuf_bytes) {
+    if (s.rbuf) (void)hipFree(s.rbuf);
+    s.rbuf = nullptr;
+    s.rbuf_bytes = 0;
+    HIPCHK(hipMalloc(&s.rbuf, off));
+    s.rbuf_bytes = off;
+  }
+  char* B = static_cast<char*>(s.rbuf);
+  HIPCHK(hipMemcpyAsync(B + o_yd, Yd.data(), Yd.size() * 4, hipMemcpyHostToDevice, s.stream));
+  HIPCHK(hipMemcpyAsync(B + o_g, G.data(), G.size() * 4, hipMemcpyHostToDevice, s.stream));
+  if (!ssd.empty())
+    HIPCHK(hipMemcpyAsync(B + o_ss, ssd.data(), ssd.size() * 4, hipMemcpyHostToDevice, s.stream));
+  HIPCHK(hipMemcpyAsync(B + o_x, x, nr * 4, hipMemcpyHostToDevice, s.stream));
+  HIPCHK(hipMemcpyAsync(B + o_y, y, nr * 4, hipMemcpyHostToDevice, s.stream));
+  HIPCHK(hipMemcpyAsync(B + o_z, z, nr * 4, hipMemcpyHostToDevice, s.stream));
+  HIPCHK(hipMemcpyAsync(B + o_mu, mu, nr * 8, hipMemcpyHostToDevice, s.stream));
+  HIPCHK(hipMemcpyAsync(B + o_phi, phi, nr * 8, hipMemcpyHostToDevice, s.stream));
+  HIPCHK(hipMemcpyAsync(B + o_dir, dir_of_ray.data(), nr * 4, hipMemcpyHostToDevice, s.stream));
+  HIPCHK(hipMemcpyAsync(B + o_top, topval.data(), nr * 4, hipMemcpyHostToDevice, s.stream));
+  HIPCHK(hipMemsetAsync(s.errflag, 0, sizeof(int), s.stream));
+  (void)hipEventRecord(s.ev[6], s.stream);
+  float* raw = reinterpret_cast<float*>(B + o_raw);
+  launch_sh_dot(s.stream, s.S, reinterpret_cast<float*>(B + o_yd), raw, s.nptsp, ldd, s.ld, ldd,
+                p.npts, ndir);
+  launch_dir_source_finish(s.stream, raw, ldd, ndir, s.ext, s.ns, s.lofj, s.tms,
+                           reinterpret_cast<float*>(B + o_g), s.ss_off, s.ss_iph, s.ss_coef,
+                           reinterpret_cast<float*>(B + o_ss), np, ML, p.npts, single_scatter,
+                           tms_on ? 1 : 0, raw);
+  const float albpi = p.d.gndalbedo / acosf(-1.0f);
+  launch_render_bottom(s.stream, s.fluxes, s.dirflux, s.bcbot, (int)p.bcbot.size(), albpi,
+                       s.bcrad_render);
+  RayArgs a;
+  a.g = GridTopo{s.gridpos, s.gridptr, s.neighptr, s.cellflags, p.npts, p.ncells};
+  a.xgrid = s.xgrid;
+  a.ygrid = s.ygrid;
+  a.zgrid = s.zgrid;
+  a.nx = p.nx;
+  a.ny = p.ny;
+  a.nz = p.nz;
+  a.nx1 = p.nx1;
+  a.ny1 = p.ny1;
+  a.nyc = p.nyc;
+  a.open_x = p.bcflag & 1;
+  a.open_y = (p.bcflag >> 1) & 1;
+  a.ext = s.ext;
+  a.dsrc = raw;
+  a.ldd = ldd;
+  a.bot_index = s.bot_index;
+  a.bcrad_bot = s.bcrad_render;
+  a.camx = reinterpret_cast<float*>(B + o_x);
+  a.camy = reinterpret_cast<float*>(B + o_y);
+  a.camz = reinterpret_cast<float*>(B + o_z);
+  a.cammu = reinterpret_cast<double*>(B + o_mu);
+  a.camphi = reinterpret_cast<double*>(B + o_phi);
+  a.dir_of_ray = reinterpret_cast<int32_t*>(B + o_dir);
+  a.topval = reinterpret_cast<float*>(B + o_top);
+  a.nrays = (int)nrays;
+  a.tautol = p.d.tautol;
+  a.transcut = p.d.transcut;
+  a.nosurface = nosurface;
+  a.out = reinterpret_cast<float*>(B + o_out);
+  a.errflag = s.errflag;
+  launch_render(s.stream, a);
+  (void)hipEventRecord(s.ev[7], s.stream);
+  HIPCHK(hipGetLastError());
+  int ef = 0;
+  HIPCHK(hipMemcpyAsync(out, B + o_out, nr * 4, hipMemcpyDeviceToHost, s.stream));
+  HIPCHK(hipMemcpyAsync(&ef, s.errflag, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+  HIPCHK(hipStreamSynchronize(s.stream));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, s.ev[6], s.ev[7]);
+  s.render_ms = ms;
+  if (ef) {
+    err = (ef & 1)   ? "INTEGRATE_1RAY: SO<0"
+          : (ef & 2) ? "FIND_BOUNDARY_RADIANCE: Not at boundary"
+                     : "LOCATE_GRID_CELL: ray start outside the grid";
+    return AT3D_ERR_NUMERIC;
+  }
+  return AT3D_OK;
+}
+
+}  // namespace at3d

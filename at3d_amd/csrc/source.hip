@@ -1,0 +1,179 @@
+// Source function, spherical-harmonic truncation and sequence acceleration.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "kernels.h"
+
+namespace at3d {
+
+__device__ inline int nterms_for_l(int l, int mm) {
+  return (l <= mm) ? l * (l + 1) + l + 1 : (2 * mm + 1) * l - mm * mm + mm + 1;
+}
+
+// RADIANCE_TRUNCATION (shdomsub1.f:1615-1803): per-point radiance SH count.
+__global__ void k_radiance_truncation(const float* __restrict__ R, int ld, const float* __restrict__ tc,
+                                      const int* __restrict__ ns, const int* __restrict__ lofj,
+                                      int* __restrict__ nr, int npts, int ml, int mm, int nlm,
+                                      float radmin, int fixsh, int highorder) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npts) return;
+  if (fixsh) {
+    int n = max(4, ns[p]);
+    if (highorder) n = nterms_for_l(ml, mm);
+    nr[p] = n;
+    return;
+  }
+  const float i0 = R[(size_t)p * ld];
+  int lr = 1;
+  for (int l = 1; l <= ml; l++)
+    if (tc[(size_t)p * (ml + 1) + l] * i0 > radmin) lr = l;
+  const int n0 = max(1, ns[p]);
+  const int ls = lofj[n0 - 1];
+  lr = min(lr, ls + ml / 8 + 2);
+  if (highorder) lr = ml;
+  nr[p] = nterms_for_l(lr, mm);
+}
+
+// COMPUTE_SOURCE for NSTOKES=1 (shdomsub1.f:967-1611).  One wavefront per grid
+// point (four points per wave in turn); SOURCET_J = w chi_l (F0 Ysun_J + I_J),
+// the acceleration dot products accumulate in FP64 and are written as one
+// partial per block for a fixed-order reduction.
+__global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
+  __shared__ double red[4][4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double dot = 0, dold = 0, dnew = 0, jn = 0;
+  const int nmax = (a.nlm + 63) / 64;
+  for (int sub = 0; sub < SRC_PTS_PER_BLOCK / 4; sub++) {
+    const int p = blockIdx.x * SRC_PTS_PER_BLOCK + sub * 4 + w;
+    if (p >= a.npts) break;
+    const float alb = a.alb[p], f0 = a.f0[p];
+    const int nrp = a.nr[p], ns_old = a.ns[p], nos_old = a.nos[p];
+    const float* Rp = a.R + (size_t)p * a.ld;
+    float* Sp = a.S + (size_t)p * a.ld;
+    float* DSp = a.DS + (size_t)p * a.ld;
+    const float* chip = a.chi + (size_t)p * (a.ml + 1);
+    int js = -1;
+    float st[8];
+    for (int it = 0; it < nmax; it++) {
+      const int J = lane + 64 * it;
+      float v = 0.f;
+      if (J < a.nlm) {
+        const float c = chip[a.lofj[J]];
+        v = f0 * alb * c * a.ysun[J];
+        if (J < nrp) v = v + alb * c * Rp[J];
+        if (fabsf(v) > a.shacc) js = J;
+      }
+      st[it] = v;
+    }
+    if (!a.first) {
+      const int nsd = a.accel ? min(ns_old, nos_old) : ns_old;
+      for (int it = 0; it < nmax; it++) {
+        const int J = lane + 64 * it;
+        if (J < nsd) {
+          const double so = Sp[J];
+          const double dn = (double)(st[it] - Sp[J]);
+          if (a.accel) {
+            const double d = DSp[J];
+            dot += dn * d;
+            dold += d * d;
+          }
+          dnew += dn * dn;
+          jn += so * so;
+        }
+      }
+      if (a.accel) {
+        for (int it = 0; it < nmax; it++) {
+          const int J = lane + 64 * it;
+          if (J < a.ld) DSp[J] = (J < ns_old) ? st[it] - Sp[J] : 0.f;
+        }
+        if (lane == 0) a.nos[p] = ns_old;
+      }
+    }
+    int nsn;
+    if (a.fixsh) {
+      nsn = ns_old;
+    } else {
+      for (int off = 32; off > 0; off >>= 1) js = max(js, __shfl_xor(js, off));
+      nsn = (js < 0) ? 0 : nterms_for_l(a.lofj[js], a.mm);
+    }
+    for (int it = 0; it < nmax; it++) {
+      const int J = lane + 64 * it;
+      if (J < a.ld) Sp[J] = (J < nsn && J < a.nlm) ? st[it] : 0.f;
+    }
+    if (lane == 0) a.ns[p] = nsn;
+    if (a.first && a.accel) {
+      for (int it = 0; it < nmax; it++) {
+        const int J = lane + 64 * it;
+        if (J < a.ld) DSp[J] = 0.f;
+      }
+      if (lane == 0) a.nos[p] = nsn;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    dot += __shfl_xor(dot, off);
+    dold += __shfl_xor(dold, off);
+    dnew += __shfl_xor(dnew, off);
+    jn += __shfl_xor(jn, off);
+  }
+  if (lane == 0) {
+    red[w][0] = dot;
+    red[w][1] = dold;
+    red[w][2] = dnew;
+    red[w][3] = jn;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int c = threadIdx.x;
+    a.partial[(size_t)blockIdx.x * 4 + c] = red[0][c] + red[1][c] + red[2][c] + red[3][c];
+  }
+}
+
+// Fixed-order reduction of the block partials.
+__global__ __launch_bounds__(1024) void k_reduce_dots(const double* __restrict__ partial, int nblocks,
+                                                      double* __restrict__ out) {
+  __shared__ double sh[1024];
+  const int c = threadIdx.x & 3, t = threadIdx.x >> 2;  // 256 threads per component
+  double s = 0;
+  for (int b = t; b < nblocks; b += 256) s += partial[(size_t)b * 4 + c];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int stride = 512; stride >= 4; stride >>= 1) {
+    if ((int)threadIdx.x < stride) sh[threadIdx.x] += sh[threadIdx.x + stride];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[threadIdx.x] = sh[threadIdx.x];
+}
+
+// ACCELERATE_SOLUTION (shdomsub1.f:1807-1831).
+__global__ void k_accelerate(float* __restrict__ S, const float* __restrict__ DS,
+                             const int* __restrict__ ns, const int* __restrict__ nos, int ld,
+                             int npts, float a) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t p = idx / ld;
+  const int J = (int)(idx % ld);
+  if (p >= (size_t)npts) return;
+  if (J < min(ns[p], nos[p])) S[idx] = S[idx] + a * DS[idx];
+}
+
+void launch_radiance_truncation(hipStream_t st, const float* R, int ld, const float* tc,
+                                const int* ns, const int* lofj, int* nr, int npts, int ml, int mm,
+                                int nlm, float radmin, int fixsh, int highorder) {
+  k_radiance_truncation<<<(npts + 255) / 256, 256, 0, st>>>(R, ld, tc, ns, lofj, nr, npts, ml, mm,
+                                                             nlm, radmin, fixsh, highorder);
+}
+
+void launch_compute_source(hipStream_t st, const SourceArgs& a, int nblocks) {
+  k_compute_source<<<nblocks, 256, 0, st>>>(a);
+}
+
+void launch_reduce_dots(hipStream_t st, const double* partial, int nblocks, double* out) {
+  k_reduce_dots<<<1, 1024, 0, st>>>(partial, nblocks, out);
+}
+
+void launch_accelerate(hipStream_t st, float* S, const float* DS, const int* ns, const int* nos,
+                       int ld, int npts, float a) {
+  const size_t n = (size_t)npts * ld;
+  k_accelerate<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(S, DS, ns, nos, ld, npts, a);
+}
+
+}  // namespace at3d

@@ -1,0 +1,146 @@
+// Spherical harmonic <-> discrete ordinate transforms as FP32 MFMA GEMMs.
+//
+// SH_TO_DO_UNPOL (shdomsub1.f:2928-3036) and DO_TO_SH_UNPOL (:3164-3254)
+// factor the transform into a Legendre sum per m and an azimuthal FFT/DFT per
+// zenith angle.  Here both are one dense matrix per direction (built on the
+// host with the reference's m cut-off at NPHI0/2-1 per zenith angle), so each
+// transform is a single [points x nlm] x [nlm x ordinates] product on the
+// matrix cores.  The >=0 clamp of the DO source (PATH_INTEGRATION :2033-2037)
+// and its product with the extinction are fused into the SH->DO epilogue; the
+// per-point radiance truncation mask is fused into the DO->SH epilogue.
+//
+// Tiling: 64x64 output tile per 256-thread workgroup, K step 16, four waves in
+// a 2x2 layout, each wave 2x2 blocks of v_mfma_f32_16x16x4_f32, whose operand
+// lane layout is A[i][k] in lane i+16k, B[k][j] in lane j+16k, and
+// D[4*(lane/16)+v][lane%16] in accumulator element v.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace at3d {
+
+typedef float float4_ __attribute__((ext_vector_type(4)));
+constexpr int TM = 64, TN = 64, TK = 16, PAD = 4;
+
+__device__ inline void mfma_tile(const float (*sA)[TM + PAD], const float (*sB)[TN + PAD], int wm,
+                                 int wn, int lane, float4_ acc[2][2]) {
+#pragma unroll
+  for (int kk = 0; kk < TK; kk += 4) {
+    const int kr = kk + (lane >> 4);
+    float av[2], bv[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) av[i] = sA[kr][wm * 32 + i * 16 + (lane & 15)];
+#pragma unroll
+    for (int j = 0; j < 2; j++) bv[j] = sB[kr][wn * 32 + j * 16 + (lane & 15)];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bv[j], acc[i][j], 0, 0, 0);
+  }
+}
+
+// C[m][n] = sum_k A[m][k] B[n][k]: both operands K-contiguous.  CLAMP_EXT
+// selects the SH->DO epilogue max(0, C) * ext[m]; without it C is stored as is.
+template <bool CLAMP_EXT>
+__global__ __launch_bounds__(256) void k_sh_to_do(const float* __restrict__ S,
+                                                  const float* __restrict__ Yc,
+                                                  const float* __restrict__ ext,
+                                                  float* __restrict__ out, int ld, int ldo,
+                                                  int npts, int nang) {
+  __shared__ float sA[TK][TM + PAD];
+  __shared__ float sB[TK][TN + PAD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  float4_ acc[2][2];
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) acc[i][j] = (float4_){0.f, 0.f, 0.f, 0.f};
+  const int lr = tid >> 2, lc = (tid & 3) * 4;
+  for (int k0 = 0; k0 < ld; k0 += TK) {
+    const float4 a = *reinterpret_cast<const float4*>(S + (size_t)(m0 + lr) * ld + k0 + lc);
+    const float4 b = *reinterpret_cast<const float4*>(Yc + (size_t)(n0 + lr) * ld + k0 + lc);
+    __syncthreads();
+    sA[lc + 0][lr] = a.x;
+    sA[lc + 1][lr] = a.y;
+    sA[lc + 2][lr] = a.z;
+    sA[lc + 3][lr] = a.w;
+    sB[lc + 0][lr] = b.x;
+    sB[lc + 1][lr] = b.y;
+    sB[lc + 2][lr] = b.z;
+    sB[lc + 3][lr] = b.w;
+    __syncthreads();
+    mfma_tile(sA, sB, wm, wn, lane, acc);
+  }
+  for (int i = 0; i < 2; i++)
+    for (int v = 0; v < 4; v++) {
+      const int p = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + v;
+      if (p >= npts) continue;
+      const float e = CLAMP_EXT ? ext[p] : 1.0f;
+      for (int j = 0; j < 2; j++) {
+        const int c = n0 + wn * 32 + j * 16 + (lane & 15);
+        if (c < nang)
+          out[(size_t)p * ldo + c] = CLAMP_EXT ? fmaxf(acc[i][j][v], 0.0f) * e : acc[i][j][v];
+      }
+    }
+}
+
+// C[m][n] = sum_k A[m][k] B[k][n]: A K-contiguous rows, B N-contiguous rows.
+__global__ __launch_bounds__(256) void k_do_to_sh(const float* __restrict__ D,
+                                                  const float* __restrict__ Wc,
+                                                  const int* __restrict__ nr,
+                                                  float* __restrict__ R, int ld, int ldo,
+                                                  int npts) {
+  __shared__ float sA[TK][TM + PAD];
+  __shared__ float sB[TK][TN + PAD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
+  float4_ acc[2][2];
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) acc[i][j] = (float4_){0.f, 0.f, 0.f, 0.f};
+  const int lr = tid >> 2, lc = (tid & 3) * 4;     // A: 64 rows x 16 k
+  const int bk = tid >> 4, bc = (tid & 15) * 4;    // B: 16 k x 64 cols
+  for (int k0 = 0; k0 < ldo; k0 += TK) {
+    const float4 a = *reinterpret_cast<const float4*>(D + (size_t)(m0 + lr) * ldo + k0 + lc);
+    const float4 b = *reinterpret_cast<const float4*>(Wc + (size_t)(k0 + bk) * ld + n0 + bc);
+    __syncthreads();
+    sA[lc + 0][lr] = a.x;
+    sA[lc + 1][lr] = a.y;
+    sA[lc + 2][lr] = a.z;
+    sA[lc + 3][lr] = a.w;
+    *reinterpret_cast<float4*>(&sB[bk][bc]) = b;
+    __syncthreads();
+    mfma_tile(sA, sB, wm, wn, lane, acc);
+  }
+  for (int i = 0; i < 2; i++)
+    for (int v = 0; v < 4; v++) {
+      const int p = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + v;
+      if (p >= npts) continue;
+      const int n = nr[p];
+      for (int j = 0; j < 2; j++) {
+        const int J = n0 + wn * 32 + j * 16 + (lane & 15);
+        R[(size_t)p * ld + J] = (J < n) ? acc[i][j][v] : 0.0f;
+      }
+    }
+}
+
+void launch_sh_to_do(hipStream_t st, const float* S, const float* Yc, const float* ext,
+                     float* dosrc, int mp, int ncp, int ld, int ldo, int npts, int nang) {
+  dim3 grid(ncp / TN, mp / TM);
+  k_sh_to_do<true><<<grid, 256, 0, st>>>(S, Yc, ext, dosrc, ld, ldo, npts, nang);
+}
+
+void launch_sh_dot(hipStream_t st, const float* S, const float* Yd, float* raw, int mp, int ndp,
+                   int ld, int ldd, int npts, int ndir) {
+  dim3 grid(ndp / TN, mp / TM);
+  k_sh_to_do<false><<<grid, 256, 0, st>>>(S, Yd, nullptr, raw, ld, ldd, npts, ndir);
+}
+
+void launch_do_to_sh(hipStream_t st, const float* dorad, const float* Wc, const int* nr, float* R,
+                     int mp, int ld, int ldo, int npts) {
+  dim3 grid(mp / TM, ld / TN);
+  k_do_to_sh<<<grid, 256, 0, st>>>(dorad, Wc, nr, R, ld, ldo, npts);
+}
+
+}  // namespace at3d

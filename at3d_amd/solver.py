@@ -193,6 +193,34 @@ class RTE:
             warnings.warn("solution did not converge to solution_accuracy={} in maxiter={} "
                           "(SOLCRIT={})".format(self.solution_accuracy, maxiter, self._solcrit))
 
+    def integrate_to_sensor(self, sensor, single_scatter=False, nosurface=False):
+        """Radiance of every ray of ``sensor`` (solver.py:633-791).
+
+        ``sensor`` maps 'ray_x', 'ray_y', 'ray_z' (km), 'ray_mu' (cosine zenith of
+        the direction towards the sensor) and 'ray_phi' (radians) to 1-D arrays;
+        'I' is added to it and the sensor is returned."""
+        for k in ("ray_x", "ray_y", "ray_z", "ray_mu", "ray_phi"):
+            if k not in sensor:
+                raise KeyError(f"sensor is missing '{k}'")
+        self.check_solved()
+        self._ensure_state()
+        x = _lib.as_f32(sensor["ray_x"])
+        y = _lib.as_f32(sensor["ray_y"])
+        z = _lib.as_f32(sensor["ray_z"])
+        mu = np.ascontiguousarray(sensor["ray_mu"], dtype=np.float64)
+        phi = np.ascontiguousarray(sensor["ray_phi"], dtype=np.float64)
+        n = x.size
+        if not all(a.size == n for a in (y, z, mu, phi)):
+            raise ValueError("ray arrays must have the same length")
+        out = np.zeros(n, np.float32)
+        err = _lib.errbuf()
+        _lib.check(_lib.lib().at3d_render(self._state, n, x.ctypes.data, y.ctypes.data,
+                                          z.ctypes.data, mu.ctypes.data, phi.ctypes.data,
+                                          int(single_scatter), int(nosurface), out.ctypes.data,
+                                          err, len(err)), err)
+        sensor["I"] = out
+        return sensor
+
     def check_solved(self, verbose=True):
         """True once SOLCRIT <= solution_accuracy (solver.py:1012-1035)."""
         ok = self._solcrit is not None and self._solcrit <= self.solution_accuracy
@@ -230,6 +258,13 @@ class RTE:
         _lib.check(_lib.lib().at3d_state_get(self._state, name.encode(), out.ctypes.data,
                                              out.nbytes, err, len(err)), err)
         return out
+
+    def set_option(self, key: str, value: int):
+        """Engine option of the device state (see at3d_state_set_option)."""
+        self._ensure_state()
+        err = _lib.errbuf()
+        _lib.check(_lib.lib().at3d_state_set_option(self._state, key.encode(), int(value), err,
+                                                    len(err)), err)
 
     def timing(self, key: str) -> float:
         if self._state is None:

@@ -1,0 +1,157 @@
+"""Benchmark: converged SHDOM solves/s on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): scalar solar
+solve on a 64x64x40 open base grid (dx=dy=0.02 km, dz=0.025 km), NMU=16,
+NPHI=32, L=M=15, solution accuracy 1e-4, a seeded -5/3 Gaussian-field cloud
+(30/km x field^1.5, 40 % cover, 200-term phase function) plus a 0.01/km
+Rayleigh species.  A step is one plain converged solve: initial radiance and
+source (device) + source-function iterations until SOLCRIT <= 1e-4.  The host
+setup (grid, optics, direct beam, two-stream start, sweep plan) and the upload
+to HBM happen once before the timed region and are reported as ``setup_ms``.
+
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: this engine runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
+FP32_MFMA_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
+
+
+def cpu_baseline(workload_name: str, iters_gpu: int, niter: int):
+    """Times the reference Fortran (oracle/_ref) on one host core for `niter`
+    iterations of the same workload and extrapolates one converged solve."""
+    from oracle import ref_shdom as R
+    from at3d_amd import synthetic as S
+    if not R.available():
+        return None
+    w = S.workload(workload_name)
+    t0 = time.perf_counter()
+    st = R.setup(R.scene_from_workload(w))
+    R.init_solution(st)
+    t1 = time.perf_counter()
+    R.solution_iterations(st, niter)
+    t2 = time.perf_counter()
+    per_iter = (t2 - t1) / max(st.iters, 1)
+    solve_s = (t1 - t0) + per_iter * iters_gpu
+    return {"value": 1.0 / solve_s, "unit": "solves/s", "cores": 1, "kind": "reference",
+            "sample": f"reference Fortran (amdflang -O2) setup+INIT_SOLUTION {t1 - t0:.2f} s and "
+                      f"{st.iters} SOLUTION_ITERATIONS at {per_iter:.2f} s each on one host core; "
+                      f"one converged solve extrapolated to the GPU's {iters_gpu} iterations "
+                      f"= {solve_s:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cloud64")
+    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this SHDOM engine runs one solve per GPU and is measured on one GPU; "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from at3d_amd import synthetic as S
+
+    w = S.workload(args.workload)
+    t0 = time.perf_counter()
+    rte = S.make_rte(w)
+    rte.solve(0, verbose=False)         # allocates the device state, uploads, initialises
+    torch.cuda.synchronize()
+    setup_ms = (time.perf_counter() - t0) * 1e3
+
+    for _ in range(args.warmup):
+        rte.solve(w.maxiter, verbose=False)
+    torch.cuda.synchronize()
+
+    keys = ["truncation", "sh_to_do", "sweep", "do_to_sh", "source", "total"]
+    ksum = {k: 0.0 for k in keys}
+    iters_total = 0
+    sweep_levels = int(rte.timing("sweep_levels"))
+    lat = []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        rte.solve(w.maxiter, verbose=False)
+        lat.append((time.perf_counter() - ts) * 1e3)
+        for k in keys:
+            ksum[k] += rte.timing(k)
+        iters_total += int(rte.timing("iters"))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = elapsed * 1e3 / args.steps
+    iters = iters_total // args.steps
+
+    npts, nang, nlm = rte._npts, rte._nang, rte._nlm
+    # algorithmic bytes / flops per launch of each kernel class
+    gemm_flops = 2.0 * nang * npts * nlm
+    kern = {}
+    n_it = max(iters_total, 1)
+    # one sweep pass = all level launches of both hemispheres; per (ordinate, point):
+    # read source*extinction and write the radiance (8 B), plus the extinction once
+    sweep_bytes = nang * npts * 8.0 + npts * 4.0
+    ms_pass = ksum["sweep"] / n_it
+    kern["sweep"] = {"ms_per_pass": ms_pass, "level_launches_per_pass": sweep_levels,
+                     "ms_per_launch": ms_pass / max(sweep_levels, 1),
+                     "bytes_per_launch": sweep_bytes / max(sweep_levels, 1),
+                     "GB/s": sweep_bytes / ms_pass / 1e6}
+    for k in ("sh_to_do", "do_to_sh"):
+        ms = ksum[k] / n_it
+        kern[k] = {"ms_per_launch": ms, "flops_per_launch": gemm_flops,
+                   "TFLOP/s": gemm_flops / ms / 1e9}
+    ms_src = ksum["source"] / n_it
+    src_bytes = npts * nlm * 4 * 6.0       # R read, S read/write, DS read/write, chi/ysun
+    kern["source"] = {"ms_per_launch": ms_src, "bytes_per_launch": src_bytes,
+                      "GB/s": src_bytes / ms_src / 1e6}
+    dominant = max(("sweep", "sh_to_do", "do_to_sh", "source"), key=lambda k: ksum[k])
+    if dominant in ("sh_to_do", "do_to_sh"):
+        ach = kern[dominant]["TFLOP/s"]
+        roof = {"kernel": dominant, "bound": "mfma", "achieved": ach, "peak": FP32_MFMA_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": ach / FP32_MFMA_PEAK_TFLOPS, "traffic": None}
+    else:
+        ach = kern[dominant]["GB/s"]
+        roof = {"kernel": dominant, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.workload, iters, args.cpu_iters)
+
+    value = args.steps / elapsed
+    line = {
+        "metric": "converged solves/s @64x64x40 L=15 (plain); p50/p99 latency; %HBM roof",
+        "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "cloud64: 64x64x40 open grid, NMU16 NPHI32 L15, solacc 1e-4, "
+                               "two species (cloud + Rayleigh), plain one-at-a-time solves",
+                   "npts": npts, "nang": nang, "nlm": nlm, "iterations": iters,
+                   "setup_ms": setup_ms, "solcrit": rte.solcrit},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "kernel_ms_per_solve": {k: ksum[k] / args.steps for k in keys},
+        "kernels": kern,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

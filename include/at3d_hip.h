@@ -123,6 +123,24 @@ int at3d_solve(at3d_state* st, int32_t maxiter, int32_t verbose, float* solcrit_
 int at3d_state_get(at3d_state* st, const char* name, void* dst, size_t nbytes, char* err,
                    size_t errlen);
 
+/* Radiances of `nrays` rays from the current solution (RENDER,
+ * shdomsub4.f:93-286, with INTEGRATE_1RAY shdomsub2.f:2311-2743 and the TMS
+ * single-scatter correction of COMPUTE_SOURCE_1CELL_UNPOL :3046-3192).
+ * x,y,z: ray positions (km); mu, phi: cosine zenith and azimuth (radians) of the
+ * direction TOWARDS the sensor (at3d/sensor.py conventions); out: I per ray.
+ * Fails with the reference's errors: negative tabulated phase function
+ * (PRECOMPUTE_PHASE_CHECK), ray below the domain. */
+int at3d_render(at3d_state* st, int64_t nrays, const float* x, const float* y, const float* z,
+                const double* mu, const double* phi, int32_t single_scatter, int32_t nosurface,
+                float* out, char* err, size_t errlen);
+
+/* Engine options of a state.  "sweep_general" (0/1): route every grid point
+ * through the general multi-cell characteristic trace instead of the analytic
+ * one-cell path (the two are bit-identical; the switch exists for tests).
+ * No counterpart in the reference. */
+int at3d_state_set_option(at3d_state* st, const char* key, int32_t value, char* err,
+                          size_t errlen);
+
 /* Time (ms, HIP events on the state's stream) of each kernel class over the
  * last at3d_solve call: keys "sh_to_do","sweep","do_to_sh","source","total";
  * also "iters".  Returns a negative value for unknown keys. */

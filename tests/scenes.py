@@ -2,6 +2,7 @@
 import numpy as np
 
 from at3d_amd import synthetic as S
+from at3d_amd.sensor import orthographic_projection
 
 
 def blob8(bc="periodic", nleg=64, seed=0):
@@ -19,3 +20,48 @@ def rel_l2(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def absorber_columns(nx=8, ny=4, nz=6, taumax=2.0, gndalbedo=0.04, bc="open"):
+    """Pure absorber (single-scattering albedo 0) whose column optical depth
+    grows linearly with x and is uniform in y and z; overhead sun, flux 1.
+
+    Closed form at the column of x index i for a nadir view from the top:
+    I = exp(-tau_i) * gndalbedo / pi * exp(-tau_i), tau_i = taumax * i / (nx - 1)."""
+    z = np.linspace(0.0, 1.0, nz).astype(np.float32)
+    tau = taumax * np.arange(nx) / (nx - 1)
+    ext = np.broadcast_to((tau / (z[-1] - z[0]))[:, None, None], (nx, ny, nz)).astype(np.float32)
+    alb = np.zeros((nx, ny, nz), np.float32)
+    w = S.Workload("absorber_columns", 0.1, 0.1, z, [(np.ascontiguousarray(ext), alb,
+                                                        S.hg_legendre(0.0, 0))], bc=bc,
+                   nmu=8, nphi=16, solacc=1e-5, solarmu=-1.0, gndalbedo=gndalbedo)
+    return w, tau
+
+
+def bbox(w):
+    nx, ny, _ = w.shape
+    return (0.0, 0.0, float(w.z[0]), (nx - 1) * w.delx, (ny - 1) * w.dely, float(w.z[-1]))
+
+
+def ortho_rays(w, views):
+    xs, ys, zs, mus, phis = [], [], [], [], []
+    for zen, az in views:
+        s = orthographic_projection(0.86, bbox(w), w.delx * 0.7, w.dely * 0.7, az, zen)
+        for k, acc in (("ray_x", xs), ("ray_y", ys), ("ray_z", zs), ("ray_mu", mus),
+                       ("ray_phi", phis)):
+            acc.append(s[k])
+    return {k: np.concatenate(v) for k, v in zip(("ray_x", "ray_y", "ray_z", "ray_mu", "ray_phi"),
+                                                 (xs, ys, zs, mus, phis))}
+
+
+def interior_rays(w, n, seed):
+    """Rays that start inside the domain with arbitrary directions (up and down)."""
+    rng = np.random.default_rng(seed)
+    nx, ny, _ = w.shape
+    x = rng.uniform(0, (nx - 1) * w.delx, n)
+    y = rng.uniform(0, (ny - 1) * w.dely, n)
+    z = rng.uniform(w.z[0], w.z[-1], n)
+    mu = rng.uniform(-0.98, 0.98, n)
+    mu[np.abs(mu) < 0.05] = 0.3
+    phi = rng.uniform(0, 2 * np.pi, n)
+    return {"ray_x": x, "ray_y": y, "ray_z": z, "ray_mu": mu, "ray_phi": phi}

@@ -92,3 +92,29 @@ def test_converged_solution_matches_reference(ref, make):
     assert rel_l2(rte.state_array("source"), _ref_dense(ref, st, "source")) < 1e-4
     assert rel_l2(rte.state_array("radiance"), _ref_dense(ref, st, "radiance")) < 1e-4
     assert rel_l2(rte.state_array("fluxes"), st.fluxes[:, :st.npts]) < 1e-4
+
+
+@pytest.mark.parametrize("make", [lambda: blob8("periodic"), lambda: blob8("open"), cloud_small],
+                         ids=["blob8_periodic", "blob8_open", "cloud12_two_species"])
+def test_fast_sweep_path_bit_identical_to_general_trace(make):
+    """The analytic one-cell sweep path must reproduce the general trace exactly."""
+    w = make()
+    a = S.make_rte(w)
+    a.solve(3, verbose=False)
+    b = S.make_rte(w)
+    b.set_option("sweep_general", 1)
+    b.solve(3, verbose=False)
+    assert np.array_equal(a.state_array("do_radiance"), b.state_array("do_radiance"))
+    assert np.array_equal(a.state_array("source"), b.state_array("source"))
+
+
+def test_sweep_graph_replay_bit_identical_to_direct_launches():
+    w = cloud_small()
+    a = S.make_rte(w)
+    a.solve(4, verbose=False)
+    b = S.make_rte(w)
+    b.set_option("sweep_graph", 0)
+    b.solve(4, verbose=False)
+    assert np.array_equal(a.state_array("do_radiance"), b.state_array("do_radiance"))
+    assert np.array_equal(a.state_array("source"), b.state_array("source"))
+    assert a.num_iterations == b.num_iterations
