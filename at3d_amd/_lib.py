@@ -36,6 +36,14 @@ class SceneDesc(C.Structure):
     ]
 
 
+class DerivDesc(C.Structure):
+    _fields_ = [
+        ("numder", C.c_int32), ("partder", _i32p), ("dext", _f32p), ("dalb", _f32p),
+        ("doexact", _i32p), ("dnumphase", C.c_int32), ("dleg", _f32p), ("diphasep", _i32p),
+        ("dphasewtp", _f32p),
+    ]
+
+
 class SHDOMError(RuntimeError):
     """Raised when the engine returns a nonzero status (mirrors at3d.exceptions.SHDOMError)."""
 
@@ -63,12 +71,16 @@ def lib():
         L.at3d_state_create.argtypes = [vp, C.POINTER(vp), C.c_char_p, C.c_size_t]
         L.at3d_state_destroy.argtypes = [vp]
         L.at3d_init_solution.argtypes = [vp, C.c_char_p, C.c_size_t]
+        L.at3d_init_solution_from.argtypes = [vp, vp, C.c_char_p, C.c_size_t]
         L.at3d_solve.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_float),
                                  C.POINTER(C.c_int32), C.c_char_p, C.c_size_t]
         L.at3d_state_get.argtypes = [vp, C.c_char_p, vp, C.c_size_t, C.c_char_p, C.c_size_t]
         L.at3d_state_set_option.argtypes = [vp, C.c_char_p, C.c_int32, C.c_char_p, C.c_size_t]
         L.at3d_render.argtypes = [vp, C.c_int64, vp, vp, vp, vp, vp, C.c_int32, C.c_int32, vp,
                                   C.c_char_p, C.c_size_t]
+        L.at3d_levis_gradient.argtypes = [vp, C.POINTER(DerivDesc), C.c_int64, vp, vp, vp, vp, vp,
+                                          C.c_int64, vp, vp, vp, vp, vp, C.c_char_p, C.c_int32,
+                                          vp, vp, vp, C.c_char_p, C.c_size_t]
         L.at3d_state_timing.argtypes = [vp, C.c_char_p]
         L.at3d_state_timing.restype = C.c_double
         _lib = L
@@ -88,7 +100,7 @@ def errbuf():
 EXPORTS = ["at3d_version", "at3d_plan_create", "at3d_plan_destroy", "at3d_plan_size",
            "at3d_plan_get", "at3d_state_create", "at3d_state_destroy", "at3d_init_solution",
            "at3d_solve", "at3d_state_get", "at3d_state_set_option", "at3d_state_timing",
-           "at3d_render"]
+           "at3d_render", "at3d_levis_gradient", "at3d_init_solution_from"]
 
 
 def as_f32(a):

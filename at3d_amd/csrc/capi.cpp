@@ -134,6 +134,21 @@ int at3d_plan_get(const at3d_plan* h, const char* name, void* dst, size_t nbytes
   if (k == "chi") return copy_out(p.chi, dst, nbytes, err, errlen);
   if (k == "alb") return copy_out(p.alb, dst, nbytes, err, errlen);
   if (k == "bcptr_top") return copy_out(p.bctop, dst, nbytes, err, errlen);
+  if (k == "sunpath_off" || k == "sunpath_ptr" || k == "sunpath_val") {
+    // per RTE point, the property points on its path to the sun and the
+    // derivative of the slant optical path with respect to their extinction
+    std::vector<int32_t> off, ptr;
+    std::vector<float> val;
+    std::string msg;
+    const int rc = at3d::plan_sun_paths(p, off, ptr, val, msg);
+    if (rc) {
+      set_err(err, errlen, msg);
+      return rc;
+    }
+    if (k == "sunpath_off") return copy_out(off, dst, nbytes, err, errlen);
+    if (k == "sunpath_ptr") return copy_out(ptr, dst, nbytes, err, errlen);
+    return copy_out(val, dst, nbytes, err, errlen);
+  }
   if (k == "bcptr_bot") return copy_out(p.bcbot, dst, nbytes, err, errlen);
   if (k == "sweepord" || k == "level") {
     std::vector<int32_t> all;
@@ -199,6 +214,17 @@ int at3d_init_solution(at3d_state* st, char* err, size_t errlen) {
   return rc;
 }
 
+int at3d_init_solution_from(at3d_state* st, at3d_state* src, char* err, size_t errlen) {
+  if (!st || !src) {
+    set_err(err, errlen, "null argument");
+    return AT3D_ERR_ARG;
+  }
+  std::string msg;
+  const int rc = at3d::state_init_solution_from(st->s, src->s, msg);
+  if (rc) set_err(err, errlen, msg);
+  return rc;
+}
+
 int at3d_solve(at3d_state* st, int32_t maxiter, int32_t verbose, float* solcrit_out,
                int32_t* iters_out, char* err, size_t errlen) {
   if (!st) {
@@ -227,8 +253,9 @@ int at3d_state_get(at3d_state* st, const char* name, void* dst, size_t nbytes, c
       set_err(err, errlen, "destination buffer too small");
       return AT3D_ERR_ARG;
     }
-    hipError_t e = hipMemcpy2D(dst, cols * sizeof(float), src, ld * sizeof(float),
-                               cols * sizeof(float), rows, hipMemcpyDeviceToHost);
+    hipError_t e = hipMemcpy2DAsync(dst, cols * sizeof(float), src, ld * sizeof(float),
+                                    cols * sizeof(float), rows, hipMemcpyDeviceToHost, s.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s.stream);
     if (e != hipSuccess) {
       set_err(err, errlen, hipGetErrorString(e));
       return AT3D_ERR_DEVICE;
@@ -240,7 +267,8 @@ int at3d_state_get(at3d_state* st, const char* name, void* dst, size_t nbytes, c
       set_err(err, errlen, "destination buffer too small");
       return AT3D_ERR_ARG;
     }
-    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s.stream);
     if (e != hipSuccess) {
       set_err(err, errlen, hipGetErrorString(e));
       return AT3D_ERR_DEVICE;
@@ -255,15 +283,16 @@ int at3d_state_get(at3d_state* st, const char* name, void* dst, size_t nbytes, c
   if (k == "radiance") return get2d(s.R, npts, s.nlm, s.ld);
   if (k == "delsource") return get2d(s.DS, npts, s.nlm, s.ld);
   if (k == "do_radiance" || k == "do_source") {
-    // device layout [point][column] -> [reference ordinate][point]
+    // device layout [column][padded point] -> [reference ordinate][point]
     const size_t nang = s.nang;
     if (nbytes < nang * npts * sizeof(float)) {
       set_err(err, errlen, "destination buffer too small");
       return AT3D_ERR_ARG;
     }
-    std::vector<float> tmp((size_t)npts * s.ldo);
-    hipError_t e = hipMemcpy(tmp.data(), k == "do_radiance" ? s.dorad : s.dosrc,
-                             tmp.size() * sizeof(float), hipMemcpyDeviceToHost);
+    std::vector<float> tmp((size_t)s.nptsp * s.ldo);
+    hipError_t e = hipMemcpyAsync(tmp.data(), k == "do_radiance" ? s.dorad : s.dosrc,
+                                  tmp.size() * sizeof(float), hipMemcpyDeviceToHost, s.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s.stream);
     if (e != hipSuccess) {
       set_err(err, errlen, hipGetErrorString(e));
       return AT3D_ERR_DEVICE;
@@ -271,7 +300,7 @@ int at3d_state_get(at3d_state* st, const char* name, void* dst, size_t nbytes, c
     float* out = static_cast<float*>(dst);
     for (size_t k2 = 0; k2 < nang; k2++) {
       const size_t c = s.col_of_ref[k2];
-      for (size_t pnt = 0; pnt < npts; pnt++) out[k2 * npts + pnt] = tmp[pnt * s.ldo + c];
+      for (size_t pnt = 0; pnt < npts; pnt++) out[k2 * npts + pnt] = tmp[c * s.nptsp + pnt];
     }
     return AT3D_OK;
   }
@@ -302,6 +331,33 @@ int at3d_render(at3d_state* st, int64_t nrays, const float* x, const float* y, c
   return rc;
 }
 
+int at3d_levis_gradient(at3d_state* st, const at3d_deriv_desc* deriv, int64_t nrays,
+                        const float* x, const float* y, const float* z, const double* mu,
+                        const double* phi, int64_t npix, const int32_t* rays_per_pixel,
+                        const double* ray_weights, const double* stokes_weights,
+                        const float* measurements, const double* uncertainties,
+                        const char* costfunc, int32_t exact_single_scatter, double* grad,
+                        double* cost, float* pixels, char* err, size_t errlen) {
+  if (!st || !deriv || !x || !y || !z || !mu || !phi || !rays_per_pixel || !ray_weights ||
+      !stokes_weights || !measurements || !uncertainties || !grad || !cost || !pixels ||
+      !deriv->partder || !deriv->dext || !deriv->dalb) {
+    set_err(err, errlen, "null argument");
+    return AT3D_ERR_ARG;
+  }
+  std::string msg;
+  int rc;
+  try {
+    rc = at3d::state_levis_gradient(st->s, *deriv, nrays, x, y, z, mu, phi, npix, rays_per_pixel,
+                                    ray_weights, stokes_weights, measurements, uncertainties,
+                                    costfunc, exact_single_scatter, grad, cost, pixels, msg);
+  } catch (const std::bad_alloc&) {
+    rc = AT3D_ERR_MEMORY;
+    msg = "out of host memory in the gradient";
+  }
+  if (rc) set_err(err, errlen, msg);
+  return rc;
+}
+
 int at3d_state_set_option(at3d_state* st, const char* key, int32_t value, char* err,
                           size_t errlen) {
   if (!st || !key) {
@@ -315,6 +371,10 @@ int at3d_state_set_option(at3d_state* st, const char* key, int32_t value, char* 
   }
   if (k == "sweep_graph") {
     st->s.use_graph = value ? 1 : 0;
+    return AT3D_OK;
+  }
+  if (k == "max_subgrid_intervals") {
+    st->s.max_subgrid = value > 0 ? value : 0;
     return AT3D_OK;
   }
   set_err(err, errlen, "unknown option '" + k + "'");
@@ -331,11 +391,12 @@ double at3d_state_timing(const at3d_state* st, const char* key) {
   if (k == "do_to_sh") return s.tsum[3];
   if (k == "source") return s.tsum[4];
   if (k == "sweep_levels")
-    return (double)(s.levels_down.size() + s.levels_up.size());
+    return (double)s.nlevels_max;
   if (k == "total") return s.tsum[0] + s.tsum[1] + s.tsum[2] + s.tsum[3] + s.tsum[4] + s.tsum[5];
   if (k == "sweep_launches") return s.nsweep_launch;
   if (k == "iters") return s.iters_last;
   if (k == "render") return s.render_ms;
+  if (k == "gradient") return s.grad_ms;
   return -1.0;
 }
 

@@ -657,9 +657,13 @@ void direct_beam_init(const Plan& p, DirectBeam& b) {
   if (btest(p.bcflag, 3)) b.ydomain = b.delyd * (npy - 1);
 }
 
-// Returns false when the reference would STOP.
+// Returns false when the reference would STOP.  With `path` the traversal also
+// records, per property-grid cell crossed, the derivative of the slant optical
+// path with respect to each of the 8 corner extinctions: the integral of that
+// corner's trilinear weight along the segment (DIRECT_BEAM_AND_PATHS_PROP,
+// shdomsub5.f:1608-2003).
 bool direct_beam_point(const Plan& p, const DirectBeam& b, float xi, float yi, float zi,
-                       float& dirflux) {
+                       float& dirflux, std::vector<std::pair<int32_t, float>>* path_rec = nullptr) {
   const at3d_scene_desc& d = p.d;
   const int npx = d.npx, npy = d.npy, npz = d.npz, bc = p.bcflag;
   const float* ZL = p.zlevels.data();
@@ -760,6 +764,7 @@ bool direct_beam_point(const Plan& p, const DirectBeam& b, float xi, float yi, f
     double uv = u0 * v0, umv = u0m * v0, uvm = u0 * v0m, umvm = u0m * v0m;
     double uw = u0 * w0, umw = u0m * w0, uwm = u0 * w0m, umwm = u0m * w0m;
     double vw = v0 * w0, vmw = v0m * w0, vwm = v0 * w0m, vmwm = v0m * w0m;
+    const double vw0 = vw;
     const double A = (E1 * u0m + E2 * u0) * vmwm + (E3 * u0m + E4 * u0) * vwm +
                      (E5 * u0m + E6 * u0) * vmw + (E7 * u0m + E8 * u0) * vw;
     const double B1 = -du * vmwm - dv * umwm - dw * umvm;
@@ -783,6 +788,20 @@ bool direct_beam_point(const Plan& p, const DirectBeam& b, float xi, float yi, f
     const double C = C1 * E1 + C2 * E2 + C3 * E3 + C4 * E4 + C5 * E5 + C6 * E6 + C7 * E7 + C8 * E8;
     const double D = du * dv * dw * (E2 + E3 + E5 + E8 - E1 - E4 - E6 - E7);
     path = path + so * (A + 0.5 * B + 0.3333333333333333 * C + 0.25 * D);
+    if (path_rec) {
+      // integral over t in [0,1] of the corner weight (u0 + t du)(v0 + t dv)(w0 + t dw)
+      // (or its complements), times the segment length
+      const double g = du * dv * dw;
+      const double wa[8] = {u0m * vmwm, u0 * vmwm, u0m * vwm, u0 * vwm,
+                            u0m * vmw,  u0 * vmw,  u0m * vw0, u0 * vw0};
+      const double wb[8] = {B1, B2, B3, B4, B5, B6, B7, B8};
+      const double wc[8] = {C1, C2, C3, C4, C5, C6, C7, C8};
+      const double sg[8] = {-1, 1, 1, -1, 1, -1, -1, 1};
+      const int32_t ii[8] = {I1, I2, I3, I4, I1 + 1, I2 + 1, I3 + 1, I4 + 1};
+      for (int c = 0; c < 8; c++)
+        path_rec->emplace_back(ii[c], (float)(so * (wa[c] + 0.5 * wb[c] + 0.3333333333333333 * wc[c] +
+                                                    0.25 * sg[c] * g)));
+    }
     xe = xp + xoffs;
     ye = yp + yoffs;
     ze = zp;
@@ -1279,6 +1298,36 @@ int render_constants(Plan& p, std::string& err) {
 }  // namespace
 
 void ylm_direction(float mu, float phi, int ml, int mm, float* yr) { ylm_unpol(mu, phi, ml, mm, yr); }
+
+void plan_mixed_legen(const Plan& p, int ipa, int ip, std::vector<float>& out) {
+  mixed_legen(p, ipa, ip, out);
+}
+
+int plan_sun_paths(const Plan& p, std::vector<int32_t>& off, std::vector<int32_t>& ptr,
+                   std::vector<float>& val, std::string& err) {
+  DirectBeam beam;
+  direct_beam_init(p, beam);
+  off.assign(p.npts + 1, 0);
+  ptr.clear();
+  val.clear();
+  std::vector<std::pair<int32_t, float>> rec;
+  for (int ip = 0; ip < p.npts; ip++) {
+    off[ip] = (int32_t)ptr.size();
+    rec.clear();
+    float df;
+    if (!direct_beam_point(p, beam, p.gridpos[ip * 3], p.gridpos[ip * 3 + 1], p.gridpos[ip * 3 + 2], df,
+                           &rec)) {
+      err = "DIRECT_BEAM_AND_PATHS_PROP: ray left the property grid";
+      return AT3D_ERR_NUMERIC;
+    }
+    for (auto& e : rec) {
+      ptr.push_back(e.first);
+      val.push_back(e.second);
+    }
+  }
+  off[p.npts] = (int32_t)ptr.size();
+  return AT3D_OK;
+}
 
 int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err) {
   p.d = desc;

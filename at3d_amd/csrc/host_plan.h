@@ -77,4 +77,14 @@ int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err);
 // shdomsub2.f:4490-4539), J-ordered, nlm values.
 void ylm_direction(float mu, float phi, int ml, int mm, float* yr);
 
+// Mixed phase function of species ipa at point ip (the reference's LEGENT,
+// 'N' interpolation), coefficients 0..nleg.
+void plan_mixed_legen(const Plan& p, int ipa, int ip, std::vector<float>& out);
+
+// Per RTE point, the property-grid points on its path to the sun and the
+// derivative of the slant optical path with respect to their (delta-M scaled)
+// extinction, as CSR (off[npts+1], 0-based ptr, val).
+int plan_sun_paths(const Plan& p, std::vector<int32_t>& off, std::vector<int32_t>& ptr,
+                   std::vector<float>& val, std::string& err);
+
 }  // namespace at3d

@@ -8,19 +8,21 @@
 namespace at3d {
 
 // ---- transforms.hip: FP32 MFMA SH <-> DO transforms ----------------------
-// dosrc[p][c] = max(0, sum_J S[p][J] * Yc[c][J]) * ext[p]
-// S [mp][ld], Yc [ncp][ld], dosrc [mp][ldo]; mp, ncp multiples of 64; ld multiple of 16.
+// dosrc[c][p] = max(0, sum_J S[p][J] * Yc[c][J]) * ext[p]
+// S [mp][ld], Yc [ncp][ld], dosrc [ncols][ldp] (ordinate-major, ldp = mp);
+// mp, ncp multiples of 64; ld multiple of 16; ext padded to mp.
 void launch_sh_to_do(hipStream_t st, const float* S, const float* Yc, const float* ext,
-                     float* dosrc, int mp, int ncp, int ld, int ldo, int npts, int nang);
-// R[p][J] = (J < nr[p]) ? sum_c dorad[p][c] * Wc[c][J] : 0
-// dorad [mp][ldo], Wc [ldo][ld] (rows beyond nang zero), R [mp][ld].
+                     float* dosrc, int mp, int ncp, int ld, int ldp, int npts, int nang);
+// R[p][J] = (J < nr[p]) ? sum_c dorad[c][p] * Wc[c][J] : 0
+// dorad [ldo][ldp] (rows beyond nang zero), Wc [ldo][ld] (rows beyond nang zero), R [mp][ld].
 void launch_do_to_sh(hipStream_t st, const float* dorad, const float* Wc, const int* nr, float* R,
-                     int mp, int ld, int ldo, int npts);
+                     int mp, int ld, int ldo, int ldp, int npts);
 
 // ---- sweep.hip: discrete-ordinate sweeps --------------------------------
 struct SweepArgs {
   GridTopo g;
   const OrdGeom* ordg;       // [ncols] device column order
+  const int8_t* col_oct;     // [ncols] octant of each column
   const float* skyrad;       // [ncols]
   const float* bcrad_bot;    // [nbot]
   const int32_t* bctop;      // [ntop]
@@ -31,37 +33,29 @@ struct SweepArgs {
   const int32_t* level;      // [8][npts]
   const int32_t* lvl_pts;    // concatenated over octants (~pt = general trace)
   const int32_t* lvl_cell;
+  const int32_t* lvl_off;    // per octant o: level L spans lvl_pts[lvl_off[lo_base[o]+L-1] ..
+                             // lvl_off[lo_base[o]+L])
+  int lo_base[8], nlev[8];
   const float* ext;          // [npts]
-  const float* srcext;       // [npts][ldo]
-  float* rad;                // [npts][ldo]
+  const float* srcext;       // [ncols][ldp]
+  float* rad;                // [ncols][ldp]
   const float* xgrid;
   const float* ygrid;
   const float* zgrid;
   int nx1, ny1, nz;
-  int ldo, npts;
+  int ldp, npts;
   int force_general;
   int* errflag;
 };
 
-// Items of one level launch: up to four octants of one hemisphere, each
-// contributing (points of this level) x (ordinates of the octant).
-struct LevelDesc {
-  int item_off[5];
-  int pts_off[4];   // into the concatenated level point list
-  int nord[4];
-  int col_base[4];
-  int oct[4];
-  int nslot;
-  int L;
-};
-
 void launch_sweep_preset(hipStream_t st, const SweepArgs& a, bool up, int col0, int ncol);
-void launch_sweep_level(hipStream_t st, const SweepArgs& a, const LevelDesc& d);
+// One workgroup per ordinate column col0 .. col0+ncol-1, all levels of its octant.
+void launch_sweep_columns(hipStream_t st, const SweepArgs& a, int col0, int ncol);
 // Lambertian surface radiance from the downwelling flux (reference ordinate order).
-void launch_bottom_bc(hipStream_t st, const float* rad, int ldo, const int32_t* bcbot, int nbot,
+void launch_bottom_bc(hipStream_t st, const float* rad, int ldp, const int32_t* bcbot, int nbot,
                       const int32_t* col_of_ref, const float* fluxw, int ndown,
                       const float* dirflux, float albpi, float* bcrad);
-void launch_fluxes(hipStream_t st, const float* rad, int ldo, const int32_t* col_of_ref,
+void launch_fluxes(hipStream_t st, const float* rad, int ldp, const int32_t* col_of_ref,
                    const float* fluxw, int ndown, int nang, int npts, float* fluxes);
 
 // ---- source.hip: truncation, source function, acceleration ---------------
@@ -128,5 +122,77 @@ void launch_render_bottom(hipStream_t st, const float* fluxes, const float* dirf
 // raw[p][d] = sum_J S[p][J] * Yd[d][J] (no clamp, no extinction)
 void launch_sh_dot(hipStream_t st, const float* S, const float* Yd, float* raw, int mp, int ndp,
                    int ld, int ldd, int npts, int ndir);
+
+// ---------------------------------------------------------------------------
+// Levis gradient
+// ---------------------------------------------------------------------------
+#define AT3D_MAX_L 63   // highest Legendre order the gradient kernels hold in registers
+
+struct AdjArgs {
+  RayArgs r;                   // geometry, dsrc (source x ext), rays, boundary
+  const float* ss8;            // [npts][ldd] exact single scatter x ext
+  const double* adj_weight;    // [nrays] adjoint x ray x Stokes weight
+  const float* dirflux;        // [npts]
+  float opi_gndalbedo;         // ground albedo / pi (FL Lambertian)
+  int exact_single_scatter;
+  int max_intervals;           // sub-step intervals a ray may record
+  double* bsrc;                // [npts][ldd] source-gradient weights
+  double* erad;                // [npts] radiance-term weights
+  double* beamw;               // [npts] direct-beam weights
+};
+
+struct GradPointArgs {
+  int npts, ndir, ldd, ld, ml, numder, npart, maxpg, numphase, dnumphase;
+  float secmu0;
+  const double* bsrc;
+  const double* erad;
+  const double* beamw;
+  double* grad;                // [numder][maxpg]
+  const float* R;              // [npts][ld] radiance SH
+  const float* Yd;             // [ndir][ld]
+  const int* nr;
+  const int* lofj;
+  const float* dirflux;
+  const float* ssd;            // [ndir][numphase] phase table at each direction's scattering angle
+  const float* dssd;           // [ndir][dnumphase] derivative phase table, same angles
+  const int32_t* partder;      // [numder] 0-based species
+  const int32_t* doexact;      // [numder]
+  const int32_t* sj_off;       // [npart*npts+1] CSR of the SINGSCATJ terms
+  const int32_t* sj_iph;
+  const float* sj_coef;
+  const float* albedo;         // [npart][npts]
+  const float* srcpre;         // [npts][ldd]
+  const int8_t* scatj_ok;      // [npart][npts]
+  const float* legt_s;         // [npart][npts][ml+1] delta-M scaled mixed phase (npart > 1)
+  const float* f_s;            // [npart][npts]
+  const float* legt_u;         // [npart][npts][ml+1] unscaled mixed phase
+  const float* optinterpwt;    // [npts][8]
+  const int32_t* interpptr;    // [npts][8]
+  const float* extinctp;       // [npart][maxpg]
+  const float* albedop;
+  const int32_t* iphasep;      // 1-based
+  const float* phasewtp;
+  const float* dext;           // [numder][maxpg]
+  const float* dalb;
+  const float* dphasewtp;
+  const int32_t* diphasep;     // 0-based
+  const float* dextm;          // [numder][maxpg]
+  const float* legp;           // [npart][maxpg][ml+1] unscaled phase x weight at property points
+  const float* dlegp;          // [numder][maxpg][ml+1]
+  const float* dfj;            // [numder][npts][8]
+  const float* dalbm;
+  const int32_t* bp_off;       // [maxpg+1] sun paths by property point
+  const int32_t* bp_ip;
+  const float* bp_w;
+};
+
+void launch_dir_source_grad(hipStream_t st, const float* raw, int ldd, int ndir, const float* ext,
+                            const int* ns, const int* lofj, const float* tms, const float* G,
+                            const int32_t* ss_off, const int32_t* ss_iph, const float* ss_coef,
+                            const float* ssd, int numphase, int ml, int npts, int single_scatter,
+                            float* dsrc, float* srcpre, float* ss8);
+void launch_adjoint_rays(hipStream_t st, const AdjArgs& a);
+void launch_grad_points(hipStream_t st, const GradPointArgs& a);
+void launch_grad_beam(hipStream_t st, const GradPointArgs& a);
 
 }  // namespace at3d

@@ -28,6 +28,69 @@ namespace at3d {
 
 static size_t rup(size_t v, size_t m) { return (v + m - 1) / m * m; }
 
+// Distinct ray directions and their tables: spherical harmonics Y_J(dir)
+// [ldd][ld], the truncated single-scatter factors G[dir][l] = sum over J of
+// degree l of Y_J(dir) Y_J(sun), and the tabulated phase functions at the
+// solar scattering angle [dir][numphase] (linear in angle, as
+// INTEGRATE_1RAY shdomsub2.f:2393-2433).
+int direction_tables(const State& s, int64_t nrays, const double* mu, const double* phi, DirTables& t,
+                     std::string& err) {
+  const Plan& p = *s.plan;
+  std::map<std::pair<double, double>, int> dmap;
+  t.dir_of_ray.assign(nrays, 0);
+  t.dirs.clear();
+  for (int64_t i = 0; i < nrays; i++) {
+    auto key = std::make_pair(mu[i], phi[i]);
+    auto it = dmap.find(key);
+    if (it == dmap.end()) {
+      it = dmap.emplace(key, (int)t.dirs.size()).first;
+      t.dirs.push_back(key);
+    }
+    t.dir_of_ray[i] = it->second;
+  }
+  const int ndir = (int)t.dirs.size();
+  const int ldd = (int)rup(ndir, 64);
+  t.ndir = ndir;
+  t.ldd = ldd;
+  if ((int64_t)s.nptsp * ldd >= ((int64_t)1 << 31)) {
+    err = "too many distinct ray directions for the direction-source table";
+    return AT3D_ERR_ARG;
+  }
+  const int ML = p.ml, nlm = p.nlm, np = p.numphase, nsa = p.nscatangle;
+  const bool tms_on = p.d.deltam != 0;
+  t.Yd.assign((size_t)ldd * s.ld, 0.f);
+  t.G.assign((size_t)ndir * (ML + 1), 0.f);
+  t.ssd.assign((size_t)ndir * np, 0.f);
+  t.sa_j.assign(ndir, 1);
+  t.sa_f.assign(ndir, 0.f);
+  std::vector<float> ylm(nlm);
+  const double PI = acos(-1.0);
+  const float solarmu = p.d.solarmu, solaraz = p.d.solaraz;
+  for (int d = 0; d < ndir; d++) {
+    const double mu2 = t.dirs[d].first, phi2 = t.dirs[d].second;
+    ylm_direction((float)mu2, (float)phi2, ML, p.mm, ylm.data());
+    for (int j = 0; j < nlm; j++) t.Yd[(size_t)d * s.ld + j] = ylm[j];
+    std::vector<double> g(ML + 1, 0.0);
+    for (int j = 0; j < nlm; j++) g[p.lofj[j]] += (double)ylm[j] * (double)p.ylmsun[j];
+    for (int l = 0; l <= ML; l++) t.G[(size_t)d * (ML + 1) + l] = (float)g[l];
+    if (tms_on) {
+      double cosscat = (double)solarmu * mu2 +
+                       sqrt((double)(1.0f - solarmu * solarmu) * (1.0 - mu2 * mu2)) *
+                           cos((double)solaraz - phi2);
+      cosscat = std::max(std::min(1.0, cosscat), -1.0);
+      float f = (float)((nsa - 1) * (acos(cosscat) / PI) + 1);
+      const int j = std::min(nsa - 1, (int)f);
+      f = f - (float)j;
+      t.sa_j[d] = j;
+      t.sa_f[d] = f;
+      for (int iph = 0; iph < np; iph++)
+        t.ssd[(size_t)d * np + iph] =
+            (1 - f) * p.phasetab[(size_t)iph * nsa + j - 1] + f * p.phasetab[(size_t)iph * nsa + j];
+    }
+  }
+  return AT3D_OK;
+}
+
 int state_render(State& s, int64_t nrays, const float* x, const float* y, const float* z,
                  const double* mu, const double* phi, int single_scatter, int nosurface,
                  float* out, std::string& err) {
@@ -52,55 +115,18 @@ int state_render(State& s, int64_t nrays, const float* x, const float* y, const 
       return AT3D_ERR_ARG;
     }
   }
-  // distinct directions
-  std::map<std::pair<double, double>, int> dmap;
-  std::vector<int32_t> dir_of_ray(nrays);
-  std::vector<std::pair<double, double>> dirs;
-  for (int64_t i = 0; i < nrays; i++) {
-    auto key = std::make_pair(mu[i], phi[i]);
-    auto it = dmap.find(key);
-    if (it == dmap.end()) {
-      it = dmap.emplace(key, (int)dirs.size()).first;
-      dirs.push_back(key);
-    }
-    dir_of_ray[i] = it->second;
+  DirTables t;
+  {
+    const int rc = direction_tables(s, nrays, mu, phi, t, err);
+    if (rc) return rc;
   }
-  const int ndir = (int)dirs.size();
-  const int ldd = (int)rup(ndir, 64);
-  if ((int64_t)s.nptsp * ldd >= ((int64_t)1 << 31)) {
-    err = "too many distinct ray directions for the direction-source table";
-    return AT3D_ERR_ARG;
-  }
-  if ((size_t)s.nptsp * ldd * sizeof(float) > ((size_t)48 << 30)) {
-    err = "too many distinct ray directions for the direction-source table";
-    return AT3D_ERR_ARG;
-  }
-  const int ML = p.ml, nlm = p.nlm, np = p.numphase, nsa = p.nscatangle;
+  const int ndir = t.ndir, ldd = t.ldd;
+  const int ML = p.ml, np = p.numphase;
   const bool tms_on = p.d.deltam != 0;
-  std::vector<float> Yd((size_t)ldd * s.ld, 0.f), G((size_t)ndir * (ML + 1), 0.f),
-      ssd((size_t)ndir * np, 0.f), ylm(nlm);
-  const double PI = acos(-1.0);
-  const float solarmu = p.d.solarmu, solaraz = p.d.solaraz;
-  for (int d = 0; d < ndir; d++) {
-    const double mu2 = dirs[d].first, phi2 = dirs[d].second;
-    ylm_direction((float)mu2, (float)phi2, ML, p.mm, ylm.data());
-    for (int j = 0; j < nlm; j++) Yd[(size_t)d * s.ld + j] = ylm[j];
-    std::vector<double> g(ML + 1, 0.0);
-    for (int j = 0; j < nlm; j++) g[p.lofj[j]] += (double)ylm[j] * (double)p.ylmsun[j];
-    for (int l = 0; l <= ML; l++) G[(size_t)d * (ML + 1) + l] = (float)g[l];
-    if (tms_on) {
-      double cosscat = (double)solarmu * mu2 +
-                       sqrt((double)(1.0f - solarmu * solarmu) * (1.0 - mu2 * mu2)) *
-                           cos((double)solaraz - phi2);
-      cosscat = std::max(std::min(1.0, cosscat), -1.0);
-      float f = (float)((nsa - 1) * (acos(cosscat) / PI) + 1);
-      const int j = std::min(nsa - 1, (int)f);
-      f = f - (float)j;
-      for (int iph = 0; iph < np; iph++)
-        ssd[(size_t)d * np + iph] =
-            (1 - f) * p.phasetab[(size_t)iph * nsa + j - 1] + f * p.phasetab[(size_t)iph * nsa + j];
-    }
-  }
+  std::vector<int32_t>& dir_of_ray = t.dir_of_ray;
+  std::vector<float>& Yd = t.Yd;
+  std::vector<float>& G = t.G;
+  std::vector<float>& ssd = t.ssd;
   std::vector<float> topval(nrays);
   for (int64_t i = 0; i < nrays; i++) topval[i] = (mu[i] < 0.0) ? p.d.skyrad : 0.0f;
   // device buffers in one block
@@ -117,6 +143,7 @@ int state_render(State& s, int64_t nrays, const float* x, const float* y, const 
   const size_t o_mu = carve(nr * 8), o_phi = carve(nr * 8), o_dir = carve(nr * 4);
   const size_t o_top = carve(nr * 4), o_out = carve(nr * 4);
   if (off > s.rbuf_bytes) {
+    std::lock_guard<std::mutex> lock(device_mutex());
     if (s.rbuf) (void)hipFree(s.rbuf);
     s.rbuf = nullptr;
     s.rbuf_bytes = 0;

@@ -5,15 +5,15 @@
 //   SH -> DO transform   (SH_TO_DO_UNPOL :2928) MFMA GEMM      transforms.hip
 //   downwelling sweeps, Lambertian surface, upwelling sweeps
 //                        (PATH_INTEGRATION :1836, BACK_INT_GRID3D_UNPOL :3695)
-//                        level launches replayed from a HIP graph   sweep.hip
+//                        one workgroup per ordinate, replayed from a HIP graph   sweep.hip
 //   DO -> SH transform   (DO_TO_SH_UNPOL :3164) MFMA GEMM      transforms.hip
 //   source function      (COMPUTE_SOURCE :967)                 source.hip
 //   criterion            (CALC_ACCEL_SOLCRIT shdom_nompi.f:317) host, single precision
 //   acceleration         (ACCELERATE_SOLUTION :1807)           source.hip
 //
 // Device layouts: SH fields [point][ld] (ld = nlm rounded up to 64); DO fields
-// [point][ldo] with the ordinates as columns grouped by octant (ldo = nang
-// rounded up to 64, padding columns zero).
+// ordinate-major [ldo][nptsp] with the ordinates grouped by octant (ldo = nang
+// rounded up to 64, nptsp = points rounded up to 64, padding zero).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
@@ -47,11 +47,18 @@ namespace at3d {
 
 static size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
 
+// Device allocations of a state are filled on the state's own stream (which
+// does not synchronise with the legacy default stream, so states driven from
+// several host threads never wait on, or disturb a graph capture of, each
+// other) and waited for before returning.
 template <typename T>
 static int upload(State& s, T*& dst, const T* src, size_t n, std::string& err) {
   HIPCHK(hipMalloc((void**)&dst, std::max<size_t>(n, 1) * sizeof(T)));
   s.allocs.push_back(dst);
-  if (n) HIPCHK(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+  if (n) {
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+  }
   return AT3D_OK;
 }
 
@@ -59,7 +66,8 @@ template <typename T>
 static int alloc0(State& s, T*& dst, size_t n, std::string& err) {
   HIPCHK(hipMalloc((void**)&dst, std::max<size_t>(n, 1) * sizeof(T)));
   s.allocs.push_back(dst);
-  HIPCHK(hipMemset(dst, 0, std::max<size_t>(n, 1) * sizeof(T)));
+  HIPCHK(hipMemsetAsync(dst, 0, std::max<size_t>(n, 1) * sizeof(T), s.stream));
+  HIPCHK(hipStreamSynchronize(s.stream));
   return AT3D_OK;
 }
 
@@ -79,6 +87,12 @@ static SweepArgs sweep_args(const State& s) {
   a.level = s.level;
   a.lvl_pts = s.lvl_pts;
   a.lvl_cell = s.lvl_cell;
+  a.lvl_off = s.lvl_off;
+  a.col_oct = s.col_oct;
+  for (int o = 0; o < 8; o++) {
+    a.lo_base[o] = s.lo_base_h[o];
+    a.nlev[o] = s.oct_nord[o] > 0 ? p.oct[o].nlevels : 0;
+  }
   a.ext = s.ext;
   a.srcext = s.dosrc;
   a.rad = s.dorad;
@@ -88,45 +102,22 @@ static SweepArgs sweep_args(const State& s) {
   a.nx1 = p.nx1;
   a.ny1 = p.ny1;
   a.nz = p.nz;
-  a.ldo = s.ldo;
+  a.ldp = s.nptsp;
   a.npts = p.npts;
   a.force_general = s.force_general;
   a.errflag = s.errflag;
   return a;
 }
 
-// Level descriptors of one hemisphere (octants 0-3 downwelling, 4-7 upwelling).
-static void build_level_descs(const Plan& p, const State& s, int hemi, std::vector<LevelDesc>& out) {
-  out.clear();
-  int maxl = 0;
-  for (int o = 4 * hemi; o < 4 * hemi + 4; o++)
-    if (s.oct_nord[o] > 0) maxl = std::max(maxl, p.oct[o].nlevels);
-  for (int L = 1; L <= maxl; L++) {
-    LevelDesc d;
-    memset(&d, 0, sizeof d);
-    d.L = L;
-    int items = 0, ns = 0;
-    for (int o = 4 * hemi; o < 4 * hemi + 4; o++) {
-      const OctantPlan& op = p.oct[o];
-      if (s.oct_nord[o] == 0 || L > op.nlevels) continue;
-      const int beg = op.lvl_off[L - 1], end = op.lvl_off[L];
-      if (end <= beg) continue;
-      d.item_off[ns] = items;
-      d.pts_off[ns] = s.oct_pts_base_h[o] + beg;
-      d.nord[ns] = s.oct_nord[o];
-      d.col_base[ns] = s.oct_col_base[o];
-      d.oct[ns] = o;
-      items += (end - beg) * s.oct_nord[o];
-      ns++;
-    }
-    d.item_off[ns] = items;
-    d.nslot = ns;
-    if (ns > 0) out.push_back(d);
-  }
+std::mutex& device_mutex() {
+  static std::mutex m;
+  return m;
 }
 
 int state_create(const Plan& p, State& s, std::string& err) {
+  std::lock_guard<std::mutex> lock(device_mutex());
   s.plan = &p;
+  HIPCHK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
   s.npts = p.npts;
   s.nlm = p.nlm;
   s.nang = p.nang;
@@ -144,7 +135,6 @@ int state_create(const Plan& p, State& s, std::string& err) {
       err = "internal: downwelling ordinates are not first";
       return AT3D_ERR_ARG;
     }
-  HIPCHK(hipStreamCreate(&s.stream));
   // device column of each ordinate: grouped by octant, reference order inside
   s.col_of_ref.assign(p.nang, 0);
   s.ref_of_col.assign(p.nang, 0);
@@ -227,21 +217,29 @@ int state_create(const Plan& p, State& s, std::string& err) {
   for (int i : p.bctop) preset[i] |= 1;
   for (int i : p.bcbot) preset[i] |= 2;
   TRY(upload(s, s.preset, preset.data(), preset.size(), err));
-  std::vector<int32_t> rank, level, lp, lc;
+  std::vector<int32_t> rank, level, lp, lc, lo;
+  s.nlevels_max = 0;
   for (int o = 0; o < 8; o++) {
     const OctantPlan& op = p.oct[o];
     rank.insert(rank.end(), op.rank.begin(), op.rank.end());
     level.insert(level.end(), op.level.begin(), op.level.end());
-    s.oct_pts_base_h[o] = (int)lp.size();
+    const int base = (int)lp.size();
+    s.oct_pts_base_h[o] = base;
     lp.insert(lp.end(), op.lvl_pts.begin(), op.lvl_pts.end());
     lc.insert(lc.end(), op.lvl_cell.begin(), op.lvl_cell.end());
+    s.lo_base_h[o] = (int)lo.size();
+    for (int L = 0; L <= op.nlevels; L++) lo.push_back(base + op.lvl_off[L]);
+    if (s.oct_nord[o] > 0) s.nlevels_max = std::max(s.nlevels_max, op.nlevels);
   }
   TRY(upload(s, s.rank, rank.data(), rank.size(), err));
   TRY(upload(s, s.level, level.data(), level.size(), err));
   TRY(upload(s, s.lvl_pts, lp.data(), lp.size(), err));
   TRY(upload(s, s.lvl_cell, lc.data(), lc.size(), err));
-  build_level_descs(p, s, 0, s.levels_down);
-  build_level_descs(p, s, 1, s.levels_up);
+  TRY(upload(s, s.lvl_off, lo.data(), lo.size(), err));
+  std::vector<int8_t> coct(s.ldo, 0);
+  for (int o = 0; o < 8; o++)
+    for (int c = s.oct_col_base[o]; c < s.oct_col_base[o] + s.oct_nord[o]; c++) coct[c] = (int8_t)o;
+  TRY(upload(s, s.col_oct, coct.data(), coct.size(), err));
   // rendering constants
   TRY(upload(s, s.tms, p.tms.data(), p.tms.size(), err));
   TRY(upload(s, s.ss_off, p.ss_off.data(), p.ss_off.size(), err));
@@ -260,6 +258,7 @@ int state_create(const Plan& p, State& s, std::string& err) {
 }
 
 void state_free(State& s) {
+  std::lock_guard<std::mutex> lock(device_mutex());
   if (s.sweep_exec) (void)hipGraphExecDestroy(s.sweep_exec);
   if (s.sweep_graph) (void)hipGraphDestroy(s.sweep_graph);
   s.sweep_exec = nullptr;
@@ -269,6 +268,9 @@ void state_free(State& s) {
   if (s.rbuf) (void)hipFree(s.rbuf);
   s.rbuf = nullptr;
   s.rbuf_bytes = 0;
+  if (s.gbuf) (void)hipFree(s.gbuf);
+  s.gbuf = nullptr;
+  s.gbuf_bytes = 0;
   for (int e = 0; e < State::NEV; e++)
     if (s.ev[e]) (void)hipEventDestroy(s.ev[e]);
   if (s.stream) (void)hipStreamDestroy(s.stream);
@@ -319,6 +321,29 @@ int state_init_solution(State& s, std::string& err) {
   return AT3D_OK;
 }
 
+int state_init_solution_from(State& s, State& src, std::string& err) {
+  const Plan& p = *s.plan;
+  const Plan& q = *src.plan;
+  if (!src.initialized || p.npts != q.npts || p.nlm != q.nlm || s.ld != src.ld) {
+    err = "warm start needs an initialised source state on the same grid and angular resolution";
+    return AT3D_ERR_ARG;
+  }
+  const size_t nsh = (size_t)s.nptsp * s.ld;
+  HIPCHK(hipStreamSynchronize(src.stream));
+  HIPCHK(hipMemcpyAsync(s.R, src.R, nsh * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
+  HIPCHK(hipMemcpyAsync(s.nr, src.nr, p.npts * sizeof(int), hipMemcpyDeviceToDevice, s.stream));
+  std::vector<int32_t> zero(p.npts, 0);
+  HIPCHK(hipMemcpyAsync(s.ns, zero.data(), p.npts * sizeof(int), hipMemcpyHostToDevice, s.stream));
+  HIPCHK(hipMemcpyAsync(s.nos, zero.data(), p.npts * sizeof(int), hipMemcpyHostToDevice, s.stream));
+  launch_compute_source(s.stream, source_args(s, 1, 0), s.nsrc_blocks);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s.stream));
+  s.solcrit = 1.0f;
+  s.iters = 0;
+  s.initialized = true;
+  return AT3D_OK;
+}
+
 // The path-integration sweeps of one iteration: downwelling levels, surface
 // boundary, upwelling levels.
 static void enqueue_sweeps(State& s) {
@@ -326,11 +351,11 @@ static void enqueue_sweeps(State& s) {
   const SweepArgs a = sweep_args(s);
   const float albpi = p.d.gndalbedo / acosf(-1.0f);
   launch_sweep_preset(s.stream, a, false, 0, s.ndown);
-  for (const LevelDesc& d : s.levels_down) launch_sweep_level(s.stream, a, d);
-  launch_bottom_bc(s.stream, s.dorad, s.ldo, s.bcbot, (int)p.bcbot.size(), s.col_of_ref_d, s.fluxw,
+  launch_sweep_columns(s.stream, a, 0, s.ndown);
+  launch_bottom_bc(s.stream, s.dorad, s.nptsp, s.bcbot, (int)p.bcbot.size(), s.col_of_ref_d, s.fluxw,
                    s.ndown, s.dirflux, albpi, s.bcrad_bot);
   launch_sweep_preset(s.stream, a, true, s.ndown, p.nang - s.ndown);
-  for (const LevelDesc& d : s.levels_up) launch_sweep_level(s.stream, a, d);
+  launch_sweep_columns(s.stream, a, s.ndown, p.nang - s.ndown);
 }
 
 static int run_sweeps(State& s, std::string& err) {
@@ -344,6 +369,7 @@ static int run_sweeps(State& s, std::string& err) {
     if (s.sweep_graph) (void)hipGraphDestroy(s.sweep_graph);
     s.sweep_exec = nullptr;
     s.sweep_graph = nullptr;
+    std::lock_guard<std::mutex> lock(device_mutex());
     HIPCHK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
     enqueue_sweeps(s);
     hipError_t e1 = hipGetLastError();
@@ -398,11 +424,11 @@ int state_solve(State& s, int maxiter, int verbose, std::string& err) {
     launch_radiance_truncation(s.stream, s.R, s.ld, s.tc, s.ns, s.lofj, s.nr, p.npts, p.ml, p.mm,
                                p.nlm, radmin, fixsh ? 1 : 0, p.d.high_order_radiance ? 1 : 0);
     (void)hipEventRecord(s.ev[1], s.stream);
-    launch_sh_to_do(s.stream, s.S, s.Y, s.ext, s.dosrc, s.nptsp, s.ldo, s.ld, s.ldo, p.npts, p.nang);
+    launch_sh_to_do(s.stream, s.S, s.Y, s.ext, s.dosrc, s.nptsp, s.ldo, s.ld, s.nptsp, p.npts, p.nang);
     (void)hipEventRecord(s.ev[2], s.stream);
     TRY(run_sweeps(s, err));
     (void)hipEventRecord(s.ev[3], s.stream);
-    launch_do_to_sh(s.stream, s.dorad, s.W, s.nr, s.R, s.nptsp, s.ld, s.ldo, p.npts);
+    launch_do_to_sh(s.stream, s.dorad, s.W, s.nr, s.R, s.nptsp, s.ld, s.ldo, s.nptsp, p.npts);
     (void)hipEventRecord(s.ev[4], s.stream);
     if (s.solcrit < 0.001f || s.iters > p.d.iterfixsh) fixsh = true;
     launch_compute_source(s.stream, source_args(s, 0, fixsh ? 1 : 0), s.nsrc_blocks);
@@ -442,7 +468,7 @@ int state_solve(State& s, int maxiter, int verbose, std::string& err) {
       fflush(stdout);
     }
   }
-  launch_fluxes(s.stream, s.dorad, s.ldo, s.col_of_ref_d, s.fluxw, s.ndown, p.nang, p.npts, s.fluxes);
+  launch_fluxes(s.stream, s.dorad, s.nptsp, s.col_of_ref_d, s.fluxw, s.ndown, p.nang, p.npts, s.fluxes);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s.stream));
   return AT3D_OK;

@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "host_plan.h"
@@ -23,7 +25,8 @@ struct State {
   // ordinate columns: grouped by octant, reference order within an octant
   std::vector<int32_t> col_of_ref, ref_of_col;
   int oct_col_base[8] = {}, oct_nord[8] = {}, oct_pts_base_h[8] = {};
-  std::vector<LevelDesc> levels_down, levels_up;
+  int lo_base_h[8] = {};   // per-octant base into lvl_off
+  int nlevels_max = 0;
   // transforms
   float *Y = nullptr, *W = nullptr, *ysun = nullptr;
   int* lofj = nullptr;
@@ -32,7 +35,7 @@ struct State {
   // SH fields [nptsp][ld]
   float *S = nullptr, *R = nullptr, *DS = nullptr, *R0 = nullptr;
   int *ns = nullptr, *nr = nullptr, *nos = nullptr;
-  // DO fields [nptsp][ldo]
+  // DO fields [ldo][nptsp] (ordinate-major)
   float *dosrc = nullptr, *dorad = nullptr;
   float* fluxes = nullptr;
   double *partial = nullptr, *dots = nullptr;
@@ -48,6 +51,8 @@ struct State {
   int32_t *bctop = nullptr, *bcbot = nullptr;
   int8_t* preset = nullptr;
   int32_t *rank = nullptr, *level = nullptr, *lvl_pts = nullptr, *lvl_cell = nullptr;
+  int32_t* lvl_off = nullptr;
+  int8_t* col_oct = nullptr;
   // options
   int force_general = 0;  // 1: every point through the general characteristic trace
   int use_graph = 1;      // replay the sweep level launches from a HIP graph
@@ -68,16 +73,44 @@ struct State {
   // per-call render buffers (grown on demand, freed with the state)
   size_t rbuf_bytes = 0;
   void* rbuf = nullptr;
+  size_t gbuf_bytes = 0;   // per-call gradient buffers
+  void* gbuf = nullptr;
+  int max_subgrid = 0;     // extra sub-step intervals a gradient ray may record
+  double grad_ms = 0.0;
   double render_ms = 0.0;
 };
+
+struct DirTables {
+  int ndir = 0, ldd = 0;
+  std::vector<int32_t> dir_of_ray;                  // [nrays]
+  std::vector<std::pair<double, double>> dirs;      // (mu, phi) of each direction
+  std::vector<float> Yd, G, ssd;                    // [ldd][ld], [ndir][ml+1], [ndir][numphase]
+  std::vector<int32_t> sa_j;                        // scattering-angle table index (1-based lower)
+  std::vector<float> sa_f;                          // and interpolation fraction
+};
+int direction_tables(const State& s, int64_t nrays, const double* mu, const double* phi, DirTables& t,
+                     std::string& err);
+
+int state_levis_gradient(State& s, const at3d_deriv_desc& dd, int64_t nrays, const float* x,
+                         const float* y, const float* z, const double* mu, const double* phi,
+                         int64_t npix, const int32_t* rays_per_pixel, const double* ray_weights,
+                         const double* stokes_weights, const float* measurements,
+                         const double* uncertainties, const char* costfunc, int ess, double* grad,
+                         double* cost, float* pixels, std::string& err);
 
 int state_render(State& s, int64_t nrays, const float* x, const float* y, const float* z,
                  const double* mu, const double* phi, int single_scatter, int nosurface,
                  float* out, std::string& err);
 
+// Serialises device allocation, release and HIP-graph capture across host
+// threads: a capture in progress on one state's stream must not overlap
+// hipMalloc / hipFree (which can synchronise the device) on another thread.
+std::mutex& device_mutex();
+
 int state_create(const Plan& p, State& s, std::string& err);
 void state_free(State& s);
 int state_init_solution(State& s, std::string& err);
+int state_init_solution_from(State& s, State& src, std::string& err);
 int state_solve(State& s, int maxiter, int verbose, std::string& err);
 
 }  // namespace at3d

@@ -5,18 +5,30 @@
 // cell face whose four corner radiances are already known.  The host planner
 // replays that order and assigns every point a dependency level per octant
 // (all upwind face points of every ordinate of the octant sit on lower
-// levels).  On the GPU one launch integrates one level for every ordinate of a
-// hemisphere: a thread owns one (point, ordinate) pair, ordinates of a point are
-// adjacent lanes, and the DO arrays are stored [point][ordinate] so a point's
-// upwind values for all ordinates of an octant share cache lines.  The whole
-// level sequence is replayed from a HIP graph.
+// levels).  On the GPU one workgroup integrates one ordinate over the whole
+// grid: it walks the levels of the ordinate's octant in order, a thread per
+// point of the level, with a workgroup barrier between levels, so a
+// hemisphere's sweep is one launch whose workgroups run independently.  The
+// DO arrays are stored ordinate-major ([ordinate][point]) so a workgroup's
+// gathers and stores stay inside one contiguous row.
 #include <hip/hip_runtime.h>
+#include <limits.h>
 
 #include "kernels.h"
 
 namespace at3d {
 
-__device__ inline float sweep_simple(const SweepArgs& a, const OrdGeom& og, int col, int pt) {
+// The part of a one-cell characteristic that does not depend on radiances
+// computed during the sweep: the upwind face points and bilinear weights, and
+// the cell integral of the source (radv = src*abs, transmission).
+struct SimpleStep {
+  int i[4];
+  double f[4];
+  double radv, transmit;
+};
+
+__device__ inline void sweep_simple_static(const SweepArgs& a, const OrdGeom& og, int col, int pt,
+                                           SimpleStep& st) {
 #pragma clang fp contract(off)
   const float* xg = a.xgrid;
   const float* yg = a.ygrid;
@@ -57,19 +69,36 @@ __device__ inline float sweep_simple(const SweepArgs& a, const OrdGeom& og, int 
     v = (xe - (double)xg[x0]) / ((double)xg[x1] - (double)xg[x0]);
   }
   const float* ext = a.ext;
-  const float* se = a.srcext + col;
-  const float* rad = a.rad + col;
-  const size_t L = a.ldo;
+  const float* se = a.srcext + (size_t)col * a.ldp;
   const double f1 = (1 - u) * (1 - v), f2 = (1 - u) * v, f3 = u * (1 - v), f4 = u * v;
   const double ext0 = f1 * ext[i1] + f2 * ext[i2] + f3 * ext[i3] + f4 * ext[i4];
-  const double srcext0 = f1 * se[i1 * L] + f2 * se[i2 * L] + f3 * se[i3 * L] + f4 * se[i4 * L];
+  const double srcext0 = f1 * se[i1] + f2 * se[i2] + f3 * se[i3] + f4 * se[i4];
   double src, absc, trc;
-  cell_integral(ext0, (double)ext[pt], srcext0, (double)se[pt * L], so, src, absc, trc);
-  double radv = 0.0 + 1.0 * src * absc;
-  const double transmit = 1.0 * trc;
-  const double rad0 = f1 * rad[i1 * L] + f2 * rad[i2 * L] + f3 * rad[i3 * L] + f4 * rad[i4 * L];
-  radv = radv + transmit * rad0;
-  return (float)radv;
+  cell_integral(ext0, (double)ext[pt], srcext0, (double)se[pt], so, src, absc, trc);
+  st.i[0] = i1;
+  st.i[1] = i2;
+  st.i[2] = i3;
+  st.i[3] = i4;
+  st.f[0] = f1;
+  st.f[1] = f2;
+  st.f[2] = f3;
+  st.f[3] = f4;
+  st.radv = 0.0 + 1.0 * src * absc;
+  st.transmit = 1.0 * trc;
+}
+
+// Adds the transmitted upwind radiance (interpolated on the face) to a static step.
+__device__ inline float sweep_simple_finish(const SimpleStep& st, const float* rad) {
+#pragma clang fp contract(off)
+  const double rad0 = st.f[0] * rad[st.i[0]] + st.f[1] * rad[st.i[1]] + st.f[2] * rad[st.i[2]] +
+                      st.f[3] * rad[st.i[3]];
+  return (float)(st.radv + st.transmit * rad0);
+}
+
+__device__ inline float sweep_simple(const SweepArgs& a, const OrdGeom& og, int col, int pt) {
+  SimpleStep st;
+  sweep_simple_static(a, og, col, pt, st);
+  return sweep_simple_finish(st, a.rad + (size_t)col * a.ldp);
 }
 
 // General characteristic: trace through cells until a face with known radiances.
@@ -80,11 +109,10 @@ __device__ inline float sweep_general(const SweepArgs& a, const OrdGeom& og, int
   const int32_t* level = a.level + (size_t)o * a.npts;
   const int rp = rank[pt];
   const float* ext = a.ext;
-  const float* se = a.srcext + col;
-  const float* rad = a.rad + col;
-  const size_t LD = a.ldo;
+  const float* se = a.srcext + (size_t)col * a.ldp;
+  const float* rad = a.rad + (size_t)col * a.ldp;
   double xe = a.g.gridpos[pt * 3], ye = a.g.gridpos[pt * 3 + 1], ze = a.g.gridpos[pt * 3 + 2];
-  double ext1 = ext[pt], srcext1 = se[pt * LD];
+  double ext1 = ext[pt], srcext1 = se[pt];
   double radv = 0.0, transmit = 1.0;
   for (int step = 0; step <= a.g.ncells; step++) {
     TraceStep t;
@@ -95,8 +123,7 @@ __device__ inline float sweep_general(const SweepArgs& a, const OrdGeom& og, int
     const double f1 = (1 - t.u) * (1 - t.v), f2 = (1 - t.u) * t.v;
     const double f3 = t.u * (1 - t.v), f4 = t.u * t.v;
     const double ext0 = f1 * ext[t.i[0]] + f2 * ext[t.i[1]] + f3 * ext[t.i[2]] + f4 * ext[t.i[3]];
-    const double srcext0 = f1 * se[t.i[0] * LD] + f2 * se[t.i[1] * LD] + f3 * se[t.i[2] * LD] +
-                           f4 * se[t.i[3] * LD];
+    const double srcext0 = f1 * se[t.i[0]] + f2 * se[t.i[1]] + f3 * se[t.i[2]] + f4 * se[t.i[3]];
     double src, absc, trc;
     cell_integral(ext0, ext1, srcext0, srcext1, t.so, src, absc, trc);
     radv = radv + transmit * src * absc;
@@ -115,8 +142,7 @@ __device__ inline float sweep_general(const SweepArgs& a, const OrdGeom& og, int
         const int q = t.i[c];
         if (!(a.preset[q] & presetbit) && level[q] >= L) atomicOr(a.errflag, 4);
       }
-      const double rad0 = f1 * rad[t.i[0] * LD] + f2 * rad[t.i[1] * LD] + f3 * rad[t.i[2] * LD] +
-                          f4 * rad[t.i[3] * LD];
+      const double rad0 = f1 * rad[t.i[0]] + f2 * rad[t.i[1]] + f3 * rad[t.i[2]] + f4 * rad[t.i[3]];
       radv = radv + transmit * rad0;
       return (float)radv;
     }
@@ -128,27 +154,61 @@ __device__ inline float sweep_general(const SweepArgs& a, const OrdGeom& og, int
   return 0.0f;
 }
 
-__global__ __launch_bounds__(256) void k_sweep_level(SweepArgs a, LevelDesc d) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.item_off[d.nslot]) return;
-  int s = 0;
-  while (s + 1 < d.nslot && i >= d.item_off[s + 1]) s++;
-  const int rel = i - d.item_off[s];
-  const int j = rel / d.nord[s];
-  const int r = rel - j * d.nord[s];
-  const int col = d.col_base[s] + r;
-  const int code = a.lvl_pts[d.pts_off[s] + j];
+constexpr int SWEEP_THREADS = 512;
+constexpr int SWEEP_K = 4;   // items per thread whose static part is prepared ahead
+
+// One ordinate (workgroup) through every level of its octant.  The static
+// part of each one-cell step of level L+1 (face points, weights, source
+// integral) is prepared while level L is finishing; after the barrier only the
+// upwind radiances are gathered.  Multi-cell traces and levels larger than
+// SWEEP_K * SWEEP_THREADS points take the unprepared path.
+__global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_columns(SweepArgs a, int col0) {
+  const int col = col0 + blockIdx.x;
+  const int o = a.col_oct[col];
   const OrdGeom og = a.ordg[col];
-  float v;
-  int pt;
-  if (code >= 0 && !a.force_general) {
-    pt = code;
-    v = sweep_simple(a, og, col, pt);
-  } else {
-    pt = code >= 0 ? code : ~code;
-    v = sweep_general(a, og, col, d.oct[s], pt, a.lvl_cell[d.pts_off[s] + j], d.L);
+  const int nlev = a.nlev[o];
+  const int32_t* off = a.lvl_off + a.lo_base[o];
+  float* rad = a.rad + (size_t)col * a.ldp;
+  const int tid = threadIdx.x;
+  const bool general = a.force_general != 0;
+  SimpleStep st[SWEEP_K];
+  int code[SWEEP_K];
+  auto prepare = [&](int L) {
+    const int beg = off[L - 1], end = off[L];
+#pragma unroll
+    for (int k = 0; k < SWEEP_K; k++) {
+      const int j = beg + tid + k * SWEEP_THREADS;
+      code[k] = (j < end) ? a.lvl_pts[j] : INT_MIN;
+      if (code[k] >= 0 && !general) sweep_simple_static(a, og, col, code[k], st[k]);
+    }
+  };
+  if (nlev >= 1) prepare(1);
+  for (int L = 1; L <= nlev; L++) {
+    __syncthreads();
+    const int beg = off[L - 1], end = off[L];
+#pragma unroll
+    for (int k = 0; k < SWEEP_K; k++) {
+      const int c = code[k];
+      if (c == INT_MIN) continue;
+      if (c >= 0 && !general) {
+        rad[c] = sweep_simple_finish(st[k], rad);
+      } else {
+        const int j = beg + tid + k * SWEEP_THREADS;
+        const int pt = c >= 0 ? c : ~c;
+        rad[pt] = sweep_general(a, og, col, o, pt, a.lvl_cell[j], L);
+      }
+    }
+    for (int j = beg + tid + SWEEP_K * SWEEP_THREADS; j < end; j += SWEEP_THREADS) {
+      const int c = a.lvl_pts[j];
+      if (c >= 0 && !general) {
+        rad[c] = sweep_simple(a, og, col, c);
+      } else {
+        const int pt = c >= 0 ? c : ~c;
+        rad[pt] = sweep_general(a, og, col, o, pt, a.lvl_cell[j], L);
+      }
+    }
+    if (L < nlev) prepare(L + 1);
   }
-  a.rad[(size_t)pt * a.ldo + col] = v;
 }
 
 // Boundary radiances for one hemisphere: sky radiance at the top points for
@@ -160,14 +220,14 @@ __global__ void k_sweep_preset(SweepArgs a, int up, int col0, int ncol) {
   if (i >= np * ncol) return;
   const int b = i / ncol, c = col0 + (i - b * ncol);
   if (up)
-    a.rad[(size_t)a.bcbot[b] * a.ldo + c] = a.bcrad_bot[b];
+    a.rad[(size_t)c * a.ldp + a.bcbot[b]] = a.bcrad_bot[b];
   else
-    a.rad[(size_t)a.bctop[b] * a.ldo + c] = a.skyrad[c];
+    a.rad[(size_t)c * a.ldp + a.bctop[b]] = a.skyrad[c];
 }
 
 // FIXED_LAMBERTIAN_BOUNDARY (shdomsub1.f:2438-2480) with the downwelling
 // hemispheric flux summed in the reference's ordinate order (:2118-2135).
-__global__ void k_bottom_bc(const float* __restrict__ rad, int ldo, const int32_t* __restrict__ bcbot,
+__global__ void k_bottom_bc(const float* __restrict__ rad, int ldp, const int32_t* __restrict__ bcbot,
                             int nbot, const int32_t* __restrict__ col_of_ref,
                             const float* __restrict__ fluxw, int ndown,
                             const float* __restrict__ dirflux, float albpi,
@@ -175,21 +235,19 @@ __global__ void k_bottom_bc(const float* __restrict__ rad, int ldo, const int32_
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nbot) return;
   const int p = bcbot[b];
-  const float* row = rad + (size_t)p * ldo;
   float f = 0.f;
-  for (int k = 0; k < ndown; k++) f = f + fluxw[k] * row[col_of_ref[k]];
+  for (int k = 0; k < ndown; k++) f = f + fluxw[k] * rad[(size_t)col_of_ref[k] * ldp + p];
   bcrad[b] = albpi * (dirflux[p] + f);
 }
 
-__global__ void k_fluxes(const float* __restrict__ rad, int ldo, const int32_t* __restrict__ col_of_ref,
+__global__ void k_fluxes(const float* __restrict__ rad, int ldp, const int32_t* __restrict__ col_of_ref,
                          const float* __restrict__ fluxw, int ndown, int nang, int npts,
                          float* __restrict__ fluxes) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npts) return;
-  const float* row = rad + (size_t)p * ldo;
   float dn = 0.f, upf = 0.f;
-  for (int k = 0; k < ndown; k++) dn = dn + fluxw[k] * row[col_of_ref[k]];
-  for (int k = ndown; k < nang; k++) upf = upf + fluxw[k] * row[col_of_ref[k]];
+  for (int k = 0; k < ndown; k++) dn = dn + fluxw[k] * rad[(size_t)col_of_ref[k] * ldp + p];
+  for (int k = ndown; k < nang; k++) upf = upf + fluxw[k] * rad[(size_t)col_of_ref[k] * ldp + p];
   fluxes[p] = dn;
   fluxes[npts + p] = upf;
 }
@@ -200,23 +258,22 @@ void launch_sweep_preset(hipStream_t st, const SweepArgs& a, bool up, int col0, 
   k_sweep_preset<<<(n + 255) / 256, 256, 0, st>>>(a, up ? 1 : 0, col0, ncol);
 }
 
-void launch_sweep_level(hipStream_t st, const SweepArgs& a, const LevelDesc& d) {
-  const int n = d.item_off[d.nslot];
-  if (n <= 0) return;
-  k_sweep_level<<<(n + 255) / 256, 256, 0, st>>>(a, d);
+void launch_sweep_columns(hipStream_t st, const SweepArgs& a, int col0, int ncol) {
+  if (ncol <= 0) return;
+  k_sweep_columns<<<ncol, SWEEP_THREADS, 0, st>>>(a, col0);
 }
 
-void launch_bottom_bc(hipStream_t st, const float* rad, int ldo, const int32_t* bcbot, int nbot,
+void launch_bottom_bc(hipStream_t st, const float* rad, int ldp, const int32_t* bcbot, int nbot,
                       const int32_t* col_of_ref, const float* fluxw, int ndown,
                       const float* dirflux, float albpi, float* bcrad) {
   if (nbot <= 0) return;
-  k_bottom_bc<<<(nbot + 63) / 64, 64, 0, st>>>(rad, ldo, bcbot, nbot, col_of_ref, fluxw, ndown,
+  k_bottom_bc<<<(nbot + 63) / 64, 64, 0, st>>>(rad, ldp, bcbot, nbot, col_of_ref, fluxw, ndown,
                                                 dirflux, albpi, bcrad);
 }
 
-void launch_fluxes(hipStream_t st, const float* rad, int ldo, const int32_t* col_of_ref,
+void launch_fluxes(hipStream_t st, const float* rad, int ldp, const int32_t* col_of_ref,
                    const float* fluxw, int ndown, int nang, int npts, float* fluxes) {
-  k_fluxes<<<(npts + 63) / 64, 64, 0, st>>>(rad, ldo, col_of_ref, fluxw, ndown, nang, npts, fluxes);
+  k_fluxes<<<(npts + 63) / 64, 64, 0, st>>>(rad, ldp, col_of_ref, fluxw, ndown, nang, npts, fluxes);
 }
 
 }  // namespace at3d

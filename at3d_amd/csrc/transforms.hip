@@ -41,7 +41,9 @@ __device__ inline void mfma_tile(const float (*sA)[TM + PAD], const float (*sB)[
 }
 
 // C[m][n] = sum_k A[m][k] B[n][k]: both operands K-contiguous.  CLAMP_EXT
-// selects the SH->DO epilogue max(0, C) * ext[m]; without it C is stored as is.
+// selects the SH->DO epilogue max(0, C) * ext[m] stored ordinate-major,
+// out[n * ldo + m] (ldo = padded point count), four consecutive points per
+// lane as one 16-byte store; without it C is stored as is, out[m * ldo + n].
 template <bool CLAMP_EXT>
 __global__ __launch_bounds__(256) void k_sh_to_do(const float* __restrict__ S,
                                                   const float* __restrict__ Yc,
@@ -72,24 +74,41 @@ __global__ __launch_bounds__(256) void k_sh_to_do(const float* __restrict__ S,
     __syncthreads();
     mfma_tile(sA, sB, wm, wn, lane, acc);
   }
-  for (int i = 0; i < 2; i++)
-    for (int v = 0; v < 4; v++) {
-      const int p = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + v;
-      if (p >= npts) continue;
-      const float e = CLAMP_EXT ? ext[p] : 1.0f;
+  if (CLAMP_EXT) {
+    // rows beyond npts hold zero sources and zero extinction: stored as zeros
+    for (int i = 0; i < 2; i++) {
+      const int p0 = m0 + wm * 32 + i * 16 + 4 * (lane >> 4);
+      const float4 e = *reinterpret_cast<const float4*>(ext + p0);
       for (int j = 0; j < 2; j++) {
         const int c = n0 + wn * 32 + j * 16 + (lane & 15);
-        if (c < nang)
-          out[(size_t)p * ldo + c] = CLAMP_EXT ? fmaxf(acc[i][j][v], 0.0f) * e : acc[i][j][v];
+        if (c >= nang) continue;
+        float4 o;
+        o.x = fmaxf(acc[i][j][0], 0.0f) * e.x;
+        o.y = fmaxf(acc[i][j][1], 0.0f) * e.y;
+        o.z = fmaxf(acc[i][j][2], 0.0f) * e.z;
+        o.w = fmaxf(acc[i][j][3], 0.0f) * e.w;
+        *reinterpret_cast<float4*>(out + (size_t)c * ldo + p0) = o;
       }
     }
+  } else {
+    for (int i = 0; i < 2; i++)
+      for (int v = 0; v < 4; v++) {
+        const int p = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + v;
+        if (p >= npts) continue;
+        for (int j = 0; j < 2; j++) {
+          const int c = n0 + wn * 32 + j * 16 + (lane & 15);
+          if (c < nang) out[(size_t)p * ldo + c] = acc[i][j][v];
+        }
+      }
+  }
 }
 
-// C[m][n] = sum_k A[m][k] B[k][n]: A K-contiguous rows, B N-contiguous rows.
+// C[m][n] = sum_k A[m][k] B[k][n]: A stored ordinate-major (A[m][k] at
+// D[k * ldp + m]), B N-contiguous rows.
 __global__ __launch_bounds__(256) void k_do_to_sh(const float* __restrict__ D,
                                                   const float* __restrict__ Wc,
                                                   const int* __restrict__ nr,
-                                                  float* __restrict__ R, int ld, int ldo,
+                                                  float* __restrict__ R, int ld, int ldo, int ldp,
                                                   int npts) {
   __shared__ float sA[TK][TM + PAD];
   __shared__ float sB[TK][TN + PAD];
@@ -99,16 +118,12 @@ __global__ __launch_bounds__(256) void k_do_to_sh(const float* __restrict__ D,
   float4_ acc[2][2];
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++) acc[i][j] = (float4_){0.f, 0.f, 0.f, 0.f};
-  const int lr = tid >> 2, lc = (tid & 3) * 4;     // A: 64 rows x 16 k
-  const int bk = tid >> 4, bc = (tid & 15) * 4;    // B: 16 k x 64 cols
+  const int bk = tid >> 4, bc = (tid & 15) * 4;    // 16 k x 64 rows/cols per tile
   for (int k0 = 0; k0 < ldo; k0 += TK) {
-    const float4 a = *reinterpret_cast<const float4*>(D + (size_t)(m0 + lr) * ldo + k0 + lc);
+    const float4 a = *reinterpret_cast<const float4*>(D + (size_t)(k0 + bk) * ldp + m0 + bc);
     const float4 b = *reinterpret_cast<const float4*>(Wc + (size_t)(k0 + bk) * ld + n0 + bc);
     __syncthreads();
-    sA[lc + 0][lr] = a.x;
-    sA[lc + 1][lr] = a.y;
-    sA[lc + 2][lr] = a.z;
-    sA[lc + 3][lr] = a.w;
+    *reinterpret_cast<float4*>(&sA[bk][bc]) = a;
     *reinterpret_cast<float4*>(&sB[bk][bc]) = b;
     __syncthreads();
     mfma_tile(sA, sB, wm, wn, lane, acc);
@@ -126,9 +141,9 @@ __global__ __launch_bounds__(256) void k_do_to_sh(const float* __restrict__ D,
 }
 
 void launch_sh_to_do(hipStream_t st, const float* S, const float* Yc, const float* ext,
-                     float* dosrc, int mp, int ncp, int ld, int ldo, int npts, int nang) {
+                     float* dosrc, int mp, int ncp, int ld, int ldp, int npts, int nang) {
   dim3 grid(ncp / TN, mp / TM);
-  k_sh_to_do<true><<<grid, 256, 0, st>>>(S, Yc, ext, dosrc, ld, ldo, npts, nang);
+  k_sh_to_do<true><<<grid, 256, 0, st>>>(S, Yc, ext, dosrc, ld, ldp, npts, nang);
 }
 
 void launch_sh_dot(hipStream_t st, const float* S, const float* Yd, float* raw, int mp, int ndp,
@@ -138,9 +153,9 @@ void launch_sh_dot(hipStream_t st, const float* S, const float* Yd, float* raw, 
 }
 
 void launch_do_to_sh(hipStream_t st, const float* dorad, const float* Wc, const int* nr, float* R,
-                     int mp, int ld, int ldo, int npts) {
+                     int mp, int ld, int ldo, int ldp, int npts) {
   dim3 grid(mp / TM, ld / TN);
-  k_do_to_sh<<<grid, 256, 0, st>>>(dorad, Wc, nr, R, ld, ldo, npts);
+  k_do_to_sh<<<grid, 256, 0, st>>>(dorad, Wc, nr, R, ld, ldo, ldp, npts);
 }
 
 }  // namespace at3d

@@ -193,6 +193,18 @@ class RTE:
             warnings.warn("solution did not converge to solution_accuracy={} in maxiter={} "
                           "(SOLCRIT={})".format(self.solution_accuracy, maxiter, self._solcrit))
 
+    def init_solution_from(self, other):
+        """Warm start from another RTE's current solution on the same grid and
+        angular resolution: its spherical-harmonic radiance becomes this
+        solver's initial radiance (instead of the two-stream start), and the
+        next ``solve(..., init_solution=False)`` iterates from there."""
+        if not isinstance(other, RTE) or other._state is None:
+            raise ValueError("`other` must be an RTE holding a solution")
+        self._ensure_state()
+        err = _lib.errbuf()
+        _lib.check(_lib.lib().at3d_init_solution_from(self._state, other._state, err, len(err)), err)
+        self._solcrit, self._iters = 1.0, 0
+
     def integrate_to_sensor(self, sensor, single_scatter=False, nosurface=False):
         """Radiance of every ray of ``sensor`` (solver.py:633-791).
 

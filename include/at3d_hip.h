@@ -110,6 +110,14 @@ void at3d_state_destroy(at3d_state* st);
  * (INIT_SOLUTION, shdomsub1.f:113-442). */
 int at3d_init_solution(at3d_state* st, char* err, size_t errlen);
 
+/* Warm start: initialise the solution from the spherical-harmonic radiance of
+ * another state on the same grid and angular resolution (typically the
+ * converged solution of a slightly different medium), then compute the first
+ * source function of this state's medium from it.  It stands where the
+ * reference continues from a previous or loaded solution (RTE.solve with
+ * init_solution=False, RTE.load_solution, at3d/solver.py:1621-1686). */
+int at3d_init_solution_from(at3d_state* st, at3d_state* src, char* err, size_t errlen);
+
 /* Source-function iterations until SOLCRIT <= solacc or `maxiter` total
  * iterations (SOLUTION_ITERATIONS, shdomsub1.f:445-818).  Continues from the
  * current state; writes the final criterion and total iteration count. */
@@ -133,6 +141,44 @@ int at3d_state_get(at3d_state* st, const char* name, void* dst, size_t nbytes, c
 int at3d_render(at3d_state* st, int64_t nrays, const float* x, const float* y, const float* z,
                 const double* mu, const double* phi, int32_t single_scatter, int32_t nosurface,
                 float* out, char* err, size_t errlen);
+
+/* Unknowns of a gradient: NUMDER partial derivatives of the optical properties
+ * of one species each, on the property grid (what
+ * RTE.calculate_microphysical_partial_derivatives, at3d/solver.py:1327-1516,
+ * hands to PREPARE_DERIV_INTERPS, shdomsub4.f:2917-3079).  One phase weight per
+ * point (num_micro = 1). */
+typedef struct at3d_deriv_desc {
+  int32_t numder;
+  const int32_t* partder;   /* [numder] 1-based species index                     */
+  const float* dext;        /* [numder][npx][npy][npz] d extinction / d unknown   */
+  const float* dalb;        /* [numder][npx][npy][npz] d albedo / d unknown       */
+  const int32_t* doexact;   /* [numder] 1: phase derivative from the dleg table,
+                               0: from dphasewtp                                   */
+  int32_t dnumphase;        /* rows of dleg (>= 1)                                */
+  const float* dleg;        /* [dnumphase][nlegp+1] d Legendre coefficients incl. (2l+1) */
+  const int32_t* diphasep;  /* [numder][npx][npy][npz] 1-based row of dleg        */
+  const float* dphasewtp;   /* [numder][npx][npy][npz] d phase weight             */
+} at3d_deriv_desc;
+
+/* Cost and gradient by the Levis approximation (LEVISAPPROX_GRADIENT adjoint
+ * branch, shdomsub4.f:288-809: forward radiances by INTEGRATE_1RAY, adjoint
+ * weights by COMPUTE_ADJOINT_WEIGHTS :3969-4034, ADJOINT_INTEGRATE_1RAY
+ * :3223-3967 with COMPUTE_SOURCE_GRAD_1CELL :1546-2042 and
+ * COMPUTE_RADIANCE_DERIVATIVE_ADJOINT :4037-4114, direct-beam term
+ * COMPUTE_DIRECT_BEAM_DERIV_ADJOINT :4117-4143 over the sun paths of
+ * MAKE_DIRECT_DERIVATIVE, shdomsub5.f:1553-2003).
+ * Rays as for at3d_render, grouped by pixel: rays_per_pixel[npix] (sum = nrays),
+ * ray_weights[nrays], stokes_weights[npix], measurements[npix],
+ * uncertainties[npix] (the 1x1 inverse error covariance of each pixel);
+ * costfunc "L2" or "LL".  Outputs: grad[numder][npx][npy][npz] (float64),
+ * *cost, pixels[npix] (the forward radiances). */
+int at3d_levis_gradient(at3d_state* st, const at3d_deriv_desc* deriv, int64_t nrays,
+                        const float* x, const float* y, const float* z, const double* mu,
+                        const double* phi, int64_t npix, const int32_t* rays_per_pixel,
+                        const double* ray_weights, const double* stokes_weights,
+                        const float* measurements, const double* uncertainties,
+                        const char* costfunc, int32_t exact_single_scatter, double* grad,
+                        double* cost, float* pixels, char* err, size_t errlen);
 
 /* Engine options of a state.  "sweep_general" (0/1): route every grid point
  * through the general multi-cell characteristic trace instead of the analytic

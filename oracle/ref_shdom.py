@@ -416,7 +416,7 @@ def init_solution(st: RefState):
              deljnew=deljnew.value, jnorm=jnorm.value, ndel=ndel, work2_size=work2_size,
              volsrc=volsrc, volsrcgridptr=volsrcgridptr, volsrcshptr=volsrcshptr,
              maxsfcpars=maxsfcpars, sfcparms=sfcparms, oldnpts=0, solcrit=1.0, iters=0,
-             nang=nang)
+             nang=nang, longest_path_pts=max(longest.value, 1))
 
 
 def solution_iterations(st: RefState, maxiter: int, verbose=False):
@@ -544,3 +544,177 @@ def scene_from_workload(w, **kw) -> Scene:
                 gndalbedo=w.gndalbedo, wavelength=w.wavelength, deltam=w.deltam)
     args.update(kw)
     return Scene(**args)
+
+
+# ----------------------------------------------------------------------------
+# Levis approximate gradient (at3d/gradient.py:164-415 -> LEVISAPPROX_GRADIENT)
+# ----------------------------------------------------------------------------
+@dataclass
+class Derivative:
+    """One unknown: derivatives of one species' optical properties
+    (medium.py:1273-1355 'exact' method)."""
+    species: int                    # 0-based species index
+    dext: np.ndarray                # (npx, npy, npz)
+    dalb: np.ndarray                # (npx, npy, npz)
+    dleg: np.ndarray                # (nleg+1, ntable) derivative Legendre table incl. (2l+1)
+    table_index: np.ndarray         # (npx, npy, npz) 1-based into dleg
+
+
+def max_subgrid_intervals(st: RefState):
+    """solver.py:2481-2508 (3D branch) and gradient.py:252."""
+    d = st.d
+    nx1, ny1, nz = d["nx1"], d["ny1"], d["nz"]
+    ext = np.asarray(d["total_ext"][:d["nbpts"]]).reshape(nx1, ny1, nz)
+    cae = (ext[1:, 1:, 1:] + ext[1:, 1:, :-1] + ext[1:, :-1, 1:] + ext[1:, :-1, :-1] +
+           ext[:-1, 1:, 1:] + ext[:-1, 1:, :-1] + ext[:-1, :-1, 1:] + ext[:-1, :-1, :-1]) / 8.0
+    vol = (np.diff(d["zgrid"]) * np.diff(d["xgrid"])[0] * np.diff(d["ygrid"])[0]) ** (1 / 3)
+    tau = vol[np.newaxis, np.newaxis, :] * cae
+    msi = int(np.nanmean(tau) / st.sc.tautol)
+    return int(d["npts"] * msi / d["nbpts"])
+
+
+def prepare_derivatives(st: RefState, derivs):
+    """calculate_microphysical_partial_derivatives (solver.py:1327-1516) and
+    calculate_direct_beam_derivative (:1266-1325)."""
+    sc, d = st.sc, st.d
+    precompute_phase(st)
+    maxpg, numder = d["maxpg"], len(derivs)
+    npx, npy, npz = d["npx"], d["npy"], d["npz"]
+    dext = np.zeros((maxpg, numder), np.float32, order="F")
+    dalb = np.zeros((maxpg, numder), np.float32, order="F")
+    diphase = np.zeros((1, maxpg, numder), np.int32, order="F")
+    dphasewt = np.zeros((1, maxpg, numder), np.float32, order="F")
+    tables = []
+    offset = 0
+    for i, dv in enumerate(derivs):
+        dext[:, i] = np.asarray(dv.dext, np.float32).ravel()
+        dalb[:, i] = np.asarray(dv.dalb, np.float32).ravel()
+        pw = sc.species[dv.species].phase_weights
+        dphasewt[0, :, i] = np.asarray(np.ones((npx, npy, npz)) if pw is None else pw,
+                                       np.float32).ravel()
+        diphase[0, :, i] = np.asarray(dv.table_index, np.int32).ravel() + offset
+        offset += dv.dleg.shape[1]
+        tables.append(np.asarray(dv.dleg, np.float64))
+    diphase[diphase == 0] = 1
+    nlegp = d["nlegp"]
+    # tables padded with zeros to the solver's table length
+    table = np.concatenate([np.pad(t[:nlegp + 1], ((0, nlegp + 1 - min(t.shape[0], nlegp + 1)), (0, 0)))
+                            for t in tables], axis=1)
+    dnumphase = table.shape[1]
+    dleg = table.copy()
+    dleg[0, :] = 0.0
+    dleg = dleg / (2.0 * np.arange(nlegp + 1) + 1.0)[:, None]
+    dleg_f = farr(dleg.reshape(1, nlegp + 1, dnumphase), np.float32)
+    dphasetab = np.zeros((1, dnumphase, d["nscatangle"]), np.float32, order="F")
+    ierr, errmsg = i4(), s("", 600)
+    fcall("PRECOMPUTE_PHASE_CHECK_GRAD", i4(d["nscatangle"]), i4(dnumphase), i4(1), i4(1),
+          i4(d["ml"]), i4(d["nlm"]), i4(1), i4(nlegp), dleg_f, dphasetab, lg(sc.deltam),
+          lg(False), ierr, errmsg)
+    _check(ierr, errmsg, "PRECOMPUTE_PHASE_CHECK_GRAD")
+    dleg_rte = farr(dleg_f[:, :d["nleg"] + 1, :], np.float32)
+    npts = d["npts"]
+    partder = np.array([dv.species + 1 for dv in derivs], np.int32)
+    doexact = np.ones(numder, np.int32)
+    optinterpwt = np.zeros((8, npts), np.float32, order="F")
+    interpptr = np.zeros((8, npts), np.int32, order="F")
+    dalbm = np.zeros((8, npts, numder), np.float32, order="F")
+    dextm = np.zeros((maxpg, numder), np.float32, order="F")
+    dfj = np.zeros((8, npts, numder), np.float32, order="F")
+    ierr, errmsg = i4(), s("", 600)
+    fcall("PREPARE_DERIV_INTERPS", farr(d["gridpos"][:, :npts], np.float32), i4(npts),
+          i4(npx), i4(npy), i4(npz), i4(maxpg), r4(sc.delx), r4(sc.dely), r4(0.0), r4(0.0),
+          d["zlevels"], optinterpwt, interpptr, ierr, errmsg, d["legen"], i4(d["numphase"]),
+          i4(1), dleg_rte, i4(dnumphase), d["phasewtp"], dphasewt, d["iphasep"], diphase,
+          dalbm, dextm, dfj, i4(d["nleg"]), i4(d["maxnmicro"]), d["albedop"], d["extinctp"],
+          i4(d["npart"]), dalb, dext, i4(numder), partder, doexact, i4(d["ml"]),
+          lg(sc.deltam), farr(d["albedo"][:npts], np.float32),
+          farr(d["iphase"][:, :npts], np.int32), farr(d["phaseinterpwt"][:, :npts], np.float32),
+          r4(0.999), s("ON"), i4(1))
+    _check(ierr, errmsg, "PREPARE_DERIV_INTERPS")
+    longest = d["longest_path_pts"]
+    dpath = np.zeros((longest, npts), np.float32, order="F")
+    dptr = np.zeros((longest, npts), np.int32, order="F")
+    ierr, errmsg = i4(), s("", 600)
+    fcall("MAKE_DIRECT_DERIVATIVE", i4(npts), i4(d["bcflag"]), i4(npx), i4(npy), i4(npz),
+          r4(sc.delx), r4(sc.dely), r4(0.0), r4(0.0), d["gridpos"], d["zlevels"],
+          i4(d["ipdirect"]), i4(d["di"]), i4(d["dj"]), i4(d["dk"]), r8(d["cx"]), r8(d["cy"]),
+          r8(d["cz"]), r8(d["cxinv"]), r8(d["cyinv"]), r8(d["czinv"]), r8(d["epss"]),
+          r8(d["epsz"]), r8(d["xdomain"]), r8(d["ydomain"]), r8(d["uniformzlev"]),
+          r8(d["delxd"]), r8(d["delyd"]), dpath, dptr, ierr, errmsg, i4(longest))
+    _check(ierr, errmsg, "MAKE_DIRECT_DERIVATIVE")
+    d.update(numder=numder, dext=dext, dalb=dalb, diphase=diphase, dphasewt=dphasewt,
+             dnumphase=dnumphase, dleg=dleg_rte, dphasetab=dphasetab, partder=partder,
+             doexact=doexact, optinterpwt=optinterpwt, interpptr=interpptr, dalbm=dalbm,
+             dextm=dextm, dfj=dfj, dpath=dpath, dptr=dptr)
+
+
+def levis_gradient(st: RefState, x, y, z, mu, phi, measurements, uncertainties=None,
+                   rays_per_pixel=None, ray_weights=None, stokes_weights=None,
+                   costfunc="L2", exact_single_scatter=True):
+    """levis_approximation_grad (gradient.py:164-415) -> LEVISAPPROX_GRADIENT
+    (shdomsub4.f:288-809, adjoint branch).  Returns (gradient (maxpg, numder),
+    cost, pixel radiances)."""
+    sc, d = st.sc, st.d
+    nrays = len(x)
+    npix = nrays if rays_per_pixel is None else len(rays_per_pixel)
+    rpp = np.ones(npix, np.int32) if rays_per_pixel is None else np.asarray(rays_per_pixel, np.int32)
+    rw = np.ones(nrays) if ray_weights is None else np.asarray(ray_weights, np.float64)
+    sw = np.ones((1, npix)) if stokes_weights is None else np.asarray(stokes_weights, np.float64).reshape(1, npix)
+    unc = np.ones((1, 1, npix)) if uncertainties is None else np.asarray(uncertainties, np.float64).reshape(1, 1, npix)
+    meas = farr(np.asarray(measurements, np.float32).reshape(1, npix), np.float32)
+    npts, maxpg, numder = d["npts"], d["maxpg"], d["numder"]
+    gradout = np.zeros((maxpg, numder, 1), np.float64, order="F")
+    cost = np.zeros(1, np.float64)
+    stokesout = np.zeros((1, npix), np.float32, order="F")
+    jacobian = np.zeros((1, numder, 1, 1), np.float32, order="F")
+    nxsfc = 0
+    nsfcpts = (nxsfc + 2) * (nxsfc + 2)
+    sfcder = np.zeros((1, nsfcpts), np.int32, order="F")
+    bcrad = d["bcrad"].copy(order="F")
+    ierr, errmsg = i4(), s("", 600)
+    fcall("LEVISAPPROX_GRADIENT",
+          i4(1), i4(d["nx"]), i4(d["ny"]), i4(d["nz"]), i4(npts), i4(d["ncells"]), i4(maxpg),
+          i4(d["ml"]), i4(d["mm"]), i4(d["nlm"]), i4(1), i4(d["nleg"]), i4(d["numphase"]),
+          i4(d["nmu"]), i4(d["nphi0max"]), d["nphi0"], d["mu"], d["phi"], d["wtdo"],
+          i4(d["bcflag"]), i4(d["ipflag"]), s("S"), lg(sc.deltam), r4(sc.solarmu),
+          r4(sc.solaraz), s("FL"), i4(1), d["sfcgridparms"], i4(d["maxnbc"]), i4(d["ntoppts"]),
+          i4(d["nbotpts"]), d["bcptr"], bcrad, r4(sc.gndtemp), r4(sc.gndalbedo), d["skyrad"],
+          farr([10000, 10001], np.float32), r4(sc.wavelength), s("R"), d["xgrid"], d["ygrid"],
+          d["zgrid"], d["gridpos"], d["gridptr"], d["neighptr"], d["treeptr"], d["cellflags"],
+          farr(d["extinct"][:npts], np.float32), farr(d["albedo"][:npts], np.float32),
+          d["legen"], farr(d["iphase"][:, :npts], np.int32), d["dirflux"], d["fluxes"],
+          d["shptr"], d["source"], farr(x, np.float32), farr(y, np.float32), farr(z, np.float32),
+          farr(mu, np.float64), farr(phi, np.float64), i4(npix), gradout, cost, meas,
+          d["rshptr"], stokesout, i4(d["npart"]), farr(d["total_ext"][:npts], np.float32),
+          d["radiance"], i4(numder), d["partder"], d["dext"], d["dalb"], d["dleg"],
+          i4(d["nscatangle"]), d["ylmsun"], d["phasetab"], i4(1), d["dphasetab"],
+          i4(d["dnumphase"]), r4(sc.solarflux), i4(d["npx"]), i4(d["npy"]), i4(d["npz"]),
+          r4(sc.delx), r4(sc.dely), r4(0.0), r4(0.0), d["zlevels"], d["extdirp"],
+          r8(d["uniformzlev"]), d["dpath"], d["dptr"], lg(exact_single_scatter),
+          farr(unc, np.float64), jacobian, lg(False), np.zeros(1, np.int32), i4(1),
+          farr(rpp, np.int32), farr(rw, np.float64), farr(sw, np.float64),
+          s(costfunc, 2), i4(1), i4(1), i4(1), r8(sc.tautol), lg(False), ierr, errmsg,
+          s("ON"), i4(d["maxnmicro"]), d["diphase"], d["iphasep"], d["phasewtp"], d["dphasewt"],
+          farr(d["phaseinterpwt"][:, :npts], np.float32), d["albedop"], d["extinctp"],
+          d["optinterpwt"], d["interpptr"], r8(d["extmin"]), r8(d["scatmin"]), d["doexact"],
+          farr(d["temp"][:npts], np.float32), r4(0.999), np.zeros((maxpg, numder), np.float32, order="F"),
+          d["dextm"], d["dalbm"], d["dfj"], i4(max_subgrid_intervals(st)), r8(sc.transcut),
+          i4(d["longest_path_pts"]), i4(1), i4(1), i4(nsfcpts), sfcder, r4(0.0), r4(0.0),
+          i4(d["nang"]), d["sfcgridrad"])
+    _check(ierr, errmsg, "LEVISAPPROX_GRADIENT")
+    return gradout[:, :, 0].copy(), float(cost[0]), stokesout[0].copy()
+
+
+def adjoint_weights(stokesout, measurement, uncertainties, costfunc):
+    """COMPUTE_ADJOINT_WEIGHTS (shdomsub4.f:3969-4034) for one pixel: returns
+    (adjoint weights, cost contribution)."""
+    ns = len(stokesout)
+    nu = uncertainties.shape[0]
+    w = np.zeros(ns, np.float64)
+    cost = np.zeros(1, np.float64)
+    ierr, errmsg = i4(), s("", 600)
+    fcall("COMPUTE_ADJOINT_WEIGHTS", farr(stokesout, np.float64), farr(measurement, np.float64),
+          farr(uncertainties, np.float64), s(costfunc, 2), i4(ns), i4(nu), w, cost, i4(1),
+          ierr, errmsg)
+    _check(ierr, errmsg, "COMPUTE_ADJOINT_WEIGHTS")
+    return w, float(cost[0])

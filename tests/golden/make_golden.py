@@ -6,8 +6,10 @@ tests/scenes.py.  Run from the repository root after `make -C oracle`:
 
 Stored per scene (prefix "<scene>/"): a checksum of the generated inputs, the
 source after one iteration, the converged source, radiance and fluxes with the
-iteration count and criterion, and rendered radiances for orthographic views
-and for interior rays in all directions (with the rays).
+iteration count and criterion, rendered radiances for orthographic views and
+for interior rays in all directions (with the rays), and the Levis gradient
+and L2 cost with respect to the first species' extinction and albedo for
+measurements 1.1 x the view radiances.
 """
 import hashlib
 import os
@@ -73,6 +75,16 @@ def main():
                 out[f"{name}/{tag}/{k}"] = np.asarray(v)
             out[f"{name}/{tag}/I"] = ref.render(st, rays["ray_x"], rays["ray_y"], rays["ray_z"],
                                                 rays["ray_mu"], rays["ray_phi"])
+        shape = w.shape
+        nleg = w.species[0][2].shape[0]
+        ref.prepare_derivatives(st, [
+            ref.Derivative(0, np.ones(shape), np.zeros(shape), np.zeros((nleg, 1)), np.ones(shape, np.int32)),
+            ref.Derivative(0, np.zeros(shape), np.ones(shape), np.zeros((nleg, 1)), np.ones(shape, np.int32))])
+        v = views(w)
+        meas = (out[f"{name}/views/I"] * 1.1).astype(np.float32)
+        g, c, _ = ref.levis_gradient(st, v["ray_x"], v["ray_y"], v["ray_z"], v["ray_mu"], v["ray_phi"], meas)
+        out[f"{name}/grad"] = g
+        out[f"{name}/cost"] = np.array(c)
     np.savez_compressed(OUT, **out)
     print(OUT, os.path.getsize(OUT), "bytes")
 

@@ -1,0 +1,69 @@
+"""Scheduling logic of the serving loop, with stand-in solvers (no GPU)."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from at3d_amd.serving import Request, SolveServer
+
+
+class FakeRTE:
+    live = 0
+    lock = threading.Lock()
+
+    def __init__(self, n, fail=False):
+        self.n, self.fail, self.warm = n, fail, None
+        self.num_iterations, self.solcrit = 0, 1.0
+        with FakeRTE.lock:
+            FakeRTE.live += 1
+
+    def init_solution_from(self, other):
+        self.warm = other.n
+
+    def solve(self, maxiter, init_solution=True, verbose=False):
+        time.sleep(0.01)
+        if self.fail:
+            raise RuntimeError("boom")
+        self.num_iterations = 3 if self.warm is not None else 7
+        self.solcrit = 1e-5
+
+    def integrate_to_sensor(self, sensor):
+        return {"I": np.full(2, float(self.n))}
+
+    def release(self):
+        with FakeRTE.lock:
+            FakeRTE.live -= 1
+
+
+def test_server_runs_all_requests_with_warm_restarts_and_frees_solutions():
+    FakeRTE.live = 0
+    reqs = [Request(0, lambda: FakeRTE(0)), Request(1, lambda: FakeRTE(1), warm_from=0),
+            Request(2, lambda: FakeRTE(2), warm_from=0, render={}),
+            Request(3, lambda: FakeRTE(3), arrival_s=0.02), Request(4, lambda: FakeRTE(4), warm_from=3, arrival_s=0.03)]
+    res, st = SolveServer(nworkers=3).run(reqs)
+    assert [r.key for r in res] == [0, 1, 2, 3, 4]
+    assert all(r.error is None for r in res)
+    assert [r.iterations for r in res] == [7, 3, 3, 7, 3]
+    assert np.array_equal(res[2].radiance, [2.0, 2.0])
+    assert st["solves"] == 5 and st["failed"] == 0
+    assert st["latency_p99_ms"] >= st["latency_p50_ms"] > 0
+    assert FakeRTE.live == 0          # every device state released
+
+
+def test_server_reports_failures_per_request():
+    FakeRTE.live = 0
+    reqs = [Request("a", lambda: FakeRTE(0, fail=True)), Request("b", lambda: FakeRTE(1)),
+            Request("c", lambda: FakeRTE(2), warm_from="a")]
+    res, st = SolveServer(nworkers=2).run(reqs)
+    assert "boom" in res[0].error
+    assert res[1].error is None
+    assert "no solution" in res[2].error
+    assert st["failed"] == 2
+    assert FakeRTE.live == 0
+
+
+def test_server_rejects_warm_start_from_a_later_request():
+    with pytest.raises(ValueError):
+        SolveServer().run([Request(0, lambda: FakeRTE(0), warm_from=1),
+                           Request(1, lambda: FakeRTE(1), arrival_s=1.0)])

@@ -1,0 +1,67 @@
+"""Warm restarts and concurrent serving on the GPU."""
+import numpy as np
+import pytest
+
+from at3d_amd import synthetic as S
+from at3d_amd.serving import Request, SolveServer
+from tests.scenes import blob8, cloud_small, ortho_rays, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _perturbed(w, seed, scale=0.05):
+    rng = np.random.default_rng(seed)
+    sp = []
+    for ext, alb, leg in w.species:
+        noise = np.exp(scale * rng.standard_normal(ext.shape) - 0.5 * scale ** 2)
+        sp.append(((ext * noise).astype(np.float32), alb, leg))
+    return S.Workload(w.name, w.delx, w.dely, w.z, sp, bc=w.bc, nmu=w.nmu, nphi=w.nphi,
+                      solacc=w.solacc)
+
+
+def test_warm_restart_converges_to_cold_solution_in_fewer_iterations():
+    w0 = cloud_small()
+    w1 = _perturbed(w0, seed=3)
+    a = S.make_rte(w0)
+    a.solve(100, verbose=False)
+    cold = S.make_rte(w1)
+    cold.solve(100, verbose=False)
+    warm = S.make_rte(w1)
+    warm.init_solution_from(a)
+    warm.solve(100, init_solution=False, verbose=False)
+    assert warm.check_solved(verbose=False)
+    assert warm.num_iterations < cold.num_iterations
+    assert rel_l2(warm.state_array("source"), cold.state_array("source")) < 2e-4
+
+
+def test_warm_restart_rejects_other_grids():
+    a = S.make_rte(blob8("open"))
+    a.solve(2, verbose=False)
+    b = S.make_rte(cloud_small())
+    with pytest.raises(Exception, match="same grid"):
+        b.init_solution_from(a)
+
+
+def test_concurrent_serving_is_bit_identical_to_serial_solves():
+    base = [blob8("open"), blob8("periodic"), cloud_small()]
+    ws = base + [_perturbed(base[2], seed=s) for s in (1, 2, 3)]
+    views = [ortho_rays(w, [(0.0, 0.0), (45.0, 30.0)]) for w in ws]
+    serial = []
+    for w, v in zip(ws, views):
+        r = S.make_rte(w)
+        r.solve(100, verbose=False)
+        serial.append(r.integrate_to_sensor(dict(v))["I"])
+        r.release()
+    reqs = [Request(i, (lambda w=w: S.make_rte(w)), render=views[i]) for i, w in enumerate(ws)]
+    res, st = SolveServer(nworkers=4).run(reqs)
+    assert st["failed"] == 0, [r.error for r in res]
+    for r, ref in zip(res, serial):
+        assert np.array_equal(r.radiance, ref)
+    # warm-restarted chain: each perturbed scene starts from the previous solution
+    reqs = [Request("c0", lambda: S.make_rte(ws[2]))] + [
+        Request(f"c{k}", (lambda w=ws[2 + k]: S.make_rte(w)), warm_from=f"c{k - 1}", render=views[2 + k])
+        for k in (1, 2, 3)]
+    res, st = SolveServer(nworkers=2).run(reqs)
+    assert st["failed"] == 0, [r.error for r in res]
+    for k in (1, 2, 3):
+        assert rel_l2(res[k].radiance, serial[2 + k]) < 2e-4
