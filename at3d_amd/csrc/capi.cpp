@@ -150,11 +150,16 @@ int at3d_plan_get(const at3d_plan* h, const char* name, void* dst, size_t nbytes
     return copy_out(val, dst, nbytes, err, errlen);
   }
   if (k == "bcptr_bot") return copy_out(p.bcbot, dst, nbytes, err, errlen);
-  if (k == "sweepord" || k == "level") {
+  if (k == "sweepord" || k == "level" || k == "lvl_pts") {
+    // per octant, concatenated: [8][npts] for sweepord and level; lvl_pts is
+    // each octant's non-preset points in (level, rank) order, ~point for the
+    // multi-cell traces, octant after octant
     std::vector<int32_t> all;
     all.reserve((size_t)8 * p.npts);
     for (int o = 0; o < 8; o++) {
-      const std::vector<int32_t>& v = (k == "sweepord") ? p.oct[o].sweepord : p.oct[o].level;
+      const std::vector<int32_t>& v = (k == "sweepord") ? p.oct[o].sweepord
+                                      : (k == "level")  ? p.oct[o].level
+                                                        : p.oct[o].lvl_pts;
       all.insert(all.end(), v.begin(), v.end());
     }
     return copy_out(all, dst, nbytes, err, errlen);
@@ -298,9 +303,17 @@ int at3d_state_get(at3d_state* st, const char* name, void* dst, size_t nbytes, c
       return AT3D_ERR_DEVICE;
     }
     float* out = static_cast<float*>(dst);
+    const bool level_order = k == "do_source" && s.dosrc_level_order;
     for (size_t k2 = 0; k2 < nang; k2++) {
       const size_t c = s.col_of_ref[k2];
-      for (size_t pnt = 0; pnt < npts; pnt++) out[k2 * npts + pnt] = tmp[c * s.nptsp + pnt];
+      if (level_order) {
+        int o = 0;
+        while (o < 7 && (int)c >= s.oct_col_base[o] + s.oct_nord[o]) o++;
+        const int32_t* pos = s.oct_pos_h.data() + (size_t)o * npts;
+        for (size_t pnt = 0; pnt < npts; pnt++) out[k2 * npts + pnt] = tmp[c * s.nptsp + pos[pnt]];
+      } else {
+        for (size_t pnt = 0; pnt < npts; pnt++) out[k2 * npts + pnt] = tmp[c * s.nptsp + pnt];
+      }
     }
     return AT3D_OK;
   }
@@ -373,6 +386,10 @@ int at3d_state_set_option(at3d_state* st, const char* key, int32_t value, char* 
     st->s.use_graph = value ? 1 : 0;
     return AT3D_OK;
   }
+  if (k == "sweep_ring") {
+    st->s.use_ring = value ? 1 : 0;
+    return AT3D_OK;
+  }
   if (k == "max_subgrid_intervals") {
     st->s.max_subgrid = value > 0 ? value : 0;
     return AT3D_OK;
@@ -394,6 +411,10 @@ double at3d_state_timing(const at3d_state* st, const char* key) {
     return (double)s.nlevels_max;
   if (k == "total") return s.tsum[0] + s.tsum[1] + s.tsum[2] + s.tsum[3] + s.tsum[4] + s.tsum[5];
   if (k == "sweep_launches") return s.nsweep_launch;
+  if (k == "sweep_ring") return (s.ring_ok && s.use_ring && !s.force_general) ? 1.0 : 0.0;
+  if (k == "sweep_ring_lds") return (double)s.ring_lds;
+  if (k == "sweep_ring_slots") return s.ring_c;
+  if (k == "sweep_general_points") return s.ngeneral;
   if (k == "iters") return s.iters_last;
   if (k == "render") return s.render_ms;
   if (k == "gradient") return s.grad_ms;

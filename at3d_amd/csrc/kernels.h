@@ -13,6 +13,13 @@ namespace at3d {
 // mp, ncp multiples of 64; ld multiple of 16; ext padded to mp.
 void launch_sh_to_do(hipStream_t st, const float* S, const float* Yc, const float* ext,
                      float* dosrc, int mp, int ncp, int ld, int ldp, int npts, int nang);
+// Level-ordered variant: dosrc[c][q] = max(0, sum_J S[perm_o[q]][J] * Yc[c][J]) * ext[perm_o[q]]
+// for each column c of octant o, perm_o = oct_perm + o * mp (the octant's
+// boundary points, then its sweep levels).  tiles[t] = {octant, first column,
+// columns (<= 48), 0}.
+void launch_sh_to_do_perm(hipStream_t st, const float* S, const float* Yc, const float* ext,
+                          const int32_t* oct_perm, const int4* tiles, int ntiles, float* dosrc,
+                          int mp, int ld, int ldp);
 // R[p][J] = (J < nr[p]) ? sum_c dorad[c][p] * Wc[c][J] : 0
 // dorad [ldo][ldp] (rows beyond nang zero), Wc [ldo][ld] (rows beyond nang zero), R [mp][ld].
 void launch_do_to_sh(hipStream_t st, const float* dorad, const float* Wc, const int* nr, float* R,
@@ -46,7 +53,42 @@ struct SweepArgs {
   int ldp, npts;
   int force_general;
   int* errflag;
+  // level-ordered sweep (k_sweep_ring): srcext rows are in each octant's
+  // position order (boundary points, then levels), radiances stay point-ordered
+  const int32_t* oct_pos;    // [8][npts] point -> position in the octant's order
+  const int32_t* top_index;  // [npts] index in bctop or -1
+  const int32_t* bot_index;  // [npts] index in bcbot or -1
+  const uint4* rec;          // [ncols][ldr][2] one-cell step records (SweepRec)
+  int ldr;                   // records per column (max swept points of an octant)
+  const uint8_t* gflag;      // [ncols][gld] level L holds a multi-cell trace
+  int gld;
+  int ring_c;                // ring entries (points of RING_LEVELS+1 consecutive levels)
+  int* ngeneral;             // multi-cell traces found by the record pass
 };
+
+// A one-cell characteristic with every coefficient that stays fixed during a
+// solve: rad = max(0, b0 * srcext_face + b1 * srcext_pt) + T * rad_face, the
+// face values bilinear in (u, v) over four LDS entries r[0..3] (preset
+// boundary slots first, then the ring of recent levels).  pt < 0 (= ~point)
+// marks a multi-cell trace.
+struct SweepRec {
+  int32_t pt;
+  uint16_t r[4];
+  float u, v, b0, b1, T;
+};
+static_assert(sizeof(SweepRec) == 32, "SweepRec is two 16-byte words");
+constexpr int RING_LEVELS = 3;     // earlier levels a one-cell record may reference
+constexpr int RING_THREADS = 512;
+
+// Bytes of LDS k_sweep_ring needs: npre boundary slots and ring_c ring
+// entries of (radiance, source), gld level offsets and gld level flags.
+inline size_t ring_lds_bytes(int npre, int ring_c, int gld) {
+  return (size_t)(npre + ring_c) * 8 + (size_t)gld * 5;
+}
+// Allows k_sweep_ring launches with lds_bytes of dynamic LDS; nonzero on failure.
+int sweep_ring_max_lds(size_t lds_bytes);
+void launch_sweep_records(hipStream_t st, const SweepArgs& a, int ncols, int max_swept);
+void launch_sweep_ring(hipStream_t st, const SweepArgs& a, int col0, int ncol, size_t lds_bytes);
 
 void launch_sweep_preset(hipStream_t st, const SweepArgs& a, bool up, int col0, int ncol);
 // One workgroup per ordinate column col0 .. col0+ncol-1, all levels of its octant.

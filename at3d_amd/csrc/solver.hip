@@ -5,7 +5,8 @@
 //   SH -> DO transform   (SH_TO_DO_UNPOL :2928) MFMA GEMM      transforms.hip
 //   downwelling sweeps, Lambertian surface, upwelling sweeps
 //                        (PATH_INTEGRATION :1836, BACK_INT_GRID3D_UNPOL :3695)
-//                        one workgroup per ordinate, replayed from a HIP graph   sweep.hip
+//                        one workgroup per ordinate over per-solve step records,
+//                        replayed from a HIP graph                               sweep.hip
 //   DO -> SH transform   (DO_TO_SH_UNPOL :3164) MFMA GEMM      transforms.hip
 //   source function      (COMPUTE_SOURCE :967)                 source.hip
 //   criterion            (CALC_ACCEL_SOLCRIT shdom_nompi.f:317) host, single precision
@@ -13,7 +14,9 @@
 //
 // Device layouts: SH fields [point][ld] (ld = nlm rounded up to 64); DO fields
 // ordinate-major [ldo][nptsp] with the ordinates grouped by octant (ldo = nang
-// rounded up to 64, nptsp = points rounded up to 64, padding zero).
+// rounded up to 64, nptsp = points rounded up to 64, padding zero).  With the
+// level-ordered sweep the DO source rows are in their octant's position order
+// (boundary points, then sweep levels) and the DO radiance rows by point.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
@@ -106,7 +109,84 @@ static SweepArgs sweep_args(const State& s) {
   a.npts = p.npts;
   a.force_general = s.force_general;
   a.errflag = s.errflag;
+  a.oct_pos = s.oct_pos;
+  a.top_index = s.top_index;
+  a.bot_index = s.bot_index;
+  a.rec = s.rec;
+  a.ldr = s.max_swept;
+  a.gflag = s.gflag;
+  a.gld = s.gld;
+  a.ring_c = s.ring_c;
+  a.ngeneral = s.ngeneral_d;
   return a;
+}
+
+static bool ring_active(const State& s) { return s.ring_ok && s.use_ring && !s.force_general; }
+
+// The level-ordered sweep: each octant's position order (boundary points of its
+// hemisphere, then its levels), the SH->DO column tiles, the ring size, and the
+// per-solve records.  Leaves ring_ok false (and the point-ordered sweep in use)
+// when the ring does not fit in LDS or the records do not fit in memory.
+static int setup_ring(const Plan& p, State& s, std::string& err) {
+  s.ring_ok = false;
+  std::vector<int32_t> top_index(p.npts, -1);
+  for (size_t b = 0; b < p.bctop.size(); b++) top_index[p.bctop[b]] = (int32_t)b;
+  TRY(upload(s, s.top_index, top_index.data(), top_index.size(), err));
+  std::vector<int32_t> perm((size_t)8 * s.nptsp);
+  s.oct_pos_h.assign((size_t)8 * p.npts, 0);
+  int C = 0, npre_max = 0;
+  bool complete = true;
+  s.max_swept = 0;
+  s.gld = s.nlevels_max + 2;
+  std::vector<int4> tiles;
+  for (int o = 0; o < 8; o++) {
+    const OctantPlan& op = p.oct[o];
+    const std::vector<int32_t>& bc = ((o >> 2) & 1) ? p.bcbot : p.bctop;
+    int32_t* pm = perm.data() + (size_t)o * s.nptsp;
+    size_t q = 0;
+    for (int b : bc) pm[q++] = b;
+    for (int v : op.lvl_pts) pm[q++] = v >= 0 ? v : ~v;
+    if (q != (size_t)p.npts) {
+      complete = false;
+      continue;
+    }
+    for (; q < (size_t)s.nptsp; q++) pm[q] = (int32_t)q;  // zero padding rows
+    for (int i = 0; i < p.npts; i++) s.oct_pos_h[(size_t)o * p.npts + pm[i]] = i;
+    if (s.oct_nord[o] <= 0) continue;
+    npre_max = std::max(npre_max, (int)bc.size());
+    s.max_swept = std::max(s.max_swept, op.lvl_off[op.nlevels]);
+    for (int L = 1; L <= op.nlevels; L++)
+      C = std::max(C, op.lvl_off[L] - op.lvl_off[std::max(L - 1 - RING_LEVELS, 0)]);
+    for (int c = s.oct_col_base[o]; c < s.oct_col_base[o] + s.oct_nord[o]; c += 48)
+      tiles.push_back(make_int4(o, c, std::min(48, s.oct_col_base[o] + s.oct_nord[o] - c), 0));
+  }
+  int dev = 0, maxlds = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  s.ring_c = std::max(C, 1);
+  s.ring_lds = ring_lds_bytes(npre_max, s.ring_c, s.gld);
+  if (!complete || s.max_swept <= 0 || npre_max + s.ring_c > 65535 || s.ring_lds > (size_t)maxlds ||
+      sweep_ring_max_lds(s.ring_lds) != 0)
+    return AT3D_OK;
+  const size_t nrec = (size_t)p.nang * s.max_swept * 2;
+  if (hipMalloc((void**)&s.rec, nrec * sizeof(uint4)) != hipSuccess) {
+    (void)hipGetLastError();
+    s.rec = nullptr;
+    return AT3D_OK;
+  }
+  s.allocs.push_back(s.rec);
+  TRY(upload(s, s.oct_perm, perm.data(), perm.size(), err));
+  TRY(upload(s, s.oct_pos, s.oct_pos_h.data(), s.oct_pos_h.size(), err));
+  TRY(upload(s, s.sh_tiles, tiles.data(), tiles.size(), err));
+  s.n_sh_tiles = (int)tiles.size();
+  TRY(alloc0(s, s.gflag, (size_t)p.nang * s.gld, err));
+  TRY(alloc0(s, s.ngeneral_d, (size_t)1, err));
+  launch_sweep_records(s.stream, sweep_args(s), p.nang, s.max_swept);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(&s.ngeneral, s.ngeneral_d, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+  HIPCHK(hipStreamSynchronize(s.stream));
+  s.ring_ok = true;
+  return AT3D_OK;
 }
 
 std::mutex& device_mutex() {
@@ -253,6 +333,7 @@ int state_create(const Plan& p, State& s, std::string& err) {
   for (int i = 0; i < p.npts; i++)
     for (int j = 0; j < 4 && j < s.ld; j++) r0[(size_t)i * s.ld + j] = p.radiance0[(size_t)i * 4 + j];
   TRY(upload(s, s.R0, r0.data(), r0.size(), err));
+  TRY(setup_ring(p, s, err));
   for (int e = 0; e < State::NEV; e++) HIPCHK(hipEventCreate(&s.ev[e]));
   return AT3D_OK;
 }
@@ -350,12 +431,19 @@ static void enqueue_sweeps(State& s) {
   const Plan& p = *s.plan;
   const SweepArgs a = sweep_args(s);
   const float albpi = p.d.gndalbedo / acosf(-1.0f);
+  const bool ring = ring_active(s);
   launch_sweep_preset(s.stream, a, false, 0, s.ndown);
-  launch_sweep_columns(s.stream, a, 0, s.ndown);
+  if (ring)
+    launch_sweep_ring(s.stream, a, 0, s.ndown, s.ring_lds);
+  else
+    launch_sweep_columns(s.stream, a, 0, s.ndown);
   launch_bottom_bc(s.stream, s.dorad, s.nptsp, s.bcbot, (int)p.bcbot.size(), s.col_of_ref_d, s.fluxw,
                    s.ndown, s.dirflux, albpi, s.bcrad_bot);
   launch_sweep_preset(s.stream, a, true, s.ndown, p.nang - s.ndown);
-  launch_sweep_columns(s.stream, a, s.ndown, p.nang - s.ndown);
+  if (ring)
+    launch_sweep_ring(s.stream, a, s.ndown, p.nang - s.ndown, s.ring_lds);
+  else
+    launch_sweep_columns(s.stream, a, s.ndown, p.nang - s.ndown);
 }
 
 static int run_sweeps(State& s, std::string& err) {
@@ -364,7 +452,8 @@ static int run_sweeps(State& s, std::string& err) {
     HIPCHK(hipGetLastError());
     return AT3D_OK;
   }
-  if (!s.sweep_exec || s.graph_general != s.force_general) {
+  const int ring = ring_active(s) ? 1 : 0;
+  if (!s.sweep_exec || s.graph_general != s.force_general || s.graph_ring != ring) {
     if (s.sweep_exec) (void)hipGraphExecDestroy(s.sweep_exec);
     if (s.sweep_graph) (void)hipGraphDestroy(s.sweep_graph);
     s.sweep_exec = nullptr;
@@ -381,6 +470,7 @@ static int run_sweeps(State& s, std::string& err) {
     }
     HIPCHK(hipGraphInstantiate(&s.sweep_exec, s.sweep_graph, nullptr, nullptr, 0));
     s.graph_general = s.force_general;
+    s.graph_ring = ring;
   }
   HIPCHK(hipGraphLaunch(s.sweep_exec, s.stream));
   return AT3D_OK;
@@ -424,7 +514,13 @@ int state_solve(State& s, int maxiter, int verbose, std::string& err) {
     launch_radiance_truncation(s.stream, s.R, s.ld, s.tc, s.ns, s.lofj, s.nr, p.npts, p.ml, p.mm,
                                p.nlm, radmin, fixsh ? 1 : 0, p.d.high_order_radiance ? 1 : 0);
     (void)hipEventRecord(s.ev[1], s.stream);
-    launch_sh_to_do(s.stream, s.S, s.Y, s.ext, s.dosrc, s.nptsp, s.ldo, s.ld, s.nptsp, p.npts, p.nang);
+    s.dosrc_level_order = ring_active(s);
+    if (s.dosrc_level_order)
+      launch_sh_to_do_perm(s.stream, s.S, s.Y, s.ext, s.oct_perm, s.sh_tiles, s.n_sh_tiles, s.dosrc,
+                           s.nptsp, s.ld, s.nptsp);
+    else
+      launch_sh_to_do(s.stream, s.S, s.Y, s.ext, s.dosrc, s.nptsp, s.ldo, s.ld, s.nptsp, p.npts,
+                      p.nang);
     (void)hipEventRecord(s.ev[2], s.stream);
     TRY(run_sweeps(s, err));
     (void)hipEventRecord(s.ev[3], s.stream);

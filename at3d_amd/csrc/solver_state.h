@@ -53,12 +53,25 @@ struct State {
   int32_t *rank = nullptr, *level = nullptr, *lvl_pts = nullptr, *lvl_cell = nullptr;
   int32_t* lvl_off = nullptr;
   int8_t* col_oct = nullptr;
+  // level-ordered sweep (k_sweep_ring)
+  bool ring_ok = false;        // ring fits in LDS and records are built
+  int use_ring = 1;            // option: sweep with k_sweep_ring when ring_ok
+  int ring_c = 0, gld = 0, max_swept = 0, ngeneral = 0;
+  size_t ring_lds = 0;         // dynamic LDS bytes of k_sweep_ring
+  int32_t *oct_perm = nullptr, *oct_pos = nullptr, *top_index = nullptr;
+  std::vector<int32_t> oct_pos_h;   // [8][npts]
+  uint4* rec = nullptr;
+  uint8_t* gflag = nullptr;
+  int* ngeneral_d = nullptr;
+  int4* sh_tiles = nullptr;
+  int n_sh_tiles = 0;
+  bool dosrc_level_order = false;   // layout of dosrc written by the last SH->DO
   // options
   int force_general = 0;  // 1: every point through the general characteristic trace
   int use_graph = 1;      // replay the sweep level launches from a HIP graph
   hipGraph_t sweep_graph = nullptr;
   hipGraphExec_t sweep_exec = nullptr;
-  int graph_general = -1;
+  int graph_general = -1, graph_ring = -1;
   // iteration state
   bool initialized = false;
   int iters = 0, iters_last = 0, nsweep_launch = 0;

@@ -6,11 +6,22 @@
 // replays that order and assigns every point a dependency level per octant
 // (all upwind face points of every ordinate of the octant sit on lower
 // levels).  On the GPU one workgroup integrates one ordinate over the whole
-// grid: it walks the levels of the ordinate's octant in order, a thread per
-// point of the level, with a workgroup barrier between levels, so a
-// hemisphere's sweep is one launch whose workgroups run independently.  The
-// DO arrays are stored ordinate-major ([ordinate][point]) so a workgroup's
-// gathers and stores stay inside one contiguous row.
+// grid, walking the levels of the ordinate's octant with a workgroup barrier
+// between levels, so a hemisphere's sweep is one launch whose workgroups run
+// independently.
+//
+// k_sweep_ring is the production sweep.  Everything about a one-cell
+// characteristic that stays fixed during a solve (face points, bilinear
+// weights, transmission, the source-integral coefficients) is computed once
+// per solve by k_sweep_records and stored in level order, so a level is read
+// as one contiguous block.  The radiances and source values of the last
+// RING_LEVELS levels and of the boundary points live in LDS; a level then
+// costs LDS gathers, a few FMAs and a barrier that waits for LDS traffic only,
+// while the next level's records stream in behind it.  Multi-cell traces
+// (periodic seams, faces older than the ring) take the general trace through
+// global memory behind a full barrier.  k_sweep_columns is the original
+// sweep on point-ordered DO arrays; it serves grids whose ring does not fit
+// in LDS and the general-trace check.
 #include <hip/hip_runtime.h>
 #include <limits.h>
 
@@ -102,17 +113,19 @@ __device__ inline float sweep_simple(const SweepArgs& a, const OrdGeom& og, int 
 }
 
 // General characteristic: trace through cells until a face with known radiances.
+// The source row `se` is indexed by point, or through `pos` (point -> position)
+// when it is stored in an octant's level order.
 __device__ inline float sweep_general(const SweepArgs& a, const OrdGeom& og, int col, int o, int pt,
-                                      int icell, int L) {
+                                      int icell, int L, const float* se, const int32_t* pos) {
   const int presetbit = og.bitz ? 2 : 1;
   const int32_t* rank = a.rank + (size_t)o * a.npts;
   const int32_t* level = a.level + (size_t)o * a.npts;
   const int rp = rank[pt];
   const float* ext = a.ext;
-  const float* se = a.srcext + (size_t)col * a.ldp;
+  auto SE = [se, pos](int q) { return se[pos ? pos[q] : q]; };
   const float* rad = a.rad + (size_t)col * a.ldp;
   double xe = a.g.gridpos[pt * 3], ye = a.g.gridpos[pt * 3 + 1], ze = a.g.gridpos[pt * 3 + 2];
-  double ext1 = ext[pt], srcext1 = se[pt];
+  double ext1 = ext[pt], srcext1 = SE(pt);
   double radv = 0.0, transmit = 1.0;
   for (int step = 0; step <= a.g.ncells; step++) {
     TraceStep t;
@@ -123,7 +136,7 @@ __device__ inline float sweep_general(const SweepArgs& a, const OrdGeom& og, int
     const double f1 = (1 - t.u) * (1 - t.v), f2 = (1 - t.u) * t.v;
     const double f3 = t.u * (1 - t.v), f4 = t.u * t.v;
     const double ext0 = f1 * ext[t.i[0]] + f2 * ext[t.i[1]] + f3 * ext[t.i[2]] + f4 * ext[t.i[3]];
-    const double srcext0 = f1 * se[t.i[0]] + f2 * se[t.i[1]] + f3 * se[t.i[2]] + f4 * se[t.i[3]];
+    const double srcext0 = f1 * SE(t.i[0]) + f2 * SE(t.i[1]) + f3 * SE(t.i[2]) + f4 * SE(t.i[3]);
     double src, absc, trc;
     cell_integral(ext0, ext1, srcext0, srcext1, t.so, src, absc, trc);
     radv = radv + transmit * src * absc;
@@ -169,6 +182,7 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_columns(SweepArgs a, in
   const int nlev = a.nlev[o];
   const int32_t* off = a.lvl_off + a.lo_base[o];
   float* rad = a.rad + (size_t)col * a.ldp;
+  const float* se_row = a.srcext + (size_t)col * a.ldp;
   const int tid = threadIdx.x;
   const bool general = a.force_general != 0;
   SimpleStep st[SWEEP_K];
@@ -195,7 +209,7 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_columns(SweepArgs a, in
       } else {
         const int j = beg + tid + k * SWEEP_THREADS;
         const int pt = c >= 0 ? c : ~c;
-        rad[pt] = sweep_general(a, og, col, o, pt, a.lvl_cell[j], L);
+        rad[pt] = sweep_general(a, og, col, o, pt, a.lvl_cell[j], L, se_row, nullptr);
       }
     }
     for (int j = beg + tid + SWEEP_K * SWEEP_THREADS; j < end; j += SWEEP_THREADS) {
@@ -204,11 +218,238 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_columns(SweepArgs a, in
         rad[c] = sweep_simple(a, og, col, c);
       } else {
         const int pt = c >= 0 ? c : ~c;
-        rad[pt] = sweep_general(a, og, col, o, pt, a.lvl_cell[j], L);
+        rad[pt] = sweep_general(a, og, col, o, pt, a.lvl_cell[j], L, se_row, nullptr);
       }
     }
     if (L < nlev) prepare(L + 1);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Level-ordered sweep with an LDS ring
+// ---------------------------------------------------------------------------
+
+// Once per solve: the first cell step of every point's characteristic for
+// every ordinate (the loop body of BACK_INT_GRID3D_UNPOL, shdomsub1.f:3822-3920,
+// with the cell integral of :3880-3905).  A step whose face is known (boundary
+// point or earlier point of the sweep order) within the last RING_LEVELS
+// levels becomes a record whose source integral is written as coefficients of
+// the face and point source values (the algebra of cell_integral rearranged:
+// both of its branches are linear in SRCEXT0 and SRCEXT1 before the clamp).
+// Every other point is marked for the general trace and its level flagged.
+__global__ __launch_bounds__(256) void k_sweep_records(SweepArgs a) {
+#pragma clang fp contract(off)
+  const int col = blockIdx.y;
+  const int o = a.col_oct[col];
+  const int nlev = a.nlev[o];
+  if (nlev <= 0) return;
+  const int32_t* off = a.lvl_off + a.lo_base[o];
+  const int base = off[0];
+  const int jl = blockIdx.x * blockDim.x + threadIdx.x;
+  if (jl >= off[nlev] - base) return;
+  const OrdGeom og = a.ordg[col];
+  const int up = og.bitz;
+  const int presetbit = up ? 2 : 1;
+  const int npre = up ? a.nbot : a.ntop;
+  const int32_t* pidx = up ? a.bot_index : a.top_index;
+  const int c = a.lvl_pts[base + jl];
+  const int pt = c >= 0 ? c : ~c;
+  const int32_t* rank = a.rank + (size_t)o * a.npts;
+  const int32_t* level = a.level + (size_t)o * a.npts;
+  const int32_t* pos = a.oct_pos + (size_t)o * a.npts;
+  const int L = level[pt];
+  SweepRec r;
+  r.pt = ~pt;
+  r.r[0] = r.r[1] = r.r[2] = r.r[3] = 0;
+  r.u = r.v = r.b0 = r.b1 = r.T = 0.0f;
+  double xe = a.g.gridpos[pt * 3], ye = a.g.gridpos[pt * 3 + 1], ze = a.g.gridpos[pt * 3 + 2];
+  TraceStep t;
+  bool ok = trace_step(a.g, og, a.lvl_cell[base + jl], xe, ye, ze, t);
+  for (int k = 0; k < 4 && ok; k++) {
+    const int q = t.i[k];
+    if (a.preset[q] & presetbit) {
+      ok = pidx[q] >= 0;
+      r.r[k] = (uint16_t)pidx[q];
+    } else {
+      const int lq = level[q];
+      ok = rank[q] < rank[pt] && lq >= 1 && lq < L && lq >= L - RING_LEVELS;
+      r.r[k] = (uint16_t)(npre + (pos[q] - npre) % a.ring_c);
+    }
+  }
+  if (ok) {
+    const float* ext = a.ext;
+    const double f1 = (1 - t.u) * (1 - t.v), f2 = (1 - t.u) * t.v;
+    const double f3 = t.u * (1 - t.v), f4 = t.u * t.v;
+    const double ext0 = f1 * ext[t.i[0]] + f2 * ext[t.i[1]] + f3 * ext[t.i[2]] + f4 * ext[t.i[3]];
+    const double ext1 = ext[pt], so = t.so;
+    const double e = 0.5 * (ext0 + ext1);
+    const double tau = e * so;
+    double absc, trc;
+    if (tau >= 0.5) {
+      trc = exp(-tau);
+      absc = 1.0 - trc;
+    } else {
+      absc = tau * (1.0 - 0.5 * tau * (1.0 - (double)0.33333333333f * tau * (1 - 0.25 * tau)));
+      trc = 1.0 - absc;
+    }
+    double c0, c1;  // src = c0 * SRCEXT0 + c1 * SRCEXT1 before the clamp at zero
+    if (tau <= 2.0) {
+      if (e == 0.0) {
+        c0 = c1 = 0.0;
+      } else {
+        const double k = (double)0.08333333333f * so;
+        c0 = (0.5 - k * ext1) / e;
+        c1 = (0.5 + k * ext0) / e;
+      }
+    } else {
+      double e0p = ext0, sc = 1.0;
+      if (tau > 4.0) {
+        e0p = ext1 + (ext0 - ext1) * 4.0 / tau;
+        if (ext0 > 0.0) sc = e0p / ext0;
+      }
+      const double d = e0p + ext1;
+      const double g = 2.0 / d * (1 - 2 / tau + 2 * trc / absc);
+      c0 = sc * (1.0 - ext1 * g) / d;
+      c1 = (1.0 + e0p * g) / d;
+    }
+    r.pt = pt;
+    r.u = (float)t.u;
+    r.v = (float)t.v;
+    r.b0 = (float)(c0 * absc);
+    r.b1 = (float)(c1 * absc);
+    r.T = (float)trc;
+  } else {
+    atomicAdd(a.ngeneral, 1);
+    const_cast<uint8_t*>(a.gflag)[(size_t)col * a.gld + L] = 1;
+  }
+  const uint4* w = reinterpret_cast<const uint4*>(&r);
+  uint4* dst = const_cast<uint4*>(a.rec) + ((size_t)col * a.ldr + jl) * 2;
+  dst[0] = w[0];
+  dst[1] = w[1];
+}
+
+// Barrier that orders LDS traffic only: outstanding global loads (the next
+// level's records) and stores (radiances) stay in flight across it.
+__device__ inline void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+constexpr int RING_K = 5;  // points per thread per level with records loaded a level ahead
+
+struct RingSet {
+  uint4 w0[RING_K], w1[RING_K];
+  float s[RING_K];
+};
+
+// One ordinate (workgroup) through every level of its octant.  LDS holds
+// (radiance, source) of the hemisphere's boundary points in slots
+// [0, npre) and of the last RING_LEVELS + 1 levels in a ring of ring_c slots
+// after them, followed by the octant's level offsets and the per-level
+// full-barrier flags.
+__global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int col0) {
+  extern __shared__ float2 slot[];
+  const int col = col0 + blockIdx.x;
+  const int o = a.col_oct[col];
+  const int nlev = a.nlev[o];
+  const OrdGeom og = a.ordg[col];
+  const int up = og.bitz;
+  const int npre = up ? a.nbot : a.ntop;
+  const int C = a.ring_c;
+  float2* ring = slot + npre;
+  int32_t* off = reinterpret_cast<int32_t*>(ring + C);
+  uint8_t* full = reinterpret_cast<uint8_t*>(off + a.gld);
+  const int32_t* goff = a.lvl_off + a.lo_base[o];
+  const int base = goff[0];
+  const float* se = a.srcext + (size_t)col * a.ldp;  // level order
+  const float* sl = se + npre;
+  float* rad = a.rad + (size_t)col * a.ldp;          // point order
+  const uint4* rec = a.rec + (size_t)col * a.ldr * 2;
+  const uint8_t* gf = a.gflag + (size_t)col * a.gld;
+  const int32_t* pos = a.oct_pos + (size_t)o * a.npts;
+  const int tid = threadIdx.x;
+  const float sky = a.skyrad[col];
+  for (int b = tid; b < npre; b += RING_THREADS) slot[b] = make_float2(up ? a.bcrad_bot[b] : sky, se[b]);
+  for (int L = tid; L <= nlev; L += RING_THREADS) off[L] = goff[L] - base;
+  for (int L = tid; L < a.gld; L += RING_THREADS) full[L] = gf[L];
+  const int nsw = goff[nlev] - base;
+
+  auto put = [&](int jl, int k0, float val, float s1) {
+    int k = k0 + jl;
+    if (k >= C) k -= C;
+    ring[k] = make_float2(val, s1);
+  };
+  // One-cell record: LDS gathers and the cell integral.  Returns false for a
+  // multi-cell trace, which the level's second pass integrates.
+  auto finish = [&](const uint4& w0, const uint4& w1, float s1, int jl, int k0) {
+    const int pt = (int)w0.x;
+    if (pt < 0) return false;
+    const float2 q0 = slot[w0.y & 0xffffu], q1 = slot[w0.y >> 16];
+    const float2 q2 = slot[w0.z & 0xffffu], q3 = slot[w0.z >> 16];
+    const float u = __uint_as_float(w0.w), v = __uint_as_float(w1.x);
+    const float f1 = (1.f - u) * (1.f - v), f2 = (1.f - u) * v, f3 = u * (1.f - v), f4 = u * v;
+    const float s0 = f1 * q0.y + f2 * q1.y + f3 * q2.y + f4 * q3.y;
+    const float r0 = f1 * q0.x + f2 * q1.x + f3 * q2.x + f4 * q3.x;
+    const float val = fmaxf(0.f, __uint_as_float(w1.y) * s0 + __uint_as_float(w1.z) * s1) +
+                      __uint_as_float(w1.w) * r0;
+    rad[pt] = val;
+    put(jl, k0, val, s1);
+    return true;
+  };
+  // Records of a level, loaded unconditionally (indices clamped into the
+  // octant's records) so the loads of every lane are always in flight.
+  auto load = [&](int L, RingSet& S) {
+    const int beg = __builtin_amdgcn_readfirstlane(off[L - 1]);
+#pragma unroll
+    for (int k = 0; k < RING_K; k++) {
+      const int jl = min(beg + tid + k * RING_THREADS, nsw - 1);
+      S.w0[k] = rec[2 * jl];
+      S.w1[k] = rec[2 * jl + 1];
+      S.s[k] = sl[jl];
+    }
+  };
+  auto step = [&](int L, RingSet& cur, RingSet& nxt) {
+    load(L + 1, nxt);  // past the last level this re-reads the last record
+    const int beg = __builtin_amdgcn_readfirstlane(off[L - 1]);
+    const int end = __builtin_amdgcn_readfirstlane(off[L]);
+    const int k0 = beg % C - beg;  // ring slot of local position jl: k0 + jl, wrapped once
+    uint32_t gen = 0;              // this thread's multi-cell traces among its first RING_K
+#pragma unroll
+    for (int k = 0; k < RING_K; k++) {
+      const int jl = beg + tid + k * RING_THREADS;
+      if (jl < end && !finish(cur.w0[k], cur.w1[k], cur.s[k], jl, k0)) gen |= 1u << k;
+    }
+    for (int jl = beg + tid + RING_K * RING_THREADS; jl < end; jl += RING_THREADS)
+      (void)finish(rec[2 * jl], rec[2 * jl + 1], sl[jl], jl, k0);
+    if (__builtin_amdgcn_readfirstlane(full[L])) {
+      // multi-cell traces read radiances of earlier levels from global memory,
+      // complete and visible after the full barrier that ended level L-1
+      int k = 0;
+      for (int jl = beg + tid; jl < end; jl += RING_THREADS, k++) {
+        const bool g = k < RING_K ? ((gen >> k) & 1) != 0 : (int)rec[2 * jl].x < 0;
+        if (!g) continue;
+        const int p = ~(int)rec[2 * jl].x;
+        const float s1 = sl[jl];
+        const float val = sweep_general(a, og, col, o, p, a.lvl_cell[base + jl], L, se, pos);
+        rad[p] = val;
+        put(jl, k0, val, s1);
+      }
+    }
+    if (L < nlev && __builtin_amdgcn_readfirstlane(full[L + 1]))
+      __syncthreads();
+    else
+      lds_barrier();
+  };
+  RingSet A, B;
+  __syncthreads();
+  if (nlev >= 1) load(1, A);
+  int L = 1;
+  for (; L + 1 <= nlev; L += 2) {
+    step(L, A, B);
+    step(L + 1, B, A);
+  }
+  if (L == nlev) step(L, A, B);
 }
 
 // Boundary radiances for one hemisphere: sky radiance at the top points for
@@ -261,6 +502,23 @@ void launch_sweep_preset(hipStream_t st, const SweepArgs& a, bool up, int col0, 
 void launch_sweep_columns(hipStream_t st, const SweepArgs& a, int col0, int ncol) {
   if (ncol <= 0) return;
   k_sweep_columns<<<ncol, SWEEP_THREADS, 0, st>>>(a, col0);
+}
+
+void launch_sweep_records(hipStream_t st, const SweepArgs& a, int ncols, int max_swept) {
+  if (ncols <= 0 || max_swept <= 0) return;
+  k_sweep_records<<<dim3((max_swept + 255) / 256, ncols), 256, 0, st>>>(a);
+}
+
+void launch_sweep_ring(hipStream_t st, const SweepArgs& a, int col0, int ncol, size_t lds_bytes) {
+  if (ncol <= 0) return;
+  k_sweep_ring<<<ncol, RING_THREADS, lds_bytes, st>>>(a, col0);
+}
+
+int sweep_ring_max_lds(size_t lds_bytes) {
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(k_sweep_ring),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) == hipSuccess
+             ? 0
+             : 1;
 }
 
 void launch_bottom_bc(hipStream_t st, const float* rad, int ldp, const int32_t* bcbot, int nbot,

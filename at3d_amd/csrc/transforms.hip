@@ -103,6 +103,71 @@ __global__ __launch_bounds__(256) void k_sh_to_do(const float* __restrict__ S,
   }
 }
 
+// SH->DO in the sweep's level order: one workgroup computes 64 consecutive
+// positions of one octant's order (S rows gathered through perm) for up to 48
+// columns of that octant; each wave owns 16 positions x 48 columns (three
+// 16x16 MFMA blocks).
+constexpr int PN = 48;
+__global__ __launch_bounds__(256) void k_sh_to_do_perm(const float* __restrict__ S,
+                                                       const float* __restrict__ Yc,
+                                                       const float* __restrict__ ext,
+                                                       const int32_t* __restrict__ oct_perm,
+                                                       const int4* __restrict__ tiles,
+                                                       float* __restrict__ out, int ld, int ldp) {
+  __shared__ float sA[TK][TM + PAD];
+  __shared__ float sB[TK][PN + PAD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int4 tile = tiles[blockIdx.y];
+  const int32_t* perm = oct_perm + (size_t)tile.x * ldp;
+  const int c0 = tile.y, nc = tile.z;
+  const int m0 = blockIdx.x * TM;
+  float4_ acc[3];
+  for (int j = 0; j < 3; j++) acc[j] = (float4_){0.f, 0.f, 0.f, 0.f};
+  const int lr = tid >> 2, lc = (tid & 3) * 4;
+  const float* arow = S + (size_t)perm[m0 + lr] * ld + lc;
+  const bool bload = lr < nc;  // threads 0..4*nc-1 load the column tile
+  const float* brow = Yc + (size_t)(c0 + (bload ? lr : 0)) * ld + lc;
+  for (int k0 = 0; k0 < ld; k0 += TK) {
+    const float4 a = *reinterpret_cast<const float4*>(arow + k0);
+    float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (bload) b = *reinterpret_cast<const float4*>(brow + k0);
+    __syncthreads();
+    sA[lc + 0][lr] = a.x;
+    sA[lc + 1][lr] = a.y;
+    sA[lc + 2][lr] = a.z;
+    sA[lc + 3][lr] = a.w;
+    if (lr < PN) {
+      sB[lc + 0][lr] = b.x;
+      sB[lc + 1][lr] = b.y;
+      sB[lc + 2][lr] = b.z;
+      sB[lc + 3][lr] = b.w;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < TK; kk += 4) {
+      const int kr = kk + (lane >> 4);
+      const float av = sA[kr][wave * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, sB[kr][j * 16 + (lane & 15)], acc[j], 0, 0, 0);
+    }
+  }
+  // rows beyond npts gather the zero padding rows of S and ext
+  const int q0 = m0 + wave * 16 + 4 * (lane >> 4);
+  float e[4];
+  for (int v = 0; v < 4; v++) e[v] = ext[perm[q0 + v]];
+  for (int j = 0; j < 3; j++) {
+    const int c = j * 16 + (lane & 15);
+    if (c >= nc) continue;
+    float4 o;
+    o.x = fmaxf(acc[j][0], 0.0f) * e[0];
+    o.y = fmaxf(acc[j][1], 0.0f) * e[1];
+    o.z = fmaxf(acc[j][2], 0.0f) * e[2];
+    o.w = fmaxf(acc[j][3], 0.0f) * e[3];
+    *reinterpret_cast<float4*>(out + (size_t)(c0 + c) * ldp + q0) = o;
+  }
+}
+
 // C[m][n] = sum_k A[m][k] B[k][n]: A stored ordinate-major (A[m][k] at
 // D[k * ldp + m]), B N-contiguous rows.
 __global__ __launch_bounds__(256) void k_do_to_sh(const float* __restrict__ D,
@@ -150,6 +215,14 @@ void launch_sh_dot(hipStream_t st, const float* S, const float* Yd, float* raw, 
                    int ld, int ldd, int npts, int ndir) {
   dim3 grid(ndp / TN, mp / TM);
   k_sh_to_do<false><<<grid, 256, 0, st>>>(S, Yd, nullptr, raw, ld, ldd, npts, ndir);
+}
+
+void launch_sh_to_do_perm(hipStream_t st, const float* S, const float* Yc, const float* ext,
+                          const int32_t* oct_perm, const int4* tiles, int ntiles, float* dosrc,
+                          int mp, int ld, int ldp) {
+  if (ntiles <= 0) return;
+  dim3 grid(mp / TM, ntiles);
+  k_sh_to_do_perm<<<grid, 256, 0, st>>>(S, Yc, ext, oct_perm, tiles, dosrc, ld, ldp);
 }
 
 void launch_do_to_sh(hipStream_t st, const float* dorad, const float* Wc, const int* nr, float* R,

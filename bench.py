@@ -9,10 +9,11 @@ Rayleigh species.  A step is one converged solve: initial radiance and source
 (grid, optics, direct beam, two-stream start, sweep plan) and the upload to
 HBM happen before the timed regions and are reported as ``setup_ms``.
 
-plain: one solver, solves back to back (--steps of them); served: --workers
-solvers, each with its own HIP stream and host thread, --steps solves each,
-all in flight together.  ``value`` is the served rate; the plain rate and the
-per-solve latency percentiles of both modes are in the same line.
+plain: one solver, --steps solves back to back; served: the same --steps
+solves spread over --workers solvers, each with its own HIP stream and host
+thread, all in flight together.  ``value`` is the served rate (--steps
+converged solves over the served wall time); the plain rate and the per-solve
+latency percentiles of both modes are in the same line.
 
 Prints one JSON line.  ``--gpus N`` with N > 1 exits: this engine runs on one GPU.
 """
@@ -104,7 +105,8 @@ def main():
     ms_per_step = elapsed * 1e3 / args.steps
     iters = iters_total // args.steps
 
-    # served: several solvers in flight, one stream and host thread each
+    # served: the same number of solves over several solvers in flight, one
+    # stream and host thread each, each taking the next solve when it is free
     import threading
     servers = [rte] + [S.make_rte(w) for _ in range(max(args.workers, 1) - 1)]
     for r in servers[1:]:
@@ -112,9 +114,14 @@ def main():
     torch.cuda.synchronize()
     served_lat = []
     lock = threading.Lock()
+    todo = [args.steps]
 
     def work(r):
-        for _ in range(args.steps):
+        while True:
+            with lock:
+                if todo[0] <= 0:
+                    return
+                todo[0] -= 1
             ts = time.perf_counter()
             r.solve(w.maxiter, verbose=False)
             with lock:
@@ -129,7 +136,7 @@ def main():
         t.join()
     torch.cuda.synchronize()
     served_elapsed = time.perf_counter() - ts0
-    served_n = args.steps * len(servers)
+    served_n = args.steps
     served_value = served_n / served_elapsed
 
     npts, nang, nlm = rte._npts, rte._nang, rte._nlm
@@ -138,14 +145,17 @@ def main():
     kern = {}
     n_it = max(iters_total, 1)
     # one sweep pass = one launch per hemisphere (a workgroup per ordinate walking
-    # the octant's dependency levels); per (ordinate, point): read source*extinction
-    # and write the radiance (8 B), plus the extinction once per hemisphere
-    sweep_bytes = nang * npts * 8.0 + 2 * npts * 4.0
+    # the octant's dependency levels); algorithmic bytes per (ordinate, point):
+    # read the DO source and the extinction, write the DO radiance (3 x 4 B)
+    sweep_bytes = nang * npts * 12.0
     ms_pass = ksum["sweep"] / n_it
     kern["sweep"] = {"ms_per_pass": ms_pass, "launches_per_pass": 2,
                      "dependency_levels": sweep_levels,
                      "ms_per_launch": ms_pass / 2, "bytes_per_launch": sweep_bytes / 2,
-                     "GB/s": sweep_bytes / ms_pass / 1e6}
+                     "GB/s": sweep_bytes / ms_pass / 1e6,
+                     "level_ordered": bool(rte.timing("sweep_ring")),
+                     "lds_bytes": int(rte.timing("sweep_ring_lds")),
+                     "multi_cell_points": int(rte.timing("sweep_general_points"))}
     for k in ("sh_to_do", "do_to_sh"):
         ms = ksum[k] / n_it
         kern[k] = {"ms_per_launch": ms, "flops_per_launch": gemm_flops,

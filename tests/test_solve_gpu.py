@@ -97,15 +97,44 @@ def test_converged_solution_matches_reference(ref, make):
 @pytest.mark.parametrize("make", [lambda: blob8("periodic"), lambda: blob8("open"), cloud_small],
                          ids=["blob8_periodic", "blob8_open", "cloud12_two_species"])
 def test_fast_sweep_path_bit_identical_to_general_trace(make):
-    """The analytic one-cell sweep path must reproduce the general trace exactly."""
+    """The point-ordered sweep's analytic one-cell path must reproduce the
+    general trace exactly."""
     w = make()
     a = S.make_rte(w)
+    a.set_option("sweep_ring", 0)
     a.solve(3, verbose=False)
     b = S.make_rte(w)
     b.set_option("sweep_general", 1)
     b.solve(3, verbose=False)
     assert np.array_equal(a.state_array("do_radiance"), b.state_array("do_radiance"))
     assert np.array_equal(a.state_array("source"), b.state_array("source"))
+
+
+def _tall_cloud():
+    # 64 x 64 x 64: the widest levels hold about 3000 points, more than the ring
+    # sweep's register-prefetched slots (5 x 512), so its overflow loop runs
+    return S.cloud_field(nx=64, ny=64, nz=64, dx=0.02, dz=0.015, z0=0.2, seed=3, nmu=8, nphi=16)
+
+
+@pytest.mark.parametrize("make", [lambda: blob8("periodic"), lambda: blob8("open"), cloud_small, _tall_cloud],
+                         ids=["blob8_periodic", "blob8_open", "cloud12_two_species", "cloud64x64x64"])
+def test_level_ordered_sweep_matches_point_ordered_sweep(make):
+    """The LDS-ring sweep over per-solve records (FP32 coefficients, level-ordered
+    DO source) against the point-ordered FP64-geometry sweep: after three
+    iterations the DO radiances and SH sources agree to rel L2 <= 2e-6 and the
+    DO source read back by point agrees likewise."""
+    w = make()
+    a = S.make_rte(w)
+    a.solve(3, verbose=False)
+    assert a.timing("sweep_ring") == 1.0
+    b = S.make_rte(w)
+    b.set_option("sweep_ring", 0)
+    b.solve(3, verbose=False)
+    assert b.timing("sweep_ring") == 0.0
+    for name in ("do_radiance", "do_source", "source"):
+        assert rel_l2(a.state_array(name), b.state_array(name)) < 2e-6, name
+    if w.bc == "periodic":
+        assert a.timing("sweep_general_points") > 0   # seams take the general trace
 
 
 def test_sweep_graph_replay_bit_identical_to_direct_launches():
