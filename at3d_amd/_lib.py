@@ -81,6 +81,10 @@ def lib():
         L.at3d_levis_gradient.argtypes = [vp, C.POINTER(DerivDesc), C.c_int64, vp, vp, vp, vp, vp,
                                           C.c_int64, vp, vp, vp, vp, vp, C.c_char_p, C.c_int32,
                                           vp, vp, vp, C.c_char_p, C.c_size_t]
+        L.at3d_levis_gradient_jacobian.argtypes = [vp, C.POINTER(DerivDesc), C.c_int64, vp, vp, vp, vp,
+                                                   vp, C.c_int64, vp, vp, vp, vp, vp, C.c_char_p,
+                                                   C.c_int32, C.c_int32, vp, vp, vp, vp, vp,
+                                                   C.c_char_p, C.c_size_t]
         L.at3d_state_timing.argtypes = [vp, C.c_char_p]
         L.at3d_state_timing.restype = C.c_double
         _lib = L
@@ -100,7 +104,8 @@ def errbuf():
 EXPORTS = ["at3d_version", "at3d_plan_create", "at3d_plan_destroy", "at3d_plan_size",
            "at3d_plan_get", "at3d_state_create", "at3d_state_destroy", "at3d_init_solution",
            "at3d_solve", "at3d_state_get", "at3d_state_set_option", "at3d_state_timing",
-           "at3d_render", "at3d_levis_gradient", "at3d_init_solution_from"]
+           "at3d_render", "at3d_levis_gradient", "at3d_levis_gradient_jacobian",
+           "at3d_init_solution_from"]
 
 
 def as_f32(a):

@@ -344,16 +344,17 @@ int at3d_render(at3d_state* st, int64_t nrays, const float* x, const float* y, c
   return rc;
 }
 
-int at3d_levis_gradient(at3d_state* st, const at3d_deriv_desc* deriv, int64_t nrays,
-                        const float* x, const float* y, const float* z, const double* mu,
-                        const double* phi, int64_t npix, const int32_t* rays_per_pixel,
-                        const double* ray_weights, const double* stokes_weights,
-                        const float* measurements, const double* uncertainties,
-                        const char* costfunc, int32_t exact_single_scatter, double* grad,
-                        double* cost, float* pixels, char* err, size_t errlen) {
+int at3d_levis_gradient_jacobian(at3d_state* st, const at3d_deriv_desc* deriv, int64_t nrays,
+                                 const float* x, const float* y, const float* z, const double* mu,
+                                 const double* phi, int64_t npix, const int32_t* rays_per_pixel,
+                                 const double* ray_weights, const double* stokes_weights,
+                                 const float* measurements, const double* uncertainties,
+                                 const char* costfunc, int32_t exact_single_scatter, int32_t njac,
+                                 const int32_t* jacobian_ptr, float* jacobian, double* grad,
+                                 double* cost, float* pixels, char* err, size_t errlen) {
   if (!st || !deriv || !x || !y || !z || !mu || !phi || !rays_per_pixel || !ray_weights ||
       !stokes_weights || !measurements || !uncertainties || !grad || !cost || !pixels ||
-      !deriv->partder || !deriv->dext || !deriv->dalb) {
+      !deriv->partder || !deriv->dext || !deriv->dalb || (njac > 0 && (!jacobian_ptr || !jacobian))) {
     set_err(err, errlen, "null argument");
     return AT3D_ERR_ARG;
   }
@@ -362,13 +363,27 @@ int at3d_levis_gradient(at3d_state* st, const at3d_deriv_desc* deriv, int64_t nr
   try {
     rc = at3d::state_levis_gradient(st->s, *deriv, nrays, x, y, z, mu, phi, npix, rays_per_pixel,
                                     ray_weights, stokes_weights, measurements, uncertainties,
-                                    costfunc, exact_single_scatter, grad, cost, pixels, msg);
+                                    costfunc, exact_single_scatter, njac, jacobian_ptr, jacobian,
+                                    grad, cost, pixels, msg);
   } catch (const std::bad_alloc&) {
     rc = AT3D_ERR_MEMORY;
     msg = "out of host memory in the gradient";
   }
   if (rc) set_err(err, errlen, msg);
   return rc;
+}
+
+int at3d_levis_gradient(at3d_state* st, const at3d_deriv_desc* deriv, int64_t nrays,
+                        const float* x, const float* y, const float* z, const double* mu,
+                        const double* phi, int64_t npix, const int32_t* rays_per_pixel,
+                        const double* ray_weights, const double* stokes_weights,
+                        const float* measurements, const double* uncertainties,
+                        const char* costfunc, int32_t exact_single_scatter, double* grad,
+                        double* cost, float* pixels, char* err, size_t errlen) {
+  return at3d_levis_gradient_jacobian(st, deriv, nrays, x, y, z, mu, phi, npix, rays_per_pixel,
+                                      ray_weights, stokes_weights, measurements, uncertainties,
+                                      costfunc, exact_single_scatter, 0, nullptr, nullptr, grad,
+                                      cost, pixels, err, errlen);
 }
 
 int at3d_state_set_option(at3d_state* st, const char* key, int32_t value, char* err,

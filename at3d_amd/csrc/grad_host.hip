@@ -1,6 +1,9 @@
-// Host orchestration of the Levis gradient (LEVISAPPROX_GRADIENT adjoint
-// branch, shdomsub4.f:288-809, as driven by
-// LevisApproxGradient.levis_approximation_grad, at3d/gradient.py:164-415).
+// Host orchestration of the Levis gradient (LEVISAPPROX_GRADIENT,
+// shdomsub4.f:288-809, as driven by
+// LevisApproxGradient.levis_approximation_grad, at3d/gradient.py:164-415):
+// the adjoint branch for the gradient and cost, and the Jacobian branch's
+// per-pixel derivatives at requested property points (:378-467) from the
+// same ray traversal.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -120,8 +123,9 @@ int state_levis_gradient(State& s, const at3d_deriv_desc& dd, int64_t nrays, con
                          const float* y, const float* z, const double* mu, const double* phi,
                          int64_t npix, const int32_t* rays_per_pixel, const double* ray_weights,
                          const double* stokes_weights, const float* measurements,
-                         const double* uncertainties, const char* costfunc, int ess, double* grad,
-                         double* cost, float* pixels, std::string& err) {
+                         const double* uncertainties, const char* costfunc, int ess, int32_t njac,
+                         const int32_t* jacptr, float* jacobian, double* grad, double* cost,
+                         float* pixels, std::string& err) {
   const Plan& p = *s.plan;
   if (!s.initialized) {
     err = "the solution is not initialised: solve before computing a gradient";
@@ -178,6 +182,57 @@ int state_levis_gradient(State& s, const at3d_deriv_desc& dd, int64_t nrays, con
   {
     const int rc = build_grad_plan(p, dd, g, err);
     if (rc) return rc;
+  }
+  // ---- Jacobian columns: the grid points whose interpolation kernel or sun
+  // path reaches one of the requested property points ----
+  if (njac < 0 || (njac > 0 && (!jacptr || !jacobian))) {
+    err = "Jacobian: need the property point list and an output array";
+    return AT3D_ERR_ARG;
+  }
+  std::vector<int32_t> rel_s, rel_b, p_of_rs, js_off(1, 0), js_r, js_nb, jb_off(1, 0), jb_r;
+  std::vector<float> jb_w;
+  int nrs = 0, nrb = 0;
+  if (njac > 0) {
+    for (int ji = 0; ji < njac; ji++)
+      if (jacptr[ji] < 0 || jacptr[ji] >= g.maxpg) {
+        err = "Jacobian: property point index out of range";
+        return AT3D_ERR_ARG;
+      }
+    std::vector<std::vector<int32_t>> cols_of_ib(g.maxpg);
+    for (int ji = 0; ji < njac; ji++) cols_of_ib[jacptr[ji]].push_back(ji);
+    rel_s.assign(p.npts, -1);
+    rel_b.assign(p.npts, -1);
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> src_of(njac);
+    for (int ip = 0; ip < p.npts; ip++)
+      for (int nb = 0; nb < 8; nb++)
+        for (int ji : cols_of_ib[g.interpptr[(size_t)ip * 8 + nb]]) {
+          if (rel_s[ip] < 0) {
+            rel_s[ip] = nrs++;
+            p_of_rs.push_back(ip);
+          }
+          src_of[ji].push_back({rel_s[ip], nb});
+        }
+    for (int ji = 0; ji < njac; ji++) {
+      for (auto& e : src_of[ji]) {
+        js_r.push_back(e.first);
+        js_nb.push_back(e.second);
+      }
+      js_off.push_back((int32_t)js_r.size());
+      const int ib = jacptr[ji];
+      for (int32_t e = g.bp_off[ib]; e < g.bp_off[ib + 1]; e++) {
+        const int ip = g.bp_ip[e];
+        if (rel_b[ip] < 0) rel_b[ip] = nrb++;
+        jb_r.push_back(rel_b[ip]);
+        jb_w.push_back(g.bp_w[e]);
+      }
+      jb_off.push_back((int32_t)jb_r.size());
+    }
+    const double bytes = (double)nrays * (2.0 * nrs + nrb) * 8.0;
+    if (bytes > 16.0e9) {
+      err = "Jacobian: per-ray weights at the grid points near the requested property points "
+            "would need more than 16 GB; request fewer points or rays per call";
+      return AT3D_ERR_ARG;
+    }
   }
   SpeciesTables sp;
   species_tables(p, g, sp);
@@ -247,6 +302,17 @@ int state_levis_gradient(State& s, const at3d_deriv_desc& dd, int64_t nrays, con
   const size_t o_bpo = carve(g.bp_off.size() * 4), o_bpi = carve(g.bp_ip.size() * 4);
   const size_t o_bpw = carve(g.bp_w.size() * 4), o_alb = carve(p.albedo.size() * 4);
   const size_t o_pd = carve(partder.size() * 4), o_dx = carve(doexact.size() * 4);
+  // Jacobian
+  const size_t o_jw = carve(njac > 0 ? nr * 8 : 0), o_rels = carve(rel_s.size() * 4);
+  const size_t o_relb = carve(rel_b.size() * 4), o_prs = carve(p_of_rs.size() * 4);
+  const size_t o_js = carve(nr * nrs * 8), o_jr = carve(nr * nrs * 8), o_jb = carve(nr * nrb * 8);
+  const size_t o_jso = carve(js_off.size() * 4), o_jsr = carve(js_r.size() * 4);
+  const size_t o_jsn = carve(js_nb.size() * 4), o_jbo = carve(jb_off.size() * 4);
+  const size_t o_jbr = carve(jb_r.size() * 4), o_jbw = carve(jb_w.size() * 4);
+  const size_t o_g8t = carve((size_t)nrs * ndir * numder * 8 * 4);
+  const size_t o_roff = carve(njac > 0 ? ((size_t)npix + 1) * 4 : 0);
+  const size_t o_jib = carve((size_t)std::max(njac, 0) * 4);
+  const size_t o_jac = carve(njac > 0 ? (size_t)npix * njac * numder * 4 : 0);
   if (off > s.gbuf_bytes) {
     std::lock_guard<std::mutex> lock(device_mutex());
     if (s.gbuf) (void)hipFree(s.gbuf);
@@ -299,6 +365,25 @@ int state_levis_gradient(State& s, const at3d_deriv_desc& dd, int64_t nrays, con
   HIPCHK(up(o_alb, p.albedo.data(), p.albedo.size() * 4));
   HIPCHK(up(o_pd, partder.data(), partder.size() * 4));
   HIPCHK(up(o_dx, doexact.data(), doexact.size() * 4));
+  std::vector<int32_t> ray_off;
+  if (njac > 0) {
+    HIPCHK(up(o_rels, rel_s.data(), rel_s.size() * 4));
+    HIPCHK(up(o_relb, rel_b.data(), rel_b.size() * 4));
+    HIPCHK(up(o_prs, p_of_rs.data(), p_of_rs.size() * 4));
+    HIPCHK(up(o_jso, js_off.data(), js_off.size() * 4));
+    HIPCHK(up(o_jsr, js_r.data(), js_r.size() * 4));
+    HIPCHK(up(o_jsn, js_nb.data(), js_nb.size() * 4));
+    HIPCHK(up(o_jbo, jb_off.data(), jb_off.size() * 4));
+    HIPCHK(up(o_jbr, jb_r.data(), jb_r.size() * 4));
+    HIPCHK(up(o_jbw, jb_w.data(), jb_w.size() * 4));
+    ray_off.assign(npix + 1, 0);
+    for (int64_t i = 0; i < npix; i++) ray_off[i + 1] = ray_off[i] + rays_per_pixel[i];
+    HIPCHK(up(o_roff, ray_off.data(), ray_off.size() * 4));
+    HIPCHK(up(o_jib, jacptr, (size_t)njac * 4));
+    HIPCHK(hipMemsetAsync(B + o_js, 0, nr * nrs * 8, s.stream));
+    HIPCHK(hipMemsetAsync(B + o_jr, 0, nr * nrs * 8, s.stream));
+    HIPCHK(hipMemsetAsync(B + o_jb, 0, nr * nrb * 8, s.stream));
+  }
   HIPCHK(hipMemsetAsync(s.errflag, 0, sizeof(int), s.stream));
 
   // ---- phase 1: forward radiances (INTEGRATE_1RAY) ----
@@ -359,7 +444,7 @@ int state_levis_gradient(State& s, const at3d_deriv_desc& dd, int64_t nrays, con
   }
 
   // ---- phase 2: pixels, cost and adjoint weights (COMPUTE_ADJOINT_WEIGHTS) ----
-  std::vector<double> adjw_pix(npix, 0.0), adjw_ray(nr, 0.0);
+  std::vector<double> adjw_pix(npix, 0.0), adjw_ray(nr, 0.0), jacw_ray(njac > 0 ? nr : 0, 0.0);
   double c = 0.0;
   {
     int64_t iray = 0;
@@ -381,13 +466,16 @@ int state_levis_gradient(State& s, const at3d_deriv_desc& dd, int64_t nrays, con
     }
     iray = 0;
     for (int64_t ip = 0; ip < npix; ip++)
-      for (int k = 0; k < rays_per_pixel[ip]; k++, iray++)
+      for (int k = 0; k < rays_per_pixel[ip]; k++, iray++) {
         adjw_ray[iray] = adjw_pix[ip] * ray_weights[iray] * stokes_weights[ip];
+        if (njac > 0) jacw_ray[iray] = ray_weights[iray] * stokes_weights[ip];
+      }
   }
   *cost = c;
 
   // ---- phase 3: adjoint traversal and contraction ----
   HIPCHK(up(o_adj, adjw_ray.data(), nr * 8));
+  if (njac > 0) HIPCHK(up(o_jw, jacw_ray.data(), nr * 8));
   HIPCHK(hipMemsetAsync(B + o_bsrc, 0, tab * 8, s.stream));
   HIPCHK(hipMemsetAsync(B + o_erad, 0, (size_t)npts * 8, s.stream));
   HIPCHK(hipMemsetAsync(B + o_bw, 0, (size_t)npts * 8, s.stream));
@@ -403,6 +491,14 @@ int state_levis_gradient(State& s, const at3d_deriv_desc& dd, int64_t nrays, con
   A.bsrc = reinterpret_cast<double*>(B + o_bsrc);
   A.erad = reinterpret_cast<double*>(B + o_erad);
   A.beamw = reinterpret_cast<double*>(B + o_bw);
+  A.jac_weight = njac > 0 ? reinterpret_cast<double*>(B + o_jw) : nullptr;
+  A.rel_s = reinterpret_cast<int32_t*>(B + o_rels);
+  A.rel_b = reinterpret_cast<int32_t*>(B + o_relb);
+  A.nrel_s = nrs;
+  A.nrel_b = nrb;
+  A.jsrc = reinterpret_cast<double*>(B + o_js);
+  A.jrad = reinterpret_cast<double*>(B + o_jr);
+  A.jbeam = reinterpret_cast<double*>(B + o_jb);
   launch_adjoint_rays(s.stream, A);
   GradPointArgs G;
   G.npts = npts;
@@ -458,9 +554,41 @@ int state_levis_gradient(State& s, const at3d_deriv_desc& dd, int64_t nrays, con
   G.bp_w = reinterpret_cast<float*>(B + o_bpw);
   launch_grad_points(s.stream, G);
   if (ess) launch_grad_beam(s.stream, G);
+  if (njac > 0) {
+    JacArgs J;
+    J.npix = (int)npix;
+    J.njac = njac;
+    J.numder = numder;
+    J.ndir = ndir;
+    J.maxpg = maxpg;
+    J.nrel_s = nrs;
+    J.nrel_b = nrb;
+    J.ess = ess;
+    J.ray_off = reinterpret_cast<int32_t*>(B + o_roff);
+    J.dir_of_ray = a.dir_of_ray;
+    J.jac_ib = reinterpret_cast<int32_t*>(B + o_jib);
+    J.js_off = reinterpret_cast<int32_t*>(B + o_jso);
+    J.js_r = reinterpret_cast<int32_t*>(B + o_jsr);
+    J.js_nb = reinterpret_cast<int32_t*>(B + o_jsn);
+    J.jb_off = reinterpret_cast<int32_t*>(B + o_jbo);
+    J.jb_r = reinterpret_cast<int32_t*>(B + o_jbr);
+    J.jb_w = reinterpret_cast<float*>(B + o_jbw);
+    J.p_of_rs = reinterpret_cast<int32_t*>(B + o_prs);
+    J.g8t = reinterpret_cast<float*>(B + o_g8t);
+    J.dextm = G.dextm;
+    J.optinterpwt = G.optinterpwt;
+    J.jsrc = A.jsrc;
+    J.jrad = A.jrad;
+    J.jbeam = A.jbeam;
+    J.jacobian = reinterpret_cast<float*>(B + o_jac);
+    launch_jacobian(s.stream, G, J, reinterpret_cast<float*>(B + o_g8t));
+  }
   (void)hipEventRecord(s.ev[7], s.stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(grad, B + o_grad, (size_t)numder * maxpg * 8, hipMemcpyDeviceToHost, s.stream));
+  if (njac > 0)
+    HIPCHK(hipMemcpyAsync(jacobian, B + o_jac, (size_t)npix * njac * numder * 4, hipMemcpyDeviceToHost,
+                          s.stream));
   HIPCHK(hipMemcpyAsync(&ef, s.errflag, sizeof(int), hipMemcpyDeviceToHost, s.stream));
   HIPCHK(hipStreamSynchronize(s.stream));
   float ms = 0.f;

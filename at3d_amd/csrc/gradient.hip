@@ -117,6 +117,11 @@ __device__ bool adjoint_ray(const AdjArgs& A, int ir, int pass, double rtot, dou
   }
   const int d = a.dir_of_ray[ir];
   const double wgt = A.adj_weight[ir];
+  const bool jac = A.jac_weight != nullptr;
+  const double jwt = jac ? A.jac_weight[ir] : 0.0;
+  double* jsrc = jac ? A.jsrc + (size_t)ir * A.nrel_s : nullptr;
+  double* jrad = jac ? A.jrad + (size_t)ir * A.nrel_s : nullptr;
+  double* jbeam = jac ? A.jbeam + (size_t)ir * A.nrel_b : nullptr;
   const float topv = a.topval[ir];
   const size_t LD = a.ldd;
   const double eps = (double)(1.0e-5f * (P[a.g.gridptr[7] * 3 + 2] - P[a.g.gridptr[0] * 3 + 2]));
@@ -159,14 +164,20 @@ __device__ bool adjoint_ray(const AdjArgs& A, int ir, int pass, double rtot, dou
     if (ext == 0.0) return;
     const double rad_k = (rtot - (prefix - pv_f)) / pv_t;   // PASSEDRAD(k)
     const double rad_k1 = future_next;                      // PASSEDRAD(k+1)
+    auto term = [&](double w, int k) {
+      const double g0 = -w * rad_k1 * pv_i0[k];
+      const double g1 = -w * rad_k * pv_i1[k];
+      const double ag = ((double)0.5f * (g0 + g1) +
+                         (double)0.08333333333f * (ext0 * g1 - ext1 * g0) * pv_dels *
+                             (1.0 - (double)0.05f * (ext1 - ext0) * pv_dels)) / ext;
+      return ag * pv_t * pv_abs;
+    };
     for (int k = 0; k < 8; k++) {
-      const double g0 = -wgt * rad_k1 * pv_i0[k];
-      const double g1 = -wgt * rad_k * pv_i1[k];
-      double ag = ((double)0.5f * (g0 + g1) +
-                   (double)0.08333333333f * (ext0 * g1 - ext1 * g0) * pv_dels *
-                       (1.0 - (double)0.05f * (ext1 - ext0) * pv_dels)) / ext;
-      ag = ag * pv_t * pv_abs;
-      atomic_add_f64(&A.erad[pv_pts[k]], ag);
+      atomic_add_f64(&A.erad[pv_pts[k]], term(wgt, k));
+      if (jac) {
+        const int r = A.rel_s[pv_pts[k]];
+        if (r >= 0) jrad[r] += term(jwt, k);
+      }
     }
   };
   float e8[8], s8[8], ss8[8];
@@ -285,6 +296,12 @@ __device__ bool adjoint_ray(const AdjArgs& A, int ir, int pass, double rtot, dou
       for (int n = 0; n < 8; n++) {
         atomic_add_f64(&A.bsrc[(size_t)pts[n] * LD + d], wgt * beta[n]);
         if (ess) atomic_add_f64(&A.beamw[pts[n]], wgt * sss[n]);
+        if (jac) {
+          const int r = A.rel_s[pts[n]];
+          if (r >= 0) jsrc[r] += jwt * beta[n];
+          const int rb = A.rel_b[pts[n]];
+          if (ess && rb >= 0) jbeam[rb] += jwt * sss[n];
+        }
       }
     }
     int iface, jface;
@@ -350,7 +367,13 @@ __device__ bool adjoint_ray(const AdjArgs& A, int ir, int pass, double rtot, dou
       radout = radout + transmit * (double)radbnd;
       rend = (double)radbnd;
       if (pass == 1 && ess)
-        for (int j = 0; j < 4; j++) atomic_add_f64(&A.beamw[bp[j]], wgt * transmit * bi[j] * (double)DR[j]);
+        for (int j = 0; j < 4; j++) {
+          atomic_add_f64(&A.beamw[bp[j]], wgt * transmit * bi[j] * (double)DR[j]);
+          if (jac) {
+            const int rb = A.rel_b[bp[j]];
+            if (rb >= 0) jbeam[rb] += jwt * transmit * bi[j] * (double)DR[j];
+          }
+        }
     } else {
       icell = inext;
     }
@@ -365,7 +388,7 @@ __device__ bool adjoint_ray(const AdjArgs& A, int ir, int pass, double rtot, dou
 __global__ __launch_bounds__(128) void k_adjoint_rays(AdjArgs A) {
   const int ir = blockIdx.x * blockDim.x + threadIdx.x;
   if (ir >= A.r.nrays) return;
-  if (A.adj_weight[ir] == 0.0) return;
+  if (A.adj_weight[ir] == 0.0 && (A.jac_weight == nullptr || A.jac_weight[ir] == 0.0)) return;
   double rtot = 0.0, r1 = 0.0;
   if (!adjoint_ray(A, ir, 0, 0.0, rtot)) return;
   adjoint_ray(A, ir, 1, rtot, r1);
@@ -374,12 +397,8 @@ __global__ __launch_bounds__(128) void k_adjoint_rays(AdjArgs A) {
 // Contraction of the per-(point, direction) weights with GRAD8
 // (COMPUTE_SOURCE_GRAD_1CELL gradient part, shdomsub4.f:1786-2013), for a
 // solar source with delta-M and one Stokes component.
-__global__ void k_grad_points(GradPointArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.npts * a.ndir) return;
-  const int p = i / a.ndir, d = i - p * a.ndir;
-  const double b = a.bsrc[(size_t)p * a.ldd + d];
-  if (b == 0.0) return;
+template <typename Emit>
+__device__ void grad8_point(const GradPointArgs& a, int p, int d, Emit&& emit) {
   const int ML = a.ml, nl1 = ML + 1;
   float ry[AT3D_MAX_L + 1];
   for (int l = 0; l <= ML; l++) ry[l] = 0.0f;
@@ -437,9 +456,62 @@ __global__ void k_grad_points(GradPointArgs a) {
       g8 = g8 + dirf * secmu0 * xi *
                     (singscatj * dfj + dsingscatp * ep * ap + dalb * (singscatp - singscatj) * ep +
                      dext * (singscatp - singscatj) * ap);
-      atomic_add_f64(&a.grad[db], b * (double)g8);
+      emit(idr, nb, db, g8);
     }
   }
+}
+
+// GRAD8 of (point, direction) times the adjoint source weight, into the gradient.
+__global__ void k_grad_points(GradPointArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.npts * a.ndir) return;
+  const int p = i / a.ndir, d = i - p * a.ndir;
+  const double b = a.bsrc[(size_t)p * a.ldd + d];
+  if (b == 0.0) return;
+  grad8_point(a, p, d, [&](int, int, size_t db, float g8) { atomic_add_f64(&a.grad[db], b * (double)g8); });
+}
+
+// GRAD8 terms of every Jacobian source slot in every ray direction.
+__global__ void k_jac_g8(GradPointArgs a, JacArgs j, float* g8t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= j.nrel_s * a.ndir) return;
+  const int r = i / a.ndir, d = i - r * a.ndir;
+  float* o = g8t + (size_t)i * a.numder * 8;
+  for (int q = 0; q < a.numder * 8; q++) o[q] = 0.0f;
+  grad8_point(a, j.p_of_rs[r], d, [&](int idr, int nb, size_t, float g8) { o[idr * 8 + nb] = g8; });
+}
+
+// JACOBIAN(:, idr, ji, pixel) = RAYGRAD_PIXEL at the column's property point
+// (LEVISAPPROX_GRADIENT shdomsub4.f:378-467): per ray of the pixel, the source
+// weights x GRAD8, the radiance weights x DEXTM x OPTINTERPWT and minus DEXTM x
+// the sun-path weights x the beam weights.  One thread per (pixel, column, derivative).
+__global__ void k_jacobian(JacArgs j) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)j.npix * j.njac * j.numder) return;
+  const int idr = (int)(i % j.numder);
+  const int ji = (int)((i / j.numder) % j.njac);
+  const int pix = (int)(i / ((int64_t)j.numder * j.njac));
+  const int ib = j.jac_ib[ji];
+  const float dm = j.dextm[(size_t)idr * j.maxpg + ib];
+  double acc = 0.0;
+  for (int ray = j.ray_off[pix]; ray < j.ray_off[pix + 1]; ray++) {
+    const int d = j.dir_of_ray[ray];
+    const double* js = j.jsrc + (size_t)ray * j.nrel_s;
+    const double* jr = j.jrad + (size_t)ray * j.nrel_s;
+    for (int q = j.js_off[ji]; q < j.js_off[ji + 1]; q++) {
+      const int r = j.js_r[q], nb = j.js_nb[q];
+      const float g8 = j.g8t[(((size_t)r * j.ndir + d) * j.numder + idr) * 8 + nb];
+      const float xi = j.optinterpwt[(size_t)j.p_of_rs[r] * 8 + nb];
+      acc = acc + js[r] * (double)g8 + (double)(dm * xi) * jr[r];
+    }
+    if (j.ess) {
+      const double* jb = j.jbeam + (size_t)ray * j.nrel_b;
+      double s = 0.0;
+      for (int q = j.jb_off[ji]; q < j.jb_off[ji + 1]; q++) s = s + (double)j.jb_w[q] * jb[j.jb_r[q]];
+      acc = acc - (double)dm * s;
+    }
+  }
+  j.jacobian[i] = (float)acc;
 }
 
 // Radiance term: GRADOUT(ib) += DEXTM(ib) * OPTINTERPWT * sum of the adjoint
@@ -498,6 +570,13 @@ void launch_grad_points(hipStream_t st, const GradPointArgs& a) {
 
 void launch_grad_beam(hipStream_t st, const GradPointArgs& a) {
   k_grad_beam<<<(a.maxpg + 255) / 256, 256, 0, st>>>(a);
+}
+
+void launch_jacobian(hipStream_t st, const GradPointArgs& g, const JacArgs& j, float* g8t) {
+  const int n = j.nrel_s * g.ndir;
+  if (n > 0) k_jac_g8<<<(n + 127) / 128, 128, 0, st>>>(g, j, g8t);
+  const int64_t m = (int64_t)j.npix * j.njac * j.numder;
+  if (m > 0) k_jacobian<<<(unsigned)((m + 255) / 256), 256, 0, st>>>(j);
 }
 
 }  // namespace at3d

@@ -181,6 +181,16 @@ struct AdjArgs {
   double* bsrc;                // [npts][ldd] source-gradient weights
   double* erad;                // [npts] radiance-term weights
   double* beamw;               // [npts] direct-beam weights
+  // Jacobian rows (GRAD_INTEGRATE_1RAY): per ray, the same three weights with
+  // the ray x Stokes weight alone, kept only at the grid points whose
+  // interpolation kernel (rel_s) or sun path (rel_b) reaches a Jacobian point
+  const double* jac_weight;    // [nrays] or null
+  const int32_t* rel_s;        // [npts] slot or -1
+  const int32_t* rel_b;        // [npts] slot or -1
+  int nrel_s, nrel_b;
+  double* jsrc;                // [nrays][nrel_s]
+  double* jrad;                // [nrays][nrel_s]
+  double* jbeam;               // [nrays][nrel_b]
 };
 
 struct GradPointArgs {
@@ -233,7 +243,30 @@ void launch_dir_source_grad(hipStream_t st, const float* raw, int ldd, int ndir,
                             const int32_t* ss_off, const int32_t* ss_iph, const float* ss_coef,
                             const float* ssd, int numphase, int ml, int npts, int single_scatter,
                             float* dsrc, float* srcpre, float* ss8);
+struct JacArgs {
+  int npix, njac, numder, ndir, maxpg, nrel_s, nrel_b, ess;
+  const int32_t* ray_off;      // [npix+1] first ray of each pixel
+  const int32_t* dir_of_ray;   // [nrays]
+  const int32_t* jac_ib;       // [njac] property point of each Jacobian column
+  const int32_t* js_off;       // [njac+1] (source slot, neighbour) pairs of each column
+  const int32_t* js_r;
+  const int32_t* js_nb;
+  const int32_t* jb_off;       // [njac+1] (beam slot, d tau / d ext) pairs of each column
+  const int32_t* jb_r;
+  const float* jb_w;
+  const int32_t* p_of_rs;      // [nrel_s] grid point of each source slot
+  const float* g8t;            // [nrel_s][ndir][numder][8] GRAD8 terms
+  const float* dextm;          // [numder][maxpg]
+  const float* optinterpwt;    // [npts][8]
+  const double* jsrc;
+  const double* jrad;
+  const double* jbeam;
+  float* jacobian;             // [npix][njac][numder]
+};
+
 void launch_adjoint_rays(hipStream_t st, const AdjArgs& a);
+// GRAD8 of every source slot in every ray direction, then the Jacobian.
+void launch_jacobian(hipStream_t st, const GradPointArgs& g, const JacArgs& j, float* g8t);
 void launch_grad_points(hipStream_t st, const GradPointArgs& a);
 void launch_grad_beam(hipStream_t st, const GradPointArgs& a);
 

@@ -108,8 +108,9 @@ int state_levis_gradient(State& s, const at3d_deriv_desc& dd, int64_t nrays, con
                          const float* y, const float* z, const double* mu, const double* phi,
                          int64_t npix, const int32_t* rays_per_pixel, const double* ray_weights,
                          const double* stokes_weights, const float* measurements,
-                         const double* uncertainties, const char* costfunc, int ess, double* grad,
-                         double* cost, float* pixels, std::string& err);
+                         const double* uncertainties, const char* costfunc, int ess, int32_t njac,
+                         const int32_t* jacptr, float* jacobian, double* grad, double* cost,
+                         float* pixels, std::string& err);
 
 int state_render(State& s, int64_t nrays, const float* x, const float* y, const float* z,
                  const double* mu, const double* phi, int single_scatter, int nosurface,

@@ -336,7 +336,7 @@ __device__ inline void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-constexpr int RING_K = 5;  // points per thread per level with records loaded a level ahead
+constexpr int RING_K = 5;  // points per thread per level with records in registers
 
 struct RingSet {
   uint4 w0[RING_K], w1[RING_K];
@@ -400,7 +400,7 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
   // Records of a level, loaded unconditionally (indices clamped into the
   // octant's records) so the loads of every lane are always in flight.
   auto load = [&](int L, RingSet& S) {
-    const int beg = __builtin_amdgcn_readfirstlane(off[L - 1]);
+    const int beg = __builtin_amdgcn_readfirstlane(off[min(L - 1, nlev)]);
 #pragma unroll
     for (int k = 0; k < RING_K; k++) {
       const int jl = min(beg + tid + k * RING_THREADS, nsw - 1);
@@ -409,8 +409,10 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
       S.s[k] = sl[jl];
     }
   };
-  auto step = [&](int L, RingSet& cur, RingSet& nxt) {
-    load(L + 1, nxt);  // past the last level this re-reads the last record
+  // Level L from `cur` while level L+2 streams into `fut` (its set was
+  // consumed by level L-1): every record has two levels' time to arrive.
+  auto step = [&](int L, RingSet& cur, RingSet& fut) {
+    load(L + 2, fut);  // past the last level this re-reads the last record
     const int beg = __builtin_amdgcn_readfirstlane(off[L - 1]);
     const int end = __builtin_amdgcn_readfirstlane(off[L]);
     const int k0 = beg % C - beg;  // ring slot of local position jl: k0 + jl, wrapped once
@@ -441,15 +443,18 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
     else
       lds_barrier();
   };
-  RingSet A, B;
+  RingSet S0, S1, S2;
   __syncthreads();
-  if (nlev >= 1) load(1, A);
+  load(1, S0);
+  load(2, S1);
   int L = 1;
-  for (; L + 1 <= nlev; L += 2) {
-    step(L, A, B);
-    step(L + 1, B, A);
+  for (; L + 2 <= nlev; L += 3) {
+    step(L, S0, S2);
+    step(L + 1, S1, S0);
+    step(L + 2, S2, S1);
   }
-  if (L == nlev) step(L, A, B);
+  if (L <= nlev) step(L, S0, S2);
+  if (L + 1 <= nlev) step(L + 1, S1, S0);
 }
 
 // Boundary radiances for one hemisphere: sky radiance at the top points for

@@ -81,14 +81,17 @@ def _derivative_tables(rte, unknowns):
 
 
 def levis_approximation_grad(rte_solver, sensor, unknowns, cost_function="L2",
-                             exact_single_scatter=True):
+                             exact_single_scatter=True, indices_for_jacobian=None):
     """Cost and gradient for one solver and one sensor (gradient.py:164-415).
 
     ``sensor`` holds the ray geometry ('ray_x', 'ray_y', 'ray_z', 'ray_mu',
     'ray_phi'), 'measurement_data' per pixel and optionally 'rays_per_pixel',
     'ray_weight', 'stokes_weights' and 'uncertainties' (the per-pixel inverse
     error variance; 1 by default).  Returns (gradient [numder, npx, npy, npz],
-    loss, sensor with the forward radiances in 'I')."""
+    loss, sensor with the forward radiances in 'I').  With
+    ``indices_for_jacobian`` (a tuple of x, y, z index arrays of property grid
+    points, as gradient.py:248-252) a fourth value is returned: the Jacobian
+    [nstokes=1, numder, npoints, npixels] (float32, the reference's layout)."""
     if cost_function not in ("L2", "LL"):
         raise NotImplementedError("`cost_function` '{}' is not valid.".format(cost_function))
     rte_solver.check_solved()
@@ -129,14 +132,26 @@ def levis_approximation_grad(rte_solver, sensor, unknowns, cost_function="L2",
     cost = C.c_double()
     pixels = np.zeros(npix, np.float32)
     err = _lib.errbuf()
-    _lib.check(_lib.lib().at3d_levis_gradient(
+    if indices_for_jacobian is None:
+        jptr = np.zeros(1, np.int32)
+        njac = 0
+    else:
+        jptr = np.ascontiguousarray(np.ravel_multi_index(tuple(indices_for_jacobian), tuple(shape)),
+                                    np.int32).reshape(-1)
+        njac = jptr.size
+    jac = np.zeros((npix, max(njac, 1), numder), np.float32)
+    _lib.check(_lib.lib().at3d_levis_gradient_jacobian(
         rte_solver._state, C.byref(dd), nrays, x.ctypes.data, y.ctypes.data, z.ctypes.data,
         mu.ctypes.data, phi.ctypes.data, npix, rpp.ctypes.data, rw.ctypes.data, sw.ctypes.data,
         meas.ctypes.data, unc.ctypes.data, cost_function.encode(), int(bool(exact_single_scatter)),
-        grad.ctypes.data, C.byref(cost), pixels.ctypes.data, err, len(err)), err)
+        njac, jptr.ctypes.data, jac.ctypes.data, grad.ctypes.data, C.byref(cost),
+        pixels.ctypes.data, err, len(err)), err)
     out = dict(sensor)
     out["I"] = pixels
-    return grad, float(cost.value), out
+    if indices_for_jacobian is None:
+        return grad, float(cost.value), out
+    # [npix][njac][numder] -> (nstokes, numder, njac, npix)
+    return grad, float(cost.value), out, np.transpose(jac, (2, 1, 0))[np.newaxis].copy()
 
 
 class LevisApproxGradientUncorrelated:

@@ -57,6 +57,29 @@ def cpu_baseline(workload_name: str, iters_gpu: int, niter: int):
                       f"= {solve_s:.1f} s"}
 
 
+# kernel class -> device kernel name in the rocprofv3 counter summaries
+KERNEL_NAMES = {"sweep": "k_sweep_ring", "sh_to_do": "k_sh_to_do_perm", "do_to_sh": "k_do_to_sh",
+                "source": "k_compute_source"}
+
+
+def measured_traffic(kernel_class: str):
+    """HBM bytes per launch of the kernel from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled
+    by the reading of a copy of known size), or None when there is none."""
+    import re
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_cloud64.md")
+    if not os.path.exists(path):
+        return None
+    name = KERNEL_NAMES.get(kernel_class)
+    with open(path) as f:
+        for line in f:
+            m = re.match(r"^\|\s*(\w+)\s*\|\s*([0-9.eE+]+)\s*\|\s*([0-9.eE+]+)\s*\|\s*([0-9.eE+]+)\s*\|",
+                         line)
+            if m and m.group(1) == name:
+                return float(m.group(4))
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +89,9 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first copy a buffer of this size once (the known-size kernel that "
+                         "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this SHDOM engine runs one solve per GPU and is measured on one GPU; "
@@ -74,6 +100,15 @@ def main():
 
     import torch
     from at3d_amd import synthetic as S
+
+    if args.copy_calibration_mib > 0:
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     w = S.workload(args.workload)
     t0 = time.perf_counter()
@@ -165,14 +200,15 @@ def main():
     kern["source"] = {"ms_per_launch": ms_src, "bytes_per_launch": src_bytes,
                       "GB/s": src_bytes / ms_src / 1e6}
     dominant = max(("sweep", "sh_to_do", "do_to_sh", "source"), key=lambda k: ksum[k])
+    traffic = measured_traffic(dominant)
     if dominant in ("sh_to_do", "do_to_sh"):
         ach = kern[dominant]["TFLOP/s"]
         roof = {"kernel": dominant, "bound": "mfma", "achieved": ach, "peak": FP32_MFMA_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": ach / FP32_MFMA_PEAK_TFLOPS, "traffic": None}
+                "unit": "TFLOP/s", "frac": ach / FP32_MFMA_PEAK_TFLOPS, "traffic": traffic}
     else:
         ach = kern[dominant]["GB/s"]
         roof = {"kernel": dominant, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None}
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic}
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -180,9 +180,31 @@ int at3d_levis_gradient(at3d_state* st, const at3d_deriv_desc* deriv, int64_t nr
                         const char* costfunc, int32_t exact_single_scatter, double* grad,
                         double* cost, float* pixels, char* err, size_t errlen);
 
+/* The same, plus the Jacobian branch of LEVISAPPROX_GRADIENT (MAKEJACOBIAN,
+ * shdomsub4.f:378-467, rays by GRAD_INTEGRATE_1RAY :811-1545 with
+ * COMPUTE_RADIANCE_DERIVATIVE :2588-2697 and COMPUTE_DIRECT_BEAM_DERIV
+ * :2778-2835): jacobian[npix][njac][numder] (float32) receives, per pixel, the
+ * ray- and Stokes-weighted derivative of its radiance with respect to each
+ * unknown at the property points jacobian_ptr[njac] (0-based flat
+ * [npx][npy][npz] indices; the reference's JACOBIANPTR minus one, as
+ * gradient.py:248-252 builds it).  grad and cost are those of the cost
+ * function, as from at3d_levis_gradient.  njac = 0 computes no Jacobian. */
+int at3d_levis_gradient_jacobian(at3d_state* st, const at3d_deriv_desc* deriv, int64_t nrays,
+                                 const float* x, const float* y, const float* z, const double* mu,
+                                 const double* phi, int64_t npix, const int32_t* rays_per_pixel,
+                                 const double* ray_weights, const double* stokes_weights,
+                                 const float* measurements, const double* uncertainties,
+                                 const char* costfunc, int32_t exact_single_scatter, int32_t njac,
+                                 const int32_t* jacobian_ptr, float* jacobian, double* grad,
+                                 double* cost, float* pixels, char* err, size_t errlen);
+
 /* Engine options of a state.  "sweep_general" (0/1): route every grid point
- * through the general multi-cell characteristic trace instead of the analytic
- * one-cell path (the two are bit-identical; the switch exists for tests).
+ * through the general multi-cell characteristic trace of the point-ordered
+ * sweep (bit-identical to its analytic one-cell path; for tests).
+ * "sweep_ring" (0/1, default 1): use the level-ordered sweep with per-solve
+ * step records and an LDS ring when the ring fits (else the point-ordered
+ * sweep).  "sweep_graph" (0/1): replay the sweep launches from a HIP graph.
+ * "max_subgrid_intervals": sub-step intervals one gradient ray may record.
  * No counterpart in the reference. */
 int at3d_state_set_option(at3d_state* st, const char* key, int32_t value, char* err,
                           size_t errlen);

@@ -650,10 +650,13 @@ def prepare_derivatives(st: RefState, derivs):
 
 def levis_gradient(st: RefState, x, y, z, mu, phi, measurements, uncertainties=None,
                    rays_per_pixel=None, ray_weights=None, stokes_weights=None,
-                   costfunc="L2", exact_single_scatter=True):
+                   costfunc="L2", exact_single_scatter=True, jacobian_ptr=None):
     """levis_approximation_grad (gradient.py:164-415) -> LEVISAPPROX_GRADIENT
-    (shdomsub4.f:288-809, adjoint branch).  Returns (gradient (maxpg, numder),
-    cost, pixel radiances)."""
+    (shdomsub4.f:288-809).  Returns (gradient (maxpg, numder), cost, pixel
+    radiances) from the adjoint branch; with `jacobian_ptr` (0-based property
+    grid points, gradient.py:248-258) the Jacobian branch runs instead
+    (GRAD_INTEGRATE_1RAY + UPDATE_COSTFUNCTION, :378-467) and the Jacobian
+    (npix, njac, numder) is returned fourth."""
     sc, d = st.sc, st.d
     nrays = len(x)
     npix = nrays if rays_per_pixel is None else len(rays_per_pixel)
@@ -666,7 +669,10 @@ def levis_gradient(st: RefState, x, y, z, mu, phi, measurements, uncertainties=N
     gradout = np.zeros((maxpg, numder, 1), np.float64, order="F")
     cost = np.zeros(1, np.float64)
     stokesout = np.zeros((1, npix), np.float32, order="F")
-    jacobian = np.zeros((1, numder, 1, 1), np.float32, order="F")
+    make_jac = jacobian_ptr is not None
+    jptr = (np.asarray(jacobian_ptr, np.int32).ravel() + 1) if make_jac else np.zeros(1, np.int32)
+    njac = len(jptr) if make_jac else 1
+    jacobian = np.zeros((1, numder, njac, npix if make_jac else 1), np.float32, order="F")
     nxsfc = 0
     nsfcpts = (nxsfc + 2) * (nxsfc + 2)
     sfcder = np.zeros((1, nsfcpts), np.int32, order="F")
@@ -691,7 +697,7 @@ def levis_gradient(st: RefState, x, y, z, mu, phi, measurements, uncertainties=N
           i4(d["dnumphase"]), r4(sc.solarflux), i4(d["npx"]), i4(d["npy"]), i4(d["npz"]),
           r4(sc.delx), r4(sc.dely), r4(0.0), r4(0.0), d["zlevels"], d["extdirp"],
           r8(d["uniformzlev"]), d["dpath"], d["dptr"], lg(exact_single_scatter),
-          farr(unc, np.float64), jacobian, lg(False), np.zeros(1, np.int32), i4(1),
+          farr(unc, np.float64), jacobian, lg(make_jac), farr(jptr, np.int32), i4(njac),
           farr(rpp, np.int32), farr(rw, np.float64), farr(sw, np.float64),
           s(costfunc, 2), i4(1), i4(1), i4(1), r8(sc.tautol), lg(False), ierr, errmsg,
           s("ON"), i4(d["maxnmicro"]), d["diphase"], d["iphasep"], d["phasewtp"], d["dphasewt"],
@@ -702,6 +708,9 @@ def levis_gradient(st: RefState, x, y, z, mu, phi, measurements, uncertainties=N
           i4(d["longest_path_pts"]), i4(1), i4(1), i4(nsfcpts), sfcder, r4(0.0), r4(0.0),
           i4(d["nang"]), d["sfcgridrad"])
     _check(ierr, errmsg, "LEVISAPPROX_GRADIENT")
+    if make_jac:
+        return (gradout[:, :, 0].copy(), float(cost[0]), stokesout[0].copy(),
+                np.transpose(jacobian[0], (2, 1, 0)).copy())
     return gradout[:, :, 0].copy(), float(cost[0]), stokesout[0].copy()
 
 

@@ -43,3 +43,34 @@ def test_oracle_sun_path_weights_sum_to_slant_length(ref, mu0):
     z = st.gridpos[2, :st.npts]
     np.testing.assert_allclose(st.dpath[:, :st.npts].sum(axis=0), (w.z[-1] - z) / mu0, rtol=1e-5,
                                atol=1e-6)
+
+
+def test_oracle_jacobian_branch_is_the_per_pixel_derivative(ref):
+    """The oracle's Jacobian branch (GRAD_INTEGRATE_1RAY + UPDATE_COSTFUNCTION)
+    against its adjoint branch: the two gradients agree (rel 1e-5), and at the
+    Jacobian's points the L2 gradient is sum_pixels (R - M) x Jacobian (the
+    chain rule that defines the Jacobian), exactly up to rounding."""
+    from tests.scenes import blob8, ortho_rays, rel_l2
+    w = blob8("open")
+    st = ref.setup(ref.scene_from_workload(w))
+    ref.init_solution(st)
+    ref.solution_iterations(st, 100)
+    shape = w.shape
+    ref.prepare_derivatives(st, [ref.Derivative(0, np.ones(shape), np.zeros(shape), np.zeros((65, 1)),
+                                                np.ones(shape, np.int32)),
+                                 ref.Derivative(0, np.zeros(shape), np.ones(shape), np.zeros((65, 1)),
+                                                np.ones(shape, np.int32))])
+    rays = ortho_rays(w, [(0.0, 0.0), (45.0, 30.0)])
+    args = (rays["ray_x"], rays["ray_y"], rays["ray_z"], rays["ray_mu"], rays["ray_phi"])
+    img = ref.render(st, *args)
+    meas = (img * 1.1).astype(np.float32)
+    g, c, px = ref.levis_gradient(st, *args, meas)
+    jp = np.array([0, 100, 219, 300, 511])
+    gj, cj, pxj, J = ref.levis_gradient(st, *args, meas, jacobian_ptr=jp)
+    assert J.shape == (px.size, jp.size, 2)
+    assert cj == pytest.approx(c, rel=1e-5)
+    for i in range(2):
+        assert rel_l2(gj[:, i], g[:, i]) < 1e-5
+    gsel = np.einsum("p,pjd->jd", (pxj - meas).astype(np.float64), J.astype(np.float64))
+    np.testing.assert_allclose(gsel, gj[jp], rtol=1e-5, atol=1e-9 * np.abs(gj).max())
+    assert np.abs(J[:, 2, 1]).max() > 0   # a point inside the blob responds to ssalb
