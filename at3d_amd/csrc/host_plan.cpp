@@ -12,8 +12,26 @@
 
 #include <algorithm>
 #include <numeric>
+#include <thread>
 
 namespace at3d {
+
+// Runs f(0..n-1) on up to 16 host threads (independent items only).
+template <typename F>
+static void parallel_for(int n, F&& f) {
+  const int nt = std::max(1, std::min(n, std::min(16, (int)std::thread::hardware_concurrency())));
+  if (nt <= 1) {
+    for (int i = 0; i < n; i++) f(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (int t = 0; t < nt; t++)
+    th.emplace_back([&, t]() {
+      for (int i = t; i < n; i += nt) f(i);
+    });
+  for (std::thread& x : th) x.join();
+}
 namespace {
 
 inline bool btest(int v, int bit) { return (v >> bit) & 1; }
@@ -393,7 +411,12 @@ void interp_optics(Plan& p) {
     const float* A = &p.albedop[(size_t)ipa * npg];
     const float* PW = &p.phasewtp[(size_t)ipa * npg];
     const int32_t* IP = &p.iphasep[(size_t)ipa * npg];
-    for (int ip = 0; ip < npts; ip++) {
+    // points are independent; the species loop stays outside so each point's
+    // total extinction is summed in species order
+    const int nchunk = 64;
+    parallel_for(nchunk, [&](int c) {
+    const int ip0 = (int)((int64_t)npts * c / nchunk), ip1 = (int)((int64_t)npts * (c + 1) / nchunk);
+    for (int ip = ip0; ip < ip1; ip++) {
       const float X = p.gridpos[ip * 3], Y = p.gridpos[ip * 3 + 1], Z = p.gridpos[ip * 3 + 2];
       int il = 0, iu = npz;
       while (iu - il > 1) {
@@ -460,6 +483,7 @@ void interp_optics(Plan& p) {
       p.albedo[(size_t)ipa * npts + ip] = albf;
       p.total_ext[ip] = p.total_ext[ip] + extf;
     }
+    });
   }
   // Tabulated phase functions divided by (2l+1) (TRILIN_INTERP_PROP init branch).
   const int nl = p.nleg + 1;
@@ -1100,15 +1124,12 @@ void sweeping_order(const Plan& p, int ioct0, std::vector<int32_t>& ord) {
   }
 }
 
-int build_levels(Plan& p, std::string& err) {
+// One octant's sweep order, dependency levels and level lists.
+static int build_octant(const Plan& p, int o, const std::vector<char>& top, const std::vector<char>& bot,
+                        OctantPlan& op, std::string& err) {
   const GridTopo g{p.gridpos.data(), p.gridptr.data(), p.neighptr.data(), p.cellflags.data(),
                    p.npts, p.ncells};
-  std::vector<char> top(p.npts, 0), bot(p.npts, 0);
-  for (int i : p.bctop) top[i] = 1;
-  for (int i : p.bcbot) bot[i] = 1;
-  p.nlevels_max = 0;
-  for (int o = 0; o < 8; o++) {
-    OctantPlan& op = p.oct[o];
+  {
     sweeping_order(p, o, op.sweepord);
     if ((int)op.sweepord.size() != p.npts) {
       err = "SWEEPING_ORDER did not visit every grid point";
@@ -1191,7 +1212,6 @@ int build_levels(Plan& p, std::string& err) {
     int nl = 0;
     for (int i = 0; i < p.npts; i++) nl = std::max(nl, op.level[i]);
     op.nlevels = nl;
-    p.nlevels_max = std::max(p.nlevels_max, nl);
     std::vector<int32_t> cnt(nl + 2, 0);
     for (int r = 0; r < p.npts; r++) {
       const int pt = p.gridptr[(size_t)(op.sweepord[r] >> 3) * 8 + (op.sweepord[r] & 7)];
@@ -1213,6 +1233,24 @@ int build_levels(Plan& p, std::string& err) {
       op.lvl_pts[pos] = op.simple_pt[pt] ? pt : ~pt;
       op.lvl_cell[pos] = cell;
     }
+  }
+  return AT3D_OK;
+}
+
+int build_levels(Plan& p, std::string& err) {
+  std::vector<char> top(p.npts, 0), bot(p.npts, 0);
+  for (int i : p.bctop) top[i] = 1;
+  for (int i : p.bcbot) bot[i] = 1;
+  std::vector<std::string> errs(8);
+  std::vector<int> rcs(8, AT3D_OK);
+  parallel_for(8, [&](int o) { rcs[o] = build_octant(p, o, top, bot, p.oct[o], errs[o]); });
+  p.nlevels_max = 0;
+  for (int o = 0; o < 8; o++) {
+    if (rcs[o]) {
+      err = errs[o];
+      return rcs[o];
+    }
+    p.nlevels_max = std::max(p.nlevels_max, p.oct[o].nlevels);
   }
   return AT3D_OK;
 }
@@ -1414,12 +1452,22 @@ int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err) {
   DirectBeam beam;
   direct_beam_init(p, beam);
   p.dirflux.assign(p.npts, 0.f);
-  for (int ip = 0; ip < p.npts; ip++)
-    if (!direct_beam_point(p, beam, p.gridpos[ip * 3], p.gridpos[ip * 3 + 1], p.gridpos[ip * 3 + 2],
-                           p.dirflux[ip])) {
-      err = "DIRECT_BEAM_PROP: ray left the property grid";
-      return AT3D_ERR_NUMERIC;
-    }
+  {
+    const int nchunk = 64;
+    std::vector<char> bad(nchunk, 0);
+    parallel_for(nchunk, [&](int c) {
+      const int i0 = (int)((int64_t)p.npts * c / nchunk), i1 = (int)((int64_t)p.npts * (c + 1) / nchunk);
+      for (int ip = i0; ip < i1; ip++)
+        if (!direct_beam_point(p, beam, p.gridpos[ip * 3], p.gridpos[ip * 3 + 1], p.gridpos[ip * 3 + 2],
+                               p.dirflux[ip]))
+          bad[c] = 1;
+    });
+    for (char b : bad)
+      if (b) {
+        err = "DIRECT_BEAM_PROP: ray left the property grid";
+        return AT3D_ERR_NUMERIC;
+      }
+  }
   if (!init_radiance(p, err)) return AT3D_ERR_NUMERIC;
   // BOUNDARY_PNTS (shdomsub1.f:2173-2215)
   p.bctop.clear();

@@ -12,7 +12,9 @@
 // Tiling: 64x64 output tile per 256-thread workgroup, K step 16, four waves in
 // a 2x2 layout, each wave 2x2 blocks of v_mfma_f32_16x16x4_f32, whose operand
 // lane layout is A[i][k] in lane i+16k, B[k][j] in lane j+16k, and
-// D[4*(lane/16)+v][lane%16] in accumulator element v.
+// D[4*(lane/16)+v][lane%16] in accumulator element v.  The LDS tiles are
+// double-buffered: step k's tile is stored, one barrier, then step k+1's
+// global loads are issued before step k's MFMAs, so they overlap.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -24,20 +26,23 @@ constexpr int TM = 64, TN = 64, TK = 16, PAD = 4;
 
 __device__ inline void mfma_tile(const float (*sA)[TM + PAD], const float (*sB)[TN + PAD], int wm,
                                  int wn, int lane, float4_ acc[2][2]) {
+  // all of the step's LDS operands first, then the 16 MFMAs back to back
+  float av[TK / 4][2], bv[TK / 4][2];
 #pragma unroll
-  for (int kk = 0; kk < TK; kk += 4) {
-    const int kr = kk + (lane >> 4);
-    float av[2], bv[2];
+  for (int q = 0; q < TK / 4; q++) {
+    const int kr = 4 * q + (lane >> 4);
 #pragma unroll
-    for (int i = 0; i < 2; i++) av[i] = sA[kr][wm * 32 + i * 16 + (lane & 15)];
+    for (int i = 0; i < 2; i++) av[q][i] = sA[kr][wm * 32 + i * 16 + (lane & 15)];
 #pragma unroll
-    for (int j = 0; j < 2; j++) bv[j] = sB[kr][wn * 32 + j * 16 + (lane & 15)];
+    for (int j = 0; j < 2; j++) bv[q][j] = sB[kr][wn * 32 + j * 16 + (lane & 15)];
+  }
+#pragma unroll
+  for (int q = 0; q < TK / 4; q++)
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
       for (int j = 0; j < 2; j++)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bv[j], acc[i][j], 0, 0, 0);
-  }
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q][i], bv[q][j], acc[i][j], 0, 0, 0);
 }
 
 // C[m][n] = sum_k A[m][k] B[n][k]: both operands K-contiguous.  CLAMP_EXT
@@ -50,8 +55,8 @@ __global__ __launch_bounds__(256) void k_sh_to_do(const float* __restrict__ S,
                                                   const float* __restrict__ ext,
                                                   float* __restrict__ out, int ld, int ldo,
                                                   int npts, int nang) {
-  __shared__ float sA[TK][TM + PAD];
-  __shared__ float sB[TK][TN + PAD];
+  __shared__ float sA[2][TK][TM + PAD];
+  __shared__ float sB[2][TK][TN + PAD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
@@ -59,20 +64,26 @@ __global__ __launch_bounds__(256) void k_sh_to_do(const float* __restrict__ S,
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++) acc[i][j] = (float4_){0.f, 0.f, 0.f, 0.f};
   const int lr = tid >> 2, lc = (tid & 3) * 4;
-  for (int k0 = 0; k0 < ld; k0 += TK) {
-    const float4 a = *reinterpret_cast<const float4*>(S + (size_t)(m0 + lr) * ld + k0 + lc);
-    const float4 b = *reinterpret_cast<const float4*>(Yc + (size_t)(n0 + lr) * ld + k0 + lc);
+  const float* arow = S + (size_t)(m0 + lr) * ld + lc;
+  const float* brow = Yc + (size_t)(n0 + lr) * ld + lc;
+  // double-buffered LDS: the next K step's global loads fly during this step's MFMAs
+  float4 a = *reinterpret_cast<const float4*>(arow);
+  float4 b = *reinterpret_cast<const float4*>(brow);
+  for (int k0 = 0, buf = 0; k0 < ld; k0 += TK, buf ^= 1) {
+    sA[buf][lc + 0][lr] = a.x;
+    sA[buf][lc + 1][lr] = a.y;
+    sA[buf][lc + 2][lr] = a.z;
+    sA[buf][lc + 3][lr] = a.w;
+    sB[buf][lc + 0][lr] = b.x;
+    sB[buf][lc + 1][lr] = b.y;
+    sB[buf][lc + 2][lr] = b.z;
+    sB[buf][lc + 3][lr] = b.w;
     __syncthreads();
-    sA[lc + 0][lr] = a.x;
-    sA[lc + 1][lr] = a.y;
-    sA[lc + 2][lr] = a.z;
-    sA[lc + 3][lr] = a.w;
-    sB[lc + 0][lr] = b.x;
-    sB[lc + 1][lr] = b.y;
-    sB[lc + 2][lr] = b.z;
-    sB[lc + 3][lr] = b.w;
-    __syncthreads();
-    mfma_tile(sA, sB, wm, wn, lane, acc);
+    if (k0 + TK < ld) {
+      a = *reinterpret_cast<const float4*>(arow + k0 + TK);
+      b = *reinterpret_cast<const float4*>(brow + k0 + TK);
+    }
+    mfma_tile(sA[buf], sB[buf], wm, wn, lane, acc);
   }
   if (CLAMP_EXT) {
     // rows beyond npts hold zero sources and zero extinction: stored as zeros
@@ -114,8 +125,8 @@ __global__ __launch_bounds__(256) void k_sh_to_do_perm(const float* __restrict__
                                                        const int32_t* __restrict__ oct_perm,
                                                        const int4* __restrict__ tiles,
                                                        float* __restrict__ out, int ld, int ldp) {
-  __shared__ float sA[TK][TM + PAD];
-  __shared__ float sB[TK][PN + PAD];
+  __shared__ float sA[2][TK][TM + PAD];
+  __shared__ float sB[2][TK][PN + PAD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int4 tile = tiles[blockIdx.y];
   const int32_t* perm = oct_perm + (size_t)tile.x * ldp;
@@ -127,30 +138,38 @@ __global__ __launch_bounds__(256) void k_sh_to_do_perm(const float* __restrict__
   const float* arow = S + (size_t)perm[m0 + lr] * ld + lc;
   const bool bload = lr < nc;  // threads 0..4*nc-1 load the column tile
   const float* brow = Yc + (size_t)(c0 + (bload ? lr : 0)) * ld + lc;
-  for (int k0 = 0; k0 < ld; k0 += TK) {
-    const float4 a = *reinterpret_cast<const float4*>(arow + k0);
-    float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (bload) b = *reinterpret_cast<const float4*>(brow + k0);
-    __syncthreads();
-    sA[lc + 0][lr] = a.x;
-    sA[lc + 1][lr] = a.y;
-    sA[lc + 2][lr] = a.z;
-    sA[lc + 3][lr] = a.w;
+  float4 a = *reinterpret_cast<const float4*>(arow);
+  float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (bload) b = *reinterpret_cast<const float4*>(brow);
+  for (int k0 = 0, buf = 0; k0 < ld; k0 += TK, buf ^= 1) {
+    sA[buf][lc + 0][lr] = a.x;
+    sA[buf][lc + 1][lr] = a.y;
+    sA[buf][lc + 2][lr] = a.z;
+    sA[buf][lc + 3][lr] = a.w;
     if (lr < PN) {
-      sB[lc + 0][lr] = b.x;
-      sB[lc + 1][lr] = b.y;
-      sB[lc + 2][lr] = b.z;
-      sB[lc + 3][lr] = b.w;
+      sB[buf][lc + 0][lr] = b.x;
+      sB[buf][lc + 1][lr] = b.y;
+      sB[buf][lc + 2][lr] = b.z;
+      sB[buf][lc + 3][lr] = b.w;
     }
     __syncthreads();
+    if (k0 + TK < ld) {
+      a = *reinterpret_cast<const float4*>(arow + k0 + TK);
+      if (bload) b = *reinterpret_cast<const float4*>(brow + k0 + TK);
+    }
+    float av[TK / 4], bv[TK / 4][3];
 #pragma unroll
-    for (int kk = 0; kk < TK; kk += 4) {
-      const int kr = kk + (lane >> 4);
-      const float av = sA[kr][wave * 16 + (lane & 15)];
+    for (int q = 0; q < TK / 4; q++) {
+      const int kr = 4 * q + (lane >> 4);
+      av[q] = sA[buf][kr][wave * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 3; j++) bv[q][j] = sB[buf][kr][j * 16 + (lane & 15)];
+    }
+#pragma unroll
+    for (int q = 0; q < TK / 4; q++)
 #pragma unroll
       for (int j = 0; j < 3; j++)
-        acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, sB[kr][j * 16 + (lane & 15)], acc[j], 0, 0, 0);
-    }
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q], bv[q][j], acc[j], 0, 0, 0);
   }
   // rows beyond npts gather the zero padding rows of S and ext
   const int q0 = m0 + wave * 16 + 4 * (lane >> 4);
@@ -175,8 +194,8 @@ __global__ __launch_bounds__(256) void k_do_to_sh(const float* __restrict__ D,
                                                   const int* __restrict__ nr,
                                                   float* __restrict__ R, int ld, int ldo, int ldp,
                                                   int npts) {
-  __shared__ float sA[TK][TM + PAD];
-  __shared__ float sB[TK][TN + PAD];
+  __shared__ float sA[2][TK][TM + PAD];
+  __shared__ float sB[2][TK][TN + PAD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
@@ -184,23 +203,34 @@ __global__ __launch_bounds__(256) void k_do_to_sh(const float* __restrict__ D,
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++) acc[i][j] = (float4_){0.f, 0.f, 0.f, 0.f};
   const int bk = tid >> 4, bc = (tid & 15) * 4;    // 16 k x 64 rows/cols per tile
-  for (int k0 = 0; k0 < ldo; k0 += TK) {
-    const float4 a = *reinterpret_cast<const float4*>(D + (size_t)(k0 + bk) * ldp + m0 + bc);
-    const float4 b = *reinterpret_cast<const float4*>(Wc + (size_t)(k0 + bk) * ld + n0 + bc);
+  const float* acol = D + (size_t)bk * ldp + m0 + bc;
+  const float* bcol = Wc + (size_t)bk * ld + n0 + bc;
+  float4 a = *reinterpret_cast<const float4*>(acol);
+  float4 b = *reinterpret_cast<const float4*>(bcol);
+  for (int k0 = 0, buf = 0; k0 < ldo; k0 += TK, buf ^= 1) {
+    *reinterpret_cast<float4*>(&sA[buf][bk][bc]) = a;
+    *reinterpret_cast<float4*>(&sB[buf][bk][bc]) = b;
     __syncthreads();
-    *reinterpret_cast<float4*>(&sA[bk][bc]) = a;
-    *reinterpret_cast<float4*>(&sB[bk][bc]) = b;
-    __syncthreads();
-    mfma_tile(sA, sB, wm, wn, lane, acc);
+    if (k0 + TK < ldo) {
+      a = *reinterpret_cast<const float4*>(acol + (size_t)(k0 + TK) * ldp);
+      b = *reinterpret_cast<const float4*>(bcol + (size_t)(k0 + TK) * ld);
+    }
+    mfma_tile(sA[buf], sB[buf], wm, wn, lane, acc);
   }
+  // the eight rows' radiance counts first (independent loads), then the stores
+  int n[2][4];
   for (int i = 0; i < 2; i++)
     for (int v = 0; v < 4; v++) {
       const int p = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + v;
-      if (p >= npts) continue;
-      const int n = nr[p];
+      n[i][v] = p < npts ? nr[p] : -1;
+    }
+  for (int i = 0; i < 2; i++)
+    for (int v = 0; v < 4; v++) {
+      if (n[i][v] < 0) continue;
+      const int p = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + v;
       for (int j = 0; j < 2; j++) {
         const int J = n0 + wn * 32 + j * 16 + (lane & 15);
-        R[(size_t)p * ld + J] = (J < n) ? acc[i][j][v] : 0.0f;
+        R[(size_t)p * ld + J] = (J < n[i][v]) ? acc[i][j][v] : 0.0f;
       }
     }
 }
