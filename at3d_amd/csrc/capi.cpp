@@ -7,6 +7,7 @@
 
 #include "../../include/at3d_hip.h"
 #include "host_plan.h"
+#include "ring_plan.h"
 #include "solver_state.h"
 
 struct at3d_plan {
@@ -150,6 +151,34 @@ int at3d_plan_get(const at3d_plan* h, const char* name, void* dst, size_t nbytes
     return copy_out(val, dst, nbytes, err, errlen);
   }
   if (k == "bcptr_bot") return copy_out(p.bcbot, dst, nbytes, err, errlen);
+  if (k == "ring_level" || k == "ring_pos" || k == "ring_staged" || k == "ring_sizes") {
+    // the level-ordered sweep's layout ([8][npts] per octant, zeros for
+    // octants without ordinates; ring_sizes = c4, c9, levels of each octant)
+    bool used[8] = {};
+    for (const at3d::Ordinate& od : p.ords) used[od.g.ioct] = true;
+    at3d::RingPlan rp;
+    std::string msg;
+    if (!at3d::build_ring_plan(p, used, rp, msg)) {
+      set_err(err, errlen, msg);
+      return AT3D_ERR_NUMERIC;
+    }
+    std::vector<int32_t> all;
+    if (k == "ring_sizes") {
+      all.push_back(rp.c4);
+      all.push_back(rp.c9);
+      for (int o = 0; o < 8; o++) all.push_back(rp.oct[o].nlev);
+    } else {
+      all.assign((size_t)8 * p.npts, 0);
+      for (int o = 0; o < 8; o++) {
+        if (!used[o]) continue;
+        const at3d::RingOctant& ro = rp.oct[o];
+        for (int i = 0; i < p.npts; i++)
+          all[(size_t)o * p.npts + i] = k == "ring_level" ? ro.level[i] : k == "ring_pos" ? ro.pos[i]
+                                                                                        : ro.staged[i];
+      }
+    }
+    return copy_out(all, dst, nbytes, err, errlen);
+  }
   if (k == "sweepord" || k == "level" || k == "lvl_pts") {
     // per octant, concatenated: [8][npts] for sweepord and level; lvl_pts is
     // each octant's non-preset points in (level, rank) order, ~point for the
@@ -426,9 +455,11 @@ double at3d_state_timing(const at3d_state* st, const char* key) {
     return (double)s.nlevels_max;
   if (k == "total") return s.tsum[0] + s.tsum[1] + s.tsum[2] + s.tsum[3] + s.tsum[4] + s.tsum[5];
   if (k == "sweep_launches") return s.nsweep_launch;
+  if (k == "sweep_kernel") return s.tsum[6];
   if (k == "sweep_ring") return (s.ring_ok && s.use_ring && !s.force_general) ? 1.0 : 0.0;
   if (k == "sweep_ring_lds") return (double)s.ring_lds;
-  if (k == "sweep_ring_slots") return s.ring_c;
+  if (k == "sweep_ring_slots") return s.c9;
+  if (k == "sweep_sectors") return s.nsectors;
   if (k == "sweep_general_points") return s.ngeneral;
   if (k == "iters") return s.iters_last;
   if (k == "render") return s.render_ms;

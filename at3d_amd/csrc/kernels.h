@@ -54,49 +54,69 @@ struct SweepArgs {
   int force_general;
   int* errflag;
   // level-ordered sweep (k_sweep_ring): srcext rows are in each octant's
-  // position order (boundary points, then levels), radiances stay point-ordered
-  const int32_t* oct_pos;    // [8][npts] point -> position in the octant's order
+  // ring order (levels, boundary points as entries of their own), radiances
+  // stay point-ordered.  Per octant o, arrays of npts entries at o * npts.
+  const int32_t* oct_pos;    // [8][npts] point -> position
+  const int32_t* ring_code;  // [8][npts] position -> point, or -1-b for boundary point b
+  const int32_t* ring_cell;  // [8][npts] start cell of a point entry
+  const int32_t* ring_level; // [8][npts] ring level of each point
+  const int8_t* ring_staged; // [8][npts] point written by its sector
+  const int32_t* ring_off;   // [8][gld] level L spans positions ring_off[o*gld+L-1] .. [o*gld+L]
+  int ring_nlev[8];
+  int oct_col0[8], oct_nord[8];  // device columns of each octant
+  const int32_t* sec_off;    // [8][gld+1] sectors completing at level L: [o*(gld+1)+L] .. [+L+1]
+  const int32_t* sec_p0;     // sectors of all octants, octant o from sec_base[o]
+  const uint4* sec_slot;     // eight 16-bit radiance ring slots per sector
+  int sec_base[8];
   const int32_t* top_index;  // [npts] index in bctop or -1
   const int32_t* bot_index;  // [npts] index in bcbot or -1
-  const uint4* rec;          // [ncols][ldr][2] one-cell step records (SweepRec)
-  int ldr;                   // records per column (max swept points of an octant)
+  const uint4* rec;          // [ncols][npts][2] step records (SweepRec)
   const uint8_t* gflag;      // [ncols][gld] level L holds a multi-cell trace
+  int* col_general;          // [ncols] nonzero: the column has multi-cell traces
   int gld;
-  int ring_c;                // ring entries (points of RING_LEVELS+1 consecutive levels)
+  int c4, c9;                // source and radiance ring slots
   int* ngeneral;             // multi-cell traces found by the record pass
 };
 
-// A one-cell characteristic with every coefficient that stays fixed during a
-// solve: rad = max(0, b0 * srcext_face + b1 * srcext_pt) + T * rad_face, the
-// face values bilinear in (u, v) over four LDS entries r[0..3] (preset
-// boundary slots first, then the ring of recent levels).  pt < 0 (= ~point)
-// marks a multi-cell trace.
+// The fixed part of one ring entry of one ordinate: word0.x = point | staged
+// << 29 | kind << 30 (kind 0: one-cell step, 1: multi-cell trace, 2:
+// boundary point).  A one-cell step's radiance is
+// max(0, b0 * srcext_face + b1 * srcext_pt) + T * rad_face, the face values
+// bilinear in (u, v) over four ring slots r[0..3]; a boundary entry holds its
+// boundary index in r[0..1].
 struct SweepRec {
   int32_t pt;
   uint16_t r[4];
   float u, v, b0, b1, T;
 };
 static_assert(sizeof(SweepRec) == 32, "SweepRec is two 16-byte words");
-constexpr int RING_LEVELS = 3;     // earlier levels a one-cell record may reference
 constexpr int RING_THREADS = 512;
 
-// Bytes of LDS k_sweep_ring needs: npre boundary slots and ring_c ring
-// entries of (radiance, source), gld level offsets and gld level flags.
-inline size_t ring_lds_bytes(int npre, int ring_c, int gld) {
-  return (size_t)(npre + ring_c) * 8 + (size_t)gld * 5;
+// Bytes of LDS k_sweep_ring needs: c9 radiance and c4 source ring slots, gld
+// level offsets, gld + 1 sector offsets and gld level flags.
+inline size_t ring_lds_bytes(int c9, int c4, int gld) {
+  return (size_t)(c9 + c4) * 4 + (size_t)gld * 9 + 4;
 }
 // Allows k_sweep_ring launches with lds_bytes of dynamic LDS; nonzero on failure.
 int sweep_ring_max_lds(size_t lds_bytes);
-void launch_sweep_records(hipStream_t st, const SweepArgs& a, int ncols, int max_swept);
+void launch_sweep_records(hipStream_t st, const SweepArgs& a);
 void launch_sweep_ring(hipStream_t st, const SweepArgs& a, int col0, int ncol, size_t lds_bytes);
 
 void launch_sweep_preset(hipStream_t st, const SweepArgs& a, bool up, int col0, int ncol);
 // One workgroup per ordinate column col0 .. col0+ncol-1, all levels of its octant.
 void launch_sweep_columns(hipStream_t st, const SweepArgs& a, int col0, int ncol);
-// Lambertian surface radiance from the downwelling flux (reference ordinate order).
+// Where the surface radiances also go: the boundary entries of the step
+// records of the upwelling columns col0 .. col1-1 (rec null: nowhere else).
+struct BottomRecs {
+  uint4* rec;
+  int npts, col0, col1;
+  const int8_t* col_oct;
+  const int32_t* oct_pos;
+};
+// Lambertian surface radiance from the downwelling flux.
 void launch_bottom_bc(hipStream_t st, const float* rad, int ldp, const int32_t* bcbot, int nbot,
                       const int32_t* col_of_ref, const float* fluxw, int ndown,
-                      const float* dirflux, float albpi, float* bcrad);
+                      const float* dirflux, float albpi, float* bcrad, const BottomRecs& br);
 void launch_fluxes(hipStream_t st, const float* rad, int ldp, const int32_t* col_of_ref,
                    const float* fluxw, int ndown, int nang, int npts, float* fluxes);
 

@@ -1,0 +1,128 @@
+// Host-side layout of the level-ordered sweep.
+//
+// The reference sweeps one ordinate at a time in SWEEPING_ORDER
+// (shdomsub1.f:3261-3351) with the boundary radiances set beforehand
+// (PATH_INTEGRATION :2042-2086).  The GPU sweep walks dependency levels (the
+// plan's levels, shifted up by one here) and keeps the last levels' values in
+// LDS; boundary points become entries of the level before their first
+// reader, so they travel through the same ring.  Sectors of eight consecutive
+// points (32 bytes of a point-ordered radiance row) whose points span at most
+// eight levels are written whole from the ring after their last point.
+#include "ring_plan.h"
+
+#include <algorithm>
+#include <climits>
+
+namespace at3d {
+
+bool build_ring_plan(const Plan& p, const bool used[8], RingPlan& r, std::string& err) {
+  const int npts = p.npts;
+  r.c4 = 1;
+  int c9raw = 1;
+  r.nlev_max = 0;
+  for (int o = 0; o < 8; o++) {
+    RingOctant& ro = r.oct[o];
+    ro = RingOctant();
+    if (!used[o]) continue;
+    const OctantPlan& op = p.oct[o];
+    const bool up = (o >> 2) & 1;
+    const std::vector<int32_t>& bc = up ? p.bcbot : p.bctop;
+    const int bitx = o & 1, bity = (o >> 1) & 1, bitz = (o >> 2) & 1;
+    std::vector<int32_t> bindex(npts, -1);
+    for (size_t b = 0; b < bc.size(); b++) bindex[bc[b]] = (int32_t)b;
+    ro.level.assign(npts, 0);
+    for (int i = 0; i < npts; i++)
+      if (op.level[i] > 0) ro.level[i] = op.level[i] + 1;
+    // boundary points: one level before the first point whose start cell has
+    // them on a far face (the faces a one-cell step can end on)
+    std::vector<int32_t> first(npts, INT_MAX);
+    const int nsw = op.lvl_off[op.nlevels];
+    for (int j = 0; j < nsw; j++) {
+      const int c = op.lvl_pts[j], pt = c >= 0 ? c : ~c, cell = op.lvl_cell[j];
+      const int8_t fl = p.cellflags[cell];
+      int faces[3], nf = 0;
+      if (!(fl & 1)) faces[nf++] = 1 - bitx;
+      if (!(fl & 2)) faces[nf++] = 3 - bity;
+      faces[nf++] = 5 - bitz;
+      for (int f = 0; f < nf; f++)
+        for (int k = 0; k < 4; k++) {
+          const int q = p.gridptr[(size_t)cell * 8 + face_corner(faces[f], k)];
+          if (bindex[q] >= 0) first[q] = std::min(first[q], ro.level[pt]);
+        }
+    }
+    int nl = 1;
+    for (int i = 0; i < npts; i++) nl = std::max(nl, ro.level[i]);
+    for (size_t b = 0; b < bc.size(); b++) {
+      const int q = bc[b];
+      ro.level[q] = first[q] == INT_MAX ? 1 : std::max(1, first[q] - 1);
+    }
+    ro.nlev = nl;
+    r.nlev_max = std::max(r.nlev_max, nl);
+    // entries by level: the plan's order (level, rank) for points, then the
+    // boundary points in boundary order
+    std::vector<int32_t> cnt(nl + 2, 0);
+    for (int i = 0; i < npts; i++) cnt[ro.level[i]]++;
+    if (cnt[0] != 0) {
+      err = "ring layout: a grid point is neither swept nor a boundary point";
+      return false;
+    }
+    ro.off.assign(nl + 1, 0);
+    for (int L = 1; L <= nl; L++) ro.off[L] = ro.off[L - 1] + cnt[L];
+    ro.code.assign(npts, 0);
+    ro.cell.assign(npts, -1);
+    ro.pos.assign(npts, 0);
+    std::vector<int32_t> fill(ro.off.begin(), ro.off.end());
+    for (int j = 0; j < nsw; j++) {
+      const int c = op.lvl_pts[j], pt = c >= 0 ? c : ~c;
+      const int q = fill[ro.level[pt] - 1]++;
+      ro.code[q] = pt;
+      ro.cell[q] = op.lvl_cell[j];
+      ro.pos[pt] = q;
+    }
+    for (size_t b = 0; b < bc.size(); b++) {
+      const int pt = bc[b];
+      const int q = fill[ro.level[pt] - 1]++;
+      ro.code[q] = -1 - (int32_t)b;
+      ro.pos[pt] = q;
+    }
+    for (int L = 1; L <= nl; L++) {
+      if (fill[L - 1] != ro.off[L]) {
+        err = "ring layout: level lists do not cover the grid";
+        return false;
+      }
+      r.c4 = std::max(r.c4, ro.off[L] - ro.off[std::max(L - 1 - RING_WINDOW, 0)]);
+      c9raw = std::max(c9raw, ro.off[L] - ro.off[std::max(L - 1 - SECTOR_POINTS, 0)]);
+    }
+  }
+  r.c9 = r.c4 * ((c9raw + r.c4 - 1) / r.c4);
+  // sectors (need c9)
+  for (int o = 0; o < 8; o++) {
+    if (!used[o]) continue;
+    RingOctant& ro = r.oct[o];
+    const int nl = ro.nlev;
+    ro.staged.assign(npts, 0);
+    std::vector<std::vector<int32_t>> by_level(nl + 1);
+    for (int s0 = 0; s0 + SECTOR_POINTS <= npts; s0 += SECTOR_POINTS) {
+      int lo = INT_MAX, hi = 0;
+      for (int k = 0; k < SECTOR_POINTS; k++) {
+        lo = std::min(lo, ro.level[s0 + k]);
+        hi = std::max(hi, ro.level[s0 + k]);
+      }
+      if (lo < hi - (SECTOR_POINTS - 1)) continue;
+      by_level[hi].push_back(s0);
+      for (int k = 0; k < SECTOR_POINTS; k++) ro.staged[s0 + k] = 1;
+    }
+    ro.sec_off.assign(nl + 2, 0);
+    for (int L = 1; L <= nl; L++) ro.sec_off[L + 1] = ro.sec_off[L] + (int32_t)by_level[L].size();
+    ro.sec_p0.clear();
+    ro.sec_slot.clear();
+    for (int L = 1; L <= nl; L++)
+      for (int s0 : by_level[L]) {
+        ro.sec_p0.push_back(s0);
+        for (int k = 0; k < SECTOR_POINTS; k++) ro.sec_slot.push_back((uint16_t)(ro.pos[s0 + k] % r.c9));
+      }
+  }
+  return true;
+}
+
+}  // namespace at3d

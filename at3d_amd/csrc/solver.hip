@@ -28,6 +28,7 @@
 
 #include "host_plan.h"
 #include "kernels.h"
+#include "ring_plan.h"
 #include "shdom_geom.h"
 #include "solver_state.h"
 
@@ -110,78 +111,117 @@ static SweepArgs sweep_args(const State& s) {
   a.force_general = s.force_general;
   a.errflag = s.errflag;
   a.oct_pos = s.oct_pos;
+  a.ring_code = s.ring_code;
+  a.ring_cell = s.ring_cell;
+  a.ring_level = s.ring_level;
+  a.ring_staged = s.ring_staged;
+  a.ring_off = s.ring_off;
+  a.sec_off = s.sec_off;
+  a.sec_p0 = s.sec_p0;
+  a.sec_slot = s.sec_slot;
+  for (int o = 0; o < 8; o++) {
+    a.ring_nlev[o] = s.ring_nlev[o];
+    a.sec_base[o] = s.sec_base[o];
+    a.oct_col0[o] = s.oct_col_base[o];
+    a.oct_nord[o] = s.oct_nord[o];
+  }
   a.top_index = s.top_index;
   a.bot_index = s.bot_index;
   a.rec = s.rec;
-  a.ldr = s.max_swept;
   a.gflag = s.gflag;
+  a.col_general = s.col_general;
   a.gld = s.gld;
-  a.ring_c = s.ring_c;
+  a.c4 = s.c4;
+  a.c9 = s.c9;
   a.ngeneral = s.ngeneral_d;
   return a;
 }
 
 static bool ring_active(const State& s) { return s.ring_ok && s.use_ring && !s.force_general; }
 
-// The level-ordered sweep: each octant's position order (boundary points of its
-// hemisphere, then its levels), the SH->DO column tiles, the ring size, and the
-// per-solve records.  Leaves ring_ok false (and the point-ordered sweep in use)
-// when the ring does not fit in LDS or the records do not fit in memory.
+// The level-ordered sweep: each octant's ring order (its levels, with the
+// hemisphere's boundary points as entries of their own), the SH->DO column
+// tiles, the ring sizes, the sectors, and the per-solve step records.  Leaves
+// ring_ok false (and the point-ordered sweep in use) when the rings do not fit
+// in LDS or the records do not fit in memory.
 static int setup_ring(const Plan& p, State& s, std::string& err) {
   s.ring_ok = false;
-  std::vector<int32_t> top_index(p.npts, -1);
-  for (size_t b = 0; b < p.bctop.size(); b++) top_index[p.bctop[b]] = (int32_t)b;
-  TRY(upload(s, s.top_index, top_index.data(), top_index.size(), err));
-  std::vector<int32_t> perm((size_t)8 * s.nptsp);
-  s.oct_pos_h.assign((size_t)8 * p.npts, 0);
-  int C = 0, npre_max = 0;
-  bool complete = true;
-  s.max_swept = 0;
-  s.gld = s.nlevels_max + 2;
+  bool used[8];
+  for (int o = 0; o < 8; o++) used[o] = s.oct_nord[o] > 0;
+  RingPlan rp;
+  if (!build_ring_plan(p, used, rp, err)) return AT3D_ERR_NUMERIC;
+  s.c4 = rp.c4;
+  s.c9 = rp.c9;
+  s.gld = rp.nlev_max + 2;
+  const size_t np = p.npts;
+  std::vector<int32_t> perm((size_t)8 * s.nptsp, 0), code(8 * np, 0), cell(8 * np, -1), lev(8 * np, 0);
+  std::vector<int32_t> roff((size_t)8 * s.gld, 0), soff((size_t)8 * (s.gld + 1), 0), sp0;
+  std::vector<int8_t> staged(8 * np, 0);
+  std::vector<uint16_t> sslot;
+  s.oct_pos_h.assign(8 * np, 0);
   std::vector<int4> tiles;
+  s.nsectors = 0;
   for (int o = 0; o < 8; o++) {
-    const OctantPlan& op = p.oct[o];
-    const std::vector<int32_t>& bc = ((o >> 2) & 1) ? p.bcbot : p.bctop;
+    s.ring_nlev[o] = 0;
+    s.sec_base[o] = (int)sp0.size();
     int32_t* pm = perm.data() + (size_t)o * s.nptsp;
-    size_t q = 0;
-    for (int b : bc) pm[q++] = b;
-    for (int v : op.lvl_pts) pm[q++] = v >= 0 ? v : ~v;
-    if (q != (size_t)p.npts) {
-      complete = false;
-      continue;
+    for (int q = 0; q < s.nptsp; q++) pm[q] = q;  // unused octants and padding rows
+    for (size_t i = 0; i < np; i++) s.oct_pos_h[o * np + i] = (int32_t)i;
+    if (!used[o]) continue;
+    const RingOctant& ro = rp.oct[o];
+    s.ring_nlev[o] = ro.nlev;
+    const std::vector<int32_t>& bc = ((o >> 2) & 1) ? p.bcbot : p.bctop;
+    for (size_t q = 0; q < np; q++) {
+      pm[q] = ro.code[q] >= 0 ? ro.code[q] : bc[-1 - ro.code[q]];
+      code[o * np + q] = ro.code[q];
+      cell[o * np + q] = ro.cell[q];
+      lev[o * np + q] = ro.level[q];
+      staged[o * np + q] = ro.staged[q];
+      s.oct_pos_h[o * np + q] = ro.pos[q];
     }
-    for (; q < (size_t)s.nptsp; q++) pm[q] = (int32_t)q;  // zero padding rows
-    for (int i = 0; i < p.npts; i++) s.oct_pos_h[(size_t)o * p.npts + pm[i]] = i;
-    if (s.oct_nord[o] <= 0) continue;
-    npre_max = std::max(npre_max, (int)bc.size());
-    s.max_swept = std::max(s.max_swept, op.lvl_off[op.nlevels]);
-    for (int L = 1; L <= op.nlevels; L++)
-      C = std::max(C, op.lvl_off[L] - op.lvl_off[std::max(L - 1 - RING_LEVELS, 0)]);
+    for (int L = 0; L <= ro.nlev; L++) roff[(size_t)o * s.gld + L] = ro.off[L];
+    for (int L = 0; L <= ro.nlev + 1; L++) soff[(size_t)o * (s.gld + 1) + L] = ro.sec_off[L];
+    sp0.insert(sp0.end(), ro.sec_p0.begin(), ro.sec_p0.end());
+    sslot.insert(sslot.end(), ro.sec_slot.begin(), ro.sec_slot.end());
     for (int c = s.oct_col_base[o]; c < s.oct_col_base[o] + s.oct_nord[o]; c += 48)
       tiles.push_back(make_int4(o, c, std::min(48, s.oct_col_base[o] + s.oct_nord[o] - c), 0));
   }
+  s.nsectors = (int)sp0.size();
   int dev = 0, maxlds = 0;
   HIPCHK(hipGetDevice(&dev));
   HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-  s.ring_c = std::max(C, 1);
-  s.ring_lds = ring_lds_bytes(npre_max, s.ring_c, s.gld);
-  if (!complete || s.max_swept <= 0 || npre_max + s.ring_c > 65535 || s.ring_lds > (size_t)maxlds ||
+  s.ring_lds = ring_lds_bytes(s.c9, s.c4, s.gld);
+  if (s.c9 > 65536 || p.npts >= (1 << 29) || s.ring_lds > (size_t)maxlds ||
       sweep_ring_max_lds(s.ring_lds) != 0)
     return AT3D_OK;
-  const size_t nrec = (size_t)p.nang * s.max_swept * 2;
+  const size_t nrec = (size_t)p.nang * np * 2;
   if (hipMalloc((void**)&s.rec, nrec * sizeof(uint4)) != hipSuccess) {
     (void)hipGetLastError();
     s.rec = nullptr;
     return AT3D_OK;
   }
   s.allocs.push_back(s.rec);
+  std::vector<int32_t> top_index(p.npts, -1);
+  for (size_t b = 0; b < p.bctop.size(); b++) top_index[p.bctop[b]] = (int32_t)b;
+  TRY(upload(s, s.top_index, top_index.data(), top_index.size(), err));
   TRY(upload(s, s.oct_perm, perm.data(), perm.size(), err));
   TRY(upload(s, s.oct_pos, s.oct_pos_h.data(), s.oct_pos_h.size(), err));
+  TRY(upload(s, s.ring_code, code.data(), code.size(), err));
+  TRY(upload(s, s.ring_cell, cell.data(), cell.size(), err));
+  TRY(upload(s, s.ring_level, lev.data(), lev.size(), err));
+  TRY(upload(s, s.ring_staged, staged.data(), staged.size(), err));
+  TRY(upload(s, s.ring_off, roff.data(), roff.size(), err));
+  TRY(upload(s, s.sec_off, soff.data(), soff.size(), err));
+  if (sp0.empty()) sp0.push_back(0);
+  sslot.resize(std::max<size_t>(sslot.size(), 8), 0);
+  TRY(upload(s, s.sec_p0, sp0.data(), sp0.size(), err));
+  TRY(upload(s, s.sec_slot, reinterpret_cast<const uint4*>(sslot.data()), sslot.size() / 8, err));
   TRY(upload(s, s.sh_tiles, tiles.data(), tiles.size(), err));
   s.n_sh_tiles = (int)tiles.size();
   TRY(alloc0(s, s.gflag, (size_t)p.nang * s.gld, err));
+  TRY(alloc0(s, s.col_general, (size_t)p.nang, err));
   TRY(alloc0(s, s.ngeneral_d, (size_t)1, err));
-  launch_sweep_records(s.stream, sweep_args(s), p.nang, s.max_swept);
+  launch_sweep_records(s.stream, sweep_args(s));
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(&s.ngeneral, s.ngeneral_d, sizeof(int), hipMemcpyDeviceToHost, s.stream));
   HIPCHK(hipStreamSynchronize(s.stream));
@@ -335,6 +375,7 @@ int state_create(const Plan& p, State& s, std::string& err) {
   TRY(upload(s, s.R0, r0.data(), r0.size(), err));
   TRY(setup_ring(p, s, err));
   for (int e = 0; e < State::NEV; e++) HIPCHK(hipEventCreate(&s.ev[e]));
+  for (int e = 0; e < 4; e++) HIPCHK(hipEventCreate(&s.ev_sweep[e]));
   return AT3D_OK;
 }
 
@@ -354,6 +395,8 @@ void state_free(State& s) {
   s.gbuf_bytes = 0;
   for (int e = 0; e < State::NEV; e++)
     if (s.ev[e]) (void)hipEventDestroy(s.ev[e]);
+  for (int e = 0; e < 4; e++)
+    if (s.ev_sweep[e]) (void)hipEventDestroy(s.ev_sweep[e]);
   if (s.stream) (void)hipStreamDestroy(s.stream);
   s.stream = nullptr;
 }
@@ -432,22 +475,33 @@ static void enqueue_sweeps(State& s) {
   const SweepArgs a = sweep_args(s);
   const float albpi = p.d.gndalbedo / acosf(-1.0f);
   const bool ring = ring_active(s);
-  launch_sweep_preset(s.stream, a, false, 0, s.ndown);
-  if (ring)
+  // the ring sweep writes the boundary radiances itself; events around its
+  // launches (never captured into a graph) time the sweep kernel alone
+  if (ring) {
+    (void)hipEventRecord(s.ev_sweep[0], s.stream);
     launch_sweep_ring(s.stream, a, 0, s.ndown, s.ring_lds);
-  else
+    (void)hipEventRecord(s.ev_sweep[1], s.stream);
+  } else {
+    launch_sweep_preset(s.stream, a, false, 0, s.ndown);
     launch_sweep_columns(s.stream, a, 0, s.ndown);
+  }
+  BottomRecs br{ring ? s.rec : nullptr, p.npts, s.ndown, p.nang, s.col_oct, s.oct_pos};
   launch_bottom_bc(s.stream, s.dorad, s.nptsp, s.bcbot, (int)p.bcbot.size(), s.col_of_ref_d, s.fluxw,
-                   s.ndown, s.dirflux, albpi, s.bcrad_bot);
-  launch_sweep_preset(s.stream, a, true, s.ndown, p.nang - s.ndown);
-  if (ring)
+                   s.ndown, s.dirflux, albpi, s.bcrad_bot, br);
+  if (ring) {
+    (void)hipEventRecord(s.ev_sweep[2], s.stream);
     launch_sweep_ring(s.stream, a, s.ndown, p.nang - s.ndown, s.ring_lds);
-  else
+    (void)hipEventRecord(s.ev_sweep[3], s.stream);
+  } else {
+    launch_sweep_preset(s.stream, a, true, s.ndown, p.nang - s.ndown);
     launch_sweep_columns(s.stream, a, s.ndown, p.nang - s.ndown);
+  }
 }
 
 static int run_sweeps(State& s, std::string& err) {
-  if (!s.use_graph) {
+  // the level-ordered sweep is three launches per pass (and is timed by
+  // events between them); the point-ordered one is replayed from a graph
+  if (!s.use_graph || ring_active(s)) {
     enqueue_sweeps(s);
     HIPCHK(hipGetLastError());
     return AT3D_OK;
@@ -550,6 +604,12 @@ int state_solve(State& s, int maxiter, int verbose, std::string& err) {
     s.tsum[2] += ms[2];  // sweeps (both hemispheres, surface boundary)
     s.tsum[3] += ms[3];  // DO -> SH
     s.tsum[4] += ms[4];  // source + reduction
+    if (ring_active(s)) {
+      float a0 = 0.f, a1 = 0.f;
+      (void)hipEventElapsedTime(&a0, s.ev_sweep[0], s.ev_sweep[1]);
+      (void)hipEventElapsedTime(&a1, s.ev_sweep[2], s.ev_sweep[3]);
+      s.tsum[6] += a0 + a1;  // the two sweep kernel launches alone
+    }
     s.nsweep_launch += 1;
     s.deljdot = (float)dots[0];
     s.deljold = (float)dots[1];

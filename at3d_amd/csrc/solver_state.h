@@ -56,12 +56,18 @@ struct State {
   // level-ordered sweep (k_sweep_ring)
   bool ring_ok = false;        // ring fits in LDS and records are built
   int use_ring = 1;            // option: sweep with k_sweep_ring when ring_ok
-  int ring_c = 0, gld = 0, max_swept = 0, ngeneral = 0;
+  int c4 = 0, c9 = 0, gld = 0, ngeneral = 0, nsectors = 0;
+  int ring_nlev[8] = {}, sec_base[8] = {};
   size_t ring_lds = 0;         // dynamic LDS bytes of k_sweep_ring
   int32_t *oct_perm = nullptr, *oct_pos = nullptr, *top_index = nullptr;
+  int32_t *ring_code = nullptr, *ring_cell = nullptr, *ring_level = nullptr, *ring_off = nullptr;
+  int8_t* ring_staged = nullptr;
+  int32_t *sec_off = nullptr, *sec_p0 = nullptr;
+  uint4* sec_slot = nullptr;
   std::vector<int32_t> oct_pos_h;   // [8][npts]
   uint4* rec = nullptr;
   uint8_t* gflag = nullptr;
+  int* col_general = nullptr;
   int* ngeneral_d = nullptr;
   int4* sh_tiles = nullptr;
   int n_sh_tiles = 0;
@@ -78,6 +84,7 @@ struct State {
   float solcrit = 1.0f, deljdot = 0, deljold = 0, deljnew = 0, jnorm = 0;
   static const int NEV = 8;
   hipEvent_t ev[NEV] = {};
+  hipEvent_t ev_sweep[4] = {};  // around the two ring sweep launches of an iteration
   double tsum[8] = {};  // ms per kernel class over the last solve
   std::vector<void*> allocs;
   // rendering

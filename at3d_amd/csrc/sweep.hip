@@ -26,6 +26,7 @@
 #include <limits.h>
 
 #include "kernels.h"
+#include "ring_plan.h"
 
 namespace at3d {
 
@@ -229,176 +230,221 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_columns(SweepArgs a, in
 // Level-ordered sweep with an LDS ring
 // ---------------------------------------------------------------------------
 
-// Once per solve: the first cell step of every point's characteristic for
-// every ordinate (the loop body of BACK_INT_GRID3D_UNPOL, shdomsub1.f:3822-3920,
-// with the cell integral of :3880-3905).  A step whose face is known (boundary
-// point or earlier point of the sweep order) within the last RING_LEVELS
-// levels becomes a record whose source integral is written as coefficients of
-// the face and point source values (the algebra of cell_integral rearranged:
-// both of its branches are linear in SRCEXT0 and SRCEXT1 before the clamp).
-// Every other point is marked for the general trace and its level flagged.
+// Once per solve: the first cell step of every point's characteristic for every
+// ordinate (the loop body of BACK_INT_GRID3D_UNPOL, shdomsub1.f:3822-3920, with
+// the cell integral of :3880-3905), in the ordinate's ring order.  A step whose
+// face corners are known (boundary points or earlier points of the sweep order)
+// and lie within the last RING_WINDOW levels becomes a one-cell record, its
+// source integral written as coefficients of the face and point source values
+// (the algebra of cell_integral rearranged: both of its branches are linear in
+// SRCEXT0 and SRCEXT1 before the clamp).  Every other point is marked for the
+// general trace and its level flagged; boundary entries carry their index.
+//
+// A wavefront takes one entry of one octant for up to 64 of the octant's
+// ordinates (lane = ordinate), so the point's geometry, ranks and levels are
+// read once per wavefront and each lane writes one 32-byte record.
+__device__ inline void sweep_record(const SweepArgs& a, int o, int col, int jl);
+
 __global__ __launch_bounds__(256) void k_sweep_records(SweepArgs a) {
+  const int o = blockIdx.y;
+  const int jl = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (a.ring_nlev[o] <= 0 || jl >= a.npts) return;
+  for (int k = threadIdx.x & 63; k < a.oct_nord[o]; k += 64) sweep_record(a, o, a.oct_col0[o] + k, jl);
+}
+
+__device__ inline void sweep_record(const SweepArgs& a, int o, int col, int jl) {
 #pragma clang fp contract(off)
-  const int col = blockIdx.y;
-  const int o = a.col_oct[col];
-  const int nlev = a.nlev[o];
-  if (nlev <= 0) return;
-  const int32_t* off = a.lvl_off + a.lo_base[o];
-  const int base = off[0];
-  const int jl = blockIdx.x * blockDim.x + threadIdx.x;
-  if (jl >= off[nlev] - base) return;
+  const size_t ob = (size_t)o * a.npts;
   const OrdGeom og = a.ordg[col];
-  const int up = og.bitz;
-  const int presetbit = up ? 2 : 1;
-  const int npre = up ? a.nbot : a.ntop;
-  const int32_t* pidx = up ? a.bot_index : a.top_index;
-  const int c = a.lvl_pts[base + jl];
-  const int pt = c >= 0 ? c : ~c;
-  const int32_t* rank = a.rank + (size_t)o * a.npts;
-  const int32_t* level = a.level + (size_t)o * a.npts;
-  const int32_t* pos = a.oct_pos + (size_t)o * a.npts;
-  const int L = level[pt];
+  const int presetbit = og.bitz ? 2 : 1;
+  const int code = a.ring_code[ob + jl];
+  const int32_t* rank = a.rank + ob;
+  const int32_t* rlev = a.ring_level + ob;
+  const int32_t* pos = a.oct_pos + ob;
   SweepRec r;
-  r.pt = ~pt;
   r.r[0] = r.r[1] = r.r[2] = r.r[3] = 0;
   r.u = r.v = r.b0 = r.b1 = r.T = 0.0f;
-  double xe = a.g.gridpos[pt * 3], ye = a.g.gridpos[pt * 3 + 1], ze = a.g.gridpos[pt * 3 + 2];
-  TraceStep t;
-  bool ok = trace_step(a.g, og, a.lvl_cell[base + jl], xe, ye, ze, t);
-  for (int k = 0; k < 4 && ok; k++) {
-    const int q = t.i[k];
-    if (a.preset[q] & presetbit) {
-      ok = pidx[q] >= 0;
-      r.r[k] = (uint16_t)pidx[q];
-    } else {
-      const int lq = level[q];
-      ok = rank[q] < rank[pt] && lq >= 1 && lq < L && lq >= L - RING_LEVELS;
-      r.r[k] = (uint16_t)(npre + (pos[q] - npre) % a.ring_c);
-    }
-  }
-  if (ok) {
-    const float* ext = a.ext;
-    const double f1 = (1 - t.u) * (1 - t.v), f2 = (1 - t.u) * t.v;
-    const double f3 = t.u * (1 - t.v), f4 = t.u * t.v;
-    const double ext0 = f1 * ext[t.i[0]] + f2 * ext[t.i[1]] + f3 * ext[t.i[2]] + f4 * ext[t.i[3]];
-    const double ext1 = ext[pt], so = t.so;
-    const double e = 0.5 * (ext0 + ext1);
-    const double tau = e * so;
-    double absc, trc;
-    if (tau >= 0.5) {
-      trc = exp(-tau);
-      absc = 1.0 - trc;
-    } else {
-      absc = tau * (1.0 - 0.5 * tau * (1.0 - (double)0.33333333333f * tau * (1 - 0.25 * tau)));
-      trc = 1.0 - absc;
-    }
-    double c0, c1;  // src = c0 * SRCEXT0 + c1 * SRCEXT1 before the clamp at zero
-    if (tau <= 2.0) {
-      if (e == 0.0) {
-        c0 = c1 = 0.0;
-      } else {
-        const double k = (double)0.08333333333f * so;
-        c0 = (0.5 - k * ext1) / e;
-        c1 = (0.5 + k * ext0) / e;
-      }
-    } else {
-      double e0p = ext0, sc = 1.0;
-      if (tau > 4.0) {
-        e0p = ext1 + (ext0 - ext1) * 4.0 / tau;
-        if (ext0 > 0.0) sc = e0p / ext0;
-      }
-      const double d = e0p + ext1;
-      const double g = 2.0 / d * (1 - 2 / tau + 2 * trc / absc);
-      c0 = sc * (1.0 - ext1 * g) / d;
-      c1 = (1.0 + e0p * g) / d;
-    }
-    r.pt = pt;
-    r.u = (float)t.u;
-    r.v = (float)t.v;
-    r.b0 = (float)(c0 * absc);
-    r.b1 = (float)(c1 * absc);
-    r.T = (float)trc;
+  if (code < 0) {
+    // boundary entry: its radiance is the boundary condition
+    const int b = -1 - code;
+    const int pt = (og.bitz ? a.bcbot : a.bctop)[b];
+    r.pt = (int32_t)((uint32_t)pt | ((uint32_t)a.ring_staged[ob + pt] << 29) | (2u << 30));
+    r.r[0] = (uint16_t)(b & 0xffff);
+    r.r[1] = (uint16_t)(b >> 16);
+    // the radiance itself in r[2..3]: the sky radiance of a downwelling
+    // ordinate here; the surface radiance of an upwelling one is written by
+    // k_bottom_bc every iteration
+    const uint32_t bits = __float_as_uint(og.bitz ? 0.0f : a.skyrad[col]);
+    r.r[2] = (uint16_t)(bits & 0xffff);
+    r.r[3] = (uint16_t)(bits >> 16);
   } else {
-    atomicAdd(a.ngeneral, 1);
-    const_cast<uint8_t*>(a.gflag)[(size_t)col * a.gld + L] = 1;
+    const int pt = code;
+    const int L = rlev[pt];
+    double xe = a.g.gridpos[pt * 3], ye = a.g.gridpos[pt * 3 + 1], ze = a.g.gridpos[pt * 3 + 2];
+    TraceStep t;
+    bool ok = trace_step(a.g, og, a.ring_cell[ob + jl], xe, ye, ze, t);
+    for (int k = 0; k < 4 && ok; k++) {
+      const int q = t.i[k];
+      const int lq = rlev[q];
+      ok = ((a.preset[q] & presetbit) || rank[q] < rank[pt]) && lq < L && lq >= L - RING_WINDOW;
+      r.r[k] = (uint16_t)(pos[q] % a.c9);
+    }
+    const int staged = a.ring_staged[ob + pt];
+    if (ok) {
+      const float* ext = a.ext;
+      const double f1 = (1 - t.u) * (1 - t.v), f2 = (1 - t.u) * t.v;
+      const double f3 = t.u * (1 - t.v), f4 = t.u * t.v;
+      const double ext0 = f1 * ext[t.i[0]] + f2 * ext[t.i[1]] + f3 * ext[t.i[2]] + f4 * ext[t.i[3]];
+      const double ext1 = ext[pt], so = t.so;
+      const double e = 0.5 * (ext0 + ext1);
+      const double tau = e * so;
+      double absc, trc;
+      if (tau >= 0.5) {
+        trc = exp(-tau);
+        absc = 1.0 - trc;
+      } else {
+        absc = tau * (1.0 - 0.5 * tau * (1.0 - (double)0.33333333333f * tau * (1 - 0.25 * tau)));
+        trc = 1.0 - absc;
+      }
+      double c0, c1;  // src = c0 * SRCEXT0 + c1 * SRCEXT1 before the clamp at zero
+      if (tau <= 2.0) {
+        if (e == 0.0) {
+          c0 = c1 = 0.0;
+        } else {
+          const double k = (double)0.08333333333f * so;
+          c0 = (0.5 - k * ext1) / e;
+          c1 = (0.5 + k * ext0) / e;
+        }
+      } else {
+        double e0p = ext0, sc = 1.0;
+        if (tau > 4.0) {
+          e0p = ext1 + (ext0 - ext1) * 4.0 / tau;
+          if (ext0 > 0.0) sc = e0p / ext0;
+        }
+        const double d = e0p + ext1;
+        const double g = 2.0 / d * (1 - 2 / tau + 2 * trc / absc);
+        c0 = sc * (1.0 - ext1 * g) / d;
+        c1 = (1.0 + e0p * g) / d;
+      }
+      r.pt = pt | (staged << 29);
+      r.u = (float)t.u;
+      r.v = (float)t.v;
+      r.b0 = (float)(c0 * absc);
+      r.b1 = (float)(c1 * absc);
+      r.T = (float)trc;
+    } else {
+      r.pt = (int32_t)((uint32_t)pt | ((uint32_t)staged << 29) | (1u << 30));
+      atomicAdd(a.ngeneral, 1);
+      atomicOr(a.col_general + col, 1);
+      const_cast<uint8_t*>(a.gflag)[(size_t)col * a.gld + L] = 1;
+    }
   }
   const uint4* w = reinterpret_cast<const uint4*>(&r);
-  uint4* dst = const_cast<uint4*>(a.rec) + ((size_t)col * a.ldr + jl) * 2;
+  uint4* dst = const_cast<uint4*>(a.rec) + ((size_t)col * a.npts + jl) * 2;
   dst[0] = w[0];
   dst[1] = w[1];
 }
 
 // Barrier that orders LDS traffic only: outstanding global loads (the next
-// level's records) and stores (radiances) stay in flight across it.
+// levels' records) and stores (radiances) stay in flight across it.
 __device__ inline void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-constexpr int RING_K = 5;  // points per thread per level with records in registers
+constexpr int RING_K = 5;  // entries per thread per level with records in registers
 
 struct RingSet {
   uint4 w0[RING_K], w1[RING_K];
   float s[RING_K];
+  uint4 sec;     // ring slots of the sector this thread writes after the level
+  int sec_p0;    // its first point, or -1
 };
 
-// One ordinate (workgroup) through every level of its octant.  LDS holds
-// (radiance, source) of the hemisphere's boundary points in slots
-// [0, npre) and of the last RING_LEVELS + 1 levels in a ring of ring_c slots
-// after them, followed by the octant's level offsets and the per-level
-// full-barrier flags.
+// One ordinate (workgroup) through every level of its octant.  LDS holds the
+// radiances of the last SECTOR_POINTS + 1 levels (ring of c9 slots) and the
+// sources of the last RING_WINDOW + 1 levels (c4 slots, c9 a multiple of c4,
+// so a point's source slot is its radiance slot mod c4), then the level
+// offsets and the full-barrier flags.  Radiances go to the point-ordered row
+// one 32-byte sector at a time, after the sector's last point, except in
+// columns with multi-cell traces (which read earlier radiances from the row)
+// and for points of sectors that span more levels.
 __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int col0) {
-  extern __shared__ float2 slot[];
+  extern __shared__ float lds[];
   const int col = col0 + blockIdx.x;
   const int o = a.col_oct[col];
-  const int nlev = a.nlev[o];
+  const int nlev = a.ring_nlev[o];
   const OrdGeom og = a.ordg[col];
   const int up = og.bitz;
-  const int npre = up ? a.nbot : a.ntop;
-  const int C = a.ring_c;
-  float2* ring = slot + npre;
-  int32_t* off = reinterpret_cast<int32_t*>(ring + C);
-  uint8_t* full = reinterpret_cast<uint8_t*>(off + a.gld);
-  const int32_t* goff = a.lvl_off + a.lo_base[o];
-  const int base = goff[0];
-  const float* se = a.srcext + (size_t)col * a.ldp;  // level order
-  const float* sl = se + npre;
+  const int C4 = a.c4, C9 = a.c9;
+  float* ring_rad = lds;
+  float* ring_src = lds + C9;
+  int32_t* off = reinterpret_cast<int32_t*>(ring_src + C4);
+  int32_t* soffl = off + a.gld;                     // [gld + 1] sectors by completion level
+  uint8_t* full = reinterpret_cast<uint8_t*>(soffl + a.gld + 1);
+  const int32_t* goff = a.ring_off + (size_t)o * a.gld;
+  const float* se = a.srcext + (size_t)col * a.ldp;  // ring order
   float* rad = a.rad + (size_t)col * a.ldp;          // point order
-  const uint4* rec = a.rec + (size_t)col * a.ldr * 2;
+  const uint4* rec = a.rec + (size_t)col * a.npts * 2;
   const uint8_t* gf = a.gflag + (size_t)col * a.gld;
   const int32_t* pos = a.oct_pos + (size_t)o * a.npts;
+  const int32_t* soff = a.sec_off + (size_t)o * (a.gld + 1);
+  const int32_t* sp0 = a.sec_p0 + a.sec_base[o];
+  const uint4* sslot = a.sec_slot + a.sec_base[o];
   const int tid = threadIdx.x;
   const float sky = a.skyrad[col];
-  for (int b = tid; b < npre; b += RING_THREADS) slot[b] = make_float2(up ? a.bcrad_bot[b] : sky, se[b]);
-  for (int L = tid; L <= nlev; L += RING_THREADS) off[L] = goff[L] - base;
+  const bool staging = a.col_general[col] == 0;
+  const int nbc = up ? a.nbot : a.ntop;
+  const int32_t* bc = up ? a.bcbot : a.bctop;
+  // boundary radiances (PATH_INTEGRATION shdomsub1.f:2042-2086) into the
+  // point-ordered row first: multi-cell traces read them there
+  for (int b = tid; b < nbc; b += RING_THREADS) rad[bc[b]] = up ? a.bcrad_bot[b] : sky;
+  for (int L = tid; L <= nlev; L += RING_THREADS) off[L] = goff[L];
+  for (int L = tid; L <= nlev + 1; L += RING_THREADS) soffl[L] = soff[L];
   for (int L = tid; L < a.gld; L += RING_THREADS) full[L] = gf[L];
-  const int nsw = goff[nlev] - base;
-
-  auto put = [&](int jl, int k0, float val, float s1) {
-    int k = k0 + jl;
-    if (k >= C) k -= C;
-    ring[k] = make_float2(val, s1);
+  const int nsw = goff[nlev];
+  const float rc4 = 1.0f / (float)C4;
+  auto mod4 = [&](uint32_t r) {  // radiance slot (< 65536) -> source slot, r mod C4
+    int m = (int)r - (int)((float)r * rc4) * C4;  // off by at most one C4 either way
+    m = m < 0 ? m + C4 : m;
+    return (uint32_t)(m >= C4 ? m - C4 : m);
   };
-  // One-cell record: LDS gathers and the cell integral.  Returns false for a
-  // multi-cell trace, which the level's second pass integrates.
-  auto finish = [&](const uint4& w0, const uint4& w1, float s1, int jl, int k0) {
-    const int pt = (int)w0.x;
-    if (pt < 0) return false;
-    const float2 q0 = slot[w0.y & 0xffffu], q1 = slot[w0.y >> 16];
-    const float2 q2 = slot[w0.z & 0xffffu], q3 = slot[w0.z >> 16];
-    const float u = __uint_as_float(w0.w), v = __uint_as_float(w1.x);
-    const float f1 = (1.f - u) * (1.f - v), f2 = (1.f - u) * v, f3 = u * (1.f - v), f4 = u * v;
-    const float s0 = f1 * q0.y + f2 * q1.y + f3 * q2.y + f4 * q3.y;
-    const float r0 = f1 * q0.x + f2 * q1.x + f3 * q2.x + f4 * q3.x;
-    const float val = fmaxf(0.f, __uint_as_float(w1.y) * s0 + __uint_as_float(w1.z) * s1) +
-                      __uint_as_float(w1.w) * r0;
-    rad[pt] = val;
-    put(jl, k0, val, s1);
+  auto put = [&](int jl, int k9, int k4, float val, float s1) {
+    int i9 = k9 + jl, i4 = k4 + jl;
+    if (i9 >= C9) i9 -= C9;
+    if (i4 >= C4) i4 -= C4;
+    ring_rad[i9] = val;
+    ring_src[i4] = s1;
+  };
+  // One-cell record or boundary entry: LDS gathers and the cell integral.
+  // Returns false for a multi-cell trace, left to the level's second pass.
+  auto finish = [&](const uint4& w0, const uint4& w1, float s1, int jl, int k9, int k4) {
+    const uint32_t head = w0.x;
+    const int kind = head >> 30;
+    if (kind == 1) return false;
+    const int pt = head & 0x1fffffff;
+    const bool store = !(staging && ((head >> 29) & 1));
+    float val;
+    if (kind == 2) {
+      val = __uint_as_float(w0.z);
+    } else {
+      const uint32_t r0 = w0.y & 0xffffu, r1 = w0.y >> 16, r2 = w0.z & 0xffffu, r3 = w0.z >> 16;
+      const float u = __uint_as_float(w0.w), v = __uint_as_float(w1.x);
+      const float f1 = (1.f - u) * (1.f - v), f2 = (1.f - u) * v, f3 = u * (1.f - v), f4 = u * v;
+      const float s0 = f1 * ring_src[mod4(r0)] + f2 * ring_src[mod4(r1)] + f3 * ring_src[mod4(r2)] +
+                       f4 * ring_src[mod4(r3)];
+      const float q0 = f1 * ring_rad[r0] + f2 * ring_rad[r1] + f3 * ring_rad[r2] + f4 * ring_rad[r3];
+      val = fmaxf(0.f, __uint_as_float(w1.y) * s0 + __uint_as_float(w1.z) * s1) +
+            __uint_as_float(w1.w) * q0;
+      if (store) rad[pt] = val;
+    }
+    put(jl, k9, k4, val, s1);
     return true;
   };
   // Records of a level, loaded unconditionally (indices clamped into the
-  // octant's records) so the loads of every lane are always in flight.
+  // column's records) so the loads of every lane are always in flight; with
+  // them the sectors that complete on the level before (written after level L).
   auto load = [&](int L, RingSet& S) {
     const int beg = __builtin_amdgcn_readfirstlane(off[min(L - 1, nlev)]);
 #pragma unroll
@@ -406,8 +452,39 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
       const int jl = min(beg + tid + k * RING_THREADS, nsw - 1);
       S.w0[k] = rec[2 * jl];
       S.w1[k] = rec[2 * jl + 1];
-      S.s[k] = sl[jl];
+      S.s[k] = se[jl];
     }
+    S.sec_p0 = -1;
+    S.sec = make_uint4(0, 0, 0, 0);
+    if (staging && L >= 2 && L <= nlev + 1) {
+      const int s0 = soffl[L - 1], s1 = soffl[L];
+      if (s1 > s0) {
+        const int sct = min(s0 + tid, s1 - 1);
+        S.sec = sslot[sct];
+        const int p0 = sp0[sct];
+        S.sec_p0 = (s0 + tid < s1) ? p0 : -1;
+      }
+    }
+  };
+  auto write_sector = [&](uint4 sl, int p0) {
+    float4 lo, hi;
+    lo.x = ring_rad[sl.x & 0xffffu];
+    lo.y = ring_rad[sl.x >> 16];
+    lo.z = ring_rad[sl.y & 0xffffu];
+    lo.w = ring_rad[sl.y >> 16];
+    hi.x = ring_rad[sl.z & 0xffffu];
+    hi.y = ring_rad[sl.z >> 16];
+    hi.z = ring_rad[sl.w & 0xffffu];
+    hi.w = ring_rad[sl.w >> 16];
+    float4* d = reinterpret_cast<float4*>(rad + p0);
+    d[0] = lo;
+    d[1] = hi;
+  };
+  // the whole sectors whose last point is on level L from index `first` on
+  // (those beyond one per thread, and those of the last level), not prefetched
+  auto flush_rest = [&](int L, int first) {
+    const int s1 = soffl[L + 1];
+    for (int sct = soffl[L] + first + tid; sct < s1; sct += RING_THREADS) write_sector(sslot[sct], sp0[sct]);
   };
   // Level L from `cur` while level L+2 streams into `fut` (its set was
   // consumed by level L-1): every record has two levels' time to arrive.
@@ -415,28 +492,34 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
     load(L + 2, fut);  // past the last level this re-reads the last record
     const int beg = __builtin_amdgcn_readfirstlane(off[L - 1]);
     const int end = __builtin_amdgcn_readfirstlane(off[L]);
-    const int k0 = beg % C - beg;  // ring slot of local position jl: k0 + jl, wrapped once
-    uint32_t gen = 0;              // this thread's multi-cell traces among its first RING_K
+    const int k9 = beg % C9 - beg, k4 = beg % C4 - beg;  // slot of position jl: k + jl, wrapped once
+    uint32_t gen = 0;  // this thread's multi-cell traces among its first RING_K entries
 #pragma unroll
     for (int k = 0; k < RING_K; k++) {
       const int jl = beg + tid + k * RING_THREADS;
-      if (jl < end && !finish(cur.w0[k], cur.w1[k], cur.s[k], jl, k0)) gen |= 1u << k;
+      if (jl < end && !finish(cur.w0[k], cur.w1[k], cur.s[k], jl, k9, k4)) gen |= 1u << k;
     }
     for (int jl = beg + tid + RING_K * RING_THREADS; jl < end; jl += RING_THREADS)
-      (void)finish(rec[2 * jl], rec[2 * jl + 1], sl[jl], jl, k0);
+      (void)finish(rec[2 * jl], rec[2 * jl + 1], se[jl], jl, k9, k4);
     if (__builtin_amdgcn_readfirstlane(full[L])) {
-      // multi-cell traces read radiances of earlier levels from global memory,
+      // multi-cell traces read radiances of earlier levels from the row,
       // complete and visible after the full barrier that ended level L-1
       int k = 0;
       for (int jl = beg + tid; jl < end; jl += RING_THREADS, k++) {
-        const bool g = k < RING_K ? ((gen >> k) & 1) != 0 : (int)rec[2 * jl].x < 0;
+        const uint32_t head = rec[2 * jl].x;
+        const bool g = k < RING_K ? ((gen >> k) & 1) != 0 : (head >> 30) == 1;
         if (!g) continue;
-        const int p = ~(int)rec[2 * jl].x;
-        const float s1 = sl[jl];
-        const float val = sweep_general(a, og, col, o, p, a.lvl_cell[base + jl], L, se, pos);
+        const int p = head & 0x1fffffff;
+        const float s1 = se[jl];
+        const float val = sweep_general(a, og, col, o, p, a.ring_cell[(size_t)o * a.npts + jl], L, se,
+                                        pos);
         rad[p] = val;
-        put(jl, k0, val, s1);
+        put(jl, k9, k4, val, s1);
       }
+    }
+    if (staging && L > 1) {
+      if (cur.sec_p0 >= 0) write_sector(cur.sec, cur.sec_p0);
+      if (soffl[L] - soffl[L - 1] > RING_THREADS) flush_rest(L - 1, RING_THREADS);
     }
     if (L < nlev && __builtin_amdgcn_readfirstlane(full[L + 1]))
       __syncthreads();
@@ -455,6 +538,7 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
   }
   if (L <= nlev) step(L, S0, S2);
   if (L + 1 <= nlev) step(L + 1, S1, S0);
+  if (staging && nlev >= 1) flush_rest(nlev, 0);
 }
 
 // Boundary radiances for one hemisphere: sky radiance at the top points for
@@ -472,30 +556,56 @@ __global__ void k_sweep_preset(SweepArgs a, int up, int col0, int ncol) {
 }
 
 // FIXED_LAMBERTIAN_BOUNDARY (shdomsub1.f:2438-2480) with the downwelling
-// hemispheric flux summed in the reference's ordinate order (:2118-2135).
-__global__ void k_bottom_bc(const float* __restrict__ rad, int ldp, const int32_t* __restrict__ bcbot,
-                            int nbot, const int32_t* __restrict__ col_of_ref,
-                            const float* __restrict__ fluxw, int ndown,
-                            const float* __restrict__ dirflux, float albpi,
-                            float* __restrict__ bcrad) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+// hemispheric flux of each bottom point (:2118-2135): one wavefront per point,
+// lane l summing the ordinates l, l+64, ... in reference order, then a fixed
+// butterfly over the lanes (deterministic; the reference sums serially).
+__global__ __launch_bounds__(256) void k_bottom_bc(const float* __restrict__ rad, int ldp,
+                                                   const int32_t* __restrict__ bcbot, int nbot,
+                                                   const int32_t* __restrict__ col_of_ref,
+                                                   const float* __restrict__ fluxw, int ndown,
+                                                   const float* __restrict__ dirflux, float albpi,
+                                                   float* __restrict__ bcrad, BottomRecs br) {
+  const int b = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (b >= nbot) return;
   const int p = bcbot[b];
   float f = 0.f;
-  for (int k = 0; k < ndown; k++) f = f + fluxw[k] * rad[(size_t)col_of_ref[k] * ldp + p];
-  bcrad[b] = albpi * (dirflux[p] + f);
+  for (int k = lane; k < ndown; k += 64) f = f + fluxw[k] * rad[(size_t)col_of_ref[k] * ldp + p];
+  for (int m = 32; m >= 1; m >>= 1) f = f + __shfl_xor(f, m);
+  const float v = albpi * (dirflux[p] + f);
+  if (lane == 0) bcrad[b] = v;
+  if (br.rec) {
+    // the surface radiance into the boundary entry (r[2..3], the third word)
+    // of every upwelling column's records
+    for (int c = br.col0 + lane; c < br.col1; c += 64) {
+      const int o = br.col_oct[c];
+      const size_t e = (size_t)c * br.npts + br.oct_pos[(size_t)o * br.npts + p];
+      reinterpret_cast<uint32_t*>(br.rec + e * 2)[2] = __float_as_uint(v);
+    }
+  }
 }
 
+// Hemispheric fluxes of every point, summed serially in reference order with
+// the loads of eight ordinates in flight.
 __global__ void k_fluxes(const float* __restrict__ rad, int ldp, const int32_t* __restrict__ col_of_ref,
                          const float* __restrict__ fluxw, int ndown, int nang, int npts,
                          float* __restrict__ fluxes) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npts) return;
-  float dn = 0.f, upf = 0.f;
-  for (int k = 0; k < ndown; k++) dn = dn + fluxw[k] * rad[(size_t)col_of_ref[k] * ldp + p];
-  for (int k = ndown; k < nang; k++) upf = upf + fluxw[k] * rad[(size_t)col_of_ref[k] * ldp + p];
-  fluxes[p] = dn;
-  fluxes[npts + p] = upf;
+  auto hemisphere = [&](int k0, int k1) {
+    float s = 0.f;
+    int k = k0;
+    for (; k + 8 <= k1; k += 8) {
+      float v[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) v[q] = rad[(size_t)col_of_ref[k + q] * ldp + p];
+#pragma unroll
+      for (int q = 0; q < 8; q++) s = s + fluxw[k + q] * v[q];
+    }
+    for (; k < k1; k++) s = s + fluxw[k] * rad[(size_t)col_of_ref[k] * ldp + p];
+    return s;
+  };
+  fluxes[p] = hemisphere(0, ndown);
+  fluxes[npts + p] = hemisphere(ndown, nang);
 }
 
 void launch_sweep_preset(hipStream_t st, const SweepArgs& a, bool up, int col0, int ncol) {
@@ -509,9 +619,8 @@ void launch_sweep_columns(hipStream_t st, const SweepArgs& a, int col0, int ncol
   k_sweep_columns<<<ncol, SWEEP_THREADS, 0, st>>>(a, col0);
 }
 
-void launch_sweep_records(hipStream_t st, const SweepArgs& a, int ncols, int max_swept) {
-  if (ncols <= 0 || max_swept <= 0) return;
-  k_sweep_records<<<dim3((max_swept + 255) / 256, ncols), 256, 0, st>>>(a);
+void launch_sweep_records(hipStream_t st, const SweepArgs& a) {
+  k_sweep_records<<<dim3((a.npts + 3) / 4, 8), 256, 0, st>>>(a);
 }
 
 void launch_sweep_ring(hipStream_t st, const SweepArgs& a, int col0, int ncol, size_t lds_bytes) {
@@ -528,10 +637,10 @@ int sweep_ring_max_lds(size_t lds_bytes) {
 
 void launch_bottom_bc(hipStream_t st, const float* rad, int ldp, const int32_t* bcbot, int nbot,
                       const int32_t* col_of_ref, const float* fluxw, int ndown,
-                      const float* dirflux, float albpi, float* bcrad) {
+                      const float* dirflux, float albpi, float* bcrad, const BottomRecs& br) {
   if (nbot <= 0) return;
-  k_bottom_bc<<<(nbot + 63) / 64, 64, 0, st>>>(rad, ldp, bcbot, nbot, col_of_ref, fluxw, ndown,
-                                                dirflux, albpi, bcrad);
+  k_bottom_bc<<<(nbot + 3) / 4, 256, 0, st>>>(rad, ldp, bcbot, nbot, col_of_ref, fluxw, ndown, dirflux,
+                                               albpi, bcrad, br);
 }
 
 void launch_fluxes(hipStream_t st, const float* rad, int ldp, const int32_t* col_of_ref,

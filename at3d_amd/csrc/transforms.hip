@@ -198,7 +198,22 @@ __global__ __launch_bounds__(256) void k_do_to_sh(const float* __restrict__ D,
   __shared__ float sB[2][TK][TN + PAD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
+  // XCD-aware tile order: workgroups are dealt to the eight XCDs round robin
+  // (each with its own L2), so the ld/TN SH tiles of one point tile are given
+  // workgroup ids that land on one XCD in a row; the point tile's DO
+  // radiances are then read from HBM once and from that XCD's L2 after.
+  const int ntn = gridDim.x, nwg = gridDim.x * gridDim.y;
+  const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+  int mt, nt;
+  if (nwg % (8 * ntn) == 0) {
+    const int xcd = bid & 7, local = bid >> 3;
+    mt = (local / ntn) * 8 + xcd;
+    nt = local % ntn;
+  } else {
+    mt = blockIdx.y;
+    nt = blockIdx.x;
+  }
+  const int m0 = mt * TM, n0 = nt * TN;
   float4_ acc[2][2];
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++) acc[i][j] = (float4_){0.f, 0.f, 0.f, 0.f};
@@ -257,7 +272,7 @@ void launch_sh_to_do_perm(hipStream_t st, const float* S, const float* Yc, const
 
 void launch_do_to_sh(hipStream_t st, const float* dorad, const float* Wc, const int* nr, float* R,
                      int mp, int ld, int ldo, int ldp, int npts) {
-  dim3 grid(mp / TM, ld / TN);
+  dim3 grid(ld / TN, mp / TM);
   k_do_to_sh<<<grid, 256, 0, st>>>(dorad, Wc, nr, R, ld, ldo, ldp, npts);
 }
 

@@ -147,6 +147,8 @@ class RTE:
         "cellflags": ("i1", lambda s: (s.plan_size("ncells"),)),
         "sweepord": ("i4", lambda s: (8, s._npts)), "level": ("i4", lambda s: (8, s._npts)),
         "lvl_pts": ("i4", lambda s: (int((s.plan_array("level") > 0).sum()),)),
+        "ring_level": ("i4", lambda s: (8, s._npts)), "ring_pos": ("i4", lambda s: (8, s._npts)),
+        "ring_staged": ("i4", lambda s: (8, s._npts)), "ring_sizes": ("i4", lambda s: (10,)),
         "chi": ("f4", lambda s: (s._npts, s._ml + 1)), "alb": ("f4", lambda s: (s._npts,)),
         "extinct": ("f4", lambda s: (s.plan_size("npart"), s._npts)),
         "albedo": ("f4", lambda s: (s.plan_size("npart"), s._npts)),

@@ -121,7 +121,7 @@ def main():
         rte.solve(w.maxiter, verbose=False)
     torch.cuda.synchronize()
 
-    keys = ["truncation", "sh_to_do", "sweep", "do_to_sh", "source", "total"]
+    keys = ["truncation", "sh_to_do", "sweep", "sweep_kernel", "do_to_sh", "source", "total"]
     ksum = {k: 0.0 for k in keys}
     iters_total = 0
     sweep_levels = int(rte.timing("sweep_levels"))
@@ -184,10 +184,13 @@ def main():
     # read the DO source and the extinction, write the DO radiance (3 x 4 B)
     sweep_bytes = nang * npts * 12.0
     ms_pass = ksum["sweep"] / n_it
+    # the sweep kernel alone (events around its two launches per pass); the
+    # pass also holds the surface boundary kernel
+    ms_launch = ksum["sweep_kernel"] / n_it / 2 if ksum["sweep_kernel"] > 0 else ms_pass / 2
     kern["sweep"] = {"ms_per_pass": ms_pass, "launches_per_pass": 2,
                      "dependency_levels": sweep_levels,
-                     "ms_per_launch": ms_pass / 2, "bytes_per_launch": sweep_bytes / 2,
-                     "GB/s": sweep_bytes / ms_pass / 1e6,
+                     "ms_per_launch": ms_launch, "bytes_per_launch": sweep_bytes / 2,
+                     "GB/s": sweep_bytes / 2 / ms_launch / 1e6,
                      "level_ordered": bool(rte.timing("sweep_ring")),
                      "lds_bytes": int(rte.timing("sweep_ring_lds")),
                      "multi_cell_points": int(rte.timing("sweep_general_points"))}
