@@ -74,6 +74,7 @@ struct SweepArgs {
   const uint8_t* gflag;      // [ncols][gld] level L holds a multi-cell trace
   int* col_general;          // [ncols] nonzero: the column has multi-cell traces
   int gld;
+  int ring_full;             // every column in the full instantiation
   int c4, c9;                // source and radiance ring slots
   int* ngeneral;             // multi-cell traces found by the record pass
 };
@@ -91,6 +92,7 @@ struct SweepRec {
 };
 static_assert(sizeof(SweepRec) == 32, "SweepRec is two 16-byte words");
 constexpr int RING_THREADS = 512;
+constexpr int RING_K = 5;  // entries per thread per level with records in registers
 
 // Bytes of LDS k_sweep_ring needs: c9 radiance and c4 source ring slots, gld
 // level offsets, gld + 1 sector offsets and gld level flags.
@@ -100,7 +102,10 @@ inline size_t ring_lds_bytes(int c9, int c4, int gld) {
 // Allows k_sweep_ring launches with lds_bytes of dynamic LDS; nonzero on failure.
 int sweep_ring_max_lds(size_t lds_bytes);
 void launch_sweep_records(hipStream_t st, const SweepArgs& a);
-void launch_sweep_ring(hipStream_t st, const SweepArgs& a, int col0, int ncol, size_t lds_bytes);
+// fast / full: launch the instantiation without / with the multi-cell,
+// overflow and extra-sector paths (a column runs in exactly one of them).
+void launch_sweep_ring(hipStream_t st, const SweepArgs& a, int col0, int ncol, size_t lds_bytes,
+                       bool fast, bool full);
 
 void launch_sweep_preset(hipStream_t st, const SweepArgs& a, bool up, int col0, int ncol);
 // One workgroup per ordinate column col0 .. col0+ncol-1, all levels of its octant.

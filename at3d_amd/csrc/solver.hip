@@ -131,6 +131,7 @@ static SweepArgs sweep_args(const State& s) {
   a.gflag = s.gflag;
   a.col_general = s.col_general;
   a.gld = s.gld;
+  a.ring_full = s.ring_full;
   a.c4 = s.c4;
   a.c9 = s.c9;
   a.ngeneral = s.ngeneral_d;
@@ -187,6 +188,16 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
       tiles.push_back(make_int4(o, c, std::min(48, s.oct_col_base[o] + s.oct_nord[o] - c), 0));
   }
   s.nsectors = (int)sp0.size();
+  // the fast instantiation holds a level in the threads' register slots and
+  // one sector per thread
+  s.ring_full = 0;
+  for (int o = 0; o < 8; o++) {
+    if (!used[o]) continue;
+    const RingOctant& ro = rp.oct[o];
+    for (int L = 1; L <= ro.nlev; L++)
+      if (ro.off[L] - ro.off[L - 1] > RING_K * RING_THREADS || ro.sec_off[L + 1] - ro.sec_off[L] > RING_THREADS)
+        s.ring_full = 1;
+  }
   int dev = 0, maxlds = 0;
   HIPCHK(hipGetDevice(&dev));
   HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
@@ -224,6 +235,9 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
   launch_sweep_records(s.stream, sweep_args(s));
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(&s.ngeneral, s.ngeneral_d, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+  s.col_general_h.assign(p.nang, 0);
+  HIPCHK(hipMemcpyAsync(s.col_general_h.data(), s.col_general, p.nang * sizeof(int32_t),
+                        hipMemcpyDeviceToHost, s.stream));
   HIPCHK(hipStreamSynchronize(s.stream));
   s.ring_ok = true;
   return AT3D_OK;
@@ -477,9 +491,20 @@ static void enqueue_sweeps(State& s) {
   const bool ring = ring_active(s);
   // the ring sweep writes the boundary radiances itself; events around its
   // launches (never captured into a graph) time the sweep kernel alone
+  // which ring instantiations each hemisphere's columns need
+  auto needs = [&](int c0, int c1, bool& fast, bool& full) {
+    fast = full = false;
+    for (int c = c0; c < c1; c++) {
+      const bool f = s.ring_full || s.col_general_h[c] != 0;
+      full = full || f;
+      fast = fast || !f;
+    }
+  };
+  bool fast = false, full = false;
   if (ring) {
+    needs(0, s.ndown, fast, full);
     (void)hipEventRecord(s.ev_sweep[0], s.stream);
-    launch_sweep_ring(s.stream, a, 0, s.ndown, s.ring_lds);
+    launch_sweep_ring(s.stream, a, 0, s.ndown, s.ring_lds, fast, full);
     (void)hipEventRecord(s.ev_sweep[1], s.stream);
   } else {
     launch_sweep_preset(s.stream, a, false, 0, s.ndown);
@@ -489,8 +514,9 @@ static void enqueue_sweeps(State& s) {
   launch_bottom_bc(s.stream, s.dorad, s.nptsp, s.bcbot, (int)p.bcbot.size(), s.col_of_ref_d, s.fluxw,
                    s.ndown, s.dirflux, albpi, s.bcrad_bot, br);
   if (ring) {
+    needs(s.ndown, p.nang, fast, full);
     (void)hipEventRecord(s.ev_sweep[2], s.stream);
-    launch_sweep_ring(s.stream, a, s.ndown, p.nang - s.ndown, s.ring_lds);
+    launch_sweep_ring(s.stream, a, s.ndown, p.nang - s.ndown, s.ring_lds, fast, full);
     (void)hipEventRecord(s.ev_sweep[3], s.stream);
   } else {
     launch_sweep_preset(s.stream, a, true, s.ndown, p.nang - s.ndown);

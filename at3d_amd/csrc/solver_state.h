@@ -68,6 +68,8 @@ struct State {
   uint4* rec = nullptr;
   uint8_t* gflag = nullptr;
   int* col_general = nullptr;
+  std::vector<int32_t> col_general_h;   // columns with multi-cell traces
+  int ring_full = 0;                    // every column needs the full ring instantiation
   int* ngeneral_d = nullptr;
   int4* sh_tiles = nullptr;
   int n_sh_tiles = 0;

@@ -353,13 +353,13 @@ __device__ inline void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-constexpr int RING_K = 5;  // entries per thread per level with records in registers
 
 struct RingSet {
   uint4 w0[RING_K], w1[RING_K];
   float s[RING_K];
   uint4 sec;     // ring slots of the sector this thread writes after the level
-  int sec_p0;    // its first point, or -1
+  int sec_p0;    // its first point
+  bool sec_ok;   // there is such a sector (known from indices, not from the loads)
 };
 
 // One ordinate (workgroup) through every level of its octant.  LDS holds the
@@ -370,9 +370,18 @@ struct RingSet {
 // one 32-byte sector at a time, after the sector's last point, except in
 // columns with multi-cell traces (which read earlier radiances from the row)
 // and for points of sectors that span more levels.
+//
+// Two instantiations: FULL handles every case (multi-cell traces, levels with
+// more entries than the threads' register slots, more sectors than threads);
+// the fast one has none of those paths, so its level loop carries nothing
+// but the record prefetch, and runs the columns (and grids) that need none
+// of them.  Each launch covers a hemisphere's columns; a workgroup whose
+// column belongs to the other instantiation returns at once.
+template <bool FULL>
 __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int col0) {
   extern __shared__ float lds[];
   const int col = col0 + blockIdx.x;
+  if (FULL != (a.ring_full || a.col_general[col] != 0)) return;
   const int o = a.col_oct[col];
   const int nlev = a.ring_nlev[o];
   const OrdGeom og = a.ordg[col];
@@ -454,16 +463,14 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
       S.w1[k] = rec[2 * jl + 1];
       S.s[k] = se[jl];
     }
-    S.sec_p0 = -1;
-    S.sec = make_uint4(0, 0, 0, 0);
+    S.sec_ok = false;
     if (staging && L >= 2 && L <= nlev + 1) {
       const int s0 = soffl[L - 1], s1 = soffl[L];
-      if (s1 > s0) {
-        const int sct = min(s0 + tid, s1 - 1);
-        S.sec = sslot[sct];
-        const int p0 = sp0[sct];
-        S.sec_p0 = (s0 + tid < s1) ? p0 : -1;
-      }
+      S.sec_ok = s0 + tid < s1;
+      // loaded unconditionally (clamped), used only if sec_ok: no wait here
+      const int sct = max(min(s0 + tid, s1 - 1), 0);
+      S.sec = sslot[sct];
+      S.sec_p0 = sp0[sct];
     }
   };
   auto write_sector = [&](uint4 sl, int p0) {
@@ -486,22 +493,77 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
     const int s1 = soffl[L + 1];
     for (int sct = soffl[L] + first + tid; sct < s1; sct += RING_THREADS) write_sector(sslot[sct], sp0[sct]);
   };
-  // Level L from `cur` while level L+2 streams into `fut` (its set was
-  // consumed by level L-1): every record has two levels' time to arrive.
+  // Level L from `cur` while level L+1 streams into `fut`.  All of the
+  // level's LDS reads (face corners: slots of levels L-3 .. L-1; the sector
+  // completed on level L-1: levels L-8 .. L-1) come before any of its LDS
+  // writes (slots of level L), so they are issued together and waited for once.
   auto step = [&](int L, RingSet& cur, RingSet& fut) {
-    load(L + 2, fut);  // past the last level this re-reads the last record
+    load(L + 1, fut);  // past the last level this re-reads the last record
     const int beg = __builtin_amdgcn_readfirstlane(off[L - 1]);
     const int end = __builtin_amdgcn_readfirstlane(off[L]);
     const int k9 = beg % C9 - beg, k4 = beg % C4 - beg;  // slot of position jl: k + jl, wrapped once
+    float gr[RING_K][4], gs[RING_K][4];
+#pragma unroll
+    for (int k = 0; k < RING_K; k++) {
+      const int jl = beg + tid + k * RING_THREADS;
+      const uint4 w0 = cur.w0[k];
+      if (jl < end && (w0.x >> 30) == 0) {
+        const uint32_t r[4] = {w0.y & 0xffffu, w0.y >> 16, w0.z & 0xffffu, w0.z >> 16};
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          gr[k][c] = ring_rad[r[c]];
+          gs[k][c] = ring_src[mod4(r[c])];
+        }
+      }
+    }
+    const bool dosec = staging && L > 1 && cur.sec_ok;
+    float4 slo = make_float4(0.f, 0.f, 0.f, 0.f), shi = slo;
+    if (dosec) {
+      const uint4 sl = cur.sec;
+      slo.x = ring_rad[sl.x & 0xffffu];
+      slo.y = ring_rad[sl.x >> 16];
+      slo.z = ring_rad[sl.y & 0xffffu];
+      slo.w = ring_rad[sl.y >> 16];
+      shi.x = ring_rad[sl.z & 0xffffu];
+      shi.y = ring_rad[sl.z >> 16];
+      shi.z = ring_rad[sl.w & 0xffffu];
+      shi.w = ring_rad[sl.w >> 16];
+    }
     uint32_t gen = 0;  // this thread's multi-cell traces among its first RING_K entries
 #pragma unroll
     for (int k = 0; k < RING_K; k++) {
       const int jl = beg + tid + k * RING_THREADS;
-      if (jl < end && !finish(cur.w0[k], cur.w1[k], cur.s[k], jl, k9, k4)) gen |= 1u << k;
+      if (jl >= end) continue;
+      const uint4 w0 = cur.w0[k], w1 = cur.w1[k];
+      const uint32_t head = w0.x;
+      const int kind = head >> 30;
+      if (kind == 1) {
+        gen |= 1u << k;
+        continue;
+      }
+      float val;
+      if (kind == 2) {
+        val = __uint_as_float(w0.z);
+      } else {
+        const float u = __uint_as_float(w0.w), v = __uint_as_float(w1.x);
+        const float f1 = (1.f - u) * (1.f - v), f2 = (1.f - u) * v, f3 = u * (1.f - v), f4 = u * v;
+        const float s0 = f1 * gs[k][0] + f2 * gs[k][1] + f3 * gs[k][2] + f4 * gs[k][3];
+        const float q0 = f1 * gr[k][0] + f2 * gr[k][1] + f3 * gr[k][2] + f4 * gr[k][3];
+        val = fmaxf(0.f, __uint_as_float(w1.y) * s0 + __uint_as_float(w1.z) * cur.s[k]) +
+              __uint_as_float(w1.w) * q0;
+        if (!(staging && ((head >> 29) & 1))) rad[head & 0x1fffffff] = val;
+      }
+      put(jl, k9, k4, val, cur.s[k]);
     }
-    for (int jl = beg + tid + RING_K * RING_THREADS; jl < end; jl += RING_THREADS)
-      (void)finish(rec[2 * jl], rec[2 * jl + 1], se[jl], jl, k9, k4);
-    if (__builtin_amdgcn_readfirstlane(full[L])) {
+    if (dosec) {
+      float4* d = reinterpret_cast<float4*>(rad + cur.sec_p0);
+      d[0] = slo;
+      d[1] = shi;
+    }
+    if (FULL)
+      for (int jl = beg + tid + RING_K * RING_THREADS; jl < end; jl += RING_THREADS)
+        (void)finish(rec[2 * jl], rec[2 * jl + 1], se[jl], jl, k9, k4);
+    if (FULL && __builtin_amdgcn_readfirstlane(full[L])) {
       // multi-cell traces read radiances of earlier levels from the row,
       // complete and visible after the full barrier that ended level L-1
       int k = 0;
@@ -517,27 +579,21 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
         put(jl, k9, k4, val, s1);
       }
     }
-    if (staging && L > 1) {
-      if (cur.sec_p0 >= 0) write_sector(cur.sec, cur.sec_p0);
-      if (soffl[L] - soffl[L - 1] > RING_THREADS) flush_rest(L - 1, RING_THREADS);
-    }
-    if (L < nlev && __builtin_amdgcn_readfirstlane(full[L + 1]))
+    if (FULL && staging && L > 1 && soffl[L] - soffl[L - 1] > RING_THREADS) flush_rest(L - 1, RING_THREADS);
+    if (FULL && L < nlev && __builtin_amdgcn_readfirstlane(full[L + 1]))
       __syncthreads();
     else
       lds_barrier();
   };
-  RingSet S0, S1, S2;
+  RingSet S0, S1;
   __syncthreads();
   load(1, S0);
-  load(2, S1);
   int L = 1;
-  for (; L + 2 <= nlev; L += 3) {
-    step(L, S0, S2);
+  for (; L + 1 <= nlev; L += 2) {
+    step(L, S0, S1);
     step(L + 1, S1, S0);
-    step(L + 2, S2, S1);
   }
-  if (L <= nlev) step(L, S0, S2);
-  if (L + 1 <= nlev) step(L + 1, S1, S0);
+  if (L == nlev) step(L, S0, S1);
   if (staging && nlev >= 1) flush_rest(nlev, 0);
 }
 
@@ -623,16 +679,20 @@ void launch_sweep_records(hipStream_t st, const SweepArgs& a) {
   k_sweep_records<<<dim3((a.npts + 3) / 4, 8), 256, 0, st>>>(a);
 }
 
-void launch_sweep_ring(hipStream_t st, const SweepArgs& a, int col0, int ncol, size_t lds_bytes) {
+void launch_sweep_ring(hipStream_t st, const SweepArgs& a, int col0, int ncol, size_t lds_bytes,
+                       bool fast, bool full) {
   if (ncol <= 0) return;
-  k_sweep_ring<<<ncol, RING_THREADS, lds_bytes, st>>>(a, col0);
+  if (fast) k_sweep_ring<false><<<ncol, RING_THREADS, lds_bytes, st>>>(a, col0);
+  if (full) k_sweep_ring<true><<<ncol, RING_THREADS, lds_bytes, st>>>(a, col0);
 }
 
 int sweep_ring_max_lds(size_t lds_bytes) {
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(k_sweep_ring),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) == hipSuccess
-             ? 0
-             : 1;
+  const bool ok =
+      hipFuncSetAttribute(reinterpret_cast<const void*>(k_sweep_ring<false>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) == hipSuccess &&
+      hipFuncSetAttribute(reinterpret_cast<const void*>(k_sweep_ring<true>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) == hipSuccess;
+  return ok ? 0 : 1;
 }
 
 void launch_bottom_bc(hipStream_t st, const float* rad, int ldp, const int32_t* bcbot, int nbot,
