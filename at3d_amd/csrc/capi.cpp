@@ -153,19 +153,20 @@ int at3d_plan_get(const at3d_plan* h, const char* name, void* dst, size_t nbytes
   if (k == "bcptr_bot") return copy_out(p.bcbot, dst, nbytes, err, errlen);
   if (k == "ring_level" || k == "ring_pos" || k == "ring_staged" || k == "ring_sizes") {
     // the level-ordered sweep's layout ([8][npts] per octant, zeros for
-    // octants without ordinates; ring_sizes = c4, c9, levels of each octant)
+    // octants without ordinates; ring_sizes = ring slots, points per sector,
+    // levels of each octant)
     bool used[8] = {};
     for (const at3d::Ordinate& od : p.ords) used[od.g.ioct] = true;
     at3d::RingPlan rp;
     std::string msg;
-    if (!at3d::build_ring_plan(p, used, rp, msg)) {
+    if (!at3d::build_ring_plan(p, used, 160 * 1024, rp, msg)) {
       set_err(err, errlen, msg);
       return AT3D_ERR_NUMERIC;
     }
     std::vector<int32_t> all;
     if (k == "ring_sizes") {
-      all.push_back(rp.c4);
-      all.push_back(rp.c9);
+      all.push_back(rp.slots);
+      all.push_back(rp.sector_points);
       for (int o = 0; o < 8; o++) all.push_back(rp.oct[o].nlev);
     } else {
       all.assign((size_t)8 * p.npts, 0);
@@ -458,7 +459,7 @@ double at3d_state_timing(const at3d_state* st, const char* key) {
   if (k == "sweep_kernel") return s.tsum[6];
   if (k == "sweep_ring") return (s.ring_ok && s.use_ring && !s.force_general) ? 1.0 : 0.0;
   if (k == "sweep_ring_lds") return (double)s.ring_lds;
-  if (k == "sweep_ring_slots") return s.c9;
+  if (k == "sweep_ring_slots") return s.slots;
   if (k == "sweep_sectors") return s.nsectors;
   if (k == "sweep_general_points") return s.ngeneral;
   if (k == "iters") return s.iters_last;

@@ -66,7 +66,7 @@ struct SweepArgs {
   int oct_col0[8], oct_nord[8];  // device columns of each octant
   const int32_t* sec_off;    // [8][gld+1] sectors completing at level L: [o*(gld+1)+L] .. [+L+1]
   const int32_t* sec_p0;     // sectors of all octants, octant o from sec_base[o]
-  const uint4* sec_slot;     // eight 16-bit radiance ring slots per sector
+  const uint4* sec_slot;     // eight 16-bit ring slots per sector (four used with 4-point sectors)
   int sec_base[8];
   const int32_t* top_index;  // [npts] index in bctop or -1
   const int32_t* bot_index;  // [npts] index in bcbot or -1
@@ -75,7 +75,8 @@ struct SweepArgs {
   int* col_general;          // [ncols] nonzero: the column has multi-cell traces
   int gld;
   int ring_full;             // every column in the full instantiation
-  int c4, c9;                // source and radiance ring slots
+  int slots;                 // ring slots
+  int sector_points;         // 8 or 4
   int* ngeneral;             // multi-cell traces found by the record pass
 };
 
@@ -94,11 +95,9 @@ static_assert(sizeof(SweepRec) == 32, "SweepRec is two 16-byte words");
 constexpr int RING_THREADS = 512;
 constexpr int RING_K = 5;  // entries per thread per level with records in registers
 
-// Bytes of LDS k_sweep_ring needs: c9 radiance and c4 source ring slots, gld
+// Bytes of LDS k_sweep_ring needs: the ring of (radiance, source) pairs, gld
 // level offsets, gld + 1 sector offsets and gld level flags.
-inline size_t ring_lds_bytes(int c9, int c4, int gld) {
-  return (size_t)(c9 + c4) * 4 + (size_t)gld * 9 + 4;
-}
+inline size_t ring_lds_bytes(int slots, int gld) { return (size_t)slots * 8 + (size_t)gld * 9 + 4; }
 // Allows k_sweep_ring launches with lds_bytes of dynamic LDS; nonzero on failure.
 int sweep_ring_max_lds(size_t lds_bytes);
 void launch_sweep_records(hipStream_t st, const SweepArgs& a);

@@ -5,9 +5,10 @@
 // (PATH_INTEGRATION :2042-2086).  The GPU sweep walks dependency levels (the
 // plan's levels, shifted up by one here) and keeps the last levels' values in
 // LDS; boundary points become entries of the level before their first
-// reader, so they travel through the same ring.  Sectors of eight consecutive
-// points (32 bytes of a point-ordered radiance row) whose points span at most
-// eight levels are written whole from the ring after their last point.
+// reader, so they travel through the same ring.  Sectors of eight (or four)
+// consecutive points (32 or 16 bytes of a point-ordered radiance row) whose
+// points span at most as many levels are written whole from the ring after
+// their last point.
 #include "ring_plan.h"
 
 #include <algorithm>
@@ -15,10 +16,9 @@
 
 namespace at3d {
 
-bool build_ring_plan(const Plan& p, const bool used[8], RingPlan& r, std::string& err) {
+bool build_ring_plan(const Plan& p, const bool used[8], size_t lds_budget, RingPlan& r,
+                     std::string& err) {
   const int npts = p.npts;
-  r.c4 = 1;
-  int c9raw = 1;
   r.nlev_max = 0;
   for (int o = 0; o < 8; o++) {
     RingOctant& ro = r.oct[o];
@@ -85,32 +85,48 @@ bool build_ring_plan(const Plan& p, const bool used[8], RingPlan& r, std::string
       ro.code[q] = -1 - (int32_t)b;
       ro.pos[pt] = q;
     }
-    for (int L = 1; L <= nl; L++) {
+    for (int L = 1; L <= nl; L++)
       if (fill[L - 1] != ro.off[L]) {
         err = "ring layout: level lists do not cover the grid";
         return false;
       }
-      r.c4 = std::max(r.c4, ro.off[L] - ro.off[std::max(L - 1 - RING_WINDOW, 0)]);
-      c9raw = std::max(c9raw, ro.off[L] - ro.off[std::max(L - 1 - SECTOR_POINTS, 0)]);
-    }
   }
-  r.c9 = r.c4 * ((c9raw + r.c4 - 1) / r.c4);
-  // sectors (need c9)
+  // a level reads RING_WINDOW levels back and writes a sector whose points
+  // span sector_points levels after its last one: the ring holds window
+  // consecutive levels
+  auto ring_slots = [&](int spts) {
+    const int window = std::max(RING_WINDOW, spts) + 1;
+    int m = 1;
+    for (int o = 0; o < 8; o++) {
+      if (!used[o]) continue;
+      const RingOctant& ro = r.oct[o];
+      for (int L = 1; L <= ro.nlev; L++) m = std::max(m, ro.off[L] - ro.off[std::max(L - window, 0)]);
+    }
+    return m;
+  };
+  r.sector_points = 8;
+  r.slots = ring_slots(8);
+  if (ring_plan_lds_bytes(r.slots, r.nlev_max) > lds_budget) {
+    r.sector_points = 4;
+    r.slots = ring_slots(4);
+  }
+  const int SP = r.sector_points;
+  // sectors (need the ring size)
   for (int o = 0; o < 8; o++) {
     if (!used[o]) continue;
     RingOctant& ro = r.oct[o];
     const int nl = ro.nlev;
     ro.staged.assign(npts, 0);
     std::vector<std::vector<int32_t>> by_level(nl + 1);
-    for (int s0 = 0; s0 + SECTOR_POINTS <= npts; s0 += SECTOR_POINTS) {
+    for (int s0 = 0; s0 + SP <= npts; s0 += SP) {
       int lo = INT_MAX, hi = 0;
-      for (int k = 0; k < SECTOR_POINTS; k++) {
+      for (int k = 0; k < SP; k++) {
         lo = std::min(lo, ro.level[s0 + k]);
         hi = std::max(hi, ro.level[s0 + k]);
       }
-      if (lo < hi - (SECTOR_POINTS - 1)) continue;
+      if (lo < hi - (SP - 1)) continue;
       by_level[hi].push_back(s0);
-      for (int k = 0; k < SECTOR_POINTS; k++) ro.staged[s0 + k] = 1;
+      for (int k = 0; k < SP; k++) ro.staged[s0 + k] = 1;
     }
     ro.sec_off.assign(nl + 2, 0);
     for (int L = 1; L <= nl; L++) ro.sec_off[L + 1] = ro.sec_off[L] + (int32_t)by_level[L].size();
@@ -119,7 +135,8 @@ bool build_ring_plan(const Plan& p, const bool used[8], RingPlan& r, std::string
     for (int L = 1; L <= nl; L++)
       for (int s0 : by_level[L]) {
         ro.sec_p0.push_back(s0);
-        for (int k = 0; k < SECTOR_POINTS; k++) ro.sec_slot.push_back((uint16_t)(ro.pos[s0 + k] % r.c9));
+        for (int k = 0; k < 8; k++)
+          ro.sec_slot.push_back(k < SP ? (uint16_t)(ro.pos[s0 + k] % r.slots) : (uint16_t)0);
       }
   }
   return true;

@@ -12,9 +12,10 @@ namespace at3d {
 // Per octant: the sweep levels with the hemisphere's boundary points inserted
 // as entries of their own (each one level before the first one-cell step that
 // reads it), so a sweep workgroup can keep every value it gathers in a ring of
-// recent levels; and the 32-byte sectors of the point-ordered radiance row
-// whose eight points all lie within eight consecutive levels, written whole
-// once their last point is done.
+// recent levels; and the sectors of the point-ordered radiance row (eight
+// points, 32 bytes, or four, 16 bytes, whichever ring fits in LDS) whose
+// points all lie within as many consecutive levels, written whole once their
+// last point is done.
 struct RingOctant {
   int nlev = 0;
   std::vector<int32_t> off;       // [nlev+1] level L spans positions off[L-1] .. off[L]
@@ -25,21 +26,25 @@ struct RingOctant {
   std::vector<int8_t> staged;     // [npts] 1: written by its sector
   std::vector<int32_t> sec_off;   // [nlev+2] sectors completing at level L: sec_off[L] .. sec_off[L+1]
   std::vector<int32_t> sec_p0;    // first point of each sector
-  std::vector<uint16_t> sec_slot; // [nsec][8] radiance ring slot of each of its points
+  std::vector<uint16_t> sec_slot; // [nsec][8] ring slot of each of its points (4 used with 4-point sectors)
 };
 
 constexpr int RING_WINDOW = 3;    // earlier levels a one-cell step may read through the ring
-constexpr int SECTOR_POINTS = 8;  // 32-byte sectors of the FP32 radiance rows
 
 struct RingPlan {
   RingOctant oct[8];
-  int c4 = 0;   // source ring slots: >= the entries of RING_WINDOW + 1 consecutive levels
-  int c9 = 0;   // radiance ring slots: a multiple of c4, >= the entries of SECTOR_POINTS + 1 levels
+  int sector_points = 8;  // 8 (32-byte sectors) or 4 (16-byte sectors)
+  int slots = 0;  // ring slots: >= the entries of max(RING_WINDOW, sector_points) + 1 consecutive levels
   int nlev_max = 0;
 };
 
-// Builds the ring layout for the octants with ordinates (used[o] != 0).
+// Bytes of LDS the level-ordered sweep needs for a ring of `slots` entries
+// and `nlev` levels.
+inline size_t ring_plan_lds_bytes(int slots, int nlev) { return (size_t)slots * 8 + (size_t)(nlev + 2) * 9 + 4; }
+
+// Builds the ring layout for the octants with ordinates (used[o] != 0), with
+// 32-byte sectors if that ring fits in lds_budget bytes, else 16-byte ones.
 // Returns false (err set) if the plan's level lists are incomplete.
-bool build_ring_plan(const Plan& p, const bool used[8], RingPlan& r, std::string& err);
+bool build_ring_plan(const Plan& p, const bool used[8], size_t lds_budget, RingPlan& r, std::string& err);
 
 }  // namespace at3d

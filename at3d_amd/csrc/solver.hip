@@ -132,8 +132,8 @@ static SweepArgs sweep_args(const State& s) {
   a.col_general = s.col_general;
   a.gld = s.gld;
   a.ring_full = s.ring_full;
-  a.c4 = s.c4;
-  a.c9 = s.c9;
+  a.slots = s.slots;
+  a.sector_points = s.sector_points;
   a.ngeneral = s.ngeneral_d;
   return a;
 }
@@ -149,10 +149,13 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
   s.ring_ok = false;
   bool used[8];
   for (int o = 0; o < 8; o++) used[o] = s.oct_nord[o] > 0;
+  int dev = 0, maxlds = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
   RingPlan rp;
-  if (!build_ring_plan(p, used, rp, err)) return AT3D_ERR_NUMERIC;
-  s.c4 = rp.c4;
-  s.c9 = rp.c9;
+  if (!build_ring_plan(p, used, (size_t)maxlds, rp, err)) return AT3D_ERR_NUMERIC;
+  s.sector_points = rp.sector_points;
+  s.slots = rp.slots;
   s.gld = rp.nlev_max + 2;
   const size_t np = p.npts;
   std::vector<int32_t> perm((size_t)8 * s.nptsp, 0), code(8 * np, 0), cell(8 * np, -1), lev(8 * np, 0);
@@ -198,11 +201,8 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
       if (ro.off[L] - ro.off[L - 1] > RING_K * RING_THREADS || ro.sec_off[L + 1] - ro.sec_off[L] > RING_THREADS)
         s.ring_full = 1;
   }
-  int dev = 0, maxlds = 0;
-  HIPCHK(hipGetDevice(&dev));
-  HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-  s.ring_lds = ring_lds_bytes(s.c9, s.c4, s.gld);
-  if (s.c9 > 65536 || p.npts >= (1 << 29) || s.ring_lds > (size_t)maxlds ||
+  s.ring_lds = ring_lds_bytes(s.slots, s.gld);
+  if (s.slots > 65536 || p.npts >= (1 << 29) || s.ring_lds > (size_t)maxlds ||
       sweep_ring_max_lds(s.ring_lds) != 0)
     return AT3D_OK;
   const size_t nrec = (size_t)p.nang * np * 2;

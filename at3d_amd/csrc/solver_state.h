@@ -56,7 +56,7 @@ struct State {
   // level-ordered sweep (k_sweep_ring)
   bool ring_ok = false;        // ring fits in LDS and records are built
   int use_ring = 1;            // option: sweep with k_sweep_ring when ring_ok
-  int c4 = 0, c9 = 0, gld = 0, ngeneral = 0, nsectors = 0;
+  int slots = 0, sector_points = 8, gld = 0, ngeneral = 0, nsectors = 0;
   int ring_nlev[8] = {}, sec_base[8] = {};
   size_t ring_lds = 0;         // dynamic LDS bytes of k_sweep_ring
   int32_t *oct_perm = nullptr, *oct_pos = nullptr, *top_index = nullptr;

@@ -287,7 +287,7 @@ __device__ inline void sweep_record(const SweepArgs& a, int o, int col, int jl) 
       const int q = t.i[k];
       const int lq = rlev[q];
       ok = ((a.preset[q] & presetbit) || rank[q] < rank[pt]) && lq < L && lq >= L - RING_WINDOW;
-      r.r[k] = (uint16_t)(pos[q] % a.c9);
+      r.r[k] = (uint16_t)(pos[q] % a.slots);
     }
     const int staged = a.ring_staged[ob + pt];
     if (ok) {
@@ -362,14 +362,14 @@ struct RingSet {
   bool sec_ok;   // there is such a sector (known from indices, not from the loads)
 };
 
-// One ordinate (workgroup) through every level of its octant.  LDS holds the
-// radiances of the last SECTOR_POINTS + 1 levels (ring of c9 slots) and the
-// sources of the last RING_WINDOW + 1 levels (c4 slots, c9 a multiple of c4,
-// so a point's source slot is its radiance slot mod c4), then the level
-// offsets and the full-barrier flags.  Radiances go to the point-ordered row
-// one 32-byte sector at a time, after the sector's last point, except in
-// columns with multi-cell traces (which read earlier radiances from the row)
-// and for points of sectors that span more levels.
+// One ordinate (workgroup) through every level of its octant.  LDS holds a
+// ring of (radiance, source) pairs of the last levels (a level reads
+// RING_WINDOW levels back; a sector written after a level spans up to eight
+// levels), then the level offsets, the sector offsets and the full-barrier
+// flags.  Radiances go to the point-ordered row one sector (32 or 16 bytes)
+// at a time, after the sector's last point, except in columns with
+// multi-cell traces (which read earlier radiances from the row) and for
+// points of sectors that span more levels.
 //
 // Two instantiations: FULL handles every case (multi-cell traces, levels with
 // more entries than the threads' register slots, more sectors than threads);
@@ -379,17 +379,15 @@ struct RingSet {
 // column belongs to the other instantiation returns at once.
 template <bool FULL>
 __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int col0) {
-  extern __shared__ float lds[];
+  extern __shared__ float2 ring[];
   const int col = col0 + blockIdx.x;
   if (FULL != (a.ring_full || a.col_general[col] != 0)) return;
   const int o = a.col_oct[col];
   const int nlev = a.ring_nlev[o];
   const OrdGeom og = a.ordg[col];
   const int up = og.bitz;
-  const int C4 = a.c4, C9 = a.c9;
-  float* ring_rad = lds;
-  float* ring_src = lds + C9;
-  int32_t* off = reinterpret_cast<int32_t*>(ring_src + C4);
+  const int C = a.slots;
+  int32_t* off = reinterpret_cast<int32_t*>(ring + C);
   int32_t* soffl = off + a.gld;                     // [gld + 1] sectors by completion level
   uint8_t* full = reinterpret_cast<uint8_t*>(soffl + a.gld + 1);
   const int32_t* goff = a.ring_off + (size_t)o * a.gld;
@@ -401,6 +399,7 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
   const int32_t* soff = a.sec_off + (size_t)o * (a.gld + 1);
   const int32_t* sp0 = a.sec_p0 + a.sec_base[o];
   const uint4* sslot = a.sec_slot + a.sec_base[o];
+  const bool wide = a.sector_points == 8;
   const int tid = threadIdx.x;
   const float sky = a.skyrad[col];
   const bool staging = a.col_general[col] == 0;
@@ -413,42 +412,32 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
   for (int L = tid; L <= nlev + 1; L += RING_THREADS) soffl[L] = soff[L];
   for (int L = tid; L < a.gld; L += RING_THREADS) full[L] = gf[L];
   const int nsw = goff[nlev];
-  const float rc4 = 1.0f / (float)C4;
-  auto mod4 = [&](uint32_t r) {  // radiance slot (< 65536) -> source slot, r mod C4
-    int m = (int)r - (int)((float)r * rc4) * C4;  // off by at most one C4 either way
-    m = m < 0 ? m + C4 : m;
-    return (uint32_t)(m >= C4 ? m - C4 : m);
+  auto put = [&](int jl, int k0, float val, float s1) {
+    int i = k0 + jl;
+    if (i >= C) i -= C;
+    ring[i] = make_float2(val, s1);
   };
-  auto put = [&](int jl, int k9, int k4, float val, float s1) {
-    int i9 = k9 + jl, i4 = k4 + jl;
-    if (i9 >= C9) i9 -= C9;
-    if (i4 >= C4) i4 -= C4;
-    ring_rad[i9] = val;
-    ring_src[i4] = s1;
-  };
-  // One-cell record or boundary entry: LDS gathers and the cell integral.
-  // Returns false for a multi-cell trace, left to the level's second pass.
-  auto finish = [&](const uint4& w0, const uint4& w1, float s1, int jl, int k9, int k4) {
+  // One-cell record or boundary entry (the unprefetched path): LDS gathers and
+  // the cell integral.  Returns false for a multi-cell trace.
+  auto finish = [&](const uint4& w0, const uint4& w1, float s1, int jl, int k0) {
     const uint32_t head = w0.x;
     const int kind = head >> 30;
     if (kind == 1) return false;
-    const int pt = head & 0x1fffffff;
-    const bool store = !(staging && ((head >> 29) & 1));
     float val;
     if (kind == 2) {
       val = __uint_as_float(w0.z);
     } else {
-      const uint32_t r0 = w0.y & 0xffffu, r1 = w0.y >> 16, r2 = w0.z & 0xffffu, r3 = w0.z >> 16;
+      const float2 q0 = ring[w0.y & 0xffffu], q1 = ring[w0.y >> 16];
+      const float2 q2 = ring[w0.z & 0xffffu], q3 = ring[w0.z >> 16];
       const float u = __uint_as_float(w0.w), v = __uint_as_float(w1.x);
       const float f1 = (1.f - u) * (1.f - v), f2 = (1.f - u) * v, f3 = u * (1.f - v), f4 = u * v;
-      const float s0 = f1 * ring_src[mod4(r0)] + f2 * ring_src[mod4(r1)] + f3 * ring_src[mod4(r2)] +
-                       f4 * ring_src[mod4(r3)];
-      const float q0 = f1 * ring_rad[r0] + f2 * ring_rad[r1] + f3 * ring_rad[r2] + f4 * ring_rad[r3];
+      const float s0 = f1 * q0.y + f2 * q1.y + f3 * q2.y + f4 * q3.y;
+      const float r0 = f1 * q0.x + f2 * q1.x + f3 * q2.x + f4 * q3.x;
       val = fmaxf(0.f, __uint_as_float(w1.y) * s0 + __uint_as_float(w1.z) * s1) +
-            __uint_as_float(w1.w) * q0;
-      if (store) rad[pt] = val;
+            __uint_as_float(w1.w) * r0;
+      if (!(staging && ((head >> 29) & 1))) rad[head & 0x1fffffff] = val;
     }
-    put(jl, k9, k4, val, s1);
+    put(jl, k0, val, s1);
     return true;
   };
   // Records of a level, loaded unconditionally (indices clamped into the
@@ -474,18 +463,19 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
     }
   };
   auto write_sector = [&](uint4 sl, int p0) {
-    float4 lo, hi;
-    lo.x = ring_rad[sl.x & 0xffffu];
-    lo.y = ring_rad[sl.x >> 16];
-    lo.z = ring_rad[sl.y & 0xffffu];
-    lo.w = ring_rad[sl.y >> 16];
-    hi.x = ring_rad[sl.z & 0xffffu];
-    hi.y = ring_rad[sl.z >> 16];
-    hi.z = ring_rad[sl.w & 0xffffu];
-    hi.w = ring_rad[sl.w >> 16];
-    float4* d = reinterpret_cast<float4*>(rad + p0);
-    d[0] = lo;
-    d[1] = hi;
+    float4 v;
+    v.x = ring[sl.x & 0xffffu].x;
+    v.y = ring[sl.x >> 16].x;
+    v.z = ring[sl.y & 0xffffu].x;
+    v.w = ring[sl.y >> 16].x;
+    *reinterpret_cast<float4*>(rad + p0) = v;
+    if (wide) {
+      v.x = ring[sl.z & 0xffffu].x;
+      v.y = ring[sl.z >> 16].x;
+      v.z = ring[sl.w & 0xffffu].x;
+      v.w = ring[sl.w >> 16].x;
+      *reinterpret_cast<float4*>(rad + p0 + 4) = v;
+    }
   };
   // the whole sectors whose last point is on level L from index `first` on
   // (those beyond one per thread, and those of the last level), not prefetched
@@ -495,39 +485,39 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
   };
   // Level L from `cur` while level L+1 streams into `fut`.  All of the
   // level's LDS reads (face corners: slots of levels L-3 .. L-1; the sector
-  // completed on level L-1: levels L-8 .. L-1) come before any of its LDS
+  // completed on level L-1: levels L-4 .. L-1) come before any of its LDS
   // writes (slots of level L), so they are issued together and waited for once.
   auto step = [&](int L, RingSet& cur, RingSet& fut) {
     load(L + 1, fut);  // past the last level this re-reads the last record
     const int beg = __builtin_amdgcn_readfirstlane(off[L - 1]);
     const int end = __builtin_amdgcn_readfirstlane(off[L]);
-    const int k9 = beg % C9 - beg, k4 = beg % C4 - beg;  // slot of position jl: k + jl, wrapped once
-    float gr[RING_K][4], gs[RING_K][4];
+    const int k0 = beg % C - beg;  // slot of position jl: k0 + jl, wrapped once
+    float2 g[RING_K][4];
 #pragma unroll
     for (int k = 0; k < RING_K; k++) {
       const int jl = beg + tid + k * RING_THREADS;
       const uint4 w0 = cur.w0[k];
       if (jl < end && (w0.x >> 30) == 0) {
-        const uint32_t r[4] = {w0.y & 0xffffu, w0.y >> 16, w0.z & 0xffffu, w0.z >> 16};
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-          gr[k][c] = ring_rad[r[c]];
-          gs[k][c] = ring_src[mod4(r[c])];
-        }
+        g[k][0] = ring[w0.y & 0xffffu];
+        g[k][1] = ring[w0.y >> 16];
+        g[k][2] = ring[w0.z & 0xffffu];
+        g[k][3] = ring[w0.z >> 16];
       }
     }
     const bool dosec = staging && L > 1 && cur.sec_ok;
-    float4 slo = make_float4(0.f, 0.f, 0.f, 0.f), shi = slo;
+    float4 sv = make_float4(0.f, 0.f, 0.f, 0.f), sw = sv;
     if (dosec) {
       const uint4 sl = cur.sec;
-      slo.x = ring_rad[sl.x & 0xffffu];
-      slo.y = ring_rad[sl.x >> 16];
-      slo.z = ring_rad[sl.y & 0xffffu];
-      slo.w = ring_rad[sl.y >> 16];
-      shi.x = ring_rad[sl.z & 0xffffu];
-      shi.y = ring_rad[sl.z >> 16];
-      shi.z = ring_rad[sl.w & 0xffffu];
-      shi.w = ring_rad[sl.w >> 16];
+      sv.x = ring[sl.x & 0xffffu].x;
+      sv.y = ring[sl.x >> 16].x;
+      sv.z = ring[sl.y & 0xffffu].x;
+      sv.w = ring[sl.y >> 16].x;
+      if (wide) {
+        sw.x = ring[sl.z & 0xffffu].x;
+        sw.y = ring[sl.z >> 16].x;
+        sw.z = ring[sl.w & 0xffffu].x;
+        sw.w = ring[sl.w >> 16].x;
+      }
     }
     uint32_t gen = 0;  // this thread's multi-cell traces among its first RING_K entries
 #pragma unroll
@@ -547,36 +537,35 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
       } else {
         const float u = __uint_as_float(w0.w), v = __uint_as_float(w1.x);
         const float f1 = (1.f - u) * (1.f - v), f2 = (1.f - u) * v, f3 = u * (1.f - v), f4 = u * v;
-        const float s0 = f1 * gs[k][0] + f2 * gs[k][1] + f3 * gs[k][2] + f4 * gs[k][3];
-        const float q0 = f1 * gr[k][0] + f2 * gr[k][1] + f3 * gr[k][2] + f4 * gr[k][3];
+        const float s0 = f1 * g[k][0].y + f2 * g[k][1].y + f3 * g[k][2].y + f4 * g[k][3].y;
+        const float r0 = f1 * g[k][0].x + f2 * g[k][1].x + f3 * g[k][2].x + f4 * g[k][3].x;
         val = fmaxf(0.f, __uint_as_float(w1.y) * s0 + __uint_as_float(w1.z) * cur.s[k]) +
-              __uint_as_float(w1.w) * q0;
+              __uint_as_float(w1.w) * r0;
         if (!(staging && ((head >> 29) & 1))) rad[head & 0x1fffffff] = val;
       }
-      put(jl, k9, k4, val, cur.s[k]);
+      put(jl, k0, val, cur.s[k]);
     }
     if (dosec) {
-      float4* d = reinterpret_cast<float4*>(rad + cur.sec_p0);
-      d[0] = slo;
-      d[1] = shi;
+      *reinterpret_cast<float4*>(rad + cur.sec_p0) = sv;
+      if (wide) *reinterpret_cast<float4*>(rad + cur.sec_p0 + 4) = sw;
     }
     if (FULL)
       for (int jl = beg + tid + RING_K * RING_THREADS; jl < end; jl += RING_THREADS)
-        (void)finish(rec[2 * jl], rec[2 * jl + 1], se[jl], jl, k9, k4);
+        (void)finish(rec[2 * jl], rec[2 * jl + 1], se[jl], jl, k0);
     if (FULL && __builtin_amdgcn_readfirstlane(full[L])) {
       // multi-cell traces read radiances of earlier levels from the row,
       // complete and visible after the full barrier that ended level L-1
       int k = 0;
       for (int jl = beg + tid; jl < end; jl += RING_THREADS, k++) {
         const uint32_t head = rec[2 * jl].x;
-        const bool g = k < RING_K ? ((gen >> k) & 1) != 0 : (head >> 30) == 1;
-        if (!g) continue;
+        const bool gn = k < RING_K ? ((gen >> k) & 1) != 0 : (head >> 30) == 1;
+        if (!gn) continue;
         const int p = head & 0x1fffffff;
         const float s1 = se[jl];
         const float val = sweep_general(a, og, col, o, p, a.ring_cell[(size_t)o * a.npts + jl], L, se,
                                         pos);
         rad[p] = val;
-        put(jl, k9, k4, val, s1);
+        put(jl, k0, val, s1);
       }
     }
     if (FULL && staging && L > 1 && soffl[L] - soffl[L - 1] > RING_THREADS) flush_rest(L - 1, RING_THREADS);

@@ -118,7 +118,7 @@ def _ring_case(w):
     pos = rte.plan_array("ring_pos")
     staged = rte.plan_array("ring_staged")
     sizes = rte.plan_array("ring_sizes")
-    return rte, lev, pos, staged, int(sizes[0]), int(sizes[1]), sizes[2:]
+    return rte, lev, pos, staged, int(sizes[0]), int(sizes[1]), sizes[2:]   # slots, sector points
 
 
 @pytest.mark.parametrize("make", [lambda: blob8("periodic"), lambda: blob8("open"), cloud_small,
@@ -126,14 +126,16 @@ def _ring_case(w):
                          ids=["blob8_periodic", "blob8_open", "cloud12", "cloud64"])
 def test_ring_layout_invariants(make):
     """Level-ordered sweep layout: per used octant the positions are a
-    permutation ordered by level; the source ring holds any four consecutive
-    levels and the radiance ring (a multiple of it) any nine; every boundary
-    point sits on a level before each point whose start cell has it on a far
-    face; staged sectors span at most eight levels."""
+    permutation ordered by level; the ring holds any max(3, s) + 1 consecutive
+    levels (a level reads three back and writes an s-point sector after it,
+    s = 8, or 4 when that ring would not fit in LDS); the swept points keep the
+    plan's levels (shifted by one); staged sectors are whole and span at most
+    s levels."""
     w = make()
-    rte, lev, pos, staged, c4, c9, nlev = _ring_case(w)
+    rte, lev, pos, staged, slots, spts, nlev = _ring_case(w)
     npts = lev.shape[1]
-    assert c9 % c4 == 0 and c9 <= 65536
+    assert spts in (4, 8) and slots <= 65536
+    win = max(3, spts) + 1
     lvl_plan = rte.plan_array("level")
     used = [o for o in range(8) if nlev[o] > 0]
     assert len(used) == 8
@@ -145,19 +147,20 @@ def test_ring_layout_invariants(make):
         assert np.all(np.diff(L[order]) >= 0)            # positions grouped by level
         cnt = np.bincount(L, minlength=nlev[o] + 1)[1:]
         off = np.concatenate([[0], np.cumsum(cnt)])
-        for span, cap in ((4, c4), (9, c9)):
-            assert max(off[k] - off[max(k - span, 0)] for k in range(1, len(off))) <= cap
+        assert max(off[k] - off[max(k - win, 0)] for k in range(1, len(off))) <= slots
         swept = lvl_plan[o] > 0
         assert np.array_equal(L[swept], lvl_plan[o][swept] + 1)
-        st = staged[o].reshape(-1, 8) if npts % 8 == 0 else staged[o][: npts // 8 * 8].reshape(-1, 8)
-        sec = L[: st.size].reshape(-1, 8)
+        n = npts // spts * spts
+        st = staged[o][:n].reshape(-1, spts)
+        sec = L[:n].reshape(-1, spts)
         full = st.all(axis=1)
         assert np.all(st.any(axis=1) == full)            # a sector is staged whole or not at all
-        assert np.all(sec[full].max(axis=1) - sec[full].min(axis=1) <= 7)
+        assert np.all(sec[full].max(axis=1) - sec[full].min(axis=1) <= spts - 1)
 
 
 def test_ring_layout_stages_most_of_the_benchmark_grid():
-    rte, lev, pos, staged, c4, c9, nlev = _ring_case(S.workload("cloud64"))
+    rte, lev, pos, staged, slots, spts, nlev = _ring_case(S.workload("cloud64"))
+    assert spts == 8            # 32-byte sectors: the nine-level ring fits
     assert staged.mean() > 0.9
-    # the rings fit next to each other in the 160 KiB of LDS of one workgroup
-    assert (c9 + c4) * 4 + (nlev.max() + 2) * 5 <= 160 * 1024
+    # the ring fits in the 160 KiB of LDS of one workgroup
+    assert slots * 8 + (nlev.max() + 2) * 9 + 4 <= 160 * 1024
