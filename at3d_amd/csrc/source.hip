@@ -35,14 +35,26 @@ __global__ void k_radiance_truncation(const float* __restrict__ R, int ld, const
 }
 
 // COMPUTE_SOURCE for NSTOKES=1 (shdomsub1.f:967-1611).  One wavefront per grid
-// point (four points per wave in turn); SOURCET_J = w chi_l (F0 Ysun_J + I_J),
-// the acceleration dot products accumulate in FP64 and are written as one
-// partial per block for a fixed-order reduction.
+// point (SRC_PTS_PER_BLOCK / 4 points per wave in turn), up to eight terms per
+// lane (NLM <= 512): every load of a point is issued before its arithmetic.
+// SOURCET_J = w chi_l (F0 Ysun_J + I_J); the acceleration dot products
+// accumulate in FP64 per lane in term order and are written as one partial per
+// block for a fixed-order reduction.
+constexpr int SRC_TERMS = 8;  // per lane: NLM <= 64 * SRC_TERMS
+
 __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
   __shared__ double red[4][4];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   double dot = 0, dold = 0, dnew = 0, jn = 0;
-  const int nmax = (a.nlm + 63) / 64;
+  // l(J) and Ysun_J of this lane's terms, the same for every point
+  int lv[SRC_TERMS];
+  float yv[SRC_TERMS];
+#pragma unroll
+  for (int it = 0; it < SRC_TERMS; it++) {
+    const int J = lane + 64 * it;
+    lv[it] = J < a.nlm ? a.lofj[J] : 0;
+    yv[it] = J < a.nlm ? a.ysun[J] : 0.f;
+  }
   for (int sub = 0; sub < SRC_PTS_PER_BLOCK / 4; sub++) {
     const int p = blockIdx.x * SRC_PTS_PER_BLOCK + sub * 4 + w;
     if (p >= a.npts) break;
@@ -52,28 +64,46 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
     float* Sp = a.S + (size_t)p * a.ld;
     float* DSp = a.DS + (size_t)p * a.ld;
     const float* chip = a.chi + (size_t)p * (a.ml + 1);
+    const bool old = !a.first;
+    const bool useds = old && a.accel;
+    float rv[SRC_TERMS], sv[SRC_TERMS], dv[SRC_TERMS], cv[SRC_TERMS];
+#pragma unroll
+    for (int it = 0; it < SRC_TERMS; it++) {
+      const int J = lane + 64 * it;
+      rv[it] = sv[it] = dv[it] = cv[it] = 0.f;
+      if (J < a.nlm) {
+        cv[it] = chip[lv[it]];
+        if (J < nrp) rv[it] = Rp[J];
+      }
+      if (J < a.ld) {
+        if (old) sv[it] = Sp[J];
+        if (useds) dv[it] = DSp[J];
+      }
+    }
     int js = -1;
-    float st[8];
-    for (int it = 0; it < nmax; it++) {
+    float st[SRC_TERMS];
+#pragma unroll
+    for (int it = 0; it < SRC_TERMS; it++) {
       const int J = lane + 64 * it;
       float v = 0.f;
       if (J < a.nlm) {
-        const float c = chip[a.lofj[J]];
-        v = f0 * alb * c * a.ysun[J];
-        if (J < nrp) v = v + alb * c * Rp[J];
+        const float c = cv[it];
+        v = f0 * alb * c * yv[it];
+        if (J < nrp) v = v + alb * c * rv[it];
         if (fabsf(v) > a.shacc) js = J;
       }
       st[it] = v;
     }
-    if (!a.first) {
+    if (old) {
       const int nsd = a.accel ? min(ns_old, nos_old) : ns_old;
-      for (int it = 0; it < nmax; it++) {
+#pragma unroll
+      for (int it = 0; it < SRC_TERMS; it++) {
         const int J = lane + 64 * it;
         if (J < nsd) {
-          const double so = Sp[J];
-          const double dn = (double)(st[it] - Sp[J]);
+          const double so = sv[it];
+          const double dn = (double)(st[it] - sv[it]);
           if (a.accel) {
-            const double d = DSp[J];
+            const double d = dv[it];
             dot += dn * d;
             dold += d * d;
           }
@@ -82,9 +112,10 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
         }
       }
       if (a.accel) {
-        for (int it = 0; it < nmax; it++) {
+#pragma unroll
+        for (int it = 0; it < SRC_TERMS; it++) {
           const int J = lane + 64 * it;
-          if (J < a.ld) DSp[J] = (J < ns_old) ? st[it] - Sp[J] : 0.f;
+          if (J < a.ld) DSp[J] = (J < ns_old) ? st[it] - sv[it] : 0.f;
         }
         if (lane == 0) a.nos[p] = ns_old;
       }
@@ -96,13 +127,15 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
       for (int off = 32; off > 0; off >>= 1) js = max(js, __shfl_xor(js, off));
       nsn = (js < 0) ? 0 : nterms_for_l(a.lofj[js], a.mm);
     }
-    for (int it = 0; it < nmax; it++) {
+#pragma unroll
+    for (int it = 0; it < SRC_TERMS; it++) {
       const int J = lane + 64 * it;
       if (J < a.ld) Sp[J] = (J < nsn && J < a.nlm) ? st[it] : 0.f;
     }
     if (lane == 0) a.ns[p] = nsn;
     if (a.first && a.accel) {
-      for (int it = 0; it < nmax; it++) {
+#pragma unroll
+      for (int it = 0; it < SRC_TERMS; it++) {
         const int J = lane + 64 * it;
         if (J < a.ld) DSp[J] = 0.f;
       }
