@@ -15,11 +15,11 @@ void launch_sh_to_do(hipStream_t st, const float* S, const float* Yc, const floa
                      float* dosrc, int mp, int ncp, int ld, int ldp, int npts, int nang);
 // Level-ordered variant: dosrc[c][q] = max(0, sum_J S[perm_o[q]][J] * Yc[c][J]) * ext[perm_o[q]]
 // for each column c of octant o, perm_o = oct_perm + o * mp (the octant's
-// boundary points, then its sweep levels).  tiles[t] = {octant, first column,
-// columns (<= 48), 0}.
+// positions: its levels, boundary points included).  tiles[t] = {octant,
+// first column, columns (<= 48), 0}.  S[p][J] must be zero for J >= ns[p].
 void launch_sh_to_do_perm(hipStream_t st, const float* S, const float* Yc, const float* ext,
-                          const int32_t* oct_perm, const int4* tiles, int ntiles, float* dosrc,
-                          int mp, int ld, int ldp);
+                          const int* ns, const int32_t* oct_perm, const int4* tiles, int ntiles,
+                          float* dosrc, int mp, int ld, int ldp, int npts);
 // R[p][J] = (J < nr[p]) ? sum_c dorad[c][p] * Wc[c][J] : 0
 // dorad [ldo][ldp] (rows beyond nang zero), Wc [ldo][ld] (rows beyond nang zero), R [mp][ld].
 void launch_do_to_sh(hipStream_t st, const float* dorad, const float* Wc, const int* nr, float* R,

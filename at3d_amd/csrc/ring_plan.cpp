@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <utility>
 
 namespace at3d {
 
@@ -20,6 +21,17 @@ bool build_ring_plan(const Plan& p, const bool used[8], size_t lds_budget, RingP
                      std::string& err) {
   const int npts = p.npts;
   r.nlev_max = 0;
+  // source class of each point: the highest Legendre order with a nonzero
+  // albedo-weighted phase term; COMPUTE_SOURCE (shdomsub1.f:967-1611) gives
+  // the point no source terms above it
+  std::vector<int32_t> cls(npts, 0);
+  if (p.chi.size() >= (size_t)npts * (p.ml + 1) && p.alb.size() >= (size_t)npts)
+    for (int i = 0; i < npts; i++)
+      for (int l = p.ml; l > 0; l--)
+        if (p.alb[i] * p.chi[(size_t)i * (p.ml + 1) + l] != 0.f) {
+          cls[i] = l;
+          break;
+        }
   for (int o = 0; o < 8; o++) {
     RingOctant& ro = r.oct[o];
     ro = RingOctant();
@@ -59,7 +71,7 @@ bool build_ring_plan(const Plan& p, const bool used[8], size_t lds_budget, RingP
     ro.nlev = nl;
     r.nlev_max = std::max(r.nlev_max, nl);
     // entries by level: the plan's order (level, rank) for points, then the
-    // boundary points in boundary order
+    // boundary points in boundary order (regrouped by source class below)
     std::vector<int32_t> cnt(nl + 2, 0);
     for (int i = 0; i < npts; i++) cnt[ro.level[i]]++;
     if (cnt[0] != 0) {
@@ -90,6 +102,38 @@ bool build_ring_plan(const Plan& p, const bool used[8], size_t lds_budget, RingP
         err = "ring layout: level lists do not cover the grid";
         return false;
       }
+    // a level's entries are independent of each other, so their order is
+    // free: entries of like source class are grouped (ascending on odd levels,
+    // descending on even ones, so neighbouring levels meet in one class), and
+    // a run of positions of the level-ordered SH->DO transform mostly needs
+    // only the few source terms of its class
+    std::vector<std::pair<int32_t, int32_t>> ent;
+    std::vector<int32_t> code2, cell2;
+    for (int L = 1; L <= nl; L++) {
+      const int a = ro.off[L - 1], b = ro.off[L];
+      ent.clear();
+      for (int q = a; q < b; q++) {
+        const int c = ro.code[q];
+        const int pt = c >= 0 ? c : bc[-1 - c];
+        ent.emplace_back((L & 1) ? cls[pt] : -cls[pt], q);
+      }
+      std::stable_sort(ent.begin(), ent.end(),
+                       [](const std::pair<int32_t, int32_t>& x, const std::pair<int32_t, int32_t>& y) {
+                         return x.first < y.first;
+                       });
+      code2.resize(b - a);
+      cell2.resize(b - a);
+      for (int k = 0; k < b - a; k++) {
+        code2[k] = ro.code[ent[k].second];
+        cell2[k] = ro.cell[ent[k].second];
+      }
+      for (int k = 0; k < b - a; k++) {
+        const int q = a + k, c = code2[k];
+        ro.code[q] = c;
+        ro.cell[q] = cell2[k];
+        ro.pos[c >= 0 ? c : bc[-1 - c]] = q;
+      }
+    }
   }
   // a level reads RING_WINDOW levels back and writes a sector whose points
   // span sector_points levels after its last one: the ring holds window

@@ -596,8 +596,8 @@ int state_solve(State& s, int maxiter, int verbose, std::string& err) {
     (void)hipEventRecord(s.ev[1], s.stream);
     s.dosrc_level_order = ring_active(s);
     if (s.dosrc_level_order)
-      launch_sh_to_do_perm(s.stream, s.S, s.Y, s.ext, s.oct_perm, s.sh_tiles, s.n_sh_tiles, s.dosrc,
-                           s.nptsp, s.ld, s.nptsp);
+      launch_sh_to_do_perm(s.stream, s.S, s.Y, s.ext, s.ns, s.oct_perm, s.sh_tiles, s.n_sh_tiles,
+                           s.dosrc, s.nptsp, s.ld, s.nptsp, p.npts);
     else
       launch_sh_to_do(s.stream, s.S, s.Y, s.ext, s.dosrc, s.nptsp, s.ldo, s.ld, s.nptsp, p.npts,
                       p.nang);

@@ -117,14 +117,19 @@ __global__ __launch_bounds__(256) void k_sh_to_do(const float* __restrict__ S,
 // SH->DO in the sweep's level order: one workgroup computes 64 consecutive
 // positions of one octant's order (S rows gathered through perm) for up to 48
 // columns of that octant; each wave owns 16 positions x 48 columns (three
-// 16x16 MFMA blocks).
+// 16x16 MFMA blocks).  S is zero beyond each point's source truncation ns
+// (COMPUTE_SOURCE keeps NS terms, shdomsub1.f:967-1611), so the K loop stops
+// at the largest ns of the tile's rows, rounded up to TK: the terms skipped
+// are exact zeros, and the sweep's order groups points of like ns.
 constexpr int PN = 48;
 __global__ __launch_bounds__(256) void k_sh_to_do_perm(const float* __restrict__ S,
                                                        const float* __restrict__ Yc,
                                                        const float* __restrict__ ext,
+                                                       const int* __restrict__ ns,
                                                        const int32_t* __restrict__ oct_perm,
                                                        const int4* __restrict__ tiles,
-                                                       float* __restrict__ out, int ld, int ldp) {
+                                                       float* __restrict__ out, int ld, int ldp,
+                                                       int npts) {
   __shared__ float sA[2][TK][TM + PAD];
   __shared__ float sB[2][TK][PN + PAD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -132,6 +137,14 @@ __global__ __launch_bounds__(256) void k_sh_to_do_perm(const float* __restrict__
   const int32_t* perm = oct_perm + (size_t)tile.x * ldp;
   const int c0 = tile.y, nc = tile.z;
   const int m0 = blockIdx.x * TM;
+  // every wave reduces the same 64 rows: the K extent is uniform in the block
+  int kn;
+  {
+    const int pr = perm[m0 + lane];
+    int v = pr < npts ? ns[pr] : 0;
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+    kn = __builtin_amdgcn_readfirstlane(min(ld, (v + TK - 1) / TK * TK));
+  }
   float4_ acc[3];
   for (int j = 0; j < 3; j++) acc[j] = (float4_){0.f, 0.f, 0.f, 0.f};
   const int lr = tid >> 2, lc = (tid & 3) * 4;
@@ -141,7 +154,7 @@ __global__ __launch_bounds__(256) void k_sh_to_do_perm(const float* __restrict__
   float4 a = *reinterpret_cast<const float4*>(arow);
   float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
   if (bload) b = *reinterpret_cast<const float4*>(brow);
-  for (int k0 = 0, buf = 0; k0 < ld; k0 += TK, buf ^= 1) {
+  for (int k0 = 0, buf = 0; k0 < kn; k0 += TK, buf ^= 1) {
     sA[buf][lc + 0][lr] = a.x;
     sA[buf][lc + 1][lr] = a.y;
     sA[buf][lc + 2][lr] = a.z;
@@ -153,7 +166,7 @@ __global__ __launch_bounds__(256) void k_sh_to_do_perm(const float* __restrict__
       sB[buf][lc + 3][lr] = b.w;
     }
     __syncthreads();
-    if (k0 + TK < ld) {
+    if (k0 + TK < kn) {
       a = *reinterpret_cast<const float4*>(arow + k0 + TK);
       if (bload) b = *reinterpret_cast<const float4*>(brow + k0 + TK);
     }
@@ -263,11 +276,11 @@ void launch_sh_dot(hipStream_t st, const float* S, const float* Yd, float* raw, 
 }
 
 void launch_sh_to_do_perm(hipStream_t st, const float* S, const float* Yc, const float* ext,
-                          const int32_t* oct_perm, const int4* tiles, int ntiles, float* dosrc,
-                          int mp, int ld, int ldp) {
+                          const int* ns, const int32_t* oct_perm, const int4* tiles, int ntiles,
+                          float* dosrc, int mp, int ld, int ldp, int npts) {
   if (ntiles <= 0) return;
   dim3 grid(mp / TM, ntiles);
-  k_sh_to_do_perm<<<grid, 256, 0, st>>>(S, Yc, ext, oct_perm, tiles, dosrc, ld, ldp);
+  k_sh_to_do_perm<<<grid, 256, 0, st>>>(S, Yc, ext, ns, oct_perm, tiles, dosrc, ld, ldp, npts);
 }
 
 void launch_do_to_sh(hipStream_t st, const float* dorad, const float* Wc, const int* nr, float* R,

@@ -196,10 +196,12 @@ def main():
                      "level_ordered": bool(rte.timing("sweep_ring")),
                      "lds_bytes": int(rte.timing("sweep_ring_lds")),
                      "multi_cell_points": int(rte.timing("sweep_general_points"))}
+    # dense-equivalent rates: both transforms skip terms beyond each point's
+    # SH truncation (exact zeros), so they execute fewer flops than 2 N M K
     for k in ("sh_to_do", "do_to_sh"):
         ms = ksum[k] / n_it
-        kern[k] = {"ms_per_launch": ms, "flops_per_launch": gemm_flops,
-                   "TFLOP/s": gemm_flops / ms / 1e9}
+        kern[k] = {"ms_per_launch": ms, "dense_flops_per_launch": gemm_flops,
+                   "dense_equiv_TFLOP/s": gemm_flops / ms / 1e9}
     ms_src = ksum["source"] / n_it
     src_bytes = npts * nlm * 4 * 6.0       # R read, S read/write, DS read/write, chi/ysun
     kern["source"] = {"ms_per_launch": ms_src, "bytes_per_launch": src_bytes,
@@ -207,7 +209,7 @@ def main():
     dominant = max(("sweep", "sh_to_do", "do_to_sh", "source"), key=lambda k: ksum[k])
     traffic = measured_traffic(dominant)
     if dominant in ("sh_to_do", "do_to_sh"):
-        ach = kern[dominant]["TFLOP/s"]
+        ach = kern[dominant]["dense_equiv_TFLOP/s"]
         roof = {"kernel": dominant, "bound": "mfma", "achieved": ach, "peak": FP32_MFMA_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": ach / FP32_MFMA_PEAK_TFLOPS, "traffic": traffic}
     else:

@@ -164,3 +164,34 @@ def test_ring_layout_stages_most_of_the_benchmark_grid():
     assert staged.mean() > 0.9
     # the ring fits in the 160 KiB of LDS of one workgroup
     assert slots * 8 + (nlev.max() + 2) * 9 + 4 <= 160 * 1024
+
+
+def test_level_order_groups_points_by_source_class():
+    """Within a level the entries are grouped by source class (the highest
+    Legendre order with a nonzero albedo-weighted phase term, which bounds the
+    point's source truncation), ascending on odd levels and descending on even
+    ones; on the benchmark grid a 64-position tile of the level-ordered SH->DO
+    transform then needs well under half of the dense K extent."""
+    w = S.workload("cloud64")
+    rte, lev, pos, staged, slots, spts, nlev = _ring_case(w)
+    chi, alb = rte.plan_array("chi"), rte.plan_array("alb")
+    ml = chi.shape[1] - 1
+    nz = (alb[:, None] * chi[:, 1:]) != 0
+    cls = np.where(nz.any(axis=1), ml - np.argmax(nz[:, ::-1], axis=1), 0)
+    nlm = rte._nlm
+    work = dense = 0
+    for o in range(8):
+        if nlev[o] == 0:
+            continue
+        order = np.argsort(pos[o])
+        assert np.array_equal(pos[o][order], np.arange(len(order)))
+        c, L = cls[order], lev[o][order]
+        for lv in np.unique(L):
+            cc = c[L == lv]
+            d = np.diff(cc)
+            assert np.all(d >= 0) if lv % 2 else np.all(d <= 0)
+        for t in range(0, len(c), 64):
+            k = min(nlm, (int(c[t:t + 64].max()) + 1) ** 2)
+            work += -(-k // 16) * 16
+            dense += nlm
+    assert work / dense < 0.45
