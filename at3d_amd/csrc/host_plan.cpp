@@ -1337,6 +1337,21 @@ int render_constants(Plan& p, std::string& err) {
 
 void ylm_direction(float mu, float phi, int ml, int mm, float* yr) { ylm_unpol(mu, phi, ml, mm, yr); }
 
+// COMPUTE_SOURCE (shdomsub1.f:967-1611) forms source term J as albedo x
+// chi_l(J) x (...), so a point whose albedo-weighted phase terms vanish above
+// order l has no source terms above l.
+void plan_source_class(const Plan& p, std::vector<int32_t>& cls) {
+  const int npts = p.npts;
+  cls.assign(npts, 0);
+  if (p.chi.size() < (size_t)npts * (p.ml + 1) || p.alb.size() < (size_t)npts) return;
+  for (int i = 0; i < npts; i++)
+    for (int l = p.ml; l > 0; l--)
+      if (p.alb[i] * p.chi[(size_t)i * (p.ml + 1) + l] != 0.f) {
+        cls[i] = l;
+        break;
+      }
+}
+
 void plan_mixed_legen(const Plan& p, int ipa, int ip, std::vector<float>& out) {
   mixed_legen(p, ipa, ip, out);
 }

@@ -80,6 +80,9 @@ void ylm_direction(float mu, float phi, int ml, int mm, float* yr);
 // Mixed phase function of species ipa at point ip (the reference's LEGENT,
 // 'N' interpolation), coefficients 0..nleg.
 void plan_mixed_legen(const Plan& p, int ipa, int ip, std::vector<float>& out);
+// Source class of each point: the highest Legendre order with a nonzero
+// albedo-weighted phase term; the point's source has no terms above it.
+void plan_source_class(const Plan& p, std::vector<int32_t>& cls);
 
 // Per RTE point, the property-grid points on its path to the sun and the
 // derivative of the slant optical path with respect to their (delta-M scaled)

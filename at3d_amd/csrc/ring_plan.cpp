@@ -21,17 +21,8 @@ bool build_ring_plan(const Plan& p, const bool used[8], size_t lds_budget, RingP
                      std::string& err) {
   const int npts = p.npts;
   r.nlev_max = 0;
-  // source class of each point: the highest Legendre order with a nonzero
-  // albedo-weighted phase term; COMPUTE_SOURCE (shdomsub1.f:967-1611) gives
-  // the point no source terms above it
-  std::vector<int32_t> cls(npts, 0);
-  if (p.chi.size() >= (size_t)npts * (p.ml + 1) && p.alb.size() >= (size_t)npts)
-    for (int i = 0; i < npts; i++)
-      for (int l = p.ml; l > 0; l--)
-        if (p.alb[i] * p.chi[(size_t)i * (p.ml + 1) + l] != 0.f) {
-          cls[i] = l;
-          break;
-        }
+  std::vector<int32_t> cls;
+  plan_source_class(p, cls);
   for (int o = 0; o < 8; o++) {
     RingOctant& ro = r.oct[o];
     ro = RingOctant();
