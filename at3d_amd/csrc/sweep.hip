@@ -440,14 +440,17 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
     put(jl, k0, val, s1);
     return true;
   };
-  // Records of a level, loaded unconditionally (indices clamped into the
-  // column's records) so the loads of every lane are always in flight; with
-  // them the sectors that complete on the level before (written after level L).
+  // Records of a level, loaded unconditionally so the loads of every lane
+  // are always in flight (a load under a branch would leave the wait counts
+  // unknown where the paths join); lanes past the level's end load its last
+  // record, one cache line for a whole wave.  With them the sectors that
+  // complete on the level before (written after level L).
   auto load = [&](int L, RingSet& S) {
     const int beg = __builtin_amdgcn_readfirstlane(off[min(L - 1, nlev)]);
+    const int last = max(__builtin_amdgcn_readfirstlane(off[min(L, nlev)]) - 1, 0);
 #pragma unroll
     for (int k = 0; k < RING_K; k++) {
-      const int jl = min(beg + tid + k * RING_THREADS, nsw - 1);
+      const int jl = min(beg + tid + k * RING_THREADS, last);
       S.w0[k] = rec[2 * jl];
       S.w1[k] = rec[2 * jl + 1];
       S.s[k] = se[jl];
