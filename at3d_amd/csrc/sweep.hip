@@ -440,29 +440,34 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
     put(jl, k0, val, s1);
     return true;
   };
-  // Records of a level, loaded unconditionally so the loads of every lane
-  // are always in flight (a load under a branch would leave the wait counts
-  // unknown where the paths join); lanes past the level's end load its last
-  // record, one cache line for a whole wave.  With them the sectors that
-  // complete on the level before (written after level L).
+  // Records of a level: register slot k of a wave is loaded only when the
+  // wave's first entry of that slot lies in the level (a wave-uniform
+  // branch); lanes past the level's end load its last record, one cache line
+  // for the wave.  With them the sectors that complete on the level before
+  // (written after level L), loaded by the waves that have one.
+  const int wfirst = tid & ~63;
   auto load = [&](int L, RingSet& S) {
     const int beg = __builtin_amdgcn_readfirstlane(off[min(L - 1, nlev)]);
-    const int last = max(__builtin_amdgcn_readfirstlane(off[min(L, nlev)]) - 1, 0);
+    const int end = __builtin_amdgcn_readfirstlane(off[min(L, nlev)]);
+    const int last = max(end - 1, 0);
 #pragma unroll
     for (int k = 0; k < RING_K; k++) {
-      const int jl = min(beg + tid + k * RING_THREADS, last);
-      S.w0[k] = rec[2 * jl];
-      S.w1[k] = rec[2 * jl + 1];
-      S.s[k] = se[jl];
+      if (beg + wfirst + k * RING_THREADS < end) {
+        const int jl = min(beg + tid + k * RING_THREADS, last);
+        S.w0[k] = rec[2 * jl];
+        S.w1[k] = rec[2 * jl + 1];
+        S.s[k] = se[jl];
+      }
     }
     S.sec_ok = false;
     if (staging && L >= 2 && L <= nlev + 1) {
       const int s0 = soffl[L - 1], s1 = soffl[L];
       S.sec_ok = s0 + tid < s1;
-      // loaded unconditionally (clamped), used only if sec_ok: no wait here
-      const int sct = max(min(s0 + tid, s1 - 1), 0);
-      S.sec = sslot[sct];
-      S.sec_p0 = sp0[sct];
+      if (__builtin_amdgcn_readfirstlane(s0 + wfirst) < s1) {
+        const int sct = min(s0 + tid, s1 - 1);  // clamped, used only if sec_ok
+        S.sec = sslot[sct];
+        S.sec_p0 = sp0[sct];
+      }
     }
   };
   auto write_sector = [&](uint4 sl, int p0) {
@@ -486,12 +491,21 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
     const int s1 = soffl[L + 1];
     for (int sct = soffl[L] + first + tid; sct < s1; sct += RING_THREADS) write_sector(sslot[sct], sp0[sct]);
   };
-  // Level L from `cur` while level L+1 streams into `fut`.  All of the
-  // level's LDS reads (face corners: slots of levels L-3 .. L-1; the sector
-  // completed on level L-1: levels L-4 .. L-1) come before any of its LDS
-  // writes (slots of level L), so they are issued together and waited for once.
+  // Level L from `cur` while level L+1 streams into `fut`.  First every
+  // register of `cur` is taken as read: one wait for all of the loads
+  // outstanding, which are this level's records (issued a level ago) and the
+  // stores of the level before; the next level's loads, issued after it, are
+  // never waited for here, and may be conditional without widening any wait.
+  // All of the level's LDS reads (face corners: slots of levels L-3 .. L-1;
+  // the sector completed on level L-1: levels L-4 .. L-1) come before any of
+  // its LDS writes (slots of level L); they are issued before the next
+  // level's loads and waited for once, after them.
   auto step = [&](int L, RingSet& cur, RingSet& fut) {
-    load(L + 1, fut);  // past the last level this re-reads the last record
+#pragma unroll
+    for (int k = 0; k < RING_K; k++)
+      asm volatile("" ::"v"(cur.w0[k].x), "v"(cur.w0[k].y), "v"(cur.w0[k].z), "v"(cur.w0[k].w),
+                   "v"(cur.w1[k].x), "v"(cur.w1[k].y), "v"(cur.w1[k].z), "v"(cur.w1[k].w), "v"(cur.s[k]));
+    asm volatile("" ::"v"(cur.sec.x), "v"(cur.sec.y), "v"(cur.sec.z), "v"(cur.sec.w), "v"(cur.sec_p0));
     const int beg = __builtin_amdgcn_readfirstlane(off[L - 1]);
     const int end = __builtin_amdgcn_readfirstlane(off[L]);
     const int k0 = beg % C - beg;  // slot of position jl: k0 + jl, wrapped once
@@ -522,6 +536,7 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
         sw.w = ring[sl.w >> 16].x;
       }
     }
+    load(L + 1, fut);  // past the last level this loads nothing
     uint32_t gen = 0;  // this thread's multi-cell traces among its first RING_K entries
 #pragma unroll
     for (int k = 0; k < RING_K; k++) {
