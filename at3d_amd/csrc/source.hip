@@ -66,6 +66,7 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
     const float* chip = a.chi + (size_t)p * (a.ml + 1);
     const bool old = !a.first;
     const bool useds = old && a.accel;
+    const int nsd = a.accel ? min(ns_old, nos_old) : ns_old;
     float rv[SRC_TERMS], sv[SRC_TERMS], dv[SRC_TERMS], cv[SRC_TERMS];
 #pragma unroll
     for (int it = 0; it < SRC_TERMS; it++) {
@@ -75,10 +76,10 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
         cv[it] = chip[lv[it]];
         if (J < nrp) rv[it] = Rp[J];
       }
-      if (J < a.ld) {
-        if (old) sv[it] = Sp[J];
-        if (useds) dv[it] = DSp[J];
-      }
+      // S is zero beyond NS and DS beyond NOS (both kept so below): only
+      // the terms that can be nonzero are read
+      if (old && J < ns_old) sv[it] = Sp[J];
+      if (useds && J < nsd) dv[it] = DSp[J];
     }
     int js = -1;
     float st[SRC_TERMS];
@@ -95,7 +96,6 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
       st[it] = v;
     }
     if (old) {
-      const int nsd = a.accel ? min(ns_old, nos_old) : ns_old;
 #pragma unroll
       for (int it = 0; it < SRC_TERMS; it++) {
         const int J = lane + 64 * it;
@@ -112,10 +112,13 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
         }
       }
       if (a.accel) {
+        // DS was zero from the previous NOS on: terms up to the larger of the
+        // two counts are written
+        const int dw = max(ns_old, nos_old);
 #pragma unroll
         for (int it = 0; it < SRC_TERMS; it++) {
           const int J = lane + 64 * it;
-          if (J < a.ld) DSp[J] = (J < ns_old) ? st[it] - sv[it] : 0.f;
+          if (J < dw) DSp[J] = (J < ns_old) ? st[it] - sv[it] : 0.f;
         }
         if (lane == 0) a.nos[p] = ns_old;
       }
@@ -127,10 +130,14 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
       for (int off = 32; off > 0; off >>= 1) js = max(js, __shfl_xor(js, off));
       nsn = (js < 0) ? 0 : nterms_for_l(a.lofj[js], a.mm);
     }
+    // the first pass writes whole rows (the rows may hold another solve's
+    // terms); later passes only the terms below the larger of the old and new
+    // NS (beyond both the row is zero already)
+    const int sw = old ? max(nsn, ns_old) : a.ld;
 #pragma unroll
     for (int it = 0; it < SRC_TERMS; it++) {
       const int J = lane + 64 * it;
-      if (J < a.ld) Sp[J] = (J < nsn && J < a.nlm) ? st[it] : 0.f;
+      if (J < sw) Sp[J] = (J < nsn && J < a.nlm) ? st[it] : 0.f;
     }
     if (lane == 0) a.ns[p] = nsn;
     if (a.first && a.accel) {
@@ -177,15 +184,18 @@ __global__ __launch_bounds__(1024) void k_reduce_dots(const double* __restrict__
   if (threadIdx.x < 4) out[threadIdx.x] = sh[threadIdx.x];
 }
 
-// ACCELERATE_SOLUTION (shdomsub1.f:1807-1831).
-__global__ void k_accelerate(float* __restrict__ S, const float* __restrict__ DS,
-                             const int* __restrict__ ns, const int* __restrict__ nos, int ld,
-                             int npts, float a) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t p = idx / ld;
-  const int J = (int)(idx % ld);
-  if (p >= (size_t)npts) return;
-  if (J < min(ns[p], nos[p])) S[idx] = S[idx] + a * DS[idx];
+// ACCELERATE_SOLUTION (shdomsub1.f:1807-1831): one wavefront per point over
+// its first min(NS, NOS) terms.
+__global__ __launch_bounds__(256) void k_accelerate(float* __restrict__ S, const float* __restrict__ DS,
+                                                    const int* __restrict__ ns, const int* __restrict__ nos,
+                                                    int ld, int npts, float a) {
+  const int lane = threadIdx.x & 63;
+  const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (p >= npts) return;
+  const int n = min(ns[p], nos[p]);
+  float* Sp = S + (size_t)p * ld;
+  const float* Dp = DS + (size_t)p * ld;
+  for (int J = lane; J < n; J += 64) Sp[J] = Sp[J] + a * Dp[J];
 }
 
 void launch_radiance_truncation(hipStream_t st, const float* R, int ld, const float* tc,
@@ -205,8 +215,7 @@ void launch_reduce_dots(hipStream_t st, const double* partial, int nblocks, doub
 
 void launch_accelerate(hipStream_t st, float* S, const float* DS, const int* ns, const int* nos,
                        int ld, int npts, float a) {
-  const size_t n = (size_t)npts * ld;
-  k_accelerate<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(S, DS, ns, nos, ld, npts, a);
+  k_accelerate<<<(npts + 3) / 4, 256, 0, st>>>(S, DS, ns, nos, ld, npts, a);
 }
 
 }  // namespace at3d
