@@ -70,7 +70,9 @@ struct SweepArgs {
   int sec_base[8];
   const int32_t* top_index;  // [npts] index in bctop or -1
   const int32_t* bot_index;  // [npts] index in bcbot or -1
-  const uint4* rec;          // [ncols][npts][2] step records (SweepRec)
+  const uint4* rec;          // step records (SweepRec) in two planes: first words
+                             // [ncols][npts], then second words [ncols][npts]
+  size_t rec_plane;          // ncols * npts: offset of the second plane
   const uint8_t* gflag;      // [ncols][gld] level L holds a multi-cell trace
   int* col_general;          // [ncols] nonzero: the column has multi-cell traces
   int gld;
@@ -112,7 +114,7 @@ void launch_sweep_columns(hipStream_t st, const SweepArgs& a, int col0, int ncol
 // Where the surface radiances also go: the boundary entries of the step
 // records of the upwelling columns col0 .. col1-1 (rec null: nowhere else).
 struct BottomRecs {
-  uint4* rec;
+  uint4* rec;  // first-word plane of the step records
   int npts, col0, col1;
   const int8_t* col_oct;
   const int32_t* oct_pos;
@@ -147,6 +149,9 @@ void launch_radiance_truncation(hipStream_t st, const float* R, int ld, const fl
                                 const int* ns, const int* lofj, int* nr, int npts, int ml, int mm,
                                 int nlm, float radmin, int fixsh, int highorder);
 void launch_compute_source(hipStream_t st, const SourceArgs& a, int nblocks);
+// R[p][J] = R0[p][J] for J < 4, zero for 4 <= J < max(4, nr[p]) (R is zero
+// beyond NR already); then nr[p] = 4.
+void launch_init_radiance(hipStream_t st, float* R, const float* R0, int* nr, int ld, int npts);
 void launch_reduce_dots(hipStream_t st, const double* partial, int nblocks, double* out);
 void launch_accelerate(hipStream_t st, float* S, const float* DS, const int* ns, const int* nos,
                        int ld, int npts, float a);

@@ -128,6 +128,7 @@ static SweepArgs sweep_args(const State& s) {
   a.top_index = s.top_index;
   a.bot_index = s.bot_index;
   a.rec = s.rec;
+  a.rec_plane = (size_t)s.plan->nang * s.plan->npts;
   a.gflag = s.gflag;
   a.col_general = s.col_general;
   a.gld = s.gld;
@@ -383,10 +384,12 @@ int state_create(const Plan& p, State& s, std::string& err) {
   std::vector<int32_t> bot_index(p.npts, -1);
   for (size_t b = 0; b < p.bcbot.size(); b++) bot_index[p.bcbot[b]] = (int32_t)b;
   TRY(upload(s, s.bot_index, bot_index.data(), bot_index.size(), err));
-  std::vector<float> r0(nsh, 0.f);
-  for (int i = 0; i < p.npts; i++)
-    for (int j = 0; j < 4 && j < s.ld; j++) r0[(size_t)i * s.ld + j] = p.radiance0[(size_t)i * 4 + j];
-  TRY(upload(s, s.R0, r0.data(), r0.size(), err));
+  TRY(upload(s, s.R0, p.radiance0.data(), p.radiance0.size(), err));
+  if (hipHostMalloc((void**)&s.host_dots, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    s.host_dots = nullptr;
+    err = "hipHostMalloc of the iteration readback buffer failed";
+    return AT3D_ERR_DEVICE;
+  }
   TRY(setup_ring(p, s, err));
   for (int e = 0; e < State::NEV; e++) HIPCHK(hipEventCreate(&s.ev[e]));
   for (int e = 0; e < 4; e++) HIPCHK(hipEventCreate(&s.ev_sweep[e]));
@@ -411,6 +414,8 @@ void state_free(State& s) {
     if (s.ev[e]) (void)hipEventDestroy(s.ev[e]);
   for (int e = 0; e < 4; e++)
     if (s.ev_sweep[e]) (void)hipEventDestroy(s.ev_sweep[e]);
+  if (s.host_dots) (void)hipHostFree(s.host_dots);
+  s.host_dots = nullptr;
   if (s.stream) (void)hipStreamDestroy(s.stream);
   s.stream = nullptr;
 }
@@ -442,14 +447,13 @@ static SourceArgs source_args(State& s, int first, int fixsh) {
   return a;
 }
 
+// INIT_SOLUTION's radiance and source (shdomsub1.f:113-442): the two-stream
+// radiance (four terms, NR = 4) and the first COMPUTE_SOURCE.  NS and NOS keep
+// the extents of the terms S and DS hold from an earlier solve, which the
+// first source pass clears.
 int state_init_solution(State& s, std::string& err) {
   const Plan& p = *s.plan;
-  const size_t nsh = (size_t)s.nptsp * s.ld;
-  HIPCHK(hipMemcpyAsync(s.R, s.R0, nsh * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
-  std::vector<int32_t> four(p.npts, 4), zero(p.npts, 0);
-  HIPCHK(hipMemcpyAsync(s.nr, four.data(), p.npts * sizeof(int), hipMemcpyHostToDevice, s.stream));
-  HIPCHK(hipMemcpyAsync(s.ns, zero.data(), p.npts * sizeof(int), hipMemcpyHostToDevice, s.stream));
-  HIPCHK(hipMemcpyAsync(s.nos, zero.data(), p.npts * sizeof(int), hipMemcpyHostToDevice, s.stream));
+  launch_init_radiance(s.stream, s.R, s.R0, s.nr, s.ld, p.npts);
   launch_compute_source(s.stream, source_args(s, 1, 0), s.nsrc_blocks);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s.stream));
@@ -470,9 +474,6 @@ int state_init_solution_from(State& s, State& src, std::string& err) {
   HIPCHK(hipStreamSynchronize(src.stream));
   HIPCHK(hipMemcpyAsync(s.R, src.R, nsh * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
   HIPCHK(hipMemcpyAsync(s.nr, src.nr, p.npts * sizeof(int), hipMemcpyDeviceToDevice, s.stream));
-  std::vector<int32_t> zero(p.npts, 0);
-  HIPCHK(hipMemcpyAsync(s.ns, zero.data(), p.npts * sizeof(int), hipMemcpyHostToDevice, s.stream));
-  HIPCHK(hipMemcpyAsync(s.nos, zero.data(), p.npts * sizeof(int), hipMemcpyHostToDevice, s.stream));
   launch_compute_source(s.stream, source_args(s, 1, 0), s.nsrc_blocks);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s.stream));
@@ -611,11 +612,13 @@ int state_solve(State& s, int maxiter, int verbose, std::string& err) {
     launch_reduce_dots(s.stream, s.partial, s.nsrc_blocks, s.dots);
     (void)hipEventRecord(s.ev[5], s.stream);
     HIPCHK(hipGetLastError());
-    double dots[4];
-    int ef = 0;
-    HIPCHK(hipMemcpyAsync(dots, s.dots, sizeof(dots), hipMemcpyDeviceToHost, s.stream));
-    HIPCHK(hipMemcpyAsync(&ef, s.errflag, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    // the four dot products and the error flag into pinned host memory
+    double* dots = s.host_dots;
+    int* efp = reinterpret_cast<int*>(s.host_dots + 4);
+    HIPCHK(hipMemcpyAsync(dots, s.dots, 4 * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    HIPCHK(hipMemcpyAsync(efp, s.errflag, sizeof(int), hipMemcpyDeviceToHost, s.stream));
     HIPCHK(hipStreamSynchronize(s.stream));
+    const int ef = *efp;
     if (ef) {
       char buf[160];
       snprintf(buf, sizeof buf, "BACK_INT_GRID: invalid characteristic (flags 0x%x) at iteration %d",

@@ -33,7 +33,9 @@ struct State {
   // optics
   float *ext = nullptr, *alb = nullptr, *chi = nullptr, *tc = nullptr, *dirflux = nullptr, *f0 = nullptr;
   // SH fields [nptsp][ld]
-  float *S = nullptr, *R = nullptr, *DS = nullptr, *R0 = nullptr;
+  float *S = nullptr, *R = nullptr, *DS = nullptr;
+  float* R0 = nullptr;   // [npts][4] two-stream initial radiance terms
+  double* host_dots = nullptr;   // pinned: four dot products, then the error flag
   int *ns = nullptr, *nr = nullptr, *nos = nullptr;
   // DO fields [ldo][nptsp] (ordinate-major)
   float *dosrc = nullptr, *dorad = nullptr;

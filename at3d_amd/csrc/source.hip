@@ -130,10 +130,10 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
       for (int off = 32; off > 0; off >>= 1) js = max(js, __shfl_xor(js, off));
       nsn = (js < 0) ? 0 : nterms_for_l(a.lofj[js], a.mm);
     }
-    // the first pass writes whole rows (the rows may hold another solve's
-    // terms); later passes only the terms below the larger of the old and new
-    // NS (beyond both the row is zero already)
-    const int sw = old ? max(nsn, ns_old) : a.ld;
+    // terms below the larger of the old and new NS (beyond both the row is
+    // zero already; in the first pass NS is the extent of an earlier solve's
+    // terms)
+    const int sw = max(nsn, ns_old);
 #pragma unroll
     for (int it = 0; it < SRC_TERMS; it++) {
       const int J = lane + 64 * it;
@@ -141,10 +141,11 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
     }
     if (lane == 0) a.ns[p] = nsn;
     if (a.first && a.accel) {
+      // DS cleared (it is zero from NOS on)
 #pragma unroll
       for (int it = 0; it < SRC_TERMS; it++) {
         const int J = lane + 64 * it;
-        if (J < a.ld) DSp[J] = 0.f;
+        if (J < nos_old) DSp[J] = 0.f;
       }
       if (lane == 0) a.nos[p] = nsn;
     }
@@ -166,6 +167,18 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
     const int c = threadIdx.x;
     a.partial[(size_t)blockIdx.x * 4 + c] = red[0][c] + red[1][c] + red[2][c] + red[3][c];
   }
+}
+
+// INIT_RADIANCE's terms into the SH radiance rows: one wavefront per point.
+__global__ __launch_bounds__(256) void k_init_radiance(float* __restrict__ R, const float* __restrict__ R0,
+                                                       int* __restrict__ nr, int ld, int npts) {
+  const int lane = threadIdx.x & 63;
+  const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (p >= npts) return;
+  const int n = min(ld, max(4, nr[p]));
+  float* Rp = R + (size_t)p * ld;
+  for (int J = lane; J < n; J += 64) Rp[J] = J < 4 ? R0[(size_t)p * 4 + J] : 0.f;
+  if (lane == 0) nr[p] = 4;
 }
 
 // Fixed-order reduction of the block partials.
@@ -207,6 +220,10 @@ void launch_radiance_truncation(hipStream_t st, const float* R, int ld, const fl
 
 void launch_compute_source(hipStream_t st, const SourceArgs& a, int nblocks) {
   k_compute_source<<<nblocks, 256, 0, st>>>(a);
+}
+
+void launch_init_radiance(hipStream_t st, float* R, const float* R0, int* nr, int ld, int npts) {
+  k_init_radiance<<<(npts + 3) / 4, 256, 0, st>>>(R, R0, nr, ld, npts);
 }
 
 void launch_reduce_dots(hipStream_t st, const double* partial, int nblocks, double* out) {

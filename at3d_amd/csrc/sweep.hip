@@ -340,9 +340,9 @@ __device__ inline void sweep_record(const SweepArgs& a, int o, int col, int jl) 
     }
   }
   const uint4* w = reinterpret_cast<const uint4*>(&r);
-  uint4* dst = const_cast<uint4*>(a.rec) + ((size_t)col * a.npts + jl) * 2;
+  uint4* dst = const_cast<uint4*>(a.rec) + (size_t)col * a.npts + jl;
   dst[0] = w[0];
-  dst[1] = w[1];
+  dst[a.rec_plane] = w[1];
 }
 
 // Barrier that orders LDS traffic only: outstanding global loads (the next
@@ -393,7 +393,10 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
   const int32_t* goff = a.ring_off + (size_t)o * a.gld;
   const float* se = a.srcext + (size_t)col * a.ldp;  // ring order
   float* rad = a.rad + (size_t)col * a.ldp;          // point order
-  const uint4* rec = a.rec + (size_t)col * a.npts * 2;
+  // the two words of the records in separate planes: the two loads of a
+  // record slot touch disjoint cache lines
+  const uint4* rec0 = a.rec + (size_t)col * a.npts;
+  const uint4* rec1 = rec0 + a.rec_plane;
   const uint8_t* gf = a.gflag + (size_t)col * a.gld;
   const int32_t* pos = a.oct_pos + (size_t)o * a.npts;
   const int32_t* soff = a.sec_off + (size_t)o * (a.gld + 1);
@@ -454,8 +457,8 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
     for (int k = 0; k < RING_K; k++) {
       if (beg + wfirst + k * RING_THREADS < end) {
         const int jl = min(beg + tid + k * RING_THREADS, last);
-        S.w0[k] = rec[2 * jl];
-        S.w1[k] = rec[2 * jl + 1];
+        S.w0[k] = rec0[jl];
+        S.w1[k] = rec1[jl];
         S.s[k] = se[jl];
       }
     }
@@ -569,13 +572,13 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
     }
     if (FULL)
       for (int jl = beg + tid + RING_K * RING_THREADS; jl < end; jl += RING_THREADS)
-        (void)finish(rec[2 * jl], rec[2 * jl + 1], se[jl], jl, k0);
+        (void)finish(rec0[jl], rec1[jl], se[jl], jl, k0);
     if (FULL && __builtin_amdgcn_readfirstlane(full[L])) {
       // multi-cell traces read radiances of earlier levels from the row,
       // complete and visible after the full barrier that ended level L-1
       int k = 0;
       for (int jl = beg + tid; jl < end; jl += RING_THREADS, k++) {
-        const uint32_t head = rec[2 * jl].x;
+        const uint32_t head = rec0[jl].x;
         const bool gn = k < RING_K ? ((gen >> k) & 1) != 0 : (head >> 30) == 1;
         if (!gn) continue;
         const int p = head & 0x1fffffff;
@@ -642,7 +645,7 @@ __global__ __launch_bounds__(256) void k_bottom_bc(const float* __restrict__ rad
     for (int c = br.col0 + lane; c < br.col1; c += 64) {
       const int o = br.col_oct[c];
       const size_t e = (size_t)c * br.npts + br.oct_pos[(size_t)o * br.npts + p];
-      reinterpret_cast<uint32_t*>(br.rec + e * 2)[2] = __float_as_uint(v);
+      reinterpret_cast<uint32_t*>(br.rec + e)[2] = __float_as_uint(v);
     }
   }
 }

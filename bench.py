@@ -217,6 +217,16 @@ def main():
         roof = {"kernel": dominant, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic}
 
+    # whole-solve algorithmic traffic (minimum bytes of a dense FP32 layout per
+    # iteration: source r/w with acceleration 5 NLM, SH->DO NLM + NANG, sweep
+    # 3 NANG, DO->SH NANG + NLM; plus the setup's optics and the first
+    # source), over the plain solve time
+    b_iter = npts * 4.0 * (7 * nlm + 5 * nang)
+    b_solve = iters * b_iter + npts * 4.0 * (2 * nlm + 16)
+    solve_gbs = b_solve / (ms_per_step * 1e-3) / 1e9
+    solve_roof = {"bytes_per_solve": b_solve, "bytes_per_iteration": b_iter, "GB/s": solve_gbs,
+                  "frac": solve_gbs / HBM_PEAK_GBS}
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.workload, iters, args.cpu_iters)
@@ -242,6 +252,7 @@ def main():
         "kernel_ms_per_solve": {k: ksum[k] / args.steps for k in keys},
         "kernels": kern,
         "roofline": roof,
+        "solve_roofline": solve_roof,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
     }
