@@ -91,6 +91,7 @@ int64_t at3d_plan_size(const at3d_plan* h, const char* key) {
   if (k == "ntop") return (int64_t)p.bctop.size();
   if (k == "nbot") return (int64_t)p.bcbot.size();
   if (k == "nlevels_max") return p.nlevels_max;
+  if (k == "footprint_bytes") return (int64_t)at3d::plan_footprint_bytes(p);
   if (k == "npart") return p.npart;
   if (k == "numphase") return p.numphase;
   if (k == "nleg") return p.nleg;
@@ -457,6 +458,7 @@ double at3d_state_timing(const at3d_state* st, const char* key) {
   if (k == "total") return s.tsum[0] + s.tsum[1] + s.tsum[2] + s.tsum[3] + s.tsum[4] + s.tsum[5];
   if (k == "sweep_launches") return s.nsweep_launch;
   if (k == "sweep_kernel") return s.tsum[6];
+  if (k == "device_bytes") return (double)s.device_bytes;
   if (k == "sweep_ring") return (s.ring_ok && s.use_ring && !s.force_general) ? 1.0 : 0.0;
   if (k == "sweep_ring_lds") return (double)s.ring_lds;
   if (k == "sweep_ring_slots") return s.slots;

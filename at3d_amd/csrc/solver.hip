@@ -59,6 +59,7 @@ template <typename T>
 static int upload(State& s, T*& dst, const T* src, size_t n, std::string& err) {
   HIPCHK(hipMalloc((void**)&dst, std::max<size_t>(n, 1) * sizeof(T)));
   s.allocs.push_back(dst);
+  s.device_bytes += std::max<size_t>(n, 1) * sizeof(T);
   if (n) {
     HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s.stream));
     HIPCHK(hipStreamSynchronize(s.stream));
@@ -70,6 +71,7 @@ template <typename T>
 static int alloc0(State& s, T*& dst, size_t n, std::string& err) {
   HIPCHK(hipMalloc((void**)&dst, std::max<size_t>(n, 1) * sizeof(T)));
   s.allocs.push_back(dst);
+  s.device_bytes += std::max<size_t>(n, 1) * sizeof(T);
   HIPCHK(hipMemsetAsync(dst, 0, std::max<size_t>(n, 1) * sizeof(T), s.stream));
   HIPCHK(hipStreamSynchronize(s.stream));
   return AT3D_OK;
@@ -213,6 +215,7 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
     return AT3D_OK;
   }
   s.allocs.push_back(s.rec);
+  s.device_bytes += nrec * sizeof(uint4);
   std::vector<int32_t> top_index(p.npts, -1);
   for (size_t b = 0; b < p.bctop.size(); b++) top_index[p.bctop[b]] = (int32_t)b;
   TRY(upload(s, s.top_index, top_index.data(), top_index.size(), err));
@@ -247,6 +250,28 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
 std::mutex& device_mutex() {
   static std::mutex m;
   return m;
+}
+
+// Device bytes a state of this plan allocates (its arrays and the sweep
+// records; not the per-call render and gradient buffers), for admission by
+// memory footprint before anything is allocated.
+size_t plan_footprint_bytes(const Plan& p) {
+  const size_t npts = p.npts, ncells = p.ncells, nang = p.nang;
+  const size_t ld = round_up(p.nlm, 64), nptsp = round_up(p.npts, 64), ldo = round_up(p.nang, 64);
+  const size_t ml1 = p.ml + 1;
+  size_t b = 0;
+  b += 2 * ldo * ld * 4;                 // Y, W
+  b += 3 * nptsp * ld * 4;               // S, R, DS
+  b += 2 * nptsp * ldo * 4;              // DO source and radiance
+  b += npts * 16;                        // two-stream radiance terms
+  b += 3 * npts * ml1 * 4;               // chi, tc, tms
+  b += npts * 4 * 12;                    // per-point scalars, counts, fluxes, partials
+  b += npts * 12 + ncells * 57;          // grid positions, cell topology and flags
+  b += 8 * npts * 4 * 4 + npts;          // sweep order: rank, level, level lists, preset
+  b += 2 * nang * npts * 16;             // sweep records (two 16-byte words)
+  b += 8 * nptsp * 4 + 8 * npts * (4 * 4 + 1) + 8 * (npts / 4 + 8) * 20;  // ring layout, sectors
+  b += p.ss_off.size() * 4 + p.ss_iph.size() * 4 + p.ss_coef.size() * 4;
+  return b;
 }
 
 int state_create(const Plan& p, State& s, std::string& err) {

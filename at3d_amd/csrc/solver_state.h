@@ -91,6 +91,7 @@ struct State {
   hipEvent_t ev_sweep[4] = {};  // around the two ring sweep launches of an iteration
   double tsum[8] = {};  // ms per kernel class over the last solve
   std::vector<void*> allocs;
+  size_t device_bytes = 0;   // bytes of the allocations in `allocs`
   // rendering
   float *tms = nullptr, *ss_coef = nullptr, *phasetab = nullptr, *bcrad_render = nullptr;
   int32_t *ss_off = nullptr, *ss_iph = nullptr, *bot_index = nullptr;
@@ -133,6 +134,7 @@ int state_render(State& s, int64_t nrays, const float* x, const float* y, const 
 std::mutex& device_mutex();
 
 int state_create(const Plan& p, State& s, std::string& err);
+size_t plan_footprint_bytes(const Plan& p);
 void state_free(State& s);
 int state_init_solution(State& s, std::string& err);
 int state_init_solution_from(State& s, State& src, std::string& err);

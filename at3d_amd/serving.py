@@ -34,6 +34,8 @@ class Request:
     warm_from: object = None       # key of an earlier request whose solution seeds this one
     render: object = None          # optional sensor dict rendered after the solve
     arrival_s: float = 0.0         # arrival time relative to the start of run()
+    group: object = None           # requests of one group (e.g. the wavelengths of one
+                                   # observation) complete together: latency per group
 
 
 @dataclass
@@ -48,8 +50,17 @@ class Result:
 
 
 class SolveServer:
-    def __init__(self, nworkers=4):
+    """``nworkers`` host threads, each with its own device states and HIP
+    streams; a request is admitted when a worker is free and its device
+    footprint (``RTE.footprint_bytes``) fits in ``memory_budget`` bytes beside
+    the solves in flight and the solutions kept for warm restarts (a request
+    is admitted regardless when nothing else is in flight, so one larger than
+    the budget still runs, alone).  ``memory_budget`` None: 80 % of the
+    device memory free at the start of ``run``."""
+
+    def __init__(self, nworkers=4, memory_budget=None):
         self.nworkers = int(nworkers)
+        self.memory_budget = memory_budget
 
     def run(self, requests):
         """Serve ``requests``; returns (results in request order, stats)."""
@@ -65,8 +76,28 @@ class SolveServer:
         q: queue.Queue = queue.Queue()
         results: dict = {}
         kept: dict = {}
+        held: dict = {}      # key -> bytes of device memory it holds (running or kept)
+        running = [0]
+        peak = [0]
+        budget = self.memory_budget
+        if budget is None:
+            budget = float("inf")
+            try:
+                import torch
+                if torch.cuda.is_available():
+                    budget = int(0.8 * torch.cuda.mem_get_info()[0])
+            except Exception:
+                pass
         cond = threading.Condition()
         t0 = time.perf_counter()
+
+        def admit(key, nbytes):
+            with cond:
+                while running[0] > 0 and sum(held.values()) + nbytes > budget:
+                    cond.wait()
+                held[key] = nbytes
+                running[0] += 1
+                peak[0] = max(peak[0], sum(held.values()))
 
         def worker():
             while True:
@@ -75,20 +106,30 @@ class SolveServer:
                     return
                 ts = time.perf_counter()
                 rte = None
+                admitted = False
                 try:
-                    rte = req.make_rte()
+                    rte = req.make_rte()           # host setup: no device memory yet
+                    src = None
                     if req.warm_from is not None:
+                        # wait for the source before taking device memory, so a
+                        # waiting request never holds memory its source needs
                         with cond:
                             while req.warm_from not in results:
                                 cond.wait()
                             src = kept.get(req.warm_from)
                         if src is None:
                             raise RuntimeError(f"request {req.warm_from!r} has no solution to start from")
+                    fp = getattr(rte, "footprint_bytes", None)
+                    admit(req.key, fp() if fp is not None else 0)
+                    admitted = True
+                    if src is not None:
                         rte.init_solution_from(src)
                         with cond:
                             pending[req.warm_from] -= 1
                             if pending[req.warm_from] == 0:
                                 kept.pop(req.warm_from).release()
+                                held.pop(req.warm_from, None)
+                                cond.notify_all()
                         rte.solve(req.maxiter, init_solution=False, verbose=False)
                     else:
                         rte.solve(req.maxiter, verbose=False)
@@ -103,10 +144,14 @@ class SolveServer:
                     res = Result(req.key, 0, float("nan"), tn - (t0 + req.arrival_s), tn - ts,
                                  None, f"{type(e).__name__}: {e}")
                 with cond:
+                    if admitted:
+                        running[0] -= 1
                     if res.error is None and pending.get(req.key, 0) > 0:
                         kept[req.key] = rte
-                    elif rte is not None:
-                        rte.release()
+                    else:
+                        if rte is not None:
+                            rte.release()
+                        held.pop(req.key, None)
                     results[req.key] = res
                     cond.notify_all()
 
@@ -127,9 +172,17 @@ class SolveServer:
             rte.release()
         res = [results[k] for k in keys]
         ok = [r for r in res if r.error is None]
-        lat = np.array([r.latency_s for r in ok])
+        # latency per group: the last of its solves to complete
+        glat: dict = {}
+        for rq in reqs:
+            r = results[rq.key]
+            if r.error is None:
+                g = rq.key if rq.group is None else ("g", rq.group)
+                glat[g] = max(glat.get(g, 0.0), r.latency_s)
+        lat = np.array(list(glat.values()))
         svc = np.array([r.service_s for r in ok])
         stats = {"solves": len(res), "failed": len(res) - len(ok), "wall_s": wall,
+                 "requests": len(glat), "peak_device_bytes": peak[0], "memory_budget": budget,
                  "solves_per_s": len(ok) / wall,
                  "latency_p50_ms": float(np.percentile(lat, 50) * 1e3) if lat.size else None,
                  "latency_p99_ms": float(np.percentile(lat, 99) * 1e3) if lat.size else None,

@@ -134,6 +134,11 @@ class RTE:
     def plan_size(self, key: str) -> int:
         return int(_lib.lib().at3d_plan_size(self._plan, key.encode()))
 
+    def footprint_bytes(self) -> int:
+        """Device bytes the solver state of this RTE allocates (known before
+        it is allocated)."""
+        return self.plan_size("footprint_bytes")
+
     _PLAN_SHAPES = {
         "gridpos": ("f4", lambda s: (s._npts, 3)), "dirflux": ("f4", lambda s: (s._npts,)),
         "total_ext": ("f4", lambda s: (s._npts,)), "radiance0": ("f4", lambda s: (s._npts, 4)),

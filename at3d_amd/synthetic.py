@@ -126,3 +126,74 @@ def make_rte(w: Workload, **overrides):
            for i, (e, a, leg) in enumerate(w.species)}
     return RTE(cfg, med, source.solar(w.wavelength, w.solarmu, w.solaraz_deg),
                surface.lambertian(w.gndalbedo))
+
+
+# Serving stream: grids, (NMU, NPHI) angle sets (L = NMU - 1) and wavelengths
+SERVE_GRIDS = ((32, 32, 32), (48, 48, 32), (64, 64, 40), (96, 96, 48))
+SERVE_ANGLES = ((8, 16), (12, 24), (16, 32))
+
+
+def serve_medium(grid, nmu, nphi, wavelength, seed, noise=None, noise_seed=0, solacc=1e-5):
+    """One medium of the serving stream: the config-1 lognormal blob (mean
+    10/km at 0.86 um, sigma 0.5, radius 0.3 km, HG g = 0.85 with NMU Legendre
+    terms, albedo 0.999) on an open grid of `grid` points over a 0.8 x 0.8 x
+    0.64 km domain; extinction scales as (0.86 / wavelength).  `noise` (a
+    relative sigma) multiplies the extinction by seeded lognormal noise, as an
+    inversion step perturbs the medium."""
+    nx, ny, nz = grid
+    dx, dz = 0.8 / nx, 0.64 / nz
+    rng = np.random.default_rng(seed)
+    x = np.arange(nx) * dx
+    y = np.arange(ny) * dx
+    z = (np.arange(nz) * dz).astype(np.float32)
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    r = np.sqrt((X - x[-1] / 2) ** 2 + (Y - y[-1] / 2) ** 2 + (Z - z[-1] / 2) ** 2)
+    inside = r < 0.3
+    ext = np.zeros((nx, ny, nz), np.float32)
+    ext[inside] = 10.0 * (0.86 / wavelength) * np.exp(0.5 * rng.standard_normal(inside.sum()) - 0.125)
+    if noise:
+        nrng = np.random.default_rng(noise_seed)
+        ext *= np.exp(noise * nrng.standard_normal(ext.shape) - 0.5 * noise ** 2).astype(np.float32)
+    alb = np.full(ext.shape, 0.999, np.float32)
+    return Workload("serve", dx, dx, z, [(ext, alb, hg_legendre(0.85, nmu - 1))], bc="open",
+                    nmu=nmu, nphi=nphi, solacc=solacc, wavelength=float(wavelength))
+
+
+def serving_stream(n_requests=512, rate_hz=20.0, warm_fraction=0.3, seed=0, grids=SERVE_GRIDS,
+                   angles=SERVE_ANGLES, max_wavelengths=4, solacc=1e-5):
+    """The serving stream's requests: Poisson arrivals at `rate_hz`; each
+    request draws a grid, an angle set and 1..max_wavelengths wavelengths in
+    0.55-2.2 um, and is one solve per wavelength (a ``Request`` each, sharing
+    the request's ``group``).  A `warm_fraction` of requests restart from an
+    earlier request (same grid, angles and wavelengths; extinction perturbed
+    by 5 % lognormal noise) and seed each solve from that request's solution.
+    Returns the list of ``serving.Request``."""
+    from .serving import Request
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    specs, reqs = [], []
+    for i in range(n_requests):
+        t += rng.exponential(1.0 / rate_hz)
+        src = None
+        if specs and rng.random() < warm_fraction:
+            src = int(rng.integers(len(specs)))
+        if src is None:
+            grid = grids[int(rng.integers(len(grids)))]
+            nmu, nphi = angles[int(rng.integers(len(angles)))]
+            nwl = int(rng.integers(1, max_wavelengths + 1))
+            wls = np.sort(rng.uniform(0.55, 2.2, nwl)).round(4).tolist()
+            mseed = int(rng.integers(1 << 31))
+            spec = dict(grid=grid, nmu=nmu, nphi=nphi, wls=wls, seed=mseed, noise=None, src=None)
+        else:
+            s = specs[src]
+            spec = dict(s, noise=0.05, noise_seed=int(rng.integers(1 << 31)), src=src)
+        specs.append(spec)
+        for j, wl in enumerate(spec["wls"]):
+            def make(spec=spec, wl=wl):
+                w = serve_medium(spec["grid"], spec["nmu"], spec["nphi"], wl, spec["seed"],
+                                 spec["noise"], spec.get("noise_seed", 0), solacc=solacc)
+                return make_rte(w)
+            reqs.append(Request(key=(i, j), make_rte=make, maxiter=100,
+                                warm_from=None if spec["src"] is None else (spec["src"], j),
+                                arrival_s=t, group=i))
+    return reqs

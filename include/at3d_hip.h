@@ -85,7 +85,9 @@ void at3d_plan_destroy(at3d_plan* plan);
 
 /* Integer sizes of a plan: keys "npts","ncells","nx1","ny1","nz","nlm","ml","mm",
  * "nmu","nang","nphi0max","bcflag","ipflag","ntop","nbot","nlevels_max",
- * "npart","numphase","nleg","nlegp","nscatangle".  Returns -1 for an unknown key. */
+ * "npart","numphase","nleg","nlegp","nscatangle"; "footprint_bytes": the
+ * device bytes a state of this plan will allocate (admission by memory
+ * footprint; no counterpart in the reference).  Returns -1 for an unknown key. */
 int64_t at3d_plan_size(const at3d_plan* plan, const char* key);
 
 /* Copy a named plan array into `dst` (capacity `nbytes`).  Names and layouts:
@@ -211,7 +213,8 @@ int at3d_state_set_option(at3d_state* st, const char* key, int32_t value, char* 
 
 /* Time (ms, HIP events on the state's stream) of each kernel class over the
  * last at3d_solve call: keys "sh_to_do","sweep","do_to_sh","source","total";
- * also "iters".  Returns a negative value for unknown keys. */
+ * also "iters", and "device_bytes" (the state's device allocations).
+ * Returns a negative value for unknown keys. */
 double at3d_state_timing(const at3d_state* st, const char* key);
 
 #ifdef __cplusplus

@@ -10,19 +10,29 @@ from at3d_amd.serving import Request, SolveServer
 
 class FakeRTE:
     live = 0
+    busy = 0
+    busy_max = 0
     lock = threading.Lock()
 
-    def __init__(self, n, fail=False):
-        self.n, self.fail, self.warm = n, fail, None
+    def __init__(self, n, fail=False, nbytes=1):
+        self.n, self.fail, self.warm, self.nbytes = n, fail, None, nbytes
         self.num_iterations, self.solcrit = 0, 1.0
         with FakeRTE.lock:
             FakeRTE.live += 1
+
+    def footprint_bytes(self):
+        return self.nbytes
 
     def init_solution_from(self, other):
         self.warm = other.n
 
     def solve(self, maxiter, init_solution=True, verbose=False):
+        with FakeRTE.lock:
+            FakeRTE.busy += 1
+            FakeRTE.busy_max = max(FakeRTE.busy_max, FakeRTE.busy)
         time.sleep(0.01)
+        with FakeRTE.lock:
+            FakeRTE.busy -= 1
         if self.fail:
             raise RuntimeError("boom")
         self.num_iterations = 3 if self.warm is not None else 7
@@ -67,3 +77,37 @@ def test_server_rejects_warm_start_from_a_later_request():
     with pytest.raises(ValueError):
         SolveServer().run([Request(0, lambda: FakeRTE(0), warm_from=1),
                            Request(1, lambda: FakeRTE(1), arrival_s=1.0)])
+
+
+def test_server_admits_by_memory_footprint():
+    """With a budget of 100 bytes, solves of 60 bytes run one at a time, 40-byte
+    ones two at a time, and a 150-byte solve (over budget) still runs, alone; a
+    kept warm-start source counts against the budget until its last user
+    starts."""
+    FakeRTE.live = FakeRTE.busy = FakeRTE.busy_max = 0
+    reqs = [Request(k, (lambda k=k: FakeRTE(k, nbytes=60))) for k in range(4)]
+    res, st = SolveServer(nworkers=4, memory_budget=100).run(reqs)
+    assert all(r.error is None for r in res)
+    assert FakeRTE.busy_max == 1 and st["peak_device_bytes"] == 60
+    FakeRTE.busy_max = 0
+    reqs = [Request(k, (lambda k=k: FakeRTE(k, nbytes=40))) for k in range(6)]
+    res, st = SolveServer(nworkers=4, memory_budget=100).run(reqs)
+    assert FakeRTE.busy_max == 2 and st["peak_device_bytes"] <= 100
+    FakeRTE.busy_max = 0
+    reqs = [Request("big", lambda: FakeRTE(0, nbytes=150)), Request("small", lambda: FakeRTE(1, nbytes=10))]
+    res, st = SolveServer(nworkers=2, memory_budget=100).run(reqs)
+    assert all(r.error is None for r in res) and FakeRTE.busy_max == 1
+    reqs = [Request(0, lambda: FakeRTE(0, nbytes=50)), Request(1, lambda: FakeRTE(1, nbytes=50), warm_from=0),
+            Request(2, lambda: FakeRTE(2, nbytes=50), warm_from=0)]
+    res, st = SolveServer(nworkers=3, memory_budget=100).run(reqs)
+    assert all(r.error is None for r in res) and [r.iterations for r in res] == [7, 3, 3]
+    assert st["peak_device_bytes"] <= 100
+    assert FakeRTE.live == 0
+
+
+def test_group_latency_is_the_last_solve_of_the_group():
+    reqs = [Request((0, j), lambda: FakeRTE(0), group=0) for j in range(3)] + \
+           [Request((1, 0), lambda: FakeRTE(1), group=1, arrival_s=0.05)]
+    res, st = SolveServer(nworkers=1).run(reqs)
+    assert st["requests"] == 2 and st["solves"] == 4
+    assert st["latency_p99_ms"] >= 25.0     # group 0 waits for three serial solves of 10 ms
