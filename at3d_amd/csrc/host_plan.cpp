@@ -556,8 +556,12 @@ void source_constants(Plan& p) {
   p.chi.assign((size_t)npts * (ML + 1), 0.f);
   p.alb.assign(npts, 0.f);
   p.tc.assign((size_t)npts * (ML + 1), 0.f);
+  // points are independent: chunks over the host threads
+  const int nchunk = 64;
+  parallel_for(nchunk, [&](int c) {
   std::vector<float> l1, legent(nl);
-  for (int i = 0; i < npts; i++) {
+  const int i0 = (int)((int64_t)npts * c / nchunk), i1 = (int)((int64_t)npts * (c + 1) / nchunk);
+  for (int i = i0; i < i1; i++) {
     const float ext = p.total_ext[i];
     double alb = 0.0;
     std::fill(legent.begin(), legent.end(), 0.f);
@@ -620,6 +624,7 @@ void source_constants(Plan& p) {
       p.tc[(size_t)i * (ML + 1) + l] = rad;
     }
   }
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -970,11 +975,16 @@ bool init_radiance(Plan& p, std::string& err) {
   for (int imu = 0; imu < p.nmu / 2; imu++)
     for (int ip = 0; ip < p.nphi0[imu]; ip++)
       skyradalb = skyradalb + fabsf(p.mu[imu]) * p.wtdo[(size_t)imu * p.nphi0max + ip] * p.d.skyrad;
-  std::vector<float> od(nz), albs(nz), asym(nz), fl;
   p.radiance0.assign((size_t)p.npts * 4, 0.f);
   auto wt1 = [&](int ipa, int ip, int q) { return p.phaseinterpwt[((size_t)ipa * p.npts + ip) * 8 + q]; };
   auto iph = [&](int ipa, int ip, int q) { return p.iphase[((size_t)ipa * p.npts + ip) * 8 + q]; };
-  for (int I = 0; I < nxy; I++) {
+  // columns are independent: chunks over the host threads
+  const int nchunk = 64;
+  std::vector<char> bad(nchunk, 0);
+  parallel_for(nchunk, [&](int c) {
+  std::vector<float> od(nz), albs(nz), asym(nz), fl;
+  const int c0 = (int)((int64_t)nxy * c / nchunk), c1 = (int)((int64_t)nxy * (c + 1) / nchunk);
+  for (int I = c0; I < c1; I++) {
     for (int iz = 1; iz <= nlayer; iz++) {
       const int L = nz - iz;
       const int ip0 = (iz - 1) + nz * I, ip1 = iz + nz * I;
@@ -1026,8 +1036,8 @@ bool init_radiance(Plan& p, std::string& err) {
     }
     const float gndemis = 1.0f - p.d.gndalbedo;
     if (!eddrtf_solar(nlayer, od, albs, asym, p.d.solarflux, p.d.solarmu, gndemis, skyradalb, fl)) {
-      err = "EDDRTF: singular two-stream system";
-      return false;
+      bad[c] = 1;
+      return;
     }
     for (int iz = 1; iz <= nz; iz++) {
       const int L = nz + 1 - iz;
@@ -1036,6 +1046,12 @@ bool init_radiance(Plan& p, std::string& err) {
       p.radiance0[ip * 4 + 2] = C1 * (fl[3 * (L - 1) + 0] - fl[3 * (L - 1) + 1]);
     }
   }
+  });
+  for (char b : bad)
+    if (b) {
+      err = "EDDRTF: singular two-stream system";
+      return false;
+    }
   return true;
 }
 
@@ -1302,34 +1318,48 @@ int render_constants(Plan& p, std::string& err) {
   p.ss_off.assign(npts + 1, 0);
   p.ss_iph.clear();
   p.ss_coef.clear();
-  std::vector<float> legt;
-  for (int ip = 0; ip < npts; ip++) {
-    p.ss_off[ip] = (int)p.ss_iph.size();
-    if (!p.d.deltam) continue;
-    const float ext = p.total_ext[ip];
-    for (int ipa = 0; ipa < npart; ipa++) {
-      const float W = (ext == 0.0f) ? 1.0f : p.extinct[(size_t)ipa * npts + ip] / ext;
-      if (W == 0.0f) continue;
-      mixed_legen(p, ipa, ip, legt);
-      const float f = legt[ML + 1];
-      for (float& v : legt) v = v / (1 - f);
-      const float da = p.albedo[(size_t)ipa * npts + ip] * p.dirflux[ip] * secmu0 * W;
-      for (int l = 0; l <= ML; l++) p.tms[(size_t)ip * (ML + 1) + l] += da * legt[l];
-      const size_t base = ((size_t)ipa * npts + ip) * 8;
-      if (p.phaseinterpwt[base] >= 0.999f) {
-        p.ss_iph.push_back(p.iphase[base] - 1);
-        p.ss_coef.push_back(da / (1 - f));
-      } else {
-        for (int q = 0; q < 8; q++) {
-          const float w = p.phaseinterpwt[base + q];
-          if (w <= 1e-5f) continue;
-          p.ss_iph.push_back(p.iphase[base + q] - 1);
-          p.ss_coef.push_back(da * w / (1 - f));
+  // per point: the TMS terms and its (phase, coefficient) list; points in
+  // chunks over the host threads, each chunk's lists joined in point order
+  const int nchunk = 64;
+  std::vector<std::vector<int32_t>> c_iph(nchunk);
+  std::vector<std::vector<float>> c_coef(nchunk);
+  std::vector<int32_t> cnt(npts, 0);
+  parallel_for(nchunk, [&](int c) {
+    std::vector<float> legt;
+    const int i0 = (int)((int64_t)npts * c / nchunk), i1 = (int)((int64_t)npts * (c + 1) / nchunk);
+    for (int ip = i0; ip < i1; ip++) {
+      if (!p.d.deltam) continue;
+      const size_t before = c_iph[c].size();
+      const float ext = p.total_ext[ip];
+      for (int ipa = 0; ipa < npart; ipa++) {
+        const float W = (ext == 0.0f) ? 1.0f : p.extinct[(size_t)ipa * npts + ip] / ext;
+        if (W == 0.0f) continue;
+        mixed_legen(p, ipa, ip, legt);
+        const float f = legt[ML + 1];
+        for (float& v : legt) v = v / (1 - f);
+        const float da = p.albedo[(size_t)ipa * npts + ip] * p.dirflux[ip] * secmu0 * W;
+        for (int l = 0; l <= ML; l++) p.tms[(size_t)ip * (ML + 1) + l] += da * legt[l];
+        const size_t base = ((size_t)ipa * npts + ip) * 8;
+        if (p.phaseinterpwt[base] >= 0.999f) {
+          c_iph[c].push_back(p.iphase[base] - 1);
+          c_coef[c].push_back(da / (1 - f));
+        } else {
+          for (int q = 0; q < 8; q++) {
+            const float w = p.phaseinterpwt[base + q];
+            if (w <= 1e-5f) continue;
+            c_iph[c].push_back(p.iphase[base + q] - 1);
+            c_coef[c].push_back(da * w / (1 - f));
+          }
         }
       }
+      cnt[ip] = (int32_t)(c_iph[c].size() - before);
     }
+  });
+  for (int ip = 0; ip < npts; ip++) p.ss_off[ip + 1] = p.ss_off[ip] + cnt[ip];
+  for (int c = 0; c < nchunk; c++) {
+    p.ss_iph.insert(p.ss_iph.end(), c_iph[c].begin(), c_iph[c].end());
+    p.ss_coef.insert(p.ss_coef.end(), c_coef[c].begin(), c_coef[c].end());
   }
-  p.ss_off[npts] = (int)p.ss_iph.size();
   return AT3D_OK;
 }
 

@@ -45,6 +45,7 @@ class Result:
     solcrit: float
     latency_s: float               # completion - arrival
     service_s: float               # completion - start of service
+    setup_s: float = 0.0           # host setup (make_rte) within the service time
     radiance: np.ndarray | None = None
     error: str | None = None
 
@@ -106,9 +107,11 @@ class SolveServer:
                     return
                 ts = time.perf_counter()
                 rte = None
+                setup = 0.0
                 admitted = False
                 try:
                     rte = req.make_rte()           # host setup: no device memory yet
+                    setup = time.perf_counter() - ts
                     src = None
                     if req.warm_from is not None:
                         # wait for the source before taking device memory, so a
@@ -138,11 +141,11 @@ class SolveServer:
                         rad = rte.integrate_to_sensor(dict(req.render))["I"]
                     tn = time.perf_counter()
                     res = Result(req.key, rte.num_iterations, rte.solcrit,
-                                 tn - (t0 + req.arrival_s), tn - ts, rad, None)
+                                 tn - (t0 + req.arrival_s), tn - ts, setup, rad, None)
                 except Exception as e:  # reported per request
                     tn = time.perf_counter()
                     res = Result(req.key, 0, float("nan"), tn - (t0 + req.arrival_s), tn - ts,
-                                 None, f"{type(e).__name__}: {e}")
+                                 0.0, None, f"{type(e).__name__}: {e}")
                 with cond:
                     if admitted:
                         running[0] -= 1
@@ -186,6 +189,8 @@ class SolveServer:
                  "solves_per_s": len(ok) / wall,
                  "latency_p50_ms": float(np.percentile(lat, 50) * 1e3) if lat.size else None,
                  "latency_p99_ms": float(np.percentile(lat, 99) * 1e3) if lat.size else None,
+                 "setup_mean_ms": float(np.mean([r.setup_s for r in ok]) * 1e3) if ok else None,
+                 "service_mean_ms": float(svc.mean() * 1e3) if svc.size else None,
                  "service_p50_ms": float(np.percentile(svc, 50) * 1e3) if svc.size else None,
                  "service_p99_ms": float(np.percentile(svc, 99) * 1e3) if svc.size else None}
         return res, stats

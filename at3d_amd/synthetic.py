@@ -135,8 +135,10 @@ SERVE_ANGLES = ((8, 16), (12, 24), (16, 32))
 
 def serve_medium(grid, nmu, nphi, wavelength, seed, noise=None, noise_seed=0, solacc=1e-5):
     """One medium of the serving stream: the config-1 lognormal blob (mean
-    10/km at 0.86 um, sigma 0.5, radius 0.3 km, HG g = 0.85 with NMU Legendre
-    terms, albedo 0.999) on an open grid of `grid` points over a 0.8 x 0.8 x
+    10/km at 0.86 um, sigma 0.5, radius 0.3 km, HG g = 0.85 as 64 Legendre
+    terms (the solve keeps L + 1 = NMU of them; a shorter table is negative at
+    back-scatter, which rendering rejects), albedo 0.999) on an open grid of
+    `grid` points over a 0.8 x 0.8 x
     0.64 km domain; extinction scales as (0.86 / wavelength).  `noise` (a
     relative sigma) multiplies the extinction by seeded lognormal noise, as an
     inversion step perturbs the medium."""
@@ -155,7 +157,7 @@ def serve_medium(grid, nmu, nphi, wavelength, seed, noise=None, noise_seed=0, so
         nrng = np.random.default_rng(noise_seed)
         ext *= np.exp(noise * nrng.standard_normal(ext.shape) - 0.5 * noise ** 2).astype(np.float32)
     alb = np.full(ext.shape, 0.999, np.float32)
-    return Workload("serve", dx, dx, z, [(ext, alb, hg_legendre(0.85, nmu - 1))], bc="open",
+    return Workload("serve", dx, dx, z, [(ext, alb, hg_legendre(0.85, 64))], bc="open",
                     nmu=nmu, nphi=nphi, solacc=solacc, wavelength=float(wavelength))
 
 

@@ -81,6 +81,32 @@ def measured_traffic(kernel_class: str):
     return None
 
 
+def serving_stream_bench(n_requests, rate_hz, workers):
+    """The mixed-size serving stream (grids 32^3 .. 96x96x48, (NMU, NPHI) from
+    (8,16) to (16,32), 1-4 wavelengths per request, 30 % warm restarts with 5 %
+    extinction noise, solution accuracy 1e-5), Poisson arrivals: served by
+    `workers` solvers with admission by device footprint, and the same
+    requests one at a time.  Host setup (plan) is part of every request."""
+    from at3d_amd import synthetic as S
+    from at3d_amd.serving import SolveServer
+    reqs = S.serving_stream(n_requests, rate_hz=rate_hz, seed=1)
+    res, st = SolveServer(nworkers=workers).run(reqs)
+    served_iters = {r.key: (r.iterations, r.solcrit) for r in res}
+    for r in reqs:
+        r.arrival_s = 0.0
+    res1, st1 = SolveServer(nworkers=1).run(reqs)
+    same = all(served_iters[r.key] == (r.iterations, r.solcrit) for r in res1)
+    return {"requests": st["requests"], "solves": st["solves"], "failed": st["failed"] + st1["failed"],
+            "offered_rate_hz": rate_hz, "workers": workers,
+            "served_solves_per_s": st["solves_per_s"], "one_at_a_time_solves_per_s": st1["solves_per_s"],
+            "speedup": st["solves_per_s"] / st1["solves_per_s"],
+            "latency_ms": {"p50": st["latency_p50_ms"], "p99": st["latency_p99_ms"]},
+            "one_at_a_time_per_solve_ms": {"host_setup": st1["setup_mean_ms"],
+                                           "total": st1["service_mean_ms"]},
+            "peak_device_GB": st["peak_device_bytes"] / 1e9,
+            "same_iterations_and_solcrit_as_one_at_a_time": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +117,10 @@ def main():
                     help="reference iterations timed for cpu_baseline (0: the GPU solve's count)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--stream-requests", type=int, default=24,
+                    help="requests of the mixed-size serving stream (0: skip it)")
+    ap.add_argument("--stream-rate", type=float, default=8.0, help="its Poisson arrival rate (1/s)")
+    ap.add_argument("--stream-workers", type=int, default=8)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy a buffer of this size once (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -227,6 +257,10 @@ def main():
     solve_roof = {"bytes_per_solve": b_solve, "bytes_per_iteration": b_iter, "GB/s": solve_gbs,
                   "frac": solve_gbs / HBM_PEAK_GBS}
 
+    stream = None
+    if args.stream_requests > 0:
+        stream = serving_stream_bench(args.stream_requests, args.stream_rate, args.stream_workers)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.workload, iters, args.cpu_iters)
@@ -253,6 +287,7 @@ def main():
         "kernels": kern,
         "roofline": roof,
         "solve_roofline": solve_roof,
+        "serving_stream": stream,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
     }

@@ -65,3 +65,41 @@ def test_concurrent_serving_is_bit_identical_to_serial_solves():
     assert st["failed"] == 0, [r.error for r in res]
     for k in (1, 2, 3):
         assert rel_l2(res[k].radiance, serial[2 + k]) < 2e-4
+
+
+def test_serving_stream_results_are_the_same_alone_and_in_the_mix():
+    """A small mixed stream (two grids, two angle sets, up to three
+    wavelengths per request, warm restarts) served by three solvers with a
+    memory budget that keeps some requests waiting: every solve's iteration
+    count, SOLCRIT and rendered radiances equal those of the same requests run
+    one at a time."""
+    reqs = S.serving_stream(8, rate_hz=50.0, seed=5, grids=((16, 16, 16), (24, 24, 16)),
+                            angles=((8, 16), (12, 24)), max_wavelengths=3, solacc=1e-4)
+    view = {"ray_x": np.linspace(0.1, 0.7, 6), "ray_y": np.full(6, 0.4), "ray_z": np.full(6, 1.0),
+            "ray_mu": np.full(6, 1.0), "ray_phi": np.zeros(6)}
+    for r in reqs:
+        r.render = view
+    fps = [r.make_rte().footprint_bytes() for r in reqs]
+    budget = 2 * max(fps)
+    res, st = SolveServer(nworkers=3, memory_budget=budget).run(reqs)
+    assert st["failed"] == 0, [r.error for r in res]
+    # over the budget only by the one request admitted when nothing else runs
+    assert st["peak_device_bytes"] <= budget + max(fps)
+    for r in reqs:
+        r.arrival_s = 0.0
+    res1, st1 = SolveServer(nworkers=1).run(reqs)
+    assert st1["failed"] == 0
+    for a, b in zip(res, res1):
+        assert (a.iterations, a.solcrit) == (b.iterations, b.solcrit)
+        assert np.array_equal(a.radiance, b.radiance)
+
+
+def test_footprint_estimate_covers_the_state():
+    """The footprint known before allocation is within 10 % above the state's
+    device allocations (it is what admission reserves)."""
+    for w in (blob8("open"), cloud_small(), S.serve_medium((24, 24, 16), 12, 24, 1.0, 0)):
+        r = S.make_rte(w)
+        r.solve(0, verbose=False)
+        est, got = r.footprint_bytes(), r.timing("device_bytes")
+        assert got <= est <= 1.1 * got + (1 << 20), (est, got)
+        r.release()
