@@ -143,6 +143,12 @@ struct SourceArgs {
   int ld, npts, ml, mm, nlm;
   float shacc;
   int first, accel, fixsh;
+  // RADIANCE_TRUNCATION of the next iteration, from this pass's NS and the
+  // radiance's first term: NR written to nr_next
+  int* nr_next;
+  const float* tc;    // [npts][ml+1]
+  float radmin;
+  int highorder;
 };
 constexpr int SRC_PTS_PER_BLOCK = 16;
 void launch_radiance_truncation(hipStream_t st, const float* R, int ld, const float* tc,

@@ -344,6 +344,7 @@ int state_create(const Plan& p, State& s, std::string& err) {
   TRY(alloc0(s, s.dorad, ndo, err));
   TRY(alloc0(s, s.ns, (size_t)p.npts, err));
   TRY(alloc0(s, s.nr, (size_t)p.npts, err));
+  TRY(alloc0(s, s.nr_next, (size_t)p.npts, err));
   TRY(alloc0(s, s.nos, (size_t)p.npts, err));
   TRY(alloc0(s, s.fluxes, (size_t)2 * p.npts, err));
   s.nsrc_blocks = (p.npts + SRC_PTS_PER_BLOCK - 1) / SRC_PTS_PER_BLOCK;
@@ -450,6 +451,10 @@ static SourceArgs source_args(State& s, int first, int fixsh) {
   SourceArgs a;
   a.R = s.R;
   a.nr = s.nr;
+  a.nr_next = s.nr_next;
+  a.tc = s.tc;
+  a.radmin = p.d.shacc;
+  a.highorder = p.d.high_order_radiance ? 1 : 0;
   a.alb = s.alb;
   a.chi = s.chi;
   a.f0 = s.f0;
@@ -480,6 +485,7 @@ int state_init_solution(State& s, std::string& err) {
   const Plan& p = *s.plan;
   launch_init_radiance(s.stream, s.R, s.R0, s.nr, s.ld, p.npts);
   launch_compute_source(s.stream, source_args(s, 1, 0), s.nsrc_blocks);
+  s.nr_next_fixsh = false;
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s.stream));
   s.solcrit = 1.0f;
@@ -500,6 +506,7 @@ int state_init_solution_from(State& s, State& src, std::string& err) {
   HIPCHK(hipMemcpyAsync(s.R, src.R, nsh * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
   HIPCHK(hipMemcpyAsync(s.nr, src.nr, p.npts * sizeof(int), hipMemcpyDeviceToDevice, s.stream));
   launch_compute_source(s.stream, source_args(s, 1, 0), s.nsrc_blocks);
+  s.nr_next_fixsh = false;
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s.stream));
   s.solcrit = 1.0f;
@@ -528,8 +535,9 @@ static void enqueue_sweeps(State& s) {
   };
   bool fast = false, full = false;
   if (ring) {
+    // the first launch is timed from the event the caller records just
+    // before this function, the second to the one just after it
     needs(0, s.ndown, fast, full);
-    (void)hipEventRecord(s.ev_sweep[0], s.stream);
     launch_sweep_ring(s.stream, a, 0, s.ndown, s.ring_lds, fast, full);
     (void)hipEventRecord(s.ev_sweep[1], s.stream);
   } else {
@@ -543,7 +551,6 @@ static void enqueue_sweeps(State& s) {
     needs(s.ndown, p.nang, fast, full);
     (void)hipEventRecord(s.ev_sweep[2], s.stream);
     launch_sweep_ring(s.stream, a, s.ndown, p.nang - s.ndown, s.ring_lds, fast, full);
-    (void)hipEventRecord(s.ev_sweep[3], s.stream);
   } else {
     launch_sweep_preset(s.stream, a, true, s.ndown, p.nang - s.ndown);
     launch_sweep_columns(s.stream, a, s.ndown, p.nang - s.ndown);
@@ -609,17 +616,23 @@ int state_solve(State& s, int maxiter, int verbose, std::string& err) {
   HIPCHK(hipMemsetAsync(s.errflag, 0, sizeof(int), s.stream));
   float a = 0.0f;
   bool fixsh = false;
-  const float radmin = p.d.shacc;
   for (auto& t : s.tsum) t = 0.0;
   s.iters_last = 0;
   s.nsweep_launch = 0;
+  // the NR waiting in nr_next was made with the last call's FIXSH; a call
+  // starts with FIXSH off, so it is remade when that differs
+  if (s.nr_next_fixsh && s.iters < maxiter && s.solcrit > p.d.solacc) {
+    launch_radiance_truncation(s.stream, s.R, s.ld, s.tc, s.ns, s.lofj, s.nr_next, p.npts, p.ml, p.mm,
+                               p.nlm, p.d.shacc, 0, p.d.high_order_radiance ? 1 : 0);
+    s.nr_next_fixsh = false;
+  }
   while (s.iters < maxiter && s.solcrit > p.d.solacc) {
     s.iters++;
     s.iters_last++;
+    // RADIANCE_TRUNCATION of this iteration was done by the last source pass
+    // (into nr_next): it becomes the current NR
+    std::swap(s.nr, s.nr_next);
     (void)hipEventRecord(s.ev[0], s.stream);
-    launch_radiance_truncation(s.stream, s.R, s.ld, s.tc, s.ns, s.lofj, s.nr, p.npts, p.ml, p.mm,
-                               p.nlm, radmin, fixsh ? 1 : 0, p.d.high_order_radiance ? 1 : 0);
-    (void)hipEventRecord(s.ev[1], s.stream);
     s.dosrc_level_order = ring_active(s);
     if (s.dosrc_level_order)
       launch_sh_to_do_perm(s.stream, s.S, s.Y, s.ext, s.ns, s.oct_perm, s.sh_tiles, s.n_sh_tiles,
@@ -634,6 +647,7 @@ int state_solve(State& s, int maxiter, int verbose, std::string& err) {
     (void)hipEventRecord(s.ev[4], s.stream);
     if (s.solcrit < 0.001f || s.iters > p.d.iterfixsh) fixsh = true;
     launch_compute_source(s.stream, source_args(s, 0, fixsh ? 1 : 0), s.nsrc_blocks);
+    s.nr_next_fixsh = fixsh;
     launch_reduce_dots(s.stream, s.partial, s.nsrc_blocks, s.dots);
     (void)hipEventRecord(s.ev[5], s.stream);
     HIPCHK(hipGetLastError());
@@ -651,17 +665,17 @@ int state_solve(State& s, int maxiter, int verbose, std::string& err) {
       err = buf;
       return AT3D_ERR_NUMERIC;
     }
-    float ms[5];
-    for (int e = 0; e < 5; e++) (void)hipEventElapsedTime(&ms[e], s.ev[e], s.ev[e + 1]);
-    s.tsum[0] += ms[0];  // radiance truncation
+    // (the radiance truncation is part of the source pass: tsum[0] stays 0)
+    float ms[5] = {};
+    for (int e = 1; e < 5; e++) (void)hipEventElapsedTime(&ms[e], s.ev[e == 1 ? 0 : e], s.ev[e + 1]);
     s.tsum[1] += ms[1];  // SH -> DO
     s.tsum[2] += ms[2];  // sweeps (both hemispheres, surface boundary)
     s.tsum[3] += ms[3];  // DO -> SH
-    s.tsum[4] += ms[4];  // source + reduction
+    s.tsum[4] += ms[4];  // source (with the next truncation) + reduction
     if (ring_active(s)) {
       float a0 = 0.f, a1 = 0.f;
-      (void)hipEventElapsedTime(&a0, s.ev_sweep[0], s.ev_sweep[1]);
-      (void)hipEventElapsedTime(&a1, s.ev_sweep[2], s.ev_sweep[3]);
+      (void)hipEventElapsedTime(&a0, s.ev[2], s.ev_sweep[1]);
+      (void)hipEventElapsedTime(&a1, s.ev_sweep[2], s.ev[3]);
       s.tsum[6] += a0 + a1;  // the two sweep kernel launches alone
     }
     s.nsweep_launch += 1;

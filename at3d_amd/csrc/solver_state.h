@@ -37,6 +37,8 @@ struct State {
   float* R0 = nullptr;   // [npts][4] two-stream initial radiance terms
   double* host_dots = nullptr;   // pinned: four dot products, then the error flag
   int *ns = nullptr, *nr = nullptr, *nos = nullptr;
+  int* nr_next = nullptr;   // NR for the next iteration, written by the source pass
+  bool nr_next_fixsh = false;   // nr_next was made with FIXSH on
   // DO fields [ldo][nptsp] (ordinate-major)
   float *dosrc = nullptr, *dorad = nullptr;
   float* fluxes = nullptr;

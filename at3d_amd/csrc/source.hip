@@ -140,6 +140,25 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
       if (J < sw) Sp[J] = (J < nsn && J < a.nlm) ? st[it] : 0.f;
     }
     if (lane == 0) a.ns[p] = nsn;
+    // the next iteration's radiance truncation (RADIANCE_TRUNCATION
+    // shdomsub1.f:1615-1803, as k_radiance_truncation): it reads NS as just
+    // written and the radiance's first term (lane 0's first term here)
+    int nrn;
+    if (a.fixsh) {
+      nrn = a.highorder ? nterms_for_l(a.ml, a.mm) : max(4, nsn);
+    } else {
+      const float i0 = __shfl(rv[0], 0);
+      const float* tcp = a.tc + (size_t)p * (a.ml + 1);
+      int lr = 1;
+      for (int l = 1 + lane; l <= a.ml; l += 64)
+        if (tcp[l] * i0 > a.radmin) lr = l;
+      for (int off = 32; off > 0; off >>= 1) lr = max(lr, __shfl_xor(lr, off));
+      const int ls = a.lofj[max(1, nsn) - 1];
+      lr = min(lr, ls + a.ml / 8 + 2);
+      if (a.highorder) lr = a.ml;
+      nrn = nterms_for_l(lr, a.mm);
+    }
+    if (lane == 0) a.nr_next[p] = nrn;
     if (a.first && a.accel) {
       // DS cleared (it is zero from NOS on)
 #pragma unroll
