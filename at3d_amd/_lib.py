@@ -63,6 +63,7 @@ def lib():
         L = C.CDLL(LIBPATH)
         vp = C.c_void_p
         L.at3d_version.restype = C.c_char_p
+        L.at3d_device_trim.restype = C.c_int64
         L.at3d_plan_create.argtypes = [C.POINTER(SceneDesc), C.POINTER(vp), C.c_char_p, C.c_size_t]
         L.at3d_plan_destroy.argtypes = [vp]
         L.at3d_plan_size.argtypes = [vp, C.c_char_p]
@@ -105,7 +106,7 @@ EXPORTS = ["at3d_version", "at3d_plan_create", "at3d_plan_destroy", "at3d_plan_s
            "at3d_plan_get", "at3d_state_create", "at3d_state_destroy", "at3d_init_solution",
            "at3d_solve", "at3d_state_get", "at3d_state_set_option", "at3d_state_timing",
            "at3d_render", "at3d_levis_gradient", "at3d_levis_gradient_jacobian",
-           "at3d_init_solution_from"]
+           "at3d_init_solution_from", "at3d_device_trim"]
 
 
 def as_f32(a):

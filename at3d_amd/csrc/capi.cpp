@@ -8,6 +8,7 @@
 #include "../../include/at3d_hip.h"
 #include "host_plan.h"
 #include "ring_plan.h"
+#include "devmem.h"
 #include "solver_state.h"
 
 struct at3d_plan {
@@ -40,6 +41,12 @@ int copy_out(const std::vector<T>& v, void* dst, size_t nbytes, char* err, size_
 extern "C" {
 
 const char* at3d_version(void) { return "at3d_hip 0.1 (gfx950)"; }
+
+int64_t at3d_device_trim(void) {
+  const size_t held = at3d::dev_cached_bytes();
+  at3d::dev_trim();
+  return (int64_t)held;
+}
 
 int at3d_plan_create(const at3d_scene_desc* desc, at3d_plan** out, char* err, size_t errlen) {
   if (!desc || !out) {

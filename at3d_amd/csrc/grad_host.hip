@@ -314,11 +314,12 @@ int state_levis_gradient(State& s, const at3d_deriv_desc& dd, int64_t nrays, con
   const size_t o_jib = carve((size_t)std::max(njac, 0) * 4);
   const size_t o_jac = carve(njac > 0 ? (size_t)npix * njac * numder * 4 : 0);
   if (off > s.gbuf_bytes) {
-    std::lock_guard<std::mutex> lock(device_mutex());
-    if (s.gbuf) (void)hipFree(s.gbuf);
+    // the stream's earlier work may still read the old buffer
+    HIPCHK(hipStreamSynchronize(s.stream));
+    dev_free(s.gbuf, s.gbuf_bytes);
     s.gbuf = nullptr;
     s.gbuf_bytes = 0;
-    HIPCHK(hipMalloc(&s.gbuf, off));
+    HIPCHK(dev_alloc(&s.gbuf, off));
     s.gbuf_bytes = off;
   }
   char* B = static_cast<char*>(s.gbuf);

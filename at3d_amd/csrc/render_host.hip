@@ -143,11 +143,12 @@ int state_render(State& s, int64_t nrays, const float* x, const float* y, const 
   const size_t o_mu = carve(nr * 8), o_phi = carve(nr * 8), o_dir = carve(nr * 4);
   const size_t o_top = carve(nr * 4), o_out = carve(nr * 4);
   if (off > s.rbuf_bytes) {
-    std::lock_guard<std::mutex> lock(device_mutex());
-    if (s.rbuf) (void)hipFree(s.rbuf);
+    // the stream's earlier work may still read the old buffer
+    HIPCHK(hipStreamSynchronize(s.stream));
+    dev_free(s.rbuf, s.rbuf_bytes);
     s.rbuf = nullptr;
     s.rbuf_bytes = 0;
-    HIPCHK(hipMalloc(&s.rbuf, off));
+    HIPCHK(dev_alloc(&s.rbuf, off));
     s.rbuf_bytes = off;
   }
   char* B = static_cast<char*>(s.rbuf);

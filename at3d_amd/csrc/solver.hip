@@ -26,6 +26,7 @@
 #include <string>
 #include <vector>
 
+#include "devmem.h"
 #include "host_plan.h"
 #include "kernels.h"
 #include "ring_plan.h"
@@ -57,9 +58,10 @@ static size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
 // other) and waited for before returning.
 template <typename T>
 static int upload(State& s, T*& dst, const T* src, size_t n, std::string& err) {
-  HIPCHK(hipMalloc((void**)&dst, std::max<size_t>(n, 1) * sizeof(T)));
-  s.allocs.push_back(dst);
-  s.device_bytes += std::max<size_t>(n, 1) * sizeof(T);
+  const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+  HIPCHK(dev_alloc((void**)&dst, bytes));
+  s.allocs.emplace_back(dst, bytes);
+  s.device_bytes += bytes;
   if (n) {
     HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s.stream));
     HIPCHK(hipStreamSynchronize(s.stream));
@@ -69,10 +71,11 @@ static int upload(State& s, T*& dst, const T* src, size_t n, std::string& err) {
 
 template <typename T>
 static int alloc0(State& s, T*& dst, size_t n, std::string& err) {
-  HIPCHK(hipMalloc((void**)&dst, std::max<size_t>(n, 1) * sizeof(T)));
-  s.allocs.push_back(dst);
-  s.device_bytes += std::max<size_t>(n, 1) * sizeof(T);
-  HIPCHK(hipMemsetAsync(dst, 0, std::max<size_t>(n, 1) * sizeof(T), s.stream));
+  const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+  HIPCHK(dev_alloc((void**)&dst, bytes));
+  s.allocs.emplace_back(dst, bytes);
+  s.device_bytes += bytes;
+  HIPCHK(hipMemsetAsync(dst, 0, bytes, s.stream));
   HIPCHK(hipStreamSynchronize(s.stream));
   return AT3D_OK;
 }
@@ -209,12 +212,12 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
       sweep_ring_max_lds(s.ring_lds) != 0)
     return AT3D_OK;
   const size_t nrec = (size_t)p.nang * np * 2;
-  if (hipMalloc((void**)&s.rec, nrec * sizeof(uint4)) != hipSuccess) {
+  if (dev_alloc((void**)&s.rec, nrec * sizeof(uint4)) != hipSuccess) {
     (void)hipGetLastError();
     s.rec = nullptr;
     return AT3D_OK;
   }
-  s.allocs.push_back(s.rec);
+  s.allocs.emplace_back(s.rec, nrec * sizeof(uint4));
   s.device_bytes += nrec * sizeof(uint4);
   std::vector<int32_t> top_index(p.npts, -1);
   for (size_t b = 0; b < p.bctop.size(); b++) top_index[p.bctop[b]] = (int32_t)b;
@@ -247,10 +250,6 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
   return AT3D_OK;
 }
 
-std::mutex& device_mutex() {
-  static std::mutex m;
-  return m;
-}
 
 // Device bytes a state of this plan allocates (its arrays and the sweep
 // records; not the per-call render and gradient buffers), for admission by
@@ -275,7 +274,6 @@ size_t plan_footprint_bytes(const Plan& p) {
 }
 
 int state_create(const Plan& p, State& s, std::string& err) {
-  std::lock_guard<std::mutex> lock(device_mutex());
   s.plan = &p;
   HIPCHK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
   s.npts = p.npts;
@@ -423,17 +421,21 @@ int state_create(const Plan& p, State& s, std::string& err) {
 }
 
 void state_free(State& s) {
-  std::lock_guard<std::mutex> lock(device_mutex());
-  if (s.sweep_exec) (void)hipGraphExecDestroy(s.sweep_exec);
-  if (s.sweep_graph) (void)hipGraphDestroy(s.sweep_graph);
+  // the stream's work is done before its blocks go back for reuse
+  if (s.stream) (void)hipStreamSynchronize(s.stream);
+  {
+    std::lock_guard<std::mutex> lock(device_mutex());
+    if (s.sweep_exec) (void)hipGraphExecDestroy(s.sweep_exec);
+    if (s.sweep_graph) (void)hipGraphDestroy(s.sweep_graph);
+  }
   s.sweep_exec = nullptr;
   s.sweep_graph = nullptr;
-  for (void* ptr : s.allocs) (void)hipFree(ptr);
+  for (auto& a : s.allocs) dev_free(a.first, a.second);
   s.allocs.clear();
-  if (s.rbuf) (void)hipFree(s.rbuf);
+  dev_free(s.rbuf, s.rbuf_bytes);
   s.rbuf = nullptr;
   s.rbuf_bytes = 0;
-  if (s.gbuf) (void)hipFree(s.gbuf);
+  dev_free(s.gbuf, s.gbuf_bytes);
   s.gbuf = nullptr;
   s.gbuf_bytes = 0;
   for (int e = 0; e < State::NEV; e++)

@@ -8,6 +8,7 @@
 #include <utility>
 #include <vector>
 
+#include "devmem.h"
 #include "host_plan.h"
 #include "kernels.h"
 
@@ -92,7 +93,7 @@ struct State {
   hipEvent_t ev[NEV] = {};
   hipEvent_t ev_sweep[4] = {};  // around the two ring sweep launches of an iteration
   double tsum[8] = {};  // ms per kernel class over the last solve
-  std::vector<void*> allocs;
+  std::vector<std::pair<void*, size_t>> allocs;   // device blocks and their sizes
   size_t device_bytes = 0;   // bytes of the allocations in `allocs`
   // rendering
   float *tms = nullptr, *ss_coef = nullptr, *phasetab = nullptr, *bcrad_render = nullptr;
@@ -130,10 +131,6 @@ int state_render(State& s, int64_t nrays, const float* x, const float* y, const 
                  const double* mu, const double* phi, int single_scatter, int nosurface,
                  float* out, std::string& err);
 
-// Serialises device allocation, release and HIP-graph capture across host
-// threads: a capture in progress on one state's stream must not overlap
-// hipMalloc / hipFree (which can synchronise the device) on another thread.
-std::mutex& device_mutex();
 
 int state_create(const Plan& p, State& s, std::string& err);
 size_t plan_footprint_bytes(const Plan& p);

@@ -20,6 +20,12 @@ from .configuration import get_config
 from .medium import Scatterer
 
 
+def device_trim() -> int:
+    """Return the device blocks kept from destroyed solver states to the
+    runtime; gives the bytes they held."""
+    return int(_lib.lib().at3d_device_trim())
+
+
 class RTE:
     """Radiative transfer solver (at3d/solver.py:143)."""
 

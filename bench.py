@@ -89,11 +89,14 @@ def serving_stream_bench(n_requests, rate_hz, workers):
     requests one at a time.  Host setup (plan) is part of every request."""
     from at3d_amd import synthetic as S
     from at3d_amd.serving import SolveServer
+    from at3d_amd.solver import device_trim
     reqs = S.serving_stream(n_requests, rate_hz=rate_hz, seed=1)
+    device_trim()            # both runs start without kept device blocks
     res, st = SolveServer(nworkers=workers).run(reqs)
     served_iters = {r.key: (r.iterations, r.solcrit) for r in res}
     for r in reqs:
         r.arrival_s = 0.0
+    device_trim()
     res1, st1 = SolveServer(nworkers=1).run(reqs)
     same = all(served_iters[r.key] == (r.iterations, r.solcrit) for r in res1)
     return {"requests": st["requests"], "solves": st["solves"], "failed": st["failed"] + st1["failed"],

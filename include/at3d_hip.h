@@ -72,6 +72,11 @@ typedef struct at3d_state at3d_state; /* device-resident solver state       */
 /* Library version string. */
 const char* at3d_version(void);
 
+/* Device blocks of destroyed states are kept by size and reused by the next
+ * state of that shape; this returns them to the runtime and gives the bytes
+ * it held.  No counterpart in the reference (its arrays are host memory). */
+int64_t at3d_device_trim(void);
+
 /* Host setup of one solve: base grid and cells (NEW_GRIDS / INIT_CELL_STRUCTURE,
  * shdomsub2.f:16-309), property transfer with delta-M (TRANSFER_PA_TO_GRID,
  * shdomsub1.f:19-112), discrete ordinates and SH transform tables
