@@ -16,22 +16,6 @@
 
 namespace at3d {
 
-// Runs f(0..n-1) on up to 16 host threads (independent items only).
-template <typename F>
-static void parallel_for(int n, F&& f) {
-  const int nt = std::max(1, std::min(n, std::min(16, (int)std::thread::hardware_concurrency())));
-  if (nt <= 1) {
-    for (int i = 0; i < n; i++) f(i);
-    return;
-  }
-  std::vector<std::thread> th;
-  th.reserve(nt);
-  for (int t = 0; t < nt; t++)
-    th.emplace_back([&, t]() {
-      for (int i = t; i < n; i += nt) f(i);
-    });
-  for (std::thread& x : th) x.join();
-}
 namespace {
 
 inline bool btest(int v, int bit) { return (v >> bit) & 1; }

@@ -171,15 +171,13 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
   s.oct_pos_h.assign(8 * np, 0);
   std::vector<int4> tiles;
   s.nsectors = 0;
-  for (int o = 0; o < 8; o++) {
-    s.ring_nlev[o] = 0;
-    s.sec_base[o] = (int)sp0.size();
+  // per-octant slices of the device arrays, octants on their own host threads
+  parallel_for(8, [&](int o) {
     int32_t* pm = perm.data() + (size_t)o * s.nptsp;
     for (int q = 0; q < s.nptsp; q++) pm[q] = q;  // unused octants and padding rows
     for (size_t i = 0; i < np; i++) s.oct_pos_h[o * np + i] = (int32_t)i;
-    if (!used[o]) continue;
+    if (!used[o]) return;
     const RingOctant& ro = rp.oct[o];
-    s.ring_nlev[o] = ro.nlev;
     const std::vector<int32_t>& bc = ((o >> 2) & 1) ? p.bcbot : p.bctop;
     for (size_t q = 0; q < np; q++) {
       pm[q] = ro.code[q] >= 0 ? ro.code[q] : bc[-1 - ro.code[q]];
@@ -189,6 +187,13 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
       staged[o * np + q] = ro.staged[q];
       s.oct_pos_h[o * np + q] = ro.pos[q];
     }
+  });
+  for (int o = 0; o < 8; o++) {
+    s.ring_nlev[o] = 0;
+    s.sec_base[o] = (int)sp0.size();
+    if (!used[o]) continue;
+    const RingOctant& ro = rp.oct[o];
+    s.ring_nlev[o] = ro.nlev;
     for (int L = 0; L <= ro.nlev; L++) roff[(size_t)o * s.gld + L] = ro.off[L];
     for (int L = 0; L <= ro.nlev + 1; L++) soff[(size_t)o * (s.gld + 1) + L] = ro.sec_off[L];
     sp0.insert(sp0.end(), ro.sec_p0.begin(), ro.sec_p0.end());

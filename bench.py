@@ -122,7 +122,7 @@ def main():
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--stream-requests", type=int, default=24,
                     help="requests of the mixed-size serving stream (0: skip it)")
-    ap.add_argument("--stream-rate", type=float, default=8.0, help="its Poisson arrival rate (1/s)")
+    ap.add_argument("--stream-rate", type=float, default=12.0, help="its Poisson arrival rate (1/s)")
     ap.add_argument("--stream-workers", type=int, default=8)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy a buffer of this size once (the known-size kernel that "
