@@ -13,6 +13,8 @@
 #include <algorithm>
 #include <numeric>
 #include <thread>
+#include <mutex>
+#include <list>
 
 namespace at3d {
 
@@ -1237,21 +1239,71 @@ static int build_octant(const Plan& p, int o, const std::vector<char>& top, cons
   return AT3D_OK;
 }
 
-int build_levels(Plan& p, std::string& err) {
-  std::vector<char> top(p.npts, 0), bot(p.npts, 0);
-  for (int i : p.bctop) top[i] = 1;
-  for (int i : p.bcbot) bot[i] = 1;
-  std::vector<std::string> errs(8);
-  std::vector<int> rcs(8, AT3D_OK);
-  parallel_for(8, [&](int o) { rcs[o] = build_octant(p, o, top, bot, p.oct[o], errs[o]); });
-  p.nlevels_max = 0;
-  for (int o = 0; o < 8; o++) {
-    if (rcs[o]) {
-      err = errs[o];
-      return rcs[o];
+// The sweep order and levels depend only on the grid and the ordinate
+// directions, which a stream of solves on a few grids and angle sets repeats:
+// the last few are kept, keyed by every input they are built from.
+namespace {
+std::mutex level_cache_mutex;
+std::list<std::pair<std::vector<uint32_t>, std::shared_ptr<const std::array<OctantPlan, 8>>>> level_cache;
+constexpr size_t LEVEL_CACHE_ENTRIES = 16;
+
+std::vector<uint32_t> level_key(const Plan& p) {
+  std::vector<uint32_t> k = {(uint32_t)p.npts, (uint32_t)p.ncells, (uint32_t)p.nx1, (uint32_t)p.ny1,
+                             (uint32_t)p.nz, (uint32_t)p.bcflag, (uint32_t)p.ipflag, (uint32_t)p.nang};
+  auto bits = [&](const std::vector<float>& v) {
+    k.push_back((uint32_t)v.size());
+    for (float f : v) {
+      uint32_t u;
+      memcpy(&u, &f, 4);
+      k.push_back(u);
     }
-    p.nlevels_max = std::max(p.nlevels_max, p.oct[o].nlevels);
+  };
+  bits(p.xgrid);
+  bits(p.ygrid);
+  bits(p.zgrid);
+  for (const Ordinate& o : p.ords) {
+    uint32_t u[3];
+    memcpy(&u[0], &o.mu, 4);
+    memcpy(&u[1], &o.phi, 4);
+    u[2] = (uint32_t)o.g.ioct;
+    k.insert(k.end(), u, u + 3);
   }
+  return k;
+}
+}  // namespace
+
+int build_levels(Plan& p, std::string& err) {
+  const std::vector<uint32_t> key = level_key(p);
+  {
+    std::lock_guard<std::mutex> l(level_cache_mutex);
+    for (auto it = level_cache.begin(); it != level_cache.end(); ++it)
+      if (it->first == key) {
+        p.oct_store = it->second;
+        level_cache.splice(level_cache.begin(), level_cache, it);
+        break;
+      }
+  }
+  if (!p.oct_store) {
+    std::vector<char> top(p.npts, 0), bot(p.npts, 0);
+    for (int i : p.bctop) top[i] = 1;
+    for (int i : p.bcbot) bot[i] = 1;
+    auto octs = std::make_shared<std::array<OctantPlan, 8>>();
+    std::vector<std::string> errs(8);
+    std::vector<int> rcs(8, AT3D_OK);
+    parallel_for(8, [&](int o) { rcs[o] = build_octant(p, o, top, bot, (*octs)[o], errs[o]); });
+    for (int o = 0; o < 8; o++)
+      if (rcs[o]) {
+        err = errs[o];
+        return rcs[o];
+      }
+    p.oct_store = octs;
+    std::lock_guard<std::mutex> l(level_cache_mutex);
+    level_cache.emplace_front(key, p.oct_store);
+    if (level_cache.size() > LEVEL_CACHE_ENTRIES) level_cache.pop_back();
+  }
+  p.oct = p.oct_store->data();
+  p.nlevels_max = 0;
+  for (int o = 0; o < 8; o++) p.nlevels_max = std::max(p.nlevels_max, p.oct[o].nlevels);
   return AT3D_OK;
 }
 

@@ -4,6 +4,8 @@
 
 #include <string>
 #include <algorithm>
+#include <array>
+#include <memory>
 #include <thread>
 #include <vector>
 
@@ -85,7 +87,10 @@ struct Plan {
   // solution setup
   std::vector<float> dirflux, radiance0;          // [npts], [npts][4]
   std::vector<int32_t> bctop, bcbot;
-  OctantPlan oct[8];
+  // per-octant sweep order and levels: geometry only (grid and ordinate
+  // directions), shared by plans of the same geometry (see build_levels)
+  std::shared_ptr<const std::array<OctantPlan, 8>> oct_store;
+  const OctantPlan* oct = nullptr;   // oct_store's eight octants
   int nlevels_max = 0;
 };
 

@@ -195,3 +195,19 @@ def test_level_order_groups_points_by_source_class():
             work += -(-k // 16) * 16
             dense += nlm
     assert work / dense < 0.45
+
+
+def test_sweep_levels_are_shared_by_plans_of_one_geometry():
+    """The sweep order and levels depend only on the grid and the ordinate
+    directions: a second medium on the same grid and angle set gets the same
+    arrays (reused from the first plan), and another angle set gets its own."""
+    a = S.make_rte(S.serve_medium((24, 24, 16), 8, 16, 0.86, 0))
+    b = S.make_rte(S.serve_medium((24, 24, 16), 8, 16, 1.6, 7, noise=0.05))
+    c = S.make_rte(S.serve_medium((24, 24, 16), 12, 24, 0.86, 0))
+    for k in ("sweepord", "level"):
+        assert np.array_equal(a.plan_array(k), b.plan_array(k))
+    assert a.plan_size("nang") != c.plan_size("nang")
+    fresh = S.make_rte(S.serve_medium((24, 24, 16), 12, 24, 2.0, 3))
+    assert np.array_equal(c.plan_array("level"), fresh.plan_array("level"))
+    # optics differ, so the media do
+    assert not np.array_equal(a.plan_array("total_ext"), b.plan_array("total_ext"))
