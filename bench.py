@@ -120,9 +120,9 @@ def main():
                     help="reference iterations timed for cpu_baseline (0: the GPU solve's count)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workers", type=int, default=4)
-    ap.add_argument("--stream-requests", type=int, default=24,
+    ap.add_argument("--stream-requests", type=int, default=48,
                     help="requests of the mixed-size serving stream (0: skip it)")
-    ap.add_argument("--stream-rate", type=float, default=12.0, help="its Poisson arrival rate (1/s)")
+    ap.add_argument("--stream-rate", type=float, default=40.0, help="its Poisson arrival rate (1/s)")
     ap.add_argument("--stream-workers", type=int, default=8)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy a buffer of this size once (the known-size kernel that "
