@@ -70,9 +70,12 @@ struct SweepArgs {
   int sec_base[8];
   const int32_t* top_index;  // [npts] index in bctop or -1
   const int32_t* bot_index;  // [npts] index in bcbot or -1
-  const uint4* rec;          // step records (SweepRec) in two planes: first words
-                             // [ncols][npts], then second words [ncols][npts]
-  size_t rec_plane;          // ncols * npts: offset of the second plane
+  const uint4* rec;          // [ncols][npts] step records (SweepRec)
+  const float4* geo;         // [ncols][ncls] one-cell step of each class: u, v, path, face
+  const int2* geo_rep;       // [8][ncls] a (point, start cell) of each class per octant
+  const float* ext_pos;      // [8][ldp] extinction in each octant's ring order
+  int ncls;                  // step classes: nz * 4 (point level x the start cell's IP flags)
+  unsigned long long nz_magic;  // ceil(2^40 / nz): point level = pt - nz * ((pt * nz_magic) >> 40)
   const uint8_t* gflag;      // [ncols][gld] level L holds a multi-cell trace
   int* col_general;          // [ncols] nonzero: the column has multi-cell traces
   int gld;
@@ -82,27 +85,36 @@ struct SweepArgs {
   int* ngeneral;             // multi-cell traces found by the record pass
 };
 
-// The fixed part of one ring entry of one ordinate: word0.x = point | staged
-// << 29 | kind << 30 (kind 0: one-cell step, 1: multi-cell trace, 2:
-// boundary point).  A one-cell step's radiance is
-// max(0, b0 * srcext_face + b1 * srcext_pt) + T * rad_face, the face values
-// bilinear in (u, v) over four ring slots r[0..3]; a boundary entry holds its
-// boundary index in r[0..1].
+// The fixed part of one ring entry of one ordinate, 16 bytes: head = point |
+// class flags << 27 (the start cell's IP flags) | staged << 29 | kind << 30
+// (kind 0: one-cell step, 1: multi-cell trace, 2: boundary point, 3: a
+// zero-length step onto the corner in r[0], whose radiance it takes).  A
+// one-cell step's face values are bilinear in (u, v) over four ring slots
+// r[0..3]; u, v and the path length are those of its step class (the point's
+// level and the class flags: every point of a class steps through a cell of
+// the same shape), and ext0 is the extinction on the face.  A boundary entry
+// holds its boundary index in r[0..1] and its radiance's bits in r[2..3].
 struct SweepRec {
-  int32_t pt;
+  uint32_t head;
   uint16_t r[4];
-  float u, v, b0, b1, T;
+  float ext0;
 };
-static_assert(sizeof(SweepRec) == 32, "SweepRec is two 16-byte words");
+static_assert(sizeof(SweepRec) == 16, "SweepRec is one 16-byte word");
+constexpr int RECORD_PT_BITS = 27;
 constexpr int RING_THREADS = 512;
 constexpr int RING_K = 5;  // entries per thread per level with records in registers
 
-// Bytes of LDS k_sweep_ring needs: the ring of (radiance, source) pairs, gld
-// level offsets, gld + 1 sector offsets and gld level flags.
-inline size_t ring_lds_bytes(int slots, int gld) { return (size_t)slots * 8 + (size_t)gld * 9 + 4; }
+// Bytes of LDS k_sweep_ring needs: the step classes (16 bytes each), the
+// ring of (radiance, source) pairs, gld level offsets, gld + 1 sector offsets
+// and gld level flags.
+inline size_t ring_lds_bytes(int slots, int gld, int ncls) {
+  return (size_t)ncls * 16 + (size_t)slots * 8 + (size_t)gld * 9 + 4;
+}
 // Allows k_sweep_ring launches with lds_bytes of dynamic LDS; nonzero on failure.
 int sweep_ring_max_lds(size_t lds_bytes);
 void launch_sweep_records(hipStream_t st, const SweepArgs& a);
+// The step class table (geo) of every column, from each class's representative.
+void launch_sweep_geo(hipStream_t st, const SweepArgs& a, int ncols);
 // fast / full: launch the instantiation without / with the multi-cell,
 // overflow and extra-sector paths (a column runs in exactly one of them).
 void launch_sweep_ring(hipStream_t st, const SweepArgs& a, int col0, int ncol, size_t lds_bytes,

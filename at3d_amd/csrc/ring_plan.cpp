@@ -155,7 +155,7 @@ bool build_ring_plan(const Plan& p, const bool used[8], size_t lds_budget, RingP
   };
   r.sector_points = 8;
   r.slots = ring_slots(8);
-  if (ring_plan_lds_bytes(r.slots, r.nlev_max) > lds_budget) {
+  if (ring_plan_lds_bytes(r.slots, r.nlev_max, p.nz * 4) > lds_budget) {
     r.sector_points = 4;
     r.slots = ring_slots(4);
   }

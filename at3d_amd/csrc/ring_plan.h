@@ -38,9 +38,11 @@ struct RingPlan {
   int nlev_max = 0;
 };
 
-// Bytes of LDS the level-ordered sweep needs for a ring of `slots` entries
-// and `nlev` levels.
-inline size_t ring_plan_lds_bytes(int slots, int nlev) { return (size_t)slots * 8 + (size_t)(nlev + 2) * 9 + 4; }
+// Bytes of LDS the level-ordered sweep needs for a ring of `slots` entries,
+// `nlev` levels and `ncls` step classes.
+inline size_t ring_plan_lds_bytes(int slots, int nlev, int ncls) {
+  return (size_t)ncls * 16 + (size_t)slots * 8 + (size_t)(nlev + 2) * 9 + 4;
+}
 
 // Builds the ring layout for the octants with ordinates (used[o] != 0), with
 // 32-byte sectors if that ring fits in lds_budget bytes, else 16-byte ones.

@@ -48,6 +48,7 @@ struct TraceStep {
   double u, v;      // face interpolation factors
   int i[4];         // face corner points
   int next;         // next cell or -1 at a domain boundary
+  int face;         // exit face (0..5: -x, +x, -y, +y, -z, +z)
 };
 
 __host__ __device__ inline bool trace_step(const GridTopo& g, const OrdGeom& o, int icell,
@@ -111,7 +112,39 @@ __host__ __device__ inline bool trace_step(const GridTopo& g, const OrdGeom& o, 
   t.i[2] = i3;
   t.i[3] = i4;
   t.next = inext;
+  t.face = iface;
   return true;
+}
+
+// Interpolation factors of the point (xe, ye, ze) on face `face` of cell
+// `icell` (the formulas of trace_step for that face) and the face's corner
+// points.  Used to express a step that leaves the cell through a shared edge
+// (a tie between two faces) on the face its class uses.
+__host__ __device__ inline void face_uv(const GridTopo& g, int icell, int face, double xe, double ye,
+                                        double ze, double& u, double& v, int corners[4]) {
+#pragma clang fp contract(off)
+  const int8_t flags = g.cellflags[icell];
+  const bool ipinx = (flags & 1) != 0;
+  const bool ipiny = (flags & 2) != 0;
+  const int* gp = g.gridptr + icell * 8;
+  const float* P = g.gridpos;
+  const int i1 = gp[face_corner(face, 0)], i2 = gp[face_corner(face, 1)];
+  const int i3 = gp[face_corner(face, 2)], i4 = gp[face_corner(face, 3)];
+  const int jface = face / 2 + 1;
+  if (jface == 1) {
+    u = (ze - (double)P[i1 * 3 + 2]) / ((double)P[i3 * 3 + 2] - (double)P[i1 * 3 + 2]);
+    v = ipiny ? 0.5 : (ye - (double)P[i1 * 3 + 1]) / ((double)P[i2 * 3 + 1] - (double)P[i1 * 3 + 1]);
+  } else if (jface == 2) {
+    u = (ze - (double)P[i1 * 3 + 2]) / ((double)P[i3 * 3 + 2] - (double)P[i1 * 3 + 2]);
+    v = ipinx ? 0.5 : (xe - (double)P[i1 * 3 + 0]) / ((double)P[i2 * 3 + 0] - (double)P[i1 * 3 + 0]);
+  } else {
+    u = ipiny ? 0.5 : (ye - (double)P[i1 * 3 + 1]) / ((double)P[i3 * 3 + 1] - (double)P[i1 * 3 + 1]);
+    v = ipinx ? 0.5 : (xe - (double)P[i1 * 3 + 0]) / ((double)P[i2 * 3 + 0] - (double)P[i1 * 3 + 0]);
+  }
+  corners[0] = i1;
+  corners[1] = i2;
+  corners[2] = i3;
+  corners[3] = i4;
 }
 
 // Cell integral of the short characteristic (shdomsub1.f:3948-3987): linear

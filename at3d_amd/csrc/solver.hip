@@ -133,7 +133,11 @@ static SweepArgs sweep_args(const State& s) {
   a.top_index = s.top_index;
   a.bot_index = s.bot_index;
   a.rec = s.rec;
-  a.rec_plane = (size_t)s.plan->nang * s.plan->npts;
+  a.geo = s.geo;
+  a.geo_rep = s.geo_rep;
+  a.ext_pos = s.ext_pos;
+  a.ncls = s.ncls;
+  a.nz_magic = s.nz_magic;
   a.gflag = s.gflag;
   a.col_general = s.col_general;
   a.gld = s.gld;
@@ -164,7 +168,20 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
   s.slots = rp.slots;
   s.gld = rp.nlev_max + 2;
   const size_t np = p.npts;
+  s.ncls = p.nz * 4;
   std::vector<int32_t> perm((size_t)8 * s.nptsp, 0), code(8 * np, 0), cell(8 * np, -1), lev(8 * np, 0);
+  std::vector<float> extpos((size_t)8 * s.nptsp, 0.f);
+  // a representative (point, start cell) of each step class, per octant: the
+  // class's point nearest the domain's centre column (away from the seams
+  // and open edges, whose steps can be degenerate)
+  std::vector<int2> reps((size_t)8 * s.ncls, make_int2(-1, -1));
+  float xmid = 0.f, ymid = 0.f;
+  for (size_t i = 0; i < np; i++) {
+    xmid = std::max(xmid, p.gridpos[i * 3]);
+    ymid = std::max(ymid, p.gridpos[i * 3 + 1]);
+  }
+  xmid *= 0.5f;
+  ymid *= 0.5f;
   std::vector<int32_t> roff((size_t)8 * s.gld, 0), soff((size_t)8 * (s.gld + 1), 0), sp0;
   std::vector<int8_t> staged(8 * np, 0);
   std::vector<uint16_t> sslot;
@@ -179,13 +196,26 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
     if (!used[o]) return;
     const RingOctant& ro = rp.oct[o];
     const std::vector<int32_t>& bc = ((o >> 2) & 1) ? p.bcbot : p.bctop;
+    float* ex = extpos.data() + (size_t)o * s.nptsp;
+    int2* rp_o = reps.data() + (size_t)o * s.ncls;
+    std::vector<float> dbest(s.ncls, 3.0e38f);
     for (size_t q = 0; q < np; q++) {
       pm[q] = ro.code[q] >= 0 ? ro.code[q] : bc[-1 - ro.code[q]];
+      ex[q] = p.total_ext[pm[q]];
       code[o * np + q] = ro.code[q];
       cell[o * np + q] = ro.cell[q];
       lev[o * np + q] = ro.level[q];
       staged[o * np + q] = ro.staged[q];
       s.oct_pos_h[o * np + q] = ro.pos[q];
+      if (ro.code[q] >= 0 && ro.cell[q] >= 0) {
+        const int pt = ro.code[q];
+        const int cls = (pt % p.nz) * 4 + (p.cellflags[ro.cell[q]] & 3);
+        const float d = fabsf(p.gridpos[pt * 3] - xmid) + fabsf(p.gridpos[pt * 3 + 1] - ymid);
+        if (d < dbest[cls]) {
+          dbest[cls] = d;
+          rp_o[cls] = make_int2(pt, ro.cell[q]);
+        }
+      }
     }
   });
   for (int o = 0; o < 8; o++) {
@@ -212,11 +242,12 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
       if (ro.off[L] - ro.off[L - 1] > RING_K * RING_THREADS || ro.sec_off[L + 1] - ro.sec_off[L] > RING_THREADS)
         s.ring_full = 1;
   }
-  s.ring_lds = ring_lds_bytes(s.slots, s.gld);
-  if (s.slots > 65536 || p.npts >= (1 << 29) || s.ring_lds > (size_t)maxlds ||
+  s.ring_lds = ring_lds_bytes(s.slots, s.gld, s.ncls);
+  if (s.slots > 65536 || p.npts >= (1 << RECORD_PT_BITS) || p.nz >= 8192 || s.ring_lds > (size_t)maxlds ||
       sweep_ring_max_lds(s.ring_lds) != 0)
     return AT3D_OK;
-  const size_t nrec = (size_t)p.nang * np * 2;
+  s.nz_magic = ((1ull << 40) + (unsigned long long)p.nz - 1) / (unsigned long long)p.nz;
+  const size_t nrec = (size_t)p.nang * np;
   if (dev_alloc((void**)&s.rec, nrec * sizeof(uint4)) != hipSuccess) {
     (void)hipGetLastError();
     s.rec = nullptr;
@@ -240,10 +271,14 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
   TRY(upload(s, s.sec_p0, sp0.data(), sp0.size(), err));
   TRY(upload(s, s.sec_slot, reinterpret_cast<const uint4*>(sslot.data()), sslot.size() / 8, err));
   TRY(upload(s, s.sh_tiles, tiles.data(), tiles.size(), err));
+  TRY(upload(s, s.ext_pos, extpos.data(), extpos.size(), err));
+  TRY(upload(s, s.geo_rep, reps.data(), reps.size(), err));
+  TRY(alloc0(s, s.geo, (size_t)p.nang * s.ncls, err));
   s.n_sh_tiles = (int)tiles.size();
   TRY(alloc0(s, s.gflag, (size_t)p.nang * s.gld, err));
   TRY(alloc0(s, s.col_general, (size_t)p.nang, err));
   TRY(alloc0(s, s.ngeneral_d, (size_t)1, err));
+  launch_sweep_geo(s.stream, sweep_args(s), p.nang);
   launch_sweep_records(s.stream, sweep_args(s));
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(&s.ngeneral, s.ngeneral_d, sizeof(int), hipMemcpyDeviceToHost, s.stream));
@@ -272,7 +307,8 @@ size_t plan_footprint_bytes(const Plan& p) {
   b += npts * 4 * 12;                    // per-point scalars, counts, fluxes, partials
   b += npts * 12 + ncells * 57;          // grid positions, cell topology and flags
   b += 8 * npts * 4 * 4 + npts;          // sweep order: rank, level, level lists, preset
-  b += 2 * nang * npts * 16;             // sweep records (two 16-byte words)
+  b += nang * npts * 16;                 // sweep records
+  b += nang * (size_t)p.nz * 4 * 16 + 8 * nptsp * 4 + 8 * (size_t)p.nz * 4 * 8;  // step classes, ring-order extinction
   b += 8 * nptsp * 4 + 8 * npts * (4 * 4 + 1) + 8 * (npts / 4 + 8) * 20;  // ring layout, sectors
   b += p.ss_off.size() * 4 + p.ss_iph.size() * 4 + p.ss_coef.size() * 4;
   return b;

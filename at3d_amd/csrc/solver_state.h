@@ -71,6 +71,11 @@ struct State {
   uint4* sec_slot = nullptr;
   std::vector<int32_t> oct_pos_h;   // [8][npts]
   uint4* rec = nullptr;
+  float4* geo = nullptr;       // [nang][ncls] step classes
+  int2* geo_rep = nullptr;     // [8][ncls] class representatives
+  float* ext_pos = nullptr;    // [8][nptsp] extinction in ring order
+  int ncls = 0;
+  unsigned long long nz_magic = 0;
   uint8_t* gflag = nullptr;
   int* col_general = nullptr;
   std::vector<int32_t> col_general_h;   // columns with multi-cell traces

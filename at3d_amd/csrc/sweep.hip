@@ -230,19 +230,47 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_columns(SweepArgs a, in
 // Level-ordered sweep with an LDS ring
 // ---------------------------------------------------------------------------
 
+// Step classes.  Every point of one level whose first cell step starts in a
+// cell with the same IP flags steps, for a given ordinate, through a cell of
+// the same shape (the grid is a tensor product), so the step's face
+// interpolation factors (u, v) and path length are those of the class; the
+// table holds them for every (column, class), traced once from a
+// representative point of the class.
+__device__ inline int step_class(const SweepArgs& a, int pt, int8_t cellflags) {
+  const int iz = pt - a.nz * (int)(((unsigned long long)pt * a.nz_magic) >> 40);
+  return iz * 4 + (cellflags & 3);
+}
+
+__global__ __launch_bounds__(256) void k_sweep_geo(SweepArgs a, int ncols) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncols * a.ncls) return;
+  const int col = i / a.ncls, cls = i - col * a.ncls;
+  const int o = a.col_oct[col];
+  const int2 rep = a.geo_rep[(size_t)o * a.ncls + cls];
+  float4 g = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+  if (rep.x >= 0) {
+    const OrdGeom og = a.ordg[col];
+    double xe = a.g.gridpos[rep.x * 3], ye = a.g.gridpos[rep.x * 3 + 1], ze = a.g.gridpos[rep.x * 3 + 2];
+    TraceStep t;
+    if (trace_step(a.g, og, rep.y, xe, ye, ze, t))
+      g = make_float4((float)t.u, (float)t.v, (float)t.so, __int_as_float(t.face));
+  }
+  const_cast<float4*>(a.geo)[i] = g;
+}
+
 // Once per solve: the first cell step of every point's characteristic for every
-// ordinate (the loop body of BACK_INT_GRID3D_UNPOL, shdomsub1.f:3822-3920, with
-// the cell integral of :3880-3905), in the ordinate's ring order.  A step whose
-// face corners are known (boundary points or earlier points of the sweep order)
-// and lie within the last RING_WINDOW levels becomes a one-cell record, its
-// source integral written as coefficients of the face and point source values
-// (the algebra of cell_integral rearranged: both of its branches are linear in
-// SRCEXT0 and SRCEXT1 before the clamp).  Every other point is marked for the
-// general trace and its level flagged; boundary entries carry their index.
+// ordinate (the loop body of BACK_INT_GRID3D_UNPOL, shdomsub1.f:3822-3920), in
+// the ordinate's ring order.  A step whose face corners are known (boundary
+// points or earlier points of the sweep order), lie within the last
+// RING_WINDOW levels and whose face, u, v and path length match its class's
+// becomes a one-cell record: the four corners' ring slots and the extinction
+// on the face (the cell integral is evaluated in the sweep, from the class's
+// path length and the point's extinction).  Every other point is marked for
+// the general trace and its level flagged; boundary entries carry their index.
 //
 // A wavefront takes one entry of one octant for up to 64 of the octant's
 // ordinates (lane = ordinate), so the point's geometry, ranks and levels are
-// read once per wavefront and each lane writes one 32-byte record.
+// read once per wavefront and each lane writes one 16-byte record.
 __device__ inline void sweep_record(const SweepArgs& a, int o, int col, int jl);
 
 __global__ __launch_bounds__(256) void k_sweep_records(SweepArgs a) {
@@ -263,12 +291,12 @@ __device__ inline void sweep_record(const SweepArgs& a, int o, int col, int jl) 
   const int32_t* pos = a.oct_pos + ob;
   SweepRec r;
   r.r[0] = r.r[1] = r.r[2] = r.r[3] = 0;
-  r.u = r.v = r.b0 = r.b1 = r.T = 0.0f;
+  r.ext0 = 0.0f;
   if (code < 0) {
     // boundary entry: its radiance is the boundary condition
     const int b = -1 - code;
     const int pt = (og.bitz ? a.bcbot : a.bctop)[b];
-    r.pt = (int32_t)((uint32_t)pt | ((uint32_t)a.ring_staged[ob + pt] << 29) | (2u << 30));
+    r.head = (uint32_t)pt | ((uint32_t)a.ring_staged[ob + pt] << 29) | (2u << 30);
     r.r[0] = (uint16_t)(b & 0xffff);
     r.r[1] = (uint16_t)(b >> 16);
     // the radiance itself in r[2..3]: the sky radiance of a downwelling
@@ -280,69 +308,59 @@ __device__ inline void sweep_record(const SweepArgs& a, int o, int col, int jl) 
   } else {
     const int pt = code;
     const int L = rlev[pt];
+    const int cell = a.ring_cell[ob + jl];
+    const int8_t fl = a.g.cellflags[cell] & 3;
     double xe = a.g.gridpos[pt * 3], ye = a.g.gridpos[pt * 3 + 1], ze = a.g.gridpos[pt * 3 + 2];
     TraceStep t;
-    bool ok = trace_step(a.g, og, a.ring_cell[ob + jl], xe, ye, ze, t);
+    bool ok = trace_step(a.g, og, cell, xe, ye, ze, t);
     for (int k = 0; k < 4 && ok; k++) {
       const int q = t.i[k];
       const int lq = rlev[q];
       ok = ((a.preset[q] & presetbit) || rank[q] < rank[pt]) && lq < L && lq >= L - RING_WINDOW;
       r.r[k] = (uint16_t)(pos[q] % a.slots);
     }
-    const int staged = a.ring_staged[ob + pt];
-    if (ok) {
+    const float4 g = a.geo[(size_t)col * a.ncls + step_class(a, pt, fl)];
+    const int gface = __float_as_int(g.w);
+    int cn[4] = {t.i[0], t.i[1], t.i[2], t.i[3]};
+    double u = t.u, v = t.v;
+    if (ok && t.so > 0.0 && gface >= 0 && gface != t.face) {
+      // a step through an edge shared with the class's face: on that face
+      face_uv(a.g, cell, gface, xe, ye, ze, u, v, cn);
+      for (int k = 0; k < 4 && ok; k++) {
+        const int q = cn[k];
+        const int lq = rlev[q];
+        ok = ((a.preset[q] & presetbit) || rank[q] < rank[pt]) && lq < L && lq >= L - RING_WINDOW;
+        r.r[k] = (uint16_t)(pos[q] % a.slots);
+      }
+    }
+    // the step is its class's when u, v and the path length agree (the float
+    // grid positions make them differ by about 1e-6 within a class)
+    const bool cls_ok = gface >= 0 && fabs(u - (double)g.x) <= 2e-5 && fabs(v - (double)g.y) <= 2e-5 &&
+                        fabs(t.so - (double)g.z) <= 2e-5 * fabs(t.so);
+    // a zero-length step ending on a corner (seams of periodic grids) takes
+    // that corner's radiance
+    int corner = -1;
+    if (ok && t.so == 0.0 && (t.u == 0.0 || t.u == 1.0) && (t.v == 0.0 || t.v == 1.0))
+      corner = (t.u == 1.0 ? 2 : 0) + (t.v == 1.0 ? 1 : 0);
+    const uint32_t staged = (uint32_t)a.ring_staged[ob + pt];
+    if (ok && corner >= 0) {
+      r.r[0] = r.r[corner];
+      r.head = (uint32_t)pt | ((uint32_t)fl << RECORD_PT_BITS) | (staged << 29) | (3u << 30);
+    } else if (ok && cls_ok) {
       const float* ext = a.ext;
-      const double f1 = (1 - t.u) * (1 - t.v), f2 = (1 - t.u) * t.v;
-      const double f3 = t.u * (1 - t.v), f4 = t.u * t.v;
-      const double ext0 = f1 * ext[t.i[0]] + f2 * ext[t.i[1]] + f3 * ext[t.i[2]] + f4 * ext[t.i[3]];
-      const double ext1 = ext[pt], so = t.so;
-      const double e = 0.5 * (ext0 + ext1);
-      const double tau = e * so;
-      double absc, trc;
-      if (tau >= 0.5) {
-        trc = exp(-tau);
-        absc = 1.0 - trc;
-      } else {
-        absc = tau * (1.0 - 0.5 * tau * (1.0 - (double)0.33333333333f * tau * (1 - 0.25 * tau)));
-        trc = 1.0 - absc;
-      }
-      double c0, c1;  // src = c0 * SRCEXT0 + c1 * SRCEXT1 before the clamp at zero
-      if (tau <= 2.0) {
-        if (e == 0.0) {
-          c0 = c1 = 0.0;
-        } else {
-          const double k = (double)0.08333333333f * so;
-          c0 = (0.5 - k * ext1) / e;
-          c1 = (0.5 + k * ext0) / e;
-        }
-      } else {
-        double e0p = ext0, sc = 1.0;
-        if (tau > 4.0) {
-          e0p = ext1 + (ext0 - ext1) * 4.0 / tau;
-          if (ext0 > 0.0) sc = e0p / ext0;
-        }
-        const double d = e0p + ext1;
-        const double g = 2.0 / d * (1 - 2 / tau + 2 * trc / absc);
-        c0 = sc * (1.0 - ext1 * g) / d;
-        c1 = (1.0 + e0p * g) / d;
-      }
-      r.pt = pt | (staged << 29);
-      r.u = (float)t.u;
-      r.v = (float)t.v;
-      r.b0 = (float)(c0 * absc);
-      r.b1 = (float)(c1 * absc);
-      r.T = (float)trc;
+      const double uc = g.x, vc = g.y;
+      const double f1 = (1 - uc) * (1 - vc), f2 = (1 - uc) * vc;
+      const double f3 = uc * (1 - vc), f4 = uc * vc;
+      r.ext0 = (float)(f1 * ext[cn[0]] + f2 * ext[cn[1]] + f3 * ext[cn[2]] + f4 * ext[cn[3]]);
+      r.head = (uint32_t)pt | ((uint32_t)fl << RECORD_PT_BITS) | (staged << 29);
     } else {
-      r.pt = (int32_t)((uint32_t)pt | ((uint32_t)staged << 29) | (1u << 30));
+      r.head = (uint32_t)pt | ((uint32_t)fl << RECORD_PT_BITS) | (staged << 29) | (1u << 30);
       atomicAdd(a.ngeneral, 1);
       atomicOr(a.col_general + col, 1);
       const_cast<uint8_t*>(a.gflag)[(size_t)col * a.gld + L] = 1;
     }
   }
-  const uint4* w = reinterpret_cast<const uint4*>(&r);
-  uint4* dst = const_cast<uint4*>(a.rec) + (size_t)col * a.npts + jl;
-  dst[0] = w[0];
-  dst[a.rec_plane] = w[1];
+  const_cast<uint4*>(a.rec)[(size_t)col * a.npts + jl] = *reinterpret_cast<const uint4*>(&r);
 }
 
 // Barrier that orders LDS traffic only: outstanding global loads (the next
@@ -354,9 +372,52 @@ __device__ inline void lds_barrier() {
 }
 
 
+// The cell integral of a one-cell step (shdomsub1.f:3948-3987, with the
+// reference's tau branches) as coefficients of the face and point sources:
+// src x abs = b0 SRCEXT0 + b1 SRCEXT1 before the clamp at zero, and the
+// cell's transmission, in single precision from the face and point
+// extinctions and the path length.
+__device__ inline void cell_coefs(float ext0, float ext1, float so, float& b0, float& b1, float& trc) {
+  const float e = 0.5f * (ext0 + ext1);
+  const float tau = e * so;
+  float absc;
+  if (tau >= 0.5f) {
+    trc = __expf(-tau);
+    absc = 1.0f - trc;
+  } else {
+    absc = tau * (1.0f - 0.5f * tau * (1.0f - 0.33333333333f * tau * (1.0f - 0.25f * tau)));
+    trc = 1.0f - absc;
+  }
+  float c0, c1;
+  if (tau <= 2.0f) {
+    if (e == 0.0f) {
+      c0 = c1 = 0.0f;
+    } else {
+      const float k = 0.08333333333f * so, re = __builtin_amdgcn_rcpf(e);
+      c0 = (0.5f - k * ext1) * re;
+      c1 = (0.5f + k * ext0) * re;
+    }
+  } else {
+    float e0p = ext0, sc = 1.0f;
+    if (tau > 4.0f) {
+      e0p = ext1 + (ext0 - ext1) * 4.0f / tau;
+      if (ext0 > 0.0f) sc = e0p / ext0;
+    }
+    const float d = e0p + ext1;
+    const float g = 2.0f / d * (1.0f - 2.0f / tau + 2.0f * trc / absc);
+    c0 = sc * (1.0f - ext1 * g) / d;
+    c1 = (1.0f + e0p * g) / d;
+  }
+  b0 = c0 * absc;
+  b1 = c1 * absc;
+}
+
+constexpr uint32_t RECORD_PT_MASK = (1u << RECORD_PT_BITS) - 1;
+
 struct RingSet {
-  uint4 w0[RING_K], w1[RING_K];
-  float s[RING_K];
+  uint4 w0[RING_K];
+  float s[RING_K];   // the entry's source x extinction
+  float e[RING_K];   // and its extinction
   uint4 sec;     // ring slots of the sector this thread writes after the level
   int sec_p0;    // its first point
   bool sec_ok;   // there is such a sector (known from indices, not from the loads)
@@ -379,7 +440,7 @@ struct RingSet {
 // column belongs to the other instantiation returns at once.
 template <bool FULL>
 __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int col0) {
-  extern __shared__ float2 ring[];
+  extern __shared__ float4 lds4[];
   const int col = col0 + blockIdx.x;
   if (FULL != (a.ring_full || a.col_general[col] != 0)) return;
   const int o = a.col_oct[col];
@@ -387,16 +448,16 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
   const OrdGeom og = a.ordg[col];
   const int up = og.bitz;
   const int C = a.slots;
+  float4* geo = lds4;                                       // [ncls] step classes
+  float2* ring = reinterpret_cast<float2*>(lds4 + a.ncls);
   int32_t* off = reinterpret_cast<int32_t*>(ring + C);
   int32_t* soffl = off + a.gld;                     // [gld + 1] sectors by completion level
   uint8_t* full = reinterpret_cast<uint8_t*>(soffl + a.gld + 1);
   const int32_t* goff = a.ring_off + (size_t)o * a.gld;
   const float* se = a.srcext + (size_t)col * a.ldp;  // ring order
   float* rad = a.rad + (size_t)col * a.ldp;          // point order
-  // the two words of the records in separate planes: the two loads of a
-  // record slot touch disjoint cache lines
   const uint4* rec0 = a.rec + (size_t)col * a.npts;
-  const uint4* rec1 = rec0 + a.rec_plane;
+  const float* ep = a.ext_pos + (size_t)o * a.ldp;  // ring order
   const uint8_t* gf = a.gflag + (size_t)col * a.gld;
   const int32_t* pos = a.oct_pos + (size_t)o * a.npts;
   const int32_t* soff = a.sec_off + (size_t)o * (a.gld + 1);
@@ -414,7 +475,23 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
   for (int L = tid; L <= nlev; L += RING_THREADS) off[L] = goff[L];
   for (int L = tid; L <= nlev + 1; L += RING_THREADS) soffl[L] = soff[L];
   for (int L = tid; L < a.gld; L += RING_THREADS) full[L] = gf[L];
+  for (int i = tid; i < a.ncls; i += RING_THREADS) geo[i] = a.geo[(size_t)col * a.ncls + i];
   const int nsw = goff[nlev];
+  // a one-cell step's radiance from its face corners, its class and the
+  // point's source and extinction
+  auto one_cell = [&](const uint4& w, const float2& q0, const float2& q1, const float2& q2,
+                      const float2& q3, const float4& g, float s1, float e1) {
+    const float u = g.x, v = g.y;
+    const float f1 = (1.f - u) * (1.f - v), f2 = (1.f - u) * v, f3 = u * (1.f - v), f4 = u * v;
+    const float s0 = f1 * q0.y + f2 * q1.y + f3 * q2.y + f4 * q3.y;
+    const float r0 = f1 * q0.x + f2 * q1.x + f3 * q2.x + f4 * q3.x;
+    float b0, b1, trc;
+    cell_coefs(__uint_as_float(w.w), e1, g.z, b0, b1, trc);
+    return fmaxf(0.f, b0 * s0 + b1 * s1) + trc * r0;
+  };
+  auto class_of = [&](uint32_t head) {
+    return step_class(a, (int)(head & RECORD_PT_MASK), (int8_t)((head >> RECORD_PT_BITS) & 3));
+  };
   auto put = [&](int jl, int k0, float val, float s1) {
     int i = k0 + jl;
     if (i >= C) i -= C;
@@ -422,23 +499,21 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
   };
   // One-cell record or boundary entry (the unprefetched path): LDS gathers and
   // the cell integral.  Returns false for a multi-cell trace.
-  auto finish = [&](const uint4& w0, const uint4& w1, float s1, int jl, int k0) {
+  auto finish = [&](const uint4& w0, float s1, float e1, int jl, int k0) {
     const uint32_t head = w0.x;
     const int kind = head >> 30;
     if (kind == 1) return false;
     float val;
     if (kind == 2) {
       val = __uint_as_float(w0.z);
+    } else if (kind == 3) {
+      val = ring[w0.y & 0xffffu].x;
+      if (!(staging && ((head >> 29) & 1))) rad[head & RECORD_PT_MASK] = val;
     } else {
       const float2 q0 = ring[w0.y & 0xffffu], q1 = ring[w0.y >> 16];
       const float2 q2 = ring[w0.z & 0xffffu], q3 = ring[w0.z >> 16];
-      const float u = __uint_as_float(w0.w), v = __uint_as_float(w1.x);
-      const float f1 = (1.f - u) * (1.f - v), f2 = (1.f - u) * v, f3 = u * (1.f - v), f4 = u * v;
-      const float s0 = f1 * q0.y + f2 * q1.y + f3 * q2.y + f4 * q3.y;
-      const float r0 = f1 * q0.x + f2 * q1.x + f3 * q2.x + f4 * q3.x;
-      val = fmaxf(0.f, __uint_as_float(w1.y) * s0 + __uint_as_float(w1.z) * s1) +
-            __uint_as_float(w1.w) * r0;
-      if (!(staging && ((head >> 29) & 1))) rad[head & 0x1fffffff] = val;
+      val = one_cell(w0, q0, q1, q2, q3, geo[class_of(head)], s1, e1);
+      if (!(staging && ((head >> 29) & 1))) rad[head & RECORD_PT_MASK] = val;
     }
     put(jl, k0, val, s1);
     return true;
@@ -458,8 +533,8 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
       if (beg + wfirst + k * RING_THREADS < end) {
         const int jl = min(beg + tid + k * RING_THREADS, last);
         S.w0[k] = rec0[jl];
-        S.w1[k] = rec1[jl];
         S.s[k] = se[jl];
+        S.e[k] = ep[jl];
       }
     }
     S.sec_ok = false;
@@ -507,21 +582,23 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
 #pragma unroll
     for (int k = 0; k < RING_K; k++)
       asm volatile("" ::"v"(cur.w0[k].x), "v"(cur.w0[k].y), "v"(cur.w0[k].z), "v"(cur.w0[k].w),
-                   "v"(cur.w1[k].x), "v"(cur.w1[k].y), "v"(cur.w1[k].z), "v"(cur.w1[k].w), "v"(cur.s[k]));
+                   "v"(cur.s[k]), "v"(cur.e[k]));
     asm volatile("" ::"v"(cur.sec.x), "v"(cur.sec.y), "v"(cur.sec.z), "v"(cur.sec.w), "v"(cur.sec_p0));
     const int beg = __builtin_amdgcn_readfirstlane(off[L - 1]);
     const int end = __builtin_amdgcn_readfirstlane(off[L]);
     const int k0 = beg % C - beg;  // slot of position jl: k0 + jl, wrapped once
     float2 g[RING_K][4];
+    float4 gc[RING_K];   // the steps' classes
 #pragma unroll
     for (int k = 0; k < RING_K; k++) {
       const int jl = beg + tid + k * RING_THREADS;
       const uint4 w0 = cur.w0[k];
-      if (jl < end && (w0.x >> 30) == 0) {
+      if (jl < end && (w0.x >> 30) != 1 && (w0.x >> 30) != 2) {
         g[k][0] = ring[w0.y & 0xffffu];
         g[k][1] = ring[w0.y >> 16];
         g[k][2] = ring[w0.z & 0xffffu];
         g[k][3] = ring[w0.z >> 16];
+        gc[k] = geo[class_of(w0.x)];
       }
     }
     const bool dosec = staging && L > 1 && cur.sec_ok;
@@ -545,7 +622,7 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
     for (int k = 0; k < RING_K; k++) {
       const int jl = beg + tid + k * RING_THREADS;
       if (jl >= end) continue;
-      const uint4 w0 = cur.w0[k], w1 = cur.w1[k];
+      const uint4 w0 = cur.w0[k];
       const uint32_t head = w0.x;
       const int kind = head >> 30;
       if (kind == 1) {
@@ -556,13 +633,10 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
       if (kind == 2) {
         val = __uint_as_float(w0.z);
       } else {
-        const float u = __uint_as_float(w0.w), v = __uint_as_float(w1.x);
-        const float f1 = (1.f - u) * (1.f - v), f2 = (1.f - u) * v, f3 = u * (1.f - v), f4 = u * v;
-        const float s0 = f1 * g[k][0].y + f2 * g[k][1].y + f3 * g[k][2].y + f4 * g[k][3].y;
-        const float r0 = f1 * g[k][0].x + f2 * g[k][1].x + f3 * g[k][2].x + f4 * g[k][3].x;
-        val = fmaxf(0.f, __uint_as_float(w1.y) * s0 + __uint_as_float(w1.z) * cur.s[k]) +
-              __uint_as_float(w1.w) * r0;
-        if (!(staging && ((head >> 29) & 1))) rad[head & 0x1fffffff] = val;
+        // kind 3: a zero-length step onto a corner, whose radiance it takes
+        val = kind == 3 ? g[k][0].x
+                        : one_cell(w0, g[k][0], g[k][1], g[k][2], g[k][3], gc[k], cur.s[k], cur.e[k]);
+        if (!(staging && ((head >> 29) & 1))) rad[head & RECORD_PT_MASK] = val;
       }
       put(jl, k0, val, cur.s[k]);
     }
@@ -572,7 +646,7 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
     }
     if (FULL)
       for (int jl = beg + tid + RING_K * RING_THREADS; jl < end; jl += RING_THREADS)
-        (void)finish(rec0[jl], rec1[jl], se[jl], jl, k0);
+        (void)finish(rec0[jl], se[jl], ep[jl], jl, k0);
     if (FULL && __builtin_amdgcn_readfirstlane(full[L])) {
       // multi-cell traces read radiances of earlier levels from the row,
       // complete and visible after the full barrier that ended level L-1
@@ -581,7 +655,7 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
         const uint32_t head = rec0[jl].x;
         const bool gn = k < RING_K ? ((gen >> k) & 1) != 0 : (head >> 30) == 1;
         if (!gn) continue;
-        const int p = head & 0x1fffffff;
+        const int p = head & RECORD_PT_MASK;
         const float s1 = se[jl];
         const float val = sweep_general(a, og, col, o, p, a.ring_cell[(size_t)o * a.npts + jl], L, se,
                                         pos);
@@ -687,6 +761,11 @@ void launch_sweep_columns(hipStream_t st, const SweepArgs& a, int col0, int ncol
 
 void launch_sweep_records(hipStream_t st, const SweepArgs& a) {
   k_sweep_records<<<dim3((a.npts + 3) / 4, 8), 256, 0, st>>>(a);
+}
+
+void launch_sweep_geo(hipStream_t st, const SweepArgs& a, int ncols) {
+  const int n = ncols * a.ncls;
+  if (n > 0) k_sweep_geo<<<(n + 255) / 256, 256, 0, st>>>(a, ncols);
 }
 
 void launch_sweep_ring(hipStream_t st, const SweepArgs& a, int col0, int ncol, size_t lds_bytes,

@@ -162,8 +162,8 @@ def test_ring_layout_stages_most_of_the_benchmark_grid():
     rte, lev, pos, staged, slots, spts, nlev = _ring_case(S.workload("cloud64"))
     assert spts == 8            # 32-byte sectors: the nine-level ring fits
     assert staged.mean() > 0.9
-    # the ring fits in the 160 KiB of LDS of one workgroup
-    assert slots * 8 + (nlev.max() + 2) * 9 + 4 <= 160 * 1024
+    # the step classes and the ring fit in the 160 KiB of LDS of one workgroup
+    assert rte.plan_size("nz") * 4 * 16 + slots * 8 + (nlev.max() + 2) * 9 + 4 <= 160 * 1024
 
 
 def test_level_order_groups_points_by_source_class():
