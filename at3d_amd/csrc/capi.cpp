@@ -467,7 +467,7 @@ double at3d_state_timing(const at3d_state* st, const char* key) {
   if (k == "sweep_kernel") return s.tsum[6];
   if (k == "device_bytes") return (double)s.device_bytes;
   if (k == "sweep_ring") return (s.ring_ok && s.use_ring && !s.force_general) ? 1.0 : 0.0;
-  if (k == "sweep_ring_lds") return (double)s.ring_lds;
+  if (k == "sweep_ring_lds") return (double)(s.ring_full ? s.ring_lds_full : s.ring_lds);
   if (k == "sweep_ring_slots") return s.slots;
   if (k == "sweep_sectors") return s.nsectors;
   if (k == "sweep_general_points") return s.ngeneral;

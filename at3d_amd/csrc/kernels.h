@@ -76,6 +76,7 @@ struct SweepArgs {
   const float* ext_pos;      // [8][ldp] extinction in each octant's ring order
   int ncls;                  // step classes: nz * 4 (point level x the start cell's IP flags)
   unsigned long long nz_magic;  // ceil(2^40 / nz): point level = pt - nz * ((pt * nz_magic) >> 40)
+  float cls_tol;             // u, v and relative path-length tolerance of a step against its class
   const uint8_t* gflag;      // [ncols][gld] level L holds a multi-cell trace
   int* col_general;          // [ncols] nonzero: the column has multi-cell traces
   int gld;
@@ -103,21 +104,24 @@ static_assert(sizeof(SweepRec) == 16, "SweepRec is one 16-byte word");
 constexpr int RECORD_PT_BITS = 27;
 constexpr int RING_THREADS = 512;
 constexpr int RING_K = 5;  // entries per thread per level with records in registers
+constexpr int RING_KX = 4; // entries per thread loaded together past those (full instantiation)
 
-// Bytes of LDS k_sweep_ring needs: the step classes (16 bytes each), the
-// ring of (radiance, source) pairs, gld level offsets, gld + 1 sector offsets
-// and gld level flags.
+// Bytes of LDS k_sweep_ring needs: the step classes (16 bytes each; ncls = 0
+// for the full instantiation, which reads them from device memory), the ring
+// of (radiance, source) pairs, gld level offsets, gld + 1 sector offsets and
+// gld level flags.
 inline size_t ring_lds_bytes(int slots, int gld, int ncls) {
   return (size_t)ncls * 16 + (size_t)slots * 8 + (size_t)gld * 9 + 4;
 }
-// Allows k_sweep_ring launches with lds_bytes of dynamic LDS; nonzero on failure.
-int sweep_ring_max_lds(size_t lds_bytes);
+// Allows k_sweep_ring launches with lds_fast / lds_full bytes of dynamic LDS
+// (lds_fast 0: the fast instantiation is not used); nonzero on failure.
+int sweep_ring_max_lds(size_t lds_fast, size_t lds_full);
 void launch_sweep_records(hipStream_t st, const SweepArgs& a);
 // The step class table (geo) of every column, from each class's representative.
 void launch_sweep_geo(hipStream_t st, const SweepArgs& a, int ncols);
 // fast / full: launch the instantiation without / with the multi-cell,
 // overflow and extra-sector paths (a column runs in exactly one of them).
-void launch_sweep_ring(hipStream_t st, const SweepArgs& a, int col0, int ncol, size_t lds_bytes,
+void launch_sweep_ring(hipStream_t st, const SweepArgs& a, int col0, int ncol, size_t lds_fast, size_t lds_full,
                        bool fast, bool full);
 
 void launch_sweep_preset(hipStream_t st, const SweepArgs& a, bool up, int col0, int ncol);

@@ -23,6 +23,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <string>
 #include <vector>
 
@@ -138,6 +139,7 @@ static SweepArgs sweep_args(const State& s) {
   a.ext_pos = s.ext_pos;
   a.ncls = s.ncls;
   a.nz_magic = s.nz_magic;
+  a.cls_tol = s.cls_tol;
   a.gflag = s.gflag;
   a.col_general = s.col_general;
   a.gld = s.gld;
@@ -242,11 +244,32 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
       if (ro.off[L] - ro.off[L - 1] > RING_K * RING_THREADS || ro.sec_off[L + 1] - ro.sec_off[L] > RING_THREADS)
         s.ring_full = 1;
   }
+  // the fast instantiation keeps the step classes in LDS beside the ring;
+  // where they do not fit, every column takes the full one
   s.ring_lds = ring_lds_bytes(s.slots, s.gld, s.ncls);
-  if (s.slots > 65536 || p.npts >= (1 << RECORD_PT_BITS) || p.nz >= 8192 || s.ring_lds > (size_t)maxlds ||
-      sweep_ring_max_lds(s.ring_lds) != 0)
+  s.ring_lds_full = ring_lds_bytes(s.slots, s.gld, 0);
+  if (s.ring_lds > (size_t)maxlds) s.ring_full = 1;
+  if (s.ring_full) s.ring_lds = 0;
+  if (s.slots > 65536 || p.npts >= (1 << RECORD_PT_BITS) || p.nz >= 8192 ||
+      s.ring_lds_full > (size_t)maxlds || sweep_ring_max_lds(s.ring_lds, s.ring_lds_full) != 0)
     return AT3D_OK;
   s.nz_magic = ((1ull << 40) + (unsigned long long)p.nz - 1) / (unsigned long long)p.nz;
+  // Steps of one class differ only by the rounding of the float grid
+  // positions: a few units in the last place of the largest coordinate over
+  // the smallest grid spacing (at least 2e-5)
+  {
+    float amax = 0.f, smin = 3.0e38f;
+    for (size_t i = 0; i < (size_t)np * 3; i++) amax = std::max(amax, fabsf(p.gridpos[i]));
+    auto spacing = [&](const std::vector<float>& gr) {
+      for (size_t i = 1; i < gr.size(); i++)
+        if (gr[i] > gr[i - 1]) smin = std::min(smin, gr[i] - gr[i - 1]);
+    };
+    spacing(p.xgrid);
+    spacing(p.ygrid);
+    spacing(p.zgrid);
+    s.cls_tol = 2e-5f;
+    if (smin < 3.0e38f) s.cls_tol = std::max(s.cls_tol, 8.f * FLT_EPSILON * amax / smin);
+  }
   const size_t nrec = (size_t)p.nang * np;
   if (dev_alloc((void**)&s.rec, nrec * sizeof(uint4)) != hipSuccess) {
     (void)hipGetLastError();
@@ -581,7 +604,7 @@ static void enqueue_sweeps(State& s) {
     // the first launch is timed from the event the caller records just
     // before this function, the second to the one just after it
     needs(0, s.ndown, fast, full);
-    launch_sweep_ring(s.stream, a, 0, s.ndown, s.ring_lds, fast, full);
+    launch_sweep_ring(s.stream, a, 0, s.ndown, s.ring_lds, s.ring_lds_full, fast, full);
     (void)hipEventRecord(s.ev_sweep[1], s.stream);
   } else {
     launch_sweep_preset(s.stream, a, false, 0, s.ndown);
@@ -593,7 +616,7 @@ static void enqueue_sweeps(State& s) {
   if (ring) {
     needs(s.ndown, p.nang, fast, full);
     (void)hipEventRecord(s.ev_sweep[2], s.stream);
-    launch_sweep_ring(s.stream, a, s.ndown, p.nang - s.ndown, s.ring_lds, fast, full);
+    launch_sweep_ring(s.stream, a, s.ndown, p.nang - s.ndown, s.ring_lds, s.ring_lds_full, fast, full);
   } else {
     launch_sweep_preset(s.stream, a, true, s.ndown, p.nang - s.ndown);
     launch_sweep_columns(s.stream, a, s.ndown, p.nang - s.ndown);

@@ -63,7 +63,8 @@ struct State {
   int use_ring = 1;            // option: sweep with k_sweep_ring when ring_ok
   int slots = 0, sector_points = 8, gld = 0, ngeneral = 0, nsectors = 0;
   int ring_nlev[8] = {}, sec_base[8] = {};
-  size_t ring_lds = 0;         // dynamic LDS bytes of k_sweep_ring
+  size_t ring_lds = 0;         // dynamic LDS bytes of k_sweep_ring's fast instantiation (0: unused)
+  size_t ring_lds_full = 0;    // and of its full instantiation
   int32_t *oct_perm = nullptr, *oct_pos = nullptr, *top_index = nullptr;
   int32_t *ring_code = nullptr, *ring_cell = nullptr, *ring_level = nullptr, *ring_off = nullptr;
   int8_t* ring_staged = nullptr;
@@ -76,6 +77,7 @@ struct State {
   float* ext_pos = nullptr;    // [8][nptsp] extinction in ring order
   int ncls = 0;
   unsigned long long nz_magic = 0;
+  float cls_tol = 2e-5f;
   uint8_t* gflag = nullptr;
   int* col_general = nullptr;
   std::vector<int32_t> col_general_h;   // columns with multi-cell traces

@@ -333,10 +333,11 @@ __device__ inline void sweep_record(const SweepArgs& a, int o, int col, int jl) 
         r.r[k] = (uint16_t)(pos[q] % a.slots);
       }
     }
-    // the step is its class's when u, v and the path length agree (the float
-    // grid positions make them differ by about 1e-6 within a class)
-    const bool cls_ok = gface >= 0 && fabs(u - (double)g.x) <= 2e-5 && fabs(v - (double)g.y) <= 2e-5 &&
-                        fabs(t.so - (double)g.z) <= 2e-5 * fabs(t.so);
+    // the step is its class's when u, v and the path length agree to within
+    // the rounding of the float grid positions
+    const double tol = a.cls_tol;
+    const bool cls_ok = gface >= 0 && fabs(u - (double)g.x) <= tol && fabs(v - (double)g.y) <= tol &&
+                        fabs(t.so - (double)g.z) <= tol * fabs(t.so);
     // a zero-length step ending on a corner (seams of periodic grids) takes
     // that corner's radiance
     int corner = -1;
@@ -448,8 +449,12 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
   const OrdGeom og = a.ordg[col];
   const int up = og.bitz;
   const int C = a.slots;
-  float4* geo = lds4;                                       // [ncls] step classes
-  float2* ring = reinterpret_cast<float2*>(lds4 + a.ncls);
+  // the step classes: in LDS for the fast instantiation, read from device
+  // memory (a few KB per column, cache-resident) by the full one, whose rings
+  // may need all of the LDS
+  float4* geo_lds = lds4;                                   // [ncls] (fast only)
+  const float4* geo = FULL ? a.geo + (size_t)col * a.ncls : geo_lds;
+  float2* ring = reinterpret_cast<float2*>(lds4 + (FULL ? 0 : a.ncls));
   int32_t* off = reinterpret_cast<int32_t*>(ring + C);
   int32_t* soffl = off + a.gld;                     // [gld + 1] sectors by completion level
   uint8_t* full = reinterpret_cast<uint8_t*>(soffl + a.gld + 1);
@@ -475,7 +480,8 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
   for (int L = tid; L <= nlev; L += RING_THREADS) off[L] = goff[L];
   for (int L = tid; L <= nlev + 1; L += RING_THREADS) soffl[L] = soff[L];
   for (int L = tid; L < a.gld; L += RING_THREADS) full[L] = gf[L];
-  for (int i = tid; i < a.ncls; i += RING_THREADS) geo[i] = a.geo[(size_t)col * a.ncls + i];
+  if (!FULL)
+    for (int i = tid; i < a.ncls; i += RING_THREADS) geo_lds[i] = a.geo[(size_t)col * a.ncls + i];
   const int nsw = goff[nlev];
   // a one-cell step's radiance from its face corners, its class and the
   // point's source and extinction
@@ -497,26 +503,52 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
     if (i >= C) i -= C;
     ring[i] = make_float2(val, s1);
   };
-  // One-cell record or boundary entry (the unprefetched path): LDS gathers and
-  // the cell integral.  Returns false for a multi-cell trace.
-  auto finish = [&](const uint4& w0, float s1, float e1, int jl, int k0) {
-    const uint32_t head = w0.x;
-    const int kind = head >> 30;
-    if (kind == 1) return false;
-    float val;
-    if (kind == 2) {
-      val = __uint_as_float(w0.z);
-    } else if (kind == 3) {
-      val = ring[w0.y & 0xffffu].x;
-      if (!(staging && ((head >> 29) & 1))) rad[head & RECORD_PT_MASK] = val;
-    } else {
-      const float2 q0 = ring[w0.y & 0xffffu], q1 = ring[w0.y >> 16];
-      const float2 q2 = ring[w0.z & 0xffffu], q3 = ring[w0.z >> 16];
-      val = one_cell(w0, q0, q1, q2, q3, geo[class_of(head)], s1, e1);
-      if (!(staging && ((head >> 29) & 1))) rad[head & RECORD_PT_MASK] = val;
+  // Entries past the register slots (levels longer than RING_K * RING_THREADS,
+  // full instantiation only), RING_KX per thread at a time: their records,
+  // sources and extinctions are all loaded, then all LDS gathers issued,
+  // then the cell integrals, so one memory latency covers RING_KX entries.
+  // Multi-cell records (kind 1) are left to the caller's trace pass.
+  auto overflow = [&](int j0, int end, int k0) {
+    uint4 w[RING_KX];
+    float sx[RING_KX], ex[RING_KX];
+#pragma unroll
+    for (int u = 0; u < RING_KX; u++) {
+      const int jl = j0 + u * RING_THREADS;
+      w[u] = make_uint4(1u << 30, 0u, 0u, 0u);
+      sx[u] = ex[u] = 0.f;
+      if (jl < end) {
+        w[u] = rec0[jl];
+        sx[u] = se[jl];
+        ex[u] = ep[jl];
+      }
     }
-    put(jl, k0, val, s1);
-    return true;
+    float2 q[RING_KX][4];
+    float4 gq[RING_KX];
+#pragma unroll
+    for (int u = 0; u < RING_KX; u++) {
+      const int kind = w[u].x >> 30;
+      if (kind == 0 || kind == 3) {
+        q[u][0] = ring[w[u].y & 0xffffu];
+        q[u][1] = ring[w[u].y >> 16];
+        q[u][2] = ring[w[u].z & 0xffffu];
+        q[u][3] = ring[w[u].z >> 16];
+      }
+      if (kind == 0) gq[u] = geo[class_of(w[u].x)];
+    }
+#pragma unroll
+    for (int u = 0; u < RING_KX; u++) {
+      const uint32_t head = w[u].x;
+      const int kind = head >> 30;
+      if (kind == 1) continue;
+      float val;
+      if (kind == 2) {
+        val = __uint_as_float(w[u].z);
+      } else {
+        val = kind == 3 ? q[u][0].x : one_cell(w[u], q[u][0], q[u][1], q[u][2], q[u][3], gq[u], sx[u], ex[u]);
+        if (!(staging && ((head >> 29) & 1))) rad[head & RECORD_PT_MASK] = val;
+      }
+      put(j0 + u * RING_THREADS, k0, val, sx[u]);
+    }
   };
   // Records of a level: register slot k of a wave is loaded only when the
   // wave's first entry of that slot lies in the level (a wave-uniform
@@ -645,8 +677,8 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
       if (wide) *reinterpret_cast<float4*>(rad + cur.sec_p0 + 4) = sw;
     }
     if (FULL)
-      for (int jl = beg + tid + RING_K * RING_THREADS; jl < end; jl += RING_THREADS)
-        (void)finish(rec0[jl], se[jl], ep[jl], jl, k0);
+      for (int j0 = beg + tid + RING_K * RING_THREADS; j0 < end; j0 += RING_KX * RING_THREADS)
+        overflow(j0, end, k0);
     if (FULL && __builtin_amdgcn_readfirstlane(full[L])) {
       // multi-cell traces read radiances of earlier levels from the row,
       // complete and visible after the full barrier that ended level L-1
@@ -768,19 +800,20 @@ void launch_sweep_geo(hipStream_t st, const SweepArgs& a, int ncols) {
   if (n > 0) k_sweep_geo<<<(n + 255) / 256, 256, 0, st>>>(a, ncols);
 }
 
-void launch_sweep_ring(hipStream_t st, const SweepArgs& a, int col0, int ncol, size_t lds_bytes,
-                       bool fast, bool full) {
+void launch_sweep_ring(hipStream_t st, const SweepArgs& a, int col0, int ncol, size_t lds_fast,
+                       size_t lds_full, bool fast, bool full) {
   if (ncol <= 0) return;
-  if (fast) k_sweep_ring<false><<<ncol, RING_THREADS, lds_bytes, st>>>(a, col0);
-  if (full) k_sweep_ring<true><<<ncol, RING_THREADS, lds_bytes, st>>>(a, col0);
+  if (fast) k_sweep_ring<false><<<ncol, RING_THREADS, lds_fast, st>>>(a, col0);
+  if (full) k_sweep_ring<true><<<ncol, RING_THREADS, lds_full, st>>>(a, col0);
 }
 
-int sweep_ring_max_lds(size_t lds_bytes) {
+int sweep_ring_max_lds(size_t lds_fast, size_t lds_full) {
   const bool ok =
-      hipFuncSetAttribute(reinterpret_cast<const void*>(k_sweep_ring<false>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) == hipSuccess &&
+      (lds_fast == 0 ||
+       hipFuncSetAttribute(reinterpret_cast<const void*>(k_sweep_ring<false>),
+                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_fast) == hipSuccess) &&
       hipFuncSetAttribute(reinterpret_cast<const void*>(k_sweep_ring<true>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) == hipSuccess;
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_full) == hipSuccess;
   return ok ? 0 : 1;
 }
 

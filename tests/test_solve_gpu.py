@@ -120,8 +120,17 @@ def _mid_cloud():
     return S.cloud_field(nx=48, ny=48, nz=32, dx=0.02, dz=0.02, z0=0.3, seed=3, nmu=8, nphi=16)
 
 
-@pytest.mark.parametrize("make", [lambda: blob8("periodic"), lambda: blob8("open"), cloud_small, _mid_cloud],
-                         ids=["blob8_periodic", "blob8_open", "cloud12_two_species", "cloud48x48x32"])
+def _large_cloud():
+    # 96 x 96 x 48 open grid: the ring needs 4-point sectors and leaves no
+    # room in LDS for the step classes, which every column then reads from
+    # device memory
+    return S.cloud_field(nx=96, ny=96, nz=48, dx=0.02, dz=0.02, z0=0.3, seed=5, nmu=8, nphi=16)
+
+
+@pytest.mark.parametrize("make", [lambda: blob8("periodic"), lambda: blob8("open"), cloud_small, _mid_cloud,
+                                  _large_cloud],
+                         ids=["blob8_periodic", "blob8_open", "cloud12_two_species", "cloud48x48x32",
+                              "cloud96x96x48"])
 def test_level_ordered_sweep_matches_point_ordered_sweep(make):
     """The LDS-ring sweep over per-solve records (FP32 coefficients, level-ordered
     DO source, radiances written by whole 32-byte sectors where the layout
@@ -129,11 +138,15 @@ def test_level_ordered_sweep_matches_point_ordered_sweep(make):
     iterations the DO radiances and SH sources agree to rel L2 <= 2e-6 and the
     DO source read back by point agrees likewise.  The periodic blob has
     multi-cell traces (general path, no sector staging), the 12x12x10 cloud
-    staged and unstaged sectors, the 48x48x32 cloud only staged ones."""
+    staged and unstaged sectors, the 48x48x32 cloud only staged ones, the
+    96x96x48 cloud its step classes from device memory."""
     w = make()
     a = S.make_rte(w)
     a.solve(3, verbose=False)
     assert a.timing("sweep_ring") == 1.0
+    if a.plan_size("npts") > 400000:
+        nz = a.plan_size("nz")
+        assert a.timing("sweep_ring_lds") + 16 * 4 * nz > 160 * 1024
     b = S.make_rte(w)
     b.set_option("sweep_ring", 0)
     b.solve(3, verbose=False)
