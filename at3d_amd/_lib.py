@@ -60,6 +60,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIBPATH):
             raise ImportError(f"{LIBPATH} is not built: run `python -m at3d_amd.build`")
+        # torch (where installed) first: the HIP runtime it bundles sees no
+        # device once this library's runtime has initialised the GPU in the
+        # same process, while this library works in either order
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIBPATH)
         vp = C.c_void_p
         L.at3d_version.restype = C.c_char_p

@@ -2,8 +2,10 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/at3d_hip.h"
 #include "host_plan.h"
@@ -13,6 +15,9 @@
 
 struct at3d_plan {
   at3d::Plan p;
+  // the host direct flux, made on first request (solves make their own)
+  mutable std::mutex m;
+  mutable std::vector<float> dirflux;
 };
 struct at3d_state {
   at3d::State s;
@@ -138,7 +143,18 @@ int at3d_plan_get(const at3d_plan* h, const char* name, void* dst, size_t nbytes
   if (k == "total_ext") return copy_out(p.total_ext, dst, nbytes, err, errlen);
   if (k == "extinct") return copy_out(p.extinct, dst, nbytes, err, errlen);
   if (k == "albedo") return copy_out(p.albedo, dst, nbytes, err, errlen);
-  if (k == "dirflux") return copy_out(p.dirflux, dst, nbytes, err, errlen);
+  if (k == "dirflux") {
+    std::lock_guard<std::mutex> lock(h->m);
+    if (h->dirflux.empty() && p.npts > 0) {
+      std::string msg;
+      const int rc = at3d::plan_direct_host(p, h->dirflux, msg);
+      if (rc) {
+        set_err(err, errlen, msg);
+        return rc;
+      }
+    }
+    return copy_out(h->dirflux, dst, nbytes, err, errlen);
+  }
   if (k == "radiance0") return copy_out(p.radiance0, dst, nbytes, err, errlen);
   if (k == "chi") return copy_out(p.chi, dst, nbytes, err, errlen);
   if (k == "alb") return copy_out(p.alb, dst, nbytes, err, errlen);
@@ -213,8 +229,10 @@ int at3d_state_create(const at3d_plan* plan, at3d_state** out, char* err, size_t
   }
   *out = nullptr;
   int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
-    set_err(err, errlen, "no HIP device available");
+  const hipError_t dc = hipGetDeviceCount(&ndev);
+  if (dc != hipSuccess || ndev < 1) {
+    set_err(err, errlen, std::string("no HIP device available (hipGetDeviceCount: ") + hipGetErrorString(dc) +
+                             ", " + std::to_string(ndev) + " devices)");
     return AT3D_ERR_DEVICE;
   }
   at3d_state* h = new (std::nothrow) at3d_state;

@@ -617,17 +617,13 @@ void source_constants(Plan& p) {
 // Direct solar beam through the property grid: DIRECT_BEAM_PROP
 // (shdom90.f90:352-867) called by MAKE_DIRECT (shdomsub2.f:393-473).
 // ---------------------------------------------------------------------------
-struct DirectBeam {
-  std::vector<float> extdirp;
-  double cx, cy, cz, cxinv, cyinv, czinv, epss, epsz, xdomain, ydomain, delxd, delyd;
-  int di, dj, dk, ipdirect;
-};
+}  // namespace
 
-void direct_beam_init(const Plan& p, DirectBeam& b) {
+void direct_beam_setup(const Plan& p, std::vector<float>& extdirp, DirectBeamGeom& b) {
   const at3d_scene_desc& d = p.d;
   const int npx = d.npx, npy = d.npy, npz = d.npz, npg = npx * npy * npz;
   const int ML = p.ml, npart = p.npart;
-  b.extdirp.assign(npg, 0.f);
+  extdirp.assign(npg, 0.f);
   for (int ip = 0; ip < npg; ip++) {
     float acc = 0.f;
     for (int ipa = 0; ipa < npart; ipa++) {
@@ -643,7 +639,7 @@ void direct_beam_init(const Plan& p, DirectBeam& b) {
       }
       acc = (float)((double)acc + ext);
     }
-    b.extdirp[ip] = acc;
+    extdirp[ip] = acc;
   }
   b.ipdirect = p.ipflag;
   if (btest(p.ipflag, 2)) b.ipdirect = 0;
@@ -670,158 +666,33 @@ void direct_beam_init(const Plan& p, DirectBeam& b) {
   if (btest(p.bcflag, 2)) b.xdomain = b.delxd * (npx - 1);
   b.ydomain = b.delyd * npy;
   if (btest(p.bcflag, 3)) b.ydomain = b.delyd * (npy - 1);
+  b.zl = p.zlevels.data();
+  b.ext = extdirp.data();
+  b.npx = npx;
+  b.npy = npy;
+  b.npz = npz;
+  b.bc = p.bcflag;
+  b.solarflux = (double)d.solarflux;
 }
 
-// Returns false when the reference would STOP.  With `path` the traversal also
-// records, per property-grid cell crossed, the derivative of the slant optical
-// path with respect to each of the 8 corner extinctions: the integral of that
-// corner's trilinear weight along the segment (DIRECT_BEAM_AND_PATHS_PROP,
-// shdomsub5.f:1608-2003).
-bool direct_beam_point(const Plan& p, const DirectBeam& b, float xi, float yi, float zi,
-                       float& dirflux, std::vector<std::pair<int32_t, float>>* path_rec = nullptr) {
-  const at3d_scene_desc& d = p.d;
-  const int npx = d.npx, npy = d.npy, npz = d.npz, bc = p.bcflag;
-  const float* ZL = p.zlevels.data();
-  double z = zi, x = (double)(xi - 0.0f), y = (double)(yi - 0.0f);
-  int il = 0, iu = npz;
-  while (iu - il > 1) {
-    int im = (iu + il) / 2;
-    if (z >= (double)ZL[im - 1]) il = im; else iu = im;
-  }
-  int k = std::max(il, 1);
-  int i = (int)(x / b.delxd) + 1;
-  if (i > npx && fabs(x - b.xdomain) < 0.001 * b.delxd) i = npx;
-  if (i < 1 || i > npx) return false;
-  int j = (int)(y / b.delyd) + 1;
-  if (j > npy && fabs(y - b.ydomain) < 0.001 * b.delyd) j = npy;
-  if (j < 1 || j > npy) return false;
-  double xe = x, ye = y, ze = z, xp = xe, yp = ye, zp = ze;
-  bool constx = btest(b.ipdirect, 0), consty = btest(b.ipdirect, 1);
-  if (b.cx == 0.0) constx = true;
-  if (b.cy == 0.0) consty = true;
-  if (btest(bc, 0) && (fabs(x) < (double)0.01f * b.delxd || fabs(x - (npx - 1) * b.delxd) < (double)0.01f * b.delxd))
-    constx = true;
-  if (btest(bc, 1) && (fabs(y) < (double)0.01f * b.delyd || fabs(y - (npy - 1) * b.delyd) < (double)0.01f * b.delyd))
-    consty = true;
+namespace {
+
+// Direct flux at one point (false when the reference would STOP).  With
+// `path` the traversal also records, per property-grid cell crossed, the
+// derivative of the slant optical path with respect to each of the 8 corner
+// extinctions (DIRECT_BEAM_AND_PATHS_PROP, shdomsub5.f:1608-2003).
+bool direct_beam_point(const DirectBeamGeom& b, float xi, float yi, float zi, float& dirflux,
+                       std::vector<std::pair<int32_t, float>>* path_rec = nullptr) {
   double path = 0.0;
-  const float* E = b.extdirp.data();
-  int guard = 0;
-  while (fabs(ze - (double)ZL[npz - 1]) > b.epsz) {
-    if (++guard > 100000000) return false;
-    int ip1 = i + 1;
-    if (i == npx) ip1 = (btest(bc, 0) || btest(bc, 2)) ? npx : 1;
-    int jp1 = j + 1;
-    if (j == npy) jp1 = (btest(bc, 1) || btest(bc, 3)) ? npy : 1;
-    const double x0 = b.delxd * (i - 1), x1 = x0 + b.delxd;
-    const double y0 = b.delyd * (j - 1), y1 = y0 + b.delyd;
-    const double z0 = ZL[k - 1], z1 = ZL[k];
-    if (i < 1 || i > npx || j < 1 || j > npy || k < 1 || k >= npz) return false;
-    const int I1 = k + npz * (j - 1) + npz * npy * (i - 1) - 1;
-    const int I2 = k + npz * (j - 1) + npz * npy * (ip1 - 1) - 1;
-    const int I3 = k + npz * (jp1 - 1) + npz * npy * (i - 1) - 1;
-    const int I4 = k + npz * (jp1 - 1) + npz * npy * (ip1 - 1) - 1;
-    const double E1 = E[I1], E2 = E[I2], E3 = E[I3], E4 = E[I4];
-    const double E5 = E[I1 + 1], E6 = E[I2 + 1], E7 = E[I3 + 1], E8 = E[I4 + 1];
-    double sox, soy, soz;
-    if (constx) sox = 1.0e30;
-    else if (b.cx > 0.0) { sox = (x1 - xe) * b.cxinv; xp = x1; }
-    else { sox = (x0 - xe) * b.cxinv; xp = x0; }
-    if (consty) soy = 1.0e30;
-    else if (b.cy > 0.0) { soy = (y1 - ye) * b.cyinv; yp = y1; }
-    else { soy = (y0 - ye) * b.cyinv; yp = y0; }
-    if (b.cz > 0.0) { soz = (z1 - ze) * b.czinv; zp = z1; }
-    else if (b.cz < 0.0) { soz = (z0 - ze) * b.czinv; zp = z0; }
-    else soz = 1.0e30;
-    double xoffs = 0.0, yoffs = 0.0, so;
-    if (soz <= sox && soz <= soy) {
-      so = soz;
-      if (!constx) xp = xe + so * b.cx;
-      if (!consty) yp = ye + so * b.cy;
-      k += b.dk;
-    } else if (sox <= soy) {
-      so = sox;
-      if (!consty) yp = ye + so * b.cy;
-      zp = ze + so * b.cz;
-      i += b.di;
-      if (i == 0) {
-        if (btest(bc, 0)) { i = 1; constx = true; }
-        else if (btest(bc, 2)) return false;
-        else { i = npx; xoffs = b.xdomain; }
-      } else if (i >= npx && btest(bc, 2)) {
-        return false;
-      } else if (i == npx + 1) {
-        if (btest(bc, 0)) { i = npx; constx = true; }
-        else { i = 1; xoffs = -b.xdomain; }
-      }
-    } else {
-      so = soy;
-      if (!constx) xp = xe + so * b.cx;
-      zp = ze + so * b.cz;
-      j += b.dj;
-      if (j == 0) {
-        if (btest(bc, 1)) { j = 1; consty = true; }
-        else if (btest(bc, 3)) return false;
-        else { j = npy; yoffs = b.ydomain; }
-      } else if (j >= npy && btest(bc, 3)) {
-        return false;
-      } else if (j == npy + 1) {
-        if (btest(bc, 1)) { j = npy; consty = true; }
-        else { j = 1; yoffs = -b.ydomain; }
-      }
-    }
-    if (so < -b.epss) return false;
-    so = std::max(so, 0.0);
-    const double ax = 1.0 / (x1 - x0), ay = 1.0 / (y1 - y0), az = 1.0 / (z1 - z0);
-    const double u0 = (xe - x0) * ax, v0 = (ye - y0) * ay, w0 = (ze - z0) * az;
-    const double u1 = (xp - x0) * ax, v1 = (yp - y0) * ay, w1 = (zp - z0) * az;
-    const double u0m = 1.0 - u0, v0m = 1.0 - v0, w0m = 1.0 - w0;
-    const double du = u1 - u0, dv = v1 - v0, dw = w1 - w0;
-    double uv = u0 * v0, umv = u0m * v0, uvm = u0 * v0m, umvm = u0m * v0m;
-    double uw = u0 * w0, umw = u0m * w0, uwm = u0 * w0m, umwm = u0m * w0m;
-    double vw = v0 * w0, vmw = v0m * w0, vwm = v0 * w0m, vmwm = v0m * w0m;
-    const double vw0 = vw;
-    const double A = (E1 * u0m + E2 * u0) * vmwm + (E3 * u0m + E4 * u0) * vwm +
-                     (E5 * u0m + E6 * u0) * vmw + (E7 * u0m + E8 * u0) * vw;
-    const double B1 = -du * vmwm - dv * umwm - dw * umvm;
-    const double B2 = du * vmwm - dv * uwm - dw * uvm;
-    const double B3 = -du * vwm + dv * umwm - dw * umv;
-    const double B4 = du * vwm + dv * uwm - dw * uv;
-    const double B5 = -du * vmw - dv * umw + dw * umvm;
-    const double B6 = du * vmw - dv * uw + dw * uvm;
-    const double B7 = -du * vw + dv * umw + dw * umv;
-    const double B8 = du * vw + dv * uw + dw * uv;
-    const double B = B1 * E1 + B2 * E2 + B3 * E3 + B4 * E4 + B5 * E5 + B6 * E6 + B7 * E7 + B8 * E8;
-    vw = dv * dw;
-    const double vwu = vw * u0, vwum = vw * u0m;
-    uw = du * dw;
-    const double uwv = uw * v0, uwvm = uw * v0m;
-    uv = du * dv;
-    const double uvw = uv * w0, uvwm = uv * w0m;
-    const double C1 = +vwum + uwvm + uvwm, C2 = +vwu - uwvm - uvwm, C3 = -vwum + uwv - uvwm;
-    const double C4 = -vwu - uwv + uvwm, C5 = -vwum - uwvm + uvw, C6 = -vwu + uwvm - uvw;
-    const double C7 = +vwum - uwv - uvw, C8 = +vwu + uwv + uvw;
-    const double C = C1 * E1 + C2 * E2 + C3 * E3 + C4 * E4 + C5 * E5 + C6 * E6 + C7 * E7 + C8 * E8;
-    const double D = du * dv * dw * (E2 + E3 + E5 + E8 - E1 - E4 - E6 - E7);
-    path = path + so * (A + 0.5 * B + 0.3333333333333333 * C + 0.25 * D);
-    if (path_rec) {
-      // integral over t in [0,1] of the corner weight (u0 + t du)(v0 + t dv)(w0 + t dw)
-      // (or its complements), times the segment length
-      const double g = du * dv * dw;
-      const double wa[8] = {u0m * vmwm, u0 * vmwm, u0m * vwm, u0 * vwm,
-                            u0m * vmw,  u0 * vmw,  u0m * vw0, u0 * vw0};
-      const double wb[8] = {B1, B2, B3, B4, B5, B6, B7, B8};
-      const double wc[8] = {C1, C2, C3, C4, C5, C6, C7, C8};
-      const double sg[8] = {-1, 1, 1, -1, 1, -1, -1, 1};
-      const int32_t ii[8] = {I1, I2, I3, I4, I1 + 1, I2 + 1, I3 + 1, I4 + 1};
-      for (int c = 0; c < 8; c++)
-        path_rec->emplace_back(ii[c], (float)(so * (wa[c] + 0.5 * wb[c] + 0.3333333333333333 * wc[c] +
-                                                    0.25 * sg[c] * g)));
-    }
-    xe = xp + xoffs;
-    ye = yp + yoffs;
-    ze = zp;
-  }
-  dirflux = (float)((double)d.solarflux * exp(-path));
+  bool ok;
+  if (path_rec)
+    ok = direct_beam_trace(b, xi, yi, zi, path, [&](const int* ii, const double* seg) {
+      for (int c = 0; c < 8; c++) path_rec->emplace_back(ii[c], (float)seg[c]);
+    });
+  else
+    ok = direct_beam_trace(b, xi, yi, zi, path, NoBeamRecord{});
+  if (!ok) return false;
+  dirflux = (float)(b.solarflux * exp(-path));
   return true;
 }
 
@@ -1373,7 +1244,7 @@ int render_constants(Plan& p, std::string& err) {
         mixed_legen(p, ipa, ip, legt);
         const float f = legt[ML + 1];
         for (float& v : legt) v = v / (1 - f);
-        const float da = p.albedo[(size_t)ipa * npts + ip] * p.dirflux[ip] * secmu0 * W;
+        const float da = p.albedo[(size_t)ipa * npts + ip] * secmu0 * W;   // per unit direct flux
         for (int l = 0; l <= ML; l++) p.tms[(size_t)ip * (ML + 1) + l] += da * legt[l];
         const size_t base = ((size_t)ipa * npts + ip) * 8;
         if (p.phaseinterpwt[base] >= 0.999f) {
@@ -1422,10 +1293,33 @@ void plan_mixed_legen(const Plan& p, int ipa, int ip, std::vector<float>& out) {
   mixed_legen(p, ipa, ip, out);
 }
 
+int plan_direct_host(const Plan& p, std::vector<float>& dirflux, std::string& err) {
+  std::vector<float> extdirp;
+  DirectBeamGeom beam;
+  direct_beam_setup(p, extdirp, beam);
+  std::vector<float> df(p.npts, 0.f);
+  const int nchunk = 64;
+  std::vector<char> bad(nchunk, 0);
+  parallel_for(nchunk, [&](int c) {
+    const int i0 = (int)((int64_t)p.npts * c / nchunk), i1 = (int)((int64_t)p.npts * (c + 1) / nchunk);
+    for (int ip = i0; ip < i1; ip++)
+      if (!direct_beam_point(beam, p.gridpos[ip * 3], p.gridpos[ip * 3 + 1], p.gridpos[ip * 3 + 2], df[ip]))
+        bad[c] = 1;
+  });
+  for (char b : bad)
+    if (b) {
+      err = "DIRECT_BEAM_PROP: ray left the property grid";
+      return AT3D_ERR_NUMERIC;
+    }
+  dirflux.swap(df);
+  return AT3D_OK;
+}
+
 int plan_sun_paths(const Plan& p, std::vector<int32_t>& off, std::vector<int32_t>& ptr,
                    std::vector<float>& val, std::string& err) {
-  DirectBeam beam;
-  direct_beam_init(p, beam);
+  std::vector<float> extdirp;
+  DirectBeamGeom beam;
+  direct_beam_setup(p, extdirp, beam);
   off.assign(p.npts + 1, 0);
   ptr.clear();
   val.clear();
@@ -1434,8 +1328,7 @@ int plan_sun_paths(const Plan& p, std::vector<int32_t>& off, std::vector<int32_t
     off[ip] = (int32_t)ptr.size();
     rec.clear();
     float df;
-    if (!direct_beam_point(p, beam, p.gridpos[ip * 3], p.gridpos[ip * 3 + 1], p.gridpos[ip * 3 + 2], df,
-                           &rec)) {
+    if (!direct_beam_point(beam, p.gridpos[ip * 3], p.gridpos[ip * 3 + 1], p.gridpos[ip * 3 + 2], df, &rec)) {
       err = "DIRECT_BEAM_AND_PATHS_PROP: ray left the property grid";
       return AT3D_ERR_NUMERIC;
     }
@@ -1529,26 +1422,9 @@ int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err) {
   make_sh_tables(p);
   interp_optics(p);
   source_constants(p);
-  // MAKE_DIRECT
-  DirectBeam beam;
-  direct_beam_init(p, beam);
-  p.dirflux.assign(p.npts, 0.f);
-  {
-    const int nchunk = 64;
-    std::vector<char> bad(nchunk, 0);
-    parallel_for(nchunk, [&](int c) {
-      const int i0 = (int)((int64_t)p.npts * c / nchunk), i1 = (int)((int64_t)p.npts * (c + 1) / nchunk);
-      for (int ip = i0; ip < i1; ip++)
-        if (!direct_beam_point(p, beam, p.gridpos[ip * 3], p.gridpos[ip * 3 + 1], p.gridpos[ip * 3 + 2],
-                               p.dirflux[ip]))
-          bad[c] = 1;
-    });
-    for (char b : bad)
-      if (b) {
-        err = "DIRECT_BEAM_PROP: ray left the property grid";
-        return AT3D_ERR_NUMERIC;
-      }
-  }
+  // MAKE_DIRECT: the direct flux is made per solve on the device (or on the
+  // host by plan_direct_host when asked for); the plan keeps the constants
+  // that scale with it per unit direct flux
   if (!init_radiance(p, err)) return AT3D_ERR_NUMERIC;
   // BOUNDARY_PNTS (shdomsub1.f:2173-2215)
   p.bctop.clear();

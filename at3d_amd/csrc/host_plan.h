@@ -9,6 +9,8 @@
 #include <thread>
 #include <vector>
 
+#include "direct_beam.h"
+
 #include "../../include/at3d_hip.h"
 #include "shdom_geom.h"
 
@@ -81,11 +83,11 @@ struct Plan {
   // rendering constants
   std::vector<float> phasetab;                    // [numphase][nscatangle]
   std::string phasetab_err;                       // negativity failure, raised by render
-  std::vector<float> tms;                         // [npts][ml+1] sum_species DA * chi'_l
+  std::vector<float> tms;                         // [npts][ml+1] sum_species DA * chi'_l per unit direct flux
   std::vector<int32_t> ss_off, ss_iph;            // CSR per point of (phase index, coefficient)
   std::vector<float> ss_coef;
   // solution setup
-  std::vector<float> dirflux, radiance0;          // [npts], [npts][4]
+  std::vector<float> radiance0;                   // [npts][4]
   std::vector<int32_t> bctop, bcbot;
   // per-octant sweep order and levels: geometry only (grid and ordinate
   // directions), shared by plans of the same geometry (see build_levels)
@@ -111,6 +113,12 @@ void plan_source_class(const Plan& p, std::vector<int32_t>& cls);
 // Per RTE point, the property-grid points on its path to the sun and the
 // derivative of the slant optical path with respect to their (delta-M scaled)
 // extinction, as CSR (off[npts+1], 0-based ptr, val).
+// The direct beam's trace geometry and delta-M scaled extinction
+// (DIRECT_BEAM_PROP's set-up); b.zl points into p, b.ext into extdirp.
+void direct_beam_setup(const Plan& p, std::vector<float>& extdirp, DirectBeamGeom& b);
+// The direct flux of every grid point on the host threads (a solve makes
+// its own on the device; the plan holds none).
+int plan_direct_host(const Plan& p, std::vector<float>& dirflux, std::string& err);
 int plan_sun_paths(const Plan& p, std::vector<int32_t>& off, std::vector<int32_t>& ptr,
                    std::vector<float>& val, std::string& err);
 

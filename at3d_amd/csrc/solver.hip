@@ -29,6 +29,7 @@
 
 #include "devmem.h"
 #include "host_plan.h"
+#include "direct_beam.h"
 #include "kernels.h"
 #include "ring_plan.h"
 #include "shdom_geom.h"
@@ -334,6 +335,7 @@ size_t plan_footprint_bytes(const Plan& p) {
   b += nang * (size_t)p.nz * 4 * 16 + 8 * nptsp * 4 + 8 * (size_t)p.nz * 4 * 8;  // step classes, ring-order extinction
   b += 8 * nptsp * 4 + 8 * npts * (4 * 4 + 1) + 8 * (npts / 4 + 8) * 20;  // ring layout, sectors
   b += p.ss_off.size() * 4 + p.ss_iph.size() * 4 + p.ss_coef.size() * 4;
+  b += (size_t)p.d.npx * p.d.npy * p.d.npz * 4 + p.zlevels.size() * 4;   // direct-beam extinction
   return b;
 }
 
@@ -393,11 +395,8 @@ int state_create(const Plan& p, State& s, std::string& err) {
   TRY(upload(s, s.alb, p.alb.data(), p.alb.size(), err));
   TRY(upload(s, s.chi, p.chi.data(), p.chi.size(), err));
   TRY(upload(s, s.tc, p.tc.data(), p.tc.size(), err));
-  TRY(upload(s, s.dirflux, p.dirflux.data(), p.dirflux.size(), err));
-  std::vector<float> f0(p.npts);
-  const float secmu0 = 1.0f / fabsf(p.d.solarmu);
-  for (int i = 0; i < p.npts; i++) f0[i] = p.dirflux[i] * secmu0;
-  TRY(upload(s, s.f0, f0.data(), f0.size(), err));
+  TRY(alloc0(s, s.dirflux, (size_t)p.npts, err));   // made by the direct-beam pass below
+  TRY(alloc0(s, s.f0, (size_t)p.npts, err));
   const size_t nsh = (size_t)s.nptsp * s.ld, ndo = (size_t)s.nptsp * s.ldo;
   TRY(alloc0(s, s.S, nsh, err));
   TRY(alloc0(s, s.R, nsh, err));
@@ -415,6 +414,39 @@ int state_create(const Plan& p, State& s, std::string& err) {
   TRY(alloc0(s, s.errflag, (size_t)1, err));
   // sweep geometry
   TRY(upload(s, s.gridpos, p.gridpos.data(), p.gridpos.size(), err));
+  {
+    // MAKE_DIRECT on the device: the direct flux of every grid point and F0
+    std::vector<float> extdirp;
+    DirectBeamGeom beam;
+    direct_beam_setup(p, extdirp, beam);
+    // a sun within 1e-6 of the horizon never reaches the top (the trace runs
+    // out of steps): the reference's error, without the wait
+    if (beam.cz == 0.0) {
+      err = "DIRECT_BEAM_PROP: ray left the property grid";
+      return AT3D_ERR_NUMERIC;
+    }
+    // a trace crosses each z level once and each x / y cell face of its
+    // horizontal run (height / cz * |c| over the spacing) once
+    const double zspan = (double)p.zlevels[p.d.npz - 1] - (double)p.zlevels[0];
+    const double run = zspan / beam.cz;
+    const double steps = p.d.npz + 8.0 + run * (fabs(beam.cx) / beam.delxd + fabs(beam.cy) / beam.delyd);
+    beam.max_steps = (long long)std::min(1.0e8, 2.0 * steps + 64.0);
+    float* d_ext = nullptr;
+    float* d_zl = nullptr;
+    TRY(upload(s, d_ext, extdirp.data(), extdirp.size(), err));
+    TRY(upload(s, d_zl, p.zlevels.data(), p.zlevels.size(), err));
+    beam.ext = d_ext;
+    beam.zl = d_zl;
+    launch_direct_beam(s.stream, beam, s.gridpos, p.npts, 1.0f / fabsf(p.d.solarmu), s.dirflux, s.f0, s.errflag);
+    HIPCHK(hipGetLastError());
+    int ef = 0;
+    HIPCHK(hipMemcpyAsync(&ef, s.errflag, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    if (ef) {
+      err = "DIRECT_BEAM_PROP: ray left the property grid";
+      return AT3D_ERR_NUMERIC;
+    }
+  }
   TRY(upload(s, s.gridptr, p.gridptr.data(), p.gridptr.size(), err));
   TRY(upload(s, s.neighptr, p.neighptr.data(), p.neighptr.size(), err));
   TRY(upload(s, s.cellflags, p.cellflags.data(), p.cellflags.size(), err));
@@ -468,6 +500,9 @@ int state_create(const Plan& p, State& s, std::string& err) {
   TRY(upload(s, s.ss_off, p.ss_off.data(), p.ss_off.size(), err));
   TRY(upload(s, s.ss_iph, p.ss_iph.data(), p.ss_iph.size(), err));
   TRY(upload(s, s.ss_coef, p.ss_coef.data(), p.ss_coef.size(), err));
+  // the plan holds them per unit direct flux
+  launch_direct_scale(s.stream, s.dirflux, p.npts, p.ml + 1, s.tms, s.ss_off, s.ss_coef);
+  HIPCHK(hipGetLastError());
   TRY(alloc0(s, s.bcrad_render, p.bcbot.size(), err));
   std::vector<int32_t> bot_index(p.npts, -1);
   for (size_t b = 0; b < p.bcbot.size(); b++) bot_index[p.bcbot[b]] = (int32_t)b;

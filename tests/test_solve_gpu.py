@@ -171,3 +171,18 @@ def test_sweep_graph_replay_bit_identical_to_direct_launches():
     assert np.array_equal(a.state_array("do_radiance"), b.state_array("do_radiance"))
     assert np.array_equal(a.state_array("source"), b.state_array("source"))
     assert a.num_iterations == b.num_iterations
+
+
+@pytest.mark.parametrize("make", [lambda: blob8("periodic"), lambda: blob8("open"), cloud_small],
+                         ids=["blob8_periodic", "blob8_open", "cloud12_two_species"])
+def test_device_direct_beam_matches_host_trace(make):
+    """The solve's direct flux, traced per point on the device, against the
+    plan's host trace of the same FP64 path (which the oracle tests pin):
+    equal to float rounding of exp."""
+    w = make()
+    rte = S.make_rte(w)
+    rte.solve(0, verbose=False)
+    dev = rte.state_array("dirflux")
+    host = rte.plan_array("dirflux")
+    assert dev.shape == host.shape and np.all(np.isfinite(dev)) and dev.max() > 0
+    np.testing.assert_allclose(dev, host, rtol=2e-7, atol=0)
