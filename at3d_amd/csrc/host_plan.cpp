@@ -1117,6 +1117,17 @@ namespace {
 std::mutex level_cache_mutex;
 std::list<std::pair<std::vector<uint32_t>, std::shared_ptr<const std::array<OctantPlan, 8>>>> level_cache;
 constexpr size_t LEVEL_CACHE_ENTRIES = 16;
+constexpr size_t LEVEL_CACHE_BYTES = size_t(2) << 30;   // host memory the kept entries may hold
+
+size_t octants_bytes(const std::array<OctantPlan, 8>& a) {
+  size_t b = 0;
+  for (const OctantPlan& o : a)
+    b += 4 * (o.sweepord.size() + o.rank.size() + o.level.size() + o.lvl_pts.size() + o.lvl_cell.size() +
+              o.lvl_off.size() + o.ring_level.size() + o.ring_off.size() + o.ring_code.size() +
+              o.ring_cell.size() + o.ring_pos.size()) +
+         o.simple_pt.size();
+  return b;
+}
 
 std::vector<uint32_t> level_key(const Plan& p) {
   std::vector<uint32_t> k = {(uint32_t)p.npts, (uint32_t)p.ncells, (uint32_t)p.nx1, (uint32_t)p.ny1,
@@ -1161,7 +1172,12 @@ int build_levels(Plan& p, std::string& err) {
     auto octs = std::make_shared<std::array<OctantPlan, 8>>();
     std::vector<std::string> errs(8);
     std::vector<int> rcs(8, AT3D_OK);
-    parallel_for(8, [&](int o) { rcs[o] = build_octant(p, o, top, bot, (*octs)[o], errs[o]); });
+    bool used[8] = {};
+    for (const Ordinate& od : p.ords) used[od.g.ioct] = true;
+    parallel_for(8, [&](int o) {
+      rcs[o] = build_octant(p, o, top, bot, (*octs)[o], errs[o]);
+      if (rcs[o] == AT3D_OK && used[o] && !ring_octant_base(p, o, (*octs)[o], errs[o])) rcs[o] = AT3D_ERR_NUMERIC;
+    });
     for (int o = 0; o < 8; o++)
       if (rcs[o]) {
         err = errs[o];
@@ -1170,7 +1186,13 @@ int build_levels(Plan& p, std::string& err) {
     p.oct_store = octs;
     std::lock_guard<std::mutex> l(level_cache_mutex);
     level_cache.emplace_front(key, p.oct_store);
-    if (level_cache.size() > LEVEL_CACHE_ENTRIES) level_cache.pop_back();
+    size_t total = 0;
+    for (const auto& e : level_cache) total += octants_bytes(*e.second);
+    while (level_cache.size() > 1 &&
+           (level_cache.size() > LEVEL_CACHE_ENTRIES || total > LEVEL_CACHE_BYTES)) {
+      total -= octants_bytes(*level_cache.back().second);
+      level_cache.pop_back();
+    }
   }
   p.oct = p.oct_store->data();
   p.nlevels_max = 0;

@@ -52,6 +52,13 @@ struct OctantPlan {
   std::vector<int32_t> lvl_off;    // [nlevels+1] offsets into lvl_pts (level 1..)
   int nlevels = 0;
   int nslow = 0;                   // points needing multi-cell traces
+  // the level-ordered sweep's entries before they are grouped by source
+  // class (ring_octant_base): ring level of each point, level offsets
+  // [ring_nlev+1], per position the point (>= 0) or -1-b for boundary point
+  // b, its start cell, and the position of each point; empty for octants
+  // without ordinates
+  std::vector<int32_t> ring_level, ring_off, ring_code, ring_cell, ring_pos;
+  int ring_nlev = 0;
 };
 
 struct Plan {
@@ -118,6 +125,9 @@ void plan_source_class(const Plan& p, std::vector<int32_t>& cls);
 void direct_beam_setup(const Plan& p, std::vector<float>& extdirp, DirectBeamGeom& b);
 // The direct flux of every grid point on the host threads (a solve makes
 // its own on the device; the plan holds none).
+// Fills op's ring_* entry order for octant o from its levels (geometry only,
+// so it is built once with the levels and shared like them).
+bool ring_octant_base(const Plan& p, int o, OctantPlan& op, std::string& err);
 int plan_direct_host(const Plan& p, std::vector<float>& dirflux, std::string& err);
 int plan_sun_paths(const Plan& p, std::vector<int32_t>& off, std::vector<int32_t>& ptr,
                    std::vector<float>& val, std::string& err);

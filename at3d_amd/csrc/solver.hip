@@ -72,6 +72,15 @@ static int upload(State& s, T*& dst, const T* src, size_t n, std::string& err) {
 }
 
 template <typename T>
+static int alloc_only(State& s, T*& dst, size_t n, std::string& err) {
+  const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+  HIPCHK(dev_alloc((void**)&dst, bytes));
+  s.allocs.emplace_back(dst, bytes);
+  s.device_bytes += bytes;
+  return AT3D_OK;
+}
+
+template <typename T>
 static int alloc0(State& s, T*& dst, size_t n, std::string& err) {
   const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
   HIPCHK(dev_alloc((void**)&dst, bytes));
@@ -172,7 +181,7 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
   s.gld = rp.nlev_max + 2;
   const size_t np = p.npts;
   s.ncls = p.nz * 4;
-  std::vector<int32_t> perm((size_t)8 * s.nptsp, 0), code(8 * np, 0), cell(8 * np, -1), lev(8 * np, 0);
+  std::vector<int32_t> perm((size_t)8 * s.nptsp, 0);
   std::vector<float> extpos((size_t)8 * s.nptsp, 0.f);
   // a representative (point, start cell) of each step class, per octant: the
   // class's point nearest the domain's centre column (away from the seams
@@ -186,7 +195,6 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
   xmid *= 0.5f;
   ymid *= 0.5f;
   std::vector<int32_t> roff((size_t)8 * s.gld, 0), soff((size_t)8 * (s.gld + 1), 0), sp0;
-  std::vector<int8_t> staged(8 * np, 0);
   std::vector<uint16_t> sslot;
   s.oct_pos_h.assign(8 * np, 0);
   std::vector<int4> tiles;
@@ -195,9 +203,12 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
   parallel_for(8, [&](int o) {
     int32_t* pm = perm.data() + (size_t)o * s.nptsp;
     for (int q = 0; q < s.nptsp; q++) pm[q] = q;  // unused octants and padding rows
-    for (size_t i = 0; i < np; i++) s.oct_pos_h[o * np + i] = (int32_t)i;
-    if (!used[o]) return;
+    if (!used[o]) {
+      for (size_t i = 0; i < np; i++) s.oct_pos_h[o * np + i] = (int32_t)i;
+      return;
+    }
     const RingOctant& ro = rp.oct[o];
+    std::copy(ro.pos.begin(), ro.pos.end(), s.oct_pos_h.begin() + o * np);
     const std::vector<int32_t>& bc = ((o >> 2) & 1) ? p.bcbot : p.bctop;
     float* ex = extpos.data() + (size_t)o * s.nptsp;
     int2* rp_o = reps.data() + (size_t)o * s.ncls;
@@ -205,11 +216,6 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
     for (size_t q = 0; q < np; q++) {
       pm[q] = ro.code[q] >= 0 ? ro.code[q] : bc[-1 - ro.code[q]];
       ex[q] = p.total_ext[pm[q]];
-      code[o * np + q] = ro.code[q];
-      cell[o * np + q] = ro.cell[q];
-      lev[o * np + q] = ro.level[q];
-      staged[o * np + q] = ro.staged[q];
-      s.oct_pos_h[o * np + q] = ro.pos[q];
       if (ro.code[q] >= 0 && ro.cell[q] >= 0) {
         const int pt = ro.code[q];
         const int cls = (pt % p.nz) * 4 + (p.cellflags[ro.cell[q]] & 3);
@@ -284,10 +290,21 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
   TRY(upload(s, s.top_index, top_index.data(), top_index.size(), err));
   TRY(upload(s, s.oct_perm, perm.data(), perm.size(), err));
   TRY(upload(s, s.oct_pos, s.oct_pos_h.data(), s.oct_pos_h.size(), err));
-  TRY(upload(s, s.ring_code, code.data(), code.size(), err));
-  TRY(upload(s, s.ring_cell, cell.data(), cell.size(), err));
-  TRY(upload(s, s.ring_level, lev.data(), lev.size(), err));
-  TRY(upload(s, s.ring_staged, staged.data(), staged.size(), err));
+  // per-octant slices straight from the ring plan (zero for octants without
+  // ordinates, which no column reads)
+  TRY(alloc0(s, s.ring_code, 8 * np, err));
+  TRY(alloc0(s, s.ring_cell, 8 * np, err));
+  TRY(alloc0(s, s.ring_level, 8 * np, err));
+  TRY(alloc0(s, s.ring_staged, 8 * np, err));
+  for (int o = 0; o < 8; o++) {
+    if (!used[o]) continue;
+    const RingOctant& ro = rp.oct[o];
+    HIPCHK(hipMemcpyAsync(s.ring_code + o * np, ro.code.data(), np * 4, hipMemcpyHostToDevice, s.stream));
+    HIPCHK(hipMemcpyAsync(s.ring_cell + o * np, ro.cell.data(), np * 4, hipMemcpyHostToDevice, s.stream));
+    HIPCHK(hipMemcpyAsync(s.ring_level + o * np, ro.level.data(), np * 4, hipMemcpyHostToDevice, s.stream));
+    HIPCHK(hipMemcpyAsync(s.ring_staged + o * np, ro.staged.data(), np, hipMemcpyHostToDevice, s.stream));
+  }
+  HIPCHK(hipStreamSynchronize(s.stream));
   TRY(upload(s, s.ring_off, roff.data(), roff.size(), err));
   TRY(upload(s, s.sec_off, soff.data(), soff.size(), err));
   if (sp0.empty()) sp0.push_back(0);
@@ -472,25 +489,27 @@ int state_create(const Plan& p, State& s, std::string& err) {
   for (int i : p.bctop) preset[i] |= 1;
   for (int i : p.bcbot) preset[i] |= 2;
   TRY(upload(s, s.preset, preset.data(), preset.size(), err));
-  std::vector<int32_t> rank, level, lp, lc, lo;
+  // rank and level of every octant (the records and the multi-cell traces
+  // read them); the point-ordered sweep's level lists wait for its first use
+  TRY(alloc_only(s, s.rank, (size_t)8 * p.npts, err));
+  TRY(alloc_only(s, s.level, (size_t)8 * p.npts, err));
   s.nlevels_max = 0;
-  for (int o = 0; o < 8; o++) {
-    const OctantPlan& op = p.oct[o];
-    rank.insert(rank.end(), op.rank.begin(), op.rank.end());
-    level.insert(level.end(), op.level.begin(), op.level.end());
-    const int base = (int)lp.size();
-    s.oct_pts_base_h[o] = base;
-    lp.insert(lp.end(), op.lvl_pts.begin(), op.lvl_pts.end());
-    lc.insert(lc.end(), op.lvl_cell.begin(), op.lvl_cell.end());
-    s.lo_base_h[o] = (int)lo.size();
-    for (int L = 0; L <= op.nlevels; L++) lo.push_back(base + op.lvl_off[L]);
-    if (s.oct_nord[o] > 0) s.nlevels_max = std::max(s.nlevels_max, op.nlevels);
+  {
+    int base = 0, lob = 0;
+    for (int o = 0; o < 8; o++) {
+      const OctantPlan& op = p.oct[o];
+      HIPCHK(hipMemcpyAsync(s.rank + (size_t)o * p.npts, op.rank.data(), (size_t)p.npts * 4,
+                            hipMemcpyHostToDevice, s.stream));
+      HIPCHK(hipMemcpyAsync(s.level + (size_t)o * p.npts, op.level.data(), (size_t)p.npts * 4,
+                            hipMemcpyHostToDevice, s.stream));
+      s.oct_pts_base_h[o] = base;
+      s.lo_base_h[o] = lob;
+      base += (int)op.lvl_pts.size();
+      lob += op.nlevels + 1;
+      if (s.oct_nord[o] > 0) s.nlevels_max = std::max(s.nlevels_max, op.nlevels);
+    }
+    HIPCHK(hipStreamSynchronize(s.stream));
   }
-  TRY(upload(s, s.rank, rank.data(), rank.size(), err));
-  TRY(upload(s, s.level, level.data(), level.size(), err));
-  TRY(upload(s, s.lvl_pts, lp.data(), lp.size(), err));
-  TRY(upload(s, s.lvl_cell, lc.data(), lc.size(), err));
-  TRY(upload(s, s.lvl_off, lo.data(), lo.size(), err));
   std::vector<int8_t> coct(s.ldo, 0);
   for (int o = 0; o < 8; o++)
     for (int c = s.oct_col_base[o]; c < s.oct_col_base[o] + s.oct_nord[o]; c++) coct[c] = (int8_t)o;
@@ -658,7 +677,27 @@ static void enqueue_sweeps(State& s) {
   }
 }
 
+// The point-ordered sweep's level lists (every octant's non-preset points
+// by level, their start cells and the level offsets), uploaded on its first use.
+static int ensure_point_order(State& s, std::string& err) {
+  if (s.lvl_pts) return AT3D_OK;
+  const Plan& p = *s.plan;
+  std::vector<int32_t> lp, lc, lo;
+  for (int o = 0; o < 8; o++) {
+    const OctantPlan& op = p.oct[o];
+    const int base = (int)lp.size();
+    lp.insert(lp.end(), op.lvl_pts.begin(), op.lvl_pts.end());
+    lc.insert(lc.end(), op.lvl_cell.begin(), op.lvl_cell.end());
+    for (int L = 0; L <= op.nlevels; L++) lo.push_back(base + op.lvl_off[L]);
+  }
+  TRY(upload(s, s.lvl_pts, lp.data(), lp.size(), err));
+  TRY(upload(s, s.lvl_cell, lc.data(), lc.size(), err));
+  TRY(upload(s, s.lvl_off, lo.data(), lo.size(), err));
+  return AT3D_OK;
+}
+
 static int run_sweeps(State& s, std::string& err) {
+  if (!ring_active(s)) TRY(ensure_point_order(s, err));
   // the level-ordered sweep is three launches per pass (and is timed by
   // events between them); the point-ordered one is replayed from a graph
   if (!s.use_graph || ring_active(s)) {
