@@ -36,6 +36,8 @@ struct SweepArgs {
   const int32_t* bcbot;      // [nbot]
   int ntop, nbot;
   const int8_t* preset;      // [npts] bit0 top point, bit1 bottom point
+  const uint8_t* face_ok;    // [8][npts] bit d: the start cell's far face across dimension d (x, y, z) has
+                             // every corner computed earlier (preset or lower rank) within the ring window
   const int32_t* rank;       // [8][npts]
   const int32_t* level;      // [8][npts]
   const int32_t* lvl_pts;    // concatenated over octants (~pt = general trace)
@@ -117,6 +119,8 @@ inline size_t ring_lds_bytes(int slots, int gld, int ncls) {
 // (lds_fast 0: the fast instantiation is not used); nonzero on failure.
 int sweep_ring_max_lds(size_t lds_fast, size_t lds_full);
 void launch_sweep_records(hipStream_t st, const SweepArgs& a);
+// face_ok of every point of the octants with ordinates.
+void launch_face_mask(hipStream_t st, const SweepArgs& a);
 // The step class table (geo) of every column, from each class's representative.
 void launch_sweep_geo(hipStream_t st, const SweepArgs& a, int ncols);
 // fast / full: launch the instantiation without / with the multi-cell,

@@ -126,6 +126,7 @@ static SweepArgs sweep_args(const State& s) {
   a.npts = p.npts;
   a.force_general = s.force_general;
   a.errflag = s.errflag;
+  a.face_ok = s.face_ok;
   a.oct_pos = s.oct_pos;
   a.ring_code = s.ring_code;
   a.ring_cell = s.ring_cell;
@@ -319,7 +320,9 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
   TRY(alloc0(s, s.gflag, (size_t)p.nang * s.gld, err));
   TRY(alloc0(s, s.col_general, (size_t)p.nang, err));
   TRY(alloc0(s, s.ngeneral_d, (size_t)1, err));
+  TRY(alloc0(s, s.face_ok, 8 * np, err));
   launch_sweep_geo(s.stream, sweep_args(s), p.nang);
+  launch_face_mask(s.stream, sweep_args(s));
   launch_sweep_records(s.stream, sweep_args(s));
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(&s.ngeneral, s.ngeneral_d, sizeof(int), hipMemcpyDeviceToHost, s.stream));
@@ -350,7 +353,7 @@ size_t plan_footprint_bytes(const Plan& p) {
   b += 8 * npts * 4 * 4 + npts;          // sweep order: rank, level, level lists, preset
   b += nang * npts * 16;                 // sweep records
   b += nang * (size_t)p.nz * 4 * 16 + 8 * nptsp * 4 + 8 * (size_t)p.nz * 4 * 8;  // step classes, ring-order extinction
-  b += 8 * nptsp * 4 + 8 * npts * (4 * 4 + 1) + 8 * (npts / 4 + 8) * 20;  // ring layout, sectors
+  b += 8 * nptsp * 4 + 8 * npts * (4 * 4 + 2) + 8 * (npts / 4 + 8) * 20;  // ring layout, face masks, sectors
   b += p.ss_off.size() * 4 + p.ss_iph.size() * 4 + p.ss_coef.size() * 4;
   b += (size_t)p.d.npx * p.d.npy * p.d.npz * 4 + p.zlevels.size() * 4;   // direct-beam extinction
   return b;

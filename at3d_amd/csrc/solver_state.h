@@ -79,6 +79,7 @@ struct State {
   unsigned long long nz_magic = 0;
   float cls_tol = 2e-5f;
   uint8_t* gflag = nullptr;
+  uint8_t* face_ok = nullptr;   // [8][npts] one-cell faces of each point (k_face_mask)
   int* col_general = nullptr;
   std::vector<int32_t> col_general_h;   // columns with multi-cell traces
   int ring_full = 0;                    // every column needs the full ring instantiation

@@ -273,6 +273,38 @@ __global__ __launch_bounds__(256) void k_sweep_geo(SweepArgs a, int ncols) {
 // read once per wavefront and each lane writes one 16-byte record.
 __device__ inline void sweep_record(const SweepArgs& a, int o, int col, int jl);
 
+// Which far faces of each point's start cell a one-cell step may end on: all
+// four corners computed before the point in the reference's sweep order
+// (preset boundary points or lower rank) and held by the ring (at most
+// RING_WINDOW levels back).  The same for every ordinate of an octant, so
+// made once per (octant, point) rather than per record.
+__global__ __launch_bounds__(256) void k_face_mask(SweepArgs a) {
+  const int o = blockIdx.y;
+  const int jl = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a.ring_nlev[o] <= 0 || jl >= a.npts) return;
+  const size_t ob = (size_t)o * a.npts;
+  const int code = a.ring_code[ob + jl];
+  if (code < 0) return;   // boundary entry
+  const int pt = code, cell = a.ring_cell[ob + jl];
+  const int bitx = o & 1, bity = (o >> 1) & 1, bitz = (o >> 2) & 1;
+  const int presetbit = bitz ? 2 : 1;
+  const int32_t* rank = a.rank + ob;
+  const int32_t* rlev = a.ring_level + ob;
+  const int L = rlev[pt], rp = rank[pt];
+  uint32_t m = 0;
+  for (int d = 0; d < 3; d++) {
+    const int face = d == 0 ? 1 - bitx : d == 1 ? 3 - bity : 5 - bitz;
+    bool ok = true;
+    for (int k = 0; k < 4 && ok; k++) {
+      const int q = a.g.gridptr[(size_t)cell * 8 + face_corner(face, k)];
+      const int lq = rlev[q];
+      ok = ((a.preset[q] & presetbit) || rank[q] < rp) && lq < L && lq >= L - RING_WINDOW;
+    }
+    if (ok) m |= 1u << d;
+  }
+  const_cast<uint8_t*>(a.face_ok)[ob + pt] = (uint8_t)m;
+}
+
 __global__ __launch_bounds__(256) void k_sweep_records(SweepArgs a) {
   const int o = blockIdx.y;
   const int jl = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -284,10 +316,7 @@ __device__ inline void sweep_record(const SweepArgs& a, int o, int col, int jl) 
 #pragma clang fp contract(off)
   const size_t ob = (size_t)o * a.npts;
   const OrdGeom og = a.ordg[col];
-  const int presetbit = og.bitz ? 2 : 1;
   const int code = a.ring_code[ob + jl];
-  const int32_t* rank = a.rank + ob;
-  const int32_t* rlev = a.ring_level + ob;
   const int32_t* pos = a.oct_pos + ob;
   SweepRec r;
   r.r[0] = r.r[1] = r.r[2] = r.r[3] = 0;
@@ -307,18 +336,14 @@ __device__ inline void sweep_record(const SweepArgs& a, int o, int col, int jl) 
     r.r[3] = (uint16_t)(bits >> 16);
   } else {
     const int pt = code;
-    const int L = rlev[pt];
     const int cell = a.ring_cell[ob + jl];
     const int8_t fl = a.g.cellflags[cell] & 3;
+    const uint32_t fm = a.face_ok[ob + pt];
     double xe = a.g.gridpos[pt * 3], ye = a.g.gridpos[pt * 3 + 1], ze = a.g.gridpos[pt * 3 + 2];
     TraceStep t;
-    bool ok = trace_step(a.g, og, cell, xe, ye, ze, t);
-    for (int k = 0; k < 4 && ok; k++) {
-      const int q = t.i[k];
-      const int lq = rlev[q];
-      ok = ((a.preset[q] & presetbit) || rank[q] < rank[pt]) && lq < L && lq >= L - RING_WINDOW;
-      r.r[k] = (uint16_t)(pos[q] % a.slots);
-    }
+    bool ok = trace_step(a.g, og, cell, xe, ye, ze, t) && ((fm >> (t.face >> 1)) & 1);
+    if (ok)
+      for (int k = 0; k < 4; k++) r.r[k] = (uint16_t)(pos[t.i[k]] % a.slots);
     const float4 g = a.geo[(size_t)col * a.ncls + step_class(a, pt, fl)];
     const int gface = __float_as_int(g.w);
     int cn[4] = {t.i[0], t.i[1], t.i[2], t.i[3]};
@@ -326,12 +351,9 @@ __device__ inline void sweep_record(const SweepArgs& a, int o, int col, int jl) 
     if (ok && t.so > 0.0 && gface >= 0 && gface != t.face) {
       // a step through an edge shared with the class's face: on that face
       face_uv(a.g, cell, gface, xe, ye, ze, u, v, cn);
-      for (int k = 0; k < 4 && ok; k++) {
-        const int q = cn[k];
-        const int lq = rlev[q];
-        ok = ((a.preset[q] & presetbit) || rank[q] < rank[pt]) && lq < L && lq >= L - RING_WINDOW;
-        r.r[k] = (uint16_t)(pos[q] % a.slots);
-      }
+      ok = (fm >> (gface >> 1)) & 1;
+      if (ok)
+        for (int k = 0; k < 4; k++) r.r[k] = (uint16_t)(pos[cn[k]] % a.slots);
     }
     // the step is its class's when u, v and the path length agree to within
     // the rounding of the float grid positions
@@ -358,7 +380,7 @@ __device__ inline void sweep_record(const SweepArgs& a, int o, int col, int jl) 
       r.head = (uint32_t)pt | ((uint32_t)fl << RECORD_PT_BITS) | (staged << 29) | (1u << 30);
       atomicAdd(a.ngeneral, 1);
       atomicOr(a.col_general + col, 1);
-      const_cast<uint8_t*>(a.gflag)[(size_t)col * a.gld + L] = 1;
+      const_cast<uint8_t*>(a.gflag)[(size_t)col * a.gld + a.ring_level[ob + pt]] = 1;
     }
   }
   const_cast<uint4*>(a.rec)[(size_t)col * a.npts + jl] = *reinterpret_cast<const uint4*>(&r);
@@ -789,6 +811,10 @@ void launch_sweep_preset(hipStream_t st, const SweepArgs& a, bool up, int col0, 
 void launch_sweep_columns(hipStream_t st, const SweepArgs& a, int col0, int ncol) {
   if (ncol <= 0) return;
   k_sweep_columns<<<ncol, SWEEP_THREADS, 0, st>>>(a, col0);
+}
+
+void launch_face_mask(hipStream_t st, const SweepArgs& a) {
+  k_face_mask<<<dim3((a.npts + 255) / 256, 8), 256, 0, st>>>(a);
 }
 
 void launch_sweep_records(hipStream_t st, const SweepArgs& a) {
