@@ -80,11 +80,14 @@ int64_t at3d_device_trim(void);
 /* Host setup of one solve: base grid and cells (NEW_GRIDS / INIT_CELL_STRUCTURE,
  * shdomsub2.f:16-309), property transfer with delta-M (TRANSFER_PA_TO_GRID,
  * shdomsub1.f:19-112), discrete ordinates and SH transform tables
- * (MAKE_ANGLE_SET / MAKE_SH_DO_COEF, shdomsub2.f:1061-1220), the direct beam
- * (MAKE_DIRECT, shdomsub2.f:393-473), the Eddington initial radiance
- * (INIT_RADIANCE, shdomsub2.f:614-760), boundary points (BOUNDARY_PNTS,
- * shdomsub1.f:2173) and the per-octant sweep order (SWEEPING_ORDER,
- * shdomsub1.f:3261) with its dependency levels.  Runs on the CPU only. */
+ * (MAKE_ANGLE_SET / MAKE_SH_DO_COEF, shdomsub2.f:1061-1220), the Eddington
+ * initial radiance (INIT_RADIANCE, shdomsub2.f:614-760), boundary points
+ * (BOUNDARY_PNTS, shdomsub1.f:2173) and the per-octant sweep order
+ * (SWEEPING_ORDER, shdomsub1.f:3261) with its dependency levels.  Runs on the
+ * CPU only.  The direct beam (MAKE_DIRECT, shdomsub2.f:393-473) is traced per
+ * state on the device by at3d_state_create, which reports its
+ * "DIRECT_BEAM_PROP: ray left the property grid" error; the plan's "dirflux"
+ * array is the same trace on the host, made when first asked for. */
 int at3d_plan_create(const at3d_scene_desc* desc, at3d_plan** out, char* err, size_t errlen);
 void at3d_plan_destroy(at3d_plan* plan);
 
