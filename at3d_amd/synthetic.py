@@ -195,7 +195,23 @@ def serving_stream(n_requests=512, rate_hz=20.0, warm_fraction=0.3, seed=0, grid
                 w = serve_medium(spec["grid"], spec["nmu"], spec["nphi"], wl, spec["seed"],
                                  spec["noise"], spec.get("noise_seed", 0), solacc=solacc)
                 return make_rte(w)
+            make.geometry = (tuple(spec["grid"]), spec["nmu"], spec["nphi"])
             reqs.append(Request(key=(i, j), make_rte=make, maxiter=100,
                                 warm_from=None if spec["src"] is None else (spec["src"], j),
                                 arrival_s=t, group=i))
     return reqs
+
+
+def warm_geometries(requests):
+    """Builds one plan for each distinct (grid, NMU, NPHI) of `requests`
+    (made by ``serving_stream``), so the host's cache of sweep levels (shared
+    by plans of one geometry) holds them all, as in a server that has seen
+    its usual geometries.  Returns the number of geometries."""
+    seen = {}
+    for r in requests:
+        g = getattr(r.make_rte, "geometry", None)
+        if g is not None and g not in seen:
+            seen[g] = r.make_rte
+    for make in seen.values():
+        make()
+    return len(seen)

@@ -111,3 +111,16 @@ def test_group_latency_is_the_last_solve_of_the_group():
     res, st = SolveServer(nworkers=1).run(reqs)
     assert st["requests"] == 2 and st["solves"] == 4
     assert st["latency_p99_ms"] >= 25.0     # group 0 waits for three serial solves of 10 ms
+
+
+def test_stream_requests_name_their_geometry_and_warm_builds_one_plan_each():
+    """serving_stream tags every request's factory with (grid, NMU, NPHI);
+    warm_geometries builds one plan per distinct geometry (host only)."""
+    from at3d_amd import synthetic as S
+    reqs = S.serving_stream(6, rate_hz=100.0, seed=2, grids=((8, 8, 6), (10, 8, 6)), angles=((4, 8),),
+                            max_wavelengths=2)
+    geoms = {r.make_rte.geometry for r in reqs}
+    assert geoms <= {((8, 8, 6), 4, 8), ((10, 8, 6), 4, 8)} and len(geoms) >= 1
+    # warm restarts share their source's geometry
+    assert all(r.make_rte.geometry in geoms for r in reqs)
+    assert S.warm_geometries(reqs) == len(geoms)
