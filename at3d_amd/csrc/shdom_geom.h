@@ -95,14 +95,14 @@ __host__ __device__ inline bool trace_step(const GridTopo& g, const OrdGeom& o, 
     v = ipinx ? 0.5 : (xe - (double)P[i1 * 3 + 0]) / ((double)P[i2 * 3 + 0] - (double)P[i1 * 3 + 0]);
   }
   // Boundary wrapping: snap the crossing coordinate to the next cell's entry corner.
+  // (selects, not branches: a branch per coordinate makes the compiler
+  // address the three as an array in scratch memory)
   if (inext >= 0) {
     const int q = g.gridptr[inext * 8 + o.ioct];
-    if (jface == 1)
-      xe = (double)P[q * 3 + 0];
-    else if (jface == 2)
-      ye = (double)P[q * 3 + 1];
-    else
-      ze = (double)P[q * 3 + 2];
+    const double snap = (double)P[q * 3 + (jface - 1)];
+    xe = jface == 1 ? snap : xe;
+    ye = jface == 2 ? snap : ye;
+    ze = jface == 3 ? snap : ze;
   }
   t.so = so;
   t.u = u;

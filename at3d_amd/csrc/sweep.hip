@@ -367,7 +367,8 @@ __device__ inline void sweep_record(const SweepArgs& a, int o, int col, int jl) 
       corner = (t.u == 1.0 ? 2 : 0) + (t.v == 1.0 ? 1 : 0);
     const uint32_t staged = (uint32_t)a.ring_staged[ob + pt];
     if (ok && corner >= 0) {
-      r.r[0] = r.r[corner];
+      // (a select chain: a runtime index would put the record in scratch)
+      r.r[0] = corner == 0 ? r.r[0] : corner == 1 ? r.r[1] : corner == 2 ? r.r[2] : r.r[3];
       r.head = (uint32_t)pt | ((uint32_t)fl << RECORD_PT_BITS) | (staged << 29) | (3u << 30);
     } else if (ok && cls_ok) {
       const float* ext = a.ext;
