@@ -227,6 +227,24 @@ __global__ __launch_bounds__(256) void k_do_to_sh(const float* __restrict__ D,
     nt = blockIdx.x;
   }
   const int m0 = mt * TM, n0 = nt * TN;
+  // an SH tile at or past every radiance count of the point tile is zero:
+  // written as such without the product (radiance truncation leaves most
+  // clear-sky points a few low-order terms)
+  __shared__ int tile_nr;
+  if (wave == 0) {
+    int v = (m0 + lane < npts) ? nr[m0 + lane] : 0;
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+    if (lane == 0) tile_nr = v;
+  }
+  __syncthreads();
+  if (n0 >= tile_nr) {
+    for (int i = tid; i < TM * (TN / 4); i += 256) {
+      const int r = i / (TN / 4), c = (i - r * (TN / 4)) * 4;
+      if (m0 + r < npts)
+        *reinterpret_cast<float4*>(R + (size_t)(m0 + r) * ld + n0 + c) = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    return;
+  }
   float4_ acc[2][2];
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++) acc[i][j] = (float4_){0.f, 0.f, 0.f, 0.f};
