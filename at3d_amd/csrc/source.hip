@@ -40,9 +40,12 @@ __global__ void k_radiance_truncation(const float* __restrict__ R, int ld, const
 // SOURCET_J = w chi_l (F0 Ysun_J + I_J); the acceleration dot products
 // accumulate in FP64 per lane in term order and are written as one partial per
 // block for a fixed-order reduction.
-constexpr int SRC_TERMS = 8;  // per lane: NLM <= 64 * SRC_TERMS
-
+// NT terms per lane: NLM <= 64 * NT (4 up to L = 15, 8 up to NLM 512; the
+// smaller instantiation holds fewer registers and keeps twice the waves in
+// flight)
+template <int NT>
 __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
+  constexpr int SRC_TERMS = NT;
   __shared__ double red[4][4];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   double dot = 0, dold = 0, dnew = 0, jn = 0;
@@ -67,6 +70,10 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
     const bool old = !a.first;
     const bool useds = old && a.accel;
     const int nsd = a.accel ? min(ns_old, nos_old) : ns_old;
+    // the truncation constants of the next radiance truncation, loaded with
+    // the point's rows (the first 64 orders)
+    const float* tcp = a.tc + (size_t)p * (a.ml + 1);
+    const float tcv = (!a.fixsh && 1 + lane <= a.ml) ? tcp[1 + lane] : 0.f;
     float rv[SRC_TERMS], sv[SRC_TERMS], dv[SRC_TERMS], cv[SRC_TERMS];
 #pragma unroll
     for (int it = 0; it < SRC_TERMS; it++) {
@@ -148,9 +155,9 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
       nrn = a.highorder ? nterms_for_l(a.ml, a.mm) : max(4, nsn);
     } else {
       const float i0 = __shfl(rv[0], 0);
-      const float* tcp = a.tc + (size_t)p * (a.ml + 1);
       int lr = 1;
-      for (int l = 1 + lane; l <= a.ml; l += 64)
+      if (1 + lane <= a.ml && tcv * i0 > a.radmin) lr = 1 + lane;
+      for (int l = 65 + lane; l <= a.ml; l += 64)   // ML >= 64 (M much below L)
         if (tcp[l] * i0 > a.radmin) lr = l;
       for (int off = 32; off > 0; off >>= 1) lr = max(lr, __shfl_xor(lr, off));
       const int ls = a.lofj[max(1, nsn) - 1];
@@ -238,7 +245,10 @@ void launch_radiance_truncation(hipStream_t st, const float* R, int ld, const fl
 }
 
 void launch_compute_source(hipStream_t st, const SourceArgs& a, int nblocks) {
-  k_compute_source<<<nblocks, 256, 0, st>>>(a);
+  if (a.nlm <= 256)
+    k_compute_source<4><<<nblocks, 256, 0, st>>>(a);
+  else
+    k_compute_source<8><<<nblocks, 256, 0, st>>>(a);
 }
 
 void launch_init_radiance(hipStream_t st, float* R, const float* R0, int* nr, int ld, int npts) {
