@@ -58,11 +58,30 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
     lv[it] = J < a.nlm ? a.lofj[J] : 0;
     yv[it] = J < a.nlm ? a.ysun[J] : 0.f;
   }
+  // a point's scalars (albedo, F0, NR, NS, NOS) are loaded one point ahead,
+  // so the row loads of a point wait for one memory round trip, not two
+  const int p0 = blockIdx.x * SRC_PTS_PER_BLOCK + w;
+  float alb_n = 0.f, f0_n = 0.f;
+  int nr_n = 0, ns_n = 0, nos_n = 0;
+  if (p0 < a.npts) {
+    alb_n = a.alb[p0];
+    f0_n = a.f0[p0];
+    nr_n = a.nr[p0];
+    ns_n = a.ns[p0];
+    nos_n = a.nos[p0];
+  }
   for (int sub = 0; sub < SRC_PTS_PER_BLOCK / 4; sub++) {
-    const int p = blockIdx.x * SRC_PTS_PER_BLOCK + sub * 4 + w;
+    const int p = p0 + sub * 4;
     if (p >= a.npts) break;
-    const float alb = a.alb[p], f0 = a.f0[p];
-    const int nrp = a.nr[p], ns_old = a.ns[p], nos_old = a.nos[p];
+    const float alb = alb_n, f0 = f0_n;
+    const int nrp = nr_n, ns_old = ns_n, nos_old = nos_n;
+    if (sub + 1 < SRC_PTS_PER_BLOCK / 4 && p + 4 < a.npts) {
+      alb_n = a.alb[p + 4];
+      f0_n = a.f0[p + 4];
+      nr_n = a.nr[p + 4];
+      ns_n = a.ns[p + 4];
+      nos_n = a.nos[p + 4];
+    }
     const float* Rp = a.R + (size_t)p * a.ld;
     float* Sp = a.S + (size_t)p * a.ld;
     float* DSp = a.DS + (size_t)p * a.ld;
