@@ -186,3 +186,17 @@ def test_device_direct_beam_matches_host_trace(make):
     host = rte.plan_array("dirflux")
     assert dev.shape == host.shape and np.all(np.isfinite(dev)) and dev.max() > 0
     np.testing.assert_allclose(dev, host, rtol=2e-7, atol=0)
+
+
+def test_wide_open_grid_keeps_every_step_in_its_class():
+    """96x96x48 open grid with (NMU, NPHI) = (16, 32): the ordinates at
+    phi = 45 degrees leave cells through shared edges, and the float grid
+    positions move u, v by about 2e-5 between points of one step class; the
+    class tolerance follows the grid's float resolution, so no step falls to
+    the multi-cell trace, and the ring sweep runs with its step classes read
+    from device memory."""
+    w = S.serve_medium((96, 96, 48), 16, 32, 0.86, 1)
+    rte = S.make_rte(w)
+    rte.solve(0, verbose=False)
+    assert rte.timing("sweep_ring") == 1.0
+    assert rte.timing("sweep_general_points") == 0
