@@ -92,6 +92,15 @@ class SolveServer:
         cond = threading.Condition()
         t0 = time.perf_counter()
 
+        def release_source(key):
+            # one dependent of `key` has started (or failed): the kept
+            # solution goes once no dependent is left.  Caller holds `cond`.
+            pending[key] -= 1
+            if pending[key] <= 0 and key in kept:
+                kept.pop(key).release()
+                held.pop(key, None)
+                cond.notify_all()
+
         def admit(key, nbytes):
             with cond:
                 while running[0] > 0 and sum(held.values()) + nbytes > budget:
@@ -109,6 +118,7 @@ class SolveServer:
                 rte = None
                 setup = 0.0
                 admitted = False
+                source_done = req.warm_from is None
                 try:
                     rte = req.make_rte()           # host setup: no device memory yet
                     setup = time.perf_counter() - ts
@@ -126,13 +136,12 @@ class SolveServer:
                     admit(req.key, fp() if fp is not None else 0)
                     admitted = True
                     if src is not None:
-                        rte.init_solution_from(src)
-                        with cond:
-                            pending[req.warm_from] -= 1
-                            if pending[req.warm_from] == 0:
-                                kept.pop(req.warm_from).release()
-                                held.pop(req.warm_from, None)
-                                cond.notify_all()
+                        try:
+                            rte.init_solution_from(src)
+                        finally:
+                            with cond:
+                                source_done = True
+                                release_source(req.warm_from)
                         rte.solve(req.maxiter, init_solution=False, verbose=False)
                     else:
                         rte.solve(req.maxiter, verbose=False)
@@ -147,6 +156,9 @@ class SolveServer:
                     res = Result(req.key, 0, float("nan"), tn - (t0 + req.arrival_s), tn - ts,
                                  0.0, None, f"{type(e).__name__}: {e}")
                 with cond:
+                    if not source_done:
+                        # failed before it could start from its source
+                        release_source(req.warm_from)
                     if admitted:
                         running[0] -= 1
                     if res.error is None and pending.get(req.key, 0) > 0:

@@ -215,3 +215,51 @@ def warm_geometries(requests):
     for make in seen.values():
         make()
     return len(seen)
+
+
+def domain_box(w: Workload):
+    """(xmin, ymin, zmin, xmax, ymax, zmax) of a workload's base grid; a
+    periodic grid spans one more cell in x and y than its points."""
+    nx, ny, _ = w.shape
+    extra = 1 if w.bc == "periodic" else 0
+    return (0.0, 0.0, float(w.z[0]), (nx - 1 + extra) * w.delx, (ny - 1 + extra) * w.dely,
+            float(w.z[-1]))
+
+
+def view_rays(w: Workload, views, npix_x, npix_y):
+    """Parallel-ray views of the domain from its top: for each (zenith,
+    azimuth) in degrees (the direction of the photons), npix_x x npix_y
+    pixels spread evenly over the footprint of the domain box on the top
+    plane (the area orthographic_projection samples), x fastest.  Returns
+    the concatenated 'ray_*' arrays."""
+    xmin, ymin, zmin, xmax, ymax, zmax = domain_box(w)
+    out = {k: [] for k in ("ray_x", "ray_y", "ray_z", "ray_mu", "ray_phi")}
+    for zen, az in views:
+        mu = float(np.cos(np.deg2rad(zen)))
+        phi = float(np.deg2rad(az))
+        t = np.sqrt(1.0 - mu * mu) / mu
+        sx = t * np.cos(phi) * np.array([0.0, zmax - zmin])
+        sy = t * np.sin(phi) * np.array([0.0, zmax - zmin])
+        xs = np.linspace(xmin + sx.min(), xmax + sx.max(), npix_x)
+        ys = np.linspace(ymin + sy.min(), ymax + sy.max(), npix_y)
+        n = npix_x * npix_y
+        out["ray_x"].append(np.tile(xs, npix_y))
+        out["ray_y"].append(np.repeat(ys, npix_x))
+        out["ray_z"].append(np.full(n, zmax))
+        out["ray_mu"].append(np.full(n, mu))
+        out["ray_phi"].append(np.full(n, phi))
+    return {k: np.concatenate(v) for k, v in out.items()}
+
+
+# the headline workload's sensors: 9 views of 64 x 64 pixels along the
+# azimuth plane 0 / 180 degrees; the small blob's: 3 views of 32 x 32
+CLOUD64_VIEWS = tuple((abs(z), 0.0 if z >= 0 else 180.0) for z in (-60, -45, -30, -15, 0, 15, 30, 45, 60))
+BLOB16_VIEWS = ((0.0, 0.0), (45.0, 0.0), (45.0, 180.0))
+
+
+def workload_views(name: str, w: Workload):
+    if name == "cloud64":
+        return view_rays(w, CLOUD64_VIEWS, 64, 64)
+    if name.startswith("blob16"):
+        return view_rays(w, BLOB16_VIEWS, 32, 32)
+    raise KeyError(name)

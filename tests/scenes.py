@@ -5,8 +5,14 @@ from at3d_amd import synthetic as S
 from at3d_amd.sensor import orthographic_projection
 
 
-def blob8(bc="periodic", nleg=64, seed=0):
-    return S.lognormal_blob(n=8, radius=0.15, seed=seed, nleg=nleg, bc=bc)
+def blob8(bc="periodic", nleg=64, seed=0, nmu=8, nphi=16):
+    return S.lognormal_blob(n=8, radius=0.15, seed=seed, nleg=nleg, bc=bc, nmu=nmu, nphi=nphi)
+
+
+def blob8_l15(bc="open"):
+    """The 8^3 blob at the headline angular resolution: NMU 16 / NPHI 32,
+    L = M = 15 (NLM 256, NANG 354), 64-term HG table (delta-M f = chi_16)."""
+    return blob8(bc, nmu=16, nphi=32)
 
 
 def cloud_small(seed=1, n=12, nz=10):
@@ -14,6 +20,15 @@ def cloud_small(seed=1, n=12, nz=10):
     w = S.cloud_field(nx=n, ny=n, nz=nz, dx=0.05, dz=0.1, z0=0.4, seed=seed, ext_scale=20.0,
                       nmu=8, nphi=16, solacc=1e-5)
     return w
+
+
+def cloud_l15(seed=2, n=16, nz=12):
+    """Two species at the headline angular resolution (NMU 16 / NPHI 32,
+    L = 15): the cloud with the 200-term phase table (delta-M f = chi_16,
+    single-scatter correction from the whole table) plus Rayleigh, on a
+    16 x 16 x 12 open grid."""
+    return S.cloud_field(nx=n, ny=n, nz=nz, dx=0.05, dz=0.08, z0=0.3, seed=seed, ext_scale=20.0,
+                         nmu=16, nphi=32, solacc=1e-5)
 
 
 def rel_l2(a, b):

@@ -15,7 +15,7 @@ import pytest
 
 from at3d_amd import gradient as GR
 from at3d_amd import synthetic as S
-from tests.scenes import blob8, cloud_small, ortho_rays, rel_l2
+from tests.scenes import blob8, blob8_l15, cloud_l15, cloud_small, ortho_rays, rel_l2
 
 pytestmark = [pytest.mark.gpu, pytest.mark.filterwarnings("ignore:RTE is not solved")]
 
@@ -75,6 +75,17 @@ def test_extinction_and_albedo_gradient_matches_reference(ref, bc):
 def test_two_species_gradient_matches_reference(ref):
     _check(_run(ref, cloud_small(), [(0, "extinction"), (1, "extinction"), (0, "ssalb")],
                 VIEWS[:2]))
+
+
+def test_two_species_l15_gradient_matches_reference(ref):
+    """NMU 16 / NPHI 32 (L = 15): the 200-term cloud table (delta-M, full-table
+    single-scatter correction) and Rayleigh; extinction of both and the cloud
+    albedo."""
+    _check(_run(ref, cloud_l15(), [(0, "extinction"), (1, "extinction"), (0, "ssalb")], VIEWS[:2]))
+
+
+def test_blob_l15_gradient_matches_reference(ref):
+    _check(_run(ref, blob8_l15("periodic"), [(0, "extinction"), (0, "ssalb")], VIEWS))
 
 
 def test_gradient_without_exact_single_scatter_matches_reference(ref):

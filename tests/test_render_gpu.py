@@ -10,12 +10,13 @@ import pytest
 
 from at3d_amd import synthetic as S
 from at3d_amd._lib import SHDOMError
-from tests.scenes import blob8, cloud_small, interior_rays, ortho_rays, rel_l2
+from tests.scenes import blob8, blob8_l15, cloud_l15, cloud_small, interior_rays, ortho_rays, rel_l2
 
 pytestmark = [pytest.mark.gpu, pytest.mark.filterwarnings("ignore:RTE is not solved")]
 
 SCENES = {"blob8_periodic": lambda: blob8("periodic"), "blob8_open": lambda: blob8("open"),
-          "cloud12_two_species": cloud_small}
+          "cloud12_two_species": cloud_small, "blob8_periodic_l15": lambda: blob8_l15("periodic"),
+          "cloud16_l15": cloud_l15}
 
 
 def _pair(ref, w, iters):
@@ -57,16 +58,18 @@ def test_interior_rays_all_directions_match_reference(ref, name):
 
 
 @pytest.mark.parametrize("flags", [(True, False), (False, True)], ids=["single_scatter", "nosurface"])
-def test_render_flags_match_reference(ref, flags):
-    w = cloud_small()
+@pytest.mark.parametrize("make", [cloud_small, cloud_l15], ids=["cloud12", "cloud16_l15"])
+def test_render_flags_match_reference(ref, flags, make):
+    w = make()
     st, rte = _pair(ref, w, 2)
     rays = ortho_rays(w, [(0.0, 0.0), (50.0, 140.0)])
     rays = {k: np.concatenate([v, interior_rays(w, 60, seed=5)[k]]) for k, v in rays.items()}
     _compare(ref, st, rte, rays, single_scatter=flags[0], nosurface=flags[1])
 
 
-def test_converged_render_matches_reference(ref):
-    w = blob8("open")
+@pytest.mark.parametrize("make", [lambda: blob8("open"), blob8_l15], ids=["blob8_open", "blob8_open_l15"])
+def test_converged_render_matches_reference(ref, make):
+    w = make()
     st, rte = _pair(ref, w, 100)
     assert rte.check_solved(verbose=False)
     _compare(ref, st, rte, ortho_rays(w, VIEWS[:2]))

@@ -124,3 +124,38 @@ def test_stream_requests_name_their_geometry_and_warm_builds_one_plan_each():
     # warm restarts share their source's geometry
     assert all(r.make_rte.geometry in geoms for r in reqs)
     assert S.warm_geometries(reqs) == len(geoms)
+
+
+def test_failed_dependent_releases_its_source():
+    """A warm-start request that fails before starting from its source (here
+    its host setup raises) still counts as started: the kept solution is
+    released once no dependent is left, and its bytes leave the budget."""
+    FakeRTE.live = 0
+
+    def bad():
+        raise ValueError("bad medium")
+
+    reqs = [Request("a", lambda: FakeRTE(0, nbytes=10)), Request("b", bad, warm_from="a"),
+            Request("c", lambda: FakeRTE(2, nbytes=10), warm_from="a", arrival_s=0.05),
+            Request("d", lambda: FakeRTE(3, nbytes=10), arrival_s=0.1)]
+    res, st = SolveServer(nworkers=1, memory_budget=25).run(reqs)
+    assert [r.error is None for r in res] == [True, False, True, True]
+    assert res[2].iterations == 3
+    assert FakeRTE.live == 0
+    # "a" is released when "c" starts from it, so "d" never waits on it
+    assert st["peak_device_bytes"] <= 20
+
+
+def test_source_released_when_every_dependent_fails():
+    FakeRTE.live = 0
+
+    def bad():
+        raise ValueError("bad medium")
+
+    reqs = [Request("a", lambda: FakeRTE(0, nbytes=10)), Request("b", bad, warm_from="a"),
+            Request("c", bad, warm_from="a"),
+            Request("d", lambda: FakeRTE(3, nbytes=20), arrival_s=0.1)]
+    res, st = SolveServer(nworkers=1, memory_budget=25).run(reqs)
+    assert [r.error is None for r in res] == [True, False, False, True]
+    assert FakeRTE.live == 0
+    assert st["peak_device_bytes"] <= 20

@@ -42,6 +42,17 @@ def test_warm_restart_rejects_other_grids():
         b.init_solution_from(a)
 
 
+def test_warm_restart_rejects_another_sh_layout_of_equal_size():
+    """NMU 16 / NPHI 1 (L = 15, M = 0) and NMU 4 / NPHI 8 (L = M = 3) both hold
+    16 spherical-harmonic terms per point, in different (l, m) orders."""
+    a = S.make_rte(blob8("open", nmu=16, nphi=1))
+    a.solve(2, verbose=False)
+    b = S.make_rte(blob8("open", nmu=4, nphi=8))
+    assert a.plan_size("nlm") == b.plan_size("nlm") == 16
+    with pytest.raises(Exception, match="same grid"):
+        b.init_solution_from(a)
+
+
 def test_concurrent_serving_is_bit_identical_to_serial_solves():
     base = [blob8("open"), blob8("periodic"), cloud_small()]
     ws = base + [_perturbed(base[2], seed=s) for s in (1, 2, 3)]
@@ -103,3 +114,49 @@ def test_footprint_estimate_covers_the_state():
         est, got = r.footprint_bytes(), r.timing("device_bytes")
         assert got <= est <= 1.1 * got + (1 << 20), (est, got)
         r.release()
+
+
+def test_stream_over_every_grid_and_angle_set_is_the_same_alone_and_in_the_mix():
+    """The serving workload's shapes: each of the four grids (32^3, 48^2 x 32,
+    64^2 x 40, 96^2 x 48) and each of the three angle sets ((NMU, NPHI, L) =
+    (8, 16, 7), (12, 24, 11), (16, 32, 15)), one to three wavelengths per
+    request, two warm restarts from perturbed earlier requests, served by
+    four solvers under a memory budget that keeps requests waiting; every
+    solve's iteration count, SOLCRIT and rendered radiances equal those of
+    the same requests run one at a time."""
+    specs = [((32, 32, 32), 8, 16, [0.86]), ((48, 48, 32), 12, 24, [0.55, 1.6]),
+             ((64, 64, 40), 16, 32, [0.86]), ((96, 96, 48), 8, 16, [2.1]),
+             ((32, 32, 32), 16, 32, [0.67, 1.2, 2.2]), ((96, 96, 48), 12, 24, [0.9]),
+             ((48, 48, 32), 16, 32, [1.1])]
+    warm = {7: 2, 8: 1}
+    view = {"ray_x": np.linspace(0.05, 0.75, 8), "ray_y": np.full(8, 0.4), "ray_z": np.full(8, 0.64),
+            "ray_mu": np.full(8, 0.9), "ray_phi": np.full(8, 0.5)}
+    reqs = []
+    for i in range(len(specs) + len(warm)):
+        if i < len(specs):
+            grid, nmu, nphi, wls = specs[i]
+            seed, noise, src = 100 + i, None, None
+        else:
+            src = warm[i]
+            grid, nmu, nphi, wls = specs[src]
+            seed, noise = 100 + src, 0.05
+        for j, wl in enumerate(wls):
+            def make(grid=grid, nmu=nmu, nphi=nphi, wl=wl, seed=seed, noise=noise, i=i):
+                return S.make_rte(S.serve_medium(grid, nmu, nphi, wl, seed, noise, noise_seed=i,
+                                                 solacc=1e-4))
+            reqs.append(Request((i, j), make, warm_from=None if src is None else (src, j),
+                                render=view, arrival_s=0.01 * i, group=i))
+    fps = [r.make_rte().footprint_bytes() for r in reqs]
+    budget = 2 * max(fps)
+    res, st = SolveServer(nworkers=4, memory_budget=budget).run(reqs)
+    assert st["failed"] == 0, [r.error for r in res]
+    assert st["peak_device_bytes"] <= budget + max(fps)
+    for r in reqs:
+        r.arrival_s = 0.0
+    res1, st1 = SolveServer(nworkers=1).run(reqs)
+    assert st1["failed"] == 0, [r.error for r in res1]
+    for a, b in zip(res, res1):
+        assert (a.iterations, a.solcrit) == (b.iterations, b.solcrit), a.key
+        assert np.array_equal(a.radiance, b.radiance), a.key
+    print(f"served {st['solves']} solves at {st['solves_per_s']:.1f}/s, one at a time "
+          f"{st1['solves_per_s']:.1f}/s")

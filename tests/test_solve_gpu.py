@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from at3d_amd import synthetic as S
-from tests.scenes import blob8, cloud_small, rel_l2
+from tests.scenes import blob8, blob8_l15, cloud_l15, cloud_small, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -29,7 +29,8 @@ def _ref_dense(ref, st, which):
     return ref.dense_sh(st.rshptr, st.radiance, npts, nlm)
 
 
-@pytest.mark.parametrize("make", [lambda: blob8("open"), cloud_small], ids=["blob8_open", "cloud12_two_species"])
+@pytest.mark.parametrize("make", [lambda: blob8("open"), cloud_small, blob8_l15, cloud_l15],
+                         ids=["blob8_open", "cloud12_two_species", "blob8_open_l15", "cloud16_l15"])
 def test_sh_to_do_gemm_matches_numpy(make):
     """The two-species scene mixes points of 9 and of many source terms, so
     the level-ordered transform's K extent varies from tile to tile."""
@@ -45,7 +46,8 @@ def test_sh_to_do_gemm_matches_numpy(make):
     assert rel_l2(got, want) < 1e-6
 
 
-@pytest.mark.parametrize("make", [lambda: blob8("open"), cloud_small], ids=["blob8_open", "cloud12_two_species"])
+@pytest.mark.parametrize("make", [lambda: blob8("open"), cloud_small, blob8_l15, cloud_l15],
+                         ids=["blob8_open", "cloud12_two_species", "blob8_open_l15", "cloud16_l15"])
 def test_do_to_sh_gemm_matches_numpy(make):
     """Columns at or beyond each point's NR must read zero."""
     w = make()
@@ -60,8 +62,10 @@ def test_do_to_sh_gemm_matches_numpy(make):
     assert rel_l2(rte.state_array("radiance"), want) < 1e-6
 
 
-@pytest.mark.parametrize("make", [lambda: blob8("periodic"), lambda: blob8("open"), cloud_small],
-                         ids=["blob8_periodic", "blob8_open", "cloud12_two_species"])
+@pytest.mark.parametrize("make", [lambda: blob8("periodic"), lambda: blob8("open"), cloud_small, blob8_l15,
+                                  cloud_l15],
+                         ids=["blob8_periodic", "blob8_open", "cloud12_two_species", "blob8_open_l15",
+                              "cloud16_l15"])
 def test_init_source_matches_reference(ref, make):
     w = make()
     st = _ref_solve(ref, w, 0)
@@ -71,8 +75,10 @@ def test_init_source_matches_reference(ref, make):
     assert rel_l2(rte.state_array("source"), _ref_dense(ref, st, "source")) < 1e-6
 
 
-@pytest.mark.parametrize("make", [lambda: blob8("periodic"), lambda: blob8("open"), cloud_small],
-                         ids=["blob8_periodic", "blob8_open", "cloud12_two_species"])
+@pytest.mark.parametrize("make", [lambda: blob8("periodic"), lambda: blob8("open"), cloud_small, blob8_l15,
+                                  lambda: blob8_l15("periodic"), cloud_l15],
+                         ids=["blob8_periodic", "blob8_open", "cloud12_two_species", "blob8_open_l15",
+                              "blob8_periodic_l15", "cloud16_l15"])
 def test_one_iteration_matches_reference(ref, make):
     w = make()
     st = _ref_solve(ref, w, 1)
@@ -85,8 +91,10 @@ def test_one_iteration_matches_reference(ref, make):
 
 
 @pytest.mark.parametrize("make", [lambda: blob8("periodic"), lambda: blob8("open"), cloud_small,
-                                  lambda: S.workload("blob16")],
-                         ids=["blob8_periodic", "blob8_open", "cloud12_two_species", "blob16"])
+                                  lambda: S.workload("blob16"), lambda: S.workload("blob16_l7"), blob8_l15,
+                                  cloud_l15],
+                         ids=["blob8_periodic", "blob8_open", "cloud12_two_species", "blob16",
+                              "blob16_hg8_config1", "blob8_open_l15", "cloud16_l15"])
 def test_converged_solution_matches_reference(ref, make):
     w = make()
     st = _ref_solve(ref, w, 100)
@@ -128,9 +136,9 @@ def _large_cloud():
 
 
 @pytest.mark.parametrize("make", [lambda: blob8("periodic"), lambda: blob8("open"), cloud_small, _mid_cloud,
-                                  _large_cloud],
+                                  _large_cloud, blob8_l15, lambda: blob8_l15("periodic"), cloud_l15],
                          ids=["blob8_periodic", "blob8_open", "cloud12_two_species", "cloud48x48x32",
-                              "cloud96x96x48"])
+                              "cloud96x96x48", "blob8_open_l15", "blob8_periodic_l15", "cloud16_l15"])
 def test_level_ordered_sweep_matches_point_ordered_sweep(make):
     """The LDS-ring sweep over per-solve records (FP32 coefficients, level-ordered
     DO source, radiances written by whole 32-byte sectors where the layout
@@ -200,3 +208,12 @@ def test_wide_open_grid_keeps_every_step_in_its_class():
     rte.solve(0, verbose=False)
     assert rte.timing("sweep_ring") == 1.0
     assert rte.timing("sweep_general_points") == 0
+    # the class tolerance keeps the results: three iterations against the
+    # point-ordered FP64-geometry sweep
+    rte.solve(3, init_solution=False, verbose=False)
+    b = S.make_rte(w)
+    b.set_option("sweep_ring", 0)
+    b.solve(3, verbose=False)
+    assert rte.num_iterations == b.num_iterations == 3
+    for name in ("do_radiance", "source"):
+        assert rel_l2(rte.state_array(name), b.state_array(name)) < 2e-6, name
