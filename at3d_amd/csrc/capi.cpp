@@ -89,6 +89,8 @@ int64_t at3d_plan_size(const at3d_plan* h, const char* key) {
   const std::string k(key);
   if (k == "npts") return p.npts;
   if (k == "ncells") return p.ncells;
+  if (k == "nx") return p.nx;
+  if (k == "ny") return p.ny;
   if (k == "nx1") return p.nx1;
   if (k == "ny1") return p.ny1;
   if (k == "nz") return p.nz;
@@ -400,6 +402,57 @@ int at3d_render(at3d_state* st, int64_t nrays, const float* x, const float* y, c
   return rc;
 }
 
+int at3d_state_load(at3d_state* st, const int32_t* shptr, const float* source,
+                    const int32_t* rshptr, const float* radiance, const float* fluxes, char* err,
+                    size_t errlen) {
+  if (!st || !shptr || !rshptr || !radiance || !fluxes || !source) {
+    set_err(err, errlen, "null argument");
+    return AT3D_ERR_ARG;
+  }
+  std::string msg;
+  int rc;
+  try {
+    rc = at3d::state_load(st->s, shptr, source, rshptr, radiance, fluxes, msg);
+  } catch (const std::bad_alloc&) {
+    rc = AT3D_ERR_MEMORY;
+    msg = "out of host memory loading a solution";
+  }
+  if (rc) set_err(err, errlen, msg);
+  return rc;
+}
+
+int at3d_state_save(at3d_state* st, int32_t* shptr, int32_t* rshptr, float* source,
+                    int64_t source_cap, float* radiance, int64_t radiance_cap, float* fluxes,
+                    char* err, size_t errlen) {
+  if (!st || !shptr || !rshptr) {
+    set_err(err, errlen, "null argument");
+    return AT3D_ERR_ARG;
+  }
+  std::string msg;
+  int rc;
+  try {
+    rc = at3d::state_save(st->s, shptr, rshptr, source, source_cap, radiance, radiance_cap, fluxes,
+                          msg);
+  } catch (const std::bad_alloc&) {
+    rc = AT3D_ERR_MEMORY;
+    msg = "out of host memory saving a solution";
+  }
+  if (rc) set_err(err, errlen, msg);
+  return rc;
+}
+
+int at3d_state_sh_terms(at3d_state* st, int32_t nshout, float* shterms, float* netfluxdiv,
+                        char* err, size_t errlen) {
+  if (!st) {
+    set_err(err, errlen, "null state");
+    return AT3D_ERR_ARG;
+  }
+  std::string msg;
+  const int rc = at3d::state_sh_terms(st->s, nshout, shterms, netfluxdiv, msg);
+  if (rc) set_err(err, errlen, msg);
+  return rc;
+}
+
 int at3d_levis_gradient_jacobian(at3d_state* st, const at3d_deriv_desc* deriv, int64_t nrays,
                                  const float* x, const float* y, const float* z, const double* mu,
                                  const double* phi, int64_t npix, const int32_t* rays_per_pixel,
@@ -491,6 +544,8 @@ double at3d_state_timing(const at3d_state* st, const char* key) {
   if (k == "sweep_general_points") return s.ngeneral;
   if (k == "iters") return s.iters_last;
   if (k == "render") return s.render_ms;
+  if (k == "render_rays") return s.render_ray_ms;
+  if (k == "adjoint_rays") return s.adjoint_ms;
   if (k == "gradient") return s.grad_ms;
   return -1.0;
 }

@@ -500,7 +500,9 @@ int state_levis_gradient(State& s, const at3d_deriv_desc& dd, int64_t nrays, con
   A.jsrc = reinterpret_cast<double*>(B + o_js);
   A.jrad = reinterpret_cast<double*>(B + o_jr);
   A.jbeam = reinterpret_cast<double*>(B + o_jb);
+  (void)hipEventRecord(s.ev_ray[0], s.stream);
   launch_adjoint_rays(s.stream, A);
+  (void)hipEventRecord(s.ev_ray[1], s.stream);
   GradPointArgs G;
   G.npts = npts;
   G.ndir = ndir;
@@ -595,6 +597,8 @@ int state_levis_gradient(State& s, const at3d_deriv_desc& dd, int64_t nrays, con
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, s.ev[6], s.ev[7]);
   s.grad_ms = ms;
+  (void)hipEventElapsedTime(&ms, s.ev_ray[0], s.ev_ray[1]);
+  s.adjoint_ms = ms;
   if (ef) {
     err = (ef & 32)  ? "ADJOINT_INTEGRATE_1RAY: ray crossed too many cells without leaving the grid"
           : (ef & 16) ? "ADJOINT_INTEGRATE_1RAY: The maximum number of subgrid intervals for calculation "

@@ -4,6 +4,7 @@
 // point on the GPU and one ray per lane.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stdio.h>
 #include <string.h>
 
 #include <map>
@@ -206,7 +207,9 @@ int state_render(State& s, int64_t nrays, const float* x, const float* y, const 
   a.nosurface = nosurface;
   a.out = reinterpret_cast<float*>(B + o_out);
   a.errflag = s.errflag;
+  (void)hipEventRecord(s.ev_ray[0], s.stream);
   launch_render(s.stream, a);
+  (void)hipEventRecord(s.ev_ray[1], s.stream);
   (void)hipEventRecord(s.ev[7], s.stream);
   HIPCHK(hipGetLastError());
   int ef = 0;
@@ -216,6 +219,8 @@ int state_render(State& s, int64_t nrays, const float* x, const float* y, const 
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, s.ev[6], s.ev[7]);
   s.render_ms = ms;
+  (void)hipEventElapsedTime(&ms, s.ev_ray[0], s.ev_ray[1]);
+  s.render_ray_ms = ms;
   if (ef) {
     err = (ef & 1)   ? "INTEGRATE_1RAY: SO<0"
           : (ef & 2) ? "FIND_BOUNDARY_RADIANCE: Not at boundary"

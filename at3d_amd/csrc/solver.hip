@@ -538,6 +538,7 @@ int state_create(const Plan& p, State& s, std::string& err) {
   TRY(setup_ring(p, s, err));
   for (int e = 0; e < State::NEV; e++) HIPCHK(hipEventCreate(&s.ev[e]));
   for (int e = 0; e < 4; e++) HIPCHK(hipEventCreate(&s.ev_sweep[e]));
+  for (int e = 0; e < 2; e++) HIPCHK(hipEventCreate(&s.ev_ray[e]));
   return AT3D_OK;
 }
 
@@ -563,6 +564,8 @@ void state_free(State& s) {
     if (s.ev[e]) (void)hipEventDestroy(s.ev[e]);
   for (int e = 0; e < 4; e++)
     if (s.ev_sweep[e]) (void)hipEventDestroy(s.ev_sweep[e]);
+  for (int e = 0; e < 2; e++)
+    if (s.ev_ray[e]) (void)hipEventDestroy(s.ev_ray[e]);
   if (s.host_dots) (void)hipHostFree(s.host_dots);
   s.host_dots = nullptr;
   if (s.stream) (void)hipStreamDestroy(s.stream);
@@ -617,10 +620,25 @@ int state_init_solution(State& s, std::string& err) {
   return AT3D_OK;
 }
 
+// Same base grid (sizes, boundaries and the coordinates bit for bit) and the
+// same spherical-harmonic layout (ML, MM), so that point i and term J mean
+// the same in both states.
+static bool same_grid_and_layout(const Plan& p, const Plan& q) {
+  return p.npts == q.npts && p.ncells == q.ncells && p.nx1 == q.nx1 && p.ny1 == q.ny1 &&
+         p.nz == q.nz && p.bcflag == q.bcflag && p.ipflag == q.ipflag && p.ml == q.ml &&
+         p.mm == q.mm && p.nlm == q.nlm && p.xgrid == q.xgrid && p.ygrid == q.ygrid &&
+         p.zgrid == q.zgrid;
+}
+
+// Warm start from another state's solution: its spherical-harmonic radiance
+// (with its term counts) and hemispherical fluxes, as a saved solution
+// carries them (a render before this state's first iteration takes the
+// surface's downwelling flux from them), then the first source function of
+// this state's medium.
 int state_init_solution_from(State& s, State& src, std::string& err) {
   const Plan& p = *s.plan;
   const Plan& q = *src.plan;
-  if (!src.initialized || p.npts != q.npts || p.nlm != q.nlm || s.ld != src.ld) {
+  if (!src.initialized || !same_grid_and_layout(p, q) || s.ld != src.ld) {
     err = "warm start needs an initialised source state on the same grid and angular resolution";
     return AT3D_ERR_ARG;
   }
@@ -628,6 +646,8 @@ int state_init_solution_from(State& s, State& src, std::string& err) {
   HIPCHK(hipStreamSynchronize(src.stream));
   HIPCHK(hipMemcpyAsync(s.R, src.R, nsh * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
   HIPCHK(hipMemcpyAsync(s.nr, src.nr, p.npts * sizeof(int), hipMemcpyDeviceToDevice, s.stream));
+  HIPCHK(hipMemcpyAsync(s.fluxes, src.fluxes, 2 * (size_t)p.npts * sizeof(float),
+                        hipMemcpyDeviceToDevice, s.stream));
   launch_compute_source(s.stream, source_args(s, 1, 0), s.nsrc_blocks);
   s.nr_next_fixsh = false;
   HIPCHK(hipGetLastError());
@@ -835,7 +855,10 @@ int state_solve(State& s, int maxiter, int verbose, std::string& err) {
       fflush(stdout);
     }
   }
-  launch_fluxes(s.stream, s.dorad, s.nptsp, s.col_of_ref_d, s.fluxw, s.ndown, p.nang, p.npts, s.fluxes);
+  // hemispherical fluxes of the last path integration (a call that ran no
+  // iteration keeps the ones the state holds)
+  if (s.iters_last > 0)
+    launch_fluxes(s.stream, s.dorad, s.nptsp, s.col_of_ref_d, s.fluxw, s.ndown, p.nang, p.npts, s.fluxes);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s.stream));
   return AT3D_OK;

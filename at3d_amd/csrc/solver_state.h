@@ -100,6 +100,7 @@ struct State {
   static const int NEV = 8;
   hipEvent_t ev[NEV] = {};
   hipEvent_t ev_sweep[4] = {};  // around the two ring sweep launches of an iteration
+  hipEvent_t ev_ray[2] = {};    // around the ray kernel of the last render / gradient call
   double tsum[8] = {};  // ms per kernel class over the last solve
   std::vector<std::pair<void*, size_t>> allocs;   // device blocks and their sizes
   size_t device_bytes = 0;   // bytes of the allocations in `allocs`
@@ -114,6 +115,8 @@ struct State {
   int max_subgrid = 0;     // extra sub-step intervals a gradient ray may record
   double grad_ms = 0.0;
   double render_ms = 0.0;
+  double render_ray_ms = 0.0;   // k_render alone in the last render call
+  double adjoint_ms = 0.0;      // k_adjoint_rays alone in the last gradient call
 };
 
 struct DirTables {
@@ -139,6 +142,12 @@ int state_render(State& s, int64_t nrays, const float* x, const float* y, const 
                  const double* mu, const double* phi, int single_scatter, int nosurface,
                  float* out, std::string& err);
 
+
+int state_load(State& s, const int32_t* shptr, const float* source, const int32_t* rshptr,
+               const float* radiance, const float* fluxes, std::string& err);
+int state_save(State& s, int32_t* shptr, int32_t* rshptr, float* source, int64_t source_cap,
+               float* radiance, int64_t radiance_cap, float* fluxes, std::string& err);
+int state_sh_terms(State& s, int nshout, float* shterms, float* netfluxdiv, std::string& err);
 
 int state_create(const Plan& p, State& s, std::string& err);
 size_t plan_footprint_bytes(const Plan& p);

@@ -50,6 +50,7 @@ class RTE:
         self._iters = 0
         self._state = None
         self._plan = None
+        self._restore_data = None   # a loaded solution (load_solution)
         self._keep = []          # arrays referenced by the C descriptor
         self._build_plan()
 
@@ -195,7 +196,10 @@ class RTE:
         L = _lib.lib()
         err = _lib.errbuf()
         if init_solution or self._solcrit is None:
-            _lib.check(L.at3d_init_solution(self._state, err, len(err)), err)
+            if self._restore_data is not None:
+                self._load_restore_data()
+            else:
+                _lib.check(L.at3d_init_solution(self._state, err, len(err)), err)
             self._solcrit, self._iters = 1.0, 0
         if not solve:
             return
@@ -218,6 +222,170 @@ class RTE:
         err = _lib.errbuf()
         _lib.check(_lib.lib().at3d_init_solution_from(self._state, other._state, err, len(err)), err)
         self._solcrit, self._iters = 1.0, 0
+
+    # -- saved solutions (solver.py:1519-1686) -----------------------------------
+    def save_solution(self, save_radiances=True):
+        """The grid and, with ``save_radiances``, the spherical-harmonic source
+        and radiance of the current solution, under the variable names and
+        layouts of the reference's saved-solution dataset (solver.py:1621-1686;
+        Fortran order: gridpos (3, npts), gridptr (8, ncells) and neighptr
+        (6, ncells) 1-based, source (nstokes, SHPTR[npts]), radiance
+        (nstokes, RSHPTR[npts])).  A dict of numpy arrays (xarray is not used)."""
+        npts, ncells = self._npts, self.plan_size("ncells")
+        out = {"nstokes": self._nstokes, "nx": self.plan_size("nx"), "ny": self.plan_size("ny"),
+               "nz": self.plan_size("nz"), "ml": self._ml, "mm": self._mm, "nlm": self._nlm,
+               "npts": npts, "ncells": ncells, "nbcells": ncells,
+               "xgrid": self.plan_array("xgrid"), "ygrid": self.plan_array("ygrid"),
+               "zgrid": self.plan_array("zgrid"),
+               "gridpos": np.asfortranarray(self.plan_array("gridpos").T),
+               "gridptr": np.asfortranarray(self.plan_array("gridptr").T + 1),
+               "neighptr": np.asfortranarray(self.plan_array("neighptr").T + 1),
+               "treeptr": np.zeros((2, ncells), np.int32, order="F"),
+               "cellflags": self.plan_array("cellflags").astype(np.int16)}
+        if save_radiances:
+            if self._state is None or self._solcrit is None:
+                raise RuntimeError("no solution to save: call solve() first")
+            shptr = np.zeros(npts + 1, np.int32)
+            rshptr = np.zeros(npts + 2, np.int32)
+            fluxes = np.zeros((2, npts), np.float32)
+            L, err = _lib.lib(), _lib.errbuf()
+            _lib.check(L.at3d_state_save(self._state, shptr.ctypes.data, rshptr.ctypes.data, None, 0,
+                                         None, 0, None, err, len(err)), err)
+            source = np.zeros((1, int(shptr[npts])), np.float32, order="F")
+            radiance = np.zeros((1, int(rshptr[npts])), np.float32, order="F")
+            _lib.check(L.at3d_state_save(self._state, shptr.ctypes.data, rshptr.ctypes.data,
+                                         source.ctypes.data, source.size, radiance.ctypes.data,
+                                         radiance.size, fluxes.ctypes.data, err, len(err)), err)
+            out.update(fluxes=fluxes, shptr=shptr, rshptr=rshptr, source=source, radiance=radiance)
+        return out
+
+    def load_solution(self, input_dataset, load_radiance=True):
+        """Take the grid and spherical-harmonic solution of a saved solution
+        (``save_solution`` here or the reference's dataset as a mapping of
+        arrays, or an ``.npz`` path); the next ``solve(init_solution=True)``
+        starts from it instead of the two-stream initialisation
+        (solver.py:1519-1619, 2654-2664).  Raises ValueError for a grid or
+        spherical-harmonic layout inconsistent with this solver."""
+        ds = input_dataset
+        if isinstance(ds, (str, bytes)) or hasattr(ds, "__fspath__"):
+            with np.load(ds, allow_pickle=False) as f:
+                ds = {k: f[k] for k in f.files}
+        val = lambda k: np.asarray(ds[k]).item() if np.ndim(ds[k]) == 0 else np.asarray(ds[k])
+        nx, ny, nz = self.plan_size("nx"), self.plan_size("ny"), self.plan_size("nz")
+        if (val("nx"), val("ny"), val("nz")) != (nx, ny, nz):
+            raise ValueError(
+                "Incompatible grid sizes in loaded solution. self: (nx={}, ny={}, nz={}). "
+                "!= loaded: (nx={}, ny={}, nz={}).".format(nx, ny, nz, val("nx"), val("ny"), val("nz")))
+        for g in ("xgrid", "ygrid", "zgrid"):
+            a, b = np.asarray(ds[g], np.float32), self.plan_array(g)
+            if a.shape != b.shape or np.any(a != b):
+                raise ValueError("Incompatible grids (xgrid, ygrid, zgrid) between loaded solution and self.")
+        if int(val("npts")) != self._npts or int(val("ncells")) != self.plan_size("ncells"):
+            raise ValueError("loaded grid has {} points and {} cells, this solver's base grid {} and {}: "
+                             "grids refined by adaptive cell splitting are not supported"
+                             .format(int(val("npts")), int(val("ncells")), self._npts,
+                                     self.plan_size("ncells")))
+        if not load_radiance:
+            return
+        if int(val("nstokes")) != self._nstokes:
+            raise ValueError("Incompatible number of Stokes components in loaded solution. self: "
+                             "nstokes={} != loaded: nstokes={}".format(self._nstokes, int(val("nstokes"))))
+        if int(val("ml")) > self._ml or int(val("mm")) > self._mm:
+            raise ValueError("Incompatible SphericalHarmonics in loaded solution. self: (ml={}, mm={}) "
+                             "< loaded: (ml={}, mm={}).".format(self._ml, self._mm, int(val("ml")),
+                                                               int(val("mm"))))
+        if (int(val("ml")), int(val("mm"))) != (self._ml, self._mm):
+            raise ValueError("a loaded solution must have this solver's (ml, mm) = ({}, {})"
+                             .format(self._ml, self._mm))
+        for name in ("radiance", "source", "fluxes", "shptr", "rshptr"):
+            if name not in ds:
+                raise KeyError("`input_dataset` does not appear to contain variable '{}' "
+                               "which is required.".format(name))
+        self._restore_data = {k: np.asarray(ds[k]) for k in ("radiance", "source", "fluxes", "shptr",
+                                                             "rshptr")}
+
+    def _load_restore_data(self):
+        d, npts = self._restore_data, self._npts
+        shptr = _lib.as_i32(d["shptr"][:npts + 1])
+        rshptr = np.zeros(npts + 2, np.int32)
+        rshptr[:npts + 1] = d["rshptr"][:npts + 1]
+        rshptr[npts + 1] = rshptr[npts]
+        source = _lib.as_f32(np.ravel(d["source"])[:int(shptr[npts])])
+        radiance = _lib.as_f32(np.ravel(d["radiance"])[:int(rshptr[npts])])
+        fluxes = _lib.as_f32(np.asarray(d["fluxes"])[:, :npts])
+        if source.size < shptr[npts] or radiance.size < rshptr[npts]:
+            raise ValueError("loaded source / radiance hold fewer terms than their pointers give")
+        err = _lib.errbuf()
+        _lib.check(_lib.lib().at3d_state_load(self._state, shptr.ctypes.data, source.ctypes.data,
+                                              rshptr.ctypes.data, radiance.ctypes.data,
+                                              fluxes.ctypes.data, err, len(err)), err)
+
+    # -- diagnostics on the base grid (solver.py:1037-1262) -----------------------
+    def _base_grid(self, arr):
+        return arr[..., :self._npts].reshape(arr.shape[:-1] + (self.plan_size("nx1"),
+                                                                self.plan_size("ny1"),
+                                                                self.plan_size("nz")))
+
+    def _coords(self):
+        return {"x": self.plan_array("xgrid"), "y": self.plan_array("ygrid"),
+                "z": self.plan_array("zgrid")}
+
+    def _sh_terms(self):
+        nshout = 5 if bool(self.numerical_params["high_order_radiance"]) else 4
+        sh = np.zeros((nshout, self._npts), np.float32)
+        nfd = np.zeros(self._npts, np.float32)
+        err = _lib.errbuf()
+        _lib.check(_lib.lib().at3d_state_sh_terms(self._state, nshout, sh.ctypes.data, nfd.ctypes.data,
+                                                  err, len(err)), err)
+        return sh, nfd
+
+    @property
+    def adaptive_shterms(self):
+        """Mean radiance and net fluxes (Fx, Fy, Fz) at every grid point
+        (COMPUTE_SH), [nshout, npts]."""
+        self._ensure_state()
+        return self._sh_terms()[0]
+
+    @property
+    def spherical_harmonics(self):
+        """Mean intensity and net fluxes Fx, Fy, Fz on the base grid, plus
+        'rms_higher_rad' with high_order_radiance (solver.py:1037-1121)."""
+        self.check_solved()
+        sh, _ = self._sh_terms()
+        out = {"mean_intensity": self._base_grid(sh[0]), "Fx": self._base_grid(sh[1]),
+               "Fy": self._base_grid(sh[2]), "Fz": self._base_grid(sh[3])}
+        if sh.shape[0] == 5:
+            out["rms_higher_rad"] = self._base_grid(sh[4]) / np.sqrt(np.pi * 4.0 * self._nlm)
+        out.update(self._coords())
+        return out
+
+    @property
+    def adaptive_fluxes(self):
+        return self.state_array("fluxes")
+
+    @property
+    def fluxes(self):
+        """Down- and upwelling hemispherical fluxes and the direct solar flux on
+        the base grid (solver.py:1148-1198)."""
+        self.check_solved()
+        f = self.state_array("fluxes")
+        out = {"flux_down": self._base_grid(f[0]), "flux_up": self._base_grid(f[1]),
+               "flux_direct": self._base_grid(self.state_array("dirflux"))}
+        out.update(self._coords())
+        return out
+
+    @property
+    def adaptive_net_flux_div(self):
+        self._ensure_state()
+        return self._sh_terms()[1]
+
+    @property
+    def net_flux_divergence(self):
+        """Net flux divergence on the base grid (solver.py:1207-1262)."""
+        self.check_solved()
+        out = {"net_flux_div": self._base_grid(self._sh_terms()[1])}
+        out.update(self._coords())
+        return out
 
     def integrate_to_sensor(self, sensor, single_scatter=False, nosurface=False):
         """Radiance of every ray of ``sensor`` (solver.py:633-791).

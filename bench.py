@@ -34,28 +34,98 @@ HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 FP32_MFMA_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
 
 
-def cpu_baseline(workload_name: str, iters_gpu: int, niter: int):
-    """Times the reference Fortran (oracle/_ref) on one host core on the same
-    workload: setup, INIT_SOLUTION and `niter` SOLUTION_ITERATIONS (0: as many
-    as the GPU solve took, i.e. one whole solve), scaled to one converged solve
-    of `iters_gpu` iterations."""
+def cpu_description():
+    """Host CPU model, and the reference build's compiler and flags."""
+    import platform
+    import subprocess
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    comp = "amdflang"
+    try:
+        out = subprocess.run(["/opt/rocm/bin/amdflang", "--version"], capture_output=True, text=True,
+                             timeout=20).stdout.splitlines()
+        comp = out[0].strip() if out else comp
+    except Exception:
+        pass
+    return {"cpu": model, "compiler": comp, "flags": "-O2 -fPIC (oracle/Makefile)"}
+
+
+def cpu_baseline(workload_name: str, runs: int):
+    """The reference Fortran (oracle/_ref) on one host core: setup,
+    INIT_SOLUTION and SOLUTION_ITERATIONS to its own convergence on the same
+    workload, `runs` times; the median run is the baseline.  Returns the
+    baseline and the last run's converged state (for the render leg)."""
     from oracle import ref_shdom as R
     from at3d_amd import synthetic as S
+    if not R.available() or runs <= 0:
+        return None, None
+    w = S.workload(workload_name)
+    times, st = [], None
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        st = R.setup(R.scene_from_workload(w))
+        R.init_solution(st)
+        R.solution_iterations(st, w.maxiter)
+        times.append(time.perf_counter() - t0)
+        print(f"bench: reference solve {len(times)}/{runs}: {times[-1]:.1f} s, {st.iters} iterations",
+              file=sys.stderr, flush=True)
+    med = float(np.median(times))
+    out = {"value": 1.0 / med, "unit": "solves/s", "cores": 1, "kind": "reference",
+           "sample": f"reference Fortran on one host core: setup + INIT_SOLUTION + SOLUTION_ITERATIONS "
+                     f"to its own convergence ({st.iters} iterations, SOLCRIT {st.solcrit:.2e}) of the "
+                     f"same workload, median of {runs} runs",
+           "runs_s": times, "iterations": int(st.iters)}
+    out.update(cpu_description())
+    return out, st
+
+
+def cpu_render(st, rays):
+    """The reference RENDER of the same rays from its converged solution, one host core."""
+    from oracle import ref_shdom as R
+    t0 = time.perf_counter()
+    R.render(st, rays["ray_x"], rays["ray_y"], rays["ray_z"], rays["ray_mu"], rays["ray_phi"])
+    return time.perf_counter() - t0
+
+
+def gradient_case():
+    """The differentiation workload: the 16^3 lognormal blob (NMU 8 / NPHI 16,
+    64-term HG(0.85)), 3 views of 32 x 32 pixels, measurements rendered from
+    the same generator with seed 1; unknowns extinction and albedo."""
+    from at3d_amd import synthetic as S
+    w = S.workload("blob16")
+    target = S.lognormal_blob(n=16, nleg=64, seed=1)
+    rays = S.workload_views("blob16", w)
+    t = S.make_rte(target)
+    t.solve(target.maxiter, verbose=False)
+    meas = t.integrate_to_sensor(dict(rays))["I"].astype(np.float32)
+    t.release()
+    return w, rays, meas
+
+
+def cpu_gradient(w, rays, meas):
+    """The reference LEVISAPPROX_GRADIENT of the differentiation workload on one
+    host core (its solve untimed)."""
+    from oracle import ref_shdom as R
     if not R.available():
         return None
-    w = S.workload(workload_name)
-    t0 = time.perf_counter()
     st = R.setup(R.scene_from_workload(w))
     R.init_solution(st)
-    t1 = time.perf_counter()
-    R.solution_iterations(st, niter if niter > 0 else iters_gpu)
-    t2 = time.perf_counter()
-    per_iter = (t2 - t1) / max(st.iters, 1)
-    solve_s = (t1 - t0) + per_iter * iters_gpu
-    return {"value": 1.0 / solve_s, "unit": "solves/s", "cores": 1, "kind": "reference",
-            "sample": f"reference Fortran (amdflang -O2) setup+INIT_SOLUTION {t1 - t0:.2f} s and "
-                      f"{st.iters} SOLUTION_ITERATIONS at {per_iter:.2f} s each on one host core; "
-                      f"one converged solve of the GPU's {iters_gpu} iterations = {solve_s:.1f} s"}
+    R.solution_iterations(st, w.maxiter)
+    shape = w.shape
+    nleg = w.species[0][2].shape[0]
+    R.prepare_derivatives(st, [
+        R.Derivative(0, np.ones(shape), np.zeros(shape), np.zeros((nleg, 1)), np.ones(shape, np.int32)),
+        R.Derivative(0, np.zeros(shape), np.ones(shape), np.zeros((nleg, 1)), np.ones(shape, np.int32))])
+    t0 = time.perf_counter()
+    R.levis_gradient(st, rays["ray_x"], rays["ray_y"], rays["ray_z"], rays["ray_mu"], rays["ray_phi"], meas)
+    return time.perf_counter() - t0
 
 
 # kernel class -> device kernel name in the rocprofv3 counter summaries
@@ -114,15 +184,98 @@ def serving_stream_bench(n_requests, rate_hz, workers):
             "same_iterations_and_solcrit_as_one_at_a_time": same}
 
 
+def cold_solves(n, maxiter):
+    """`n` solves of new media on the headline geometry (the cloud field
+    re-drawn with seeds 1..n): host plan of the new medium (the sweep levels
+    of the geometry are cached), device state, records pass, initialisation
+    and iterations to convergence, each timed whole."""
+    from at3d_amd import synthetic as S
+    import torch
+    out = []
+    for k in range(1, n + 1):
+        wk = S.cloud_field(z0=0.35, seed=k)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = S.make_rte(wk)
+        t1 = time.perf_counter()
+        r.solve(maxiter, verbose=False)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        out.append({"plan_ms": (t1 - t0) * 1e3, "state_and_solve_ms": (t2 - t1) * 1e3,
+                    "total_ms": (t2 - t0) * 1e3, "iterations": r.num_iterations,
+                    "device_ms": r.timing("total")})
+        r.release()
+    return out
+
+
+def render_leg(rte, w, reps):
+    """The headline workload's 9 views of 64 x 64 pixels rendered from its
+    converged solution `reps` times (after one untimed call)."""
+    from at3d_amd import synthetic as S
+    import torch
+    rays = S.workload_views("cloud64", w)
+    rte.integrate_to_sensor(dict(rays))
+    torch.cuda.synchronize()
+    wall, dev, kr = [], [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        rte.integrate_to_sensor(dict(rays))
+        wall.append((time.perf_counter() - t0) * 1e3)
+        dev.append(rte.timing("render"))
+        kr.append(rte.timing("render_rays"))
+    nrays = rays["ray_x"].size
+    npts, nlm = rte._npts, rte._nlm
+    ndir = 9
+    # algorithmic bytes: the direction source reads each point's source terms
+    # once and writes one value per (point, view); the rays read each point's
+    # extinction and direction source once per view and read / write their
+    # own 36 bytes (x, y, z, mu, phi, radiance)
+    ns = rte.state_array("ns").astype(np.float64)
+    b_dir = 4.0 * ns.sum() + 4.0 * ndir * npts
+    b_rays = 8.0 * ndir * npts + 36.0 * nrays
+    ms_rays = float(np.median(kr))
+    return rays, {"views": ndir, "rays": nrays, "wall_ms": float(np.median(wall)),
+                  "device_ms": float(np.median(dev)), "ray_kernel_ms": ms_rays,
+                  "rays_per_s": nrays / (float(np.median(wall)) * 1e-3),
+                  "ray_kernel_bytes": b_rays, "ray_kernel_GB/s": b_rays / ms_rays / 1e6,
+                  "ray_kernel_frac_hbm": b_rays / ms_rays / 1e6 / HBM_PEAK_GBS,
+                  "direction_source_bytes": b_dir}
+
+
+def gradient_leg(w, rays, meas, reps):
+    """The Levis gradient of the differentiation workload (solve untimed),
+    `reps` times after one untimed call."""
+    from at3d_amd import gradient as GR
+    from at3d_amd import synthetic as S
+    rte = S.make_rte(w)
+    rte.solve(w.maxiter, verbose=False)
+    sp = list(rte.medium)[0]
+    unknowns = [(sp, "extinction"), (sp, "ssalb")]
+    sensor = dict(rays, measurement_data=meas)
+    GR.levis_approximation_grad(rte, sensor, unknowns)
+    wall, dev, adj = [], [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        GR.levis_approximation_grad(rte, sensor, unknowns)
+        wall.append((time.perf_counter() - t0) * 1e3)
+        dev.append(rte.timing("gradient"))
+        adj.append(rte.timing("adjoint_rays"))
+    rte.release()
+    return {"pixels": int(meas.size), "unknowns": 2, "wall_ms": float(np.median(wall)),
+            "device_ms": float(np.median(dev)), "adjoint_ray_kernel_ms": float(np.median(adj))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cloud64")
-    ap.add_argument("--cpu-iters", type=int, default=0,
-                    help="reference iterations timed for cpu_baseline (0: the GPU solve's count)")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-runs", type=int, default=3,
+                    help="reference solves timed for cpu_baseline (the median is reported; 0: skip)")
+    ap.add_argument("--cold-steps", type=int, default=3, help="solves of new media timed whole")
+    ap.add_argument("--render-reps", type=int, default=3)
+    ap.add_argument("--gradient-reps", type=int, default=3)
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--stream-requests", type=int, default=48,
                     help="requests of the mixed-size serving stream (0: skip it)")
@@ -159,6 +312,7 @@ def main():
     for _ in range(args.warmup):
         rte.solve(w.maxiter, verbose=False)
     torch.cuda.synchronize()
+    print("bench: warm-up done", file=sys.stderr, flush=True)
 
     keys = ["truncation", "sh_to_do", "sweep", "sweep_kernel", "do_to_sh", "source", "total"]
     ksum = {k: 0.0 for k in keys}
@@ -178,6 +332,7 @@ def main():
     elapsed = time.perf_counter() - t_start
     ms_per_step = elapsed * 1e3 / args.steps
     iters = iters_total // args.steps
+    print("bench: plain solves done", file=sys.stderr, flush=True)
 
     # served: the same number of solves over several solvers in flight, one
     # stream and host thread each, each taking the next solve when it is free
@@ -212,10 +367,13 @@ def main():
     served_elapsed = time.perf_counter() - ts0
     served_n = args.steps
     served_value = served_n / served_elapsed
+    for r in servers[1:]:
+        r.release()
+    print("bench: served solves done", file=sys.stderr, flush=True)
 
     npts, nang, nlm = rte._npts, rte._nang, rte._nlm
-    # algorithmic bytes / flops per launch of each kernel class
-    gemm_flops = 2.0 * nang * npts * nlm
+    ns = rte.state_array("ns").astype(np.float64)
+    nr = rte.state_array("nr").astype(np.float64)
     kern = {}
     n_it = max(iters_total, 1)
     # one sweep pass = one launch per hemisphere (a workgroup per ordinate walking
@@ -233,26 +391,27 @@ def main():
                      "level_ordered": bool(rte.timing("sweep_ring")),
                      "lds_bytes": int(rte.timing("sweep_ring_lds")),
                      "multi_cell_points": int(rte.timing("sweep_general_points"))}
-    # dense-equivalent rates: both transforms skip terms beyond each point's
-    # SH truncation (exact zeros), so they execute fewer flops than 2 N M K
-    for k in ("sh_to_do", "do_to_sh"):
+    # the transforms on the terms that can be nonzero (each point's source
+    # truncation NS for SH->DO, radiance truncation NR for DO->SH): useful
+    # flops 2 NANG sum(NS) and 2 NANG sum(NR), bytes one read of those terms,
+    # one read or write of NANG values per point
+    for k, cnt in (("sh_to_do", ns), ("do_to_sh", nr)):
         ms = ksum[k] / n_it
-        kern[k] = {"ms_per_launch": ms, "dense_flops_per_launch": gemm_flops,
-                   "dense_equiv_TFLOP/s": gemm_flops / ms / 1e9}
+        fl = 2.0 * nang * cnt.sum()
+        by = 4.0 * (cnt.sum() + nang * npts)
+        kern[k] = {"ms_per_launch": ms, "useful_flops_per_launch": fl,
+                   "useful_TFLOP/s": fl / ms / 1e9, "bytes_per_launch": by, "GB/s": by / ms / 1e6}
+    # the source pass reads R (NR terms), the old S (NS) and DS, writes S and
+    # DS, and reads each point's Legendre terms and scalars
     ms_src = ksum["source"] / n_it
-    src_bytes = npts * nlm * 4 * 6.0       # R read, S read/write, DS read/write, chi/ysun
+    src_bytes = 4.0 * (nr.sum() + 4.0 * ns.sum()) + npts * 4.0 * (rte._ml + 1 + 8)
     kern["source"] = {"ms_per_launch": ms_src, "bytes_per_launch": src_bytes,
                       "GB/s": src_bytes / ms_src / 1e6}
     dominant = max(("sweep", "sh_to_do", "do_to_sh", "source"), key=lambda k: ksum[k])
     traffic = measured_traffic(dominant)
-    if dominant in ("sh_to_do", "do_to_sh"):
-        ach = kern[dominant]["dense_equiv_TFLOP/s"]
-        roof = {"kernel": dominant, "bound": "mfma", "achieved": ach, "peak": FP32_MFMA_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": ach / FP32_MFMA_PEAK_TFLOPS, "traffic": traffic}
-    else:
-        ach = kern[dominant]["GB/s"]
-        roof = {"kernel": dominant, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic}
+    ach = kern[dominant]["GB/s"]
+    roof = {"kernel": dominant, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic}
 
     # whole-solve algorithmic traffic (minimum bytes of a dense FP32 layout per
     # iteration: source r/w with acceleration 5 NLM, SH->DO NLM + NANG, sweep
@@ -264,15 +423,30 @@ def main():
     solve_roof = {"bytes_per_solve": b_solve, "bytes_per_iteration": b_iter, "GB/s": solve_gbs,
                   "frac": solve_gbs / HBM_PEAK_GBS}
 
+    cold = cold_solves(args.cold_steps, w.maxiter) if args.cold_steps > 0 else []
+    print("bench: cold solves done", file=sys.stderr, flush=True)
+    rays, render = render_leg(rte, w, args.render_reps)
+    print("bench: render done", file=sys.stderr, flush=True)
+    gw, grays, gmeas = gradient_case()
+    grad = gradient_leg(gw, grays, gmeas, args.gradient_reps)
+    print("bench: gradient done", file=sys.stderr, flush=True)
+
     stream = None
     if args.stream_requests > 0:
         stream = serving_stream_bench(args.stream_requests, args.stream_rate, args.stream_workers)
+    print("bench: serving stream done", file=sys.stderr, flush=True)
 
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args.workload, iters, args.cpu_iters)
+    cpu, ref_st = cpu_baseline(args.workload, args.cpu_runs)
+    if cpu is not None:
+        render["cpu_reference_s"] = cpu_render(ref_st, rays)
+        render["speedup_vs_cpu"] = render["cpu_reference_s"] * 1e3 / render["wall_ms"]
+        g_cpu = cpu_gradient(gw, grays, gmeas)
+        if g_cpu is not None:
+            grad["cpu_reference_s"] = g_cpu
+            grad["speedup_vs_cpu"] = g_cpu * 1e3 / grad["wall_ms"]
 
     plain_value = args.steps / elapsed
+    cold_ms = float(np.median([c["total_ms"] for c in cold])) if cold else None
     value = served_value
     line = {
         "metric": "converged solves/s @64x64x40 L=15 (served vs plain); p50/p99 latency; %HBM roof",
@@ -290,13 +464,17 @@ def main():
         "plain": {"solves_per_s": plain_value, "solves": args.steps, "ms_per_solve": ms_per_step,
                   "latency_ms": {"p50": float(np.percentile(lat, 50)),
                                  "p99": float(np.percentile(lat, 99))}},
+        "cold": {"solves": cold, "median_ms": cold_ms},
         "kernel_ms_per_solve": {k: ksum[k] / args.steps for k in keys},
         "kernels": kern,
         "roofline": roof,
         "solve_roofline": solve_roof,
+        "render": render,
+        "gradient": grad,
         "serving_stream": stream,
         "cpu_baseline": cpu,
-        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "speedup_vs_cpu": {"cold_solve": (1e3 / cold_ms / cpu["value"]) if (cpu and cold_ms) else None,
+                           "plain_solve": (plain_value / cpu["value"]) if cpu else None},
     }
     print(json.dumps(line))
 

@@ -91,7 +91,7 @@ int64_t at3d_device_trim(void);
 int at3d_plan_create(const at3d_scene_desc* desc, at3d_plan** out, char* err, size_t errlen);
 void at3d_plan_destroy(at3d_plan* plan);
 
-/* Integer sizes of a plan: keys "npts","ncells","nx1","ny1","nz","nlm","ml","mm",
+/* Integer sizes of a plan: keys "npts","ncells","nx","ny","nx1","ny1","nz","nlm","ml","mm",
  * "nmu","nang","nphi0max","bcflag","ipflag","ntop","nbot","nlevels_max",
  * "npart","numphase","nleg","nlegp","nscatangle"; "footprint_bytes": the
  * device bytes a state of this plan will allocate (admission by memory
@@ -151,6 +151,35 @@ int at3d_state_get(at3d_state* st, const char* name, void* dst, size_t nbytes, c
 int at3d_render(at3d_state* st, int64_t nrays, const float* x, const float* y, const float* z,
                 const double* mu, const double* phi, int32_t single_scatter, int32_t nosurface,
                 float* out, char* err, size_t errlen);
+
+/* Solution transfer in the reference's saved-solution format
+ * (RTE.save_solution / RTE.load_solution, at3d/solver.py:1519-1686; SAVE_STATE /
+ * RESTORE_STATE of SHDOM): ragged spherical-harmonic rows, 0-based offsets.
+ *   shptr[npts+1], source[shptr[npts]]      SOURCE terms of each point
+ *   rshptr[npts+2], radiance[rshptr[npts]]  RADIANCE terms (rshptr[npts+1] = rshptr[npts])
+ *   fluxes[2][npts]                         hemispherical fluxes (down, up)
+ * at3d_state_load makes them the state's solution as INIT_SOLUTION does with
+ * INRADFLAG false (shdomsub1.f:356, reached from RTE._init_solution after
+ * load_solution, solver.py:2654-2664): no two-stream start and no first
+ * source pass, DELSOURCE zero; the next at3d_solve continues from it.  Term
+ * counts must be 0..nlm (source) and 1..nlm (radiance) per point.
+ * at3d_state_save writes shptr and rshptr always, and source / radiance /
+ * fluxes when non-null (capacities in floats; too small is AT3D_ERR_ARG). */
+int at3d_state_load(at3d_state* st, const int32_t* shptr, const float* source,
+                    const int32_t* rshptr, const float* radiance, const float* fluxes, char* err,
+                    size_t errlen);
+int at3d_state_save(at3d_state* st, int32_t* shptr, int32_t* rshptr, float* source,
+                    int64_t source_cap, float* radiance, int64_t radiance_cap, float* fluxes,
+                    char* err, size_t errlen);
+
+/* Diagnostics of the current solution on the device: COMPUTE_SH
+ * (shdomsub2.f:1770-1845; RTE.spherical_harmonics, solver.py:1037) into
+ * shterms[nshout][npts] (mean radiance, Fx, Fy, Fz, and for nshout 5 the rms
+ * of the higher-order radiance terms), and COMPUTE_NETFLUXDIV
+ * (shdomsub2.f:1725-1765; RTE.net_flux_divergence, solver.py:1207) into
+ * netfluxdiv[npts].  Either output may be null.  Solar source. */
+int at3d_state_sh_terms(at3d_state* st, int32_t nshout, float* shterms, float* netfluxdiv,
+                        char* err, size_t errlen);
 
 /* Unknowns of a gradient: NUMDER partial derivatives of the optical properties
  * of one species each, on the property grid (what
@@ -221,7 +250,10 @@ int at3d_state_set_option(at3d_state* st, const char* key, int32_t value, char* 
 
 /* Time (ms, HIP events on the state's stream) of each kernel class over the
  * last at3d_solve call: keys "sh_to_do","sweep","do_to_sh","source","total";
- * also "iters", and "device_bytes" (the state's device allocations).
+ * also "iters", and "device_bytes" (the state's device allocations); of the
+ * last render call "render" (direction source and rays) and "render_rays"
+ * (the ray kernel alone); of the last gradient call "gradient" and
+ * "adjoint_rays" (the adjoint ray kernel alone).
  * Returns a negative value for unknown keys. */
 double at3d_state_timing(const at3d_state* st, const char* key);
 

@@ -331,8 +331,12 @@ def make_sh_do_coef(ml, mm, nlm, nmu, nphi0, nphi0max, fftflag, mu, phi, wtmu, w
     return dict(cmu1=cmu1, cmu2=cmu2, cphi1=cphi1, cphi2=cphi2, wphisave=wphisave)
 
 
-def init_solution(st: RefState):
-    """RTE._init_solution (solver.py:2539-2798) for a fixed Lambertian surface, solar source."""
+def init_solution(st: RefState, restore=None):
+    """RTE._init_solution (solver.py:2539-2798) for a fixed Lambertian surface, solar source.
+
+    ``restore``: a saved solution (keys shptr, rshptr, source, radiance,
+    fluxes as RTE.save_solution writes them); its arrays are placed in the
+    state and INIT_SOLUTION runs with INRADFLAG false (solver.py:2654-2664)."""
     sc, d = st.sc, st.d
     nmu, nphi0max, nstokes = d["nmu"], d["nphi0max"], 1
     ang = make_angle_set(nmu, d["nphi"], sc.angle_set, nphi0max)
@@ -358,6 +362,14 @@ def init_solution(st: RefState):
     shptr = np.zeros(maxig + 1, np.int32)
     rshptr = np.zeros(maxig + 2, np.int32)
     oshptr = np.zeros(maxig + 1, np.int32)
+    inradflag = restore is None
+    if restore is not None:
+        npts = d["npts"]
+        shptr[:npts + 1] = restore["shptr"][:npts + 1]
+        rshptr[:npts + 2] = restore["rshptr"][:npts + 2]
+        source[:, :shptr[npts]] = np.reshape(restore["source"], (1, -1))[:, :shptr[npts]]
+        radiance[:, :rshptr[npts]] = np.reshape(restore["radiance"], (1, -1))[:, :rshptr[npts]]
+        fluxes[:, :npts] = restore["fluxes"][:, :npts]
     maxsfcpars = 4
     sfcgridparms = np.zeros(maxsfcpars * maxnbc, np.float32)
     bcptr = np.zeros((maxnbc, 2), np.int32, order="F")
@@ -398,7 +410,7 @@ def init_solution(st: RefState):
           ipdirect, di, dj, dk, i4(nphi0max), epss, epsz, xdomain, ydomain, delxd, delyd,
           deljdot, deljold, deljnew, jnorm, i4(d["nbcells"]), ang["fftflag"], d["cmu1"],
           d["cmu2"], ang["wtmu"], d["cphi1"], d["cphi2"], d["wphisave"], i4(maxsfcpars),
-          work, work1, work2, i4(1), i4(1), uniform_sfc_brdf, sfc_brdf_do, lg(True), i4(ndel),
+          work, work1, work2, i4(1), i4(1), uniform_sfc_brdf, sfc_brdf_do, lg(inradflag), i4(ndel),
           ierr, errmsg, i4(maxpg), i4(work2_size), d["phaseinterpwt"], r4(0.999), s("ON"),
           i4(d["nlegp"]), lg(False), i4(d["maxnmicro"]), d["phasewtp"], i4(sc.angle_set),
           lg(True), longest, r4(0.0), surface_rad, volsrc, volsrcgridptr, volsrcshptr, i4(1))
@@ -520,6 +532,40 @@ def render(st: RefState, x, y, z, mu, phi, single_scatter=False, nosurface=False
           d["sfcgridrad"], i4(d["nang"]))
     _check(ierr, errmsg, "RENDER")
     return stokes[0].copy()
+
+
+def save_solution(st: RefState):
+    """RTE.save_solution(save_radiances=True) (solver.py:1621-1686): the
+    radiance variables of the saved dataset as arrays."""
+    d = st.d
+    npts = d["npts"]
+    return {"shptr": d["shptr"][:npts + 1].copy(), "rshptr": d["rshptr"][:npts + 2].copy(),
+            "source": d["source"][:, :d["shptr"][npts]].copy(),
+            "radiance": d["radiance"][:, :d["rshptr"][npts]].copy(),
+            "fluxes": d["fluxes"][:, :npts].copy()}
+
+
+def compute_sh(st: RefState, nshout=4):
+    """RTE.spherical_harmonics' COMPUTE_SH (solver.py:1037-1121, shdomsub2.f:1770)."""
+    d, sc = st.d, st.sc
+    npts = d["npts"]
+    shterms = np.zeros((nshout, npts), np.float32, order="F")
+    fcall("COMPUTE_SH", i4(nshout), i4(1), i4(npts), s("S"), r4(sc.solarmu), r4(sc.solaraz),
+          farr(d["dirflux"][:npts], np.float32), farr(d["rshptr"][:npts + 1], np.int32),
+          i4(d["ml"]), i4(d["mm"]), d["radiance"], shterms)
+    return shterms
+
+
+def compute_netfluxdiv(st: RefState):
+    """RTE.net_flux_divergence's COMPUTE_NETFLUXDIV (solver.py:1207-1262, shdomsub2.f:1725)."""
+    d, sc = st.d, st.sc
+    npts, npart = d["npts"], d["npart"]
+    out = np.zeros(npts, np.float32)
+    fcall("COMPUTE_NETFLUXDIV", i4(1), i4(npts), farr(d["rshptr"][:npts + 1], np.int32), s("S"),
+          r4(sc.solarmu), farr(d["extinct"][:npts], np.float32), farr(d["albedo"][:npts], np.float32),
+          farr(d["planck"][:npts], np.float32), farr(d["dirflux"][:npts], np.float32), d["radiance"],
+          out, i4(npart))
+    return out
 
 
 # ----------------------------------------------------------------------------

@@ -74,3 +74,30 @@ def test_oracle_jacobian_branch_is_the_per_pixel_derivative(ref):
     gsel = np.einsum("p,pjd->jd", (pxj - meas).astype(np.float64), J.astype(np.float64))
     np.testing.assert_allclose(gsel, gj[jp], rtol=1e-5, atol=1e-9 * np.abs(gj).max())
     assert np.abs(J[:, 2, 1]).max() > 0   # a point inside the blob responds to ssalb
+
+
+def test_levis_approx_gradient_argument_checks():
+    """The reference's constructor checks (gradient.py:61-127): a missing
+    cost function, a non-boolean 'add_noise', an uncertainty model for
+    another cost function; NullUncertainty fills unit inverse covariances."""
+    import warnings
+
+    from at3d_amd import gradient as GR
+    sens = {"measurement_data": np.ones(5, np.float32), "wavelength": 0.86}
+    meas = lambda model=None: {"cam": {"sensor_list": [dict(sens)], "uncertainty_model": model}}
+    with pytest.raises(ValueError, match="cost_function"):
+        GR.LevisApproxGradientUncorrelated(meas(), {}, {}, {}, {}, {}, {"add_noise": False})
+    with pytest.raises(TypeError, match="add_noise"):
+        GR.LevisApproxGradientUncorrelated(meas(), {}, {}, {}, {}, {"cost_function": "L2"},
+                                           {"add_noise": 1})
+    with pytest.raises(ValueError, match="inconsistent"):
+        GR.LevisApproxGradientUncorrelated(meas(GR.NullUncertainty("LL")), {}, {}, {}, {},
+                                           {"cost_function": "L2"}, {"add_noise": False})
+    m = meas()
+    with warnings.catch_warnings(record=True) as rec:
+        warnings.simplefilter("always")
+        lg = GR.LevisApproxGradientUncorrelated(m, {}, {}, {}, {}, {"cost_function": "L2"}, {})
+    assert any("add_noise" in str(r.message) for r in rec)
+    assert any("NullUncertainty" in str(r.message) for r in rec)
+    assert np.array_equal(m["cam"]["sensor_list"][0]["uncertainties"], np.ones(5))
+    assert lg.nmeasurements == 5

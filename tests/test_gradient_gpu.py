@@ -249,3 +249,60 @@ def test_jacobian_with_several_rays_per_pixel_matches_reference(ref):
     assert rel_l2(px, px_ref) < 1e-4
     for d in range(2):
         assert rel_l2(J[:, :, d], J_ref[:, :, d]) < 1e-3, (d, rel_l2(J[:, :, d], J_ref[:, :, d]))
+
+
+def test_levis_approx_gradient_uncorrelated_over_two_wavelengths(ref):
+    """LevisApproxGradientUncorrelated (gradient.py:417-449) with two solvers
+    keyed by wavelength and one instrument holding a sensor of each: the
+    loss and gradient are the per-sensor reference gradients summed and
+    divided by the number of measurements; the Jacobian dataset holds each
+    wavelength's pixels."""
+    base = blob8("open")
+    ws = {}
+    for wl, view in ((0.86, (0.0, 0.0)), (0.66, (45.0, 30.0))):
+        ext = (base.species[0][0] * (0.86 / wl)).astype(np.float32)
+        ws[wl] = (S.Workload(base.name, base.delx, base.dely, base.z,
+                             [(ext, base.species[0][1], base.species[0][2])], bc=base.bc,
+                             nmu=base.nmu, nphi=base.nphi, solacc=base.solacc, wavelength=wl), view)
+    solvers, meas_list, fwd_list = {}, [], []
+    g_sum, c_sum, n = None, 0.0, 0
+    for wl, (w, view) in ws.items():
+        st = ref.setup(ref.scene_from_workload(w))
+        ref.init_solution(st)
+        ref.solution_iterations(st, 100)
+        ref.prepare_derivatives(st, _ref_derivs(ref, w, [(0, "extinction"), (0, "ssalb")]))
+        rays = ortho_rays(w, [view])
+        args = [rays[k] for k in ("ray_x", "ray_y", "ray_z", "ray_mu", "ray_phi")]
+        m = (ref.render(st, *args) * 1.15).astype(np.float32)
+        g_ref, c_ref, _ = ref.levis_gradient(st, *args, m)
+        g_sum = g_ref if g_sum is None else g_sum + g_ref
+        c_sum += c_ref
+        n += m.size
+        rte = S.make_rte(w)
+        rte.solve(100, verbose=False)
+        solvers[wl] = rte
+        meas_list.append(dict(rays, measurement_data=m, wavelength=wl))
+        fwd_list.append(dict(rays, wavelength=wl))
+    name = list(solvers[0.86].medium)[0]
+    jac_pts = (np.array([3, 4]), np.array([4, 4]), np.array([3, 5]))
+    lg = GR.LevisApproxGradientUncorrelated(
+        {"cam": {"sensor_list": meas_list, "uncertainty_model": None}}, solvers,
+        {"cam": {"sensor_list": fwd_list}}, {name: ["extinction", "ssalb"]},
+        {"maxiter": 100, "verbose": False, "n_jobs": 4},
+        {"cost_function": "L2", "exact_single_scatter": True, "indices_for_jacobian": jac_pts},
+        {"add_noise": False})
+    loss, gds, jds = lg()
+    assert loss == pytest.approx(c_sum / n, rel=5e-4)
+    grad = gds["gradient"]
+    assert grad.shape == base.shape + (2,)
+    assert gds["derivative_index"] == [(name, "extinction"), (name, "ssalb")]
+    for i in range(2):
+        err = rel_l2(grad[..., i].ravel(), g_sum[:, i] / n)
+        print(f"derivative {i}: rel L2 {err:.2e} vs the summed reference gradients")
+        assert err < 1e-3
+    for k, (wl, (w, view)) in enumerate(ws.items()):
+        assert np.all(np.isfinite(fwd_list[k]["I"])) and fwd_list[k]["I"].size == meas_list[k]["ray_x"].size
+        jac = jds["jacobian_{:1.3f}".format(wl)]
+        assert jac.shape == (1, 2, 2, meas_list[k]["ray_x"].size)
+        assert np.any(jac != 0)
+    assert len(jds["grid_index"]) == 2

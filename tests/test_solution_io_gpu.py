@@ -1,0 +1,126 @@
+"""Saved solutions and the solution diagnostics on the GPU against the
+reference Fortran.
+
+* RTE.fluxes / spherical_harmonics / net_flux_divergence (COMPUTE_SH,
+  COMPUTE_NETFLUXDIV) of converged solutions: rel L2 <= 1e-4 (the converged
+  fields' own tolerance).
+* A solution the reference saved (RTE.save_solution's arrays) loaded into
+  this engine renders the reference's pixels (rel L2 <= 1e-4) without any
+  iteration here, and a continued solve from it converges at once.
+* This engine's saved solution: the term pointers equal the reference's
+  (integer, exact) and the terms agree at 1e-4; save -> load -> render is
+  bit-identical, through a dict and through an .npz file.
+"""
+import numpy as np
+import pytest
+
+from at3d_amd import synthetic as S
+from tests.scenes import blob8, cloud_l15, ortho_rays, rel_l2
+
+pytestmark = [pytest.mark.gpu, pytest.mark.filterwarnings("ignore:RTE is not solved")]
+
+SCENES = {"blob8_open": lambda: blob8("open"), "blob8_periodic": lambda: blob8("periodic"),
+          "cloud16_l15": cloud_l15}
+VIEWS = [(0.0, 0.0), (45.0, 30.0), (60.0, 200.0)]
+
+
+def _converged(ref, w, **overrides):
+    sc_kw = {}
+    if overrides.get("high_order_radiance"):
+        sc_kw["high_order_radiance"] = True
+    st = ref.setup(ref.scene_from_workload(w, **sc_kw))
+    ref.init_solution(st)
+    ref.solution_iterations(st, 100)
+    rte = S.make_rte(w, **overrides)
+    rte.solve(100, verbose=False)
+    return st, rte
+
+
+def _rays(w):
+    return ortho_rays(w, VIEWS)
+
+
+def _ref_render(ref, st, rays):
+    return ref.render(st, rays["ray_x"], rays["ray_y"], rays["ray_z"], rays["ray_mu"], rays["ray_phi"])
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_diagnostics_match_reference(ref, name):
+    w = SCENES[name]()
+    st, rte = _converged(ref, w)
+    npts = st.npts
+    shape = (rte.plan_size("nx1"), rte.plan_size("ny1"), rte.plan_size("nz"))
+    f = rte.fluxes
+    assert f["flux_down"].shape == shape
+    assert rel_l2(f["flux_down"].ravel(), st.fluxes[0, :npts]) < 1e-4
+    assert rel_l2(f["flux_up"].ravel(), st.fluxes[1, :npts]) < 1e-4
+    assert rel_l2(f["flux_direct"].ravel(), st.dirflux[:npts]) < 1e-6
+    want = ref.compute_sh(st, 4)
+    got = rte.spherical_harmonics
+    for k, key in enumerate(("mean_intensity", "Fx", "Fy", "Fz")):
+        err = rel_l2(got[key].ravel(), want[k])
+        print(f"{key}: rel L2 {err:.2e}")
+        assert err < 1e-4, key
+    nfd = rte.net_flux_divergence["net_flux_div"].ravel()
+    assert rel_l2(nfd, ref.compute_netfluxdiv(st)) < 1e-4
+
+
+def test_high_order_radiance_rms_matches_reference(ref):
+    w = blob8("open")
+    st, rte = _converged(ref, w, high_order_radiance=True)
+    want = ref.compute_sh(st, 5)
+    got = rte.spherical_harmonics
+    assert "rms_higher_rad" in got
+    nlm = rte.plan_size("nlm")
+    assert rel_l2(got["rms_higher_rad"].ravel() * np.sqrt(np.pi * 4.0 * nlm), want[4]) < 1e-4
+
+
+@pytest.mark.parametrize("name", ["blob8_open", "cloud16_l15"])
+def test_reference_saved_solution_loads_and_renders(ref, name):
+    w = SCENES[name]()
+    st = ref.setup(ref.scene_from_workload(w))
+    ref.init_solution(st)
+    ref.solution_iterations(st, 100)
+    rays = _rays(w)
+    want = _ref_render(ref, st, rays)
+    rte = S.make_rte(w)
+    ds = rte.save_solution(save_radiances=False)     # this grid, in the reference's layout
+    ds.update(ref.save_solution(st))                 # the reference's solution
+    rte.load_solution(ds)
+    rte.solve(0, verbose=False)                      # initialise from it; no iteration
+    assert rte.num_iterations == 0
+    assert rel_l2(rte.state_array("source"), ref.dense_sh(st.shptr, st.source, st.npts, st.nlm)) == 0.0
+    got = rte.integrate_to_sensor(dict(rays))["I"]
+    err = rel_l2(got, want)
+    print(f"render from the reference's saved solution: rel L2 {err:.2e}")
+    assert err < 1e-4
+    # a continued solve starts from the converged field
+    rte.solve(100, verbose=False)
+    assert rte.check_solved(verbose=False) and rte.num_iterations <= 2
+    assert rel_l2(rte.state_array("source"), ref.dense_sh(st.shptr, st.source, st.npts, st.nlm)) < 1e-4
+
+
+def test_engine_saved_solution_matches_reference_and_round_trips(ref, tmp_path):
+    w = cloud_l15()
+    st, rte = _converged(ref, w)
+    ds = rte.save_solution()
+    want = ref.save_solution(st)
+    assert np.array_equal(ds["shptr"], want["shptr"])
+    assert np.array_equal(ds["rshptr"], want["rshptr"])
+    assert rel_l2(ds["source"], want["source"]) < 1e-4
+    assert rel_l2(ds["radiance"], want["radiance"]) < 1e-4
+    assert ds["gridptr"].shape == (8, rte.plan_size("ncells"))
+    assert np.array_equal(ds["gridptr"], st.gridptr[:, :st.ncells])
+    assert np.array_equal(ds["neighptr"], st.neighptr[:, :st.ncells])
+    rays = _rays(w)
+    img = rte.integrate_to_sensor(dict(rays))["I"]
+    path = tmp_path / "solution.npz"
+    np.savez(path, **ds)
+    for source in (ds, str(path)):
+        b = S.make_rte(w)
+        b.load_solution(source)
+        b.solve(0, verbose=False)
+        for name in ("source", "radiance", "fluxes"):
+            assert np.array_equal(b.state_array(name), rte.state_array(name)), name
+        assert np.array_equal(b.integrate_to_sensor(dict(rays))["I"], img)
+        b.release()
