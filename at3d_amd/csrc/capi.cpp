@@ -514,6 +514,22 @@ int at3d_state_set_option(at3d_state* st, const char* key, int32_t value, char* 
     st->s.use_ring = value ? 1 : 0;
     return AT3D_OK;
   }
+  if (k == "iteration_graph") {
+    st->s.use_iter_graph = value ? 1 : 0;
+    return AT3D_OK;
+  }
+  if (k == "overlap") {
+    st->s.use_overlap = value ? 1 : 0;
+    return AT3D_OK;
+  }
+  if (k == "sweep_prefetch") {
+    if (value != 1 && value != 2) {
+      set_err(err, errlen, "sweep_prefetch must be 1 or 2");
+      return AT3D_ERR_ARG;
+    }
+    st->s.sweep_prefetch = value;
+    return AT3D_OK;
+  }
   if (k == "max_subgrid_intervals") {
     st->s.max_subgrid = value > 0 ? value : 0;
     return AT3D_OK;
@@ -545,6 +561,7 @@ double at3d_state_timing(const at3d_state* st, const char* key) {
   if (k == "iters") return s.iters_last;
   if (k == "render") return s.render_ms;
   if (k == "render_rays") return s.render_ray_ms;
+  if (k == "iteration_graph_active") return s.last_solve_graph ? 1.0 : 0.0;
   if (k == "adjoint_rays") return s.adjoint_ms;
   if (k == "gradient") return s.grad_ms;
   return -1.0;

@@ -7,23 +7,49 @@
 
 namespace at3d {
 
+// Device control of a solve's iterations, so that they run from captured
+// HIP graphs without the host between them (device memory, one per state).
+// Every iteration kernel returns at once while `stop` is set: k_iter_begin
+// sets it when the solve has converged or reached maxiter, k_solcrit when a
+// kernel flagged an error.
+struct IterCtl {
+  int stop;         // nonzero: the rest of this solve call's kernels do nothing
+  int iters;        // iterations run in this solve call
+  int total;        // iterations of the solution (state's count at the call + iters)
+  int fixsh;        // FIXSH for the source pass (sticky within a call)
+  float solcrit;    // SOLCRIT after the last iteration
+  float a;          // the acceleration parameter of the last iteration (0: none)
+  float deljdot, deljold, deljnew, jnorm;
+  // parameters of the call
+  int maxiter;      // total iteration limit
+  int iterfixsh;
+  int accel;        // acceleration on
+  int one_iter;     // ALBMAX < SOLACC: SOLCRIT is set to SOLACC after an iteration
+  float solacc;
+  int pad[3];
+};
+
 // ---- transforms.hip: FP32 MFMA SH <-> DO transforms ----------------------
 // dosrc[c][p] = max(0, sum_J S[p][J] * Yc[c][J]) * ext[p]
 // S [mp][ld], Yc [ncp][ld], dosrc [ncols][ldp] (ordinate-major, ldp = mp);
 // mp, ncp multiples of 64; ld multiple of 16; ext padded to mp.
 void launch_sh_to_do(hipStream_t st, const float* S, const float* Yc, const float* ext,
-                     float* dosrc, int mp, int ncp, int ld, int ldp, int npts, int nang);
+                     float* dosrc, int mp, int ncp, int ld, int ldp, int npts, int nang,
+                     const int* stop = nullptr);
 // Level-ordered variant: dosrc[c][q] = max(0, sum_J S[perm_o[q]][J] * Yc[c][J]) * ext[perm_o[q]]
 // for each column c of octant o, perm_o = oct_perm + o * mp (the octant's
 // positions: its levels, boundary points included).  tiles[t] = {octant,
 // first column, columns (<= 48), 0}.  S[p][J] must be zero for J >= ns[p].
 void launch_sh_to_do_perm(hipStream_t st, const float* S, const float* Yc, const float* ext,
                           const int* ns, const int32_t* oct_perm, const int4* tiles, int ntiles,
-                          float* dosrc, int mp, int ld, int ldp, int npts);
-// R[p][J] = (J < nr[p]) ? sum_c dorad[c][p] * Wc[c][J] : 0
+                          float* dosrc, int mp, int ld, int ldp, int npts, const int* stop = nullptr);
+// R[p][J] = (J < nr[p]) ? sum_c dorad[c][p] * Wc[c][J] : 0 over the columns
+// c0 <= c < c1 (multiples of 16 or ldo; Wc rows outside the sum's range must
+// be zero), plus R[p][J] when `accumulate`.
 // dorad [ldo][ldp] (rows beyond nang zero), Wc [ldo][ld] (rows beyond nang zero), R [mp][ld].
 void launch_do_to_sh(hipStream_t st, const float* dorad, const float* Wc, const int* nr, float* R,
-                     int mp, int ld, int ldo, int ldp, int npts);
+                     int mp, int ld, int ldo, int ldp, int npts, int c0 = 0, int c1 = -1,
+                     int accumulate = 0, const int* stop = nullptr);
 
 // ---- sweep.hip: discrete-ordinate sweeps --------------------------------
 struct SweepArgs {
@@ -86,6 +112,7 @@ struct SweepArgs {
   int slots;                 // ring slots
   int sector_points;         // 8 or 4
   int* ngeneral;             // multi-cell traces found by the record pass
+  const int* stop = nullptr; // IterCtl::stop of the state, or null
 };
 
 // The fixed part of one ring entry of one ordinate, 16 bytes: head = point |
@@ -125,8 +152,9 @@ void launch_face_mask(hipStream_t st, const SweepArgs& a);
 void launch_sweep_geo(hipStream_t st, const SweepArgs& a, int ncols);
 // fast / full: launch the instantiation without / with the multi-cell,
 // overflow and extra-sector paths (a column runs in exactly one of them).
+// depth: levels of record loads in flight in the fast instantiation (1 or 2).
 void launch_sweep_ring(hipStream_t st, const SweepArgs& a, int col0, int ncol, size_t lds_fast, size_t lds_full,
-                       bool fast, bool full);
+                       bool fast, bool full, int depth = 1);
 
 void launch_sweep_preset(hipStream_t st, const SweepArgs& a, bool up, int col0, int ncol);
 // One workgroup per ordinate column col0 .. col0+ncol-1, all levels of its octant.
@@ -142,7 +170,8 @@ struct BottomRecs {
 // Lambertian surface radiance from the downwelling flux.
 void launch_bottom_bc(hipStream_t st, const float* rad, int ldp, const int32_t* bcbot, int nbot,
                       const int32_t* col_of_ref, const float* fluxw, int ndown,
-                      const float* dirflux, float albpi, float* bcrad, const BottomRecs& br);
+                      const float* dirflux, float albpi, float* bcrad, const BottomRecs& br,
+                      const int* stop = nullptr);
 void launch_fluxes(hipStream_t st, const float* rad, int ldp, const int32_t* col_of_ref,
                    const float* fluxw, int ndown, int nang, int npts, float* fluxes);
 
@@ -169,6 +198,7 @@ struct SourceArgs {
   const float* tc;    // [npts][ml+1]
   float radmin;
   int highorder;
+  const IterCtl* ctl = nullptr;  // null: `fixsh` as given; else FIXSH and stop from it
 };
 constexpr int SRC_PTS_PER_BLOCK = 16;
 void launch_radiance_truncation(hipStream_t st, const float* R, int ld, const float* tc,
@@ -178,9 +208,15 @@ void launch_compute_source(hipStream_t st, const SourceArgs& a, int nblocks);
 // R[p][J] = R0[p][J] for J < 4, zero for 4 <= J < max(4, nr[p]) (R is zero
 // beyond NR already); then nr[p] = 4.
 void launch_init_radiance(hipStream_t st, float* R, const float* R0, int* nr, int ld, int npts);
-void launch_reduce_dots(hipStream_t st, const double* partial, int nblocks, double* out);
+void launch_reduce_dots(hipStream_t st, const double* partial, int nblocks, double* out,
+                        const int* stop = nullptr);
+// ctl null: S += a DS; else with ctl's parameter, when it is positive.
 void launch_accelerate(hipStream_t st, float* S, const float* DS, const int* ns, const int* nos,
-                       int ld, int npts, float a);
+                       int ld, int npts, float a, const IterCtl* ctl = nullptr);
+// The start of an iteration (the loop test of SOLUTION_ITERATIONS and FIXSH)
+// and its end (CALC_ACCEL_SOLCRIT from the reduced dot products), one thread each.
+void launch_iter_begin(hipStream_t st, IterCtl* ctl);
+void launch_solcrit(hipStream_t st, IterCtl* ctl, const double* dots, const int* errflag);
 
 // ---- render.hip: rays ----------------------------------------------------
 struct RayArgs {

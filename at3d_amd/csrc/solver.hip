@@ -158,6 +158,7 @@ static SweepArgs sweep_args(const State& s) {
   a.slots = s.slots;
   a.sector_points = s.sector_points;
   a.ngeneral = s.ngeneral_d;
+  a.stop = nullptr;
   return a;
 }
 
@@ -242,6 +243,8 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
       tiles.push_back(make_int4(o, c, std::min(48, s.oct_col_base[o] + s.oct_nord[o] - c), 0));
   }
   s.nsectors = (int)sp0.size();
+  s.n_sh_tiles_dn = 0;   // tiles are in octant order: the downwelling ones first
+  for (const int4& t : tiles) s.n_sh_tiles_dn += t.x < 4 ? 1 : 0;
   // the fast instantiation holds a level in the threads' register slots and
   // one sector per thread
   s.ring_full = 0;
@@ -343,7 +346,8 @@ size_t plan_footprint_bytes(const Plan& p) {
   const size_t ld = round_up(p.nlm, 64), nptsp = round_up(p.npts, 64), ldo = round_up(p.nang, 64);
   const size_t ml1 = p.ml + 1;
   size_t b = 0;
-  b += 2 * ldo * ld * 4;                 // Y, W
+  b += 4 * ldo * ld * 4;                 // Y, W and W's two hemisphere shares
+  b += 256;                              // iteration control block
   b += 3 * nptsp * ld * 4;               // S, R, DS
   b += 2 * nptsp * ldo * 4;              // DO source and radiance
   b += npts * 16;                        // two-stream radiance terms
@@ -405,6 +409,13 @@ int state_create(const Plan& p, State& s, std::string& err) {
   }
   TRY(upload(s, s.Y, Y.data(), Y.size(), err));
   TRY(upload(s, s.W, W.data(), W.size(), err));
+  {
+    std::vector<float> wd(W), wu(W);
+    for (int c = 0; c < s.ldo; c++)
+      for (int j = 0; j < s.ld; j++) (c < s.ndown ? wu : wd)[(size_t)c * s.ld + j] = 0.f;
+    TRY(upload(s, s.Wdn, wd.data(), wd.size(), err));
+    TRY(upload(s, s.Wup, wu.data(), wu.size(), err));
+  }
   std::vector<float> ysun(s.ld, 0.f);
   std::copy(p.ylmsun.begin(), p.ylmsun.end(), ysun.begin());
   TRY(upload(s, s.ysun, ysun.data(), ysun.size(), err));
@@ -535,6 +546,16 @@ int state_create(const Plan& p, State& s, std::string& err) {
     err = "hipHostMalloc of the iteration readback buffer failed";
     return AT3D_ERR_DEVICE;
   }
+  if (hipHostMalloc((void**)&s.ctl_h, sizeof(IterCtl), hipHostMallocDefault) != hipSuccess) {
+    s.ctl_h = nullptr;
+    err = "hipHostMalloc of the iteration control block failed";
+    return AT3D_ERR_DEVICE;
+  }
+  TRY(alloc0(s, s.ctl, 1, err));
+  s.nr_buf0 = s.nr;
+  HIPCHK(hipStreamCreateWithFlags(&s.stream2, hipStreamNonBlocking));
+  for (int b = 0; b < State::ITER_BATCH; b++)
+    for (int e = 0; e < 4; e++) HIPCHK(hipEventCreateWithFlags(&s.oev[b][e], hipEventDisableTiming));
   TRY(setup_ring(p, s, err));
   for (int e = 0; e < State::NEV; e++) HIPCHK(hipEventCreate(&s.ev[e]));
   for (int e = 0; e < 4; e++) HIPCHK(hipEventCreate(&s.ev_sweep[e]));
@@ -568,6 +589,19 @@ void state_free(State& s) {
     if (s.ev_ray[e]) (void)hipEventDestroy(s.ev_ray[e]);
   if (s.host_dots) (void)hipHostFree(s.host_dots);
   s.host_dots = nullptr;
+  if (s.ctl_h) (void)hipHostFree(s.ctl_h);
+  s.ctl_h = nullptr;
+  for (int k = 0; k < 2; k++) {
+    if (s.iter_exec[k]) (void)hipGraphExecDestroy(s.iter_exec[k]);
+    if (s.iter_graph[k]) (void)hipGraphDestroy(s.iter_graph[k]);
+    s.iter_exec[k] = nullptr;
+    s.iter_graph[k] = nullptr;
+  }
+  for (int b = 0; b < State::ITER_BATCH; b++)
+    for (int e = 0; e < 4; e++)
+      if (s.oev[b][e]) (void)hipEventDestroy(s.oev[b][e]);
+  if (s.stream2) (void)hipStreamDestroy(s.stream2);
+  s.stream2 = nullptr;
   if (s.stream) (void)hipStreamDestroy(s.stream);
   s.stream = nullptr;
 }
@@ -660,13 +694,25 @@ int state_init_solution_from(State& s, State& src, std::string& err) {
 
 // The path-integration sweeps of one iteration: downwelling levels, surface
 // boundary, upwelling levels.
-static void enqueue_sweeps(State& s) {
+// The hemisphere overlap's events of one iteration: `fork` after the
+// downwelling SH->DO, `up_src` when the upwelling SH->DO is done, `down_done`
+// after the downwelling sweep, `down_sh` when the downwelling share of
+// DO->SH is done.
+struct Overlap {
+  hipEvent_t fork, up_src, down_done, down_sh;
+};
+
+static void enqueue_sweeps(State& s, const int* stop = nullptr, hipEvent_t ev_down_end = nullptr,
+                           hipEvent_t ev_up_begin = nullptr, const Overlap* ov = nullptr) {
   const Plan& p = *s.plan;
-  const SweepArgs a = sweep_args(s);
+  SweepArgs a = sweep_args(s);
+  a.stop = stop;
+  if (!ev_down_end) ev_down_end = s.ev_sweep[1];
+  if (!ev_up_begin) ev_up_begin = s.ev_sweep[2];
   const float albpi = p.d.gndalbedo / acosf(-1.0f);
   const bool ring = ring_active(s);
   // the ring sweep writes the boundary radiances itself; events around its
-  // launches (never captured into a graph) time the sweep kernel alone
+  // launches time the sweep kernel alone
   // which ring instantiations each hemisphere's columns need
   auto needs = [&](int c0, int c1, bool& fast, bool& full) {
     fast = full = false;
@@ -681,19 +727,31 @@ static void enqueue_sweeps(State& s) {
     // the first launch is timed from the event the caller records just
     // before this function, the second to the one just after it
     needs(0, s.ndown, fast, full);
-    launch_sweep_ring(s.stream, a, 0, s.ndown, s.ring_lds, s.ring_lds_full, fast, full);
-    (void)hipEventRecord(s.ev_sweep[1], s.stream);
+    launch_sweep_ring(s.stream, a, 0, s.ndown, s.ring_lds, s.ring_lds_full, fast, full, s.sweep_prefetch);
+    (void)hipEventRecord(ev_down_end, s.stream);
+    if (ov) {
+      // the downwelling rows of DO->SH on the second stream, while the
+      // surface and the upwelling sweep run here
+      const int kd = (int)round_up(s.ndown, 16);
+      (void)hipEventRecord(ov->down_done, s.stream);
+      (void)hipStreamWaitEvent(s.stream2, ov->down_done, 0);
+      launch_do_to_sh(s.stream2, s.dorad, s.Wdn, s.nr, s.R, s.nptsp, s.ld, s.ldo, s.nptsp, p.npts, 0, kd,
+                      0, stop);
+      (void)hipEventRecord(ov->down_sh, s.stream2);
+    }
   } else {
     launch_sweep_preset(s.stream, a, false, 0, s.ndown);
     launch_sweep_columns(s.stream, a, 0, s.ndown);
   }
   BottomRecs br{ring ? s.rec : nullptr, p.npts, s.ndown, p.nang, s.col_oct, s.oct_pos};
   launch_bottom_bc(s.stream, s.dorad, s.nptsp, s.bcbot, (int)p.bcbot.size(), s.col_of_ref_d, s.fluxw,
-                   s.ndown, s.dirflux, albpi, s.bcrad_bot, br);
+                   s.ndown, s.dirflux, albpi, s.bcrad_bot, br, stop);
   if (ring) {
     needs(s.ndown, p.nang, fast, full);
-    (void)hipEventRecord(s.ev_sweep[2], s.stream);
-    launch_sweep_ring(s.stream, a, s.ndown, p.nang - s.ndown, s.ring_lds, s.ring_lds_full, fast, full);
+    if (ov) (void)hipStreamWaitEvent(s.stream, ov->up_src, 0);
+    (void)hipEventRecord(ev_up_begin, s.stream);
+    launch_sweep_ring(s.stream, a, s.ndown, p.nang - s.ndown, s.ring_lds, s.ring_lds_full, fast, full,
+                      s.sweep_prefetch);
   } else {
     launch_sweep_preset(s.stream, a, true, s.ndown, p.nang - s.ndown);
     launch_sweep_columns(s.stream, a, s.ndown, p.nang - s.ndown);
@@ -773,6 +831,139 @@ static void accel_solcrit(bool doaccel, float deljdot, float deljold, float delj
     solcrit = 0.0f;
 }
 
+// One iteration under device control (IterCtl): the loop test and FIXSH,
+// SH->DO, the sweeps, DO->SH, the source pass with the next radiance
+// truncation, the reduction, SOLCRIT and the acceleration; every kernel
+// returns at once once the control block says stop.  The NR buffers are
+// those the host pointers name now.  (Inside a stream capture an event
+// record only marks a dependency: kernel times come from the iterations
+// launched one by one.)
+static void enqueue_iteration(State& s, const hipEvent_t* oev) {
+  const Plan& p = *s.plan;
+  const int* stop = &s.ctl->stop;
+  const bool ovl = oev && s.use_overlap && ring_active(s) && s.stream2 && s.n_sh_tiles_dn > 0 &&
+                   s.n_sh_tiles > s.n_sh_tiles_dn;
+  Overlap ov{};
+  launch_iter_begin(s.stream, s.ctl);
+  if (ovl) {
+    ov = Overlap{oev[0], oev[1], oev[2], oev[3]};
+    launch_sh_to_do_perm(s.stream, s.S, s.Y, s.ext, s.ns, s.oct_perm, s.sh_tiles, s.n_sh_tiles_dn, s.dosrc,
+                         s.nptsp, s.ld, s.nptsp, p.npts, stop);
+    (void)hipEventRecord(ov.fork, s.stream);
+    (void)hipStreamWaitEvent(s.stream2, ov.fork, 0);
+    launch_sh_to_do_perm(s.stream2, s.S, s.Y, s.ext, s.ns, s.oct_perm, s.sh_tiles + s.n_sh_tiles_dn,
+                         s.n_sh_tiles - s.n_sh_tiles_dn, s.dosrc, s.nptsp, s.ld, s.nptsp, p.npts, stop);
+    (void)hipEventRecord(ov.up_src, s.stream2);
+  } else if (ring_active(s)) {
+    launch_sh_to_do_perm(s.stream, s.S, s.Y, s.ext, s.ns, s.oct_perm, s.sh_tiles, s.n_sh_tiles, s.dosrc,
+                         s.nptsp, s.ld, s.nptsp, p.npts, stop);
+  } else {
+    launch_sh_to_do(s.stream, s.S, s.Y, s.ext, s.dosrc, s.nptsp, s.ldo, s.ld, s.nptsp, p.npts, p.nang, stop);
+  }
+  enqueue_sweeps(s, stop, nullptr, nullptr, ovl ? &ov : nullptr);
+  if (ovl) {
+    // the upwelling rows added to the downwelling share
+    (void)hipStreamWaitEvent(s.stream, ov.down_sh, 0);
+    const int ku = s.ndown / 16 * 16;
+    launch_do_to_sh(s.stream, s.dorad, s.Wup, s.nr, s.R, s.nptsp, s.ld, s.ldo, s.nptsp, p.npts, ku, s.ldo,
+                    1, stop);
+  } else {
+    launch_do_to_sh(s.stream, s.dorad, s.W, s.nr, s.R, s.nptsp, s.ld, s.ldo, s.nptsp, p.npts, 0, -1, 0, stop);
+  }
+  SourceArgs sa = source_args(s, 0, 0);
+  sa.ctl = s.ctl;
+  launch_compute_source(s.stream, sa, s.nsrc_blocks);
+  launch_reduce_dots(s.stream, s.partial, s.nsrc_blocks, s.dots, stop);
+  launch_solcrit(s.stream, s.ctl, s.dots, s.errflag);
+  launch_accelerate(s.stream, s.S, s.DS, s.ns, s.nos, s.ld, p.npts, 0.0f, s.ctl);
+}
+
+// The graph of ITER_BATCH iterations for the NR buffer parity k (0: s.nr is
+// the first buffer), captured on the state's stream.
+static int capture_iterations(State& s, int k, std::string& err) {
+  if (s.iter_exec[k]) (void)hipGraphExecDestroy(s.iter_exec[k]);
+  if (s.iter_graph[k]) (void)hipGraphDestroy(s.iter_graph[k]);
+  s.iter_exec[k] = nullptr;
+  s.iter_graph[k] = nullptr;
+  std::lock_guard<std::mutex> lock(device_mutex());
+  HIPCHK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+  for (int b = 0; b < State::ITER_BATCH; b++) {
+    std::swap(s.nr, s.nr_next);
+    enqueue_iteration(s, s.oev[b]);
+  }
+  hipError_t e1 = hipGetLastError();
+  hipError_t e2 = hipStreamEndCapture(s.stream, &s.iter_graph[k]);
+  // ITER_BATCH is even: the host pointers are back where they were
+  if (e1 != hipSuccess || e2 != hipSuccess) {
+    err = std::string("iteration graph capture failed: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2);
+    return AT3D_ERR_DEVICE;
+  }
+  HIPCHK(hipGraphInstantiate(&s.iter_exec[k], s.iter_graph[k], nullptr, nullptr, 0));
+  s.iter_key[k] = (ring_active(s) ? 1 : 0) | (s.force_general ? 2 : 0) | (s.ring_full ? 4 : 0) |
+                  (s.use_overlap ? 8 : 0) | (s.sweep_prefetch == 2 ? 16 : 0);
+  return AT3D_OK;
+}
+
+// SOLUTION_ITERATIONS from captured graphs: the host launches a batch of
+// iterations and reads the control block once per batch; the device decides
+// when the solve stops.
+static int state_solve_graph(State& s, int maxiter, std::string& err) {
+  const Plan& p = *s.plan;
+  IterCtl& c = *s.ctl_h;
+  memset(&c, 0, sizeof c);
+  c.total = s.iters;
+  c.solcrit = s.solcrit;
+  c.maxiter = maxiter;
+  c.iterfixsh = p.d.iterfixsh;
+  c.accel = p.d.accelflag ? 1 : 0;
+  c.one_iter = p.albmax < p.d.solacc ? 1 : 0;
+  c.solacc = p.d.solacc;
+  HIPCHK(hipMemcpyAsync(s.ctl, &c, sizeof c, hipMemcpyHostToDevice, s.stream));
+  const int parity = s.nr == s.nr_buf0 ? 0 : 1;
+  s.dosrc_level_order = ring_active(s);
+  const int key = (ring_active(s) ? 1 : 0) | (s.force_general ? 2 : 0) | (s.ring_full ? 4 : 0) |
+                  (s.use_overlap ? 8 : 0) | (s.sweep_prefetch == 2 ? 16 : 0);
+  if (!s.iter_exec[parity] || s.iter_key[parity] != key) TRY(capture_iterations(s, parity, err));
+  // every batch runs ITER_BATCH iterations unless the device stops: the
+  // host bound only guards against a control block that never moves
+  const int max_batches = (maxiter - s.iters) / State::ITER_BATCH + 2;
+  for (int batch = 0;; batch++) {
+    if (batch >= max_batches) {
+      char buf[200];
+      snprintf(buf, sizeof buf,
+               "iteration graph: the device control block did not reach its stop condition "
+               "(stop %d, iterations %d, total %d, SOLCRIT %g after %d batches)",
+               c.stop, c.iters, c.total, (double)c.solcrit, batch);
+      err = buf;
+      return AT3D_ERR_DEVICE;
+    }
+    HIPCHK(hipGraphLaunch(s.iter_exec[parity], s.stream));
+    HIPCHK(hipMemcpyAsync(&c, s.ctl, sizeof c, hipMemcpyDeviceToHost, s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    s.iters_last = c.iters;
+    if (c.stop || c.total >= maxiter || c.solcrit <= p.d.solacc) break;
+  }
+  if (c.iters & 1) std::swap(s.nr, s.nr_next);
+  s.iters = c.total;
+  s.solcrit = c.solcrit;
+  s.deljdot = c.deljdot;
+  s.deljold = c.deljold;
+  s.deljnew = c.deljnew;
+  s.jnorm = c.jnorm;
+  s.nr_next_fixsh = c.fixsh != 0;
+  s.nsweep_launch = c.iters;
+  if (c.stop == 2) {
+    int ef = 0;
+    HIPCHK(hipMemcpy(&ef, s.errflag, sizeof(int), hipMemcpyDeviceToHost));
+    char buf[160];
+    snprintf(buf, sizeof buf, "BACK_INT_GRID: invalid characteristic (flags 0x%x) at iteration %d", ef,
+             s.iters);
+    err = buf;
+    return AT3D_ERR_NUMERIC;
+  }
+  return AT3D_OK;
+}
+
 int state_solve(State& s, int maxiter, int verbose, std::string& err) {
   const Plan& p = *s.plan;
   if (!s.initialized) TRY(state_init_solution(s, err));
@@ -780,6 +971,7 @@ int state_solve(State& s, int maxiter, int verbose, std::string& err) {
   float a = 0.0f;
   bool fixsh = false;
   for (auto& t : s.tsum) t = 0.0;
+  s.last_solve_graph = false;
   s.iters_last = 0;
   s.nsweep_launch = 0;
   // the NR waiting in nr_next was made with the last call's FIXSH; a call
@@ -788,6 +980,16 @@ int state_solve(State& s, int maxiter, int verbose, std::string& err) {
     launch_radiance_truncation(s.stream, s.R, s.ld, s.tc, s.ns, s.lofj, s.nr_next, p.npts, p.ml, p.mm,
                                p.nlm, p.d.shacc, 0, p.d.high_order_radiance ? 1 : 0);
     s.nr_next_fixsh = false;
+  }
+  // graphs replay the level-ordered sweep; the point-ordered one (grids whose
+  // ring does not fit in LDS, and the test options) iterates from the host
+  if (s.use_iter_graph && !verbose && ring_active(s) && s.iters < maxiter && s.solcrit > p.d.solacc) {
+    TRY(state_solve_graph(s, maxiter, err));
+    s.last_solve_graph = true;
+    launch_fluxes(s.stream, s.dorad, s.nptsp, s.col_of_ref_d, s.fluxw, s.ndown, p.nang, p.npts, s.fluxes);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s.stream));
+    return AT3D_OK;
   }
   while (s.iters < maxiter && s.solcrit > p.d.solacc) {
     s.iters++;

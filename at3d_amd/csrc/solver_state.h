@@ -30,6 +30,10 @@ struct State {
   int nlevels_max = 0;
   // transforms
   float *Y = nullptr, *W = nullptr, *ysun = nullptr;
+  // W with the upwelling rows zeroed / with the downwelling rows zeroed: the
+  // two hemispheres' shares of DO->SH
+  float *Wdn = nullptr, *Wup = nullptr;
+  int n_sh_tiles_dn = 0;       // SH->DO tiles of the downwelling octants (the first ones)
   int* lofj = nullptr;
   // optics
   float *ext = nullptr, *alb = nullptr, *chi = nullptr, *tc = nullptr, *dirflux = nullptr, *f0 = nullptr;
@@ -61,6 +65,7 @@ struct State {
   // level-ordered sweep (k_sweep_ring)
   bool ring_ok = false;        // ring fits in LDS and records are built
   int use_ring = 1;            // option: sweep with k_sweep_ring when ring_ok
+  int sweep_prefetch = 1;      // option: levels of record loads in flight (1 or 2)
   int slots = 0, sector_points = 8, gld = 0, ngeneral = 0, nsectors = 0;
   int ring_nlev[8] = {}, sec_base[8] = {};
   size_t ring_lds = 0;         // dynamic LDS bytes of k_sweep_ring's fast instantiation (0: unused)
@@ -93,6 +98,23 @@ struct State {
   hipGraph_t sweep_graph = nullptr;
   hipGraphExec_t sweep_exec = nullptr;
   int graph_general = -1, graph_ring = -1;
+  // iterations replayed from captured graphs under device control (IterCtl):
+  // ITER_BATCH iterations per graph, one graph per NR buffer parity
+  static const int ITER_BATCH = 4;
+  IterCtl* ctl = nullptr;      // device
+  IterCtl* ctl_h = nullptr;    // pinned host mirror
+  int use_iter_graph = 1;      // option "iteration_graph"
+  int* nr_buf0 = nullptr;      // the first NR buffer (the graphs' parity reference)
+  hipGraph_t iter_graph[2] = {};
+  hipGraphExec_t iter_exec[2] = {};
+  int iter_key[2] = {-1, -1};  // sweep configuration each graph was captured with
+  // the hemisphere overlap (option "overlap"): a second stream runs the
+  // upwelling SH->DO during the downwelling sweep and the downwelling share
+  // of DO->SH during the upwelling sweep; events per iteration slot
+  int use_overlap = 0;
+  hipStream_t stream2 = nullptr;
+  hipEvent_t oev[ITER_BATCH][4] = {};
+  bool last_solve_graph = false;   // the last solve call ran its iterations from graphs
   // iteration state
   bool initialized = false;
   int iters = 0, iters_last = 0, nsweep_launch = 0;

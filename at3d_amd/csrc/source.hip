@@ -45,6 +45,8 @@ __global__ void k_radiance_truncation(const float* __restrict__ R, int ld, const
 // flight)
 template <int NT>
 __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
+  if (a.ctl && a.ctl->stop) return;
+  const int fixsh = a.ctl ? a.ctl->fixsh : a.fixsh;
   constexpr int SRC_TERMS = NT;
   __shared__ double red[4][4];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -92,7 +94,7 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
     // the truncation constants of the next radiance truncation, loaded with
     // the point's rows (the first 64 orders)
     const float* tcp = a.tc + (size_t)p * (a.ml + 1);
-    const float tcv = (!a.fixsh && 1 + lane <= a.ml) ? tcp[1 + lane] : 0.f;
+    const float tcv = (!fixsh && 1 + lane <= a.ml) ? tcp[1 + lane] : 0.f;
     float rv[SRC_TERMS], sv[SRC_TERMS], dv[SRC_TERMS], cv[SRC_TERMS];
 #pragma unroll
     for (int it = 0; it < SRC_TERMS; it++) {
@@ -150,7 +152,7 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
       }
     }
     int nsn;
-    if (a.fixsh) {
+    if (fixsh) {
       nsn = ns_old;
     } else {
       for (int off = 32; off > 0; off >>= 1) js = max(js, __shfl_xor(js, off));
@@ -170,7 +172,7 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
     // shdomsub1.f:1615-1803, as k_radiance_truncation): it reads NS as just
     // written and the radiance's first term (lane 0's first term here)
     int nrn;
-    if (a.fixsh) {
+    if (fixsh) {
       nrn = a.highorder ? nterms_for_l(a.ml, a.mm) : max(4, nsn);
     } else {
       const float i0 = __shfl(rv[0], 0);
@@ -228,7 +230,9 @@ __global__ __launch_bounds__(256) void k_init_radiance(float* __restrict__ R, co
 
 // Fixed-order reduction of the block partials.
 __global__ __launch_bounds__(1024) void k_reduce_dots(const double* __restrict__ partial, int nblocks,
-                                                      double* __restrict__ out) {
+                                                      double* __restrict__ out,
+                                                      const int* __restrict__ stop) {
+  if (stop && *stop) return;
   __shared__ double sh[1024];
   const int c = threadIdx.x & 3, t = threadIdx.x >> 2;  // 256 threads per component
   double s = 0;
@@ -246,7 +250,12 @@ __global__ __launch_bounds__(1024) void k_reduce_dots(const double* __restrict__
 // its first min(NS, NOS) terms.
 __global__ __launch_bounds__(256) void k_accelerate(float* __restrict__ S, const float* __restrict__ DS,
                                                     const int* __restrict__ ns, const int* __restrict__ nos,
-                                                    int ld, int npts, float a) {
+                                                    int ld, int npts, float a,
+                                                    const IterCtl* __restrict__ ctl) {
+  if (ctl) {
+    if (ctl->stop || !(ctl->a > 0.0f)) return;
+    a = ctl->a;
+  }
   const int lane = threadIdx.x & 63;
   const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (p >= npts) return;
@@ -274,13 +283,69 @@ void launch_init_radiance(hipStream_t st, float* R, const float* R0, int* nr, in
   k_init_radiance<<<(npts + 3) / 4, 256, 0, st>>>(R, R0, nr, ld, npts);
 }
 
-void launch_reduce_dots(hipStream_t st, const double* partial, int nblocks, double* out) {
-  k_reduce_dots<<<1, 1024, 0, st>>>(partial, nblocks, out);
+void launch_reduce_dots(hipStream_t st, const double* partial, int nblocks, double* out,
+                        const int* stop) {
+  k_reduce_dots<<<1, 1024, 0, st>>>(partial, nblocks, out, stop);
 }
 
 void launch_accelerate(hipStream_t st, float* S, const float* DS, const int* ns, const int* nos,
-                       int ld, int npts, float a) {
-  k_accelerate<<<(npts + 3) / 4, 256, 0, st>>>(S, DS, ns, nos, ld, npts, a);
+                       int ld, int npts, float a, const IterCtl* ctl) {
+  k_accelerate<<<(npts + 3) / 4, 256, 0, st>>>(S, DS, ns, nos, ld, npts, a, ctl);
+}
+
+// The loop test of SOLUTION_ITERATIONS (shdomsub1.f:674) and FIXSH
+// (:679-681; FIXSH once SOLCRIT < 0.001 or past ITERFIXSH iterations).
+__global__ void k_iter_begin(IterCtl* ctl) {
+  if (threadIdx.x != 0 || ctl->stop) return;
+  if (ctl->total >= ctl->maxiter || ctl->solcrit <= ctl->solacc) {
+    ctl->stop = 1;
+    return;
+  }
+  ctl->total = ctl->total + 1;
+  ctl->iters = ctl->iters + 1;
+  if (ctl->solcrit < 0.001f || ctl->total > ctl->iterfixsh) ctl->fixsh = 1;
+}
+
+// CALC_ACCEL_SOLCRIT (shdom_nompi.f:317-349) in single precision from the
+// reduced FP64 dot products: the acceleration parameter of this iteration
+// (alternating: none after an accelerated iteration) and SOLCRIT; with
+// ALBMAX < SOLACC one iteration is enough (shdomsub1.f:787).  An error flag
+// raised by a kernel of the iteration stops the solve.
+__global__ void k_solcrit(IterCtl* ctl, const double* __restrict__ dots, const int* __restrict__ errflag) {
+  if (threadIdx.x != 0 || ctl->stop) return;
+  const float deljdot = (float)dots[0], deljold = (float)dots[1], deljnew = (float)dots[2],
+              jnorm = (float)dots[3];
+  float a = ctl->a;
+  if (ctl->accel && a == 0.0f && deljnew < deljold) {
+    const float r = sqrtf(deljnew / deljold);
+    float c = deljdot / sqrtf(deljold * deljnew);
+    c = fmaxf(-1.0f, fminf(1.0f, c));
+    const float theta = acosf(c);
+    a = (1 - r * cosf(theta) + powf(r, 1 + 0.5f * 3.14159f / theta)) / (1 + r * r - 2 * r * cosf(theta)) -
+        1.0f;
+    a = fminf(10.0f, fmaxf(0.0f, a));
+  } else {
+    a = 0.0f;
+  }
+  ctl->a = a;
+  float solcrit = ctl->solcrit;
+  if (jnorm > 0.0f)
+    solcrit = sqrtf(deljnew / jnorm);
+  else if (deljnew == 0.0f)
+    solcrit = 0.0f;
+  if (ctl->one_iter) solcrit = ctl->solacc;
+  ctl->solcrit = solcrit;
+  ctl->deljdot = deljdot;
+  ctl->deljold = deljold;
+  ctl->deljnew = deljnew;
+  ctl->jnorm = jnorm;
+  if (*errflag) ctl->stop = 2;
+}
+
+void launch_iter_begin(hipStream_t st, IterCtl* ctl) { k_iter_begin<<<1, 64, 0, st>>>(ctl); }
+
+void launch_solcrit(hipStream_t st, IterCtl* ctl, const double* dots, const int* errflag) {
+  k_solcrit<<<1, 64, 0, st>>>(ctl, dots, errflag);
 }
 
 }  // namespace at3d

@@ -177,6 +177,7 @@ constexpr int SWEEP_K = 4;   // items per thread whose static part is prepared a
 // upwind radiances are gathered.  Multi-cell traces and levels larger than
 // SWEEP_K * SWEEP_THREADS points take the unprepared path.
 __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_columns(SweepArgs a, int col0) {
+  if (a.stop && *a.stop) return;
   const int col = col0 + blockIdx.x;
   const int o = a.col_oct[col];
   const OrdGeom og = a.ordg[col];
@@ -462,8 +463,17 @@ struct RingSet {
 // but the record prefetch, and runs the columns (and grids) that need none
 // of them.  Each launch covers a hemisphere's columns; a workgroup whose
 // column belongs to the other instantiation returns at once.
-template <bool FULL>
+//
+// DEPTH is how many levels ahead the records are loaded: 1 issues level
+// L+1's loads while level L is computed; 2 (fast instantiation only) keeps
+// two levels of loads in flight: level L+2's are issued at the end of level
+// L, after its stores, and every register slot loads unconditionally (lanes
+// past the level's end take its last entry), so that the waits the compiler
+// places before level L+1 cover level L+1's loads and not level L+2's.
+template <bool FULL, int DEPTH>
 __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int col0) {
+  static_assert(DEPTH == 1 || (DEPTH == 2 && !FULL), "two-level prefetch only in the fast instantiation");
+  if (a.stop && *a.stop) return;
   extern __shared__ float4 lds4[];
   const int col = col0 + blockIdx.x;
   if (FULL != (a.ring_full || a.col_general[col] != 0)) return;
@@ -583,13 +593,15 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
     const int beg = __builtin_amdgcn_readfirstlane(off[min(L - 1, nlev)]);
     const int end = __builtin_amdgcn_readfirstlane(off[min(L, nlev)]);
     const int last = max(end - 1, 0);
+    if (DEPTH == 1) {
 #pragma unroll
-    for (int k = 0; k < RING_K; k++) {
-      if (beg + wfirst + k * RING_THREADS < end) {
-        const int jl = min(beg + tid + k * RING_THREADS, last);
-        S.w0[k] = rec0[jl];
-        S.s[k] = se[jl];
-        S.e[k] = ep[jl];
+      for (int k = 0; k < RING_K; k++) {
+        if (beg + wfirst + k * RING_THREADS < end) {
+          const int jl = min(beg + tid + k * RING_THREADS, last);
+          S.w0[k] = rec0[jl];
+          S.s[k] = se[jl];
+          S.e[k] = ep[jl];
+        }
       }
     }
     S.sec_ok = false;
@@ -600,6 +612,17 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
         const int sct = min(s0 + tid, s1 - 1);  // clamped, used only if sec_ok
         S.sec = sslot[sct];
         S.sec_p0 = sp0[sct];
+      }
+    }
+    if (DEPTH == 2) {
+      // after the (conditional) sector loads: the records' loads are the
+      // last of the level and always issued
+#pragma unroll
+      for (int k = 0; k < RING_K; k++) {
+        const int jl = min(beg + tid + k * RING_THREADS, last);
+        S.w0[k] = rec0[jl];
+        S.s[k] = se[jl];
+        S.e[k] = ep[jl];
       }
     }
   };
@@ -671,7 +694,7 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
         sw.w = ring[sl.w >> 16].x;
       }
     }
-    load(L + 1, fut);  // past the last level this loads nothing
+    if (DEPTH == 1) load(L + 1, fut);  // past the last level this loads nothing
     uint32_t gen = 0;  // this thread's multi-cell traces among its first RING_K entries
 #pragma unroll
     for (int k = 0; k < RING_K; k++) {
@@ -719,20 +742,33 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
       }
     }
     if (FULL && staging && L > 1 && soffl[L] - soffl[L - 1] > RING_THREADS) flush_rest(L - 1, RING_THREADS);
+    if (DEPTH == 2) load(L + 2, fut);  // into the set of level L-1, after this level's stores
     if (FULL && L < nlev && __builtin_amdgcn_readfirstlane(full[L + 1]))
       __syncthreads();
     else
       lds_barrier();
   };
-  RingSet S0, S1;
+  RingSet S0, S1, S2;
   __syncthreads();
   load(1, S0);
+  if (DEPTH == 2) load(2, S1);
   int L = 1;
-  for (; L + 1 <= nlev; L += 2) {
-    step(L, S0, S1);
-    step(L + 1, S1, S0);
+  if (DEPTH == 1) {
+    for (; L + 1 <= nlev; L += 2) {
+      step(L, S0, S1);
+      step(L + 1, S1, S0);
+    }
+    if (L == nlev) step(L, S0, S1);
+  } else {
+    // S0 holds level L and S1 level L+1 at the top of each round
+    for (; L + 2 <= nlev; L += 3) {
+      step(L, S0, S2);
+      step(L + 1, S1, S0);
+      step(L + 2, S2, S1);
+    }
+    if (L <= nlev) step(L, S0, S2);
+    if (L + 1 <= nlev) step(L + 1, S1, S0);
   }
-  if (L == nlev) step(L, S0, S1);
   if (staging && nlev >= 1) flush_rest(nlev, 0);
 }
 
@@ -740,6 +776,7 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
 // downwelling ordinates, surface radiance at the bottom points for upwelling
 // ones (PATH_INTEGRATION shdomsub1.f:2042-2086).
 __global__ void k_sweep_preset(SweepArgs a, int up, int col0, int ncol) {
+  if (a.stop && *a.stop) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int np = up ? a.nbot : a.ntop;
   if (i >= np * ncol) return;
@@ -759,7 +796,9 @@ __global__ __launch_bounds__(256) void k_bottom_bc(const float* __restrict__ rad
                                                    const int32_t* __restrict__ col_of_ref,
                                                    const float* __restrict__ fluxw, int ndown,
                                                    const float* __restrict__ dirflux, float albpi,
-                                                   float* __restrict__ bcrad, BottomRecs br) {
+                                                   float* __restrict__ bcrad, BottomRecs br,
+                                                   const int* __restrict__ stop) {
+  if (stop && *stop) return;
   const int b = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (b >= nbot) return;
   const int p = bcbot[b];
@@ -828,28 +867,34 @@ void launch_sweep_geo(hipStream_t st, const SweepArgs& a, int ncols) {
 }
 
 void launch_sweep_ring(hipStream_t st, const SweepArgs& a, int col0, int ncol, size_t lds_fast,
-                       size_t lds_full, bool fast, bool full) {
+                       size_t lds_full, bool fast, bool full, int depth) {
   if (ncol <= 0) return;
-  if (fast) k_sweep_ring<false><<<ncol, RING_THREADS, lds_fast, st>>>(a, col0);
-  if (full) k_sweep_ring<true><<<ncol, RING_THREADS, lds_full, st>>>(a, col0);
+  if (fast) {
+    if (depth == 2)
+      k_sweep_ring<false, 2><<<ncol, RING_THREADS, lds_fast, st>>>(a, col0);
+    else
+      k_sweep_ring<false, 1><<<ncol, RING_THREADS, lds_fast, st>>>(a, col0);
+  }
+  if (full) k_sweep_ring<true, 1><<<ncol, RING_THREADS, lds_full, st>>>(a, col0);
 }
 
 int sweep_ring_max_lds(size_t lds_fast, size_t lds_full) {
-  const bool ok =
-      (lds_fast == 0 ||
-       hipFuncSetAttribute(reinterpret_cast<const void*>(k_sweep_ring<false>),
-                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_fast) == hipSuccess) &&
-      hipFuncSetAttribute(reinterpret_cast<const void*>(k_sweep_ring<true>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_full) == hipSuccess;
+  const auto set = [](const void* f, size_t b) {
+    return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b) == hipSuccess;
+  };
+  const bool ok = (lds_fast == 0 || (set(reinterpret_cast<const void*>(k_sweep_ring<false, 1>), lds_fast) &&
+                                     set(reinterpret_cast<const void*>(k_sweep_ring<false, 2>), lds_fast))) &&
+                  set(reinterpret_cast<const void*>(k_sweep_ring<true, 1>), lds_full);
   return ok ? 0 : 1;
 }
 
 void launch_bottom_bc(hipStream_t st, const float* rad, int ldp, const int32_t* bcbot, int nbot,
                       const int32_t* col_of_ref, const float* fluxw, int ndown,
-                      const float* dirflux, float albpi, float* bcrad, const BottomRecs& br) {
+                      const float* dirflux, float albpi, float* bcrad, const BottomRecs& br,
+                      const int* stop) {
   if (nbot <= 0) return;
   k_bottom_bc<<<(nbot + 3) / 4, 256, 0, st>>>(rad, ldp, bcbot, nbot, col_of_ref, fluxw, ndown, dirflux,
-                                               albpi, bcrad, br);
+                                               albpi, bcrad, br, stop);
 }
 
 void launch_fluxes(hipStream_t st, const float* rad, int ldp, const int32_t* col_of_ref,

@@ -54,7 +54,8 @@ __global__ __launch_bounds__(256) void k_sh_to_do(const float* __restrict__ S,
                                                   const float* __restrict__ Yc,
                                                   const float* __restrict__ ext,
                                                   float* __restrict__ out, int ld, int ldo,
-                                                  int npts, int nang) {
+                                                  int npts, int nang, const int* __restrict__ stop) {
+  if (stop && *stop) return;
   __shared__ float sA[2][TK][TM + PAD];
   __shared__ float sB[2][TK][TN + PAD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -129,7 +130,8 @@ __global__ __launch_bounds__(256) void k_sh_to_do_perm(const float* __restrict__
                                                        const int32_t* __restrict__ oct_perm,
                                                        const int4* __restrict__ tiles,
                                                        float* __restrict__ out, int ld, int ldp,
-                                                       int npts) {
+                                                       int npts, const int* __restrict__ stop) {
+  if (stop && *stop) return;
   __shared__ float sA[2][TK][TM + PAD];
   __shared__ float sB[2][TK][PN + PAD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -206,7 +208,9 @@ __global__ __launch_bounds__(256) void k_do_to_sh(const float* __restrict__ D,
                                                   const float* __restrict__ Wc,
                                                   const int* __restrict__ nr,
                                                   float* __restrict__ R, int ld, int ldo, int ldp,
-                                                  int npts) {
+                                                  int npts, int kbeg, int kend, int accumulate,
+                                                  const int* __restrict__ stop) {
+  if (stop && *stop) return;
   __shared__ float sA[2][TK][TM + PAD];
   __shared__ float sB[2][TK][TN + PAD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -249,15 +253,15 @@ __global__ __launch_bounds__(256) void k_do_to_sh(const float* __restrict__ D,
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++) acc[i][j] = (float4_){0.f, 0.f, 0.f, 0.f};
   const int bk = tid >> 4, bc = (tid & 15) * 4;    // 16 k x 64 rows/cols per tile
-  const float* acol = D + (size_t)bk * ldp + m0 + bc;
-  const float* bcol = Wc + (size_t)bk * ld + n0 + bc;
+  const float* acol = D + (size_t)(kbeg + bk) * ldp + m0 + bc;
+  const float* bcol = Wc + (size_t)(kbeg + bk) * ld + n0 + bc;
   float4 a = *reinterpret_cast<const float4*>(acol);
   float4 b = *reinterpret_cast<const float4*>(bcol);
-  for (int k0 = 0, buf = 0; k0 < ldo; k0 += TK, buf ^= 1) {
+  for (int k0 = 0, buf = 0; k0 < kend - kbeg; k0 += TK, buf ^= 1) {
     *reinterpret_cast<float4*>(&sA[buf][bk][bc]) = a;
     *reinterpret_cast<float4*>(&sB[buf][bk][bc]) = b;
     __syncthreads();
-    if (k0 + TK < ldo) {
+    if (k0 + TK < kend - kbeg) {
       a = *reinterpret_cast<const float4*>(acol + (size_t)(k0 + TK) * ldp);
       b = *reinterpret_cast<const float4*>(bcol + (size_t)(k0 + TK) * ld);
     }
@@ -276,35 +280,40 @@ __global__ __launch_bounds__(256) void k_do_to_sh(const float* __restrict__ D,
       const int p = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + v;
       for (int j = 0; j < 2; j++) {
         const int J = n0 + wn * 32 + j * 16 + (lane & 15);
-        R[(size_t)p * ld + J] = (J < n[i][v]) ? acc[i][j][v] : 0.0f;
+        float* r = R + (size_t)p * ld + J;
+        *r = (J < n[i][v]) ? (accumulate ? *r + acc[i][j][v] : acc[i][j][v]) : 0.0f;
       }
     }
 }
 
 void launch_sh_to_do(hipStream_t st, const float* S, const float* Yc, const float* ext,
-                     float* dosrc, int mp, int ncp, int ld, int ldp, int npts, int nang) {
+                     float* dosrc, int mp, int ncp, int ld, int ldp, int npts, int nang,
+                     const int* stop) {
   dim3 grid(ncp / TN, mp / TM);
-  k_sh_to_do<true><<<grid, 256, 0, st>>>(S, Yc, ext, dosrc, ld, ldp, npts, nang);
+  k_sh_to_do<true><<<grid, 256, 0, st>>>(S, Yc, ext, dosrc, ld, ldp, npts, nang, stop);
 }
 
 void launch_sh_dot(hipStream_t st, const float* S, const float* Yd, float* raw, int mp, int ndp,
                    int ld, int ldd, int npts, int ndir) {
   dim3 grid(ndp / TN, mp / TM);
-  k_sh_to_do<false><<<grid, 256, 0, st>>>(S, Yd, nullptr, raw, ld, ldd, npts, ndir);
+  k_sh_to_do<false><<<grid, 256, 0, st>>>(S, Yd, nullptr, raw, ld, ldd, npts, ndir, nullptr);
 }
 
 void launch_sh_to_do_perm(hipStream_t st, const float* S, const float* Yc, const float* ext,
                           const int* ns, const int32_t* oct_perm, const int4* tiles, int ntiles,
-                          float* dosrc, int mp, int ld, int ldp, int npts) {
+                          float* dosrc, int mp, int ld, int ldp, int npts, const int* stop) {
   if (ntiles <= 0) return;
   dim3 grid(mp / TM, ntiles);
-  k_sh_to_do_perm<<<grid, 256, 0, st>>>(S, Yc, ext, ns, oct_perm, tiles, dosrc, ld, ldp, npts);
+  k_sh_to_do_perm<<<grid, 256, 0, st>>>(S, Yc, ext, ns, oct_perm, tiles, dosrc, ld, ldp, npts, stop);
 }
 
 void launch_do_to_sh(hipStream_t st, const float* dorad, const float* Wc, const int* nr, float* R,
-                     int mp, int ld, int ldo, int ldp, int npts) {
+                     int mp, int ld, int ldo, int ldp, int npts, int c0, int c1, int accumulate,
+                     const int* stop) {
+  if (c1 < 0) c1 = ldo;
+  if (c1 <= c0) return;
   dim3 grid(ld / TN, mp / TM);
-  k_do_to_sh<<<grid, 256, 0, st>>>(dorad, Wc, nr, R, ld, ldo, ldp, npts);
+  k_do_to_sh<<<grid, 256, 0, st>>>(dorad, Wc, nr, R, ld, ldo, ldp, npts, c0, c1, accumulate, stop);
 }
 
 }  // namespace at3d

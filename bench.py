@@ -325,14 +325,52 @@ def main():
         ts = time.perf_counter()
         rte.solve(w.maxiter, verbose=False)
         lat.append((time.perf_counter() - ts) * 1e3)
-        for k in keys:
-            ksum[k] += rte.timing(k)
         iters_total += int(rte.timing("iters"))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     ms_per_step = elapsed * 1e3 / args.steps
     iters = iters_total // args.steps
+    graphs = bool(rte.timing("iteration_graph_active"))
     print("bench: plain solves done", file=sys.stderr, flush=True)
+    # per-kernel times: the same solves with the iterations launched one by
+    # one, HIP events on the state's stream between the kernel classes and
+    # around each sweep kernel launch (the kernels are those the graphs run)
+    rte.set_option("iteration_graph", 0)
+    kiters = 0
+    for _ in range(args.steps):
+        rte.solve(w.maxiter, verbose=False)
+        for k in keys:
+            ksum[k] += rte.timing(k)
+        kiters += int(rte.timing("iters"))
+    rte.set_option("iteration_graph", 1)
+    torch.cuda.synchronize()
+
+    # engine variants of the same plain solve (for the design record): the
+    # iterations launched one by one with the host deciding after each, and
+    # the hemisphere overlap on a second stream
+    variants = {}
+    for vname, opts, reset in (("host_controlled_iterations", {"iteration_graph": 0}, {"iteration_graph": 1}),
+                               ("hemisphere_overlap", {"overlap": 1}, {"overlap": 0}),
+                               ("sweep_prefetch_2", {"sweep_prefetch": 2}, {"sweep_prefetch": 1})):
+        for k, v in opts.items():
+            rte.set_option(k, v)
+        rte.solve(w.maxiter, verbose=False)
+        torch.cuda.synchronize()
+        tv = time.perf_counter()
+        for _ in range(args.steps):
+            rte.solve(w.maxiter, verbose=False)
+        torch.cuda.synchronize()
+        variants[vname] = {"ms_per_solve": (time.perf_counter() - tv) * 1e3 / args.steps,
+                           "iterations": rte.num_iterations}
+        if vname == "sweep_prefetch_2":
+            # its sweep kernel time, from a solve launched iteration by iteration
+            rte.set_option("iteration_graph", 0)
+            rte.solve(w.maxiter, verbose=False)
+            variants[vname]["sweep_kernel_ms_per_solve"] = rte.timing("sweep_kernel")
+            rte.set_option("iteration_graph", 1)
+        for k, v in reset.items():
+            rte.set_option(k, v)
+    print("bench: variants done", file=sys.stderr, flush=True)
 
     # served: the same number of solves over several solvers in flight, one
     # stream and host thread each, each taking the next solve when it is free
@@ -375,7 +413,7 @@ def main():
     ns = rte.state_array("ns").astype(np.float64)
     nr = rte.state_array("nr").astype(np.float64)
     kern = {}
-    n_it = max(iters_total, 1)
+    n_it = max(kiters, 1)
     # one sweep pass = one launch per hemisphere (a workgroup per ordinate walking
     # the octant's dependency levels); algorithmic bytes per (ordinate, point):
     # read the DO source and the extinction, write the DO radiance (3 x 4 B)
@@ -465,7 +503,9 @@ def main():
                   "latency_ms": {"p50": float(np.percentile(lat, 50)),
                                  "p99": float(np.percentile(lat, 99))}},
         "cold": {"solves": cold, "median_ms": cold_ms},
+        "variants": variants,
         "kernel_ms_per_solve": {k: ksum[k] / args.steps for k in keys},
+        "iterations_from_graphs": graphs,
         "kernels": kern,
         "roofline": roof,
         "solve_roofline": solve_roof,

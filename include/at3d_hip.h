@@ -243,6 +243,15 @@ int at3d_levis_gradient_jacobian(at3d_state* st, const at3d_deriv_desc* deriv, i
  * "sweep_ring" (0/1, default 1): use the level-ordered sweep with per-solve
  * step records and an LDS ring when the ring fits (else the point-ordered
  * sweep).  "sweep_graph" (0/1): replay the sweep launches from a HIP graph.
+ * "iteration_graph" (0/1, default 1): run a solve's iterations from captured
+ * HIP graphs of four iterations each under device control (the convergence
+ * test, FIXSH and the acceleration on the device; the host reads the control
+ * block once per four iterations) when the level-ordered sweep is in use; 0,
+ * verbose, or the point-ordered sweep runs them one by one with the host
+ * deciding after each.  "overlap" (0/1, default 0): with iteration graphs, a
+ * second stream computes the upwelling SH->DO during the downwelling sweep
+ * and the downwelling share of DO->SH during the upwelling sweep.
+ * "sweep_prefetch" (1 or 2, default 1): levels of sweep records in flight.
  * "max_subgrid_intervals": sub-step intervals one gradient ray may record.
  * No counterpart in the reference. */
 int at3d_state_set_option(at3d_state* st, const char* key, int32_t value, char* err,
