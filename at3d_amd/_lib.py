@@ -90,6 +90,10 @@ def lib():
         L.at3d_state_save.argtypes = [vp, vp, vp, vp, C.c_int64, vp, C.c_int64, vp, C.c_char_p,
                                       C.c_size_t]
         L.at3d_state_sh_terms.argtypes = [vp, C.c_int32, vp, vp, C.c_char_p, C.c_size_t]
+        L.at3d_render_noalloc.argtypes = [vp, C.c_int64, vp, vp, vp, vp, vp, C.c_int32, C.c_int32,
+                                          vp, C.c_char_p, C.c_size_t]
+        L.at3d_render_scratch_bytes.argtypes = []
+        L.at3d_render_scratch_bytes.restype = C.c_int64
         L.at3d_levis_gradient.argtypes = [vp, C.POINTER(DerivDesc), C.c_int64, vp, vp, vp, vp, vp,
                                           C.c_int64, vp, vp, vp, vp, vp, C.c_char_p, C.c_int32,
                                           vp, vp, vp, C.c_char_p, C.c_size_t]
@@ -118,7 +122,8 @@ EXPORTS = ["at3d_version", "at3d_plan_create", "at3d_plan_destroy", "at3d_plan_s
            "at3d_solve", "at3d_state_get", "at3d_state_set_option", "at3d_state_timing",
            "at3d_render", "at3d_levis_gradient", "at3d_levis_gradient_jacobian",
            "at3d_init_solution_from", "at3d_device_trim", "at3d_state_load", "at3d_state_save",
-           "at3d_state_sh_terms"]
+           "at3d_state_sh_terms", "at3d_render_noalloc",
+           "at3d_render_scratch_bytes"]
 
 
 def as_f32(a):

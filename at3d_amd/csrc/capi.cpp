@@ -453,6 +453,28 @@ int at3d_state_sh_terms(at3d_state* st, int32_t nshout, float* shterms, float* n
   return rc;
 }
 
+int at3d_render_noalloc(at3d_state* st, int64_t nrays, const float* x, const float* y,
+                        const float* z, const double* mu, const double* phi, int32_t single_scatter,
+                        int32_t nosurface, float* out, char* err, size_t errlen) {
+  if (!st || (nrays > 0 && (!x || !y || !z || !mu || !phi || !out))) {
+    set_err(err, errlen, "null argument");
+    return AT3D_ERR_ARG;
+  }
+  std::string msg;
+  int rc;
+  try {
+    rc = at3d::state_render_noalloc(st->s, nrays, x, y, z, mu, phi, single_scatter, nosurface, out,
+                                    msg);
+  } catch (const std::bad_alloc&) {
+    rc = AT3D_ERR_MEMORY;
+    msg = "out of host memory in render";
+  }
+  if (rc) set_err(err, errlen, msg);
+  return rc;
+}
+
+int64_t at3d_render_scratch_bytes(void) { return (int64_t)at3d::RENDER_SCRATCH_BYTES; }
+
 int at3d_levis_gradient_jacobian(at3d_state* st, const at3d_deriv_desc* deriv, int64_t nrays,
                                  const float* x, const float* y, const float* z, const double* mu,
                                  const double* phi, int64_t npix, const int32_t* rays_per_pixel,

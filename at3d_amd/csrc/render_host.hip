@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <map>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -92,9 +93,13 @@ int direction_tables(const State& s, int64_t nrays, const double* mu, const doub
   return AT3D_OK;
 }
 
-int state_render(State& s, int64_t nrays, const float* x, const float* y, const float* z,
-                 const double* mu, const double* phi, int single_scatter, int nosurface,
-                 float* out, std::string& err) {
+// The render body.  Its device tables (direction harmonics and factors, the
+// direction source of every grid point, the rays and their radiances) are
+// carved from one block: the state's render buffer, grown on demand, or, when
+// `fixed` is given, that block of `fixed_cap` bytes and nothing else.
+static int render_impl(State& s, int64_t nrays, const float* x, const float* y, const float* z,
+                       const double* mu, const double* phi, int single_scatter, int nosurface,
+                       float* out, void* fixed, size_t fixed_cap, std::string& err) {
   const Plan& p = *s.plan;
   if (nrays <= 0) return AT3D_OK;
   if (nrays > (int64_t)1 << 30) {
@@ -143,16 +148,30 @@ int state_render(State& s, int64_t nrays, const float* x, const float* y, const 
   const size_t o_x = carve(nr * 4), o_y = carve(nr * 4), o_z = carve(nr * 4);
   const size_t o_mu = carve(nr * 8), o_phi = carve(nr * 8), o_dir = carve(nr * 4);
   const size_t o_top = carve(nr * 4), o_out = carve(nr * 4);
-  if (off > s.rbuf_bytes) {
-    // the stream's earlier work may still read the old buffer
-    HIPCHK(hipStreamSynchronize(s.stream));
-    dev_free(s.rbuf, s.rbuf_bytes);
-    s.rbuf = nullptr;
-    s.rbuf_bytes = 0;
-    HIPCHK(dev_alloc(&s.rbuf, off));
-    s.rbuf_bytes = off;
+  char* B = nullptr;
+  if (fixed) {
+    if (off > fixed_cap) {
+      char buf[200];
+      snprintf(buf, sizeof buf,
+               "render needs %zu bytes of device scratch (%d grid points x %d directions, %lld rays); "
+               "the allocation-free entry point holds %zu",
+               off, p.npts, ndir, (long long)nrays, fixed_cap);
+      err = buf;
+      return AT3D_ERR_ARG;
+    }
+    B = static_cast<char*>(fixed);
+  } else {
+    if (off > s.rbuf_bytes) {
+      // the stream's earlier work may still read the old buffer
+      HIPCHK(hipStreamSynchronize(s.stream));
+      dev_free(s.rbuf, s.rbuf_bytes);
+      s.rbuf = nullptr;
+      s.rbuf_bytes = 0;
+      HIPCHK(dev_alloc(&s.rbuf, off));
+      s.rbuf_bytes = off;
+    }
+    B = static_cast<char*>(s.rbuf);
   }
-  char* B = static_cast<char*>(s.rbuf);
   HIPCHK(hipMemcpyAsync(B + o_yd, Yd.data(), Yd.size() * 4, hipMemcpyHostToDevice, s.stream));
   HIPCHK(hipMemcpyAsync(B + o_g, G.data(), G.size() * 4, hipMemcpyHostToDevice, s.stream));
   if (!ssd.empty())
@@ -228,6 +247,52 @@ int state_render(State& s, int64_t nrays, const float* x, const float* y, const 
     return AT3D_ERR_NUMERIC;
   }
   return AT3D_OK;
+}
+
+int state_render(State& s, int64_t nrays, const float* x, const float* y, const float* z,
+                 const double* mu, const double* phi, int single_scatter, int nosurface,
+                 float* out, std::string& err) {
+  return render_impl(s, nrays, x, y, z, mu, phi, single_scatter, nosurface, out, nullptr, 0, err);
+}
+
+// Device scratch of the allocation-free render (at3d_render_noalloc): a
+// file-scope array every GPU holds a copy of, at its own address, from the
+// library's load until the process ends.  Host code never passes `scratch`
+// itself to a kernel or copy (its host name is only a stand-in): the device
+// address of each GPU's copy is read once, with that GPU current, and kept.
+static __device__ __align__(256) unsigned char scratch[RENDER_SCRATCH_BYTES];
+
+static std::mutex scratch_mutex;
+static std::vector<void*> scratch_ptr;   // per device ordinal
+
+static int scratch_address(int device, void** ptr, std::string& err) {
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if ((int)scratch_ptr.size() <= device) scratch_ptr.resize(device + 1, nullptr);
+  if (!scratch_ptr[device]) {
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != device) HIPCHK(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) HIPCHK(hipSetDevice(cur));
+    if (e != hipSuccess || !p) {
+      err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return AT3D_ERR_DEVICE;
+    }
+    scratch_ptr[device] = p;
+  }
+  *ptr = scratch_ptr[device];
+  return AT3D_OK;
+}
+
+int state_render_noalloc(State& s, int64_t nrays, const float* x, const float* y, const float* z,
+                         const double* mu, const double* phi, int single_scatter, int nosurface,
+                         float* out, std::string& err) {
+  void* buf = nullptr;
+  const int rc = scratch_address(s.device, &buf, err);
+  if (rc) return rc;
+  return render_impl(s, nrays, x, y, z, mu, phi, single_scatter, nosurface, out, buf,
+                     RENDER_SCRATCH_BYTES, err);
 }
 
 }  // namespace at3d

@@ -365,6 +365,7 @@ size_t plan_footprint_bytes(const Plan& p) {
 
 int state_create(const Plan& p, State& s, std::string& err) {
   s.plan = &p;
+  HIPCHK(hipGetDevice(&s.device));
   HIPCHK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
   s.npts = p.npts;
   s.nlm = p.nlm;

@@ -14,8 +14,15 @@
 
 namespace at3d {
 
+// Device scratch bytes of the allocation-free render (at3d_render_noalloc):
+// the direction source of every grid point for every distinct ray direction
+// plus the rays; 64 MiB holds a 64 x 64 x 40 grid with up to 64 directions
+// and 2^20 rays.
+constexpr size_t RENDER_SCRATCH_BYTES = (size_t)64 << 20;
+
 struct State {
   const Plan* plan = nullptr;
+  int device = 0;   // HIP device the state was created on
   hipStream_t stream = nullptr;
   int npts = 0, nlm = 0, nang = 0;
   int ld = 0;      // row stride of SH arrays (nlm rounded up to 64)
@@ -164,6 +171,10 @@ int state_render(State& s, int64_t nrays, const float* x, const float* y, const 
                  const double* mu, const double* phi, int single_scatter, int nosurface,
                  float* out, std::string& err);
 
+
+int state_render_noalloc(State& s, int64_t nrays, const float* x, const float* y, const float* z,
+                         const double* mu, const double* phi, int single_scatter, int nosurface,
+                         float* out, std::string& err);
 
 int state_load(State& s, const int32_t* shptr, const float* source, const int32_t* rshptr,
                const float* radiance, const float* fluxes, std::string& err);

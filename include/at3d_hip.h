@@ -181,6 +181,24 @@ int at3d_state_save(at3d_state* st, int32_t* shptr, int32_t* rshptr, float* sour
 int at3d_state_sh_terms(at3d_state* st, int32_t nshout, float* shterms, float* netfluxdiv,
                         char* err, size_t errlen);
 
+/* at3d_render without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and errors as at3d_render, except
+ * that the call's device tables (the direction source of every grid point
+ * for every distinct ray direction, the rays and their radiances) live in
+ * one library-wide device scratch array of at3d_render_scratch_bytes()
+ * bytes per GPU (64 MiB), held from the library's load until the process
+ * ends; a call that needs more fails with AT3D_ERR_ARG before any device
+ * work.  The per-state render buffer of at3d_render is not touched.  Calls
+ * on one GPU share that array: calls that overlap on one GPU (from several
+ * host threads, or on several states of that GPU) race and must be
+ * serialised by the caller. */
+int at3d_render_noalloc(at3d_state* st, int64_t nrays, const float* x, const float* y,
+                        const float* z, const double* mu, const double* phi, int32_t single_scatter,
+                        int32_t nosurface, float* out, char* err, size_t errlen);
+int64_t at3d_render_scratch_bytes(void);
+
 /* Unknowns of a gradient: NUMDER partial derivatives of the optical properties
  * of one species each, on the property grid (what
  * RTE.calculate_microphysical_partial_derivatives, at3d/solver.py:1327-1516,
