@@ -540,18 +540,6 @@ int at3d_state_set_option(at3d_state* st, const char* key, int32_t value, char* 
     st->s.use_iter_graph = value ? 1 : 0;
     return AT3D_OK;
   }
-  if (k == "overlap") {
-    st->s.use_overlap = value ? 1 : 0;
-    return AT3D_OK;
-  }
-  if (k == "sweep_prefetch") {
-    if (value != 1 && value != 2) {
-      set_err(err, errlen, "sweep_prefetch must be 1 or 2");
-      return AT3D_ERR_ARG;
-    }
-    st->s.sweep_prefetch = value;
-    return AT3D_OK;
-  }
   if (k == "max_subgrid_intervals") {
     st->s.max_subgrid = value > 0 ? value : 0;
     return AT3D_OK;

@@ -43,13 +43,10 @@ void launch_sh_to_do(hipStream_t st, const float* S, const float* Yc, const floa
 void launch_sh_to_do_perm(hipStream_t st, const float* S, const float* Yc, const float* ext,
                           const int* ns, const int32_t* oct_perm, const int4* tiles, int ntiles,
                           float* dosrc, int mp, int ld, int ldp, int npts, const int* stop = nullptr);
-// R[p][J] = (J < nr[p]) ? sum_c dorad[c][p] * Wc[c][J] : 0 over the columns
-// c0 <= c < c1 (multiples of 16 or ldo; Wc rows outside the sum's range must
-// be zero), plus R[p][J] when `accumulate`.
+// R[p][J] = (J < nr[p]) ? sum_c dorad[c][p] * Wc[c][J] : 0
 // dorad [ldo][ldp] (rows beyond nang zero), Wc [ldo][ld] (rows beyond nang zero), R [mp][ld].
 void launch_do_to_sh(hipStream_t st, const float* dorad, const float* Wc, const int* nr, float* R,
-                     int mp, int ld, int ldo, int ldp, int npts, int c0 = 0, int c1 = -1,
-                     int accumulate = 0, const int* stop = nullptr);
+                     int mp, int ld, int ldo, int ldp, int npts, const int* stop = nullptr);
 
 // ---- sweep.hip: discrete-ordinate sweeps --------------------------------
 struct SweepArgs {
@@ -152,9 +149,8 @@ void launch_face_mask(hipStream_t st, const SweepArgs& a);
 void launch_sweep_geo(hipStream_t st, const SweepArgs& a, int ncols);
 // fast / full: launch the instantiation without / with the multi-cell,
 // overflow and extra-sector paths (a column runs in exactly one of them).
-// depth: levels of record loads in flight in the fast instantiation (1 or 2).
 void launch_sweep_ring(hipStream_t st, const SweepArgs& a, int col0, int ncol, size_t lds_fast, size_t lds_full,
-                       bool fast, bool full, int depth = 1);
+                       bool fast, bool full);
 
 void launch_sweep_preset(hipStream_t st, const SweepArgs& a, bool up, int col0, int ncol);
 // One workgroup per ordinate column col0 .. col0+ncol-1, all levels of its octant.

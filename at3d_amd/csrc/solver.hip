@@ -243,8 +243,6 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
       tiles.push_back(make_int4(o, c, std::min(48, s.oct_col_base[o] + s.oct_nord[o] - c), 0));
   }
   s.nsectors = (int)sp0.size();
-  s.n_sh_tiles_dn = 0;   // tiles are in octant order: the downwelling ones first
-  for (const int4& t : tiles) s.n_sh_tiles_dn += t.x < 4 ? 1 : 0;
   // the fast instantiation holds a level in the threads' register slots and
   // one sector per thread
   s.ring_full = 0;
@@ -346,7 +344,7 @@ size_t plan_footprint_bytes(const Plan& p) {
   const size_t ld = round_up(p.nlm, 64), nptsp = round_up(p.npts, 64), ldo = round_up(p.nang, 64);
   const size_t ml1 = p.ml + 1;
   size_t b = 0;
-  b += 4 * ldo * ld * 4;                 // Y, W and W's two hemisphere shares
+  b += 2 * ldo * ld * 4;                 // Y, W
   b += 256;                              // iteration control block
   b += 3 * nptsp * ld * 4;               // S, R, DS
   b += 2 * nptsp * ldo * 4;              // DO source and radiance
@@ -410,13 +408,6 @@ int state_create(const Plan& p, State& s, std::string& err) {
   }
   TRY(upload(s, s.Y, Y.data(), Y.size(), err));
   TRY(upload(s, s.W, W.data(), W.size(), err));
-  {
-    std::vector<float> wd(W), wu(W);
-    for (int c = 0; c < s.ldo; c++)
-      for (int j = 0; j < s.ld; j++) (c < s.ndown ? wu : wd)[(size_t)c * s.ld + j] = 0.f;
-    TRY(upload(s, s.Wdn, wd.data(), wd.size(), err));
-    TRY(upload(s, s.Wup, wu.data(), wu.size(), err));
-  }
   std::vector<float> ysun(s.ld, 0.f);
   std::copy(p.ylmsun.begin(), p.ylmsun.end(), ysun.begin());
   TRY(upload(s, s.ysun, ysun.data(), ysun.size(), err));
@@ -554,9 +545,6 @@ int state_create(const Plan& p, State& s, std::string& err) {
   }
   TRY(alloc0(s, s.ctl, 1, err));
   s.nr_buf0 = s.nr;
-  HIPCHK(hipStreamCreateWithFlags(&s.stream2, hipStreamNonBlocking));
-  for (int b = 0; b < State::ITER_BATCH; b++)
-    for (int e = 0; e < 4; e++) HIPCHK(hipEventCreateWithFlags(&s.oev[b][e], hipEventDisableTiming));
   TRY(setup_ring(p, s, err));
   for (int e = 0; e < State::NEV; e++) HIPCHK(hipEventCreate(&s.ev[e]));
   for (int e = 0; e < 4; e++) HIPCHK(hipEventCreate(&s.ev_sweep[e]));
@@ -598,11 +586,6 @@ void state_free(State& s) {
     s.iter_exec[k] = nullptr;
     s.iter_graph[k] = nullptr;
   }
-  for (int b = 0; b < State::ITER_BATCH; b++)
-    for (int e = 0; e < 4; e++)
-      if (s.oev[b][e]) (void)hipEventDestroy(s.oev[b][e]);
-  if (s.stream2) (void)hipStreamDestroy(s.stream2);
-  s.stream2 = nullptr;
   if (s.stream) (void)hipStreamDestroy(s.stream);
   s.stream = nullptr;
 }
@@ -695,21 +678,10 @@ int state_init_solution_from(State& s, State& src, std::string& err) {
 
 // The path-integration sweeps of one iteration: downwelling levels, surface
 // boundary, upwelling levels.
-// The hemisphere overlap's events of one iteration: `fork` after the
-// downwelling SH->DO, `up_src` when the upwelling SH->DO is done, `down_done`
-// after the downwelling sweep, `down_sh` when the downwelling share of
-// DO->SH is done.
-struct Overlap {
-  hipEvent_t fork, up_src, down_done, down_sh;
-};
-
-static void enqueue_sweeps(State& s, const int* stop = nullptr, hipEvent_t ev_down_end = nullptr,
-                           hipEvent_t ev_up_begin = nullptr, const Overlap* ov = nullptr) {
+static void enqueue_sweeps(State& s, const int* stop = nullptr) {
   const Plan& p = *s.plan;
   SweepArgs a = sweep_args(s);
   a.stop = stop;
-  if (!ev_down_end) ev_down_end = s.ev_sweep[1];
-  if (!ev_up_begin) ev_up_begin = s.ev_sweep[2];
   const float albpi = p.d.gndalbedo / acosf(-1.0f);
   const bool ring = ring_active(s);
   // the ring sweep writes the boundary radiances itself; events around its
@@ -728,18 +700,8 @@ static void enqueue_sweeps(State& s, const int* stop = nullptr, hipEvent_t ev_do
     // the first launch is timed from the event the caller records just
     // before this function, the second to the one just after it
     needs(0, s.ndown, fast, full);
-    launch_sweep_ring(s.stream, a, 0, s.ndown, s.ring_lds, s.ring_lds_full, fast, full, s.sweep_prefetch);
-    (void)hipEventRecord(ev_down_end, s.stream);
-    if (ov) {
-      // the downwelling rows of DO->SH on the second stream, while the
-      // surface and the upwelling sweep run here
-      const int kd = (int)round_up(s.ndown, 16);
-      (void)hipEventRecord(ov->down_done, s.stream);
-      (void)hipStreamWaitEvent(s.stream2, ov->down_done, 0);
-      launch_do_to_sh(s.stream2, s.dorad, s.Wdn, s.nr, s.R, s.nptsp, s.ld, s.ldo, s.nptsp, p.npts, 0, kd,
-                      0, stop);
-      (void)hipEventRecord(ov->down_sh, s.stream2);
-    }
+    launch_sweep_ring(s.stream, a, 0, s.ndown, s.ring_lds, s.ring_lds_full, fast, full);
+    (void)hipEventRecord(s.ev_sweep[1], s.stream);
   } else {
     launch_sweep_preset(s.stream, a, false, 0, s.ndown);
     launch_sweep_columns(s.stream, a, 0, s.ndown);
@@ -749,10 +711,8 @@ static void enqueue_sweeps(State& s, const int* stop = nullptr, hipEvent_t ev_do
                    s.ndown, s.dirflux, albpi, s.bcrad_bot, br, stop);
   if (ring) {
     needs(s.ndown, p.nang, fast, full);
-    if (ov) (void)hipStreamWaitEvent(s.stream, ov->up_src, 0);
-    (void)hipEventRecord(ev_up_begin, s.stream);
-    launch_sweep_ring(s.stream, a, s.ndown, p.nang - s.ndown, s.ring_lds, s.ring_lds_full, fast, full,
-                      s.sweep_prefetch);
+    (void)hipEventRecord(s.ev_sweep[2], s.stream);
+    launch_sweep_ring(s.stream, a, s.ndown, p.nang - s.ndown, s.ring_lds, s.ring_lds_full, fast, full);
   } else {
     launch_sweep_preset(s.stream, a, true, s.ndown, p.nang - s.ndown);
     launch_sweep_columns(s.stream, a, s.ndown, p.nang - s.ndown);
@@ -839,38 +799,17 @@ static void accel_solcrit(bool doaccel, float deljdot, float deljold, float delj
 // those the host pointers name now.  (Inside a stream capture an event
 // record only marks a dependency: kernel times come from the iterations
 // launched one by one.)
-static void enqueue_iteration(State& s, const hipEvent_t* oev) {
+static void enqueue_iteration(State& s) {
   const Plan& p = *s.plan;
   const int* stop = &s.ctl->stop;
-  const bool ovl = oev && s.use_overlap && ring_active(s) && s.stream2 && s.n_sh_tiles_dn > 0 &&
-                   s.n_sh_tiles > s.n_sh_tiles_dn;
-  Overlap ov{};
   launch_iter_begin(s.stream, s.ctl);
-  if (ovl) {
-    ov = Overlap{oev[0], oev[1], oev[2], oev[3]};
-    launch_sh_to_do_perm(s.stream, s.S, s.Y, s.ext, s.ns, s.oct_perm, s.sh_tiles, s.n_sh_tiles_dn, s.dosrc,
-                         s.nptsp, s.ld, s.nptsp, p.npts, stop);
-    (void)hipEventRecord(ov.fork, s.stream);
-    (void)hipStreamWaitEvent(s.stream2, ov.fork, 0);
-    launch_sh_to_do_perm(s.stream2, s.S, s.Y, s.ext, s.ns, s.oct_perm, s.sh_tiles + s.n_sh_tiles_dn,
-                         s.n_sh_tiles - s.n_sh_tiles_dn, s.dosrc, s.nptsp, s.ld, s.nptsp, p.npts, stop);
-    (void)hipEventRecord(ov.up_src, s.stream2);
-  } else if (ring_active(s)) {
+  if (ring_active(s))
     launch_sh_to_do_perm(s.stream, s.S, s.Y, s.ext, s.ns, s.oct_perm, s.sh_tiles, s.n_sh_tiles, s.dosrc,
                          s.nptsp, s.ld, s.nptsp, p.npts, stop);
-  } else {
+  else
     launch_sh_to_do(s.stream, s.S, s.Y, s.ext, s.dosrc, s.nptsp, s.ldo, s.ld, s.nptsp, p.npts, p.nang, stop);
-  }
-  enqueue_sweeps(s, stop, nullptr, nullptr, ovl ? &ov : nullptr);
-  if (ovl) {
-    // the upwelling rows added to the downwelling share
-    (void)hipStreamWaitEvent(s.stream, ov.down_sh, 0);
-    const int ku = s.ndown / 16 * 16;
-    launch_do_to_sh(s.stream, s.dorad, s.Wup, s.nr, s.R, s.nptsp, s.ld, s.ldo, s.nptsp, p.npts, ku, s.ldo,
-                    1, stop);
-  } else {
-    launch_do_to_sh(s.stream, s.dorad, s.W, s.nr, s.R, s.nptsp, s.ld, s.ldo, s.nptsp, p.npts, 0, -1, 0, stop);
-  }
+  enqueue_sweeps(s, stop);
+  launch_do_to_sh(s.stream, s.dorad, s.W, s.nr, s.R, s.nptsp, s.ld, s.ldo, s.nptsp, p.npts, stop);
   SourceArgs sa = source_args(s, 0, 0);
   sa.ctl = s.ctl;
   launch_compute_source(s.stream, sa, s.nsrc_blocks);
@@ -890,7 +829,7 @@ static int capture_iterations(State& s, int k, std::string& err) {
   HIPCHK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
   for (int b = 0; b < State::ITER_BATCH; b++) {
     std::swap(s.nr, s.nr_next);
-    enqueue_iteration(s, s.oev[b]);
+    enqueue_iteration(s);
   }
   hipError_t e1 = hipGetLastError();
   hipError_t e2 = hipStreamEndCapture(s.stream, &s.iter_graph[k]);
@@ -900,8 +839,7 @@ static int capture_iterations(State& s, int k, std::string& err) {
     return AT3D_ERR_DEVICE;
   }
   HIPCHK(hipGraphInstantiate(&s.iter_exec[k], s.iter_graph[k], nullptr, nullptr, 0));
-  s.iter_key[k] = (ring_active(s) ? 1 : 0) | (s.force_general ? 2 : 0) | (s.ring_full ? 4 : 0) |
-                  (s.use_overlap ? 8 : 0) | (s.sweep_prefetch == 2 ? 16 : 0);
+  s.iter_key[k] = (ring_active(s) ? 1 : 0) | (s.force_general ? 2 : 0) | (s.ring_full ? 4 : 0);
   return AT3D_OK;
 }
 
@@ -922,8 +860,7 @@ static int state_solve_graph(State& s, int maxiter, std::string& err) {
   HIPCHK(hipMemcpyAsync(s.ctl, &c, sizeof c, hipMemcpyHostToDevice, s.stream));
   const int parity = s.nr == s.nr_buf0 ? 0 : 1;
   s.dosrc_level_order = ring_active(s);
-  const int key = (ring_active(s) ? 1 : 0) | (s.force_general ? 2 : 0) | (s.ring_full ? 4 : 0) |
-                  (s.use_overlap ? 8 : 0) | (s.sweep_prefetch == 2 ? 16 : 0);
+  const int key = (ring_active(s) ? 1 : 0) | (s.force_general ? 2 : 0) | (s.ring_full ? 4 : 0);
   if (!s.iter_exec[parity] || s.iter_key[parity] != key) TRY(capture_iterations(s, parity, err));
   // every batch runs ITER_BATCH iterations unless the device stops: the
   // host bound only guards against a control block that never moves

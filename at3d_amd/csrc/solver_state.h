@@ -37,10 +37,6 @@ struct State {
   int nlevels_max = 0;
   // transforms
   float *Y = nullptr, *W = nullptr, *ysun = nullptr;
-  // W with the upwelling rows zeroed / with the downwelling rows zeroed: the
-  // two hemispheres' shares of DO->SH
-  float *Wdn = nullptr, *Wup = nullptr;
-  int n_sh_tiles_dn = 0;       // SH->DO tiles of the downwelling octants (the first ones)
   int* lofj = nullptr;
   // optics
   float *ext = nullptr, *alb = nullptr, *chi = nullptr, *tc = nullptr, *dirflux = nullptr, *f0 = nullptr;
@@ -72,7 +68,6 @@ struct State {
   // level-ordered sweep (k_sweep_ring)
   bool ring_ok = false;        // ring fits in LDS and records are built
   int use_ring = 1;            // option: sweep with k_sweep_ring when ring_ok
-  int sweep_prefetch = 1;      // option: levels of record loads in flight (1 or 2)
   int slots = 0, sector_points = 8, gld = 0, ngeneral = 0, nsectors = 0;
   int ring_nlev[8] = {}, sec_base[8] = {};
   size_t ring_lds = 0;         // dynamic LDS bytes of k_sweep_ring's fast instantiation (0: unused)
@@ -115,12 +110,6 @@ struct State {
   hipGraph_t iter_graph[2] = {};
   hipGraphExec_t iter_exec[2] = {};
   int iter_key[2] = {-1, -1};  // sweep configuration each graph was captured with
-  // the hemisphere overlap (option "overlap"): a second stream runs the
-  // upwelling SH->DO during the downwelling sweep and the downwelling share
-  // of DO->SH during the upwelling sweep; events per iteration slot
-  int use_overlap = 0;
-  hipStream_t stream2 = nullptr;
-  hipEvent_t oev[ITER_BATCH][4] = {};
   bool last_solve_graph = false;   // the last solve call ran its iterations from graphs
   // iteration state
   bool initialized = false;

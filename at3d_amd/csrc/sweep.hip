@@ -463,16 +463,8 @@ struct RingSet {
 // but the record prefetch, and runs the columns (and grids) that need none
 // of them.  Each launch covers a hemisphere's columns; a workgroup whose
 // column belongs to the other instantiation returns at once.
-//
-// DEPTH is how many levels ahead the records are loaded: 1 issues level
-// L+1's loads while level L is computed; 2 (fast instantiation only) keeps
-// two levels of loads in flight: level L+2's are issued at the end of level
-// L, after its stores, and every register slot loads unconditionally (lanes
-// past the level's end take its last entry), so that the waits the compiler
-// places before level L+1 cover level L+1's loads and not level L+2's.
-template <bool FULL, int DEPTH>
+template <bool FULL>
 __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int col0) {
-  static_assert(DEPTH == 1 || (DEPTH == 2 && !FULL), "two-level prefetch only in the fast instantiation");
   if (a.stop && *a.stop) return;
   extern __shared__ float4 lds4[];
   const int col = col0 + blockIdx.x;
@@ -593,15 +585,13 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
     const int beg = __builtin_amdgcn_readfirstlane(off[min(L - 1, nlev)]);
     const int end = __builtin_amdgcn_readfirstlane(off[min(L, nlev)]);
     const int last = max(end - 1, 0);
-    if (DEPTH == 1) {
 #pragma unroll
-      for (int k = 0; k < RING_K; k++) {
-        if (beg + wfirst + k * RING_THREADS < end) {
-          const int jl = min(beg + tid + k * RING_THREADS, last);
-          S.w0[k] = rec0[jl];
-          S.s[k] = se[jl];
-          S.e[k] = ep[jl];
-        }
+    for (int k = 0; k < RING_K; k++) {
+      if (beg + wfirst + k * RING_THREADS < end) {
+        const int jl = min(beg + tid + k * RING_THREADS, last);
+        S.w0[k] = rec0[jl];
+        S.s[k] = se[jl];
+        S.e[k] = ep[jl];
       }
     }
     S.sec_ok = false;
@@ -612,17 +602,6 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
         const int sct = min(s0 + tid, s1 - 1);  // clamped, used only if sec_ok
         S.sec = sslot[sct];
         S.sec_p0 = sp0[sct];
-      }
-    }
-    if (DEPTH == 2) {
-      // after the (conditional) sector loads: the records' loads are the
-      // last of the level and always issued
-#pragma unroll
-      for (int k = 0; k < RING_K; k++) {
-        const int jl = min(beg + tid + k * RING_THREADS, last);
-        S.w0[k] = rec0[jl];
-        S.s[k] = se[jl];
-        S.e[k] = ep[jl];
       }
     }
   };
@@ -694,7 +673,7 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
         sw.w = ring[sl.w >> 16].x;
       }
     }
-    if (DEPTH == 1) load(L + 1, fut);  // past the last level this loads nothing
+    load(L + 1, fut);  // past the last level this loads nothing
     uint32_t gen = 0;  // this thread's multi-cell traces among its first RING_K entries
 #pragma unroll
     for (int k = 0; k < RING_K; k++) {
@@ -742,33 +721,20 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
       }
     }
     if (FULL && staging && L > 1 && soffl[L] - soffl[L - 1] > RING_THREADS) flush_rest(L - 1, RING_THREADS);
-    if (DEPTH == 2) load(L + 2, fut);  // into the set of level L-1, after this level's stores
     if (FULL && L < nlev && __builtin_amdgcn_readfirstlane(full[L + 1]))
       __syncthreads();
     else
       lds_barrier();
   };
-  RingSet S0, S1, S2;
+  RingSet S0, S1;
   __syncthreads();
   load(1, S0);
-  if (DEPTH == 2) load(2, S1);
   int L = 1;
-  if (DEPTH == 1) {
-    for (; L + 1 <= nlev; L += 2) {
-      step(L, S0, S1);
-      step(L + 1, S1, S0);
-    }
-    if (L == nlev) step(L, S0, S1);
-  } else {
-    // S0 holds level L and S1 level L+1 at the top of each round
-    for (; L + 2 <= nlev; L += 3) {
-      step(L, S0, S2);
-      step(L + 1, S1, S0);
-      step(L + 2, S2, S1);
-    }
-    if (L <= nlev) step(L, S0, S2);
-    if (L + 1 <= nlev) step(L + 1, S1, S0);
+  for (; L + 1 <= nlev; L += 2) {
+    step(L, S0, S1);
+    step(L + 1, S1, S0);
   }
+  if (L == nlev) step(L, S0, S1);
   if (staging && nlev >= 1) flush_rest(nlev, 0);
 }
 
@@ -867,24 +833,19 @@ void launch_sweep_geo(hipStream_t st, const SweepArgs& a, int ncols) {
 }
 
 void launch_sweep_ring(hipStream_t st, const SweepArgs& a, int col0, int ncol, size_t lds_fast,
-                       size_t lds_full, bool fast, bool full, int depth) {
+                       size_t lds_full, bool fast, bool full) {
   if (ncol <= 0) return;
-  if (fast) {
-    if (depth == 2)
-      k_sweep_ring<false, 2><<<ncol, RING_THREADS, lds_fast, st>>>(a, col0);
-    else
-      k_sweep_ring<false, 1><<<ncol, RING_THREADS, lds_fast, st>>>(a, col0);
-  }
-  if (full) k_sweep_ring<true, 1><<<ncol, RING_THREADS, lds_full, st>>>(a, col0);
+  if (fast) k_sweep_ring<false><<<ncol, RING_THREADS, lds_fast, st>>>(a, col0);
+  if (full) k_sweep_ring<true><<<ncol, RING_THREADS, lds_full, st>>>(a, col0);
 }
 
 int sweep_ring_max_lds(size_t lds_fast, size_t lds_full) {
-  const auto set = [](const void* f, size_t b) {
-    return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b) == hipSuccess;
-  };
-  const bool ok = (lds_fast == 0 || (set(reinterpret_cast<const void*>(k_sweep_ring<false, 1>), lds_fast) &&
-                                     set(reinterpret_cast<const void*>(k_sweep_ring<false, 2>), lds_fast))) &&
-                  set(reinterpret_cast<const void*>(k_sweep_ring<true, 1>), lds_full);
+  const bool ok =
+      (lds_fast == 0 ||
+       hipFuncSetAttribute(reinterpret_cast<const void*>(k_sweep_ring<false>),
+                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_fast) == hipSuccess) &&
+      hipFuncSetAttribute(reinterpret_cast<const void*>(k_sweep_ring<true>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_full) == hipSuccess;
   return ok ? 0 : 1;
 }
 

@@ -208,8 +208,7 @@ __global__ __launch_bounds__(256) void k_do_to_sh(const float* __restrict__ D,
                                                   const float* __restrict__ Wc,
                                                   const int* __restrict__ nr,
                                                   float* __restrict__ R, int ld, int ldo, int ldp,
-                                                  int npts, int kbeg, int kend, int accumulate,
-                                                  const int* __restrict__ stop) {
+                                                  int npts, const int* __restrict__ stop) {
   if (stop && *stop) return;
   __shared__ float sA[2][TK][TM + PAD];
   __shared__ float sB[2][TK][TN + PAD];
@@ -253,15 +252,15 @@ __global__ __launch_bounds__(256) void k_do_to_sh(const float* __restrict__ D,
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++) acc[i][j] = (float4_){0.f, 0.f, 0.f, 0.f};
   const int bk = tid >> 4, bc = (tid & 15) * 4;    // 16 k x 64 rows/cols per tile
-  const float* acol = D + (size_t)(kbeg + bk) * ldp + m0 + bc;
-  const float* bcol = Wc + (size_t)(kbeg + bk) * ld + n0 + bc;
+  const float* acol = D + (size_t)bk * ldp + m0 + bc;
+  const float* bcol = Wc + (size_t)bk * ld + n0 + bc;
   float4 a = *reinterpret_cast<const float4*>(acol);
   float4 b = *reinterpret_cast<const float4*>(bcol);
-  for (int k0 = 0, buf = 0; k0 < kend - kbeg; k0 += TK, buf ^= 1) {
+  for (int k0 = 0, buf = 0; k0 < ldo; k0 += TK, buf ^= 1) {
     *reinterpret_cast<float4*>(&sA[buf][bk][bc]) = a;
     *reinterpret_cast<float4*>(&sB[buf][bk][bc]) = b;
     __syncthreads();
-    if (k0 + TK < kend - kbeg) {
+    if (k0 + TK < ldo) {
       a = *reinterpret_cast<const float4*>(acol + (size_t)(k0 + TK) * ldp);
       b = *reinterpret_cast<const float4*>(bcol + (size_t)(k0 + TK) * ld);
     }
@@ -280,8 +279,7 @@ __global__ __launch_bounds__(256) void k_do_to_sh(const float* __restrict__ D,
       const int p = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + v;
       for (int j = 0; j < 2; j++) {
         const int J = n0 + wn * 32 + j * 16 + (lane & 15);
-        float* r = R + (size_t)p * ld + J;
-        *r = (J < n[i][v]) ? (accumulate ? *r + acc[i][j][v] : acc[i][j][v]) : 0.0f;
+        R[(size_t)p * ld + J] = (J < n[i][v]) ? acc[i][j][v] : 0.0f;
       }
     }
 }
@@ -308,12 +306,9 @@ void launch_sh_to_do_perm(hipStream_t st, const float* S, const float* Yc, const
 }
 
 void launch_do_to_sh(hipStream_t st, const float* dorad, const float* Wc, const int* nr, float* R,
-                     int mp, int ld, int ldo, int ldp, int npts, int c0, int c1, int accumulate,
-                     const int* stop) {
-  if (c1 < 0) c1 = ldo;
-  if (c1 <= c0) return;
+                     int mp, int ld, int ldo, int ldp, int npts, const int* stop) {
   dim3 grid(ld / TN, mp / TM);
-  k_do_to_sh<<<grid, 256, 0, st>>>(dorad, Wc, nr, R, ld, ldo, ldp, npts, c0, c1, accumulate, stop);
+  k_do_to_sh<<<grid, 256, 0, st>>>(dorad, Wc, nr, R, ld, ldo, ldp, npts, stop);
 }
 
 }  // namespace at3d

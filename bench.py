@@ -346,12 +346,9 @@ def main():
     torch.cuda.synchronize()
 
     # engine variants of the same plain solve (for the design record): the
-    # iterations launched one by one with the host deciding after each, and
-    # the hemisphere overlap on a second stream
+    # iterations launched one by one with the host deciding after each
     variants = {}
-    for vname, opts, reset in (("host_controlled_iterations", {"iteration_graph": 0}, {"iteration_graph": 1}),
-                               ("hemisphere_overlap", {"overlap": 1}, {"overlap": 0}),
-                               ("sweep_prefetch_2", {"sweep_prefetch": 2}, {"sweep_prefetch": 1})):
+    for vname, opts, reset in (("host_controlled_iterations", {"iteration_graph": 0}, {"iteration_graph": 1}),):
         for k, v in opts.items():
             rte.set_option(k, v)
         rte.solve(w.maxiter, verbose=False)
@@ -362,12 +359,6 @@ def main():
         torch.cuda.synchronize()
         variants[vname] = {"ms_per_solve": (time.perf_counter() - tv) * 1e3 / args.steps,
                            "iterations": rte.num_iterations}
-        if vname == "sweep_prefetch_2":
-            # its sweep kernel time, from a solve launched iteration by iteration
-            rte.set_option("iteration_graph", 0)
-            rte.solve(w.maxiter, verbose=False)
-            variants[vname]["sweep_kernel_ms_per_solve"] = rte.timing("sweep_kernel")
-            rte.set_option("iteration_graph", 1)
         for k, v in reset.items():
             rte.set_option(k, v)
     print("bench: variants done", file=sys.stderr, flush=True)

@@ -266,10 +266,7 @@ int at3d_levis_gradient_jacobian(at3d_state* st, const at3d_deriv_desc* deriv, i
  * test, FIXSH and the acceleration on the device; the host reads the control
  * block once per four iterations) when the level-ordered sweep is in use; 0,
  * verbose, or the point-ordered sweep runs them one by one with the host
- * deciding after each.  "overlap" (0/1, default 0): with iteration graphs, a
- * second stream computes the upwelling SH->DO during the downwelling sweep
- * and the downwelling share of DO->SH during the upwelling sweep.
- * "sweep_prefetch" (1 or 2, default 1): levels of sweep records in flight.
+ * deciding after each.
  * "max_subgrid_intervals": sub-step intervals one gradient ray may record.
  * No counterpart in the reference. */
 int at3d_state_set_option(at3d_state* st, const char* key, int32_t value, char* err,

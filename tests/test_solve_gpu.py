@@ -254,37 +254,3 @@ def test_iteration_graph_matches_host_controlled_iterations(make):
     assert c.num_iterations == d.num_iterations
     assert rel_l2(c.state_array("source"), d.state_array("source")) < 1e-6
 
-
-@pytest.mark.parametrize("make", [lambda: blob8_l15("open"), cloud_l15, cloud_small],
-                         ids=["blob8_open_l15", "cloud16_l15", "cloud12_two_species"])
-def test_hemisphere_overlap_matches_one_stream(make):
-    """The upwelling SH->DO during the downwelling sweep and the downwelling
-    share of DO->SH during the upwelling sweep, on a second stream: the same
-    iterations and the same fields as one stream to FP32 rounding (DO->SH sums
-    the two hemispheres' shares in two passes)."""
-    w = make()
-    a = S.make_rte(w)
-    a.solve(100, verbose=False)
-    b = S.make_rte(w)
-    b.set_option("overlap", 1)
-    b.solve(100, verbose=False)
-    assert a.num_iterations == b.num_iterations
-    for name in ("source", "radiance", "fluxes", "do_radiance"):
-        assert rel_l2(b.state_array(name), a.state_array(name)) < 1e-6, name
-
-
-@pytest.mark.parametrize("make", [lambda: blob8_l15("open"), cloud_l15, cloud_small,
-                                  lambda: S.serve_medium((48, 48, 32), 12, 24, 0.86, 3)],
-                         ids=["blob8_open_l15", "cloud16_l15", "cloud12_two_species", "blob48x48x32_l11"])
-def test_two_level_record_prefetch_is_bit_identical(make):
-    """The ring sweep with two levels of record loads in flight computes the
-    same operations in the same order as with one: bit-identical fields."""
-    w = make()
-    a = S.make_rte(w)
-    a.solve(4, verbose=False)
-    b = S.make_rte(w)
-    b.set_option("sweep_prefetch", 2)
-    b.solve(4, verbose=False)
-    assert a.num_iterations == b.num_iterations == 4
-    for name in ("do_radiance", "source", "radiance"):
-        assert np.array_equal(a.state_array(name), b.state_array(name)), name
