@@ -33,6 +33,7 @@ class SceneDesc(C.Structure):
         ("iterfixsh", C.c_int32), ("angle_set", C.c_int32),
         ("shacc", C.c_float), ("solacc", C.c_float), ("splitacc", C.c_float),
         ("transmin", C.c_float), ("tautol", C.c_double), ("transcut", C.c_double),
+        ("maxig", C.c_int32), ("maxic", C.c_int32), ("maxiv", C.c_int32), ("maxido", C.c_int32),
     ]
 
 
@@ -103,6 +104,8 @@ def lib():
                                                    C.c_char_p, C.c_size_t]
         L.at3d_state_timing.argtypes = [vp, C.c_char_p]
         L.at3d_state_timing.restype = C.c_double
+        L.at3d_state_size.argtypes = [vp, C.c_char_p]
+        L.at3d_state_size.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -123,7 +126,7 @@ EXPORTS = ["at3d_version", "at3d_plan_create", "at3d_plan_destroy", "at3d_plan_s
            "at3d_render", "at3d_levis_gradient", "at3d_levis_gradient_jacobian",
            "at3d_init_solution_from", "at3d_device_trim", "at3d_state_load", "at3d_state_save",
            "at3d_state_sh_terms", "at3d_render_noalloc",
-           "at3d_render_scratch_bytes"]
+           "at3d_render_scratch_bytes", "at3d_state_size"]
 
 
 def as_f32(a):

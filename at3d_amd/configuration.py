@@ -2,8 +2,8 @@
 default_config.json of the reference).
 
 ``get_config()`` returns a plain dict keyed like the reference's xr.Dataset
-variables.  Adaptive cell splitting is not implemented by this engine, so the
-default ``split_accuracy`` is 0.0 (the reference default is 0.03).
+variables, with the reference's defaults (adaptive cell splitting on at
+``split_accuracy`` 0.03).
 """
 from __future__ import annotations
 
@@ -14,7 +14,7 @@ DEFAULTS = {
     "y_boundary_condition": "open",
     "num_mu_bins": 16,
     "num_phi_bins": 32,
-    "split_accuracy": 0.0,
+    "split_accuracy": 0.03,
     "deltam": True,
     "spherical_harmonics_accuracy": 0.0,
     "acceleration_flag": True,

@@ -375,6 +375,19 @@ int at3d_state_get(at3d_state* st, const char* name, void* dst, size_t nbytes, c
     }
     return AT3D_OK;
   }
+  // the current grid (host copies in the state's plan)
+  const at3d::Plan& p = *s.plan;
+  if (k == "gridpos") return copy_out(p.gridpos, dst, nbytes, err, errlen);
+  if (k == "gridptr") return copy_out(p.gridptr, dst, nbytes, err, errlen);
+  if (k == "neighptr") return copy_out(p.neighptr, dst, nbytes, err, errlen);
+  if (k == "cellflags") return copy_out(p.cellflags, dst, nbytes, err, errlen);
+  if (k == "total_ext") return copy_out(p.total_ext, dst, nbytes, err, errlen);
+  if (k == "extinct") return copy_out(p.extinct, dst, nbytes, err, errlen);
+  if (k == "albedo") return copy_out(p.albedo, dst, nbytes, err, errlen);
+  if (k == "treeptr") {
+    if (!p.treeptr.empty()) return copy_out(p.treeptr, dst, nbytes, err, errlen);
+    return copy_out(std::vector<int32_t>((size_t)p.ncells * 2, -1), dst, nbytes, err, errlen);
+  }
   if (k == "fluxes") return get1d(s.fluxes, 2 * npts * sizeof(float));
   if (k == "dirflux") return get1d(s.dirflux, npts * sizeof(float));
   if (k == "ns") return get1d(s.ns, npts * sizeof(int));
@@ -574,7 +587,22 @@ double at3d_state_timing(const at3d_state* st, const char* key) {
   if (k == "iteration_graph_active") return s.last_solve_graph ? 1.0 : 0.0;
   if (k == "adjoint_rays") return s.adjoint_ms;
   if (k == "gradient") return s.grad_ms;
+  if (k == "splitcrit") return s.splitcrit;
   return -1.0;
+}
+
+int64_t at3d_state_size(const at3d_state* st, const char* key) {
+  if (!st || !key || !st->s.plan) return -1;
+  const at3d::Plan& p = *st->s.plan;
+  const std::string k(key);
+  if (k == "npts") return p.npts;
+  if (k == "ncells") return p.ncells;
+  if (k == "nbpts") return p.nbpts;
+  if (k == "nbcells") return p.nbcells;
+  if (k == "ntop") return (int64_t)p.bctop.size();
+  if (k == "nbot") return (int64_t)p.bcbot.size();
+  if (k == "refined") return p.treeptr.empty() ? 0 : 1;
+  return -1;
 }
 
 }  // extern "C"

@@ -399,7 +399,7 @@ int state_levis_gradient(State& s, const at3d_deriv_desc& dd, int64_t nrays, con
   const float albpi = p.d.gndalbedo / acosf(-1.0f);
   launch_render_bottom(s.stream, s.fluxes, s.dirflux, s.bcbot, (int)p.bcbot.size(), albpi, s.bcrad_render);
   RayArgs a;
-  a.g = GridTopo{s.gridpos, s.gridptr, s.neighptr, s.cellflags, p.npts, p.ncells};
+  a.g = GridTopo{s.gridpos, s.gridptr, s.neighptr, s.cellflags, p.npts, p.ncells, s.treeptr};
   a.xgrid = s.xgrid;
   a.ygrid = s.ygrid;
   a.zgrid = s.zgrid;

@@ -381,9 +381,87 @@ void make_sh_tables(Plan& p) {
 // Property transfer: TRILIN_INTERP_PROP (shdom90.f90:17-346, 'ON' method),
 // INTERP_GRID (shdomsub2.f:313-387) and PREPARE_PROP (shdomsub2.f:479-608).
 // ---------------------------------------------------------------------------
-void interp_optics(Plan& p) {
+}  // namespace
+
+// TRILIN_INTERP_PROP (shdom90.f90:17-346) for one species at one point
+// (X, Y, Z): trilinear extinction, the scattering-weighted albedo and the
+// phase-table entries of the eight property-grid corners with their weights
+// (merged by table index and sorted by weight, 'N' interpolation), before
+// delta-M scaling.
+void point_optics(const Plan& p, int ipa, float X, float Y, float Z, float& extf, float& albf, int32_t* iph_out,
+                  float* wt_out) {
   const at3d_scene_desc& d = p.d;
   const int npx = d.npx, npy = d.npy, npz = d.npz, npg = npx * npy * npz;
+  const float* E = &p.extinctp[(size_t)ipa * npg];
+  const float* A = &p.albedop[(size_t)ipa * npg];
+  const float* PW = &p.phasewtp[(size_t)ipa * npg];
+  const int32_t* IP = &p.iphasep[(size_t)ipa * npg];
+  int il = 0, iu = npz;
+  while (iu - il > 1) {
+    int im = (iu + il) / 2;
+    if (Z >= p.zlevels[im - 1]) il = im; else iu = im;
+  }
+  const int iz = std::max(il, 1);
+  double w = (double)(Z - p.zlevels[iz - 1]) / (double)(p.zlevels[iz] - p.zlevels[iz - 1]);
+  w = std::max(std::min(w, 1.0), 0.0);
+  int ix = (int)((X - 0.0f) / d.delx) + 1;
+  if (fabsf(X - 0.0f - (float)npx * d.delx) < 0.01f * d.delx) ix = npx;
+  int ixp = ix % npx + 1;
+  double u = (double)(X - 0.0f - d.delx * (float)(ix - 1)) / (double)d.delx;
+  u = std::max(std::min(u, 1.0), 0.0);
+  if (u < 1.0e-5) u = 0.0;
+  if (u > 1.0 - 1.0e-5) u = 1.0;
+  int iy = (int)((Y - 0.0f) / d.dely) + 1;
+  if (fabsf(Y - 0.0f - (float)npy * d.dely) < 0.01f * d.dely) iy = npy;
+  int iyp = iy % npy + 1;
+  double v = (double)(Y - 0.0f - d.dely * (float)(iy - 1)) / (double)d.dely;
+  v = std::max(std::min(v, 1.0), 0.0);
+  if (v < 1.0e-5) v = 0.0;
+  if (v > 1.0 - 1.0e-5) v = 1.0;
+  const double F[8] = {(1 - u) * (1 - v) * (1 - w), u * (1 - v) * (1 - w), (1 - u) * v * (1 - w),
+                       u * v * (1 - w),             (1 - u) * (1 - v) * w, u * (1 - v) * w,
+                       (1 - u) * v * w,             u * v * w};
+  int I[8];
+  I[0] = iz + npz * (iy - 1) + npz * npy * (ix - 1) - 1;
+  I[1] = iz + npz * (iy - 1) + npz * npy * (ixp - 1) - 1;
+  I[2] = iz + npz * (iyp - 1) + npz * npy * (ix - 1) - 1;
+  I[3] = iz + npz * (iyp - 1) + npz * npy * (ixp - 1) - 1;
+  for (int q = 0; q < 4; q++) I[q + 4] = I[q] + 1;
+  double ext = 0.0, scat[8], scatter = 0.0;
+  for (int q = 0; q < 8; q++) ext += F[q] * (double)E[I[q]];
+  for (int q = 0; q < 8; q++) {
+    scat[q] = F[q] * (double)E[I[q]] * (double)A[I[q]];
+    scatter += scat[q];
+  }
+  extf = (float)ext;
+  albf = ((double)extf > p.extmin) ? (float)(scatter / (double)extf) : (float)(scatter / p.extmin);
+  int iph[8];
+  float wt[8];
+  for (int q = 0; q < 8; q++) {
+    iph[q] = IP[I[q]];
+    wt[q] = (scatter >= p.scatmin) ? (float)((double)PW[I[q]] * scat[q] / scatter)
+                                   : (float)((double)PW[I[q]] * scat[q] / p.scatmin);
+  }
+  for (int q = 0; q < 8; q++)
+    for (int q2 = q + 1; q2 < 8; q2++)
+      if (iph[q] == iph[q2]) {
+        wt[q] += wt[q2];
+        wt[q2] = 0.f;
+      }
+  int ord[8];
+  std::iota(ord, ord + 8, 0);
+  std::stable_sort(ord, ord + 8, [&](int a, int b) { return wt[a] > wt[b]; });
+  for (int q = 0; q < 8; q++) {
+    iph_out[q] = iph[ord[q]];
+    wt_out[q] = wt[ord[q]];
+  }
+}
+
+namespace {
+
+void interp_optics(Plan& p) {
+  const at3d_scene_desc& d = p.d;
+  const int npz = d.npz;
   const int npts = p.npts, npart = p.npart;
   p.extmin = (double)(1.0e-5f / ((p.zlevels[npz - 1] - p.zlevels[0]) / (float)npz));
   p.scatmin = (double)0.1f * p.extmin;
@@ -393,78 +471,16 @@ void interp_optics(Plan& p) {
   p.phaseinterpwt.assign((size_t)npart * npts * 8, 0.f);
   p.total_ext.assign(npts, 0.f);
   for (int ipa = 0; ipa < npart; ipa++) {
-    const float* E = &p.extinctp[(size_t)ipa * npg];
-    const float* A = &p.albedop[(size_t)ipa * npg];
-    const float* PW = &p.phasewtp[(size_t)ipa * npg];
-    const int32_t* IP = &p.iphasep[(size_t)ipa * npg];
     // points are independent; the species loop stays outside so each point's
     // total extinction is summed in species order
     const int nchunk = 64;
     parallel_for(nchunk, [&](int c) {
     const int ip0 = (int)((int64_t)npts * c / nchunk), ip1 = (int)((int64_t)npts * (c + 1) / nchunk);
     for (int ip = ip0; ip < ip1; ip++) {
-      const float X = p.gridpos[ip * 3], Y = p.gridpos[ip * 3 + 1], Z = p.gridpos[ip * 3 + 2];
-      int il = 0, iu = npz;
-      while (iu - il > 1) {
-        int im = (iu + il) / 2;
-        if (Z >= p.zlevels[im - 1]) il = im; else iu = im;
-      }
-      const int iz = std::max(il, 1);
-      double w = (double)(Z - p.zlevels[iz - 1]) / (double)(p.zlevels[iz] - p.zlevels[iz - 1]);
-      w = std::max(std::min(w, 1.0), 0.0);
-      int ix = (int)((X - 0.0f) / d.delx) + 1;
-      if (fabsf(X - 0.0f - (float)npx * d.delx) < 0.01f * d.delx) ix = npx;
-      int ixp = ix % npx + 1;
-      double u = (double)(X - 0.0f - d.delx * (float)(ix - 1)) / (double)d.delx;
-      u = std::max(std::min(u, 1.0), 0.0);
-      if (u < 1.0e-5) u = 0.0;
-      if (u > 1.0 - 1.0e-5) u = 1.0;
-      int iy = (int)((Y - 0.0f) / d.dely) + 1;
-      if (fabsf(Y - 0.0f - (float)npy * d.dely) < 0.01f * d.dely) iy = npy;
-      int iyp = iy % npy + 1;
-      double v = (double)(Y - 0.0f - d.dely * (float)(iy - 1)) / (double)d.dely;
-      v = std::max(std::min(v, 1.0), 0.0);
-      if (v < 1.0e-5) v = 0.0;
-      if (v > 1.0 - 1.0e-5) v = 1.0;
-      const double F[8] = {(1 - u) * (1 - v) * (1 - w), u * (1 - v) * (1 - w), (1 - u) * v * (1 - w),
-                           u * v * (1 - w),             (1 - u) * (1 - v) * w, u * (1 - v) * w,
-                           (1 - u) * v * w,             u * v * w};
-      int I[8];
-      I[0] = iz + npz * (iy - 1) + npz * npy * (ix - 1) - 1;
-      I[1] = iz + npz * (iy - 1) + npz * npy * (ixp - 1) - 1;
-      I[2] = iz + npz * (iyp - 1) + npz * npy * (ix - 1) - 1;
-      I[3] = iz + npz * (iyp - 1) + npz * npy * (ixp - 1) - 1;
-      for (int q = 0; q < 4; q++) I[q + 4] = I[q] + 1;
-      double ext = 0.0, scat[8], scatter = 0.0;
-      for (int q = 0; q < 8; q++) ext += F[q] * (double)E[I[q]];
-      for (int q = 0; q < 8; q++) {
-        scat[q] = F[q] * (double)E[I[q]] * (double)A[I[q]];
-        scatter += scat[q];
-      }
-      const float extf = (float)ext;
-      float albf = ((double)extf > p.extmin) ? (float)(scatter / (double)extf)
-                                             : (float)(scatter / p.extmin);
-      int iph[8];
-      float wt[8];
-      for (int q = 0; q < 8; q++) {
-        iph[q] = IP[I[q]];
-        wt[q] = (scatter >= p.scatmin) ? (float)((double)PW[I[q]] * scat[q] / scatter)
-                                       : (float)((double)PW[I[q]] * scat[q] / p.scatmin);
-      }
-      for (int q = 0; q < 8; q++)
-        for (int q2 = q + 1; q2 < 8; q2++)
-          if (iph[q] == iph[q2]) {
-            wt[q] += wt[q2];
-            wt[q2] = 0.f;
-          }
-      int ord[8];
-      std::iota(ord, ord + 8, 0);
-      std::stable_sort(ord, ord + 8, [&](int a, int b) { return wt[a] > wt[b]; });
       const size_t base = ((size_t)ipa * npts + ip) * 8;
-      for (int q = 0; q < 8; q++) {
-        p.iphase[base + q] = iph[ord[q]];
-        p.phaseinterpwt[base + q] = wt[ord[q]];
-      }
+      float extf, albf;
+      point_optics(p, ipa, p.gridpos[ip * 3], p.gridpos[ip * 3 + 1], p.gridpos[ip * 3 + 2], extf, albf,
+                   &p.iphase[base], &p.phaseinterpwt[base]);
       p.extinct[(size_t)ipa * npts + ip] = extf;
       p.albedo[(size_t)ipa * npts + ip] = albf;
       p.total_ext[ip] = p.total_ext[ip] + extf;
@@ -675,14 +691,12 @@ void direct_beam_setup(const Plan& p, std::vector<float>& extdirp, DirectBeamGeo
   b.solarflux = (double)d.solarflux;
 }
 
-namespace {
-
 // Direct flux at one point (false when the reference would STOP).  With
 // `path` the traversal also records, per property-grid cell crossed, the
 // derivative of the slant optical path with respect to each of the 8 corner
 // extinctions (DIRECT_BEAM_AND_PATHS_PROP, shdomsub5.f:1608-2003).
 bool direct_beam_point(const DirectBeamGeom& b, float xi, float yi, float zi, float& dirflux,
-                       std::vector<std::pair<int32_t, float>>* path_rec = nullptr) {
+                       std::vector<std::pair<int32_t, float>>* path_rec) {
   double path = 0.0;
   bool ok;
   if (path_rec)
@@ -695,6 +709,8 @@ bool direct_beam_point(const DirectBeamGeom& b, float xi, float yi, float zi, fl
   dirflux = (float)(b.solarflux * exp(-path));
   return true;
 }
+
+namespace {
 
 // ---------------------------------------------------------------------------
 // Eddington two-stream initial radiance: INIT_RADIANCE (shdomsub2.f:614-760),
@@ -948,8 +964,9 @@ void make_ordinates(Plan& p) {
 
 // ---------------------------------------------------------------------------
 // Sweep order (SWEEPING_ORDER shdomsub1.f:3261-3351 with SWEEP_BASE_CELL
-// :4529-4619 for a tree-less base grid) and the dependency levels that let all
-// points of one level be integrated in parallel.
+// :4529-4619 and, on a refined grid, SWEEP_NEXT_CELL :4624-4696) and the
+// dependency levels that let all points of one level be integrated in
+// parallel.
 // ---------------------------------------------------------------------------
 void sweeping_order(const Plan& p, int ioct0, std::vector<int32_t>& ord) {
   const int ioct = ioct0 + 1, nz = p.nz, nxc = p.nxc, nyc = p.nyc, bc = p.bcflag;
@@ -972,15 +989,36 @@ void sweeping_order(const Plan& p, int ioct0, std::vector<int32_t>& ord) {
   std::vector<char> visited(p.npts, 0);
   ord.clear();
   ord.reserve(p.npts);
+  const bool tree = !p.treeptr.empty();
+  std::vector<int32_t> stack;
   int ix = six, iy = siy, iz = siz;
   for (;;) {
-    const int cell = (iz + (nz - 1) * (iy - 1) + (nz - 1) * nyc * (ix - 1)) - 1;
-    for (int k = 0; k < 8; k++) {
-      const int corner = k ^ (7 - ioct0);  // CORNDOG(k+1, IOCT) - 1
-      const int ipt = p.gridptr[(size_t)cell * 8 + corner];
-      if (visited[ipt]) continue;
-      visited[ipt] = 1;
-      ord.push_back((cell << 3) | corner);
+    // the end cells of this base cell's tree, depth first, the child on the
+    // side the ray comes from first (SWEEP_NEXT_CELL :4624-4696)
+    int cell = (iz + (nz - 1) * (iy - 1) + (nz - 1) * nyc * (ix - 1)) - 1;
+    for (;;) {
+      if (tree && p.treeptr[(size_t)cell * 2 + 1] >= 0) {
+        const int idir = ((p.cellflags[cell] >> 2) & 3) - 1;
+        const int c1 = p.treeptr[(size_t)cell * 2 + 1];
+        if ((ioct0 >> idir) & 1) {
+          stack.push_back(c1 + 1);
+          cell = c1;
+        } else {
+          stack.push_back(c1);
+          cell = c1 + 1;
+        }
+        continue;
+      }
+      for (int k = 0; k < 8; k++) {
+        const int corner = k ^ (7 - ioct0);  // CORNDOG(k+1, IOCT) - 1
+        const int ipt = p.gridptr[(size_t)cell * 8 + corner];
+        if (visited[ipt]) continue;
+        visited[ipt] = 1;
+        ord.push_back((cell << 3) | corner);
+      }
+      if (stack.empty()) break;
+      cell = stack.back();
+      stack.pop_back();
     }
     if (ix == eix) {
       if (iy == eiy) {
@@ -1000,8 +1038,9 @@ void sweeping_order(const Plan& p, int ioct0, std::vector<int32_t>& ord) {
 // One octant's sweep order, dependency levels and level lists.
 static int build_octant(const Plan& p, int o, const std::vector<char>& top, const std::vector<char>& bot,
                         OctantPlan& op, std::string& err) {
-  const GridTopo g{p.gridpos.data(), p.gridptr.data(), p.neighptr.data(), p.cellflags.data(),
-                   p.npts, p.ncells};
+  GridTopo g{p.gridpos.data(), p.gridptr.data(), p.neighptr.data(), p.cellflags.data(), p.npts, p.ncells};
+  const bool tree = !p.treeptr.empty();
+  if (tree) g.treeptr = p.treeptr.data();
   {
     sweeping_order(p, o, op.sweepord);
     if ((int)op.sweepord.size() != p.npts) {
@@ -1027,7 +1066,9 @@ static int build_octant(const Plan& p, int o, const std::vector<char>& top, cons
       int lvl = -1;
       bool simple = false;
       if (corner == o) {
-        // one-cell characteristic for every ordinate of the octant if all far faces are known
+        // one-cell characteristic for every ordinate of the octant if all far
+        // faces are known (and are the cell's own: a face onto smaller cells
+        // takes its corners from the cell across)
         const int8_t fl = p.cellflags[cell];
         int faces[3], nf = 0;
         if (!(fl & 1)) faces[nf++] = 1 - bitx;
@@ -1035,15 +1076,21 @@ static int build_octant(const Plan& p, int o, const std::vector<char>& top, cons
         faces[nf++] = 5 - bitz;
         bool ok = true;
         int mx = 0;
-        for (int f = 0; f < nf && ok; f++)
+        for (int f = 0; f < nf && ok; f++) {
+          if (tree && nb_smaller(p.neighptr[(size_t)cell * 6 + faces[f]])) {
+            ok = false;
+            break;
+          }
           for (int c = 0; c < 4; c++) {
             const int q = p.gridptr[(size_t)cell * 8 + face_corner(faces[f], c)];
             if (!valid(q)) { ok = false; break; }
             mx = std::max(mx, op.level[q]);
           }
+        }
         if (ok) {
           lvl = mx + 1;
-          simple = (fl & 3) == 0;  // regular cell: the GPU may use analytic geometry
+          // a regular base cell: the GPU may use analytic geometry
+          simple = (fl & 3) == 0 && (!tree || p.treeptr[(size_t)cell * 2] < 0);
         }
       }
       if (lvl < 0) {
@@ -1154,9 +1201,22 @@ std::vector<uint32_t> level_key(const Plan& p) {
 }
 }  // namespace
 
+// BOUNDARY_PNTS (shdomsub1.f:2173-2215): the points on the top and bottom
+// boundaries, in point order.
+void boundary_points(Plan& p) {
+  p.bctop.clear();
+  p.bcbot.clear();
+  for (int i = 0; i < p.npts; i++) {
+    if (p.gridpos[i * 3 + 2] >= p.zgrid[p.nz - 1]) p.bctop.push_back(i);
+    if (p.gridpos[i * 3 + 2] <= p.zgrid[0]) p.bcbot.push_back(i);
+  }
+}
+
 int build_levels(Plan& p, std::string& err) {
-  const std::vector<uint32_t> key = level_key(p);
-  {
+  // a refined grid is its solve's own: its levels are neither looked up nor kept
+  const bool tree = !p.treeptr.empty();
+  const std::vector<uint32_t> key = tree ? std::vector<uint32_t>() : level_key(p);
+  if (!tree) {
     std::lock_guard<std::mutex> l(level_cache_mutex);
     for (auto it = level_cache.begin(); it != level_cache.end(); ++it)
       if (it->first == key) {
@@ -1176,7 +1236,9 @@ int build_levels(Plan& p, std::string& err) {
     for (const Ordinate& od : p.ords) used[od.g.ioct] = true;
     parallel_for(8, [&](int o) {
       rcs[o] = build_octant(p, o, top, bot, (*octs)[o], errs[o]);
-      if (rcs[o] == AT3D_OK && used[o] && !ring_octant_base(p, o, (*octs)[o], errs[o])) rcs[o] = AT3D_ERR_NUMERIC;
+      // (the level-ordered sweep serves base grids only)
+      if (rcs[o] == AT3D_OK && used[o] && !tree && !ring_octant_base(p, o, (*octs)[o], errs[o]))
+        rcs[o] = AT3D_ERR_NUMERIC;
     });
     for (int o = 0; o < 8; o++)
       if (rcs[o]) {
@@ -1184,6 +1246,12 @@ int build_levels(Plan& p, std::string& err) {
         return rcs[o];
       }
     p.oct_store = octs;
+    if (tree) {
+      p.oct = p.oct_store->data();
+      p.nlevels_max = 0;
+      for (int o = 0; o < 8; o++) p.nlevels_max = std::max(p.nlevels_max, p.oct[o].nlevels);
+      return AT3D_OK;
+    }
     std::lock_guard<std::mutex> l(level_cache_mutex);
     level_cache.emplace_front(key, p.oct_store);
     size_t total = 0;
@@ -1325,7 +1393,8 @@ int plan_direct_host(const Plan& p, std::vector<float>& dirflux, std::string& er
   parallel_for(nchunk, [&](int c) {
     const int i0 = (int)((int64_t)p.npts * c / nchunk), i1 = (int)((int64_t)p.npts * (c + 1) / nchunk);
     for (int ip = i0; ip < i1; ip++)
-      if (!direct_beam_point(beam, p.gridpos[ip * 3], p.gridpos[ip * 3 + 1], p.gridpos[ip * 3 + 2], df[ip]))
+      if (!direct_beam_point(beam, p.gridpos[ip * 3], p.gridpos[ip * 3 + 1], p.gridpos[ip * 3 + 2], df[ip],
+                             nullptr))
         bad[c] = 1;
   });
   for (char b : bad)
@@ -1368,10 +1437,6 @@ int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err) {
   const at3d_scene_desc& d = desc;
   if (d.npx < 2 || d.npy < 2 || d.npz < 2) {
     err = "grid must be at least 2x2x2 (1D/2D independent-pixel grids are not supported)";
-    return AT3D_ERR_ARG;
-  }
-  if (d.splitacc > 0.0f) {
-    err = "adaptive cell splitting (split_accuracy > 0) is not supported";
     return AT3D_ERR_ARG;
   }
   if (d.transmin != 1.0f) {
@@ -1448,19 +1513,32 @@ int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err) {
   // host by plan_direct_host when asked for); the plan keeps the constants
   // that scale with it per unit direct flux
   if (!init_radiance(p, err)) return AT3D_ERR_NUMERIC;
-  // BOUNDARY_PNTS (shdomsub1.f:2173-2215)
-  p.bctop.clear();
-  p.bcbot.clear();
-  for (int i = 0; i < p.npts; i++) {
-    if (p.gridpos[i * 3 + 2] >= p.zgrid[p.nz - 1]) p.bctop.push_back(i);
-    if (p.gridpos[i * 3 + 2] <= p.zgrid[0]) p.bcbot.push_back(i);
-  }
+  boundary_points(p);
   make_ordinates(p);
   rc = build_levels(p, err);
   if (rc) return rc;
   rc = render_constants(p, err);
   if (rc) return rc;
+  // the memory parameters bound the adaptive grid (0: the base grid's sizes)
+  p.nbpts = p.npts;
+  p.nbcells = p.ncells;
+  p.maxig = std::max(d.maxig, p.npts);
+  p.maxic = std::max(d.maxic, p.ncells);
+  p.maxiv = d.maxiv > 0 ? d.maxiv : p.nlm * p.npts;
+  p.maxido = d.maxido > 0 ? d.maxido : p.maxig * p.nphi0max;
   return AT3D_OK;
+}
+
+int plan_refresh_points(Plan& p, std::string& err) {
+  source_constants(p);
+  // the two-stream start is a base-grid quantity; new points start from zero
+  p.radiance0.resize((size_t)p.npts * 4, 0.f);
+  boundary_points(p);
+  p.oct_store.reset();
+  p.oct = nullptr;
+  int rc = build_levels(p, err);
+  if (rc) return rc;
+  return render_constants(p, err);
 }
 
 }  // namespace at3d

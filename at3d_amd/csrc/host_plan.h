@@ -70,6 +70,13 @@ struct Plan {
   std::vector<float> xgrid, ygrid, zgrid, zlevels, gridpos;
   std::vector<int32_t> gridptr, neighptr;
   std::vector<int8_t> cellflags;
+  // adaptive cell splitting: the cell tree (empty on a base grid), the base
+  // grid's sizes and the array capacities the reference's memory parameters
+  // give (MAXIG points, MAXIC cells, MAXIV source terms, MAXIDO = MAXIG *
+  // NPHI0MAX), which bound the splitting as in the reference
+  std::vector<int32_t> treeptr;    // [ncells][2] parent, first child (-1: none)
+  int nbpts = 0, nbcells = 0;
+  int maxig = 0, maxic = 0, maxiv = 0, maxido = 0;
   // property grid (kept for the direct beam and the render phase table)
   std::vector<float> extinctp, albedop, phasewtp, legenp;
   std::vector<int32_t> iphasep;
@@ -105,6 +112,22 @@ struct Plan {
 
 // Builds the plan; returns AT3D_OK or an error status with a message.
 int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err);
+
+// After the grid of a plan changed (cells split, points appended, the
+// per-species optics of every point in place): the per-point source and
+// render constants, boundary point lists and the sweep order and levels of
+// the new grid.
+int plan_refresh_points(Plan& p, std::string& err);
+
+// TRILIN_INTERP_PROP for species ipa at one point, before delta-M scaling:
+// extinction, albedo and the eight (table index, weight) pairs.
+void point_optics(const Plan& p, int ipa, float X, float Y, float Z, float& ext, float& alb, int32_t* iph,
+                  float* wt);
+
+// DIRECT_BEAM_PROP at one point (host trace); false if the ray leaves the
+// property grid.
+bool direct_beam_point(const DirectBeamGeom& b, float xi, float yi, float zi, float& dirflux,
+                       std::vector<std::pair<int32_t, float>>* path_rec);
 
 // Real generalised spherical harmonics of one direction (YLMALL_UNPOL,
 // shdomsub2.f:4490-4539), J-ordered, nlm values.

@@ -93,7 +93,8 @@ __global__ __launch_bounds__(128) void k_render(RayArgs a) {
   const float xm = 0.5f * (a.xgrid[0] + a.xgrid[a.nx - 1]);
   const float ym = 0.5f * (a.ygrid[0] + a.ygrid[a.ny - 1]);
   double xe = x0, ye = y0, ze = z0;
-  int icell = locate_cell_ray(a, xe, ye, ze);
+  int icell = locate_cell_ray(a, xe, ye, ze);   // (wraps xe, ye into a periodic domain)
+  if (icell >= 0 && icell < a.g.ncells) icell = locate_in_tree(a.g, icell, xe, ye, ze);
   if (icell < 0 || icell >= a.g.ncells) {
     atomicOr(a.errflag, 4);
     a.out[ir] = 0.0f;
@@ -183,10 +184,13 @@ __global__ __launch_bounds__(128) void k_render(RayArgs a) {
       jface = 3;
       openbcface = false;
     }
-    const int inext = a.g.neighptr[(size_t)icell * 6 + iface];
+    // a face onto smaller cells continues down the neighbour's tree, and the
+    // boundary interpolation takes that smaller face (NEXT_CELL)
+    const int nbv = a.g.neighptr[(size_t)icell * 6 + iface];
+    const int inext = nb_smaller(nbv) ? next_cell_down(a.g, nb_cell(nbv), iface, jface, xn, yn, zn) : nbv;
     int kface, ic;
     bool zface = iface >= 4;
-    if (!openbcface) {
+    if (!openbcface && !nb_smaller(nbv)) {
       kface = iface;
       ic = icell;
     } else {

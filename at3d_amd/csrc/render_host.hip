@@ -196,7 +196,7 @@ static int render_impl(State& s, int64_t nrays, const float* x, const float* y, 
   launch_render_bottom(s.stream, s.fluxes, s.dirflux, s.bcbot, (int)p.bcbot.size(), albpi,
                        s.bcrad_render);
   RayArgs a;
-  a.g = GridTopo{s.gridpos, s.gridptr, s.neighptr, s.cellflags, p.npts, p.ncells};
+  a.g = GridTopo{s.gridpos, s.gridptr, s.neighptr, s.cellflags, p.npts, p.ncells, s.treeptr};
   a.xgrid = s.xgrid;
   a.ygrid = s.ygrid;
   a.zgrid = s.zgrid;

@@ -22,13 +22,66 @@ __host__ __device__ inline int face_corner(int face, int k) {
 }
 
 // Static grid topology (device or host pointers).
+//
+// Cells refined by adaptive splitting form a binary tree per base cell
+// (DIVIDE_CELL, shdomsub1.f:5286-5368): a split cell keeps its index and
+// points to its two children, which are consecutive cells (the negative side
+// first).  Neighbour entries follow the reference's signed convention in
+// 0-based form: c >= 0 is an end cell whose face contains this cell's face
+// (same size or larger), -1 a domain boundary, and -(c+2) a cell c whose
+// face is larger than the cell's own only before it was split: the
+// neighbours across that face are smaller cells found down c's tree.
 struct GridTopo {
   const float* gridpos;     // [npts][3]
   const int32_t* gridptr;   // [ncells][8]
-  const int32_t* neighptr;  // [ncells][6], -1 for a domain boundary
-  const int8_t* cellflags;  // [ncells] bit0: independent pixel in X, bit1: in Y
+  const int32_t* neighptr;  // [ncells][6], see above
+  const int8_t* cellflags;  // [ncells] bit0: independent pixel in X, bit1: in Y,
+                            // bits 2-3: split direction (1 X, 2 Y, 3 Z) of a split cell
   int npts, ncells;
+  const int32_t* treeptr = nullptr;  // [ncells][2] parent, first child (-1: none); null without a tree
 };
+
+// Neighbour entry helpers (see GridTopo).
+__host__ __device__ inline bool nb_smaller(int v) { return v <= -2; }
+__host__ __device__ inline int nb_cell(int v) { return v >= 0 ? v : (v == -1 ? -1 : -v - 2); }
+__host__ __device__ inline int nb_encode(int cell, bool smaller) { return smaller ? -cell - 2 : cell; }
+
+// NEXT_CELL (shdomsub1.f:4470-4518): the end cell across face `iface`
+// (0..5) of a cell whose neighbour entry there is -(c+2), found by walking
+// down c's tree with the crossing point (xe, ye, ze); jface = 1, 2, 3 for an
+// x, y, z face.
+__host__ __device__ inline int next_cell_down(const GridTopo& g, int c, int iface, int jface, double xe,
+                                              double ye, double ze) {
+  int ic = c;
+  while (g.treeptr[ic * 2 + 1] >= 0) {
+    const int dir = (g.cellflags[ic] >> 2) & 3;
+    const int ic1 = g.treeptr[ic * 2 + 1];
+    if (dir == jface) {
+      ic = ic1 + 1 - (iface & 1);
+    } else {
+      const float* split = g.gridpos + (size_t)g.gridptr[ic1 * 8 + 7] * 3;
+      const double coord = dir == 1 ? xe : (dir == 2 ? ye : ze);
+      ic = coord > (double)split[dir - 1] ? ic1 + 1 : ic1;
+    }
+  }
+  return ic;
+}
+
+// The tree descent of LOCATE_GRID_CELL (shdomsub2.f:4181-4199): the end cell
+// of base cell `c` that holds (x, y, z).
+__host__ __device__ inline int locate_in_tree(const GridTopo& g, int c, double x, double y, double z) {
+  if (!g.treeptr) return c;
+  int ic = c;
+  while (g.treeptr[ic * 2 + 1] >= 0) {
+    const int dir = (g.cellflags[ic] >> 2) & 3;
+    int child = g.treeptr[ic * 2 + 1] + 1;
+    const float* p1 = g.gridpos + (size_t)g.gridptr[child * 8] * 3;
+    const double coord = dir == 1 ? x : (dir == 2 ? y : z);
+    if (dir >= 1 && dir <= 3 && coord < (double)p1[dir - 1]) child -= 1;
+    ic = child;
+  }
+  return ic;
+}
 
 // Per-ordinate trace-back direction (BACK_INT_GRID3D_UNPOL shdomsub1.f:3736-3785).
 struct OrdGeom {
@@ -79,10 +132,19 @@ __host__ __device__ inline bool trace_step(const GridTopo& g, const OrdGeom& o, 
     iface = 5 - o.bitz;
     jface = 3;
   }
-  const int inext = g.neighptr[icell * 6 + iface];
-  const int* gp = g.gridptr + icell * 8;
-  const int i1 = gp[face_corner(iface, 0)], i2 = gp[face_corner(iface, 1)];
-  const int i3 = gp[face_corner(iface, 2)], i4 = gp[face_corner(iface, 3)];
+  const int nb = g.neighptr[icell * 6 + iface];
+  int inext = nb;
+  // a face onto smaller cells: the next cell down the neighbour's tree, and
+  // its (smaller, more accurate) face holds the interpolation corners
+  int kcell = icell, kface = iface;
+  if (nb_smaller(nb)) {
+    inext = next_cell_down(g, nb_cell(nb), iface, jface, xe, ye, ze);
+    kcell = inext;
+    kface = iface ^ 1;
+  }
+  const int* gp = g.gridptr + kcell * 8;
+  const int i1 = gp[face_corner(kface, 0)], i2 = gp[face_corner(kface, 1)];
+  const int i3 = gp[face_corner(kface, 2)], i4 = gp[face_corner(kface, 3)];
   double u, v;
   if (jface == 1) {
     u = (ze - (double)P[i1 * 3 + 2]) / ((double)P[i3 * 3 + 2] - (double)P[i1 * 3 + 2]);

@@ -34,6 +34,7 @@
 #include "ring_plan.h"
 #include "shdom_geom.h"
 #include "solver_state.h"
+#include "split_grid.h"
 
 namespace at3d {
 
@@ -94,7 +95,7 @@ static int alloc0(State& s, T*& dst, size_t n, std::string& err) {
 static SweepArgs sweep_args(const State& s) {
   const Plan& p = *s.plan;
   SweepArgs a;
-  a.g = GridTopo{s.gridpos, s.gridptr, s.neighptr, s.cellflags, p.npts, p.ncells};
+  a.g = GridTopo{s.gridpos, s.gridptr, s.neighptr, s.cellflags, p.npts, p.ncells, s.treeptr};
   a.ordg = s.ordg;
   a.skyrad = s.skyrad;
   a.bcrad_bot = s.bcrad_bot;
@@ -171,6 +172,8 @@ static bool ring_active(const State& s) { return s.ring_ok && s.use_ring && !s.f
 // in LDS or the records do not fit in memory.
 static int setup_ring(const Plan& p, State& s, std::string& err) {
   s.ring_ok = false;
+  // the step classes and ring layout assume the base grid's regular cells
+  if (!p.treeptr.empty()) return AT3D_OK;
   bool used[8];
   for (int o = 0; o < 8; o++) used[o] = s.oct_nord[o] > 0;
   int dev = 0, maxlds = 0;
@@ -363,6 +366,7 @@ size_t plan_footprint_bytes(const Plan& p) {
 
 int state_create(const Plan& p, State& s, std::string& err) {
   s.plan = &p;
+  if (!s.base_plan) s.base_plan = &p;
   HIPCHK(hipGetDevice(&s.device));
   HIPCHK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
   s.npts = p.npts;
@@ -472,6 +476,7 @@ int state_create(const Plan& p, State& s, std::string& err) {
   }
   TRY(upload(s, s.gridptr, p.gridptr.data(), p.gridptr.size(), err));
   TRY(upload(s, s.neighptr, p.neighptr.data(), p.neighptr.size(), err));
+  if (!p.treeptr.empty()) TRY(upload(s, s.treeptr, p.treeptr.data(), p.treeptr.size(), err));
   TRY(upload(s, s.cellflags, p.cellflags.data(), p.cellflags.size(), err));
   TRY(upload(s, s.xgrid, p.xgrid.data(), p.xgrid.size(), err));
   TRY(upload(s, s.ygrid, p.ygrid.data(), p.ygrid.size(), err));
@@ -590,6 +595,75 @@ void state_free(State& s) {
   s.stream = nullptr;
 }
 
+// Replaces s by a new state of the refined grid `owned` (null: s's base
+// plan), keeping the options and the iteration state; the solution arrays
+// come from `f` (the refined grid's points; null: none, as before an
+// initialisation).
+static int state_replace(State& s, std::shared_ptr<Plan> owned, SplitFields* f, std::string& err) {
+  const Plan& q = owned ? *owned : *s.base_plan;
+  State n;
+  n.base_plan = s.base_plan;
+  n.use_ring = s.use_ring;
+  n.force_general = s.force_general;
+  n.use_graph = s.use_graph;
+  n.use_iter_graph = s.use_iter_graph;
+  int rc = state_create(q, n, err);
+  if (rc) {
+    state_free(n);
+    return rc;
+  }
+  if (f) {
+    const size_t np = q.npts, ld = n.ld;
+    if ((int)ld != f->ld || f->S.size() < np * ld || f->ns.size() < np) {
+      state_free(n);
+      err = "internal: refined grid fields do not match the new state";
+      return AT3D_ERR_ARG;
+    }
+    std::vector<float> flux((size_t)2 * np, 0.f);
+    std::vector<float> old((size_t)2 * s.npts);
+    HIPCHK(hipMemcpyAsync(old.data(), s.fluxes, old.size() * sizeof(float), hipMemcpyDeviceToHost, s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    for (int i = 0; i < s.npts; i++) {
+      flux[i] = old[i];
+      flux[np + i] = old[(size_t)s.npts + i];
+    }
+    HIPCHK(hipMemcpyAsync(n.S, f->S.data(), np * ld * sizeof(float), hipMemcpyHostToDevice, n.stream));
+    HIPCHK(hipMemcpyAsync(n.R, f->R.data(), np * ld * sizeof(float), hipMemcpyHostToDevice, n.stream));
+    HIPCHK(hipMemcpyAsync(n.DS, f->DS.data(), np * ld * sizeof(float), hipMemcpyHostToDevice, n.stream));
+    HIPCHK(hipMemcpyAsync(n.ns, f->ns.data(), np * sizeof(int), hipMemcpyHostToDevice, n.stream));
+    HIPCHK(hipMemcpyAsync(n.nr, f->nr.data(), np * sizeof(int), hipMemcpyHostToDevice, n.stream));
+    HIPCHK(hipMemcpyAsync(n.nos, f->nos.data(), np * sizeof(int), hipMemcpyHostToDevice, n.stream));
+    HIPCHK(hipMemcpyAsync(n.fluxes, flux.data(), flux.size() * sizeof(float), hipMemcpyHostToDevice, n.stream));
+    // this iteration's radiance truncation (RADIANCE_TRUNCATION follows
+    // SPLIT_GRID, shdomsub1.f:723-731) with the FIXSH of the last one
+    launch_radiance_truncation(n.stream, n.R, n.ld, n.tc, n.ns, n.lofj, n.nr_next, q.npts, q.ml, q.mm, q.nlm,
+                               q.d.shacc, s.nr_next_fixsh ? 1 : 0, q.d.high_order_radiance ? 1 : 0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(n.stream));
+    n.initialized = s.initialized;
+    n.iters = s.iters;
+    n.iters_last = s.iters_last;
+    n.nsweep_launch = s.nsweep_launch;
+    n.solcrit = s.solcrit;
+    n.deljdot = s.deljdot;
+    n.deljold = s.deljold;
+    n.deljnew = s.deljnew;
+    n.jnorm = s.jnorm;
+    n.nr_next_fixsh = s.nr_next_fixsh;
+    n.splitcrit = s.splitcrit;
+    for (int k = 0; k < 8; k++) n.tsum[k] = s.tsum[k];
+  }
+  n.own_plan = owned;
+  state_free(s);
+  s = n;
+  return AT3D_OK;
+}
+
+int state_reset_grid(State& s, std::string& err) {
+  if (!s.own_plan) return AT3D_OK;
+  return state_replace(s, nullptr, nullptr, err);
+}
+
 static SourceArgs source_args(State& s, int first, int fixsh) {
   const Plan& p = *s.plan;
   SourceArgs a;
@@ -626,6 +700,8 @@ static SourceArgs source_args(State& s, int first, int fixsh) {
 // the extents of the terms S and DS hold from an earlier solve, which the
 // first source pass clears.
 int state_init_solution(State& s, std::string& err) {
+  // a state on a refined grid starts again from its base grid
+  if (s.own_plan) TRY(state_replace(s, nullptr, nullptr, err));
   const Plan& p = *s.plan;
   launch_init_radiance(s.stream, s.R, s.R0, s.nr, s.ld, p.npts);
   launch_compute_source(s.stream, source_args(s, 1, 0), s.nsrc_blocks);
@@ -645,7 +721,8 @@ static bool same_grid_and_layout(const Plan& p, const Plan& q) {
   return p.npts == q.npts && p.ncells == q.ncells && p.nx1 == q.nx1 && p.ny1 == q.ny1 &&
          p.nz == q.nz && p.bcflag == q.bcflag && p.ipflag == q.ipflag && p.ml == q.ml &&
          p.mm == q.mm && p.nlm == q.nlm && p.xgrid == q.xgrid && p.ygrid == q.ygrid &&
-         p.zgrid == q.zgrid;
+         p.zgrid == q.zgrid && p.treeptr == q.treeptr &&
+         (p.treeptr.empty() || (p.gridpos == q.gridpos && p.gridptr == q.gridptr));
 }
 
 // Warm start from another state's solution: its spherical-harmonic radiance
@@ -902,9 +979,163 @@ static int state_solve_graph(State& s, int maxiter, std::string& err) {
   return AT3D_OK;
 }
 
-int state_solve(State& s, int maxiter, int verbose, std::string& err) {
+// One iteration under host control (the loop body of SOLUTION_ITERATIONS,
+// shdomsub1.f:723-790, after the splitting): the NR made by the last source
+// pass, SH->DO, the sweeps, DO->SH, the source with the next radiance
+// truncation, and the dots read back for SOLCRIT and the acceleration.
+static int host_iteration(State& s, bool& fixsh, float& a, int verbose, std::string& err) {
   const Plan& p = *s.plan;
+  s.iters++;
+  s.iters_last++;
+  // RADIANCE_TRUNCATION of this iteration was done by the last source pass
+  // (into nr_next): it becomes the current NR
+  std::swap(s.nr, s.nr_next);
+  (void)hipEventRecord(s.ev[0], s.stream);
+  s.dosrc_level_order = ring_active(s);
+  if (s.dosrc_level_order)
+    launch_sh_to_do_perm(s.stream, s.S, s.Y, s.ext, s.ns, s.oct_perm, s.sh_tiles, s.n_sh_tiles,
+                         s.dosrc, s.nptsp, s.ld, s.nptsp, p.npts);
+  else
+    launch_sh_to_do(s.stream, s.S, s.Y, s.ext, s.dosrc, s.nptsp, s.ldo, s.ld, s.nptsp, p.npts,
+                    p.nang);
+  (void)hipEventRecord(s.ev[2], s.stream);
+  TRY(run_sweeps(s, err));
+  (void)hipEventRecord(s.ev[3], s.stream);
+  launch_do_to_sh(s.stream, s.dorad, s.W, s.nr, s.R, s.nptsp, s.ld, s.ldo, s.nptsp, p.npts);
+  (void)hipEventRecord(s.ev[4], s.stream);
+  if (s.solcrit < 0.001f || s.iters > p.d.iterfixsh) fixsh = true;
+  launch_compute_source(s.stream, source_args(s, 0, fixsh ? 1 : 0), s.nsrc_blocks);
+  s.nr_next_fixsh = fixsh;
+  launch_reduce_dots(s.stream, s.partial, s.nsrc_blocks, s.dots);
+  (void)hipEventRecord(s.ev[5], s.stream);
+  HIPCHK(hipGetLastError());
+  // the four dot products and the error flag into pinned host memory
+  double* dots = s.host_dots;
+  int* efp = reinterpret_cast<int*>(s.host_dots + 4);
+  HIPCHK(hipMemcpyAsync(dots, s.dots, 4 * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+  HIPCHK(hipMemcpyAsync(efp, s.errflag, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+  HIPCHK(hipStreamSynchronize(s.stream));
+  const int ef = *efp;
+  if (ef) {
+    char buf[160];
+    snprintf(buf, sizeof buf, "BACK_INT_GRID: invalid characteristic (flags 0x%x) at iteration %d",
+             ef, s.iters);
+    err = buf;
+    return AT3D_ERR_NUMERIC;
+  }
+  // (the radiance truncation is part of the source pass: tsum[0] stays 0)
+  float ms[5] = {};
+  for (int e = 1; e < 5; e++) (void)hipEventElapsedTime(&ms[e], s.ev[e == 1 ? 0 : e], s.ev[e + 1]);
+  s.tsum[1] += ms[1];  // SH -> DO
+  s.tsum[2] += ms[2];  // sweeps (both hemispheres, surface boundary)
+  s.tsum[3] += ms[3];  // DO -> SH
+  s.tsum[4] += ms[4];  // source (with the next truncation) + reduction
+  if (ring_active(s)) {
+    float a0 = 0.f, a1 = 0.f;
+    (void)hipEventElapsedTime(&a0, s.ev[2], s.ev_sweep[1]);
+    (void)hipEventElapsedTime(&a1, s.ev_sweep[2], s.ev[3]);
+    s.tsum[6] += a0 + a1;  // the two sweep kernel launches alone
+  }
+  s.nsweep_launch += 1;
+  s.deljdot = (float)dots[0];
+  s.deljold = (float)dots[1];
+  s.deljnew = (float)dots[2];
+  s.jnorm = (float)dots[3];
+  float accelpar;
+  accel_solcrit(p.d.accelflag != 0, s.deljdot, s.deljold, s.deljnew, s.jnorm, a, accelpar, s.solcrit);
+  if (accelpar > 0.0f) launch_accelerate(s.stream, s.S, s.DS, s.ns, s.nos, s.ld, p.npts, accelpar);
+  if (p.albmax < p.d.solacc) s.solcrit = p.d.solacc;
+  if (verbose) {
+    printf("  %4d %8.3f %10.3e %8d\n", s.iters, log10f(std::max(s.solcrit, 1.0e-20f)), s.splitcrit, p.npts);
+    fflush(stdout);
+  }
+  return AT3D_OK;
+}
+
+// The splitting test of one iteration (SPLIT_GRID, shdomsub1.f:694-715): the
+// source (and, to divide cells, the radiance) comes to the host, the grid is
+// refined there, and the state moves to the refined grid when it changed.
+static int split_step(State& s, bool dosplit, SplitSchedule& sc, std::string& err) {
+  const Plan& p = *s.plan;
+  const size_t np = p.npts, ld = s.ld;
+  SplitFields f;
+  f.ld = s.ld;
+  f.S.resize(np * ld);
+  f.ns.resize(np);
+  HIPCHK(hipMemcpyAsync(f.S.data(), s.S, np * ld * sizeof(float), hipMemcpyDeviceToHost, s.stream));
+  HIPCHK(hipMemcpyAsync(f.ns.data(), s.ns, np * sizeof(int), hipMemcpyDeviceToHost, s.stream));
+  if (dosplit) {
+    f.R.resize(np * ld);
+    f.DS.resize(np * ld);
+    f.nr.resize(np);
+    f.nos.resize(np);
+    HIPCHK(hipMemcpyAsync(f.R.data(), s.R, np * ld * sizeof(float), hipMemcpyDeviceToHost, s.stream));
+    HIPCHK(hipMemcpyAsync(f.DS.data(), s.DS, np * ld * sizeof(float), hipMemcpyDeviceToHost, s.stream));
+    HIPCHK(hipMemcpyAsync(f.nr.data(), s.nr, np * sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    HIPCHK(hipMemcpyAsync(f.nos.data(), s.nos, np * sizeof(int), hipMemcpyDeviceToHost, s.stream));
+  }
+  HIPCHK(hipStreamSynchronize(s.stream));
+  if (!dosplit) {
+    sc.splitcrit = split_criterion(p, f);
+    s.splitcrit = sc.splitcrit;
+    return AT3D_OK;
+  }
+  auto q = std::make_shared<Plan>(p);
+  bool outofmem = sc.outofmem;
+  float crit = 0.0f;
+  TRY(split_grid(*q, f, true, outofmem, sc.cursplitacc, crit, err));
+  sc.outofmem = outofmem;
+  sc.splitcrit = crit;
+  s.splitcrit = crit;
+  if (q->ncells == p.ncells) return AT3D_OK;   // nothing was divided
+  TRY(plan_refresh_points(*q, err));
+  return state_replace(s, q, &f, err);
+}
+
+// SOLUTION_ITERATIONS with adaptive cell splitting (shdomsub1.f:636-790):
+// the splitting accuracy starts at SPLITACC * ADAPTRANGE and follows the
+// geometric mean of SOLCRIT down to SPLITACC while SOLCRIT falls from 0.1 to
+// 0.001; the loop also runs on while cells remain above the splitting
+// accuracy.  Iterations run under host control (the grid can change between
+// them), the level-ordered sweep stays off (it needs the base grid).
+static int solve_split(State& s, int maxiter, int verbose, std::string& err) {
+  const float splitacc = s.plan->d.splitacc, solacc = s.plan->d.solacc;
+  const float endadaptsol = 0.001f, startadaptsol = 0.1f;
+  const float adaptrange = startadaptsol / (3.0f * endadaptsol);
+  SplitSchedule sc;
+  sc.cursplitacc = splitacc * adaptrange;
+  sc.startsplitacc = sc.cursplitacc;
+  sc.avgsolcrit = s.solcrit;
+  sc.splitcrit = 0.0f;
+  float a = 0.0f;
+  bool fixsh = false;
+  while (s.iters < maxiter &&
+         (s.solcrit > solacc || (sc.splitcrit > splitacc && sc.cursplitacc > splitacc && !sc.outofmem))) {
+    sc.avgsolcrit = sqrtf(sc.avgsolcrit * s.solcrit);
+    const bool dosplit =
+        s.solcrit <= startadaptsol && (s.solcrit > endadaptsol || sc.cursplitacc > splitacc) && !sc.outofmem;
+    const float beta = logf(sc.startsplitacc / splitacc) / logf(adaptrange);
+    sc.cursplitacc = std::min(sc.cursplitacc,
+                              std::max(splitacc, splitacc * powf(sc.avgsolcrit / (3.0f * endadaptsol), beta)));
+    if (s.solcrit <= endadaptsol) sc.cursplitacc = splitacc;
+    if (sc.testing) {
+      TRY(split_step(s, dosplit, sc, err));
+      if (s.solcrit > startadaptsol) sc.startsplitacc = sc.splitcrit;
+    }
+    if (s.solcrit <= endadaptsol) sc.testing = false;
+    TRY(host_iteration(s, fixsh, a, verbose, err));
+  }
+  const Plan& p = *s.plan;
+  if (s.iters_last > 0)
+    launch_fluxes(s.stream, s.dorad, s.nptsp, s.col_of_ref_d, s.fluxw, s.ndown, p.nang, p.npts, s.fluxes);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s.stream));
+  return AT3D_OK;
+}
+
+int state_solve(State& s, int maxiter, int verbose, std::string& err) {
   if (!s.initialized) TRY(state_init_solution(s, err));
+  const Plan& p = *s.plan;
   HIPCHK(hipMemsetAsync(s.errflag, 0, sizeof(int), s.stream));
   float a = 0.0f;
   bool fixsh = false;
@@ -919,6 +1150,8 @@ int state_solve(State& s, int maxiter, int verbose, std::string& err) {
                                p.nlm, p.d.shacc, 0, p.d.high_order_radiance ? 1 : 0);
     s.nr_next_fixsh = false;
   }
+  // (p is not used past this call: the state's plan changes when it splits)
+  if (p.d.splitacc > 0.0f) return solve_split(s, maxiter, verbose, err);
   // graphs replay the level-ordered sweep; the point-ordered one (grids whose
   // ring does not fit in LDS, and the test options) iterates from the host
   if (s.use_iter_graph && !verbose && ring_active(s) && s.iters < maxiter && s.solcrit > p.d.solacc) {
@@ -929,72 +1162,7 @@ int state_solve(State& s, int maxiter, int verbose, std::string& err) {
     HIPCHK(hipStreamSynchronize(s.stream));
     return AT3D_OK;
   }
-  while (s.iters < maxiter && s.solcrit > p.d.solacc) {
-    s.iters++;
-    s.iters_last++;
-    // RADIANCE_TRUNCATION of this iteration was done by the last source pass
-    // (into nr_next): it becomes the current NR
-    std::swap(s.nr, s.nr_next);
-    (void)hipEventRecord(s.ev[0], s.stream);
-    s.dosrc_level_order = ring_active(s);
-    if (s.dosrc_level_order)
-      launch_sh_to_do_perm(s.stream, s.S, s.Y, s.ext, s.ns, s.oct_perm, s.sh_tiles, s.n_sh_tiles,
-                           s.dosrc, s.nptsp, s.ld, s.nptsp, p.npts);
-    else
-      launch_sh_to_do(s.stream, s.S, s.Y, s.ext, s.dosrc, s.nptsp, s.ldo, s.ld, s.nptsp, p.npts,
-                      p.nang);
-    (void)hipEventRecord(s.ev[2], s.stream);
-    TRY(run_sweeps(s, err));
-    (void)hipEventRecord(s.ev[3], s.stream);
-    launch_do_to_sh(s.stream, s.dorad, s.W, s.nr, s.R, s.nptsp, s.ld, s.ldo, s.nptsp, p.npts);
-    (void)hipEventRecord(s.ev[4], s.stream);
-    if (s.solcrit < 0.001f || s.iters > p.d.iterfixsh) fixsh = true;
-    launch_compute_source(s.stream, source_args(s, 0, fixsh ? 1 : 0), s.nsrc_blocks);
-    s.nr_next_fixsh = fixsh;
-    launch_reduce_dots(s.stream, s.partial, s.nsrc_blocks, s.dots);
-    (void)hipEventRecord(s.ev[5], s.stream);
-    HIPCHK(hipGetLastError());
-    // the four dot products and the error flag into pinned host memory
-    double* dots = s.host_dots;
-    int* efp = reinterpret_cast<int*>(s.host_dots + 4);
-    HIPCHK(hipMemcpyAsync(dots, s.dots, 4 * sizeof(double), hipMemcpyDeviceToHost, s.stream));
-    HIPCHK(hipMemcpyAsync(efp, s.errflag, sizeof(int), hipMemcpyDeviceToHost, s.stream));
-    HIPCHK(hipStreamSynchronize(s.stream));
-    const int ef = *efp;
-    if (ef) {
-      char buf[160];
-      snprintf(buf, sizeof buf, "BACK_INT_GRID: invalid characteristic (flags 0x%x) at iteration %d",
-               ef, s.iters);
-      err = buf;
-      return AT3D_ERR_NUMERIC;
-    }
-    // (the radiance truncation is part of the source pass: tsum[0] stays 0)
-    float ms[5] = {};
-    for (int e = 1; e < 5; e++) (void)hipEventElapsedTime(&ms[e], s.ev[e == 1 ? 0 : e], s.ev[e + 1]);
-    s.tsum[1] += ms[1];  // SH -> DO
-    s.tsum[2] += ms[2];  // sweeps (both hemispheres, surface boundary)
-    s.tsum[3] += ms[3];  // DO -> SH
-    s.tsum[4] += ms[4];  // source (with the next truncation) + reduction
-    if (ring_active(s)) {
-      float a0 = 0.f, a1 = 0.f;
-      (void)hipEventElapsedTime(&a0, s.ev[2], s.ev_sweep[1]);
-      (void)hipEventElapsedTime(&a1, s.ev_sweep[2], s.ev[3]);
-      s.tsum[6] += a0 + a1;  // the two sweep kernel launches alone
-    }
-    s.nsweep_launch += 1;
-    s.deljdot = (float)dots[0];
-    s.deljold = (float)dots[1];
-    s.deljnew = (float)dots[2];
-    s.jnorm = (float)dots[3];
-    float accelpar;
-    accel_solcrit(p.d.accelflag != 0, s.deljdot, s.deljold, s.deljnew, s.jnorm, a, accelpar, s.solcrit);
-    if (accelpar > 0.0f) launch_accelerate(s.stream, s.S, s.DS, s.ns, s.nos, s.ld, p.npts, accelpar);
-    if (p.albmax < p.d.solacc) s.solcrit = p.d.solacc;
-    if (verbose) {
-      printf("  %4d %8.3f %8d\n", s.iters, log10f(std::max(s.solcrit, 1.0e-20f)), p.npts);
-      fflush(stdout);
-    }
-  }
+  while (s.iters < maxiter && s.solcrit > p.d.solacc) TRY(host_iteration(s, fixsh, a, verbose, err));
   // hemispherical fluxes of the last path integration (a call that ran no
   // iteration keeps the ones the state holds)
   if (s.iters_last > 0)

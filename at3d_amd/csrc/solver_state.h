@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <memory>
 #include <mutex>
 #include <string>
 #include <utility>
@@ -22,6 +23,12 @@ constexpr size_t RENDER_SCRATCH_BYTES = (size_t)64 << 20;
 
 struct State {
   const Plan* plan = nullptr;
+  // adaptive cell splitting: the refined grid's plan, owned by the state
+  // (null while the state is on its base grid), and the plan it started from
+  std::shared_ptr<Plan> own_plan;
+  const Plan* base_plan = nullptr;
+  int32_t* treeptr = nullptr;   // device [ncells][2] of a refined grid, else null
+  float splitcrit = 0.0f;       // SPLITCRIT of the last splitting test
   int device = 0;   // HIP device the state was created on
   hipStream_t stream = nullptr;
   int npts = 0, nlm = 0, nang = 0;
@@ -172,6 +179,9 @@ int state_save(State& s, int32_t* shptr, int32_t* rshptr, float* source, int64_t
 int state_sh_terms(State& s, int nshout, float* shterms, float* netfluxdiv, std::string& err);
 
 int state_create(const Plan& p, State& s, std::string& err);
+// Moves a state on a refined grid back to its base grid (no solution: the
+// next call initialises or loads one).
+int state_reset_grid(State& s, std::string& err);
 size_t plan_footprint_bytes(const Plan& p);
 void state_free(State& s);
 int state_init_solution(State& s, std::string& err);

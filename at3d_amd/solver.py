@@ -124,6 +124,7 @@ class RTE:
         d.transmin = float(npar["transmin"])
         d.tautol = float(npar["tautol"])
         d.transcut = float(npar["transcut"])
+        d.maxig, d.maxic, d.maxiv, d.maxido = self._memory_params(npar, npx, npy, npz, table)
         self._keep = [ext, alb, iph, pw, zl, legenp]
         self._desc = d
         L = _lib.lib()
@@ -131,11 +132,70 @@ class RTE:
         err = _lib.errbuf()
         _lib.check(L.at3d_plan_create(C.byref(d), C.byref(h), err, len(err)), err)
         self._plan = h
-        self._npts = int(L.at3d_plan_size(h, b"npts"))
+        self._npts = int(L.at3d_plan_size(h, b"npts"))   # the current grid's (splitting adds points)
+        self._nbpts = self._npts                          # the base grid's
         self._nlm = int(L.at3d_plan_size(h, b"nlm"))
         self._nang = int(L.at3d_plan_size(h, b"nang"))
         self._ml = int(L.at3d_plan_size(h, b"ml"))
         self._mm = int(L.at3d_plan_size(h, b"mm"))
+
+    def _memory_params(self, npar, npx, npy, npz, table):
+        """Array capacities of the adaptive grid as RTE._setup_memory sizes
+        them (solver.py:2170-2322): MAXIG, MAXIC, MAXIV and MAXIDO from
+        max_total_mb, adapt_grid_factor, cell_to_point_ratio and
+        num_sh_term_factor; the splitting stops where these run out."""
+        import psutil
+        bc_open = npar["x_boundary_condition"] == "open"
+        nx1 = npx if bc_open else npx + 1
+        ny1 = npy if bc_open else npy + 1
+        nbpts = nx1 * ny1 * npz
+        nbcells = (npx + 1 if bc_open else npx) * (npy + 1 if bc_open else npy) * (npz - 1)
+        nmu = max(2, 2 * ((int(npar["num_mu_bins"]) + 1) // 2))
+        nphi = max(1, int(npar["num_phi_bins"]))
+        ml, mm = nmu - 1, max(0, nphi // 2 - 1)
+        nlm = (2 * mm + 1) * (ml + 1) - mm * (mm + 1)
+        nphi0max = nphi
+        max_total_mb = float(npar["max_total_mb"])
+        if max_total_mb * 1024 ** 2 > 0.9 * psutil.virtual_memory().total:
+            max_total_mb = 0.9 * psutil.virtual_memory().total / 1024 ** 2
+        splitacc = float(npar["split_accuracy"])
+        adapt = float(npar["adapt_grid_factor"])
+        if splitacc < 0.0:
+            adapt = 1.0
+        big_arrays = 3 if bool(npar["acceleration_flag"]) else 2
+        maxpg = npx * npy * npz
+        numphase, maxleg = table.shape[1], table.shape[0]   # (the legendre_index size)
+        memword = nmu * (2 + 2 * nphi + 2 * nlm + 2 * 33 * 32) + 4.5 * maxpg + 2 * numphase * maxleg
+        ratio, shfac = float(npar["cell_to_point_ratio"]), float(npar["num_sh_term_factor"])
+        wantmem = adapt * nbpts * (28 + 16.5 * ratio + nphi0max + shfac * nlm * big_arrays)
+        reduce = min(1.0, ((max_total_mb * 1024 ** 2) / 4 - memword) / wantmem)
+        adapt *= reduce
+        if adapt < 1.0:
+            raise _lib.SHDOMError(
+                3, "max_total_mb memory limit ({}) exceeded with just base grid (final "
+                "adapt_grid_factor={}). Either increase memory or reduce accuracy parameters "
+                "(num_mu_bins={}, num_phi_bins={}, num_sh_term_factor={})".format(
+                    max_total_mb, adapt, npar["num_mu_bins"], npar["num_phi_bins"], shfac))
+        maxig = int(adapt * nbpts)
+        maxic = max(int(ratio * maxig), nbcells)
+        maxiv = max(int(shfac * nlm * maxig), nbpts * 4)
+        return maxig, maxic, maxiv, maxig * nphi0max
+
+    def _sync_grid(self):
+        """The current grid's size after a solve (adaptive splitting adds points)."""
+        if self._state is not None:
+            self._npts = int(_lib.lib().at3d_state_size(self._state, b"npts"))
+
+    def state_size(self, key: str) -> int:
+        """Sizes of the solver state's current grid ("npts", "ncells", "nbpts",
+        "nbcells", "ntop", "nbot", "refined")."""
+        self._ensure_state()
+        return int(_lib.lib().at3d_state_size(self._state, key.encode()))
+
+    @property
+    def final_adapt_grid_factor(self):
+        """Points of the (refined) grid over the base grid's (solver.py:255)."""
+        return None if self._state is None else self._npts / self._nbpts
 
     # -- plan inspection ---------------------------------------------------------
     def plan_size(self, key: str) -> int:
@@ -147,8 +207,8 @@ class RTE:
         return self.plan_size("footprint_bytes")
 
     _PLAN_SHAPES = {
-        "gridpos": ("f4", lambda s: (s._npts, 3)), "dirflux": ("f4", lambda s: (s._npts,)),
-        "total_ext": ("f4", lambda s: (s._npts,)), "radiance0": ("f4", lambda s: (s._npts, 4)),
+        "gridpos": ("f4", lambda s: (s._nbpts, 3)), "dirflux": ("f4", lambda s: (s._nbpts,)),
+        "total_ext": ("f4", lambda s: (s._nbpts,)), "radiance0": ("f4", lambda s: (s._nbpts, 4)),
         "ymat": ("f4", lambda s: (s._nang, s._nlm)), "wmat": ("f4", lambda s: (s._nang, s._nlm)),
         "ylmsun": ("f4", lambda s: (s._nlm,)), "mu": ("f4", lambda s: (s.plan_size("nmu"),)),
         "wtmu": ("f4", lambda s: (s.plan_size("nmu"),)), "nphi0": ("i4", lambda s: (s.plan_size("nmu"),)),
@@ -157,13 +217,13 @@ class RTE:
         "gridptr": ("i4", lambda s: (s.plan_size("ncells"), 8)),
         "neighptr": ("i4", lambda s: (s.plan_size("ncells"), 6)),
         "cellflags": ("i1", lambda s: (s.plan_size("ncells"),)),
-        "sweepord": ("i4", lambda s: (8, s._npts)), "level": ("i4", lambda s: (8, s._npts)),
+        "sweepord": ("i4", lambda s: (8, s._nbpts)), "level": ("i4", lambda s: (8, s._nbpts)),
         "lvl_pts": ("i4", lambda s: (int((s.plan_array("level") > 0).sum()),)),
-        "ring_level": ("i4", lambda s: (8, s._npts)), "ring_pos": ("i4", lambda s: (8, s._npts)),
-        "ring_staged": ("i4", lambda s: (8, s._npts)), "ring_sizes": ("i4", lambda s: (10,)),
-        "chi": ("f4", lambda s: (s._npts, s._ml + 1)), "alb": ("f4", lambda s: (s._npts,)),
-        "extinct": ("f4", lambda s: (s.plan_size("npart"), s._npts)),
-        "albedo": ("f4", lambda s: (s.plan_size("npart"), s._npts)),
+        "ring_level": ("i4", lambda s: (8, s._nbpts)), "ring_pos": ("i4", lambda s: (8, s._nbpts)),
+        "ring_staged": ("i4", lambda s: (8, s._nbpts)), "ring_sizes": ("i4", lambda s: (10,)),
+        "chi": ("f4", lambda s: (s._nbpts, s._ml + 1)), "alb": ("f4", lambda s: (s._nbpts,)),
+        "extinct": ("f4", lambda s: (s.plan_size("npart"), s._nbpts)),
+        "albedo": ("f4", lambda s: (s.plan_size("npart"), s._nbpts)),
         "bcptr_top": ("i4", lambda s: (s.plan_size("ntop"),)),
         "bcptr_bot": ("i4", lambda s: (s.plan_size("nbot"),)),
         "xgrid": ("f4", lambda s: (s.plan_size("nx1"),)), "ygrid": ("f4", lambda s: (s.plan_size("ny1"),)),
@@ -201,12 +261,14 @@ class RTE:
             else:
                 _lib.check(L.at3d_init_solution(self._state, err, len(err)), err)
             self._solcrit, self._iters = 1.0, 0
+            self._sync_grid()
         if not solve:
             return
         sc, it = C.c_float(), C.c_int32()
         _lib.check(L.at3d_solve(self._state, int(maxiter), int(verbose), C.byref(sc), C.byref(it),
                                 err, len(err)), err)
         self._solcrit, self._iters = float(sc.value), int(it.value)
+        self._sync_grid()
         if not self.check_solved(verbose=False):
             warnings.warn("solution did not converge to solution_accuracy={} in maxiter={} "
                           "(SOLCRIT={})".format(self.solution_accuracy, maxiter, self._solcrit))
@@ -231,17 +293,20 @@ class RTE:
         Fortran order: gridpos (3, npts), gridptr (8, ncells) and neighptr
         (6, ncells) 1-based, source (nstokes, SHPTR[npts]), radiance
         (nstokes, RSHPTR[npts])).  A dict of numpy arrays (xarray is not used)."""
-        npts, ncells = self._npts, self.plan_size("ncells")
+        g = self.grid_arrays()
+        npts, ncells = g["gridpos"].shape[0], g["gridptr"].shape[0]
+        # the 0-based pointers here map to the reference's 1-based ones by +1
+        # (-1 = none -> 0; -(c+2), a pointer to split cell c -> -(c+1))
         out = {"nstokes": self._nstokes, "nx": self.plan_size("nx"), "ny": self.plan_size("ny"),
                "nz": self.plan_size("nz"), "ml": self._ml, "mm": self._mm, "nlm": self._nlm,
-               "npts": npts, "ncells": ncells, "nbcells": ncells,
+               "npts": npts, "ncells": ncells, "nbcells": self.plan_size("ncells"),
                "xgrid": self.plan_array("xgrid"), "ygrid": self.plan_array("ygrid"),
                "zgrid": self.plan_array("zgrid"),
-               "gridpos": np.asfortranarray(self.plan_array("gridpos").T),
-               "gridptr": np.asfortranarray(self.plan_array("gridptr").T + 1),
-               "neighptr": np.asfortranarray(self.plan_array("neighptr").T + 1),
-               "treeptr": np.zeros((2, ncells), np.int32, order="F"),
-               "cellflags": self.plan_array("cellflags").astype(np.int16)}
+               "gridpos": np.asfortranarray(g["gridpos"].T),
+               "gridptr": np.asfortranarray(g["gridptr"].T + 1),
+               "neighptr": np.asfortranarray(g["neighptr"].T + 1),
+               "treeptr": np.asfortranarray(g["treeptr"].T + 1),
+               "cellflags": g["cellflags"].astype(np.int16)}
         if save_radiances:
             if self._state is None or self._solcrit is None:
                 raise RuntimeError("no solution to save: call solve() first")
@@ -257,6 +322,30 @@ class RTE:
                                          source.ctypes.data, source.size, radiance.ctypes.data,
                                          radiance.size, fluxes.ctypes.data, err, len(err)), err)
             out.update(fluxes=fluxes, shptr=shptr, rshptr=rshptr, source=source, radiance=radiance)
+        return out
+
+    def grid_arrays(self) -> dict:
+        """The solver's current grid, refined by adaptive cell splitting after a
+        solve: gridpos (npts, 3), gridptr (ncells, 8), neighptr (ncells, 6),
+        treeptr (ncells, 2) and cellflags (ncells), 0-based (neighptr: c an end
+        cell, -1 a boundary, -(c+2) a split cell c; treeptr: parent, first
+        child, -1 none)."""
+        if self._state is None:
+            nc = self.plan_size("ncells")
+            return {"gridpos": self.plan_array("gridpos"), "gridptr": self.plan_array("gridptr"),
+                    "neighptr": self.plan_array("neighptr"),
+                    "treeptr": np.full((nc, 2), -1, np.int32), "cellflags": self.plan_array("cellflags")}
+        npts, nc = self.state_size("npts"), self.state_size("ncells")
+        shapes = {"gridpos": ((npts, 3), np.float32), "gridptr": ((nc, 8), np.int32),
+                  "neighptr": ((nc, 6), np.int32), "treeptr": ((nc, 2), np.int32),
+                  "cellflags": ((nc,), np.int8)}
+        out = {}
+        L, err = _lib.lib(), _lib.errbuf()
+        for name, (shape, dt) in shapes.items():
+            a = np.zeros(shape, dt)
+            _lib.check(L.at3d_state_get(self._state, name.encode(), a.ctypes.data, a.nbytes, err,
+                                        len(err)), err)
+            out[name] = a
         return out
 
     def load_solution(self, input_dataset, load_radiance=True):
@@ -280,10 +369,10 @@ class RTE:
             a, b = np.asarray(ds[g], np.float32), self.plan_array(g)
             if a.shape != b.shape or np.any(a != b):
                 raise ValueError("Incompatible grids (xgrid, ygrid, zgrid) between loaded solution and self.")
-        if int(val("npts")) != self._npts or int(val("ncells")) != self.plan_size("ncells"):
+        if int(val("npts")) != self._nbpts or int(val("ncells")) != self.plan_size("ncells"):
             raise ValueError("loaded grid has {} points and {} cells, this solver's base grid {} and {}: "
-                             "grids refined by adaptive cell splitting are not supported"
-                             .format(int(val("npts")), int(val("ncells")), self._npts,
+                             "loading grids refined by adaptive cell splitting is not supported"
+                             .format(int(val("npts")), int(val("ncells")), self._nbpts,
                                      self.plan_size("ncells")))
         if not load_radiance:
             return
@@ -305,7 +394,7 @@ class RTE:
                                                              "rshptr")}
 
     def _load_restore_data(self):
-        d, npts = self._restore_data, self._npts
+        d, npts = self._restore_data, self._nbpts
         shptr = _lib.as_i32(d["shptr"][:npts + 1])
         rshptr = np.zeros(npts + 2, np.int32)
         rshptr[:npts + 1] = d["rshptr"][:npts + 1]
@@ -322,7 +411,7 @@ class RTE:
 
     # -- diagnostics on the base grid (solver.py:1037-1262) -----------------------
     def _base_grid(self, arr):
-        return arr[..., :self._npts].reshape(arr.shape[:-1] + (self.plan_size("nx1"),
+        return arr[..., :self._nbpts].reshape(arr.shape[:-1] + (self.plan_size("nx1"),
                                                                 self.plan_size("ny1"),
                                                                 self.plan_size("nz")))
 

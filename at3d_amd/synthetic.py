@@ -118,9 +118,11 @@ def make_rte(w: Workload, **overrides):
     from . import configuration, source, surface
     from .medium import Scatterer
     from .solver import RTE
+    # the workloads are defined on fixed base grids (no adaptive splitting)
     cfg = configuration.get_config(num_mu_bins=w.nmu, num_phi_bins=w.nphi,
                                    x_boundary_condition=w.bc, y_boundary_condition=w.bc,
-                                   solution_accuracy=w.solacc, deltam=w.deltam)
+                                   solution_accuracy=w.solacc, deltam=w.deltam,
+                                   split_accuracy=0.0)
     cfg.update(overrides)
     med = {f"species_{i}": Scatterer(e, a, leg, w.delx, w.dely, w.z)
            for i, (e, a, leg) in enumerate(w.species)}

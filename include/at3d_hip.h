@@ -60,10 +60,15 @@ typedef struct at3d_scene_desc {
   int32_t angle_set;       /* 1 gaussian, 2 reduced gaussian                    */
   float shacc;             /* spherical_harmonics_accuracy                     */
   float solacc;            /* solution_accuracy                                */
-  float splitacc;          /* split_accuracy; must be <= 0 (no cell splitting) */
+  float splitacc;          /* split_accuracy; > 0 refines the grid by adaptive cell splitting */
   float transmin;          /* must be 1.0 (one-cell characteristics)           */
   double tautol;           /* render sub-step optical depth                   */
   double transcut;         /* render transmission cutoff                      */
+  /* array capacities of the adaptive grid (RTE._setup_memory, solver.py:2170-2322):
+   * grid points, cells, source SH terms and discrete-ordinate work words; the
+   * splitting stops where the reference's would run out of them.  0 or less
+   * than the base grid's: the base grid's sizes (no room to split). */
+  int32_t maxig, maxic, maxiv, maxido;
 } at3d_scene_desc;
 
 typedef struct at3d_plan at3d_plan;   /* host-side setup (no GPU needed)     */
@@ -134,12 +139,23 @@ int at3d_init_solution_from(at3d_state* st, at3d_state* src, char* err, size_t e
 int at3d_solve(at3d_state* st, int32_t maxiter, int32_t verbose, float* solcrit_out,
                int32_t* iters_out, char* err, size_t errlen);
 
-/* Copy a named device array to host.  Names: "source" f32[npts][nlm],
+/* Copy a named state array to host.  Names: "source" f32[npts][nlm],
  * "radiance" f32[npts][nlm], "delsource" f32[npts][nlm], "fluxes" f32[2][npts],
  * "dirflux" f32[npts], "ns" i32[npts], "nr" i32[npts], "do_radiance" f32[nang][npts],
- * "do_source" f32[nang][npts] (max(0,S)*ext). */
+ * "do_source" f32[nang][npts] (max(0,S)*ext); and the state's current grid,
+ * which adaptive cell splitting refines during at3d_solve (the base grid
+ * before): "gridpos" f32[npts][3], "gridptr" i32[ncells][8], "neighptr"
+ * i32[ncells][6] (0-based: c an end cell, -1 the domain boundary, -(c+2) the
+ * reference's negative pointer to a split cell c), "treeptr" i32[ncells][2]
+ * (parent, first child; -1 none), "cellflags" i8[ncells] (bits 2-3: split
+ * direction), "total_ext" f32[npts], "extinct" / "albedo" f32[npart][npts]. */
 int at3d_state_get(at3d_state* st, const char* name, void* dst, size_t nbytes, char* err,
                    size_t errlen);
+
+/* Sizes of a state's current grid: keys "npts", "ncells", "nbpts", "nbcells"
+ * (the base grid's), "ntop", "nbot", "refined" (1 once cells were split).
+ * Returns -1 for an unknown key. */
+int64_t at3d_state_size(const at3d_state* st, const char* key);
 
 /* Radiances of `nrays` rays from the current solution (RENDER,
  * shdomsub4.f:93-286, with INTEGRATE_1RAY shdomsub2.f:2311-2743 and the TMS
@@ -274,7 +290,9 @@ int at3d_state_set_option(at3d_state* st, const char* key, int32_t value, char* 
 
 /* Time (ms, HIP events on the state's stream) of each kernel class over the
  * last at3d_solve call: keys "sh_to_do","sweep","do_to_sh","source","total";
- * also "iters", and "device_bytes" (the state's device allocations); of the
+ * also "iters", "splitcrit" (SPLITCRIT, the largest adaptive splitting
+ * criterion of the last splitting test) and "device_bytes" (the state's
+ * device allocations); of the
  * last render call "render" (direction source and rays) and "render_rays"
  * (the ray kernel alone); of the last gradient call "gradient" and
  * "adjoint_rays" (the adjoint ray kernel alone).

@@ -154,6 +154,10 @@ class Scene:
     accelflag: bool = True
     solacc: float = 1e-5
     splitacc: float = 0.0
+    adapt_grid_factor: float = 5.0
+    cell_to_point_ratio: float = 1.5
+    num_sh_term_factor: float = 1.0
+    max_total_mb: float = 3000.0
     high_order_radiance: bool = False
     iterfixsh: int = 30
     tautol: float = 0.1
@@ -216,14 +220,30 @@ def setup(sc: Scene) -> RefState:
           r4(0.0), r4(0.0), r4(sc.delx), r4(sc.dely), zlev, xgrid, ygrid, zgrid)
     xgrid = np.asfortranarray(xgrid[:nx1])
     ygrid = np.asfortranarray(ygrid[:ny1])
-    # _setup_memory (solver.py:2170-2322) with adapt_grid_factor = 1 (no cell splitting)
+    # _setup_memory (solver.py:2170-2322); without cell splitting the arrays
+    # are kept at the base grid's size (adapt_grid_factor = 1)
     ml = nmu - 1
     mm = max(0, int(nphi / 2) - 1)
     nlm = (2 * mm + 1) * (ml + 1) - mm * (mm + 1)
     nphi0max = nphi
-    maxig = nbpts
-    maxic = max(int(1.5 * maxig), nbcells)
-    maxiv = max(nlm * maxig, nbpts * 4)
+    maxig, maxic, maxiv = nbpts, max(int(1.5 * nbpts), nbcells), max(nlm * nbpts, nbpts * 4)
+    if sc.splitacc > 0.0:
+        import psutil
+        max_total_mb = sc.max_total_mb
+        if max_total_mb * 1024 ** 2 > 0.9 * psutil.virtual_memory().total:
+            max_total_mb = 0.9 * psutil.virtual_memory().total / 1024 ** 2
+        big_arrays = 3 if sc.accelflag else 2
+        numphase = sum(sp.legcoef.shape[1] for sp in sc.species)
+        max_legendre = max(sp.legcoef.shape[0] for sp in sc.species)   # legendre_index size
+        memword = nmu * (2 + 2 * nphi + 2 * nlm + 2 * 33 * 32) + 4.5 * maxpg + \
+            2 * numphase * max_legendre
+        wantmem = sc.adapt_grid_factor * nbpts * (28 + 16.5 * sc.cell_to_point_ratio + nphi0max +
+                                                  sc.num_sh_term_factor * nlm * big_arrays)
+        reduce = min(1.0, ((max_total_mb * 1024 ** 2) / 4 - memword) / wantmem)
+        adapt = sc.adapt_grid_factor * reduce
+        maxig = int(adapt * nbpts)
+        maxic = max(int(sc.cell_to_point_ratio * maxig), nbcells)
+        maxiv = max(int(sc.num_sh_term_factor * nlm * maxig), nbpts * 4)
     maxido = maxig * nphi0max
     maxnbc = int(maxig * 3 / nz)
     npts, ncells = i4(), i4()
@@ -480,6 +500,44 @@ def solution_iterations(st: RefState, maxiter: int, verbose=False):
              deljdot=deljdot.value, deljold=deljold.value, deljnew=deljnew.value,
              jnorm=jnorm.value, comptime=comptime.value, sfcgridrad=sfcgridrad)
     return st
+
+
+def split_grid(st: RefState, dosplit: bool, cursplitacc: float, outofmem: bool = False):
+    """One SPLIT_GRID call (shdomsub1.f:4703) on the oracle's current
+    solution, as SOLUTION_ITERATIONS makes it (shdomsub1.f:694-713, WORK and
+    WORK2 as the criterion and index scratch); the grid, optics, source and
+    radiance arrays are extended in place.  Returns (splitcrit, outofmem)."""
+    sc, d = st.sc, st.d
+    npts, ncells = i4(d["npts"]), i4(d["ncells"])
+    oom, splitcrit = lg(outofmem), r4()
+    adaptcrit = np.zeros(max(d["maxig"], d["maxic"]), np.float32)
+    adaptind = np.zeros(max(d["maxig"], d["maxic"]), np.int32)
+    ierr, errmsg = i4(), s("", 600)
+    fcall("SPLIT_GRID",
+          i4(d["maxig"]), i4(d["maxic"]), i4(d["maxiv"]), i4(d["maxido"]), i4(d["nphi0max"]),
+          lg(dosplit), oom, r4(cursplitacc), splitcrit,
+          npts, ncells, d["gridptr"], d["neighptr"], d["treeptr"], d["cellflags"],
+          d["gridpos"], i4(d["nx"]), d["xgrid"], i4(d["ny"]), d["ygrid"],
+          i4(1), i4(d["ml"]), i4(d["mm"]), i4(d["nlm"]), i4(1), i4(d["nleg"]), i4(d["numphase"]),
+          lg(sc.deltam), i4(d["bcflag"]), i4(d["ipflag"]), lg(sc.accelflag),
+          d["extinct"], d["albedo"], d["legen"], d["iphase"],
+          d["rshptr"], d["radiance"], d["shptr"], d["source"], d["oshptr"],
+          d["temp"], d["planck"], d["dirflux"], s("S"),
+          r4(sc.solarflux), r4(sc.solarmu), r4(sc.solaraz), d["ylmsun"],
+          s("R"), farr([10000, 10001], np.float32), r4(sc.wavelength), adaptcrit, adaptind,
+          i4(d["npx"]), i4(d["npy"]), i4(d["npz"]), r4(sc.delx), r4(sc.dely),
+          r4(0.0), r4(0.0), d["zlevels"], d["tempp"], d["extinctp"],
+          d["albedop"], d["legenp"], d["extdirp"], d["iphasep"], i4(0),
+          d["zckd"], d["gasabs"], r8(d["extmin"]), r8(d["scatmin"]), r8(d["cx"]), r8(d["cy"]),
+          r8(d["cz"]), r8(d["cxinv"]), r8(d["cyinv"]), r8(d["czinv"]), i4(d["di"]), i4(d["dj"]),
+          i4(d["dk"]), i4(d["ipdirect"]), r8(d["delxd"]), r8(d["delyd"]),
+          r8(d["xdomain"]), r8(d["ydomain"]), r8(d["epss"]), r8(d["epsz"]), r8(d["uniformzlev"]),
+          i4(d["npart"]), i4(d["maxpg"]), d["total_ext"], s("ON"), ierr, errmsg,
+          d["phaseinterpwt"], r4(0.999), i4(d["nlegp"]), i4(d["maxnmicro"]),
+          d["phasewtp"], i4(1), d["volsrcgridptr"], d["volsrcshptr"], d["volsrc"])
+    _check(ierr, errmsg, "SPLIT_GRID")
+    d.update(npts=npts.value, ncells=ncells.value)
+    return splitcrit.value, bool(oom.value)
 
 
 def solve(sc: Scene, maxiter: int = 100, verbose=False) -> RefState:

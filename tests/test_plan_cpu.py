@@ -107,9 +107,28 @@ def test_ordinate_counts_match_reference_files():
 def test_unsupported_options_raise():
     w = blob8("open")
     with pytest.raises(_lib.SHDOMError):
-        S.make_rte(w, split_accuracy=0.03)
-    with pytest.raises(_lib.SHDOMError):
         S.make_rte(w, transmin=0.5)
+
+
+def test_memory_parameters_follow_the_reference():
+    """MAXIG / MAXIC / MAXIV / MAXIDO as RTE._setup_memory sizes them
+    (solver.py:2244-2299): adapt_grid_factor x base points, cells 1.5x points,
+    SH terms num_sh_term_factor x NLM x MAXIG, and the max_total_mb cut."""
+    w = blob8("open")
+    rte = S.make_rte(w, split_accuracy=0.03)
+    d = rte._desc
+    nbpts, nlm = rte.plan_size("npts"), rte.plan_size("nlm")
+    assert d.maxig == 5 * nbpts and d.maxic == int(1.5 * d.maxig)
+    assert d.maxiv == nlm * d.maxig and d.maxido == d.maxig * rte.plan_size("nphi0max")
+    # a memory cap below the base grid's needs is the reference's error
+    with pytest.raises(_lib.SHDOMError, match="max_total_mb memory limit"):
+        S.make_rte(w, split_accuracy=0.03, max_total_mb=0.001)
+    # a cap that binds scales adapt_grid_factor down
+    small = S.make_rte(w, split_accuracy=0.03, max_total_mb=2.0)
+    assert nbpts <= small._desc.maxig < 5 * nbpts
+    # no splitting asked for (negative accuracy): the base grid's sizes
+    none = S.make_rte(w, split_accuracy=-1.0)
+    assert none._desc.maxig == nbpts
 
 
 def _ring_case(w):
