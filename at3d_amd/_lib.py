@@ -126,7 +126,7 @@ EXPORTS = ["at3d_version", "at3d_plan_create", "at3d_plan_destroy", "at3d_plan_s
            "at3d_render", "at3d_levis_gradient", "at3d_levis_gradient_jacobian",
            "at3d_init_solution_from", "at3d_device_trim", "at3d_state_load", "at3d_state_save",
            "at3d_state_sh_terms", "at3d_render_noalloc",
-           "at3d_render_scratch_bytes", "at3d_state_size"]
+           "at3d_render_scratch_bytes", "at3d_state_size", "at3d_state_set_grid"]
 
 
 def as_f32(a):

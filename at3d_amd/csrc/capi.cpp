@@ -434,6 +434,25 @@ int at3d_state_load(at3d_state* st, const int32_t* shptr, const float* source,
   return rc;
 }
 
+int at3d_state_set_grid(at3d_state* st, int32_t npts, int32_t ncells, const float* gridpos,
+                        const int32_t* gridptr, const int32_t* neighptr, const int32_t* treeptr,
+                        const int8_t* cellflags, char* err, size_t errlen) {
+  if (!st || !gridpos || !gridptr || !neighptr || !treeptr || !cellflags || npts <= 0 || ncells <= 0) {
+    set_err(err, errlen, "null argument");
+    return AT3D_ERR_ARG;
+  }
+  std::string msg;
+  int rc;
+  try {
+    rc = at3d::state_set_grid(st->s, npts, ncells, gridpos, gridptr, neighptr, treeptr, cellflags, msg);
+  } catch (const std::bad_alloc&) {
+    rc = AT3D_ERR_MEMORY;
+    msg = "out of host memory setting a grid";
+  }
+  if (rc) set_err(err, errlen, msg);
+  return rc;
+}
+
 int at3d_state_save(at3d_state* st, int32_t* shptr, int32_t* rshptr, float* source,
                     int64_t source_cap, float* radiance, int64_t radiance_cap, float* fluxes,
                     char* err, size_t errlen) {

@@ -1529,6 +1529,49 @@ int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err) {
   return AT3D_OK;
 }
 
+int plan_set_grid(Plan& p, int npts, int ncells, const float* gridpos, const int32_t* gridptr,
+                  const int32_t* neighptr, const int32_t* treeptr, const int8_t* cellflags, std::string& err) {
+  // the grid must be a tree of end cells over the base cells whose pointers
+  // stay in range, and descents must end (children come after their parent)
+  if (npts < p.nbpts || ncells < p.nbcells || npts > p.maxig || ncells > p.maxic) {
+    err = "Cannot load solution as loaded grid has more points or cells than supported by the current "
+          "RTE's memory parameters (or fewer than its base grid)";
+    return AT3D_ERR_ARG;
+  }
+  for (size_t i = 0; i < (size_t)ncells * 8; i++)
+    if (gridptr[i] < 0 || gridptr[i] >= npts) {
+      err = "loaded grid: corner pointer out of range";
+      return AT3D_ERR_ARG;
+    }
+  for (size_t i = 0; i < (size_t)ncells * 6; i++)
+    if (nb_cell(neighptr[i]) >= ncells || neighptr[i] < -ncells - 1) {
+      err = "loaded grid: neighbour pointer out of range";
+      return AT3D_ERR_ARG;
+    }
+  for (int c = 0; c < ncells; c++) {
+    const int par = treeptr[(size_t)c * 2], ch = treeptr[(size_t)c * 2 + 1];
+    const int dir = (cellflags[c] >> 2) & 3;
+    const bool bad_parent = par >= c || par < -1 || (c < p.nbcells) != (par < 0);
+    const bool bad_child = ch >= 0 && (ch <= c || ch + 1 >= ncells || dir < 1 || treeptr[(size_t)ch * 2] != c ||
+                                       treeptr[(size_t)(ch + 1) * 2] != c);
+    if (bad_parent || bad_child || ch < -1) {
+      err = "loaded grid: inconsistent cell tree";
+      return AT3D_ERR_ARG;
+    }
+  }
+  p.npts = npts;
+  p.ncells = ncells;
+  p.gridpos.assign(gridpos, gridpos + (size_t)npts * 3);
+  p.gridptr.assign(gridptr, gridptr + (size_t)ncells * 8);
+  p.neighptr.assign(neighptr, neighptr + (size_t)ncells * 6);
+  p.treeptr.assign(treeptr, treeptr + (size_t)ncells * 2);
+  p.cellflags.assign(cellflags, cellflags + ncells);
+  // the optical properties of every point from the property grid
+  // (_prepare_optical_properties after load_solution, solver.py:1595)
+  interp_optics(p);
+  return plan_refresh_points(p, err);
+}
+
 int plan_refresh_points(Plan& p, std::string& err) {
   source_constants(p);
   // the two-stream start is a base-grid quantity; new points start from zero

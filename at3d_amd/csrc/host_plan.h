@@ -119,6 +119,13 @@ int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err);
 // the new grid.
 int plan_refresh_points(Plan& p, std::string& err);
 
+// A refined grid given by the caller (a saved solution's): checked for range
+// and tree consistency, then the optics of every point from the property
+// grid and the derived plan data as plan_refresh_points.  p is a copy of the
+// base plan; pointers are 0-based as in GridTopo.
+int plan_set_grid(Plan& p, int npts, int ncells, const float* gridpos, const int32_t* gridptr,
+                  const int32_t* neighptr, const int32_t* treeptr, const int8_t* cellflags, std::string& err);
+
 // TRILIN_INTERP_PROP for species ipa at one point, before delta-M scaling:
 // extinction, albedo and the eight (table index, weight) pairs.
 void point_optics(const Plan& p, int ipa, float X, float Y, float Z, float& ext, float& alb, int32_t* iph,

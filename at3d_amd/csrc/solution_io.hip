@@ -130,11 +130,6 @@ static int check_ptr(const int32_t* ptr, int npts, int nmax, int nmin, const cha
 // starts from the radiance truncation of the loaded radiance.
 int state_load(State& s, const int32_t* shptr, const float* source, const int32_t* rshptr,
                const float* radiance, const float* fluxes, std::string& err) {
-  // a saved solution is on the base grid: a state refined by splitting goes back to it
-  {
-    const int rc = state_reset_grid(s, err);
-    if (rc) return rc;
-  }
   const Plan& p = *s.plan;
   const int npts = p.npts;
   {

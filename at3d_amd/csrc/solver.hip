@@ -664,6 +664,27 @@ int state_reset_grid(State& s, std::string& err) {
   return state_replace(s, nullptr, nullptr, err);
 }
 
+int state_set_grid(State& s, int npts, int ncells, const float* gridpos, const int32_t* gridptr,
+                   const int32_t* neighptr, const int32_t* treeptr, const int8_t* cellflags, std::string& err) {
+  const Plan& b = *s.base_plan;
+  bool base = npts == b.npts && ncells == b.ncells;
+  for (size_t i = 0; base && i < (size_t)ncells * 2; i++) base = treeptr[i] == -1;
+  if (base) {
+    // the base grid itself (a saved solution of an unrefined solve)
+    base = memcmp(gridpos, b.gridpos.data(), (size_t)npts * 3 * sizeof(float)) == 0 &&
+           memcmp(gridptr, b.gridptr.data(), (size_t)ncells * 8 * sizeof(int32_t)) == 0 &&
+           memcmp(neighptr, b.neighptr.data(), (size_t)ncells * 6 * sizeof(int32_t)) == 0;
+    if (!base) {
+      err = "loaded grid: base grid differs from this solver's";
+      return AT3D_ERR_ARG;
+    }
+    return state_reset_grid(s, err);
+  }
+  auto q = std::make_shared<Plan>(b);
+  TRY(plan_set_grid(*q, npts, ncells, gridpos, gridptr, neighptr, treeptr, cellflags, err));
+  return state_replace(s, q, nullptr, err);
+}
+
 static SourceArgs source_args(State& s, int first, int fixsh) {
   const Plan& p = *s.plan;
   SourceArgs a;

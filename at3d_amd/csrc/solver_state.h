@@ -182,6 +182,10 @@ int state_create(const Plan& p, State& s, std::string& err);
 // Moves a state on a refined grid back to its base grid (no solution: the
 // next call initialises or loads one).
 int state_reset_grid(State& s, std::string& err);
+// Moves a state to the grid of a saved solution (the base grid or a refined
+// one); no solution until the next load or initialisation.
+int state_set_grid(State& s, int npts, int ncells, const float* gridpos, const int32_t* gridptr,
+                   const int32_t* neighptr, const int32_t* treeptr, const int8_t* cellflags, std::string& err);
 size_t plan_footprint_bytes(const Plan& p);
 void state_free(State& s);
 int state_init_solution(State& s, std::string& err);

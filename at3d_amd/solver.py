@@ -369,11 +369,20 @@ class RTE:
             a, b = np.asarray(ds[g], np.float32), self.plan_array(g)
             if a.shape != b.shape or np.any(a != b):
                 raise ValueError("Incompatible grids (xgrid, ygrid, zgrid) between loaded solution and self.")
-        if int(val("npts")) != self._nbpts or int(val("ncells")) != self.plan_size("ncells"):
-            raise ValueError("loaded grid has {} points and {} cells, this solver's base grid {} and {}: "
-                             "loading grids refined by adaptive cell splitting is not supported"
-                             .format(int(val("npts")), int(val("ncells")), self._nbpts,
-                                     self.plan_size("ncells")))
+        # the grid (possibly refined by adaptive splitting), 0-based; it becomes
+        # the solver's grid when the solution is loaded (solver.py:1567-1595)
+        npts, ncells = int(val("npts")), int(val("ncells"))
+        if npts > self._desc.maxig or ncells > max(self._desc.maxic, self.plan_size("ncells")):
+            raise _lib.SHDOMError(
+                1, "Cannot load solution as loaded grid has more points ({}) or cells ({}) than "
+                   "supported by the current RTE's memory parameters ({}, {})".format(
+                       npts, ncells, self._desc.maxig, self._desc.maxic))
+        grid = {"npts": npts, "ncells": ncells,
+                "gridpos": np.ascontiguousarray(np.asarray(ds["gridpos"], np.float32)[:, :npts].T),
+                "gridptr": np.ascontiguousarray(np.asarray(ds["gridptr"], np.int32)[:, :ncells].T - 1),
+                "neighptr": np.ascontiguousarray(np.asarray(ds["neighptr"], np.int32)[:, :ncells].T - 1),
+                "treeptr": np.ascontiguousarray(np.asarray(ds["treeptr"], np.int32)[:, :ncells].T - 1),
+                "cellflags": np.ascontiguousarray(np.asarray(ds["cellflags"])[:ncells].astype(np.int8))}
         if not load_radiance:
             return
         if int(val("nstokes")) != self._nstokes:
@@ -392,9 +401,17 @@ class RTE:
                                "which is required.".format(name))
         self._restore_data = {k: np.asarray(ds[k]) for k in ("radiance", "source", "fluxes", "shptr",
                                                              "rshptr")}
+        self._restore_data["grid"] = grid
 
     def _load_restore_data(self):
-        d, npts = self._restore_data, self._nbpts
+        d, g = self._restore_data, self._restore_data["grid"]
+        npts = g["npts"]
+        err = _lib.errbuf()
+        _lib.check(_lib.lib().at3d_state_set_grid(
+            self._state, npts, g["ncells"], g["gridpos"].ctypes.data, g["gridptr"].ctypes.data,
+            g["neighptr"].ctypes.data, g["treeptr"].ctypes.data, g["cellflags"].ctypes.data, err,
+            len(err)), err)
+        self._npts = npts
         shptr = _lib.as_i32(d["shptr"][:npts + 1])
         rshptr = np.zeros(npts + 2, np.int32)
         rshptr[:npts + 1] = d["rshptr"][:npts + 1]

@@ -168,12 +168,26 @@ int at3d_render(at3d_state* st, int64_t nrays, const float* x, const float* y, c
                 const double* mu, const double* phi, int32_t single_scatter, int32_t nosurface,
                 float* out, char* err, size_t errlen);
 
+/* The grid of a saved solution, 0-based as at3d_state_get gives it (npts,
+ * ncells, gridpos [npts][3], gridptr [ncells][8], neighptr [ncells][6],
+ * treeptr [ncells][2], cellflags [ncells]): the base grid, or one refined by
+ * adaptive cell splitting, whose optical properties then come from the
+ * property grid at every point as after the reference's load_solution
+ * (solver.py:1552-1605, _prepare_optical_properties).  Checked for range and
+ * tree consistency; more points or cells than the memory parameters allow is
+ * the reference's "Cannot load solution" error.  The state holds no
+ * solution afterwards: at3d_state_load (or an initialisation) follows. */
+int at3d_state_set_grid(at3d_state* st, int32_t npts, int32_t ncells, const float* gridpos,
+                        const int32_t* gridptr, const int32_t* neighptr, const int32_t* treeptr,
+                        const int8_t* cellflags, char* err, size_t errlen);
+
 /* Solution transfer in the reference's saved-solution format
  * (RTE.save_solution / RTE.load_solution, at3d/solver.py:1519-1686; SAVE_STATE /
  * RESTORE_STATE of SHDOM): ragged spherical-harmonic rows, 0-based offsets.
  *   shptr[npts+1], source[shptr[npts]]      SOURCE terms of each point
  *   rshptr[npts+2], radiance[rshptr[npts]]  RADIANCE terms (rshptr[npts+1] = rshptr[npts])
  *   fluxes[2][npts]                         hemispherical fluxes (down, up)
+ * on the state's current grid (set a saved grid first with at3d_state_set_grid).
  * at3d_state_load makes them the state's solution as INIT_SOLUTION does with
  * INRADFLAG false (shdomsub1.f:356, reached from RTE._init_solution after
  * load_solution, solver.py:2654-2664): no two-stream start and no first

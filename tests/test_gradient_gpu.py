@@ -32,8 +32,8 @@ def _ref_derivs(ref, w, unknowns):
     return out
 
 
-def _run(ref, w, unknowns, views, cost="L2", ess=True, scale=1.1):
-    st = ref.setup(ref.scene_from_workload(w))
+def _run(ref, w, unknowns, views, cost="L2", ess=True, scale=1.1, splitacc=0.0):
+    st = ref.setup(ref.scene_from_workload(w, splitacc=splitacc))
     ref.init_solution(st)
     ref.solution_iterations(st, 100)
     ref.prepare_derivatives(st, _ref_derivs(ref, w, unknowns))
@@ -45,8 +45,9 @@ def _run(ref, w, unknowns, views, cost="L2", ess=True, scale=1.1):
     g_ref, c_ref, px_ref = ref.levis_gradient(st, rays["ray_x"], rays["ray_y"], rays["ray_z"],
                                               rays["ray_mu"], rays["ray_phi"], meas, costfunc=cost,
                                               exact_single_scatter=ess)
-    rte = S.make_rte(w)
+    rte = S.make_rte(w, split_accuracy=splitacc)
     rte.solve(100, verbose=False)
+    assert rte.state_size("npts") == st.npts
     sensor = dict(rays, measurement_data=meas)
     names = list(rte.medium)
     g, c, out = GR.levis_approximation_grad(rte, sensor, [(names[i], v) for i, v in unknowns],
@@ -70,6 +71,14 @@ VIEWS = [(0.0, 0.0), (45.0, 30.0), (60.0, 200.0)]
 @pytest.mark.parametrize("bc", ["open", "periodic"])
 def test_extinction_and_albedo_gradient_matches_reference(ref, bc):
     _check(_run(ref, blob8(bc), [(0, "extinction"), (0, "ssalb")], VIEWS))
+
+
+@pytest.mark.parametrize("bc", ["open", "periodic"])
+def test_gradient_on_refined_grid_matches_reference(ref, bc):
+    """Adaptive cell splitting (split_accuracy 0.03) grows the grid 4-5x; the
+    adjoint rays then cross split cells (tree descents, faces onto smaller
+    cells) and the sun paths start from the new points."""
+    _check(_run(ref, blob8(bc), [(0, "extinction"), (0, "ssalb")], VIEWS, splitacc=0.03))
 
 
 def test_two_species_gradient_matches_reference(ref):

@@ -86,3 +86,26 @@ def test_split_saved_grid_and_reinitialisation(ref):
     rte.solve(100, verbose=False)
     assert rte.state_size("npts") == n
     assert np.array_equal(rte.state_array("source"), first)
+
+
+def test_refined_solution_saves_and_loads(ref):
+    """A saved solution on a refined grid loads into a new solver (grid, then
+    source and radiance, as the reference's load_solution); its pixels equal
+    the saving solver's.  A solver whose memory parameters cannot hold the
+    refined grid refuses it with the reference's error."""
+    from at3d_amd._lib import SHDOMError
+    w, st, rte = _pair(ref, "cloud12_two_species")
+    ds = rte.save_solution()
+    rays = ortho_rays(w, [(0.0, 0.0), (45.0, 30.0)])
+    want = rte.integrate_to_sensor(dict(rays))["I"]
+    other = S.make_rte(w, split_accuracy=0.01)
+    other.load_solution(ds)
+    other.solve(0)                  # the initialisation takes the loaded solution
+    assert other.state_size("npts") == st.npts and other.state_size("refined") == 1
+    for k, v in rte.grid_arrays().items():
+        assert np.array_equal(other.grid_arrays()[k], v), k
+    got = other.integrate_to_sensor(dict(rays))["I"]
+    assert rel_l2(got, want) < 1e-6, rel_l2(got, want)
+    small = S.make_rte(w, split_accuracy=-1.0)      # no room beyond the base grid
+    with pytest.raises(SHDOMError, match="Cannot load solution"):
+        small.load_solution(ds)
