@@ -106,6 +106,10 @@ def lib():
         L.at3d_state_timing.restype = C.c_double
         L.at3d_state_size.argtypes = [vp, C.c_char_p]
         L.at3d_state_size.restype = C.c_int64
+        L.at3d_state_set_grid.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, C.c_char_p,
+                                          C.c_size_t]
+        L.at3d_version.argtypes = []
+        L.at3d_device_trim.argtypes = []
         _lib = L
     return _lib
 

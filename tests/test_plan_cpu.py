@@ -17,7 +17,12 @@ def test_library_exports_every_declared_symbol():
     for name in _lib.EXPORTS:
         assert name + "(" in hdr, name
         assert hasattr(L, name), name
+        # without argument types ctypes passes 64-bit pointers as 32-bit ints
+        assert getattr(L, name).argtypes is not None, name
     assert b"gfx950" in L.at3d_version()
+    import re
+    declared = set(re.findall(r"\b(at3d_\w+)\(", hdr)) - {"at3d_scene_desc", "at3d_deriv_desc"}
+    assert declared <= set(_lib.EXPORTS), declared - set(_lib.EXPORTS)
 
 
 def _pair(ref, w):
