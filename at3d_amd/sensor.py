@@ -204,26 +204,29 @@ def perspective_projection(wavelength, fov, x_resolution, y_resolution, position
     nx, ny = int(x_resolution), int(y_resolution)
     if nx != x_resolution or ny != y_resolution or nx < 1 or ny < 1:
         raise ValueError("x_resolution and y_resolution are pixel counts (integers >= 1)")
+    # the reference's dtypes are kept (positions and the viewing axis in
+    # single precision, the camera's intrinsic matrix inverted in single
+    # precision) so that the ray directions are the same to the last bit
     pos = np.asarray(position_vector, np.float32)
     look = np.asarray(lookat_vector, np.float32)
-    up = np.asarray(up_vector, np.float64)
-    wax = (look - pos).astype(np.float64)
-    wax = wax / np.linalg.norm(wax)
-    uax = np.cross(up, wax)
-    uax = uax / np.linalg.norm(uax)
+    up = np.asarray(up_vector)
+    unit = lambda v: v / np.linalg.norm(v, axis=0)
+    wax = unit(look - pos)
+    uax = unit(np.cross(up, wax))
     vax = np.cross(wax, uax)
+    rot = np.stack((uax, vax, wax), axis=1)
     big = max(nx, ny)
     half_x, half_y = nx / big, ny / big
     cell_x, cell_y = 2.0 * half_x / nx, 2.0 * half_y / ny
     focal = 1.0 / np.tan(np.deg2rad(fov) / 2.0)
+    inv_k = np.linalg.inv(np.array([[focal, 0, 0], [0, focal, 0], [0, 0, 1]], dtype=np.float32))
     ax = np.linspace(-half_x + cell_x / 2, half_x - cell_x / 2, nx)
     by = np.linspace(-half_y + cell_y / 2, half_y - cell_y / 2, ny)
     a = np.tile(ax, ny)
     b = np.repeat(by, nx)
 
     def directions(a, b):
-        d = np.outer(uax, a / focal) + np.outer(vax, b / focal) + wax[:, None]
-        d = d / np.linalg.norm(d, axis=0)
+        d = unit(rot @ (inv_k @ np.stack([a, b, np.ones_like(a)])))
         return -d[2].astype(np.float64), (np.arctan2(d[1], d[0]) + np.pi).astype(np.float64)
 
     mu, phi = directions(a, b)

@@ -69,6 +69,17 @@ def ortho_rays(w, views):
                                                  (xs, ys, zs, mus, phis))}
 
 
+def perspective_rays(w, nx=8, ny=6, fov=40.0, height=0.0, sub=None):
+    """A pinhole camera on the domain's top surface (plus `height`), above its
+    centre and offset in x, looking at the domain's centre."""
+    from at3d_amd.sensor import perspective_projection
+    x0, y0, z0, x1, y1, z1 = bbox(w)
+    pos = (x0 + 0.3 * (x1 - x0), 0.5 * (y0 + y1), z1 + height)
+    look = (0.5 * (x0 + x1), 0.5 * (y0 + y1), 0.5 * (z0 + z1))
+    s = perspective_projection(0.86, fov, nx, ny, pos, look, (0.0, 1.0, 0.0), sub_pixel_ray_args=sub)
+    return {k: np.asarray(s[k]) for k in ("ray_x", "ray_y", "ray_z", "ray_mu", "ray_phi")}, s
+
+
 def interior_rays(w, n, seed):
     """Rays that start inside the domain with arbitrary directions (up and down)."""
     rng = np.random.default_rng(seed)

@@ -57,6 +57,20 @@ def test_interior_rays_all_directions_match_reference(ref, name):
     _compare(ref, st, rte, interior_rays(w, 150, seed=3))
 
 
+@pytest.mark.parametrize("name", ["blob8_open", "cloud12_two_species"])
+def test_perspective_camera_with_subpixel_rays_matches_reference(ref, name):
+    """A pinhole camera on the domain top (perspective_projection) with 2 x 2
+    Gauss-Legendre sub-pixel rays: every ray against the reference's RENDER,
+    then the pixel averages (AVERAGE_SUBPIXEL_RAYS) of both."""
+    from at3d_amd.sensor import average_subpixel_rays, gaussian
+    from tests.scenes import perspective_rays
+    w = SCENES[name]()
+    st, rte = _pair(ref, w, 2)
+    rays, sens = perspective_rays(w, sub={"method": gaussian, "degree": (2, 2)})
+    got, want = _compare(ref, st, rte, rays)
+    assert rel_l2(average_subpixel_rays(sens, got), average_subpixel_rays(sens, want)) < 1e-4
+
+
 @pytest.mark.parametrize("flags", [(True, False), (False, True)], ids=["single_scatter", "nosurface"])
 @pytest.mark.parametrize("make", [cloud_small, cloud_l15], ids=["cloud12", "cloud16_l15"])
 def test_render_flags_match_reference(ref, flags, make):
