@@ -184,6 +184,41 @@ def serving_stream_bench(n_requests, rate_hz, workers):
             "same_iterations_and_solcrit_as_one_at_a_time": same}
 
 
+def split_leg(cpu_runs):
+    """Adaptive cell splitting (split_accuracy 0.03, the reference's default)
+    on a 32 x 32 x 20 open cloud at L = 15: one solve of a new medium timed
+    whole on the GPU (plan, state, iterations, every refinement of the grid),
+    and, when the CPU baseline runs, the reference on one host core."""
+    from at3d_amd import synthetic as S
+    import torch
+    w = S.cloud_field(nx=32, ny=32, nz=20, dx=0.04, dz=0.05, z0=0.3, seed=3, ext_scale=20.0,
+                      solacc=1e-4)
+    S.make_rte(w, split_accuracy=0.03).solve(w.maxiter, verbose=False)    # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rte = S.make_rte(w, split_accuracy=0.03)
+    rte.solve(w.maxiter, verbose=False)
+    torch.cuda.synchronize()
+    out = {"workload": "32x32x20 open cloud, NMU16 NPHI32 L15, split_accuracy 0.03, solacc 1e-4",
+           "base_points": rte.plan_size("npts"), "points": rte.state_size("npts"),
+           "iterations": rte.num_iterations, "gpu_s": time.perf_counter() - t0}
+    rte.release()
+    if cpu_runs > 0:
+        try:
+            from oracle import ref_shdom as ref
+        except Exception:
+            return out
+        if ref.available():
+            t0 = time.perf_counter()
+            st = ref.setup(ref.scene_from_workload(w, splitacc=0.03))
+            ref.init_solution(st)
+            ref.solution_iterations(st, w.maxiter)
+            out.update(cpu_reference_s=time.perf_counter() - t0, reference_points=st.npts,
+                       reference_iterations=st.iters)
+            out["speedup_vs_cpu"] = out["cpu_reference_s"] / out["gpu_s"]
+    return out
+
+
 def cold_solves(n, maxiter):
     """`n` solves of new media on the headline geometry (the cloud field
     re-drawn with seeds 1..n): host plan of the new medium (the sweep levels
@@ -460,6 +495,9 @@ def main():
     grad = gradient_leg(gw, grays, gmeas, args.gradient_reps)
     print("bench: gradient done", file=sys.stderr, flush=True)
 
+    split = split_leg(args.cpu_runs)
+    print("bench: adaptive splitting done", file=sys.stderr, flush=True)
+
     stream = None
     if args.stream_requests > 0:
         stream = serving_stream_bench(args.stream_requests, args.stream_rate, args.stream_workers)
@@ -502,6 +540,7 @@ def main():
         "solve_roofline": solve_roof,
         "render": render,
         "gradient": grad,
+        "adaptive_splitting": split,
         "serving_stream": stream,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": {"cold_solve": (1e3 / cold_ms / cpu["value"]) if (cpu and cold_ms) else None,
