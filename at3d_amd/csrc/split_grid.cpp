@@ -15,6 +15,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <array>
 #include <stdexcept>
 
 #pragma clang fp contract(off)
@@ -512,7 +513,6 @@ struct Splitter {
   int64_t sum_ns = 0, sum_nr = 0;
   int old_npts = 0;
   std::vector<PointOptics> added;   // optics of the points appended in this call
-  std::vector<float> legent;
 
   Splitter(Plan& p_, SplitFields& f_) : p(p_), f(f_) {
     for (int i = 0; i < p.npts; i++) {
@@ -522,25 +522,20 @@ struct Splitter {
     old_npts = p.npts;
   }
 
-  // Medium, direct beam, radiance and source of the new points
-  // (INTERPOLATE_POINT, shdomsub1.f:4934-5283).
+  // New points of the batch whose medium, direct beam and source are still
+  // to be computed: (point, first parent, second parent).
+  std::vector<std::array<int, 3>> pending;
+
+  // INTERPOLATE_POINT (shdomsub1.f:4934-5283) for the new points of one
+  // division: the radiance (mean of the parents' expansions) and the term
+  // counts at once, since the smoothing test and the capacity bounds read
+  // them; the medium, direct beam and source wait for finish_points (they
+  // depend only on the point and its parents).
   void interpolate_points(const NewPoint np[4]) {
     for (int i = 0; i < 4; i++) {
       if (np[i].ip < 0) continue;
       const int ip1 = np[i].p1, ip2 = np[i].p2, ip = np[i].ip;
-      const float x = pos(p, ip, 1), y = pos(p, ip, 2), z = pos(p, ip, 3);
-      PointOptics o;
-      new_point_optics(p, x, y, z, o);
-      p.total_ext.push_back(o.total_ext);
-      if (!beam_ready) {
-        direct_beam_setup(p, extdirp, beam);
-        beam_ready = true;
-      }
-      float dirflux = 0.f;
-      if (!direct_beam_point(beam, x, y, z, dirflux, nullptr))
-        throw SplitError("DIRECT_BEAM_PROP: ray left the property grid");
       const int ld = f.ld;
-      // radiance: the mean of the parents' expansions
       const int nr1 = f.nr[ip1], nr2 = f.nr[ip2], nr = std::max(nr1, nr2);
       f.R.resize((size_t)(ip + 1) * ld, 0.f);
       float* R = &f.R[(size_t)ip * ld];
@@ -550,14 +545,47 @@ struct Splitter {
         R[j] = 0.5f * (r1 + r2);
       }
       f.nr.push_back(nr);
-      // source from that radiance and the direct beam (CALC_SOURCE_PNT_UNPOL,
-      // shdomsub1.f:913-958), kept to the parents' larger truncation
       const int ns = std::max(f.ns[ip1], f.ns[ip2]);
+      f.S.resize((size_t)(ip + 1) * ld, 0.f);
+      f.DS.resize((size_t)(ip + 1) * ld, 0.f);
+      f.ns.push_back(ns);
+      f.nos.push_back(0);
+      p.total_ext.push_back(0.0f);
+      added.emplace_back();
+      sum_ns += ns;
+      sum_nr += nr;
+      pending.push_back({ip, ip1, ip2});
+    }
+  }
+
+  // The medium from the property grid, the direct beam and the source
+  // (CALC_SOURCE_PNT_UNPOL, shdomsub1.f:913-958, kept to the parents' larger
+  // truncation) of the pending points, on the host threads.
+  void finish_points() {
+    if (pending.empty()) return;
+    if (!beam_ready) {
+      direct_beam_setup(p, extdirp, beam);
+      beam_ready = true;
+    }
+    const int n = (int)pending.size();
+    std::vector<char> bad(n, 0);
+    parallel_for(n, [&](int k) {
+      const int ip = pending[k][0];
+      const float x = pos(p, ip, 1), y = pos(p, ip, 2), z = pos(p, ip, 3);
+      PointOptics& o = added[ip - old_npts];
+      new_point_optics(p, x, y, z, o);
+      p.total_ext[ip] = o.total_ext;
+      float dirflux = 0.f;
+      if (!direct_beam_point(beam, x, y, z, dirflux, nullptr)) {
+        bad[k] = 1;
+        return;
+      }
+      std::vector<float> legent;
       float albedo;
       new_point_phase(p, o, legent, albedo);
       const float flux0 = dirflux * (1.0f / fabsf(p.d.solarmu));
-      f.S.resize((size_t)(ip + 1) * ld, 0.f);
-      f.DS.resize((size_t)(ip + 1) * ld, 0.f);
+      const int ld = f.ld, ns = f.ns[ip], nr = f.nr[ip];
+      const float* R = &f.R[(size_t)ip * ld];
       float* S = &f.S[(size_t)ip * ld];
       for (int j = 0; j < ns; j++) {
         const int l = p.lofj[j];
@@ -565,12 +593,10 @@ struct Splitter {
         if (j < nr) s = s + albedo * legent[l] * R[j];
         S[j] = s;
       }
-      f.ns.push_back(ns);
-      f.nos.push_back(0);
-      sum_ns += ns;
-      sum_nr += nr;
-      added.push_back(std::move(o));
-    }
+    });
+    pending.clear();
+    for (char b : bad)
+      if (b) throw SplitError("DIRECT_BEAM_PROP: ray left the property grid");
   }
 
   bool over(int maxcells, int maxpts, int maxwork, int maxsh, int maxrad) const {
@@ -621,13 +647,23 @@ int split_grid(Plan& p, SplitFields& f, bool dosplit, bool& outofmem, float curs
       while (icell1 < p.ncells) {
         crit.clear();
         ind.clear();
-        for (int ic = icell1; ic < p.ncells; ic++) {
-          if (p.treeptr[(size_t)ic * 2 + 1] >= 0) continue;
-          float a;
-          int idir;
-          cell_split_test(p, f, ic, a, idir);
-          crit.push_back(a);
-          ind.push_back(4 * (int64_t)ic + idir);
+        {
+          // the end cells' criteria on the host threads, kept in cell order
+          const int nc = p.ncells - icell1;
+          std::vector<float> ca(nc);
+          std::vector<int> cd(nc, -1);
+          parallel_for((nc + 255) / 256, [&](int b) {
+            for (int k = b * 256; k < std::min(nc, b * 256 + 256); k++) {
+              const int ic = icell1 + k;
+              if (p.treeptr[(size_t)ic * 2 + 1] >= 0) continue;
+              cell_split_test(p, f, ic, ca[k], cd[k]);
+            }
+          });
+          for (int k = 0; k < nc; k++)
+            if (cd[k] >= 0) {
+              crit.push_back(ca[k]);
+              ind.push_back(4 * (int64_t)(icell1 + k) + cd[k]);
+            }
         }
         icell1 = p.ncells;
         const int n = (int)crit.size();
@@ -669,6 +705,7 @@ int split_grid(Plan& p, SplitFields& f, bool dosplit, bool& outofmem, float curs
           }
           i++;
         }
+        sp.finish_points();   // the next batch's criteria read the new points' source
       }
       if (outofmem0) outofmem = true;
     }
