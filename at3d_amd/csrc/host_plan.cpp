@@ -388,76 +388,46 @@ void make_sh_tables(Plan& p) {
 // phase-table entries of the eight property-grid corners with their weights
 // (merged by table index and sorted by weight, 'N' interpolation), before
 // delta-M scaling.
+PropGrid plan_prop_grid(const Plan& p) {
+  const at3d_scene_desc& d = p.d;
+  return PropGrid{p.extinctp.data(), p.albedop.data(), p.iphasep.data(), p.phasewtp.data(), p.zlevels.data(),
+                  d.npx, d.npy, d.npz, d.delx, d.dely, p.extmin, p.scatmin};
+}
+
+LegTable plan_leg_table(const Plan& p) {
+  return LegTable{p.legen.data(), p.nleg + 1, p.ml, p.legen.size()};
+}
+
+float sky_flux_weight(const Plan& p) {
+  float skyradalb = 0.f;
+  for (int imu = 0; imu < p.nmu / 2; imu++)
+    for (int ip = 0; ip < p.nphi0[imu]; ip++)
+      skyradalb = skyradalb + fabsf(p.mu[imu]) * p.wtdo[(size_t)imu * p.nphi0max + ip] * p.d.skyrad;
+  return skyradalb;
+}
+
 void point_optics(const Plan& p, int ipa, float X, float Y, float Z, float& extf, float& albf, int32_t* iph_out,
                   float* wt_out) {
-  const at3d_scene_desc& d = p.d;
-  const int npx = d.npx, npy = d.npy, npz = d.npz, npg = npx * npy * npz;
-  const float* E = &p.extinctp[(size_t)ipa * npg];
-  const float* A = &p.albedop[(size_t)ipa * npg];
-  const float* PW = &p.phasewtp[(size_t)ipa * npg];
-  const int32_t* IP = &p.iphasep[(size_t)ipa * npg];
-  int il = 0, iu = npz;
-  while (iu - il > 1) {
-    int im = (iu + il) / 2;
-    if (Z >= p.zlevels[im - 1]) il = im; else iu = im;
-  }
-  const int iz = std::max(il, 1);
-  double w = (double)(Z - p.zlevels[iz - 1]) / (double)(p.zlevels[iz] - p.zlevels[iz - 1]);
-  w = std::max(std::min(w, 1.0), 0.0);
-  int ix = (int)((X - 0.0f) / d.delx) + 1;
-  if (fabsf(X - 0.0f - (float)npx * d.delx) < 0.01f * d.delx) ix = npx;
-  int ixp = ix % npx + 1;
-  double u = (double)(X - 0.0f - d.delx * (float)(ix - 1)) / (double)d.delx;
-  u = std::max(std::min(u, 1.0), 0.0);
-  if (u < 1.0e-5) u = 0.0;
-  if (u > 1.0 - 1.0e-5) u = 1.0;
-  int iy = (int)((Y - 0.0f) / d.dely) + 1;
-  if (fabsf(Y - 0.0f - (float)npy * d.dely) < 0.01f * d.dely) iy = npy;
-  int iyp = iy % npy + 1;
-  double v = (double)(Y - 0.0f - d.dely * (float)(iy - 1)) / (double)d.dely;
-  v = std::max(std::min(v, 1.0), 0.0);
-  if (v < 1.0e-5) v = 0.0;
-  if (v > 1.0 - 1.0e-5) v = 1.0;
-  const double F[8] = {(1 - u) * (1 - v) * (1 - w), u * (1 - v) * (1 - w), (1 - u) * v * (1 - w),
-                       u * v * (1 - w),             (1 - u) * (1 - v) * w, u * (1 - v) * w,
-                       (1 - u) * v * w,             u * v * w};
-  int I[8];
-  I[0] = iz + npz * (iy - 1) + npz * npy * (ix - 1) - 1;
-  I[1] = iz + npz * (iy - 1) + npz * npy * (ixp - 1) - 1;
-  I[2] = iz + npz * (iyp - 1) + npz * npy * (ix - 1) - 1;
-  I[3] = iz + npz * (iyp - 1) + npz * npy * (ixp - 1) - 1;
-  for (int q = 0; q < 4; q++) I[q + 4] = I[q] + 1;
-  double ext = 0.0, scat[8], scatter = 0.0;
-  for (int q = 0; q < 8; q++) ext += F[q] * (double)E[I[q]];
-  for (int q = 0; q < 8; q++) {
-    scat[q] = F[q] * (double)E[I[q]] * (double)A[I[q]];
-    scatter += scat[q];
-  }
-  extf = (float)ext;
-  albf = ((double)extf > p.extmin) ? (float)(scatter / (double)extf) : (float)(scatter / p.extmin);
-  int iph[8];
-  float wt[8];
-  for (int q = 0; q < 8; q++) {
-    iph[q] = IP[I[q]];
-    wt[q] = (scatter >= p.scatmin) ? (float)((double)PW[I[q]] * scat[q] / scatter)
-                                   : (float)((double)PW[I[q]] * scat[q] / p.scatmin);
-  }
-  for (int q = 0; q < 8; q++)
-    for (int q2 = q + 1; q2 < 8; q2++)
-      if (iph[q] == iph[q2]) {
-        wt[q] += wt[q2];
-        wt[q2] = 0.f;
-      }
-  int ord[8];
-  std::iota(ord, ord + 8, 0);
-  std::stable_sort(ord, ord + 8, [&](int a, int b) { return wt[a] > wt[b]; });
-  for (int q = 0; q < 8; q++) {
-    iph_out[q] = iph[ord[q]];
-    wt_out[q] = wt[ord[q]];
-  }
+  prop_point(plan_prop_grid(p), ipa, X, Y, Z, extf, albf, iph_out, wt_out);
 }
 
 namespace {
+
+// Tabulated phase functions divided by (2l+1) (TRILIN_INTERP_PROP's set-up)
+// and, with delta-M, f = chi_{ML+1} subtracted from l <= ML (PREPARE_PROP,
+// 'N' interpolation).
+void prepare_legen(Plan& p) {
+  const int nl = p.nleg + 1, ML = p.ml;
+  p.legen.assign((size_t)p.numphase * nl, 0.f);
+  for (int iph = 0; iph < p.numphase; iph++)
+    for (int l = 0; l < nl; l++)
+      p.legen[(size_t)iph * nl + l] = p.legenp[(size_t)iph * (p.nlegp + 1) + l] / (float)(2 * l + 1);
+  if (p.d.deltam)
+    for (int iph = 0; iph < p.numphase; iph++) {
+      const float f = p.legen[(size_t)iph * nl + ML + 1];
+      for (int l = 0; l <= ML; l++) p.legen[(size_t)iph * nl + l] = p.legen[(size_t)iph * nl + l] - f;
+    }
+}
 
 void interp_optics(Plan& p) {
   const at3d_scene_desc& d = p.d;
@@ -487,20 +457,9 @@ void interp_optics(Plan& p) {
     }
     });
   }
-  // Tabulated phase functions divided by (2l+1) (TRILIN_INTERP_PROP init branch).
-  const int nl = p.nleg + 1;
-  p.legen.assign((size_t)p.numphase * nl, 0.f);
-  for (int iph = 0; iph < p.numphase; iph++)
-    for (int l = 0; l < nl; l++)
-      p.legen[(size_t)iph * nl + l] = p.legenp[(size_t)iph * (p.nlegp + 1) + l] / (float)(2 * l + 1);
-  // PREPARE_PROP, 'N' interpolation: subtract f from l<=ML, scale ext/albedo.
-  const int ML = p.ml;
-  const float phasemax = 0.999f;
+  prepare_legen(p);
+  const LegTable lt = plan_leg_table(p);
   if (p.d.deltam) {
-    for (int iph = 0; iph < p.numphase; iph++) {
-      float f = p.legen[(size_t)iph * nl + ML + 1];
-      for (int l = 0; l <= ML; l++) p.legen[(size_t)iph * nl + l] = p.legen[(size_t)iph * nl + l] - f;
-    }
     for (int ip = 0; ip < npts; ip++) {
       float s = 0.f;
       for (int ipa = 0; ipa < npart; ipa++) s += p.extinct[(size_t)ipa * npts + ip];
@@ -510,19 +469,9 @@ void interp_optics(Plan& p) {
     for (int ipa = 0; ipa < npart; ipa++)
       for (int ip = 0; ip < npts; ip++) {
         const size_t base = ((size_t)ipa * npts + ip) * 8;
-        float f;
-        if (p.phaseinterpwt[base] >= phasemax) {
-          f = p.legen[(size_t)(p.iphase[base] - 1) * nl + ML + 1];
-        } else {
-          f = 0.f;
-          for (int q = 0; q < 8; q++)
-            f = f + p.legen[(size_t)(p.iphase[base + q] - 1) * nl + ML + 1] * p.phaseinterpwt[base + q];
-        }
+        const float f = prepare_f(lt, &p.iphase[base], &p.phaseinterpwt[base]);
         float& ext = p.extinct[(size_t)ipa * npts + ip];
-        float& alb = p.albedo[(size_t)ipa * npts + ip];
-        float alb0 = alb;
-        ext = (1.0f - alb0 * f) * ext;
-        alb = (1.0f - f) * alb0 / (1.0f - alb0 * f);
+        deltam_scale(f, ext, p.albedo[(size_t)ipa * npts + ip]);
         p.total_ext[ip] = p.total_ext[ip] + ext;
       }
   }
@@ -554,78 +503,19 @@ void mixed_legen(const Plan& p, int ipa, int ip, std::vector<float>& out) {
 // shdomsub1.f:1075-1140) and the radiance-truncation weights
 // (RADIANCE_TRUNCATION, shdomsub1.f:1665-1735).
 void source_constants(Plan& p) {
-  const int ML = p.ml, npts = p.npts, npart = p.npart, nl = p.nleg + 1;
+  const int ML = p.ml, npts = p.npts, npart = p.npart;
   p.chi.assign((size_t)npts * (ML + 1), 0.f);
   p.alb.assign(npts, 0.f);
   p.tc.assign((size_t)npts * (ML + 1), 0.f);
+  const LegTable lt = plan_leg_table(p);
   // points are independent: chunks over the host threads
   const int nchunk = 64;
   parallel_for(nchunk, [&](int c) {
-  std::vector<float> l1, legent(nl);
-  const int i0 = (int)((int64_t)npts * c / nchunk), i1 = (int)((int64_t)npts * (c + 1) / nchunk);
-  for (int i = i0; i < i1; i++) {
-    const float ext = p.total_ext[i];
-    double alb = 0.0;
-    std::fill(legent.begin(), legent.end(), 0.f);
-    for (int ipa = 0; ipa < npart; ipa++) {
-      const double scat = (double)(p.extinct[(size_t)ipa * npts + i] * p.albedo[(size_t)ipa * npts + i]);
-      alb += scat;
-      mixed_legen(p, ipa, i, l1);
-      if (p.d.deltam) {
-        const float f = l1[ML + 1];
-        for (int l = 0; l <= ML; l++) l1[l] = l1[l] / (1 - f);
-      }
-      for (int l = 0; l < nl; l++) legent[l] = (float)((double)legent[l] + scat * (double)l1[l]);
-    }
-    if (alb > 1e-10) {
-      for (int l = 0; l < nl; l++) legent[l] = (float)((double)legent[l] / alb);
-    } else {
-      for (int l = 0; l < nl; l++) legent[l] = legent[l] / (float)npart;
-    }
-    alb = (ext > 1e-10f) ? alb / (double)ext : 0.0;
-    legent[0] = 1.0f;
-    for (int l = 0; l <= ML; l++) p.chi[(size_t)i * (ML + 1) + l] = legent[l];
-    p.alb[i] = (float)alb;
-    // truncation weights sum_ipa W*ALBEDO*LEGENT*F
-    for (int l = 1; l <= ML; l++) {
-      float rad = 0.f;
-      for (int ipa = 0; ipa < npart; ipa++) {
-        const float W = (ext == 0.0f) ? 1.0f : p.extinct[(size_t)ipa * npts + i] / ext;
-        if (W == 0.0f) continue;
-        const size_t base = ((size_t)ipa * npts + i) * 8;
-        float legt;
-        if (p.phaseinterpwt[base] >= 0.999f) {
-          legt = p.legen[(size_t)(p.iphase[base] - 1) * nl + l];
-        } else {
-          legt = 0.f;
-          for (int q = 0; q < 8; q++) {
-            if (p.phaseinterpwt[base + q] <= 1e-5f) continue;
-            legt = legt + p.legen[(size_t)(p.iphase[base + q] - 1) * nl + l] * p.phaseinterpwt[base + q];
-          }
-        }
-        if (p.d.deltam) {
-          float F;
-          if (p.phaseinterpwt[base] >= 0.999f) {
-            F = p.legen[(size_t)(p.iphase[base] - 1) * nl + ML + 1];
-          } else {
-            // The reference indexes LEGEN(Q,ML+1,IPHASE(Q)) here (first index Q,
-            // shdomsub1.f:1688-1689); with NSTLEG=1 that is coefficient ML+Q.
-            F = 0.f;
-            const size_t total = p.legen.size();
-            for (int q = 0; q < 8; q++) {
-              if (p.phaseinterpwt[base + q] <= 1e-5f) continue;
-              size_t flat = (size_t)(p.iphase[base + q] - 1) * nl + ML + 1 + q;
-              float c = flat < total ? p.legen[flat] : 0.f;
-              F = F + c * p.phaseinterpwt[base + q];
-            }
-          }
-          legt = legt * (1.0f / (1 - F));
-        }
-        rad = rad + W * p.albedo[(size_t)ipa * npts + i] * legt;
-      }
-      p.tc[(size_t)i * (ML + 1) + l] = rad;
-    }
-  }
+    const int i0 = (int)((int64_t)npts * c / nchunk), i1 = (int)((int64_t)npts * (c + 1) / nchunk);
+    for (int i = i0; i < i1; i++)
+      point_source_constants(lt, npart, p.d.deltam != 0, p.total_ext[i], &p.extinct[i], &p.albedo[i],
+                             &p.iphase[(size_t)i * 8], &p.phaseinterpwt[(size_t)i * 8], (size_t)npts,
+                             &p.chi[(size_t)i * (ML + 1)], p.alb[i], &p.tc[(size_t)i * (ML + 1)]);
   });
 }
 
@@ -716,209 +606,34 @@ namespace {
 // Eddington two-stream initial radiance: INIT_RADIANCE (shdomsub2.f:614-760),
 // EDDRTF (shdomsub2.f:765-993) solar branch and TRIDIAG (shdomsub2.f:997-1054).
 // ---------------------------------------------------------------------------
-bool tridiag(int n, std::vector<double>& lower, std::vector<double>& diag,
-             std::vector<double>& upper, std::vector<double>& rhs) {
-  // 1-based arrays of size n+1
-  if (n == 1) {
-    if (diag[1] == 0.0) return false;
-    rhs[1] = rhs[1] / diag[1];
-  }
-  lower[1] = diag[1];
-  diag[1] = upper[1];
-  upper[1] = 0.0;
-  upper[n] = 0.0;
-  for (int k = 1; k <= n - 1; k++) {
-    if (fabs(lower[k + 1]) >= fabs(lower[k])) {
-      std::swap(lower[k + 1], lower[k]);
-      std::swap(diag[k + 1], diag[k]);
-      std::swap(upper[k + 1], upper[k]);
-      std::swap(rhs[k + 1], rhs[k]);
-    }
-    if (lower[k] == 0.0) return false;
-    double t = -lower[k + 1] / lower[k];
-    lower[k + 1] = diag[k + 1] + t * diag[k];
-    diag[k + 1] = upper[k + 1] + t * upper[k];
-    upper[k + 1] = 0.0;
-    rhs[k + 1] = rhs[k + 1] + t * rhs[k];
-  }
-  if (lower[n] == 0.0) return false;
-  rhs[n] = rhs[n] / lower[n];
-  rhs[n - 1] = (rhs[n - 1] - diag[n - 1] * rhs[n]) / lower[n - 1];
-  for (int kb = 1; kb <= n - 2; kb++) {
-    int k = n - 2 - kb + 1;
-    rhs[k] = (rhs[k] - diag[k] * rhs[k + 1] - upper[k] * rhs[k + 2]) / lower[k];
-  }
-  return true;
-}
-
-bool eddrtf_solar(int nlayer, const std::vector<float>& od, const std::vector<float>& alb,
-                  const std::vector<float>& asym, float solarflux, float solarmu, float gndemis,
-                  float skyrad, std::vector<float>& fluxes /*3*(nlayer+1)*/) {
-  const int N = 2 * nlayer + 2;
-  std::vector<double> lower(N + 1), upper(N + 1), diag(N + 1), rhs(N + 1);
-  const double PI = (double)3.1415926535f;
-  const double mu0 = fabs((double)solarmu);
-  double tau = 0.0;
-  int I = 2;
-  fluxes.assign(3 * (nlayer + 1), 0.f);
-  for (int L = 1; L <= nlayer; L++) {
-    double deltau = od[L - 1], trans, reflect, sourcep, sourcem;
-    if (deltau < 0.0) return false;
-    if (deltau == 0.0) {
-      trans = 1.0;
-      reflect = 0.0;
-      sourcep = 0.0;
-      sourcem = 0.0;
-    } else {
-      const double omega = alb[L - 1], g = asym[L - 1];
-      const double R = (1.0 - omega * (4.0 - 3.0 * g)) / 4.0;
-      const double T = (7.0 - omega * (4.0 + 3.0 * g)) / 4.0;
-      const double lambda = sqrt(3.0 * (1.0 - omega) * (1.0 - omega * g));
-      double D, X1 = 0, X2 = 0, exlp = 0, exlm = 0;
-      if (lambda == 0.0) {
-        D = 1.0 / (1.0 + T * deltau);
-        trans = D;
-        reflect = -R * deltau * D;
-      } else {
-        X1 = -R;
-        X2 = lambda + T;
-        exlp = exp(std::min(lambda * deltau, 75.0));
-        exlm = 1.0 / exlp;
-        trans = 2. * lambda / (X2 * exlp + (lambda - T) * exlm);
-        reflect = X1 * (exlp - exlm) * trans / (2. * lambda);
-        D = 1.0 / (X2 * X2 * exlp - X1 * X1 * exlm);
-      }
-      fluxes[3 * (L - 1) + 2] = (float)((double)solarflux * exp(-tau / mu0));
-      const double ds = 1.0 / (lambda * lambda - 1.0 / (mu0 * mu0));
-      const double b1 = 0.5 * omega * ((double)solarflux / mu0) * exp(-tau / mu0) * ds;
-      const double b2 = 0.5 * omega * ((double)solarflux / mu0) * exp(-(tau + deltau) / mu0) * ds;
-      const double solpp = 1.0 + 1.5 * g * mu0, solpm = -1.0 + 1.5 * g * mu0;
-      const double radp1p = ((T + 1.0 / mu0) * solpp + R * solpm) * b1;
-      const double radp2m = ((-T + 1.0 / mu0) * solpm - R * solpp) * b2;
-      const double radp2p = ((T + 1.0 / mu0) * solpp + R * solpm) * b2;
-      const double radp1m = ((-T + 1.0 / mu0) * solpm - R * solpp) * b1;
-      if (lambda == 0.0) {
-        const double A = (R * deltau * radp1p - radp2m) * D;
-        const double B = -(R * radp1p + T * radp2m) * D;
-        sourcep = (B - T * (A + B * deltau)) / R + radp2p;
-        sourcem = A + radp1m;
-      } else {
-        const double cp = (X1 * exlm * radp1p - X2 * radp2m) * D;
-        const double cm = (-X2 * exlp * radp1p + X1 * radp2m) * D;
-        sourcep = X1 * cp * exlp + X2 * cm * exlm + radp2p;
-        sourcem = X2 * cp + X1 * cm + radp1m;
-      }
-      tau = tau + deltau;
-    }
-    diag[I] = -reflect;
-    diag[I + 1] = -reflect;
-    lower[I] = 1.0;
-    lower[I + 1] = -trans;
-    upper[I] = -trans;
-    upper[I + 1] = 1.0;
-    rhs[I] = sourcem;
-    rhs[I + 1] = sourcep;
-    I += 2;
-  }
-  fluxes[3 * nlayer + 2] = (float)((double)solarflux * exp(-tau / mu0));
-  double gndflux = (1.0 - (double)gndemis) * (double)solarflux * exp(-tau / mu0);
-  const double skyflux = PI * (double)skyrad;
-  gndflux = gndflux + 0.0;  // no surface emission source
-  rhs[1] = skyflux;
-  diag[1] = 0.0;
-  upper[1] = 1.0;
-  diag[N] = -(1.0 - (double)gndemis);
-  lower[N] = 1.0;
-  rhs[N] = gndflux;
-  if (!tridiag(N, lower, diag, upper, rhs)) return false;
-  for (int L = 1, i = 1; L <= nlayer + 1; L++, i += 2) {
-    fluxes[3 * (L - 1) + 0] = (float)rhs[i];
-    fluxes[3 * (L - 1) + 1] = (float)rhs[i + 1];
-  }
-  return true;
-}
-
+// The two-stream start of every column (INIT_RADIANCE with EDDRTF,
+// shdomsub2.f:614-993): layers from the points' optics, the delta-Eddington
+// fluxes, and the radiance's first terms I0 = C0 (F_down + F_up),
+// I1 = C1 (F_down - F_up).
 bool init_radiance(Plan& p, std::string& err) {
-  const int nz = p.nz, nxy = p.nx1 * p.ny1, nlayer = nz - 1, npart = p.npart, ML = p.ml;
-  const int nl = p.nleg + 1;
-  const float PI = acosf(-1.0f);
-  const float C0 = sqrtf(1.0f / PI), C1 = sqrtf(3.0f / (4 * PI));
-  // sky radiance flux over the downwelling ordinates (INIT_SOLUTION shdomsub1.f:355-361)
-  float skyradalb = 0.f;
-  for (int imu = 0; imu < p.nmu / 2; imu++)
-    for (int ip = 0; ip < p.nphi0[imu]; ip++)
-      skyradalb = skyradalb + fabsf(p.mu[imu]) * p.wtdo[(size_t)imu * p.nphi0max + ip] * p.d.skyrad;
+  const int nz = p.nz, nxy = p.nx1 * p.ny1, nlayer = nz - 1, npart = p.npart;
+  const float skyradalb = sky_flux_weight(p);
   p.radiance0.assign((size_t)p.npts * 4, 0.f);
-  auto wt1 = [&](int ipa, int ip, int q) { return p.phaseinterpwt[((size_t)ipa * p.npts + ip) * 8 + q]; };
-  auto iph = [&](int ipa, int ip, int q) { return p.iphase[((size_t)ipa * p.npts + ip) * 8 + q]; };
-  // columns are independent: chunks over the host threads
+  const LegTable lt = plan_leg_table(p);
   const int nchunk = 64;
   std::vector<char> bad(nchunk, 0);
   parallel_for(nchunk, [&](int c) {
-  std::vector<float> od(nz), albs(nz), asym(nz), fl;
-  const int c0 = (int)((int64_t)nxy * c / nchunk), c1 = (int)((int64_t)nxy * (c + 1) / nchunk);
-  for (int I = c0; I < c1; I++) {
-    for (int iz = 1; iz <= nlayer; iz++) {
-      const int L = nz - iz;
-      const int ip0 = (iz - 1) + nz * I, ip1 = iz + nz * I;
-      const float ext0 = p.total_ext[ip0], ext1 = p.total_ext[ip1];
-      float scat0 = 0.f, scat1 = 0.f;
-      for (int ipa = 0; ipa < npart; ipa++) {
-        scat0 += p.albedo[(size_t)ipa * p.npts + ip0] * p.extinct[(size_t)ipa * p.npts + ip0];
-        scat1 += p.albedo[(size_t)ipa * p.npts + ip1] * p.extinct[(size_t)ipa * p.npts + ip1];
+    std::vector<float> od(nz), albs(nz), asym(nz), fl(3 * (size_t)(nlayer + 1));
+    std::vector<double> work(4 * (size_t)(2 * nlayer + 3));
+    double* wk = work.data();
+    const size_t nw = 2 * (size_t)nlayer + 3;
+    const int c0 = (int)((int64_t)nxy * c / nchunk), c1 = (int)((int64_t)nxy * (c + 1) / nchunk);
+    for (int I = c0; I < c1; I++) {
+      eddington_layers(lt, npart, p.d.deltam != 0, nz, p.zgrid.data(), p.total_ext.data(), p.extinct.data(),
+                       p.albedo.data(), p.iphase.data(), p.phaseinterpwt.data(), (size_t)p.npts, I, od.data(),
+                       albs.data(), asym.data());
+      if (!eddrtf_column(nlayer, od.data(), albs.data(), asym.data(), p.d.solarflux, p.d.solarmu,
+                         1.0f - p.d.gndalbedo, skyradalb, fl.data(), wk, wk + nw, wk + 2 * nw, wk + 3 * nw)) {
+        bad[c] = 1;
+        return;
       }
-      od[L - 1] = (p.zgrid[iz] - p.zgrid[iz - 1]) * (ext0 + ext1) / 2;
-      albs[L - 1] = (ext0 + ext1 > 0.f) ? (scat0 + scat1) / (ext0 + ext1) : 0.f;
-      float g0 = 0.f, g1 = 0.f;
-      for (int ipa = 0; ipa < npart; ipa++) {
-        float le0, le1;
-        // branch on species 1's leading weight, as the reference does
-        if (wt1(0, ip0, 0) >= 0.999f) {
-          le0 = p.legen[(size_t)(iph(ipa, ip0, 0) - 1) * nl + 1];
-        } else {
-          le0 = 0.f;
-          for (int q = 0; q < 8; q++) le0 = le0 + p.legen[(size_t)(iph(ipa, ip0, q) - 1) * nl + 1] * wt1(ipa, ip0, q);
-        }
-        if (wt1(0, ip1, 0) >= 0.999f) {
-          le1 = p.legen[(size_t)(iph(ipa, ip1, 0) - 1) * nl + 1];
-        } else {
-          le1 = 0.f;
-          for (int q = 0; q < 8; q++) le1 = le1 + p.legen[(size_t)(iph(ipa, ip1, q) - 1) * nl + 1] * wt1(ipa, ip1, q);
-        }
-        if (p.d.deltam) {
-          float f0, f1;
-          if (wt1(0, ip0, 0) >= 0.999f) {
-            f0 = p.legen[(size_t)(iph(ipa, ip0, 0) - 1) * nl + ML + 1];
-          } else {
-            f0 = 0.f;
-            for (int q = 0; q < 8; q++) f0 = f0 + p.legen[(size_t)(iph(ipa, ip0, q) - 1) * nl + ML + 1] * wt1(ipa, ip0, q);
-          }
-          if (wt1(0, ip1, 0) >= 0.999f) {
-            f1 = p.legen[(size_t)(iph(ipa, ip1, 0) - 1) * nl + ML + 1];
-          } else {
-            f1 = 0.f;
-            for (int q = 0; q < 8; q++) f1 = f1 + p.legen[(size_t)(iph(ipa, ip1, q) - 1) * nl + ML + 1] * wt1(ipa, ip1, q);
-          }
-          le0 = le0 / (1 - f0);
-          le1 = le1 / (1 - f1);
-        }
-        g0 += p.albedo[(size_t)ipa * p.npts + ip0] * p.extinct[(size_t)ipa * p.npts + ip0] * le0;
-        g1 += p.albedo[(size_t)ipa * p.npts + ip1] * p.extinct[(size_t)ipa * p.npts + ip1] * le1;
-      }
-      asym[L - 1] = (scat0 + scat1 > 0.f) ? (g0 + g1) / (scat0 + scat1) : 0.f;
+      column_radiance(nz, I, fl.data(), p.radiance0.data());
     }
-    const float gndemis = 1.0f - p.d.gndalbedo;
-    if (!eddrtf_solar(nlayer, od, albs, asym, p.d.solarflux, p.d.solarmu, gndemis, skyradalb, fl)) {
-      bad[c] = 1;
-      return;
-    }
-    for (int iz = 1; iz <= nz; iz++) {
-      const int L = nz + 1 - iz;
-      const size_t ip = (size_t)(iz - 1) + (size_t)nz * I;
-      p.radiance0[ip * 4 + 0] = C0 * (fl[3 * (L - 1) + 0] + fl[3 * (L - 1) + 1]);
-      p.radiance0[ip * 4 + 2] = C1 * (fl[3 * (L - 1) + 0] - fl[3 * (L - 1) + 1]);
-    }
-  }
   });
   for (char b : bad)
     if (b) {
@@ -1276,6 +991,8 @@ int build_levels(Plan& p, std::string& err) {
 // truncated single-scatter weights sum_species DA * LEGENT_l / (1-f) and the
 // (phase table, DA * w / (1-f)) pairs of the exact single-scatter term.
 // ---------------------------------------------------------------------------
+int render_point_constants(Plan& p);
+
 int render_constants(Plan& p, std::string& err) {
   const int nsa = p.nscatangle, nlegp = p.nlegp, np = p.numphase;
   p.phasetab.assign((size_t)np * nsa, 0.f);
@@ -1309,12 +1026,20 @@ int render_constants(Plan& p, std::string& err) {
       p.phasetab[(size_t)iph * nsa + j] = (float)(a1 * ofourpi);
     }
   }
+  return render_point_constants(p);
+}
+
+// The per-point single-scatter constants (TMS terms and the exact
+// single-scatter pairs, CSR in point order) of every grid point.
+int render_point_constants(Plan& p) {
   const int ML = p.ml, npts = p.npts, npart = p.npart;
   const float secmu0 = (float)(1.0 / fabs((double)p.d.solarmu));
   p.tms.assign((size_t)npts * (ML + 1), 0.f);
   p.ss_off.assign(npts + 1, 0);
   p.ss_iph.clear();
   p.ss_coef.clear();
+  if (!p.d.deltam) return AT3D_OK;
+  const LegTable lt = plan_leg_table(p);
   // per point: the TMS terms and its (phase, coefficient) list; points in
   // chunks over the host threads, each chunk's lists joined in point order
   const int nchunk = 64;
@@ -1322,34 +1047,17 @@ int render_constants(Plan& p, std::string& err) {
   std::vector<std::vector<float>> c_coef(nchunk);
   std::vector<int32_t> cnt(npts, 0);
   parallel_for(nchunk, [&](int c) {
-    std::vector<float> legt;
+    std::vector<int32_t> ib(8 * (size_t)npart);
+    std::vector<float> cb(8 * (size_t)npart);
     const int i0 = (int)((int64_t)npts * c / nchunk), i1 = (int)((int64_t)npts * (c + 1) / nchunk);
     for (int ip = i0; ip < i1; ip++) {
-      if (!p.d.deltam) continue;
-      const size_t before = c_iph[c].size();
-      const float ext = p.total_ext[ip];
-      for (int ipa = 0; ipa < npart; ipa++) {
-        const float W = (ext == 0.0f) ? 1.0f : p.extinct[(size_t)ipa * npts + ip] / ext;
-        if (W == 0.0f) continue;
-        mixed_legen(p, ipa, ip, legt);
-        const float f = legt[ML + 1];
-        for (float& v : legt) v = v / (1 - f);
-        const float da = p.albedo[(size_t)ipa * npts + ip] * secmu0 * W;   // per unit direct flux
-        for (int l = 0; l <= ML; l++) p.tms[(size_t)ip * (ML + 1) + l] += da * legt[l];
-        const size_t base = ((size_t)ipa * npts + ip) * 8;
-        if (p.phaseinterpwt[base] >= 0.999f) {
-          c_iph[c].push_back(p.iphase[base] - 1);
-          c_coef[c].push_back(da / (1 - f));
-        } else {
-          for (int q = 0; q < 8; q++) {
-            const float w = p.phaseinterpwt[base + q];
-            if (w <= 1e-5f) continue;
-            c_iph[c].push_back(p.iphase[base + q] - 1);
-            c_coef[c].push_back(da * w / (1 - f));
-          }
-        }
-      }
-      cnt[ip] = (int32_t)(c_iph[c].size() - before);
+      const int n = point_render_constants(lt, npart, p.total_ext[ip], &p.extinct[ip], &p.albedo[ip],
+                                           &p.iphase[(size_t)ip * 8], &p.phaseinterpwt[(size_t)ip * 8],
+                                           (size_t)npts, secmu0, &p.tms[(size_t)ip * (ML + 1)], ib.data(),
+                                           cb.data());
+      c_iph[c].insert(c_iph[c].end(), ib.begin(), ib.begin() + n);
+      c_coef[c].insert(c_coef[c].end(), cb.begin(), cb.begin() + n);
+      cnt[ip] = n;
     }
   });
   for (int ip = 0; ip < npts; ip++) p.ss_off[ip + 1] = p.ss_off[ip] + cnt[ip];

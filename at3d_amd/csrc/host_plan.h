@@ -12,6 +12,7 @@
 #include "direct_beam.h"
 
 #include "../../include/at3d_hip.h"
+#include "medium_point.h"
 #include "shdom_geom.h"
 
 namespace at3d {
@@ -125,6 +126,14 @@ int plan_refresh_points(Plan& p, std::string& err);
 // base plan; pointers are 0-based as in GridTopo.
 int plan_set_grid(Plan& p, int npts, int ncells, const float* gridpos, const int32_t* gridptr,
                   const int32_t* neighptr, const int32_t* treeptr, const int8_t* cellflags, std::string& err);
+
+// The property grid and the prepared Legendre table of a plan, as the
+// shared per-point set-up (medium_point.h) reads them.
+PropGrid plan_prop_grid(const Plan& p);
+LegTable plan_leg_table(const Plan& p);
+// The sky radiance's flux weight over the downwelling ordinates (INIT_SOLUTION,
+// shdomsub1.f:355-361), the two-stream start's top boundary.
+float sky_flux_weight(const Plan& p);
 
 // TRILIN_INTERP_PROP for species ipa at one point, before delta-M scaling:
 // extinction, albedo and the eight (table index, weight) pairs.
