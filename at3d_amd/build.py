@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libat3d_hip.so")
 SOURCES = ["devmem.cpp", "host_plan.cpp", "grad_plan.cpp", "ring_plan.cpp", "direct.hip", "transforms.hip", "sweep.hip", "source.hip",
-           "render.hip", "gradient.hip", "solver.hip", "render_host.hip", "grad_host.hip", "solution_io.hip", "split_grid.cpp", "capi.cpp"]
+           "render.hip", "gradient.hip", "solver.hip", "render_host.hip", "grad_host.hip", "solution_io.hip", "split_grid.cpp", "medium.hip", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("AT3D_OFFLOAD_ARCH", "gfx950")
 # Packed FP32 instructions are disabled for every HIP translation unit.

@@ -127,6 +127,16 @@ int at3d_plan_get(const at3d_plan* h, const char* name, void* dst, size_t nbytes
   }
   const at3d::Plan& p = h->p;
   const std::string k(name);
+  // (the ring layout groups points by source class, made from the medium)
+  if (k == "total_ext" || k == "extinct" || k == "albedo" || k == "radiance0" || k == "chi" || k == "alb" ||
+      k.compare(0, 5, "ring_") == 0) {
+    std::string msg;
+    const int rc = at3d::ensure_host_medium(p, msg);
+    if (rc) {
+      set_err(err, errlen, msg);
+      return rc;
+    }
+  }
   if (k == "gridpos") return copy_out(p.gridpos, dst, nbytes, err, errlen);
   if (k == "gridptr") return copy_out(p.gridptr, dst, nbytes, err, errlen);
   if (k == "neighptr") return copy_out(p.neighptr, dst, nbytes, err, errlen);
@@ -377,6 +387,14 @@ int at3d_state_get(at3d_state* st, const char* name, void* dst, size_t nbytes, c
   }
   // the current grid (host copies in the state's plan)
   const at3d::Plan& p = *s.plan;
+  if (k == "total_ext" || k == "extinct" || k == "albedo") {
+    std::string msg;
+    const int rc = at3d::ensure_host_medium(p, msg);
+    if (rc) {
+      set_err(err, errlen, msg);
+      return rc;
+    }
+  }
   if (k == "gridpos") return copy_out(p.gridpos, dst, nbytes, err, errlen);
   if (k == "gridptr") return copy_out(p.gridptr, dst, nbytes, err, errlen);
   if (k == "neighptr") return copy_out(p.neighptr, dst, nbytes, err, errlen);
@@ -388,10 +406,27 @@ int at3d_state_get(at3d_state* st, const char* name, void* dst, size_t nbytes, c
     if (!p.treeptr.empty()) return copy_out(p.treeptr, dst, nbytes, err, errlen);
     return copy_out(std::vector<int32_t>((size_t)p.ncells * 2, -1), dst, nbytes, err, errlen);
   }
+  // the state's own per-point medium (uploaded from the plan or made on the device)
+  const size_t ml1 = p.ml + 1;
+  if (k == "point_ext") return get1d(s.ext, npts * sizeof(float));
+  if (k == "point_alb") return get1d(s.alb, npts * sizeof(float));
+  if (k == "point_chi") return get1d(s.chi, npts * ml1 * sizeof(float));
+  if (k == "point_tc") return get1d(s.tc, npts * ml1 * sizeof(float));
+  if (k == "point_tms") return get1d(s.tms, npts * ml1 * sizeof(float));   // times the direct flux
+  if (k == "point_radiance0") return get1d(s.R0, npts * 4 * sizeof(float));
+  if (k == "point_ss_off") return get1d(s.ss_off, (npts + 1) * sizeof(int32_t));
   if (k == "fluxes") return get1d(s.fluxes, 2 * npts * sizeof(float));
   if (k == "dirflux") return get1d(s.dirflux, npts * sizeof(float));
   if (k == "ns") return get1d(s.ns, npts * sizeof(int));
   if (k == "nr") return get1d(s.nr, npts * sizeof(int));
+  if (k == "point_ss_iph" || k == "point_ss_coef") {
+    int32_t n = 0;
+    if (hipMemcpy(&n, s.ss_off + npts, sizeof n, hipMemcpyDeviceToHost) != hipSuccess) {
+      set_err(err, errlen, "device copy failed");
+      return AT3D_ERR_DEVICE;
+    }
+    return k == "point_ss_iph" ? get1d(s.ss_iph, (size_t)n * 4) : get1d(s.ss_coef, (size_t)n * 4);
+  }
   set_err(err, errlen, "unknown state array '" + k + "'");
   return AT3D_ERR_ARG;
 }

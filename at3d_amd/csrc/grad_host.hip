@@ -131,6 +131,11 @@ int state_levis_gradient(State& s, const at3d_deriv_desc& dd, int64_t nrays, con
     err = "the solution is not initialised: solve before computing a gradient";
     return AT3D_ERR_ARG;
   }
+  {
+    // the derivative set-up reads the per-point optics on the host
+    const int rc = ensure_host_medium(p, err);
+    if (rc) return rc;
+  }
   if (!p.d.deltam) {
     err = "the gradient requires delta-M scaling (deltam = True)";
     return AT3D_ERR_ARG;

@@ -429,12 +429,16 @@ void prepare_legen(Plan& p) {
     }
 }
 
-void interp_optics(Plan& p) {
-  const at3d_scene_desc& d = p.d;
-  const int npz = d.npz;
-  const int npts = p.npts, npart = p.npart;
+// EXTMIN and SCATMIN of TRILIN_INTERP_PROP (shdom90.f90:60-62).
+void medium_scalars(Plan& p) {
+  const int npz = p.d.npz;
   p.extmin = (double)(1.0e-5f / ((p.zlevels[npz - 1] - p.zlevels[0]) / (float)npz));
   p.scatmin = (double)0.1f * p.extmin;
+}
+
+void interp_optics(Plan& p) {
+  const int npts = p.npts, npart = p.npart;
+  medium_scalars(p);
   p.extinct.assign((size_t)npart * npts, 0.f);
   p.albedo.assign((size_t)npart * npts, 0.f);
   p.iphase.assign((size_t)npart * npts * 8, 1);
@@ -547,6 +551,13 @@ void direct_beam_setup(const Plan& p, std::vector<float>& extdirp, DirectBeamGeo
     }
     extdirp[ip] = acc;
   }
+  direct_beam_geometry(p, b);
+  b.ext = extdirp.data();
+}
+
+void direct_beam_geometry(const Plan& p, DirectBeamGeom& b) {
+  const at3d_scene_desc& d = p.d;
+  const int npx = d.npx, npy = d.npy, npz = d.npz;
   b.ipdirect = p.ipflag;
   if (btest(p.ipflag, 2)) b.ipdirect = 0;
   const double sunmu = -(double)d.solarmu;
@@ -573,7 +584,7 @@ void direct_beam_setup(const Plan& p, std::vector<float>& extdirp, DirectBeamGeo
   b.ydomain = b.delyd * npy;
   if (btest(p.bcflag, 3)) b.ydomain = b.delyd * (npy - 1);
   b.zl = p.zlevels.data();
-  b.ext = extdirp.data();
+  b.ext = nullptr;
   b.npx = npx;
   b.npy = npy;
   b.npz = npz;
@@ -1026,6 +1037,7 @@ int render_constants(Plan& p, std::string& err) {
       p.phasetab[(size_t)iph * nsa + j] = (float)(a1 * ofourpi);
     }
   }
+  if (!p.host_medium) return AT3D_OK;
   return render_point_constants(p);
 }
 
@@ -1215,12 +1227,20 @@ int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err) {
   int rc = make_angle_set(p, err);
   if (rc) return rc;
   make_sh_tables(p);
-  interp_optics(p);
-  source_constants(p);
-  // MAKE_DIRECT: the direct flux is made per solve on the device (or on the
-  // host by plan_direct_host when asked for); the plan keeps the constants
-  // that scale with it per unit direct flux
-  if (!init_radiance(p, err)) return AT3D_ERR_NUMERIC;
+  if (d.device_medium) {
+    // the state makes the per-point medium on the device
+    medium_scalars(p);
+    prepare_legen(p);
+    p.medium_on_device = true;
+    p.host_medium = false;
+  } else {
+    interp_optics(p);
+    source_constants(p);
+    // MAKE_DIRECT: the direct flux is made per solve on the device (or on the
+    // host by plan_direct_host when asked for); the plan keeps the constants
+    // that scale with it per unit direct flux
+    if (!init_radiance(p, err)) return AT3D_ERR_NUMERIC;
+  }
   boundary_points(p);
   make_ordinates(p);
   rc = build_levels(p, err);
@@ -1278,6 +1298,19 @@ int plan_set_grid(Plan& p, int npts, int ncells, const float* gridpos, const int
   // (_prepare_optical_properties after load_solution, solver.py:1595)
   interp_optics(p);
   return plan_refresh_points(p, err);
+}
+
+int ensure_host_medium(const Plan& pc, std::string& err) {
+  static std::mutex m;
+  std::lock_guard<std::mutex> lock(m);
+  if (pc.host_medium) return AT3D_OK;
+  Plan& p = const_cast<Plan&>(pc);   // a plan object is never const itself
+  interp_optics(p);
+  source_constants(p);
+  if (!init_radiance(p, err)) return AT3D_ERR_NUMERIC;
+  render_point_constants(p);
+  p.host_medium = true;
+  return AT3D_OK;
 }
 
 int plan_refresh_points(Plan& p, std::string& err) {

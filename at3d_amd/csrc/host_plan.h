@@ -94,6 +94,11 @@ struct Plan {
   std::vector<float> legen;                       // [numphase][nleg+1] (post delta-M prep)
   std::vector<float> chi, alb, tc;                // [npts][ml+1], [npts], [npts][ml+1]
   float albmax = 0.f;
+  // the per-point medium (optics, source and single-scatter constants,
+  // two-stream start) is held here only when host_medium; a plan built for
+  // the device set-up leaves it to the state and makes it here on request
+  // (ensure_host_medium)
+  bool medium_on_device = false, host_medium = true;
   double extmin = 0, scatmin = 0;
   // rendering constants
   std::vector<float> phasetab;                    // [numphase][nscatangle]
@@ -113,6 +118,10 @@ struct Plan {
 
 // Builds the plan; returns AT3D_OK or an error status with a message.
 int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err);
+
+// The plan's per-point medium on the host, made once when the plan was built
+// for the device set-up (thread safe; the plan object is filled in place).
+int ensure_host_medium(const Plan& p, std::string& err);
 
 // After the grid of a plan changed (cells split, points appended, the
 // per-species optics of every point in place): the per-point source and
@@ -162,6 +171,8 @@ void plan_source_class(const Plan& p, std::vector<int32_t>& cls);
 // The direct beam's trace geometry and delta-M scaled extinction
 // (DIRECT_BEAM_PROP's set-up); b.zl points into p, b.ext into extdirp.
 void direct_beam_setup(const Plan& p, std::vector<float>& extdirp, DirectBeamGeom& b);
+// The same geometry without the extinction (b.ext null; b.zl into p).
+void direct_beam_geometry(const Plan& p, DirectBeamGeom& b);
 // The direct flux of every grid point on the host threads (a solve makes
 // its own on the device; the plan holds none).
 // Fills op's ring_* entry order for octant o from its levels (geometry only,

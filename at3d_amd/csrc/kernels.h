@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "medium_point.h"
 #include "shdom_geom.h"
 
 namespace at3d {
@@ -204,6 +205,50 @@ void launch_compute_source(hipStream_t st, const SourceArgs& a, int nblocks);
 // R[p][J] = R0[p][J] for J < 4, zero for 4 <= J < max(4, nr[p]) (R is zero
 // beyond NR already); then nr[p] = 4.
 void launch_init_radiance(hipStream_t st, float* R, const float* R0, int* nr, int ld, int npts);
+
+// ---------------------------------------------------------------------------
+// The per-point medium on the device (medium.hip)
+// ---------------------------------------------------------------------------
+struct MediumArgs {
+  PropGrid g;                  // property grid (device pointers)
+  LegTable t;                  // prepared Legendre table (device)
+  const float* legenp = nullptr;   // the raw table [numphase][nlegp+1] (device)
+  int nlegp = 0;
+  int npart = 0, npts = 0, nz = 0, ncol = 0;
+  bool deltam = false;
+  float secmu0 = 0.f, solarflux = 0.f, solarmu = 0.f, gndemis = 0.f, skyflux_weight = 0.f;
+  const float* gridpos = nullptr;
+  const float* zgrid = nullptr;
+  // species-major optics [npart][npts], phase pairs [npart][npts][8]
+  float* extinct = nullptr;
+  float* albedo = nullptr;
+  int32_t* iphase = nullptr;
+  float* wt = nullptr;
+  // per point
+  float* total_ext = nullptr;
+  float* alb = nullptr;
+  float* chi = nullptr;        // [npts][ml+1]
+  float* tc = nullptr;         // [npts][ml+1]
+  float* tms = nullptr;        // [npts][ml+1] per unit direct flux
+  int32_t* ss_cnt = nullptr;
+  int32_t* ss_off = nullptr;   // [npts+1]
+  int32_t* ss_iph = nullptr;
+  float* ss_coef = nullptr;
+  int32_t* cls = nullptr;
+  unsigned int* albmax = nullptr;
+  // two-stream start: per column 4 (2 nlayer + 3) doubles and 6 nz floats
+  double* work = nullptr;
+  float* fwork = nullptr;
+  float* radiance0 = nullptr;  // [npts][4]
+  float* beam_ext = nullptr;   // [npx * npy * npz]
+  int* err = nullptr;
+};
+// Optics, constants and the pair counts' offsets (total pairs into *total).
+void launch_medium_points(hipStream_t st, const MediumArgs& m, int32_t* block_sum, int32_t* total);
+// The pairs themselves (after ss_iph / ss_coef are allocated).
+void launch_medium_pairs(hipStream_t st, const MediumArgs& m);
+// The direct beam's extinction and the two-stream start.
+void launch_medium_start(hipStream_t st, const MediumArgs& m);
 void launch_reduce_dots(hipStream_t st, const double* partial, int nblocks, double* out,
                         const int* stop = nullptr);
 // ctl null: S += a DS; else with ctl's parameter, when it is positive.

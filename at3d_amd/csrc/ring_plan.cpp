@@ -141,12 +141,15 @@ static bool build_octant_levels(const Plan& p, int o, bool used, const std::vect
   return true;
 }
 
-bool build_ring_plan(const Plan& p, const bool used[8], size_t lds_budget, RingPlan& r,
-                     std::string& err) {
+bool build_ring_plan(const Plan& p, const bool used[8], size_t lds_budget, RingPlan& r, std::string& err,
+                     const std::vector<int32_t>* cls_in) {
   const int npts = p.npts;
   r.nlev_max = 0;
   std::vector<int32_t> cls;
-  plan_source_class(p, cls);
+  if (cls_in)
+    cls = *cls_in;
+  else
+    plan_source_class(p, cls);
   int ncls = 1;
   for (int c : cls) ncls = std::max(ncls, c + 1);
   // octants are independent: one host thread each

@@ -47,6 +47,9 @@ inline size_t ring_plan_lds_bytes(int slots, int nlev, int ncls) {
 // Builds the ring layout for the octants with ordinates (used[o] != 0), with
 // 32-byte sectors if that ring fits in lds_budget bytes, else 16-byte ones.
 // Returns false (err set) if the plan's level lists are incomplete.
-bool build_ring_plan(const Plan& p, const bool used[8], size_t lds_budget, RingPlan& r, std::string& err);
+// cls: each point's source class (the plan's, from its source constants,
+// when null).
+bool build_ring_plan(const Plan& p, const bool used[8], size_t lds_budget, RingPlan& r, std::string& err,
+                     const std::vector<int32_t>* cls = nullptr);
 
 }  // namespace at3d

@@ -180,7 +180,9 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
   HIPCHK(hipGetDevice(&dev));
   HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
   RingPlan rp;
-  if (!build_ring_plan(p, used, (size_t)maxlds, rp, err)) return AT3D_ERR_NUMERIC;
+  if (!build_ring_plan(p, used, (size_t)maxlds, rp, err, p.medium_on_device ? &s.medium_cls_h : nullptr))
+    return AT3D_ERR_NUMERIC;
+  const float* total_ext = p.medium_on_device ? s.medium_ext_h.data() : p.total_ext.data();
   s.sector_points = rp.sector_points;
   s.slots = rp.slots;
   s.gld = rp.nlev_max + 2;
@@ -220,7 +222,7 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
     std::vector<float> dbest(s.ncls, 3.0e38f);
     for (size_t q = 0; q < np; q++) {
       pm[q] = ro.code[q] >= 0 ? ro.code[q] : bc[-1 - ro.code[q]];
-      ex[q] = p.total_ext[pm[q]];
+      ex[q] = total_ext[pm[q]];
       if (ro.code[q] >= 0 && ro.cell[q] >= 0) {
         const int pt = ro.code[q];
         const int cls = (pt % p.nz) * 4 + (p.cellflags[ro.cell[q]] & 3);
@@ -359,9 +361,110 @@ size_t plan_footprint_bytes(const Plan& p) {
   b += nang * npts * 16;                 // sweep records
   b += nang * (size_t)p.nz * 4 * 16 + 8 * nptsp * 4 + 8 * (size_t)p.nz * 4 * 8;  // step classes, ring-order extinction
   b += 8 * nptsp * 4 + 8 * npts * (4 * 4 + 2) + 8 * (npts / 4 + 8) * 20;  // ring layout, face masks, sectors
-  b += p.ss_off.size() * 4 + p.ss_iph.size() * 4 + p.ss_coef.size() * 4;
+  if (p.host_medium)
+    b += p.ss_off.size() * 4 + p.ss_iph.size() * 4 + p.ss_coef.size() * 4;
+  else   // the device set-up: at most 8 pairs per species and point, its work arrays
+    b += npts * 4 * (1 + 16 * (size_t)p.npart) + npts * 4 * (4 + 3 * ml1 + 18 * (size_t)p.npart) +
+         (size_t)p.d.npx * p.d.npy * p.d.npz * 4 * (4 * (size_t)p.npart + 1) + (npts / p.nz) * (size_t)p.nz * 80;
   b += (size_t)p.d.npx * p.d.npy * p.d.npz * 4 + p.zlevels.size() * 4;   // direct-beam extinction
   return b;
+}
+
+// The per-point medium made on the device (medium.hip) for a plan built for
+// it: total extinction, mixed albedo, source constants, single-scatter
+// constants (CSR), the two-stream start and the direct beam's extinction;
+// the source classes and the total extinction come back for the ring layout.
+static int device_medium(const Plan& p, State& s, std::string& err) {
+  const at3d_scene_desc& d = p.d;
+  const size_t npg = (size_t)d.npx * d.npy * d.npz, np = p.npts, ml1 = p.ml + 1;
+  MediumArgs m;
+  float *extp = nullptr, *albp = nullptr, *wtp = nullptr, *zl = nullptr, *leg = nullptr, *legp = nullptr;
+  int32_t* iphp = nullptr;
+  TRY(upload(s, extp, p.extinctp.data(), p.extinctp.size(), err));
+  TRY(upload(s, albp, p.albedop.data(), p.albedop.size(), err));
+  TRY(upload(s, iphp, p.iphasep.data(), p.iphasep.size(), err));
+  TRY(upload(s, wtp, p.phasewtp.data(), p.phasewtp.size(), err));
+  TRY(upload(s, zl, p.zlevels.data(), p.zlevels.size(), err));
+  TRY(upload(s, leg, p.legen.data(), p.legen.size(), err));
+  TRY(upload(s, legp, p.legenp.data(), p.legenp.size(), err));
+  m.g = PropGrid{extp, albp, iphp, wtp, zl, d.npx, d.npy, d.npz, d.delx, d.dely, p.extmin, p.scatmin};
+  m.t = LegTable{leg, p.nleg + 1, p.ml, p.legen.size()};
+  m.legenp = legp;
+  m.nlegp = p.nlegp;
+  m.npart = p.npart;
+  m.npts = p.npts;
+  m.nz = p.nz;
+  m.ncol = p.npts / p.nz;
+  m.deltam = d.deltam != 0;
+  m.secmu0 = (float)(1.0 / fabs((double)d.solarmu));
+  m.solarflux = d.solarflux;
+  m.solarmu = d.solarmu;
+  m.gndemis = 1.0f - d.gndalbedo;
+  m.skyflux_weight = sky_flux_weight(p);
+  m.gridpos = s.gridpos;
+  m.zgrid = s.zgrid;
+  TRY(alloc0(s, m.extinct, (size_t)p.npart * np, err));
+  TRY(alloc0(s, m.albedo, (size_t)p.npart * np, err));
+  TRY(alloc0(s, m.iphase, (size_t)p.npart * np * 8, err));
+  TRY(alloc0(s, m.wt, (size_t)p.npart * np * 8, err));
+  TRY(alloc0(s, s.ext, (size_t)s.nptsp, err));
+  m.total_ext = s.ext;
+  TRY(alloc0(s, s.alb, np, err));
+  TRY(alloc0(s, s.chi, np * ml1, err));
+  TRY(alloc0(s, s.tc, np * ml1, err));
+  TRY(alloc0(s, s.tms, np * ml1, err));
+  TRY(alloc0(s, s.ss_off, np + 1, err));
+  m.alb = s.alb;
+  m.chi = s.chi;
+  m.tc = s.tc;
+  m.tms = s.tms;
+  m.ss_off = s.ss_off;
+  int32_t *cnt = nullptr, *cls = nullptr, *bsum = nullptr, *total = nullptr;
+  unsigned int* amax = nullptr;
+  TRY(alloc0(s, cnt, np, err));
+  TRY(alloc0(s, cls, np, err));
+  TRY(alloc0(s, bsum, (np + 1023) / 1024 + 1, err));
+  TRY(alloc0(s, total, (size_t)1, err));
+  TRY(alloc0(s, amax, (size_t)1, err));
+  m.ss_cnt = cnt;
+  m.cls = cls;
+  m.albmax = amax;
+  launch_medium_points(s.stream, m, bsum, total);
+  HIPCHK(hipGetLastError());
+  int32_t npairs = 0;
+  unsigned int amax_h = 0;
+  HIPCHK(hipMemcpyAsync(&npairs, total, sizeof npairs, hipMemcpyDeviceToHost, s.stream));
+  HIPCHK(hipMemcpyAsync(&amax_h, amax, sizeof amax_h, hipMemcpyDeviceToHost, s.stream));
+  s.medium_cls_h.resize(np);
+  s.medium_ext_h.resize(np);
+  HIPCHK(hipMemcpyAsync(s.medium_cls_h.data(), cls, np * sizeof(int32_t), hipMemcpyDeviceToHost, s.stream));
+  HIPCHK(hipMemcpyAsync(s.medium_ext_h.data(), s.ext, np * sizeof(float), hipMemcpyDeviceToHost, s.stream));
+  HIPCHK(hipStreamSynchronize(s.stream));
+  memcpy(&s.albmax, &amax_h, sizeof(float));
+  TRY(alloc0(s, s.ss_iph, (size_t)std::max(npairs, 1), err));
+  TRY(alloc0(s, s.ss_coef, (size_t)std::max(npairs, 1), err));
+  m.ss_iph = s.ss_iph;
+  m.ss_coef = s.ss_coef;
+  launch_medium_pairs(s.stream, m);
+  // the two-stream start and the direct beam's extinction
+  const size_t nlayer = p.nz - 1, ncol = m.ncol;
+  TRY(alloc0(s, m.work, ncol * 4 * (2 * nlayer + 3), err));
+  TRY(alloc0(s, m.fwork, ncol * (3 * (size_t)p.nz + 3 * (nlayer + 1)), err));
+  TRY(alloc0(s, s.R0, np * 4, err));
+  TRY(alloc0(s, s.beam_ext, npg, err));
+  m.radiance0 = s.R0;
+  m.beam_ext = s.beam_ext;
+  m.err = s.errflag;
+  launch_medium_start(s.stream, m);
+  HIPCHK(hipGetLastError());
+  int ef = 0;
+  HIPCHK(hipMemcpyAsync(&ef, s.errflag, sizeof ef, hipMemcpyDeviceToHost, s.stream));
+  HIPCHK(hipStreamSynchronize(s.stream));
+  if (ef) {
+    err = "EDDRTF: singular two-stream system";
+    return AT3D_ERR_NUMERIC;
+  }
+  return AT3D_OK;
 }
 
 int state_create(const Plan& p, State& s, std::string& err) {
@@ -416,12 +519,15 @@ int state_create(const Plan& p, State& s, std::string& err) {
   std::copy(p.ylmsun.begin(), p.ylmsun.end(), ysun.begin());
   TRY(upload(s, s.ysun, ysun.data(), ysun.size(), err));
   TRY(upload(s, s.lofj, p.lofj.data(), p.lofj.size(), err));
-  std::vector<float> ext(s.nptsp, 0.f);
-  std::copy(p.total_ext.begin(), p.total_ext.end(), ext.begin());
-  TRY(upload(s, s.ext, ext.data(), ext.size(), err));
-  TRY(upload(s, s.alb, p.alb.data(), p.alb.size(), err));
-  TRY(upload(s, s.chi, p.chi.data(), p.chi.size(), err));
-  TRY(upload(s, s.tc, p.tc.data(), p.tc.size(), err));
+  if (!p.medium_on_device) {
+    std::vector<float> ext(s.nptsp, 0.f);
+    std::copy(p.total_ext.begin(), p.total_ext.end(), ext.begin());
+    TRY(upload(s, s.ext, ext.data(), ext.size(), err));
+    TRY(upload(s, s.alb, p.alb.data(), p.alb.size(), err));
+    TRY(upload(s, s.chi, p.chi.data(), p.chi.size(), err));
+    TRY(upload(s, s.tc, p.tc.data(), p.tc.size(), err));
+    s.albmax = p.albmax;
+  }
   TRY(alloc0(s, s.dirflux, (size_t)p.npts, err));   // made by the direct-beam pass below
   TRY(alloc0(s, s.f0, (size_t)p.npts, err));
   const size_t nsh = (size_t)s.nptsp * s.ld, ndo = (size_t)s.nptsp * s.ldo;
@@ -441,11 +547,16 @@ int state_create(const Plan& p, State& s, std::string& err) {
   TRY(alloc0(s, s.errflag, (size_t)1, err));
   // sweep geometry
   TRY(upload(s, s.gridpos, p.gridpos.data(), p.gridpos.size(), err));
+  TRY(upload(s, s.zgrid, p.zgrid.data(), p.zgrid.size(), err));
+  if (p.medium_on_device) TRY(device_medium(p, s, err));
   {
     // MAKE_DIRECT on the device: the direct flux of every grid point and F0
     std::vector<float> extdirp;
     DirectBeamGeom beam;
-    direct_beam_setup(p, extdirp, beam);
+    if (p.medium_on_device)
+      direct_beam_geometry(p, beam);
+    else
+      direct_beam_setup(p, extdirp, beam);
     // a sun within 1e-6 of the horizon never reaches the top (the trace runs
     // out of steps): the reference's error, without the wait
     if (beam.cz == 0.0) {
@@ -458,9 +569,9 @@ int state_create(const Plan& p, State& s, std::string& err) {
     const double run = zspan / beam.cz;
     const double steps = p.d.npz + 8.0 + run * (fabs(beam.cx) / beam.delxd + fabs(beam.cy) / beam.delyd);
     beam.max_steps = (long long)std::min(1.0e8, 2.0 * steps + 64.0);
-    float* d_ext = nullptr;
+    float* d_ext = s.beam_ext;
     float* d_zl = nullptr;
-    TRY(upload(s, d_ext, extdirp.data(), extdirp.size(), err));
+    if (!p.medium_on_device) TRY(upload(s, d_ext, extdirp.data(), extdirp.size(), err));
     TRY(upload(s, d_zl, p.zlevels.data(), p.zlevels.size(), err));
     beam.ext = d_ext;
     beam.zl = d_zl;
@@ -480,7 +591,6 @@ int state_create(const Plan& p, State& s, std::string& err) {
   TRY(upload(s, s.cellflags, p.cellflags.data(), p.cellflags.size(), err));
   TRY(upload(s, s.xgrid, p.xgrid.data(), p.xgrid.size(), err));
   TRY(upload(s, s.ygrid, p.ygrid.data(), p.ygrid.size(), err));
-  TRY(upload(s, s.zgrid, p.zgrid.data(), p.zgrid.size(), err));
   std::vector<OrdGeom> og(p.nang);
   std::vector<float> sky(p.nang, 0.f), fluxw(p.nang);
   for (int k = 0; k < p.nang; k++) {
@@ -526,10 +636,12 @@ int state_create(const Plan& p, State& s, std::string& err) {
     for (int c = s.oct_col_base[o]; c < s.oct_col_base[o] + s.oct_nord[o]; c++) coct[c] = (int8_t)o;
   TRY(upload(s, s.col_oct, coct.data(), coct.size(), err));
   // rendering constants
-  TRY(upload(s, s.tms, p.tms.data(), p.tms.size(), err));
-  TRY(upload(s, s.ss_off, p.ss_off.data(), p.ss_off.size(), err));
-  TRY(upload(s, s.ss_iph, p.ss_iph.data(), p.ss_iph.size(), err));
-  TRY(upload(s, s.ss_coef, p.ss_coef.data(), p.ss_coef.size(), err));
+  if (!p.medium_on_device) {
+    TRY(upload(s, s.tms, p.tms.data(), p.tms.size(), err));
+    TRY(upload(s, s.ss_off, p.ss_off.data(), p.ss_off.size(), err));
+    TRY(upload(s, s.ss_iph, p.ss_iph.data(), p.ss_iph.size(), err));
+    TRY(upload(s, s.ss_coef, p.ss_coef.data(), p.ss_coef.size(), err));
+  }
   // the plan holds them per unit direct flux
   launch_direct_scale(s.stream, s.dirflux, p.npts, p.ml + 1, s.tms, s.ss_off, s.ss_coef);
   HIPCHK(hipGetLastError());
@@ -537,7 +649,7 @@ int state_create(const Plan& p, State& s, std::string& err) {
   std::vector<int32_t> bot_index(p.npts, -1);
   for (size_t b = 0; b < p.bcbot.size(); b++) bot_index[p.bcbot[b]] = (int32_t)b;
   TRY(upload(s, s.bot_index, bot_index.data(), bot_index.size(), err));
-  TRY(upload(s, s.R0, p.radiance0.data(), p.radiance0.size(), err));
+  if (!p.medium_on_device) TRY(upload(s, s.R0, p.radiance0.data(), p.radiance0.size(), err));
   if (hipHostMalloc((void**)&s.host_dots, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     s.host_dots = nullptr;
     err = "hipHostMalloc of the iteration readback buffer failed";
@@ -681,6 +793,8 @@ int state_set_grid(State& s, int npts, int ncells, const float* gridpos, const i
     return state_reset_grid(s, err);
   }
   auto q = std::make_shared<Plan>(b);
+  q->medium_on_device = false;   // the loaded grid's medium is made on the host
+  q->host_medium = true;
   TRY(plan_set_grid(*q, npts, ncells, gridpos, gridptr, neighptr, treeptr, cellflags, err));
   return state_replace(s, q, nullptr, err);
 }
@@ -953,7 +1067,7 @@ static int state_solve_graph(State& s, int maxiter, std::string& err) {
   c.maxiter = maxiter;
   c.iterfixsh = p.d.iterfixsh;
   c.accel = p.d.accelflag ? 1 : 0;
-  c.one_iter = p.albmax < p.d.solacc ? 1 : 0;
+  c.one_iter = s.albmax < p.d.solacc ? 1 : 0;
   c.solacc = p.d.solacc;
   HIPCHK(hipMemcpyAsync(s.ctl, &c, sizeof c, hipMemcpyHostToDevice, s.stream));
   const int parity = s.nr == s.nr_buf0 ? 0 : 1;
@@ -1065,7 +1179,7 @@ static int host_iteration(State& s, bool& fixsh, float& a, int verbose, std::str
   float accelpar;
   accel_solcrit(p.d.accelflag != 0, s.deljdot, s.deljold, s.deljnew, s.jnorm, a, accelpar, s.solcrit);
   if (accelpar > 0.0f) launch_accelerate(s.stream, s.S, s.DS, s.ns, s.nos, s.ld, p.npts, accelpar);
-  if (p.albmax < p.d.solacc) s.solcrit = p.d.solacc;
+  if (s.albmax < p.d.solacc) s.solcrit = p.d.solacc;
   if (verbose) {
     printf("  %4d %8.3f %10.3e %8d\n", s.iters, log10f(std::max(s.solcrit, 1.0e-20f)), s.splitcrit, p.npts);
     fflush(stdout);
@@ -1078,6 +1192,7 @@ static int host_iteration(State& s, bool& fixsh, float& a, int verbose, std::str
 // refined there, and the state moves to the refined grid when it changed.
 static int split_step(State& s, bool dosplit, SplitSchedule& sc, std::string& err) {
   const Plan& p = *s.plan;
+  TRY(ensure_host_medium(p, err));   // the splitting reads and extends the host optics
   const size_t np = p.npts, ld = s.ld;
   SplitFields f;
   f.ld = s.ld;
@@ -1102,6 +1217,7 @@ static int split_step(State& s, bool dosplit, SplitSchedule& sc, std::string& er
     return AT3D_OK;
   }
   auto q = std::make_shared<Plan>(p);
+  q->medium_on_device = false;   // the refined grid's medium is made on the host
   bool outofmem = sc.outofmem;
   float crit = 0.0f;
   TRY(split_grid(*q, f, true, outofmem, sc.cursplitacc, crit, err));

@@ -29,6 +29,10 @@ struct State {
   const Plan* base_plan = nullptr;
   int32_t* treeptr = nullptr;   // device [ncells][2] of a refined grid, else null
   float splitcrit = 0.0f;       // SPLITCRIT of the last splitting test
+  float albmax = 0.0f;          // the largest scattering albedo (a single iteration below SOLACC)
+  float* beam_ext = nullptr;    // device medium: the direct beam's property-grid extinction
+  std::vector<int32_t> medium_cls_h;   // device medium: source classes and total extinction
+  std::vector<float> medium_ext_h;     // of every point (for the ring layout)
   int device = 0;   // HIP device the state was created on
   hipStream_t stream = nullptr;
   int npts = 0, nlm = 0, nang = 0;

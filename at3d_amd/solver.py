@@ -30,7 +30,7 @@ class RTE:
     """Radiative transfer solver (at3d/solver.py:143)."""
 
     def __init__(self, numerical_params, medium, source, surface, num_stokes=1, name=None,
-                 atmosphere=None):
+                 atmosphere=None, device_setup=True):
         if num_stokes != 1:
             raise ValueError("only num_stokes=1 is supported by this engine")
         if atmosphere is not None:
@@ -52,6 +52,10 @@ class RTE:
         self._plan = None
         self._restore_data = None   # a loaded solution (load_solution)
         self._keep = []          # arrays referenced by the C descriptor
+        # the per-point medium (property transfer, delta-M, constants,
+        # two-stream start) made on the device with the solver state rather
+        # than on the host with the plan; the same values either way
+        self._device_setup = bool(device_setup)
         self._build_plan()
 
     # -- setup -----------------------------------------------------------------
@@ -125,6 +129,7 @@ class RTE:
         d.tautol = float(npar["tautol"])
         d.transcut = float(npar["transcut"])
         d.maxig, d.maxic, d.maxiv, d.maxido = self._memory_params(npar, npx, npy, npz, table)
+        d.device_medium = 1 if self._device_setup else 0
         self._keep = [ext, alb, iph, pw, zl, legenp]
         self._desc = d
         L = _lib.lib()
@@ -545,7 +550,20 @@ class RTE:
         dirflux [npts]; ns, nr [npts]; do_radiance, do_source [nang, npts]."""
         if self._state is None:
             raise RuntimeError("solve() has not been called")
-        shapes = {"source": ((self._npts, self._nlm), np.float32),
+        n, ml1 = self._npts, self._ml + 1
+        if name in ("point_ss_iph", "point_ss_coef"):
+            off = self.state_array("point_ss_off")
+            shp, dt = (int(off[-1]),), (np.int32 if name == "point_ss_iph" else np.float32)
+            out = np.zeros(shp, dt)
+            err = _lib.errbuf()
+            _lib.check(_lib.lib().at3d_state_get(self._state, name.encode(), out.ctypes.data,
+                                                 out.nbytes, err, len(err)), err)
+            return out
+        shapes = {"point_ext": ((n,), np.float32), "point_alb": ((n,), np.float32),
+                  "point_chi": ((n, ml1), np.float32), "point_tc": ((n, ml1), np.float32),
+                  "point_tms": ((n, ml1), np.float32), "point_radiance0": ((n, 4), np.float32),
+                  "point_ss_off": ((n + 1,), np.int32),
+                  "source": ((self._npts, self._nlm), np.float32),
                   "radiance": ((self._npts, self._nlm), np.float32),
                   "delsource": ((self._npts, self._nlm), np.float32),
                   "fluxes": ((2, self._npts), np.float32), "dirflux": ((self._npts,), np.float32),

@@ -113,7 +113,7 @@ def workload(name: str) -> Workload:
     return w
 
 
-def make_rte(w: Workload, **overrides):
+def make_rte(w: Workload, device_setup=True, **overrides):
     """An RTE for a workload (solar source, fixed Lambertian surface)."""
     from . import configuration, source, surface
     from .medium import Scatterer
@@ -127,7 +127,7 @@ def make_rte(w: Workload, **overrides):
     med = {f"species_{i}": Scatterer(e, a, leg, w.delx, w.dely, w.z)
            for i, (e, a, leg) in enumerate(w.species)}
     return RTE(cfg, med, source.solar(w.wavelength, w.solarmu, w.solaraz_deg),
-               surface.lambertian(w.gndalbedo))
+               surface.lambertian(w.gndalbedo), device_setup=device_setup)
 
 
 # Serving stream: grids, (NMU, NPHI) angle sets (L = NMU - 1) and wavelengths

@@ -69,6 +69,12 @@ typedef struct at3d_scene_desc {
    * splitting stops where the reference's would run out of them.  0 or less
    * than the base grid's: the base grid's sizes (no room to split). */
   int32_t maxig, maxic, maxiv, maxido;
+  /* 1: the per-point medium (property transfer, delta-M, source and
+   * single-scatter constants, the two-stream start) is computed on the
+   * device by at3d_state_create instead of on the host by at3d_plan_create;
+   * the plan's per-point arrays are then made on the host only when asked
+   * for.  Same values either way. */
+  int32_t device_medium;
 } at3d_scene_desc;
 
 typedef struct at3d_plan at3d_plan;   /* host-side setup (no GPU needed)     */
@@ -148,7 +154,12 @@ int at3d_solve(at3d_state* st, int32_t maxiter, int32_t verbose, float* solcrit_
  * i32[ncells][6] (0-based: c an end cell, -1 the domain boundary, -(c+2) the
  * reference's negative pointer to a split cell c), "treeptr" i32[ncells][2]
  * (parent, first child; -1 none), "cellflags" i8[ncells] (bits 2-3: split
- * direction), "total_ext" f32[npts], "extinct" / "albedo" f32[npart][npts]. */
+ * direction), "total_ext" f32[npts], "extinct" / "albedo" f32[npart][npts];
+ * and the state's own per-point medium as the solve uses it: "point_ext"
+ * f32[npts], "point_alb" f32[npts], "point_chi" / "point_tc" /
+ * "point_tms" f32[npts][ml+1] (tms times the direct flux),
+ * "point_radiance0" f32[npts][4], "point_ss_off" i32[npts+1],
+ * "point_ss_iph" i32[n], "point_ss_coef" f32[n] (n = point_ss_off[npts]). */
 int at3d_state_get(at3d_state* st, const char* name, void* dst, size_t nbytes, char* err,
                    size_t errlen);
 

@@ -27,6 +27,7 @@ extern "C" int split_hook(const at3d_scene_desc* desc, int ld, const float* S, c
   std::string msg;
   at3d::Plan p;
   int rc = at3d::build_plan(*desc, p, msg);
+  if (!rc) rc = at3d::ensure_host_medium(p, msg);   // the splitting reads the host optics
   at3d::SplitFields f;
   f.ld = ld;
   if (!rc) {
