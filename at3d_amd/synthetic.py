@@ -16,7 +16,7 @@ class Workload:
     delx: float
     dely: float
     z: np.ndarray
-    species: list = field(default_factory=list)   # list of (ext, ssalb, legcoef[nleg+1, ntab])
+    species: list = field(default_factory=list)   # list of (ext, ssalb, legcoef[(6,) nleg+1, ntab])
     bc: str = "open"
     nmu: int = 16
     nphi: int = 32
@@ -27,6 +27,7 @@ class Workload:
     gndalbedo: float = 0.05
     wavelength: float = 0.86
     deltam: bool = True
+    nstokes: int = 1    # 1, or 3 for a polarized solve (legcoef then (6, nleg+1, ntab))
 
     @property
     def shape(self):
@@ -99,6 +100,70 @@ def cloud_field(nx=64, ny=64, nz=40, dx=0.02, dz=0.025, z0=0.0, seed=0, cover=0.
     return Workload("cloud_field", dx, dx, z,
                     [(ext, alb, cloud_leg[:, None]), (rext, ralb, rleg[:, None])],
                     bc="open", nmu=nmu, nphi=nphi, solacc=solacc)
+
+
+def rayleigh_phase_matrix(depol=0.0, nleg=2):
+    """The six Legendre expansions (a1, a2, a3, a4, b1, b2, each including
+    the 2l+1 factor) of the Rayleigh phase matrix with depolarization factor
+    `depol`: a1 = (1, 0, D/2), a2_2 = 3D, a4_1 = 3 D D'/2, b1_2 = sqrt(6) D/2
+    with D = (1 - depol)/(1 + depol/2) and D' = (1 - 2 depol)/(1 - depol)
+    (Hovenier, van der Mee and Domke, 2004).  Returns (6, nleg+1, 1)."""
+    dd = (1.0 - depol) / (1.0 + 0.5 * depol)
+    dp = (1.0 - 2.0 * depol) / (1.0 - depol)
+    t = np.zeros((6, max(nleg, 2) + 1, 1))
+    t[0, 0, 0] = 1.0
+    t[0, 2, 0] = 0.5 * dd
+    t[1, 2, 0] = 3.0 * dd
+    t[3, 1, 0] = 1.5 * dd * dp
+    t[4, 2, 0] = np.sqrt(6.0) * 0.5 * dd
+    return t
+
+
+def polarized_cloud_table(g=0.85, nleg=199, seed=0):
+    """A smooth decaying six-element phase matrix expansion (Henyey-Greenstein
+    a1 damped by exp(-l/30); a2 = a3 and a4 proportional to it from l = 2 and
+    l = 1; b1, b2 negative and decaying faster), with seeded proportions.
+    Returns (6, nleg+1, 1)."""
+    rng = np.random.default_rng(seed)
+    l = np.arange(nleg + 1, dtype=np.float64)
+    a1 = np.exp(-l / 30.0) * (2 * l + 1) * g ** l
+    a1[0] = 1.0
+    c2, c3, c4 = rng.uniform(0.85, 0.95), rng.uniform(0.8, 0.9), rng.uniform(0.75, 0.85)
+    cb1, cb2 = rng.uniform(0.05, 0.15), rng.uniform(0.0, 0.05)
+    t = np.zeros((6, nleg + 1, 1))
+    t[0, :, 0] = a1
+    t[1, 2:, 0] = c2 * a1[2:]
+    t[2, 2:, 0] = c3 * a1[2:]
+    t[3, 1:, 0] = c4 * a1[1:]
+    t[4, 2:, 0] = -cb1 * a1[2:] * np.exp(-(l[2:] - 2) / 10.0)
+    t[5, 2:, 0] = -cb2 * a1[2:] * np.exp(-(l[2:] - 2) / 10.0)
+    return t
+
+
+def polarized_cloud_field(nx=128, ny=128, nz=64, dx=0.02, dz=0.02, z0=0.0, seed=0, wavelength=0.66,
+                          ext_scale=30.0, cloud_base=0.3, cloud_top=1.0, ssalb=0.9995, g=0.85,
+                          nleg_table=199, depol=0.03, nmu=16, nphi=32, solacc=1e-4):
+    """Polarized medium (NSTOKES = 3): the cloud of cloud_field on this grid
+    with a six-element table (polarized_cloud_table) and its extinction
+    scaled by (0.86 / wavelength), plus Rayleigh scattering with
+    depolarization `depol` whose extinction 0.0116 / km at 0.86 um scales as
+    wavelength^-4.  The grid spans z0 .. z0 + (nz - 1) dz km; the cloud
+    layer is clipped to it."""
+    z = (z0 + np.arange(nz) * dz).astype(np.float32)
+    f = gaussian_field((nx, ny, nz), seed=seed)
+    thr = np.quantile(f, 0.6)
+    cl = np.clip(f - thr, 0.0, None)
+    cl = cl / max(cl.max(), 1e-12)
+    layer = (z >= cloud_base) & (z <= cloud_top)
+    scale = 0.86 / wavelength
+    ext = (ext_scale * scale * cl ** 1.5 * layer[None, None, :]).astype(np.float32)
+    alb = np.full(ext.shape, ssalb, np.float32)
+    rext = np.full(ext.shape, 0.0116 * (0.86 / wavelength) ** 4, np.float32)
+    ralb = np.ones(ext.shape, np.float32)
+    return Workload("polarized_cloud", dx, dx, z,
+                    [(ext, alb, polarized_cloud_table(g, nleg_table, seed)),
+                     (rext, ralb, rayleigh_phase_matrix(depol))],
+                    bc="open", nmu=nmu, nphi=nphi, solacc=solacc, wavelength=wavelength, nstokes=3)
 
 
 def workload(name: str) -> Workload:

@@ -127,7 +127,9 @@ def _check(ierr, errmsg, where):
 class Species:
     extinction: np.ndarray          # (npx, npy, npz)
     ssalb: np.ndarray               # (npx, npy, npz)
-    legcoef: np.ndarray             # (nleg+1, ntable) scalar I-I coefficients incl. (2l+1)
+    legcoef: np.ndarray             # (nleg+1, ntable) scalar I-I coefficients incl. (2l+1),
+                                    # or (6, nleg+1, ntable): the six phase-matrix elements
+                                    # a1, a2, a3, a4, b1, b2 (medium.py stokes_index order)
     table_index: np.ndarray | None = None   # (npx,npy,npz) 1-based, default 1
     phase_weights: np.ndarray | None = None
 
@@ -165,6 +167,7 @@ class Scene:
     transcut: float = 1e-5
     transmin: float = 1.0
     ip_flag: int = 0
+    nstokes: int = 1                # 1 or 3 (num_stokes)
 
     @property
     def shape(self):
@@ -226,6 +229,8 @@ def setup(sc: Scene) -> RefState:
     mm = max(0, int(nphi / 2) - 1)
     nlm = (2 * mm + 1) * (ml + 1) - mm * (mm + 1)
     nphi0max = nphi
+    nstokes = sc.nstokes
+    nstleg = 1 if nstokes == 1 else 6
     maxig, maxic, maxiv = nbpts, max(int(1.5 * nbpts), nbcells), max(nlm * nbpts, nbpts * 4)
     if sc.splitacc > 0.0:
         import psutil
@@ -256,7 +261,7 @@ def setup(sc: Scene) -> RefState:
           npts, ncells, xgrid, ygrid, zgrid, gridpos, i4(maxic), i4(maxig),
           gridptr, neighptr, treeptr, cellflags)
     npts, ncells = npts.value, ncells.value
-    d.update(nmu=nmu, nphi=nphi, ml=ml, mm=mm, nlm=nlm, nphi0max=nphi0max, nx=nx, ny=ny, nz=nz,
+    d.update(nstokes=nstokes, nstleg=nstleg, nmu=nmu, nphi=nphi, ml=ml, mm=mm, nlm=nlm, nphi0max=nphi0max, nx=nx, ny=ny, nz=nz,
              nx1=nx1, ny1=ny1, npx=npx, npy=npy, npz=npz, bcflag=bcflag, ipflag=ipflag,
              nbpts=nbpts, nbcells=nbcells, maxpg=maxpg, maxig=maxig, maxic=maxic, maxiv=maxiv,
              maxido=maxido, maxnbc=maxnbc, xgrid=xgrid, ygrid=ygrid, zgrid=zgrid, zlevels=zlev,
@@ -266,8 +271,18 @@ def setup(sc: Scene) -> RefState:
     return st
 
 
+def _legcoef3(leg, nstleg):
+    """A species' table as (nstleg, nleg+1, ntable): a scalar (nleg+1, ntable)
+    table is the I-I element with the others zero; a six-element table keeps
+    only its first element for a scalar solve (solver.py:2407-2411)."""
+    leg = np.asarray(leg, np.float64)
+    if leg.ndim == 2:
+        leg = np.concatenate([leg[None], np.zeros((5,) + leg.shape)], axis=0)
+    return leg[:nstleg]
+
+
 def _prepare_optical_properties(st: RefState):
-    """solver.py:2324-2528 for NSTOKES=1."""
+    """solver.py:2324-2528."""
     sc, d = st.sc, st.d
     maxpg, npart = d["maxpg"], len(sc.species)
     extinctp = np.zeros((maxpg, npart), np.float32, order="F")
@@ -285,16 +300,16 @@ def _prepare_optical_properties(st: RefState):
         pw = np.ones(maxpg, np.float32) if sp.phase_weights is None else \
             np.asarray(sp.phase_weights, np.float32).ravel()
         phasewtp[0, :, i] = pw
-        tables.append(np.asarray(sp.legcoef, np.float64))
+        tables.append(_legcoef3(sp.legcoef, d["nstleg"]))
     iphasep[iphasep == 0] = 1
-    maxleg = max(t.shape[0] for t in tables)
-    table = np.concatenate([np.pad(t, ((0, maxleg - t.shape[0]), (0, 0))) for t in tables], axis=1)
-    numphase = table.shape[1]
+    maxleg = max(t.shape[1] for t in tables)
+    table = np.concatenate([np.pad(t, ((0, 0), (0, maxleg - t.shape[1]), (0, 0))) for t in tables], axis=2)
+    numphase = table.shape[2]
     nleg = d["ml"] + 1 if sc.deltam else d["ml"]
-    nlegp = max(table.shape[0] - 1, nleg)
+    nlegp = max(table.shape[1] - 1, nleg)
     nscatangle = max(36, min(721, 2 * nlegp))
-    if nlegp + 1 > table.shape[0]:
-        table = np.pad(table, ((0, 1 + nlegp - table.shape[0]), (0, 0)))
+    if nlegp + 1 > table.shape[1]:
+        table = np.pad(table, ((0, 0), (0, 1 + nlegp - table.shape[1]), (0, 0)))
     legenp = farr(table.ravel(order="F"), np.float32)
     tempp = np.zeros(maxpg, np.float32) + 273.0
     maxig = d["maxig"]
@@ -302,7 +317,7 @@ def _prepare_optical_properties(st: RefState):
     planck = np.zeros((maxig, npart), np.float32, order="F")
     extinct = np.zeros((maxig, npart), np.float32, order="F")
     albedo = np.zeros((maxig, npart), np.float32, order="F")
-    legen = np.zeros((1, nleg + 1, numphase), np.float32, order="F")
+    legen = np.zeros((d["nstleg"], nleg + 1, numphase), np.float32, order="F")
     iphase = np.zeros((8 * maxnmicro, maxig, npart), np.int32, order="F")
     total_ext = np.zeros(maxig, np.float32, order="F")
     phaseinterpwt = np.zeros((8 * maxnmicro, maxig, npart), np.float32, order="F")
@@ -315,7 +330,7 @@ def _prepare_optical_properties(st: RefState):
           temp, planck, extinct, albedo, legen, iphase, i4(d["npts"]), d["gridpos"],
           i4(d["npx"]), i4(d["npy"]), i4(d["npz"]), r4(sc.delx), r4(sc.dely), r4(0.0), r4(0.0),
           d["zlevels"], tempp, extinctp, i4(maxpg), albedop, legenp, iphasep, i4(0), zckd, gasabs,
-          i4(npart), total_ext, extmin, scatmin, albmax, i4(1), s("ON"), ierr, errmsg,
+          i4(npart), total_ext, extmin, scatmin, albmax, i4(d["nstleg"]), s("ON"), ierr, errmsg,
           phaseinterpwt, r4(0.999), i4(nlegp), phasewtp, i4(maxnmicro))
     _check(ierr, errmsg, "TRANSFER_PA_TO_GRID")
     d.update(extinctp=extinctp, albedop=albedop, iphasep=iphasep, phasewtp=phasewtp,
@@ -340,13 +355,13 @@ def make_angle_set(nmu, nphi, itype, nphi0max):
                 nang=nang.value)
 
 
-def make_sh_do_coef(ml, mm, nlm, nmu, nphi0, nphi0max, fftflag, mu, phi, wtmu, wtdo):
-    cmu1 = np.zeros((1, nlm, nmu), np.float32, order="F")
-    cmu2 = np.zeros((1, nmu, nlm), np.float32, order="F")
+def make_sh_do_coef(ml, mm, nlm, nmu, nphi0, nphi0max, fftflag, mu, phi, wtmu, wtdo, nstleg=1):
+    cmu1 = np.zeros((nstleg, nlm, nmu), np.float32, order="F")
+    cmu2 = np.zeros((nstleg, nmu, nlm), np.float32, order="F")
     cphi1 = np.zeros((33, 32, nmu), np.float32, order="F")
     cphi2 = np.zeros((32, 33, nmu), np.float32, order="F")
     wphisave = np.zeros((3 * nphi0max + 15, nmu), np.float32, order="F")
-    fcall("MAKE_SH_DO_COEF", i4(1), i4(ml), i4(mm), i4(nlm), i4(nmu), nphi0, i4(nphi0max),
+    fcall("MAKE_SH_DO_COEF", i4(nstleg), i4(ml), i4(mm), i4(nlm), i4(nmu), nphi0, i4(nphi0max),
           fftflag, mu, phi, wtmu, wtdo, cmu1, cmu2, cphi1, cphi2, wphisave)
     return dict(cmu1=cmu1, cmu2=cmu2, cphi1=cphi1, cphi2=cphi2, wphisave=wphisave)
 
@@ -358,26 +373,27 @@ def init_solution(st: RefState, restore=None):
     fluxes as RTE.save_solution writes them); its arrays are placed in the
     state and INIT_SOLUTION runs with INRADFLAG false (solver.py:2654-2664)."""
     sc, d = st.sc, st.d
-    nmu, nphi0max, nstokes = d["nmu"], d["nphi0max"], 1
+    nmu, nphi0max, nstokes, nstleg = d["nmu"], d["nphi0max"], d["nstokes"], d["nstleg"]
     ang = make_angle_set(nmu, d["nphi"], sc.angle_set, nphi0max)
     d.update(ang)
     d.update(make_sh_do_coef(d["ml"], d["mm"], d["nlm"], nmu, ang["nphi0"], nphi0max,
-                             ang["fftflag"], ang["mu"], ang["phi"], ang["wtmu"], ang["wtdo"]))
+                             ang["fftflag"], ang["mu"], ang["phi"], ang["wtmu"], ang["wtdo"], nstleg))
     nang = ang["nang"]
-    skyrad = np.full((1, nmu // 2, nphi0max), sc.skyrad, np.float32, order="F")
+    skyrad = np.zeros((nstokes, nmu // 2, nphi0max), np.float32, order="F")
+    skyrad[0] = sc.skyrad
     surface_rad = np.zeros((nang // 2 + 1, d["nx1"] + 1, d["ny1"] + 1), np.float32, order="F")
     maxig, maxiv, maxic, maxido = d["maxig"], d["maxiv"], d["maxic"], d["maxido"]
     maxnbc, maxbcrad, maxpg = d["maxnbc"], d["maxbcrad"], d["maxpg"]
-    source = np.zeros((1, maxiv), np.float32, order="F")
+    source = np.zeros((nstokes, maxiv), np.float32, order="F")
     ndel = maxiv if sc.accelflag else 1
-    delsource = np.zeros((1, ndel), np.float32, order="F")
-    radiance = np.zeros((1, maxiv + maxig), np.float32, order="F")
+    delsource = np.zeros((nstokes, ndel), np.float32, order="F")
+    radiance = np.zeros((nstokes, maxiv + maxig), np.float32, order="F")
     dirflux = np.zeros(maxig, np.float32, order="F")
     work1 = np.zeros(8 * maxig, np.int32, order="F")
-    work = np.zeros(maxido, np.float32, order="F")
-    work2_size = max(maxig, maxic)
+    work = np.zeros(maxido * nstokes, np.float32, order="F")
+    work2_size = max(maxig * nstokes, maxic)
     work2 = np.zeros(work2_size, np.float32, order="F")
-    bcrad = np.zeros((1, maxbcrad), np.float32, order="F")
+    bcrad = np.zeros((nstokes, maxbcrad), np.float32, order="F")
     fluxes = np.zeros((2, maxig), np.float32, order="F")
     shptr = np.zeros(maxig + 1, np.int32)
     rshptr = np.zeros(maxig + 2, np.int32)
@@ -387,15 +403,15 @@ def init_solution(st: RefState, restore=None):
         npts = d["npts"]
         shptr[:npts + 1] = restore["shptr"][:npts + 1]
         rshptr[:npts + 2] = restore["rshptr"][:npts + 2]
-        source[:, :shptr[npts]] = np.reshape(restore["source"], (1, -1))[:, :shptr[npts]]
-        radiance[:, :rshptr[npts]] = np.reshape(restore["radiance"], (1, -1))[:, :rshptr[npts]]
+        source[:, :shptr[npts]] = np.reshape(restore["source"], (nstokes, -1))[:, :shptr[npts]]
+        radiance[:, :rshptr[npts]] = np.reshape(restore["radiance"], (nstokes, -1))[:, :rshptr[npts]]
         fluxes[:, :npts] = restore["fluxes"][:, :npts]
     maxsfcpars = 4
     sfcgridparms = np.zeros(maxsfcpars * maxnbc, np.float32)
     bcptr = np.zeros((maxnbc, 2), np.int32, order="F")
-    ylmsun = np.zeros((1, d["nlm"]), np.float32, order="F")
+    ylmsun = np.zeros((nstleg, d["nlm"]), np.float32, order="F")
     extdirp = np.zeros(maxpg, np.float32)
-    sfc_brdf_do = np.zeros((1, nmu // 2, nphi0max, 1, nmu // 2, nphi0max), np.float32, order="F")
+    sfc_brdf_do = np.zeros((nstokes, nmu // 2, nphi0max, nstokes, nmu // 2, nphi0max), np.float32, order="F")
     ntoppts, nbotpts = i4(), i4()
     uniformzlev = r8()
     cx, cy, cz, cxinv, cyinv, czinv = r8(), r8(), r8(), r8(), r8(), r8()
@@ -406,7 +422,7 @@ def init_solution(st: RefState, restore=None):
     ierr, errmsg = i4(), s("", 600)
     longest = i4()
     nang_ref = i4(nang)
-    volsrc = np.zeros((1, 1), np.float32, order="F")
+    volsrc = np.zeros((nstokes, 1), np.float32, order="F")
     volsrcgridptr = np.zeros((2, maxig), np.int32, order="F")
     volsrcshptr = np.zeros(2, np.int32)
     sfcparms = np.zeros(1, np.float32)
@@ -430,7 +446,7 @@ def init_solution(st: RefState, restore=None):
           ipdirect, di, dj, dk, i4(nphi0max), epss, epsz, xdomain, ydomain, delxd, delyd,
           deljdot, deljold, deljnew, jnorm, i4(d["nbcells"]), ang["fftflag"], d["cmu1"],
           d["cmu2"], ang["wtmu"], d["cphi1"], d["cphi2"], d["wphisave"], i4(maxsfcpars),
-          work, work1, work2, i4(1), i4(1), uniform_sfc_brdf, sfc_brdf_do, lg(inradflag), i4(ndel),
+          work, work1, work2, i4(nstleg), i4(nstokes), uniform_sfc_brdf, sfc_brdf_do, lg(inradflag), i4(ndel),
           ierr, errmsg, i4(maxpg), i4(work2_size), d["phaseinterpwt"], r4(0.999), s("ON"),
           i4(d["nlegp"]), lg(False), i4(d["maxnmicro"]), d["phasewtp"], i4(sc.angle_set),
           lg(True), longest, r4(0.0), surface_rad, volsrc, volsrcgridptr, volsrcshptr, i4(1))
@@ -466,9 +482,9 @@ def solution_iterations(st: RefState, maxiter: int, verbose=False):
     ierr, errmsg = i4(), s("", 600)
     comptime, splitcrit = r4(), r4()
     fcall("SOLUTION_ITERATIONS",
-          i4(1), i4(d["nx"]), i4(d["ny"]), i4(d["nz"]), i4(d["nx1"]), i4(d["ny1"]),
+          i4(d["nstokes"]), i4(d["nx"]), i4(d["ny"]), i4(d["nz"]), i4(d["nx1"]), i4(d["ny1"]),
           i4(d["nang"]), i4(d["ml"]), i4(d["mm"]), i4(2), i4(d["nlm"]), i4(nmu), i4(d["nphi"]),
-          i4(d["nleg"]), i4(1), i4(d["numphase"]), d["nphi0"], d["mu"], d["phi"], d["wtdo"],
+          i4(d["nleg"]), i4(d["nstleg"]), i4(d["numphase"]), d["nphi0"], d["mu"], d["phi"], d["wtdo"],
           i4(d["maxiv"]), i4(d["maxic"]), i4(d["maxig"]), i4(d["maxido"]), i4(d["bcflag"]),
           i4(d["ipflag"]), lg(sc.deltam), s("S"), lg(sc.high_order_radiance),
           r4(sc.solarflux), r4(sc.solarmu), r4(sc.solaraz), d["skyrad"], s("FL"),
@@ -518,8 +534,8 @@ def split_grid(st: RefState, dosplit: bool, cursplitacc: float, outofmem: bool =
           lg(dosplit), oom, r4(cursplitacc), splitcrit,
           npts, ncells, d["gridptr"], d["neighptr"], d["treeptr"], d["cellflags"],
           d["gridpos"], i4(d["nx"]), d["xgrid"], i4(d["ny"]), d["ygrid"],
-          i4(1), i4(d["ml"]), i4(d["mm"]), i4(d["nlm"]), i4(1), i4(d["nleg"]), i4(d["numphase"]),
-          lg(sc.deltam), i4(d["bcflag"]), i4(d["ipflag"]), lg(sc.accelflag),
+          i4(d["nstokes"]), i4(d["ml"]), i4(d["mm"]), i4(d["nlm"]), i4(d["nstleg"]), i4(d["nleg"]),
+          i4(d["numphase"]), lg(sc.deltam), i4(d["bcflag"]), i4(d["ipflag"]), lg(sc.accelflag),
           d["extinct"], d["albedo"], d["legen"], d["iphase"],
           d["rshptr"], d["radiance"], d["shptr"], d["source"], d["oshptr"],
           d["temp"], d["planck"], d["dirflux"], s("S"),
@@ -548,13 +564,17 @@ def solve(sc: Scene, maxiter: int = 100, verbose=False) -> RefState:
 
 
 def precompute_phase(st: RefState):
+    """PRECOMPUTE_PHASE_CHECK (shdomsub4.f:2388): the I-I phase function and,
+    with polarization, the I-Q element (NSTPHASE = 2) per tabulated phase
+    function and scattering angle (solver.py:2806-2825)."""
     d = st.d
-    numphase, nlegp = d["numphase"], d["nlegp"]
-    legen = farr(d["legenp"].reshape((1, nlegp + 1, numphase), order="F"), np.float32)
-    phasetab = np.zeros((1, numphase, d["nscatangle"]), np.float32, order="F")
+    numphase, nlegp, nstleg, nstokes = d["numphase"], d["nlegp"], d["nstleg"], d["nstokes"]
+    nstphase = min(nstleg, 2)
+    legen = farr(d["legenp"].reshape((nstleg, nlegp + 1, numphase), order="F"), np.float32)
+    phasetab = np.zeros((nstphase, numphase, d["nscatangle"]), np.float32, order="F")
     ierr, errmsg = i4(), s("", 600)
-    fcall("PRECOMPUTE_PHASE_CHECK", i4(d["nscatangle"]), i4(numphase), i4(1), i4(1),
-          i4(d["ml"]), i4(d["nlm"]), i4(1), i4(nlegp), legen, phasetab, lg(st.sc.deltam),
+    fcall("PRECOMPUTE_PHASE_CHECK", i4(d["nscatangle"]), i4(numphase), i4(nstphase), i4(nstokes),
+          i4(d["ml"]), i4(d["nlm"]), i4(nstleg), i4(nlegp), legen, phasetab, lg(st.sc.deltam),
           lg(True), errmsg, ierr)
     _check(ierr, errmsg, "PRECOMPUTE_PHASE_CHECK")
     d["phasetab"] = phasetab
@@ -562,18 +582,20 @@ def precompute_phase(st: RefState):
 
 
 def render(st: RefState, x, y, z, mu, phi, single_scatter=False, nosurface=False):
-    """RTE.integrate_to_sensor (solver.py:633-791) -> RENDER (shdomsub4.f:93)."""
+    """RTE.integrate_to_sensor (solver.py:633-791) -> RENDER (shdomsub4.f:93).
+    Returns I per ray, or (nstokes, nrays) I, Q, U with polarization."""
     sc, d = st.sc, st.d
     precompute_phase(st)
+    nstokes, nstleg = d["nstokes"], d["nstleg"]
     npix = len(x)
     camx, camy, camz = farr(x, np.float32), farr(y, np.float32), farr(z, np.float32)
     cammu, camphi = farr(mu, np.float64), farr(phi, np.float64)
-    stokes = np.zeros((1, npix), np.float32, order="F")
+    stokes = np.zeros((nstokes, npix), np.float32, order="F")
     bcrad = d["bcrad"].copy(order="F")
     ierr, errmsg = i4(), s("", 600)
     npts = d["npts"]
-    fcall("RENDER", i4(1), i4(d["nx"]), i4(d["ny"]), i4(d["nz"]), i4(npts), i4(d["ncells"]),
-          i4(d["ml"]), i4(d["mm"]), i4(2), i4(d["nlm"]), i4(1), i4(d["nleg"]),
+    fcall("RENDER", i4(nstokes), i4(d["nx"]), i4(d["ny"]), i4(d["nz"]), i4(npts), i4(d["ncells"]),
+          i4(d["ml"]), i4(d["mm"]), i4(2), i4(d["nlm"]), i4(nstleg), i4(d["nleg"]),
           i4(d["numphase"]), i4(d["nmu"]), i4(d["nphi0max"]), d["nphi0"], d["mu"], d["phi"],
           d["wtdo"], i4(d["bcflag"]), i4(d["ipflag"]), s("S"), lg(sc.deltam), r4(sc.solarmu),
           r4(sc.solaraz), s("FL"), i4(1), d["sfcgridparms"], i4(d["maxnbc"]), i4(d["ntoppts"]),
@@ -584,12 +606,12 @@ def render(st: RefState, x, y, z, mu, phi, single_scatter=False, nosurface=False
           d["legen"], farr(d["iphase"][:, :npts], np.int32), d["dirflux"], d["fluxes"],
           d["shptr"], d["source"], camx, camy, camz, cammu, camphi, i4(npix), i4(d["npart"]),
           farr(d["total_ext"][:npts], np.float32), stokes, i4(d["nscatangle"]), d["ylmsun"],
-          d["phasetab"], i4(1), ierr, errmsg, s("ON"),
+          d["phasetab"], i4(min(nstleg, 2)), ierr, errmsg, s("ON"),
           farr(d["phaseinterpwt"][:, :npts], np.float32), r4(0.999), i4(d["maxnmicro"]),
           r8(sc.tautol), lg(nosurface), lg(True), r8(sc.transcut), lg(single_scatter),
           d["sfcgridrad"], i4(d["nang"]))
     _check(ierr, errmsg, "RENDER")
-    return stokes[0].copy()
+    return stokes[0].copy() if nstokes == 1 else stokes.copy()
 
 
 def save_solution(st: RefState):
@@ -629,10 +651,11 @@ def compute_netfluxdiv(st: RefState):
 # ----------------------------------------------------------------------------
 # dense views of the ragged reference arrays
 # ----------------------------------------------------------------------------
-def dense_sh(ptr, arr, npts, nlm):
-    """Expand a ragged SH array (pointer SHPTR/RSHPTR) to dense (npts, nlm)."""
+def dense_sh(ptr, arr, npts, nlm, k=0):
+    """Expand Stokes component k of a ragged SH array (pointer SHPTR/RSHPTR)
+    to dense (npts, nlm)."""
     out = np.zeros((npts, nlm), np.float32)
-    a = arr[0]
+    a = arr[k]
     for i in range(npts):
         lo, hi = ptr[i], ptr[i + 1]
         out[i, :hi - lo] = a[lo:hi]
@@ -645,7 +668,8 @@ def scene_from_workload(w, **kw) -> Scene:
     args = dict(delx=w.delx, dely=w.dely, zlevels=w.z, species=sp, bc=w.bc,
                 num_mu_bins=w.nmu, num_phi_bins=w.nphi, solacc=w.solacc,
                 solarmu=w.solarmu, solaraz=float(np.deg2rad(w.solaraz_deg)),
-                gndalbedo=w.gndalbedo, wavelength=w.wavelength, deltam=w.deltam)
+                gndalbedo=w.gndalbedo, wavelength=w.wavelength, deltam=w.deltam,
+                nstokes=getattr(w, "nstokes", 1))
     args.update(kw)
     return Scene(**args)
 
