@@ -111,6 +111,7 @@ int64_t at3d_plan_size(const at3d_plan* h, const char* key) {
   if (k == "nleg") return p.nleg;
   if (k == "nlegp") return p.nlegp;
   if (k == "nscatangle") return p.nscatangle;
+  if (k == "nstokes") return p.nst;
   if (k == "nslow") {
     int64_t s = 0;
     for (int o = 0; o < 8; o++) s += p.oct[o].nslow;
@@ -152,6 +153,18 @@ int at3d_plan_get(const at3d_plan* h, const char* name, void* dst, size_t nbytes
   if (k == "ymat") return copy_out(p.ymat, dst, nbytes, err, errlen);
   if (k == "wmat") return copy_out(p.wmat, dst, nbytes, err, errlen);
   if (k == "ylmsun") return copy_out(p.ylmsun, dst, nbytes, err, errlen);
+  // polarized plans: the six-element transform tables, the I-Q phase table,
+  // the a2 / a3 / b1 source terms and the b1 single-scatter terms
+  if (k == "ypol") return copy_out(p.ypol, dst, nbytes, err, errlen);
+  if (k == "wpol") return copy_out(p.wpol, dst, nbytes, err, errlen);
+  if (k == "phasetab") return copy_out(p.phasetab, dst, nbytes, err, errlen);
+  if (k == "phasetab2") return copy_out(p.phasetab2, dst, nbytes, err, errlen);
+  if (k == "chip") return copy_out(p.chip, dst, nbytes, err, errlen);
+  if (k == "legen") return copy_out(p.legen, dst, nbytes, err, errlen);
+  if (k == "legen2") return copy_out(p.legen2, dst, nbytes, err, errlen);
+  if (k == "legen3") return copy_out(p.legen3, dst, nbytes, err, errlen);
+  if (k == "legen5") return copy_out(p.legen5, dst, nbytes, err, errlen);
+  if (k == "tms5") return copy_out(p.tms5, dst, nbytes, err, errlen);
   if (k == "total_ext") return copy_out(p.total_ext, dst, nbytes, err, errlen);
   if (k == "extinct") return copy_out(p.extinct, dst, nbytes, err, errlen);
   if (k == "albedo") return copy_out(p.albedo, dst, nbytes, err, errlen);
@@ -321,20 +334,6 @@ int at3d_state_get(at3d_state* st, const char* name, void* dst, size_t nbytes, c
   at3d::State& s = st->s;
   const std::string k(name);
   const size_t npts = s.npts;
-  auto get2d = [&](const float* src, size_t rows, size_t cols, size_t ld) -> int {
-    if (nbytes < rows * cols * sizeof(float)) {
-      set_err(err, errlen, "destination buffer too small");
-      return AT3D_ERR_ARG;
-    }
-    hipError_t e = hipMemcpy2DAsync(dst, cols * sizeof(float), src, ld * sizeof(float),
-                                    cols * sizeof(float), rows, hipMemcpyDeviceToHost, s.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(s.stream);
-    if (e != hipSuccess) {
-      set_err(err, errlen, hipGetErrorString(e));
-      return AT3D_ERR_DEVICE;
-    }
-    return AT3D_OK;
-  };
   auto get1d = [&](const void* src, size_t bytes) -> int {
     if (nbytes < bytes) {
       set_err(err, errlen, "destination buffer too small");
@@ -352,9 +351,29 @@ int at3d_state_get(at3d_state* st, const char* name, void* dst, size_t nbytes, c
     set_err(err, errlen, "stream synchronize failed");
     return AT3D_ERR_DEVICE;
   }
-  if (k == "source") return get2d(s.S, npts, s.nlm, s.ld);
-  if (k == "radiance") return get2d(s.R, npts, s.nlm, s.ld);
-  if (k == "delsource") return get2d(s.DS, npts, s.nlm, s.ld);
+  if (k == "source" || k == "radiance" || k == "delsource") {
+    // [nstokes][npts][nlm]: the I plane, then Q and U of a polarized state
+    const float* src = k == "source" ? s.S : (k == "radiance" ? s.R : s.DS);
+    const size_t one = npts * (size_t)s.nlm * sizeof(float);
+    if (nbytes < one * s.nst) {
+      set_err(err, errlen, "destination buffer too small");
+      return AT3D_ERR_ARG;
+    }
+    for (int q = 0; q < s.nst; q++) {
+      hipError_t e = hipMemcpy2DAsync(static_cast<char*>(dst) + q * one, s.nlm * sizeof(float), src + q * s.psh,
+                                      s.ld * sizeof(float), s.nlm * sizeof(float), npts, hipMemcpyDeviceToHost,
+                                      s.stream);
+      if (e != hipSuccess) {
+        set_err(err, errlen, hipGetErrorString(e));
+        return AT3D_ERR_DEVICE;
+      }
+    }
+    if (hipStreamSynchronize(s.stream) != hipSuccess) {
+      set_err(err, errlen, "stream synchronize failed");
+      return AT3D_ERR_DEVICE;
+    }
+    return AT3D_OK;
+  }
   if (k == "do_radiance" || k == "do_source") {
     // device layout [column][padded point] -> [reference ordinate][point]
     const size_t nang = s.nang;
@@ -591,6 +610,10 @@ int at3d_state_set_option(at3d_state* st, const char* key, int32_t value, char* 
     return AT3D_ERR_ARG;
   }
   const std::string k(key);
+  if (st->s.nst != 1 && ((k == "sweep_general" && value) || (k == "sweep_ring" && !value))) {
+    set_err(err, errlen, "a polarized state sweeps on the level-ordered layout only");
+    return AT3D_ERR_ARG;
+  }
   if (k == "sweep_general") {
     st->s.force_general = value ? 1 : 0;
     return AT3D_OK;

@@ -36,7 +36,8 @@ __global__ __launch_bounds__(256) void k_direct_scale(const float* __restrict__ 
   if (i >= npts) return;
   const float d = dirflux[i];
   for (int l = 0; l < ml1; l++) tms[(size_t)i * ml1 + l] *= d;
-  for (int k = ss_off[i]; k < ss_off[i + 1]; k++) ss_coef[k] *= d;
+  if (ss_coef)
+    for (int k = ss_off[i]; k < ss_off[i + 1]; k++) ss_coef[k] *= d;
 }
 
 void launch_direct_beam(hipStream_t st, const DirectBeamGeom& b, const float* gridpos, int npts, float secmu0,

@@ -127,6 +127,10 @@ int state_levis_gradient(State& s, const at3d_deriv_desc& dd, int64_t nrays, con
                          const int32_t* jacptr, float* jacobian, double* grad, double* cost,
                          float* pixels, std::string& err) {
   const Plan& p = *s.plan;
+  if (s.nst != 1) {
+    err = "the Levis gradient is for NSTOKES = 1 solves";
+    return AT3D_ERR_ARG;
+  }
   if (!s.initialized) {
     err = "the solution is not initialised: solve before computing a gradient";
     return AT3D_ERR_ARG;

@@ -303,6 +303,53 @@ inline int sh_index(int l, int m, int mm) {  // 0-based J for (l,m)
   return (l <= mm) ? l * (l + 1) + m : (2 * mm + 1) * l - mm * mm + m;
 }
 
+// The normalised Wigner functions of PLMALL / YLMALL for NSTLEG = 6
+// (WIGNERFCT02P2M_NORMALIZED, shdomsub2.f:4363-4450): d^l_{m0}(mu) as
+// wigner_dm0_plm, and d^l_{m2}(mu), (-1)^(l+m) d^l_{m2}(-mu), each times
+// sqrt(l + 1/2).
+void wigner_d2pm(double x, int nrank, int m, std::vector<double>& dm2p, std::vector<double>& dm2m) {
+  dm2p.assign(nrank + 1, 0.0);
+  dm2m.assign(nrank + 1, 0.0);
+  if (m > nrank || nrank < 2) return;
+  wigner_d(x, nrank, m, 2, dm2p);
+  wigner_d(-x, nrank, m, 2, dm2m);
+  for (int n = 0; n <= nrank; n++) {
+    if ((n + m) % 2) dm2m[n] = -dm2m[n];
+    dm2p[n] = sqrt(n + 0.5) * dm2p[n];
+    dm2m[n] = sqrt(n + 0.5) * dm2m[n];
+  }
+}
+
+// The zenith part of the six-element real generalised spherical harmonics
+// (PLMALL, shdomsub2.f:4650-4752), [nlm][6] in single precision: with
+// P1 = d^l_{m0}, P2 = -(D2P + D2M) / 2, P3 = -(D2P - D2M) / 2 (times
+// 1 / sqrt(2 pi)), elements (P1, P2, P2, P1, P3, P3) for m >= 0 and
+// (P1, P2, -P2, -P1, s P3, -s P3) for m < 0, s = -1 for the transposed set
+// the analysis uses, +1 otherwise.
+void plm_pol(double x, int ml, int mm, bool transpose, std::vector<float>& prc) {
+  const int nlm = (2 * mm + 1) * (ml + 1) - mm * (mm + 1);
+  prc.assign((size_t)nlm * 6, 0.f);
+  const double fct = 1.0 / sqrt(2.0 * acos(-1.0));
+  const double sgn = transpose ? -1.0 : 1.0;
+  std::vector<double> dm0, dp, dm;
+  for (int m = 0; m <= mm; m++) {
+    wigner_dm0_plm(x, ml, m, dm0);
+    wigner_d2pm(x, ml, m, dp, dm);
+    for (int l = m; l <= ml; l++) {
+      const float p1 = (float)(fct * dm0[l]);
+      const float p2 = (float)(-0.5 * fct * (dp[l] + dm[l]));
+      const float p3 = (float)(-0.5 * fct * (dp[l] - dm[l]));
+      float* a = &prc[(size_t)sh_index(l, m, mm) * 6];
+      a[0] = p1; a[1] = p2; a[2] = p2; a[3] = p1; a[4] = p3; a[5] = p3;
+      if (m > 0) {
+        float* b = &prc[(size_t)sh_index(l, -m, mm) * 6];
+        b[0] = p1; b[1] = p2; b[2] = -p2; b[3] = -p1;
+        b[4] = (float)(sgn * p3); b[5] = (float)(-sgn * p3);
+      }
+    }
+  }
+}
+
 // YLMALL_UNPOL (shdomsub2.f:4490-4539): real generalised SH of one direction.
 void ylm_unpol(float mu, float phi, int ml, int mm, float* yr) {
   const double fct = 1.0 / sqrt(2.0 * acos(-1.0));
@@ -321,6 +368,8 @@ void ylm_unpol(float mu, float phi, int ml, int mm, float* yr) {
     }
   }
 }
+
+void make_sh_tables_pol(Plan& p);
 
 // Dense SH<->DO transform matrices.  SH_TO_DO_UNPOL (shdomsub1.f:2928-3036)
 // sums P_lm(mu) over l per m (CMU1 = PLMALL, MAKE_SH_DO_COEF shdomsub2.f:1149-1220)
@@ -375,6 +424,56 @@ void make_sh_tables(Plan& p) {
   }
   p.ylmsun.assign(nlm, 0.f);
   ylm_unpol(p.d.solarmu, p.d.solaraz, ml, mm, p.ylmsun.data());
+  if (p.nst == 3) make_sh_tables_pol(p);
+}
+
+// The polarized transforms (SH_TO_DO / DO_TO_SH with NSTOKES = 3,
+// shdomsub1.f:2789-2927 and 3041-3160) as dense tables per phase-matrix
+// element e: Stokes component I of an ordinate is sum_J Y1_J S^I_J, Q is
+// sum_J Y2_J S^Q_J + Y5_J S^U_J, U is sum_J Y6_J S^Q_J + Y3_J S^U_J, with
+// Ye_J = CMU1(e, J, mu) times the azimuth factor of the component: 1 for
+// m = 0, cos(m phi) -/+ sin(m phi) for I and Q and cos(m phi) +/- sin(m phi)
+// for U (upper sign m > 0), which is how SUMCS combines the +m and -m sums;
+// the analysis is the weighted transpose with CMU2 (the transposed set),
+// Q's azimuth factors on the elements fed by Q (2, 6) and U's on those fed
+// by U (3, 5).  The same m cut-off at NPHI0/2 - 1 as the scalar tables; the
+// I tables are the scalar ones.  The sun's functions (element 1 of YLMALL,
+// INIT_SOLUTION's YLMSUN) come from the same zenith functions.
+void make_sh_tables_pol(Plan& p) {
+  const int ml = p.ml, mm = p.mm, nlm = p.nlm, nang = p.nang;
+  p.ypol.assign((size_t)6 * nang * nlm, 0.f);
+  p.wpol.assign((size_t)6 * nang * nlm, 0.f);
+  std::vector<float> prc, prct;
+  int k = 0;
+  for (int imu = 0; imu < p.nmu; imu++) {
+    plm_pol((double)p.mu[imu], ml, mm, false, prc);
+    plm_pol((double)p.mu[imu], ml, mm, true, prct);
+    const int n0 = p.nphi0[imu];
+    const int me = std::max(0, std::min(n0 / 2 - 1, mm));
+    const double dphi = 2.0 * acos(-1.0) / n0;
+    const float wmu = p.wtmu[imu];
+    for (int ip = 0; ip < n0; ip++, k++) {
+      const double ph = ip * dphi;
+      for (int j = 0; j < nlm; j++) {
+        const int m = p.mofj[j], am = std::abs(m);
+        if (am > me) continue;
+        const double c = cos(am * ph), sn = sin(am * ph);
+        const double fi = m == 0 ? 1.0 : (m > 0 ? c - sn : c + sn);   // I and Q
+        const double fu = m == 0 ? 1.0 : (m > 0 ? c + sn : c - sn);   // U
+        const double yf[6] = {fi, fi, fu, 0.0, fi, fu};
+        const double wf[6] = {fi, fi, fu, 0.0, fu, fi};
+        for (int e = 0; e < 6; e++) {
+          if (e == 3) continue;
+          const size_t o = ((size_t)e * nang + k) * nlm + j;
+          p.ypol[o] = (float)((double)prc[(size_t)j * 6 + e] * yf[e]);
+          p.wpol[o] = (float)((double)prct[(size_t)j * 6 + e] * (double)wmu * dphi * wf[e]);
+        }
+      }
+    }
+  }
+  std::vector<float> ys((size_t)nlm * 6);
+  ylm_direction_pol(p.d.solarmu, p.d.solaraz, ml, mm, ys.data());
+  for (int j = 0; j < nlm; j++) p.ylmsun[j] = ys[(size_t)j * 6];
 }
 
 // ---------------------------------------------------------------------------
@@ -520,6 +619,58 @@ void source_constants(Plan& p) {
       point_source_constants(lt, npart, p.d.deltam != 0, p.total_ext[i], &p.extinct[i], &p.albedo[i],
                              &p.iphase[(size_t)i * 8], &p.phaseinterpwt[(size_t)i * 8], (size_t)npts,
                              &p.chi[(size_t)i * (ML + 1)], p.alb[i], &p.tc[(size_t)i * (ML + 1)]);
+  });
+}
+
+// The prepared tables of phase-matrix elements a2, a3 and b1 (NSTLEG = 6):
+// divided by (2l+1) (TRILIN_INTERP_PROP's set-up); with delta-M, element 1's
+// f subtracted from l <= ML of a2 and a3, b1 left alone (PREPARE_PROP,
+// shdomsub2.f:526-543, 'N' interpolation).
+void prepare_legen_pol(Plan& p) {
+  const int nl = p.nleg + 1, ML = p.ml, np = p.numphase;
+  const int el[3] = {1, 2, 4};
+  std::vector<float>* out[3] = {&p.legen2, &p.legen3, &p.legen5};
+  for (int k = 0; k < 3; k++) {
+    std::vector<float>& t = *out[k];
+    t.assign((size_t)np * nl, 0.f);
+    for (int iph = 0; iph < np; iph++)
+      for (int l = 0; l < nl; l++)
+        t[(size_t)iph * nl + l] =
+            p.legenp6[((size_t)el[k] * np + iph) * (p.nlegp + 1) + l] / (float)(2 * l + 1);
+    if (p.d.deltam && k < 2)
+      for (int iph = 0; iph < np; iph++) {
+        const float f = p.legen[(size_t)iph * nl + ML + 1];
+        for (int l = 0; l <= ML; l++) t[(size_t)iph * nl + l] = t[(size_t)iph * nl + l] - f;
+      }
+  }
+}
+
+// Per point, the a2, a3 and b1 terms of the polarized source
+// (CALC_SOURCE_PNT, shdomsub1.f:823-910, takes LEGEN elements 1, 2, 3, 5)
+// and the b1 single-scatter terms of rendering, per unit direct flux.
+void polarized_constants(Plan& p) {
+  const int ML = p.ml, npts = p.npts, npart = p.npart, nl = p.nleg + 1;
+  prepare_legen_pol(p);
+  p.chip.assign((size_t)npts * 3 * (ML + 1), 0.f);
+  p.tms5.assign((size_t)npts * (ML + 1), 0.f);
+  const LegTable t1 = plan_leg_table(p);
+  const LegTable te[3] = {LegTable{p.legen2.data(), nl, ML, p.legen2.size()},
+                          LegTable{p.legen3.data(), nl, ML, p.legen3.size()},
+                          LegTable{p.legen5.data(), nl, ML, p.legen5.size()}};
+  const float secmu0 = (float)(1.0 / fabs((double)p.d.solarmu));
+  const int nchunk = 64;
+  parallel_for(nchunk, [&](int c) {
+    const int i0 = (int)((int64_t)npts * c / nchunk), i1 = (int)((int64_t)npts * (c + 1) / nchunk);
+    for (int i = i0; i < i1; i++) {
+      for (int k = 0; k < 3; k++)
+        point_element_constants(t1, te[k], npart, p.d.deltam != 0, &p.extinct[i], &p.albedo[i],
+                                &p.iphase[(size_t)i * 8], &p.phaseinterpwt[(size_t)i * 8], (size_t)npts,
+                                &p.chip[((size_t)i * 3 + k) * (ML + 1)]);
+      if (p.d.deltam)
+        point_element_tms(t1, te[2], npart, p.total_ext[i], &p.extinct[i], &p.albedo[i],
+                          &p.iphase[(size_t)i * 8], &p.phaseinterpwt[(size_t)i * 8], (size_t)npts, secmu0,
+                          &p.tms5[(size_t)i * (ML + 1)]);
+    }
   });
 }
 
@@ -1037,6 +1188,23 @@ int render_constants(Plan& p, std::string& err) {
       p.phasetab[(size_t)iph * nsa + j] = (float)(a1 * ofourpi);
     }
   }
+  if (p.nst == 3) {
+    // the I-Q element: -sum (2l+1) b1_l d^l_{20}(cos T) / (4 pi) (PHASETAB(2))
+    p.phasetab2.assign((size_t)np * nsa, 0.f);
+    std::vector<double> d2;
+    for (int j = 0; j < nsa; j++) {
+      const double x = cos(PI * (double)j / (double)(nsa - 1));
+      wigner_d(x, nlegp, 2, 0, d2);
+      for (int iph = 0; iph < np; iph++) {
+        double b1 = 0.0;
+        for (int l = 0; l <= nlegp; l++) {
+          const float unsc = p.legenp6[((size_t)4 * np + iph) * (nlegp + 1) + l] / (float)(2 * l + 1);
+          b1 = b1 - (2.0 * l + 1.0) * (double)unsc * d2[l];
+        }
+        p.phasetab2[(size_t)iph * nsa + j] = (float)(b1 * ofourpi);
+      }
+    }
+  }
   if (!p.host_medium) return AT3D_OK;
   return render_point_constants(p);
 }
@@ -1083,6 +1251,67 @@ int render_point_constants(Plan& p) {
 }  // namespace
 
 void ylm_direction(float mu, float phi, int ml, int mm, float* yr) { ylm_unpol(mu, phi, ml, mm, yr); }
+
+// Wigner d^n_{m,m1}(x): the value at n0 = max(m, m1) (DMM1_N0) and the
+// three-term recurrence in n above it.
+void wigner_d(double x, int nrank, int m, int m1, std::vector<double>& d) {
+  d.assign(nrank + 1, 0.0);
+  const int n0 = std::max(m, m1);
+  if (n0 > nrank) return;
+  if (n0 == 0) {
+    d[0] = 1.0;
+    if (nrank >= 1) d[1] = x;
+    for (int n = 1; n <= nrank - 1; n++) {
+      const double f1 = (double)(2 * n + 1) * x / (double)(n + 1);
+      const double f2 = (double)n / (double)(n + 1);
+      d[n + 1] = f1 * d[n] - f2 * d[n - 1];
+    }
+    return;
+  }
+  d[n0] = dmm1_n0(x, m, m1);
+  for (int n = n0; n <= nrank - 1; n++) {
+    const double rm = sqrt((double)((n + 1) * (n + 1) - m * m));
+    const double rm1 = sqrt((double)((n + 1) * (n + 1) - m1 * m1));
+    double f1 = (double)(n * (n + 1)) * x - (double)(m * m1);
+    f1 = f1 / rm / rm1 * (double)(2 * n + 1) / (double)n;
+    double f2 = sqrt((double)(n * n - m * m)) * sqrt((double)(n * n - m1 * m1));
+    f2 = f2 / rm / rm1 * (double)(n + 1) / (double)n;
+    d[n + 1] = f1 * d[n] - f2 * d[n - 1];
+  }
+}
+
+// YLMALL with NSTLEG = 6 (shdomsub2.f:4244-4359): the zenith functions of
+// plm_pol times cos / sin of m phi in the combinations the polarized basis
+// uses, [nlm][6].
+void ylm_direction_pol(float mu, float phi, int ml, int mm, float* yr6) {
+  const double fct = 1.0 / sqrt(2.0 * acos(-1.0));
+  std::vector<double> dm0, dp, dm;
+  for (int m = 0; m <= mm; m++) {
+    wigner_dm0_plm((double)mu, ml, m, dm0);
+    wigner_d2pm((double)mu, ml, m, dp, dm);
+    double c = 1.0, sn = 0.0;
+    if (m > 0) {
+      c = (double)cosf((float)m * phi);
+      sn = (double)sinf((float)m * phi);
+    }
+    for (int l = m; l <= ml; l++) {
+      const double p1 = fct * dm0[l];
+      const double p2 = -0.5 * fct * (dp[l] + dm[l]);
+      const double p3 = -0.5 * fct * (dp[l] - dm[l]);
+      float* a = yr6 + (size_t)sh_index(l, m, mm) * 6;
+      if (m == 0) {
+        a[0] = (float)p1; a[1] = (float)p2; a[2] = (float)p2;
+        a[3] = (float)p1; a[4] = (float)p3; a[5] = (float)p3;
+        continue;
+      }
+      a[0] = (float)(p1 * c - p1 * sn); a[1] = (float)(p2 * c - p2 * sn); a[2] = (float)(p2 * c + p2 * sn);
+      a[3] = (float)(p1 * c + p1 * sn); a[4] = (float)(p3 * c - p3 * sn); a[5] = (float)(p3 * c + p3 * sn);
+      float* b = yr6 + (size_t)sh_index(l, -m, mm) * 6;
+      b[0] = (float)(p1 * sn + p1 * c); b[1] = (float)(p2 * sn + p2 * c); b[2] = (float)(p2 * sn - p2 * c);
+      b[3] = (float)(p1 * sn - p1 * c); b[4] = (float)(p3 * sn + p3 * c); b[5] = (float)(p3 * sn - p3 * c);
+    }
+  }
+}
 
 // COMPUTE_SOURCE (shdomsub1.f:967-1611) forms source term J as albedo x
 // chi_l(J) x (...), so a point whose albedo-weighted phase terms vanish above
@@ -1209,10 +1438,30 @@ int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err) {
   const int nlegp_in = d.nlegp;
   p.nlegp = std::max(nlegp_in, p.nleg);
   p.nscatangle = std::max(36, std::min(721, 2 * p.nlegp));
+  // Stokes components: the polarized input table holds six elements per
+  // order; element 1 (a1) is the scalar table
+  p.nst = d.nstokes == 0 ? 1 : d.nstokes;
+  if (p.nst != 1 && p.nst != 3) {
+    err = "nstokes must be 1 or 3";
+    return AT3D_ERR_ARG;
+  }
+  if (p.nst == 3 && d.splitacc > 0.0f) {
+    err = "polarized solves run on the base grid: split_accuracy must be 0";
+    return AT3D_ERR_ARG;
+  }
+  const int nstleg = p.nst == 3 ? 6 : 1;
   p.legenp.assign((size_t)p.numphase * (p.nlegp + 1), 0.f);
   for (int iph = 0; iph < p.numphase; iph++)
     for (int l = 0; l <= nlegp_in; l++)
-      p.legenp[(size_t)iph * (p.nlegp + 1) + l] = d.legenp[(size_t)iph * (nlegp_in + 1) + l];
+      p.legenp[(size_t)iph * (p.nlegp + 1) + l] = d.legenp[((size_t)iph * (nlegp_in + 1) + l) * nstleg];
+  if (p.nst == 3) {
+    p.legenp6.assign((size_t)6 * p.numphase * (p.nlegp + 1), 0.f);
+    for (int e = 0; e < 6; e++)
+      for (int iph = 0; iph < p.numphase; iph++)
+        for (int l = 0; l <= nlegp_in; l++)
+          p.legenp6[((size_t)e * p.numphase + iph) * (p.nlegp + 1) + l] =
+              d.legenp[((size_t)iph * (nlegp_in + 1) + l) * 6 + e];
+  }
   // _setup_grid (solver.py:2036-2168)
   p.nx = d.npx;
   p.ny = d.npy;
@@ -1227,7 +1476,8 @@ int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err) {
   int rc = make_angle_set(p, err);
   if (rc) return rc;
   make_sh_tables(p);
-  if (d.device_medium) {
+  // the polarized per-point terms are made on the host
+  if (d.device_medium && p.nst == 1) {
     // the state makes the per-point medium on the device
     medium_scalars(p);
     prepare_legen(p);
@@ -1240,6 +1490,7 @@ int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err) {
     // host by plan_direct_host when asked for); the plan keeps the constants
     // that scale with it per unit direct flux
     if (!init_radiance(p, err)) return AT3D_ERR_NUMERIC;
+    if (p.nst == 3) polarized_constants(p);
   }
   boundary_points(p);
   make_ordinates(p);

@@ -100,6 +100,14 @@ struct Plan {
   // (ensure_host_medium)
   bool medium_on_device = false, host_medium = true;
   double extmin = 0, scatmin = 0;
+  // polarization (NSTOKES = 3): the six-element input table [6][numphase]
+  // [nlegp+1] (element 1 is also legenp), the prepared elements 2, 3 and 5
+  // (a2, a3, b1) like legen, per point their albedo-weighted delta-M terms
+  // [npts][3][ml+1] and the b1 single-scatter terms [npts][ml+1], the I-Q
+  // phase table [numphase][nscatangle], and the six real generalised
+  // spherical harmonic tables of the ordinates for the transforms [6][nang][nlm]
+  int nst = 1;
+  std::vector<float> legenp6, legen2, legen3, legen5, chip, tms5, phasetab2, ypol, wpol;
   // rendering constants
   std::vector<float> phasetab;                    // [numphase][nscatangle]
   std::string phasetab_err;                       // negativity failure, raised by render
@@ -157,6 +165,11 @@ bool direct_beam_point(const DirectBeamGeom& b, float xi, float yi, float zi, fl
 // Real generalised spherical harmonics of one direction (YLMALL_UNPOL,
 // shdomsub2.f:4490-4539), J-ordered, nlm values.
 void ylm_direction(float mu, float phi, int ml, int mm, float* yr);
+// The six-element set of one direction (YLMALL with NSTLEG = 6,
+// shdomsub2.f:4244-4359), [nlm][6].
+void ylm_direction_pol(float mu, float phi, int ml, int mm, float* yr6);
+// Wigner d^n_{m,m1}(x), n = 0..nrank (WIGNERFCT, shdomsub2.f:4604-4646).
+void wigner_d(double x, int nrank, int m, int m1, std::vector<double>& d);
 
 // Mixed phase function of species ipa at point ip (the reference's LEGENT,
 // 'N' interpolation), coefficients 0..nleg.

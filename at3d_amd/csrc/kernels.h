@@ -44,10 +44,22 @@ void launch_sh_to_do(hipStream_t st, const float* S, const float* Yc, const floa
 void launch_sh_to_do_perm(hipStream_t st, const float* S, const float* Yc, const float* ext,
                           const int* ns, const int32_t* oct_perm, const int4* tiles, int ntiles,
                           float* dosrc, int mp, int ld, int ldp, int npts, const int* stop = nullptr);
+// The polarized Q and U rows of the same (NSTOKES = 3): S holds the Q plane
+// and, `plane` floats after it, the U plane ([mp][ld] each); Yc rows are
+// 2 ld long (the Q-plane terms then the U-plane terms), one row per output
+// column of a Q / U tile; no clamp at zero.
+void launch_sh_to_do_perm_qu(hipStream_t st, const float* S, size_t plane, const float* Yc, const float* ext,
+                             const int* ns, const int32_t* oct_perm, const int4* tiles, int ntiles,
+                             float* dosrc, int mp, int ld, int ldp, int npts, const int* stop = nullptr);
 // R[p][J] = (J < nr[p]) ? sum_c dorad[c][p] * Wc[c][J] : 0
 // dorad [ldo][ldp] (rows beyond nang zero), Wc [ldo][ld] (rows beyond nang zero), R [mp][ld].
 void launch_do_to_sh(hipStream_t st, const float* dorad, const float* Wc, const int* nr, float* R,
                      int mp, int ld, int ldo, int ldp, int npts, const int* stop = nullptr);
+// The polarized Q and U planes of R (R_Q at R, R_U `plane` floats after it)
+// from the Q and U rows of dorad (2 ldo rows from `dorad`): Wc [2 ldo][2 ld],
+// columns 0..ld-1 into R_Q and ld..2ld-1 into R_U, the same truncation mask.
+void launch_do_to_sh_qu(hipStream_t st, const float* dorad, const float* Wc, const int* nr, float* R, size_t plane,
+                        int mp, int ld, int ldo, int ldp, int npts, const int* stop = nullptr);
 
 // ---- sweep.hip: discrete-ordinate sweeps --------------------------------
 struct SweepArgs {
@@ -111,6 +123,11 @@ struct SweepArgs {
   int sector_points;         // 8 or 4
   int* ngeneral;             // multi-cell traces found by the record pass
   const int* stop = nullptr; // IterCtl::stop of the state, or null
+  // polarization: Stokes components swept (1 or 3; the ring launch's grid y)
+  // and the stride between their srcext / rad planes.  Q and U have zero
+  // boundary radiances and no clamp of the cell source at zero.
+  int nst = 1;
+  size_t pdo = 0;
 };
 
 // The fixed part of one ring entry of one ordinate, 16 bytes: head = point |
@@ -196,6 +213,11 @@ struct SourceArgs {
   float radmin;
   int highorder;
   const IterCtl* ctl = nullptr;  // null: `fixsh` as given; else FIXSH and stop from it
+  // polarization (NSTOKES = 3): R, S, DS hold the I, Q and U planes `psh`
+  // floats apart, chip [npts][3][ml+1] the a2, a3 and b1 terms
+  int nst = 1;
+  size_t psh = 0;
+  const float* chip = nullptr;
 };
 constexpr int SRC_PTS_PER_BLOCK = 16;
 void launch_radiance_truncation(hipStream_t st, const float* R, int ld, const float* tc,
@@ -253,7 +275,7 @@ void launch_reduce_dots(hipStream_t st, const double* partial, int nblocks, doub
                         const int* stop = nullptr);
 // ctl null: S += a DS; else with ctl's parameter, when it is positive.
 void launch_accelerate(hipStream_t st, float* S, const float* DS, const int* ns, const int* nos,
-                       int ld, int npts, float a, const IterCtl* ctl = nullptr);
+                       int ld, int npts, float a, const IterCtl* ctl = nullptr, int nst = 1, size_t psh = 0);
 // The start of an iteration (the loop test of SOLUTION_ITERATIONS and FIXSH)
 // and its end (CALC_ACCEL_SOLCRIT from the reduced dot products), one thread each.
 void launch_iter_begin(hipStream_t st, IterCtl* ctl);
@@ -270,6 +292,8 @@ struct RayArgs {
   const float* ext;        // [npts]
   const float* dsrc;       // [npts][ldd]
   int ldd;
+  int nst = 1;             // Stokes components: dsrc rows hold nst blocks of `cst`
+  int cst = 0;             // directions (I, Q, U); out is [nrays][nst]
   const int32_t* bot_index;  // [npts] bottom boundary index or -1
   const float* bcrad_bot;    // [nbot]
   const float* camx;
@@ -291,11 +315,25 @@ void launch_dir_source_finish(hipStream_t st, const float* raw, int ldd, int ndi
                               const float* ssd, int numphase, int ml, int npts, int single_scatter,
                               int tms_on, float* dsrc);
 void launch_render(hipStream_t st, const RayArgs& a);
+// The polarized direction source: raw rows [npts][ldd] hold I, Q, U blocks of
+// `cst` columns each; G3 [3][ndir][ml+1] the truncated single-scatter
+// factors of I (element 1) and of Q and U (elements 2 and 6 with the sun's
+// element 1), tms / tms5 [npts][ml+1] the a1 and b1 weights, ssd / ssq / ssu
+// [ndir][numphase] the phase table's I, Q and U at each direction.
+void launch_dir_source_finish_pol(hipStream_t st, const float* raw, int ldd, int cst, int ndir, const float* ext,
+                                  const int* ns, const int* lofj, const float* tms, const float* tms5,
+                                  const float* G3, const int32_t* ss_off, const int32_t* ss_iph,
+                                  const float* ss_coef, const float* ssd, const float* ssq, const float* ssu,
+                                  int numphase, int ml, int npts, int single_scatter, int tms_on, float* dsrc);
 void launch_render_bottom(hipStream_t st, const float* fluxes, const float* dirflux,
                           const int32_t* bcbot, int nbot, float albpi, float* bcrad);
 // raw[p][d] = sum_J S[p][J] * Yd[d][J] (no clamp, no extinction)
 void launch_sh_dot(hipStream_t st, const float* S, const float* Yd, float* raw, int mp, int ndp,
                    int ld, int ldd, int npts, int ndir);
+// raw[p][d] = sum_J S_Q[p][J] * Yd[d][J] + S_U[p][J] * Yd[d][ld + J] for
+// ndp rows of Yd (2 ld long); S_U `plane` floats after S_Q; raw row stride ldo.
+void launch_sh_dot_qu(hipStream_t st, const float* S, size_t plane, const float* Yd, float* raw, int mp,
+                      int ndp, int ld, int ldo, int npts);
 
 // ---------------------------------------------------------------------------
 // Levis gradient

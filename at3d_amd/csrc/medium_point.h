@@ -215,6 +215,55 @@ __host__ __device__ inline void point_source_constants(const LegTable& t, int np
   }
 }
 
+// Terms 0..ML of one further phase-matrix element of one point (table te:
+// a2, a3 or b1, prepared like t1): the albedo-weighted mix over species of
+// the element's mixed coefficients, each divided by 1 - f of element 1 with
+// delta-M (COMPUTE_SOURCE's LEGENT(2:6,L) with NEWMETHOD, NSTLEG = 6,
+// shdomsub1.f:1075-1120); as chi of point_source_constants, without the
+// l = 0 normalisation, which holds for element 1 only.
+__host__ __device__ inline void point_element_constants(const LegTable& t1, const LegTable& te, int npart,
+                                                        bool deltam, const float* ext, const float* albs,
+                                                        const int32_t* iph, const float* wt, size_t stride,
+                                                        float* chi_e) {
+#pragma clang fp contract(off)
+  const int ML = t1.ml;
+  double alb = 0.0;
+  for (int ipa = 0; ipa < npart; ipa++) alb = alb + (double)(ext[ipa * stride] * albs[ipa * stride]);
+  for (int l = 0; l <= ML; l++) {
+    float legent = 0.f;
+    for (int ipa = 0; ipa < npart; ipa++) {
+      const double scat = (double)(ext[ipa * stride] * albs[ipa * stride]);
+      const int32_t* ip = iph + ipa * stride * 8;
+      const float* w = wt + ipa * stride * 8;
+      float l1 = mixed_leg(te, ip, w, l);
+      if (deltam) l1 = l1 / (1 - mixed_leg(t1, ip, w, ML + 1));
+      legent = (float)((double)legent + scat * (double)l1);
+    }
+    chi_e[l] = alb > 1e-10 ? (float)((double)legent / alb) : legent / (float)npart;
+  }
+}
+
+// The truncated single-scatter terms of a further element (b1) per unit
+// direct flux: sum_species DA * LEGENT_l / (1 - f), as tms of
+// point_render_constants (COMPUTE_SOURCE_1CELL, shdomsub2.f:2960-3010).
+__host__ __device__ inline void point_element_tms(const LegTable& t1, const LegTable& te, int npart,
+                                                  float total_ext, const float* ext, const float* albs,
+                                                  const int32_t* iph, const float* wt, size_t stride,
+                                                  float secmu0, float* tms_e) {
+#pragma clang fp contract(off)
+  const int ML = t1.ml;
+  for (int l = 0; l <= ML; l++) tms_e[l] = 0.f;
+  for (int ipa = 0; ipa < npart; ipa++) {
+    const float W = (total_ext == 0.0f) ? 1.0f : ext[ipa * stride] / total_ext;
+    if (W == 0.0f) continue;
+    const int32_t* ip = iph + ipa * stride * 8;
+    const float* w = wt + ipa * stride * 8;
+    const float f = mixed_leg(t1, ip, w, ML + 1);
+    const float da = albs[ipa * stride] * secmu0 * W;
+    for (int l = 0; l <= ML; l++) tms_e[l] += da * (mixed_leg(te, ip, w, l) / (1 - f));
+  }
+}
+
 // Single-scatter constants of one point per unit direct flux
 // (COMPUTE_SOURCE_1CELL_UNPOL's TMS terms, shdomsub2.f:3096-3175), with
 // delta-M: tms[l] = sum_species DA * LEGENT_l / (1 - f) for l <= ML and the

@@ -49,12 +49,58 @@ __global__ void k_dir_source_finish(const float* raw, int ldd, int ndir,
 
 
 
+// The polarized direction source (COMPUTE_SOURCE_1CELL, shdomsub2.f:2868-3043)
+// for I, Q and U: each block's spherical-harmonic sum less its truncated
+// single scatter (a1 for I, b1 for Q and U, with their direction factors) plus
+// the exact single scatter of the tabulated I, rotated Q and U.
+__global__ void k_dir_source_finish_pol(const float* raw, int ldd, int cst, int ndir,
+                                        const float* __restrict__ ext, const int* __restrict__ ns,
+                                        const int* __restrict__ lofj, const float* __restrict__ tms,
+                                        const float* __restrict__ tms5, const float* __restrict__ G3,
+                                        const int32_t* __restrict__ ss_off, const int32_t* __restrict__ ss_iph,
+                                        const float* __restrict__ ss_coef, const float* __restrict__ ssd,
+                                        const float* __restrict__ ssq, const float* __restrict__ ssu,
+                                        int numphase, int ml, int npts, int single_scatter, int tms_on,
+                                        float* dsrc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npts * ndir) return;
+  const int p = i / ndir, d = i - p * ndir;
+  const size_t row = (size_t)p * ldd;
+  float v[3];
+  for (int k = 0; k < 3; k++) v[k] = single_scatter ? 0.0f : raw[row + k * cst + d];
+  if (tms_on) {
+    const int nsp = ns[p];
+    if (!single_scatter && nsp > 0) {
+      const int ls = lofj[nsp - 1];
+      const size_t gd = (size_t)ndir * (ml + 1);
+      for (int l = 0; l <= ls; l++) {
+        const float t1 = tms[(size_t)p * (ml + 1) + l], t5 = tms5[(size_t)p * (ml + 1) + l];
+        v[0] = v[0] - t1 * G3[d * (ml + 1) + l];
+        v[1] = v[1] - t5 * G3[gd + d * (ml + 1) + l];
+        v[2] = v[2] - t5 * G3[2 * gd + d * (ml + 1) + l];
+      }
+    }
+    for (int q = ss_off[p]; q < ss_off[p + 1]; q++) {
+      const float c = ss_coef[q];
+      const int t = d * numphase + ss_iph[q];
+      v[0] = v[0] + c * ssd[t];
+      v[1] = v[1] + c * ssq[t];
+      v[2] = v[2] + c * ssu[t];
+    }
+  }
+  for (int k = 0; k < 3; k++) dsrc[row + k * cst + d] = v[k] * ext[p];
+}
+
 __device__ inline float trilin(const float* v8, double u, double v, double w) {
 #pragma clang fp contract(off)
   return (float)((1 - w) * ((1 - v) * ((1 - u) * v8[0] + u * v8[1]) + v * ((1 - u) * v8[2] + u * v8[3])) +
                  w * ((1 - v) * ((1 - u) * v8[4] + u * v8[5]) + v * ((1 - u) * v8[6] + u * v8[7])));
 }
 
+// NST = 3: I, Q and U integrated along the ray with the same transmission;
+// only I's source is clamped at zero, and Q and U have zero boundary
+// radiances (the sky is unpolarized, the Lambertian surface depolarizes).
+template <int NST>
 __global__ __launch_bounds__(128) void k_render(RayArgs a) {
 #pragma clang fp contract(off)
   const int ir = blockIdx.x * blockDim.x + threadIdx.x;
@@ -68,7 +114,7 @@ __global__ __launch_bounds__(128) void k_render(RayArgs a) {
   // extrapolate a sensor above the domain to the top (RENDER :223-236)
   if (z0 > (double)ZG[a.nz - 1]) {
     if (muray >= 0.0) {
-      a.out[ir] = 0.0f;
+      for (int k = 0; k < NST; k++) a.out[(size_t)ir * NST + k] = 0.0f;
       return;
     }
     const double r = ((double)ZG[a.nz - 1] - z0) / muray;
@@ -97,20 +143,21 @@ __global__ __launch_bounds__(128) void k_render(RayArgs a) {
   if (icell >= 0 && icell < a.g.ncells) icell = locate_in_tree(a.g, icell, xe, ye, ze);
   if (icell < 0 || icell >= a.g.ncells) {
     atomicOr(a.errflag, 4);
-    a.out[ir] = 0.0f;
+    for (int k = 0; k < NST; k++) a.out[(size_t)ir * NST + k] = 0.0f;
     return;
   }
-  double radiance = 0.0, transmit = 1.0;
+  double radiance[NST], transmit = 1.0;
+  for (int k = 0; k < NST; k++) radiance[k] = 0.0;
   int ngrid = 0;
   bool validrad = false;
-  float e8[8], s8[8];
+  float e8[8], s8[NST][8];
   while (!validrad && icell >= 0) {
     ngrid++;
     const int* gp = a.g.gridptr + (size_t)icell * 8;
     for (int c = 0; c < 8; c++) {
       const int q = gp[c];
       e8[c] = a.ext[q];
-      s8[c] = dsrc[(size_t)q * LD];
+      for (int k = 0; k < NST; k++) s8[k][c] = dsrc[(size_t)q * LD + k * a.cst];
     }
     const int ipt1 = gp[0], ipt2 = gp[7];
     const double delx = (double)(P[ipt2 * 3] - P[ipt1 * 3]);
@@ -121,7 +168,9 @@ __global__ __launch_bounds__(128) void k_render(RayArgs a) {
     const double invdelz = 1.0 / delz;
     const double px1 = P[ipt1 * 3], py1 = P[ipt1 * 3 + 1], pz1 = P[ipt1 * 3 + 2];
     double u = (xe - px1) * invdelx, v = (ye - py1) * invdely, w = (ze - pz1) * invdelz;
-    float srcext1 = fmaxf(0.0f, trilin(s8, u, v, w));
+    float srcext1[NST];
+    for (int k = 0; k < NST; k++) srcext1[k] = trilin(s8[k], u, v, w);
+    srcext1[0] = fmaxf(0.0f, srcext1[0]);
     float ext1 = trilin(e8, u, v, w);
     const int8_t fl = a.g.cellflags[icell];
     const bool ipinx = (fl & 1) && !(a.open_x && ((cx > 0 && xe < xm) || (cx < 0 && xe > xm)));
@@ -149,25 +198,28 @@ __global__ __launch_bounds__(128) void k_render(RayArgs a) {
       v = (ye + s * cy - py1) * invdely;
       w = (ze + s * cz - pz1) * invdelz;
       const float ext0 = trilin(e8, u, v, w);
-      const float srcext0 = fmaxf(0.0f, trilin(s8, u, v, w));
+      float srcext0[NST];
+      for (int k = 0; k < NST; k++) srcext0[k] = trilin(s8[k], u, v, w);
+      srcext0[0] = fmaxf(0.0f, srcext0[0]);
       const double ext = (double)(0.5f * (ext0 + ext1));
-      double absc, trc, src;
+      double absc, trc;
       if (ext != 0.0) {
         const double tau = ext * dels;
         absc = tau * (1.0 - 0.5 * tau * (1.0 - (double)0.33333333333f * tau));
         trc = 1.0 - absc;
-        src = ((double)(0.5f * (srcext0 + srcext1)) +
-               (double)(0.08333333333f * (ext0 * srcext1 - ext1 * srcext0)) * dels *
-                   (1.0 - (double)(0.05f * (ext1 - ext0)) * dels)) / ext;
+        for (int k = 0; k < NST; k++) {
+          const double src = ((double)(0.5f * (srcext0[k] + srcext1[k])) +
+                              (double)(0.08333333333f * (ext0 * srcext1[k] - ext1 * srcext0[k])) * dels *
+                                  (1.0 - (double)(0.05f * (ext1 - ext0)) * dels)) / ext;
+          radiance[k] = radiance[k] + transmit * src * absc;
+        }
       } else {
         absc = 0.0;
         trc = 1.0;
-        src = 0.0;
       }
-      radiance = radiance + transmit * src * absc;
       transmit = transmit * trc;
       ext1 = ext0;
-      srcext1 = srcext0;
+      for (int k = 0; k < NST; k++) srcext1[k] = srcext0[k];
     }
     int iface, jface;
     bool openbcface;
@@ -228,7 +280,7 @@ __global__ __launch_bounds__(128) void k_render(RayArgs a) {
       const float vb = (Y[2] - Y[0] > 0.0f) ? (float)((yn - Y[0]) / (double)(Y[2] - Y[0])) : 0.0f;
       const float radbnd = (1 - ub) * (1 - vb) * R[0] + ub * (1 - vb) * R[1] +
                            (1 - ub) * vb * R[2] + ub * vb * R[3];
-      if (!a.nosurface) radiance = radiance + transmit * (double)radbnd;
+      if (!a.nosurface) radiance[0] = radiance[0] + transmit * (double)radbnd;
     } else {
       icell = inext;
     }
@@ -236,7 +288,7 @@ __global__ __launch_bounds__(128) void k_render(RayArgs a) {
     ye = yn;
     ze = zn;
   }
-  a.out[ir] = (float)radiance;
+  for (int k = 0; k < NST; k++) a.out[(size_t)ir * NST + k] = (float)radiance[k];
 }
 
 // Surface radiance from the final fluxes (RENDER :201-209).
@@ -261,7 +313,21 @@ void launch_dir_source_finish(hipStream_t st, const float* raw, int ldd, int ndi
 }
 
 void launch_render(hipStream_t st, const RayArgs& a) {
-  k_render<<<(a.nrays + 127) / 128, 128, 0, st>>>(a);
+  if (a.nst == 3)
+    k_render<3><<<(a.nrays + 127) / 128, 128, 0, st>>>(a);
+  else
+    k_render<1><<<(a.nrays + 127) / 128, 128, 0, st>>>(a);
+}
+
+void launch_dir_source_finish_pol(hipStream_t st, const float* raw, int ldd, int cst, int ndir, const float* ext,
+                                  const int* ns, const int* lofj, const float* tms, const float* tms5,
+                                  const float* G3, const int32_t* ss_off, const int32_t* ss_iph,
+                                  const float* ss_coef, const float* ssd, const float* ssq, const float* ssu,
+                                  int numphase, int ml, int npts, int single_scatter, int tms_on, float* dsrc) {
+  const int n = npts * ndir;
+  k_dir_source_finish_pol<<<(n + 255) / 256, 256, 0, st>>>(raw, ldd, cst, ndir, ext, ns, lofj, tms, tms5, G3,
+                                                            ss_off, ss_iph, ss_coef, ssd, ssq, ssu, numphase,
+                                                            ml, npts, single_scatter, tms_on, dsrc);
 }
 
 void launch_render_bottom(hipStream_t st, const float* fluxes, const float* dirflux,

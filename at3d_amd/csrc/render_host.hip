@@ -60,18 +60,51 @@ int direction_tables(const State& s, int64_t nrays, const double* mu, const doub
   }
   const int ML = p.ml, nlm = p.nlm, np = p.numphase, nsa = p.nscatangle;
   const bool tms_on = p.d.deltam != 0;
-  t.Yd.assign((size_t)ldd * s.ld, 0.f);
-  t.G.assign((size_t)ndir * (ML + 1), 0.f);
+  const bool pol = p.nst == 3;
+  const int ld = s.ld;
+  t.Yd.assign((size_t)ldd * ld, 0.f);
+  t.G.assign((size_t)(pol ? 3 : 1) * ndir * (ML + 1), 0.f);
   t.ssd.assign((size_t)ndir * np, 0.f);
+  if (pol) {
+    t.Yqu.assign((size_t)4 * ldd * ld, 0.f);
+    t.ssq.assign((size_t)ndir * np, 0.f);
+    t.ssu.assign((size_t)ndir * np, 0.f);
+  }
   t.sa_j.assign(ndir, 1);
   t.sa_f.assign(ndir, 0.f);
-  std::vector<float> ylm(nlm);
+  std::vector<float> ylm(nlm), y6(pol ? (size_t)nlm * 6 : 0);
   const double PI = acos(-1.0);
   const float solarmu = p.d.solarmu, solaraz = p.d.solaraz;
   for (int d = 0; d < ndir; d++) {
     const double mu2 = t.dirs[d].first, phi2 = t.dirs[d].second;
-    ylm_direction((float)mu2, (float)phi2, ML, p.mm, ylm.data());
-    for (int j = 0; j < nlm; j++) t.Yd[(size_t)d * s.ld + j] = ylm[j];
+    if (pol) {
+      // YLMALL with NSTLEG = 6 (INTEGRATE_1RAY, shdomsub2.f:2411): element 1
+      // for I; Q from elements 2 (S^Q) and 5 (S^U), U from 6 and 3
+      ylm_direction_pol((float)mu2, (float)phi2, ML, p.mm, y6.data());
+      for (int j = 0; j < nlm; j++) {
+        ylm[j] = y6[(size_t)j * 6];
+        float* q = &t.Yqu[(size_t)d * 2 * ld];
+        float* u = &t.Yqu[(size_t)(ldd + d) * 2 * ld];
+        q[j] = y6[(size_t)j * 6 + 1];
+        q[ld + j] = y6[(size_t)j * 6 + 4];
+        u[j] = y6[(size_t)j * 6 + 5];
+        u[ld + j] = y6[(size_t)j * 6 + 2];
+      }
+      // the truncated single scatter of Q and U (b1 with elements 2 and 6
+      // of the direction and element 1 of the sun)
+      std::vector<double> gq(ML + 1, 0.0), gu(ML + 1, 0.0);
+      for (int j = 0; j < nlm; j++) {
+        gq[p.lofj[j]] += (double)y6[(size_t)j * 6 + 1] * (double)p.ylmsun[j];
+        gu[p.lofj[j]] += (double)y6[(size_t)j * 6 + 5] * (double)p.ylmsun[j];
+      }
+      for (int l = 0; l <= ML; l++) {
+        t.G[(size_t)(ndir + d) * (ML + 1) + l] = (float)gq[l];
+        t.G[(size_t)(2 * ndir + d) * (ML + 1) + l] = (float)gu[l];
+      }
+    } else {
+      ylm_direction((float)mu2, (float)phi2, ML, p.mm, ylm.data());
+    }
+    for (int j = 0; j < nlm; j++) t.Yd[(size_t)d * ld + j] = ylm[j];
     std::vector<double> g(ML + 1, 0.0);
     for (int j = 0; j < nlm; j++) g[p.lofj[j]] += (double)ylm[j] * (double)p.ylmsun[j];
     for (int l = 0; l <= ML; l++) t.G[(size_t)d * (ML + 1) + l] = (float)g[l];
@@ -88,6 +121,27 @@ int direction_tables(const State& s, int64_t nrays, const double* mu, const doub
       for (int iph = 0; iph < np; iph++)
         t.ssd[(size_t)d * np + iph] =
             (1 - f) * p.phasetab[(size_t)iph * nsa + j - 1] + f * p.phasetab[(size_t)iph * nsa + j];
+      if (pol) {
+        // ROTATE_POL_PLANE (shdomsub2.f:3277-3316): the I-Q element from the
+        // scattering plane into the outgoing meridian plane
+        const float mu = (float)mu2, delphi = solaraz - (float)phi2;
+        const double sin_scat = sqrt(std::max(0.0, 1.0 - cosscat * cosscat));
+        const double sin_t1 = sqrt(1.0 - (double)(solarmu * solarmu));
+        const double sin_t2 = sqrt(1.0 - (double)(mu * mu));
+        const double sinphi = sin((double)delphi), cosphi = cos((double)delphi);
+        double sin2 = 0.0, cos2 = -1.0;
+        if (sin_scat != 0.0) {
+          sin2 = sin_t1 * sinphi / sin_scat;
+          cos2 = (sin_t2 * (double)solarmu - sin_t1 * (double)mu * cosphi) / sin_scat;
+        }
+        const double sin22 = 2.0 * sin2 * cos2, cos22 = 1.0 - 2.0 * sin2 * sin2;
+        for (int iph = 0; iph < np; iph++) {
+          const float b1 =
+              (1 - f) * p.phasetab2[(size_t)iph * nsa + j - 1] + f * p.phasetab2[(size_t)iph * nsa + j];
+          t.ssq[(size_t)d * np + iph] = (float)((double)b1 * cos22);
+          t.ssu[(size_t)d * np + iph] = (float)((double)b1 * sin22);
+        }
+      }
     }
   }
   return AT3D_OK;
@@ -129,6 +183,9 @@ static int render_impl(State& s, int64_t nrays, const float* x, const float* y, 
   const int ndir = t.ndir, ldd = t.ldd;
   const int ML = p.ml, np = p.numphase;
   const bool tms_on = p.d.deltam != 0;
+  // polarized: the direction source rows hold I, Q and U blocks of ldd
+  // directions; the rays' Stokes vectors come back as [nrays][3]
+  const int nst = p.nst, ldr = nst * ldd;
   std::vector<int32_t>& dir_of_ray = t.dir_of_ray;
   std::vector<float>& Yd = t.Yd;
   std::vector<float>& G = t.G;
@@ -144,10 +201,11 @@ static int render_impl(State& s, int64_t nrays, const float* x, const float* y, 
     return o;
   };
   const size_t o_yd = carve(Yd.size() * 4), o_g = carve(G.size() * 4), o_ss = carve(ssd.size() * 4 + 4);
-  const size_t o_raw = carve((size_t)s.nptsp * ldd * 4);
+  const size_t o_yqu = carve(t.Yqu.size() * 4), o_sq = carve(t.ssq.size() * 4 + 4), o_su = carve(t.ssu.size() * 4 + 4);
+  const size_t o_raw = carve((size_t)s.nptsp * ldr * 4);
   const size_t o_x = carve(nr * 4), o_y = carve(nr * 4), o_z = carve(nr * 4);
   const size_t o_mu = carve(nr * 8), o_phi = carve(nr * 8), o_dir = carve(nr * 4);
-  const size_t o_top = carve(nr * 4), o_out = carve(nr * 4);
+  const size_t o_top = carve(nr * 4), o_out = carve(nr * 4 * nst);
   char* B = nullptr;
   if (fixed) {
     if (off > fixed_cap) {
@@ -176,6 +234,11 @@ static int render_impl(State& s, int64_t nrays, const float* x, const float* y, 
   HIPCHK(hipMemcpyAsync(B + o_g, G.data(), G.size() * 4, hipMemcpyHostToDevice, s.stream));
   if (!ssd.empty())
     HIPCHK(hipMemcpyAsync(B + o_ss, ssd.data(), ssd.size() * 4, hipMemcpyHostToDevice, s.stream));
+  if (nst == 3) {
+    HIPCHK(hipMemcpyAsync(B + o_yqu, t.Yqu.data(), t.Yqu.size() * 4, hipMemcpyHostToDevice, s.stream));
+    HIPCHK(hipMemcpyAsync(B + o_sq, t.ssq.data(), t.ssq.size() * 4, hipMemcpyHostToDevice, s.stream));
+    HIPCHK(hipMemcpyAsync(B + o_su, t.ssu.data(), t.ssu.size() * 4, hipMemcpyHostToDevice, s.stream));
+  }
   HIPCHK(hipMemcpyAsync(B + o_x, x, nr * 4, hipMemcpyHostToDevice, s.stream));
   HIPCHK(hipMemcpyAsync(B + o_y, y, nr * 4, hipMemcpyHostToDevice, s.stream));
   HIPCHK(hipMemcpyAsync(B + o_z, z, nr * 4, hipMemcpyHostToDevice, s.stream));
@@ -186,12 +249,22 @@ static int render_impl(State& s, int64_t nrays, const float* x, const float* y, 
   HIPCHK(hipMemsetAsync(s.errflag, 0, sizeof(int), s.stream));
   (void)hipEventRecord(s.ev[6], s.stream);
   float* raw = reinterpret_cast<float*>(B + o_raw);
-  launch_sh_dot(s.stream, s.S, reinterpret_cast<float*>(B + o_yd), raw, s.nptsp, ldd, s.ld, ldd,
+  launch_sh_dot(s.stream, s.S, reinterpret_cast<float*>(B + o_yd), raw, s.nptsp, ldd, s.ld, ldr,
                 p.npts, ndir);
-  launch_dir_source_finish(s.stream, raw, ldd, ndir, s.ext, s.ns, s.lofj, s.tms,
-                           reinterpret_cast<float*>(B + o_g), s.ss_off, s.ss_iph, s.ss_coef,
-                           reinterpret_cast<float*>(B + o_ss), np, ML, p.npts, single_scatter,
-                           tms_on ? 1 : 0, raw);
+  if (nst == 3) {
+    launch_sh_dot_qu(s.stream, s.S + s.psh, s.psh, reinterpret_cast<float*>(B + o_yqu), raw + ldd, s.nptsp,
+                     2 * ldd, s.ld, ldr, p.npts);
+    launch_dir_source_finish_pol(s.stream, raw, ldr, ldd, ndir, s.ext, s.ns, s.lofj, s.tms, s.tms5,
+                                 reinterpret_cast<float*>(B + o_g), s.ss_off, s.ss_iph, s.ss_coef,
+                                 reinterpret_cast<float*>(B + o_ss), reinterpret_cast<float*>(B + o_sq),
+                                 reinterpret_cast<float*>(B + o_su), np, ML, p.npts, single_scatter,
+                                 tms_on ? 1 : 0, raw);
+  } else {
+    launch_dir_source_finish(s.stream, raw, ldd, ndir, s.ext, s.ns, s.lofj, s.tms,
+                             reinterpret_cast<float*>(B + o_g), s.ss_off, s.ss_iph, s.ss_coef,
+                             reinterpret_cast<float*>(B + o_ss), np, ML, p.npts, single_scatter,
+                             tms_on ? 1 : 0, raw);
+  }
   const float albpi = p.d.gndalbedo / acosf(-1.0f);
   launch_render_bottom(s.stream, s.fluxes, s.dirflux, s.bcbot, (int)p.bcbot.size(), albpi,
                        s.bcrad_render);
@@ -210,7 +283,9 @@ static int render_impl(State& s, int64_t nrays, const float* x, const float* y, 
   a.open_y = (p.bcflag >> 1) & 1;
   a.ext = s.ext;
   a.dsrc = raw;
-  a.ldd = ldd;
+  a.ldd = ldr;
+  a.nst = nst;
+  a.cst = ldd;
   a.bot_index = s.bot_index;
   a.bcrad_bot = s.bcrad_render;
   a.camx = reinterpret_cast<float*>(B + o_x);
@@ -232,7 +307,7 @@ static int render_impl(State& s, int64_t nrays, const float* x, const float* y, 
   (void)hipEventRecord(s.ev[7], s.stream);
   HIPCHK(hipGetLastError());
   int ef = 0;
-  HIPCHK(hipMemcpyAsync(out, B + o_out, nr * 4, hipMemcpyDeviceToHost, s.stream));
+  HIPCHK(hipMemcpyAsync(out, B + o_out, nr * 4 * nst, hipMemcpyDeviceToHost, s.stream));
   HIPCHK(hipMemcpyAsync(&ef, s.errflag, sizeof(int), hipMemcpyDeviceToHost, s.stream));
   HIPCHK(hipStreamSynchronize(s.stream));
   float ms = 0.f;

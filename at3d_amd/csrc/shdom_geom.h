@@ -212,10 +212,11 @@ __host__ __device__ inline void face_uv(const GridTopo& g, int icell, int face, 
 // Cell integral of the short characteristic (shdomsub1.f:3948-3987): linear
 // extinction and linear source*extinction across the cell, with the reference's
 // tau branches.  The literal constants are the reference's single-precision
-// literals promoted to double.
+// literals promoted to double.  The source is clamped at zero for I, not for
+// the Q and U of a polarized sweep (clamp false; BACK_INT_GRID3D :3640).
 __host__ __device__ inline void cell_integral(double ext0, double ext1, double srcext0,
                                               double srcext1, double so, double& src,
-                                              double& abscell, double& transcell) {
+                                              double& abscell, double& transcell, bool clamp = true) {
 #pragma clang fp contract(off)
   const double ext = 0.5 * (ext0 + ext1);
   const double tau = ext * so;
@@ -244,7 +245,7 @@ __host__ __device__ inline void cell_integral(double ext0, double ext1, double s
            (ext0p * srcext1 - ext1 * srcext0p) * 2.0 / (ext0p + ext1) *
                (1 - 2 / tau + 2 * transcell / abscell));
   }
-  if (src < 0.0) src = 0.0;
+  if (clamp && src < 0.0) src = 0.0;
 }
 
 }  // namespace at3d

@@ -132,6 +132,10 @@ int state_load(State& s, const int32_t* shptr, const float* source, const int32_
                const float* radiance, const float* fluxes, std::string& err) {
   const Plan& p = *s.plan;
   const int npts = p.npts;
+  if (s.nst != 1) {
+    err = "saved solutions and spherical-harmonic diagnostics are for NSTOKES = 1 solves";
+    return AT3D_ERR_ARG;
+  }
   {
     int rc = check_ptr(shptr, npts, p.nlm, 0, "SHPTR", err);
     if (!rc) rc = check_ptr(rshptr, npts, p.nlm, 1, "RSHPTR", err);
@@ -182,6 +186,10 @@ int state_save(State& s, int32_t* shptr, int32_t* rshptr, float* source, int64_t
                float* radiance, int64_t radiance_cap, float* fluxes, std::string& err) {
   const Plan& p = *s.plan;
   const int npts = p.npts;
+  if (s.nst != 1) {
+    err = "saved solutions and spherical-harmonic diagnostics are for NSTOKES = 1 solves";
+    return AT3D_ERR_ARG;
+  }
   if (!s.initialized) {
     err = "no solution to save: initialise or solve first";
     return AT3D_ERR_ARG;
@@ -237,6 +245,10 @@ int state_save(State& s, int32_t* shptr, int32_t* rshptr, float* source, int64_t
 int state_sh_terms(State& s, int nshout, float* shterms, float* netfluxdiv, std::string& err) {
   const Plan& p = *s.plan;
   const int npts = p.npts;
+  if (s.nst != 1) {
+    err = "saved solutions and spherical-harmonic diagnostics are for NSTOKES = 1 solves";
+    return AT3D_ERR_ARG;
+  }
   if (!s.initialized) {
     err = "no solution: initialise or solve first";
     return AT3D_ERR_ARG;

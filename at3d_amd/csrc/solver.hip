@@ -160,6 +160,8 @@ static SweepArgs sweep_args(const State& s) {
   a.sector_points = s.sector_points;
   a.ngeneral = s.ngeneral_d;
   a.stop = nullptr;
+  a.nst = s.nst;
+  a.pdo = s.pdo;
   return a;
 }
 
@@ -319,6 +321,13 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
   TRY(upload(s, s.sec_p0, sp0.data(), sp0.size(), err));
   TRY(upload(s, s.sec_slot, reinterpret_cast<const uint4*>(sslot.data()), sslot.size() / 8, err));
   TRY(upload(s, s.sh_tiles, tiles.data(), tiles.size(), err));
+  if (s.nst == 3) {
+    // the Q and U rows of each tile's columns: Q at the same columns of the
+    // Q plane, U ldo rows on
+    std::vector<int4> qu(tiles);
+    for (const int4& t : tiles) qu.push_back(make_int4(t.x, t.y + s.ldo, t.z, 0));
+    TRY(upload(s, s.sh_tiles_qu, qu.data(), qu.size(), err));
+  }
   TRY(upload(s, s.ext_pos, extpos.data(), extpos.size(), err));
   TRY(upload(s, s.geo_rep, reps.data(), reps.size(), err));
   TRY(alloc0(s, s.geo, (size_t)p.nang * s.ncls, err));
@@ -351,8 +360,9 @@ size_t plan_footprint_bytes(const Plan& p) {
   size_t b = 0;
   b += 2 * ldo * ld * 4;                 // Y, W
   b += 256;                              // iteration control block
-  b += 3 * nptsp * ld * 4;               // S, R, DS
-  b += 2 * nptsp * ldo * 4;              // DO source and radiance
+  b += 3 * nptsp * ld * 4 * p.nst;       // S, R, DS (each Stokes plane)
+  b += 2 * nptsp * ldo * 4 * p.nst;      // DO source and radiance
+  if (p.nst == 3) b += 4 * ldo * ld * 4 * 2 + 4 * npts * ml1 * 4;   // Q / U tables, a2 a3 b1 and b1 TMS terms
   b += npts * 16;                        // two-stream radiance terms
   b += 3 * npts * ml1 * 4;               // chi, tc, tms
   b += npts * 4 * 12;                    // per-point scalars, counts, fluxes, partials
@@ -482,6 +492,9 @@ int state_create(const Plan& p, State& s, std::string& err) {
   s.ld = (int)round_up(p.nlm, 64);
   s.nptsp = (int)round_up(p.npts, 64);
   s.ldo = (int)round_up(p.nang, 64);
+  s.nst = p.nst;
+  s.psh = (size_t)s.nptsp * s.ld;
+  s.pdo = (size_t)s.nptsp * s.ldo;
   s.ndown = 0;
   for (const Ordinate& o : p.ords) s.ndown += o.down ? 1 : 0;
   for (int k = 0; k < p.nang; k++)
@@ -515,6 +528,37 @@ int state_create(const Plan& p, State& s, std::string& err) {
   }
   TRY(upload(s, s.Y, Y.data(), Y.size(), err));
   TRY(upload(s, s.W, W.data(), W.size(), err));
+  if (p.nst == 3) {
+    // the Q / U transform tables by device column (make_sh_tables_pol's
+    // elements): SH->DO rows [Y2 | Y5] for Q, [Y6 | Y3] for U; DO->SH rows
+    // [W2 | W6] for a Q ordinate, [W5 | W3] for a U ordinate
+    const size_t row = 2 * (size_t)s.ld;
+    std::vector<float> Yq((size_t)2 * s.ldo * row, 0.f), Wq((size_t)2 * s.ldo * row, 0.f);
+    auto el = [&](const std::vector<float>& t, int e, int k, int j) {
+      return t[((size_t)e * p.nang + k) * p.nlm + j];
+    };
+    for (int k = 0; k < p.nang; k++) {
+      const int c = s.col_of_ref[k];
+      float* yq = &Yq[(size_t)c * row];
+      float* yu = &Yq[(size_t)(s.ldo + c) * row];
+      float* wq = &Wq[(size_t)c * row];
+      float* wu = &Wq[(size_t)(s.ldo + c) * row];
+      for (int j = 0; j < p.nlm; j++) {
+        yq[j] = el(p.ypol, 1, k, j);
+        yq[s.ld + j] = el(p.ypol, 4, k, j);
+        yu[j] = el(p.ypol, 5, k, j);
+        yu[s.ld + j] = el(p.ypol, 2, k, j);
+        wq[j] = el(p.wpol, 1, k, j);
+        wq[s.ld + j] = el(p.wpol, 5, k, j);
+        wu[j] = el(p.wpol, 4, k, j);
+        wu[s.ld + j] = el(p.wpol, 2, k, j);
+      }
+    }
+    TRY(upload(s, s.Yqu, Yq.data(), Yq.size(), err));
+    TRY(upload(s, s.Wqu, Wq.data(), Wq.size(), err));
+    TRY(upload(s, s.chip, p.chip.data(), p.chip.size(), err));
+    TRY(upload(s, s.tms5, p.tms5.data(), p.tms5.size(), err));
+  }
   std::vector<float> ysun(s.ld, 0.f);
   std::copy(p.ylmsun.begin(), p.ylmsun.end(), ysun.begin());
   TRY(upload(s, s.ysun, ysun.data(), ysun.size(), err));
@@ -530,7 +574,7 @@ int state_create(const Plan& p, State& s, std::string& err) {
   }
   TRY(alloc0(s, s.dirflux, (size_t)p.npts, err));   // made by the direct-beam pass below
   TRY(alloc0(s, s.f0, (size_t)p.npts, err));
-  const size_t nsh = (size_t)s.nptsp * s.ld, ndo = (size_t)s.nptsp * s.ldo;
+  const size_t nsh = (size_t)s.nptsp * s.ld * s.nst, ndo = (size_t)s.nptsp * s.ldo * s.nst;
   TRY(alloc0(s, s.S, nsh, err));
   TRY(alloc0(s, s.R, nsh, err));
   TRY(alloc0(s, s.DS, nsh, err));
@@ -644,6 +688,7 @@ int state_create(const Plan& p, State& s, std::string& err) {
   }
   // the plan holds them per unit direct flux
   launch_direct_scale(s.stream, s.dirflux, p.npts, p.ml + 1, s.tms, s.ss_off, s.ss_coef);
+  if (p.nst == 3) launch_direct_scale(s.stream, s.dirflux, p.npts, p.ml + 1, s.tms5, s.ss_off, nullptr);
   HIPCHK(hipGetLastError());
   TRY(alloc0(s, s.bcrad_render, p.bcbot.size(), err));
   std::vector<int32_t> bot_index(p.npts, -1);
@@ -663,6 +708,10 @@ int state_create(const Plan& p, State& s, std::string& err) {
   TRY(alloc0(s, s.ctl, 1, err));
   s.nr_buf0 = s.nr;
   TRY(setup_ring(p, s, err));
+  if (p.nst == 3 && !s.ring_ok) {
+    err = "polarized solves need the level-ordered sweep, whose rings this grid does not fit";
+    return AT3D_ERR_ARG;
+  }
   for (int e = 0; e < State::NEV; e++) HIPCHK(hipEventCreate(&s.ev[e]));
   for (int e = 0; e < 4; e++) HIPCHK(hipEventCreate(&s.ev_sweep[e]));
   for (int e = 0; e < 2; e++) HIPCHK(hipEventCreate(&s.ev_ray[e]));
@@ -779,6 +828,10 @@ int state_reset_grid(State& s, std::string& err) {
 int state_set_grid(State& s, int npts, int ncells, const float* gridpos, const int32_t* gridptr,
                    const int32_t* neighptr, const int32_t* treeptr, const int8_t* cellflags, std::string& err) {
   const Plan& b = *s.base_plan;
+  if (s.nst != 1) {
+    err = "a polarized solve stays on its base grid";
+    return AT3D_ERR_ARG;
+  }
   bool base = npts == b.npts && ncells == b.ncells;
   for (size_t i = 0; base && i < (size_t)ncells * 2; i++) base = treeptr[i] == -1;
   if (base) {
@@ -827,6 +880,9 @@ static SourceArgs source_args(State& s, int first, int fixsh) {
   a.first = first;
   a.accel = p.d.accelflag ? 1 : 0;
   a.fixsh = fixsh;
+  a.nst = s.nst;
+  a.psh = s.psh;
+  a.chip = s.chip;
   return a;
 }
 
@@ -839,6 +895,8 @@ int state_init_solution(State& s, std::string& err) {
   if (s.own_plan) TRY(state_replace(s, nullptr, nullptr, err));
   const Plan& p = *s.plan;
   launch_init_radiance(s.stream, s.R, s.R0, s.nr, s.ld, p.npts);
+  // the two-stream start is unpolarized: Q and U start at zero
+  if (s.nst == 3) HIPCHK(hipMemsetAsync(s.R + s.psh, 0, 2 * s.psh * sizeof(float), s.stream));
   launch_compute_source(s.stream, source_args(s, 1, 0), s.nsrc_blocks);
   s.nr_next_fixsh = false;
   HIPCHK(hipGetLastError());
@@ -872,7 +930,11 @@ int state_init_solution_from(State& s, State& src, std::string& err) {
     err = "warm start needs an initialised source state on the same grid and angular resolution";
     return AT3D_ERR_ARG;
   }
-  const size_t nsh = (size_t)s.nptsp * s.ld;
+  if (s.nst != src.nst) {
+    err = "warm start needs a source state with the same number of Stokes components";
+    return AT3D_ERR_ARG;
+  }
+  const size_t nsh = (size_t)s.nptsp * s.ld * s.nst;
   HIPCHK(hipStreamSynchronize(src.stream));
   HIPCHK(hipMemcpyAsync(s.R, src.R, nsh * sizeof(float), hipMemcpyDeviceToDevice, s.stream));
   HIPCHK(hipMemcpyAsync(s.nr, src.nr, p.npts * sizeof(int), hipMemcpyDeviceToDevice, s.stream));
@@ -1004,6 +1066,22 @@ static void accel_solcrit(bool doaccel, float deljdot, float deljold, float delj
     solcrit = 0.0f;
 }
 
+// The Q and U rows of SH->DO and the Q and U planes of DO->SH of a polarized
+// state (its sweeps run on the level-ordered layout); nothing for NSTOKES = 1.
+static void enqueue_sh_to_do_qu(State& s, const int* stop) {
+  if (s.nst != 3) return;
+  const Plan& p = *s.plan;
+  launch_sh_to_do_perm_qu(s.stream, s.S + s.psh, s.psh, s.Yqu, s.ext, s.ns, s.oct_perm, s.sh_tiles_qu,
+                          2 * s.n_sh_tiles, s.dosrc + s.pdo, s.nptsp, s.ld, s.nptsp, p.npts, stop);
+}
+
+static void enqueue_do_to_sh_qu(State& s, const int* stop) {
+  if (s.nst != 3) return;
+  const Plan& p = *s.plan;
+  launch_do_to_sh_qu(s.stream, s.dorad + s.pdo, s.Wqu, s.nr, s.R + s.psh, s.psh, s.nptsp, s.ld, s.ldo, s.nptsp,
+                     p.npts, stop);
+}
+
 // One iteration under device control (IterCtl): the loop test and FIXSH,
 // SH->DO, the sweeps, DO->SH, the source pass with the next radiance
 // truncation, the reduction, SOLCRIT and the acceleration; every kernel
@@ -1020,14 +1098,16 @@ static void enqueue_iteration(State& s) {
                          s.nptsp, s.ld, s.nptsp, p.npts, stop);
   else
     launch_sh_to_do(s.stream, s.S, s.Y, s.ext, s.dosrc, s.nptsp, s.ldo, s.ld, s.nptsp, p.npts, p.nang, stop);
+  enqueue_sh_to_do_qu(s, stop);
   enqueue_sweeps(s, stop);
   launch_do_to_sh(s.stream, s.dorad, s.W, s.nr, s.R, s.nptsp, s.ld, s.ldo, s.nptsp, p.npts, stop);
+  enqueue_do_to_sh_qu(s, stop);
   SourceArgs sa = source_args(s, 0, 0);
   sa.ctl = s.ctl;
   launch_compute_source(s.stream, sa, s.nsrc_blocks);
   launch_reduce_dots(s.stream, s.partial, s.nsrc_blocks, s.dots, stop);
   launch_solcrit(s.stream, s.ctl, s.dots, s.errflag);
-  launch_accelerate(s.stream, s.S, s.DS, s.ns, s.nos, s.ld, p.npts, 0.0f, s.ctl);
+  launch_accelerate(s.stream, s.S, s.DS, s.ns, s.nos, s.ld, p.npts, 0.0f, s.ctl, s.nst, s.psh);
 }
 
 // The graph of ITER_BATCH iterations for the NR buffer parity k (0: s.nr is
@@ -1133,10 +1213,12 @@ static int host_iteration(State& s, bool& fixsh, float& a, int verbose, std::str
   else
     launch_sh_to_do(s.stream, s.S, s.Y, s.ext, s.dosrc, s.nptsp, s.ldo, s.ld, s.nptsp, p.npts,
                     p.nang);
+  enqueue_sh_to_do_qu(s, nullptr);
   (void)hipEventRecord(s.ev[2], s.stream);
   TRY(run_sweeps(s, err));
   (void)hipEventRecord(s.ev[3], s.stream);
   launch_do_to_sh(s.stream, s.dorad, s.W, s.nr, s.R, s.nptsp, s.ld, s.ldo, s.nptsp, p.npts);
+  enqueue_do_to_sh_qu(s, nullptr);
   (void)hipEventRecord(s.ev[4], s.stream);
   if (s.solcrit < 0.001f || s.iters > p.d.iterfixsh) fixsh = true;
   launch_compute_source(s.stream, source_args(s, 0, fixsh ? 1 : 0), s.nsrc_blocks);
@@ -1178,7 +1260,8 @@ static int host_iteration(State& s, bool& fixsh, float& a, int verbose, std::str
   s.jnorm = (float)dots[3];
   float accelpar;
   accel_solcrit(p.d.accelflag != 0, s.deljdot, s.deljold, s.deljnew, s.jnorm, a, accelpar, s.solcrit);
-  if (accelpar > 0.0f) launch_accelerate(s.stream, s.S, s.DS, s.ns, s.nos, s.ld, p.npts, accelpar);
+  if (accelpar > 0.0f)
+    launch_accelerate(s.stream, s.S, s.DS, s.ns, s.nos, s.ld, p.npts, accelpar, nullptr, s.nst, s.psh);
   if (s.albmax < p.d.solacc) s.solcrit = p.d.solacc;
   if (verbose) {
     printf("  %4d %8.3f %10.3e %8d\n", s.iters, log10f(std::max(s.solcrit, 1.0e-20f)), s.splitcrit, p.npts);

@@ -34,6 +34,14 @@ struct State {
   std::vector<int32_t> medium_cls_h;   // device medium: source classes and total extinction
   std::vector<float> medium_ext_h;     // of every point (for the ring layout)
   int device = 0;   // HIP device the state was created on
+  // polarization (NSTOKES = 3): S, R, DS hold the I, Q, U planes psh floats
+  // apart, dosrc / dorad theirs pdo floats apart; the Q / U transform tables
+  // ([2 ldo][2 ld], by device column), the a2 / a3 / b1 source terms and the
+  // b1 single-scatter terms; the SH->DO tiles of the Q and U rows
+  int nst = 1;
+  size_t psh = 0, pdo = 0;
+  float *Yqu = nullptr, *Wqu = nullptr, *chip = nullptr, *tms5 = nullptr;
+  int4* sh_tiles_qu = nullptr;
   hipStream_t stream = nullptr;
   int npts = 0, nlm = 0, nang = 0;
   int ld = 0;      // row stride of SH arrays (nlm rounded up to 64)
@@ -153,6 +161,10 @@ struct DirTables {
   std::vector<int32_t> dir_of_ray;                  // [nrays]
   std::vector<std::pair<double, double>> dirs;      // (mu, phi) of each direction
   std::vector<float> Yd, G, ssd;                    // [ldd][ld], [ndir][ml+1], [ndir][numphase]
+  // polarized: the Q / U rows [2 ldd][2 ld] (Q of direction d: elements 2 and
+  // 5; U at ldd + d: elements 6 and 3), the Q and U truncation factors
+  // [2][ndir][ml+1] after G, and the rotated I-Q phase values [ndir][numphase]
+  std::vector<float> Yqu, ssq, ssu;
   std::vector<int32_t> sa_j;                        // scattering-angle table index (1-based lower)
   std::vector<float> sa_f;                          // and interpolation fraction
 };

@@ -42,8 +42,13 @@ __global__ void k_radiance_truncation(const float* __restrict__ R, int ld, const
 // block for a fixed-order reduction.
 // NT terms per lane: NLM <= 64 * NT (4 up to L = 15, 8 up to NLM 512; the
 // smaller instantiation holds fewer registers and keeps twice the waves in
-// flight)
-template <int NT>
+// flight).  NST = 3: the polarized source (CALC_SOURCE_PNT, shdomsub1.f:
+// 823-910) of the I, Q and U planes from the a1, a2, a3 and b1 terms,
+//   S^I_J = w (a1_l R^I_J + b1_l R^Q_J) + F0 w a1_l Ysun_J
+//   S^Q_J = w (b1_l R^I_J + a2_l R^Q_J) + F0 w b1_l Ysun_J    (l >= 2)
+//   S^U_J = w a3_l R^U_J                                       (l >= 2)
+// the truncation over all three, the dot products summed over them.
+template <int NT, int NST>
 __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
   if (a.ctl && a.ctl->stop) return;
   const int fixsh = a.ctl ? a.ctl->fixsh : a.fixsh;
@@ -88,6 +93,7 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
     float* Sp = a.S + (size_t)p * a.ld;
     float* DSp = a.DS + (size_t)p * a.ld;
     const float* chip = a.chi + (size_t)p * (a.ml + 1);
+    const float* chpp = NST == 3 ? a.chip + (size_t)p * 3 * (a.ml + 1) : nullptr;
     const bool old = !a.first;
     const bool useds = old && a.accel;
     const int nsd = a.accel ? min(ns_old, nos_old) : ns_old;
@@ -95,48 +101,87 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
     // the point's rows (the first 64 orders)
     const float* tcp = a.tc + (size_t)p * (a.ml + 1);
     const float tcv = (!fixsh && 1 + lane <= a.ml) ? tcp[1 + lane] : 0.f;
-    float rv[SRC_TERMS], sv[SRC_TERMS], dv[SRC_TERMS], cv[SRC_TERMS];
+    // per Stokes plane k: radiance, old source, delta source; the phase terms
+    // a1 (and a2, a3, b1)
+    float rv[NST][SRC_TERMS], sv[NST][SRC_TERMS], dv[NST][SRC_TERMS], cv[SRC_TERMS];
+    float cp[NST == 3 ? 3 : 1][SRC_TERMS];
 #pragma unroll
     for (int it = 0; it < SRC_TERMS; it++) {
       const int J = lane + 64 * it;
-      rv[it] = sv[it] = dv[it] = cv[it] = 0.f;
+      cv[it] = 0.f;
+#pragma unroll
+      for (int k = 0; k < NST; k++) rv[k][it] = sv[k][it] = dv[k][it] = 0.f;
+#pragma unroll
+      for (int e = 0; e < (NST == 3 ? 3 : 1); e++) cp[e][it] = 0.f;
       if (J < a.nlm) {
         cv[it] = chip[lv[it]];
-        if (J < nrp) rv[it] = Rp[J];
+        if constexpr (NST == 3) {
+#pragma unroll
+          for (int e = 0; e < 3; e++) cp[e][it] = chpp[e * (a.ml + 1) + lv[it]];
+        }
+#pragma unroll
+        for (int k = 0; k < NST; k++)
+          if (J < nrp) rv[k][it] = Rp[k * a.psh + J];
       }
       // S is zero beyond NS and DS beyond NOS (both kept so below): only
       // the terms that can be nonzero are read
-      if (old && J < ns_old) sv[it] = Sp[J];
-      if (useds && J < nsd) dv[it] = DSp[J];
+#pragma unroll
+      for (int k = 0; k < NST; k++) {
+        if (old && J < ns_old) sv[k][it] = Sp[k * a.psh + J];
+        if (useds && J < nsd) dv[k][it] = DSp[k * a.psh + J];
+      }
     }
     int js = -1;
-    float st[SRC_TERMS];
+    float st[NST][SRC_TERMS];
 #pragma unroll
     for (int it = 0; it < SRC_TERMS; it++) {
       const int J = lane + 64 * it;
-      float v = 0.f;
-      if (J < a.nlm) {
-        const float c = cv[it];
-        v = f0 * alb * c * yv[it];
-        if (J < nrp) v = v + alb * c * rv[it];
-        if (fabsf(v) > a.shacc) js = J;
+      if constexpr (NST == 1) {
+        float v = 0.f;
+        if (J < a.nlm) {
+          const float c = cv[it];
+          v = f0 * alb * c * yv[it];
+          if (J < nrp) v = v + alb * c * rv[0][it];
+          if (fabsf(v) > a.shacc) js = J;
+        }
+        st[0][it] = v;
+      } else {
+        float vi = 0.f, vq = 0.f, vu = 0.f;
+        if (J < a.nlm) {
+          const float c1 = cv[it], c2 = cp[0][it], c3 = cp[1][it], c5 = cp[2][it];
+          if (J < nrp) vi = alb * (c1 * rv[0][it] + c5 * rv[1][it]);
+          vi = vi + f0 * alb * c1 * yv[it];
+          if (J >= 4) {
+            if (J < nrp) {
+              vq = alb * (c5 * rv[0][it] + c2 * rv[1][it]);
+              vu = alb * (c3 * rv[2][it]);
+            }
+            vq = vq + f0 * alb * c5 * yv[it];
+          }
+          if (fabsf(vi) > a.shacc || fabsf(vq) > a.shacc || fabsf(vu) > a.shacc) js = J;
+        }
+        st[0][it] = vi;
+        st[1][it] = vq;
+        st[2][it] = vu;
       }
-      st[it] = v;
     }
     if (old) {
 #pragma unroll
       for (int it = 0; it < SRC_TERMS; it++) {
         const int J = lane + 64 * it;
         if (J < nsd) {
-          const double so = sv[it];
-          const double dn = (double)(st[it] - sv[it]);
-          if (a.accel) {
-            const double d = dv[it];
-            dot += dn * d;
-            dold += d * d;
+#pragma unroll
+          for (int k = 0; k < NST; k++) {
+            const double so = sv[k][it];
+            const double dn = (double)(st[k][it] - sv[k][it]);
+            if (a.accel) {
+              const double d = dv[k][it];
+              dot += dn * d;
+              dold += d * d;
+            }
+            dnew += dn * dn;
+            jn += so * so;
           }
-          dnew += dn * dn;
-          jn += so * so;
         }
       }
       if (a.accel) {
@@ -146,7 +191,9 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
 #pragma unroll
         for (int it = 0; it < SRC_TERMS; it++) {
           const int J = lane + 64 * it;
-          if (J < dw) DSp[J] = (J < ns_old) ? st[it] - sv[it] : 0.f;
+#pragma unroll
+          for (int k = 0; k < NST; k++)
+            if (J < dw) DSp[k * a.psh + J] = (J < ns_old) ? st[k][it] - sv[k][it] : 0.f;
         }
         if (lane == 0) a.nos[p] = ns_old;
       }
@@ -165,7 +212,9 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
 #pragma unroll
     for (int it = 0; it < SRC_TERMS; it++) {
       const int J = lane + 64 * it;
-      if (J < sw) Sp[J] = (J < nsn && J < a.nlm) ? st[it] : 0.f;
+#pragma unroll
+      for (int k = 0; k < NST; k++)
+        if (J < sw) Sp[k * a.psh + J] = (J < nsn && J < a.nlm) ? st[k][it] : 0.f;
     }
     if (lane == 0) a.ns[p] = nsn;
     // the next iteration's radiance truncation (RADIANCE_TRUNCATION
@@ -175,7 +224,7 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
     if (fixsh) {
       nrn = a.highorder ? nterms_for_l(a.ml, a.mm) : max(4, nsn);
     } else {
-      const float i0 = __shfl(rv[0], 0);
+      const float i0 = __shfl(rv[0][0], 0);
       int lr = 1;
       if (1 + lane <= a.ml && tcv * i0 > a.radmin) lr = 1 + lane;
       for (int l = 65 + lane; l <= a.ml; l += 64)   // ML >= 64 (M much below L)
@@ -192,7 +241,9 @@ __global__ __launch_bounds__(256) void k_compute_source(SourceArgs a) {
 #pragma unroll
       for (int it = 0; it < SRC_TERMS; it++) {
         const int J = lane + 64 * it;
-        if (J < nos_old) DSp[J] = 0.f;
+#pragma unroll
+        for (int k = 0; k < NST; k++)
+          if (J < nos_old) DSp[k * a.psh + J] = 0.f;
       }
       if (lane == 0) a.nos[p] = nsn;
     }
@@ -251,7 +302,7 @@ __global__ __launch_bounds__(1024) void k_reduce_dots(const double* __restrict__
 __global__ __launch_bounds__(256) void k_accelerate(float* __restrict__ S, const float* __restrict__ DS,
                                                     const int* __restrict__ ns, const int* __restrict__ nos,
                                                     int ld, int npts, float a,
-                                                    const IterCtl* __restrict__ ctl) {
+                                                    const IterCtl* __restrict__ ctl, size_t psh) {
   if (ctl) {
     if (ctl->stop || !(ctl->a > 0.0f)) return;
     a = ctl->a;
@@ -260,8 +311,9 @@ __global__ __launch_bounds__(256) void k_accelerate(float* __restrict__ S, const
   const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (p >= npts) return;
   const int n = min(ns[p], nos[p]);
-  float* Sp = S + (size_t)p * ld;
-  const float* Dp = DS + (size_t)p * ld;
+  // blockIdx.y: the Stokes plane
+  float* Sp = S + blockIdx.y * psh + (size_t)p * ld;
+  const float* Dp = DS + blockIdx.y * psh + (size_t)p * ld;
   for (int J = lane; J < n; J += 64) Sp[J] = Sp[J] + a * Dp[J];
 }
 
@@ -273,10 +325,17 @@ void launch_radiance_truncation(hipStream_t st, const float* R, int ld, const fl
 }
 
 void launch_compute_source(hipStream_t st, const SourceArgs& a, int nblocks) {
+  if (a.nst == 3) {
+    if (a.nlm <= 256)
+      k_compute_source<4, 3><<<nblocks, 256, 0, st>>>(a);
+    else
+      k_compute_source<8, 3><<<nblocks, 256, 0, st>>>(a);
+    return;
+  }
   if (a.nlm <= 256)
-    k_compute_source<4><<<nblocks, 256, 0, st>>>(a);
+    k_compute_source<4, 1><<<nblocks, 256, 0, st>>>(a);
   else
-    k_compute_source<8><<<nblocks, 256, 0, st>>>(a);
+    k_compute_source<8, 1><<<nblocks, 256, 0, st>>>(a);
 }
 
 void launch_init_radiance(hipStream_t st, float* R, const float* R0, int* nr, int ld, int npts) {
@@ -289,8 +348,8 @@ void launch_reduce_dots(hipStream_t st, const double* partial, int nblocks, doub
 }
 
 void launch_accelerate(hipStream_t st, float* S, const float* DS, const int* ns, const int* nos,
-                       int ld, int npts, float a, const IterCtl* ctl) {
-  k_accelerate<<<(npts + 3) / 4, 256, 0, st>>>(S, DS, ns, nos, ld, npts, a, ctl);
+                       int ld, int npts, float a, const IterCtl* ctl, int nst, size_t psh) {
+  k_accelerate<<<dim3((npts + 3) / 4, nst), 256, 0, st>>>(S, DS, ns, nos, ld, npts, a, ctl, psh);
 }
 
 // The loop test of SOLUTION_ITERATIONS (shdomsub1.f:674) and FIXSH

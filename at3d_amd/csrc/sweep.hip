@@ -117,14 +117,14 @@ __device__ inline float sweep_simple(const SweepArgs& a, const OrdGeom& og, int 
 // The source row `se` is indexed by point, or through `pos` (point -> position)
 // when it is stored in an octant's level order.
 __device__ inline float sweep_general(const SweepArgs& a, const OrdGeom& og, int col, int o, int pt,
-                                      int icell, int L, const float* se, const int32_t* pos) {
+                                      int icell, int L, const float* se, const int32_t* pos, int kst = 0) {
   const int presetbit = og.bitz ? 2 : 1;
   const int32_t* rank = a.rank + (size_t)o * a.npts;
   const int32_t* level = a.level + (size_t)o * a.npts;
   const int rp = rank[pt];
   const float* ext = a.ext;
   auto SE = [se, pos](int q) { return se[pos ? pos[q] : q]; };
-  const float* rad = a.rad + (size_t)col * a.ldp;
+  const float* rad = a.rad + kst * a.pdo + (size_t)col * a.ldp;
   double xe = a.g.gridpos[pt * 3], ye = a.g.gridpos[pt * 3 + 1], ze = a.g.gridpos[pt * 3 + 2];
   double ext1 = ext[pt], srcext1 = SE(pt);
   double radv = 0.0, transmit = 1.0;
@@ -139,7 +139,7 @@ __device__ inline float sweep_general(const SweepArgs& a, const OrdGeom& og, int
     const double ext0 = f1 * ext[t.i[0]] + f2 * ext[t.i[1]] + f3 * ext[t.i[2]] + f4 * ext[t.i[3]];
     const double srcext0 = f1 * SE(t.i[0]) + f2 * SE(t.i[1]) + f3 * SE(t.i[2]) + f4 * SE(t.i[3]);
     double src, absc, trc;
-    cell_integral(ext0, ext1, srcext0, srcext1, t.so, src, absc, trc);
+    cell_integral(ext0, ext1, srcext0, srcext1, t.so, src, absc, trc, kst == 0);
     radv = radv + transmit * src * absc;
     transmit = transmit * trc;
     bool valid = true;
@@ -469,6 +469,10 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
   extern __shared__ float4 lds4[];
   const int col = col0 + blockIdx.x;
   if (FULL != (a.ring_full || a.col_general[col] != 0)) return;
+  // Stokes component of this workgroup: Q and U (kst 1, 2) see zero boundary
+  // radiances and keep the sign of the cell source (BACK_INT_GRID3D clamps
+  // only I, shdomsub1.f:3640)
+  const int kst = blockIdx.y;
   const int o = a.col_oct[col];
   const int nlev = a.ring_nlev[o];
   const OrdGeom og = a.ordg[col];
@@ -484,8 +488,8 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
   int32_t* soffl = off + a.gld;                     // [gld + 1] sectors by completion level
   uint8_t* full = reinterpret_cast<uint8_t*>(soffl + a.gld + 1);
   const int32_t* goff = a.ring_off + (size_t)o * a.gld;
-  const float* se = a.srcext + (size_t)col * a.ldp;  // ring order
-  float* rad = a.rad + (size_t)col * a.ldp;          // point order
+  const float* se = a.srcext + kst * a.pdo + (size_t)col * a.ldp;  // ring order
+  float* rad = a.rad + kst * a.pdo + (size_t)col * a.ldp;          // point order
   const uint4* rec0 = a.rec + (size_t)col * a.npts;
   const float* ep = a.ext_pos + (size_t)o * a.ldp;  // ring order
   const uint8_t* gf = a.gflag + (size_t)col * a.gld;
@@ -495,13 +499,13 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
   const uint4* sslot = a.sec_slot + a.sec_base[o];
   const bool wide = a.sector_points == 8;
   const int tid = threadIdx.x;
-  const float sky = a.skyrad[col];
+  const float sky = kst ? 0.0f : a.skyrad[col];
   const bool staging = a.col_general[col] == 0;
   const int nbc = up ? a.nbot : a.ntop;
   const int32_t* bc = up ? a.bcbot : a.bctop;
   // boundary radiances (PATH_INTEGRATION shdomsub1.f:2042-2086) into the
   // point-ordered row first: multi-cell traces read them there
-  for (int b = tid; b < nbc; b += RING_THREADS) rad[bc[b]] = up ? a.bcrad_bot[b] : sky;
+  for (int b = tid; b < nbc; b += RING_THREADS) rad[bc[b]] = kst ? 0.0f : (up ? a.bcrad_bot[b] : sky);
   for (int L = tid; L <= nlev; L += RING_THREADS) off[L] = goff[L];
   for (int L = tid; L <= nlev + 1; L += RING_THREADS) soffl[L] = soff[L];
   for (int L = tid; L < a.gld; L += RING_THREADS) full[L] = gf[L];
@@ -518,7 +522,8 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
     const float r0 = f1 * q0.x + f2 * q1.x + f3 * q2.x + f4 * q3.x;
     float b0, b1, trc;
     cell_coefs(__uint_as_float(w.w), e1, g.z, b0, b1, trc);
-    return fmaxf(0.f, b0 * s0 + b1 * s1) + trc * r0;
+    const float sv = b0 * s0 + b1 * s1;
+    return (kst ? sv : fmaxf(0.f, sv)) + trc * r0;
   };
   auto class_of = [&](uint32_t head) {
     return step_class(a, (int)(head & RECORD_PT_MASK), (int8_t)((head >> RECORD_PT_BITS) & 3));
@@ -567,7 +572,7 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
       if (kind == 1) continue;
       float val;
       if (kind == 2) {
-        val = __uint_as_float(w[u].z);
+        val = kst ? 0.0f : __uint_as_float(w[u].z);
       } else {
         val = kind == 3 ? q[u][0].x : one_cell(w[u], q[u][0], q[u][1], q[u][2], q[u][3], gq[u], sx[u], ex[u]);
         if (!(staging && ((head >> 29) & 1))) rad[head & RECORD_PT_MASK] = val;
@@ -688,7 +693,7 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
       }
       float val;
       if (kind == 2) {
-        val = __uint_as_float(w0.z);
+        val = kst ? 0.0f : __uint_as_float(w0.z);
       } else {
         // kind 3: a zero-length step onto a corner, whose radiance it takes
         val = kind == 3 ? g[k][0].x
@@ -715,7 +720,7 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
         const int p = head & RECORD_PT_MASK;
         const float s1 = se[jl];
         const float val = sweep_general(a, og, col, o, p, a.ring_cell[(size_t)o * a.npts + jl], L, se,
-                                        pos);
+                                        pos, kst);
         rad[p] = val;
         put(jl, k0, val, s1);
       }
@@ -835,8 +840,9 @@ void launch_sweep_geo(hipStream_t st, const SweepArgs& a, int ncols) {
 void launch_sweep_ring(hipStream_t st, const SweepArgs& a, int col0, int ncol, size_t lds_fast,
                        size_t lds_full, bool fast, bool full) {
   if (ncol <= 0) return;
-  if (fast) k_sweep_ring<false><<<ncol, RING_THREADS, lds_fast, st>>>(a, col0);
-  if (full) k_sweep_ring<true><<<ncol, RING_THREADS, lds_full, st>>>(a, col0);
+  const dim3 grid(ncol, a.nst);
+  if (fast) k_sweep_ring<false><<<grid, RING_THREADS, lds_fast, st>>>(a, col0);
+  if (full) k_sweep_ring<true><<<grid, RING_THREADS, lds_full, st>>>(a, col0);
 }
 
 int sweep_ring_max_lds(size_t lds_fast, size_t lds_full) {

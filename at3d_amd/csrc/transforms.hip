@@ -49,12 +49,15 @@ __device__ inline void mfma_tile(const float (*sA)[TM + PAD], const float (*sB)[
 // selects the SH->DO epilogue max(0, C) * ext[m] stored ordinate-major,
 // out[n * ldo + m] (ldo = padded point count), four consecutive points per
 // lane as one 16-byte store; without it C is stored as is, out[m * ldo + n].
-template <bool CLAMP_EXT>
+// QU: K runs over two planes of A (Q, then U `plane` floats on) and B rows
+// are 2 ld long.
+template <bool CLAMP_EXT, bool QU = false>
 __global__ __launch_bounds__(256) void k_sh_to_do(const float* __restrict__ S,
                                                   const float* __restrict__ Yc,
                                                   const float* __restrict__ ext,
                                                   float* __restrict__ out, int ld, int ldo,
-                                                  int npts, int nang, const int* __restrict__ stop) {
+                                                  int npts, int nang, const int* __restrict__ stop,
+                                                  size_t plane = 0) {
   if (stop && *stop) return;
   __shared__ float sA[2][TK][TM + PAD];
   __shared__ float sB[2][TK][TN + PAD];
@@ -65,12 +68,15 @@ __global__ __launch_bounds__(256) void k_sh_to_do(const float* __restrict__ S,
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++) acc[i][j] = (float4_){0.f, 0.f, 0.f, 0.f};
   const int lr = tid >> 2, lc = (tid & 3) * 4;
+  const int kt = QU ? 2 * ld : ld;   // K extent and B row length
   const float* arow = S + (size_t)(m0 + lr) * ld + lc;
-  const float* brow = Yc + (size_t)(n0 + lr) * ld + lc;
+  const float* brow = Yc + (size_t)(n0 + lr) * kt + lc;
+  // K index k of A: the Q plane below ld, the U plane above
+  auto aptr = [&](int k) { return (QU && k >= ld) ? arow + plane + (k - ld) : arow + k; };
   // double-buffered LDS: the next K step's global loads fly during this step's MFMAs
   float4 a = *reinterpret_cast<const float4*>(arow);
   float4 b = *reinterpret_cast<const float4*>(brow);
-  for (int k0 = 0, buf = 0; k0 < ld; k0 += TK, buf ^= 1) {
+  for (int k0 = 0, buf = 0; k0 < kt; k0 += TK, buf ^= 1) {
     sA[buf][lc + 0][lr] = a.x;
     sA[buf][lc + 1][lr] = a.y;
     sA[buf][lc + 2][lr] = a.z;
@@ -80,8 +86,8 @@ __global__ __launch_bounds__(256) void k_sh_to_do(const float* __restrict__ S,
     sB[buf][lc + 2][lr] = b.z;
     sB[buf][lc + 3][lr] = b.w;
     __syncthreads();
-    if (k0 + TK < ld) {
-      a = *reinterpret_cast<const float4*>(arow + k0 + TK);
+    if (k0 + TK < kt) {
+      a = *reinterpret_cast<const float4*>(aptr(k0 + TK));
       b = *reinterpret_cast<const float4*>(brow + k0 + TK);
     }
     mfma_tile(sA[buf], sB[buf], wm, wn, lane, acc);
@@ -122,7 +128,10 @@ __global__ __launch_bounds__(256) void k_sh_to_do(const float* __restrict__ S,
 // (COMPUTE_SOURCE keeps NS terms, shdomsub1.f:967-1611), so the K loop stops
 // at the largest ns of the tile's rows, rounded up to TK: the terms skipped
 // are exact zeros, and the sweep's order groups points of like ns.
+// QU: the polarized Q / U rows, K over the Q plane's kn terms then the U
+// plane's (`plane` floats on), B rows 2 ld long, no clamp at zero.
 constexpr int PN = 48;
+template <bool QU>
 __global__ __launch_bounds__(256) void k_sh_to_do_perm(const float* __restrict__ S,
                                                        const float* __restrict__ Yc,
                                                        const float* __restrict__ ext,
@@ -130,7 +139,8 @@ __global__ __launch_bounds__(256) void k_sh_to_do_perm(const float* __restrict__
                                                        const int32_t* __restrict__ oct_perm,
                                                        const int4* __restrict__ tiles,
                                                        float* __restrict__ out, int ld, int ldp,
-                                                       int npts, const int* __restrict__ stop) {
+                                                       int npts, const int* __restrict__ stop,
+                                                       size_t plane) {
   if (stop && *stop) return;
   __shared__ float sA[2][TK][TM + PAD];
   __shared__ float sB[2][TK][PN + PAD];
@@ -152,11 +162,15 @@ __global__ __launch_bounds__(256) void k_sh_to_do_perm(const float* __restrict__
   const int lr = tid >> 2, lc = (tid & 3) * 4;
   const float* arow = S + (size_t)perm[m0 + lr] * ld + lc;
   const bool bload = lr < nc;  // threads 0..4*nc-1 load the column tile
-  const float* brow = Yc + (size_t)(c0 + (bload ? lr : 0)) * ld + lc;
+  const float* brow = Yc + (size_t)(c0 + (bload ? lr : 0)) * (QU ? 2 * ld : ld) + lc;
+  // step kk of the K loop: term kk of the Q plane, or kk - kn of the U plane
+  const int kend = QU ? 2 * kn : kn;
+  auto aptr = [&](int kk) { return (QU && kk >= kn) ? arow + plane + (kk - kn) : arow + kk; };
+  auto bptr = [&](int kk) { return (QU && kk >= kn) ? brow + ld + (kk - kn) : brow + kk; };
   float4 a = *reinterpret_cast<const float4*>(arow);
   float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
   if (bload) b = *reinterpret_cast<const float4*>(brow);
-  for (int k0 = 0, buf = 0; k0 < kn; k0 += TK, buf ^= 1) {
+  for (int k0 = 0, buf = 0; k0 < kend; k0 += TK, buf ^= 1) {
     sA[buf][lc + 0][lr] = a.x;
     sA[buf][lc + 1][lr] = a.y;
     sA[buf][lc + 2][lr] = a.z;
@@ -168,9 +182,9 @@ __global__ __launch_bounds__(256) void k_sh_to_do_perm(const float* __restrict__
       sB[buf][lc + 3][lr] = b.w;
     }
     __syncthreads();
-    if (k0 + TK < kn) {
-      a = *reinterpret_cast<const float4*>(arow + k0 + TK);
-      if (bload) b = *reinterpret_cast<const float4*>(brow + k0 + TK);
+    if (k0 + TK < kend) {
+      a = *reinterpret_cast<const float4*>(aptr(k0 + TK));
+      if (bload) b = *reinterpret_cast<const float4*>(bptr(k0 + TK));
     }
     float av[TK / 4], bv[TK / 4][3];
 #pragma unroll
@@ -194,21 +208,31 @@ __global__ __launch_bounds__(256) void k_sh_to_do_perm(const float* __restrict__
     const int c = j * 16 + (lane & 15);
     if (c >= nc) continue;
     float4 o;
-    o.x = fmaxf(acc[j][0], 0.0f) * e[0];
-    o.y = fmaxf(acc[j][1], 0.0f) * e[1];
-    o.z = fmaxf(acc[j][2], 0.0f) * e[2];
-    o.w = fmaxf(acc[j][3], 0.0f) * e[3];
+    if (QU) {
+      o.x = acc[j][0] * e[0];
+      o.y = acc[j][1] * e[1];
+      o.z = acc[j][2] * e[2];
+      o.w = acc[j][3] * e[3];
+    } else {
+      o.x = fmaxf(acc[j][0], 0.0f) * e[0];
+      o.y = fmaxf(acc[j][1], 0.0f) * e[1];
+      o.z = fmaxf(acc[j][2], 0.0f) * e[2];
+      o.w = fmaxf(acc[j][3], 0.0f) * e[3];
+    }
     *reinterpret_cast<float4*>(out + (size_t)(c0 + c) * ldp + q0) = o;
   }
 }
 
 // C[m][n] = sum_k A[m][k] B[k][n]: A stored ordinate-major (A[m][k] at
-// D[k * ldp + m]), B N-contiguous rows.
+// D[k * ldp + m]), B N-contiguous rows.  QU: K over 2 ldo rows (the Q rows,
+// then the U rows that follow them), B rows 2 ld long, output column n into
+// plane n / ld of R (R_Q, then R_U `plane` floats on) at term n % ld.
+template <bool QU>
 __global__ __launch_bounds__(256) void k_do_to_sh(const float* __restrict__ D,
                                                   const float* __restrict__ Wc,
                                                   const int* __restrict__ nr,
                                                   float* __restrict__ R, int ld, int ldo, int ldp,
-                                                  int npts, const int* __restrict__ stop) {
+                                                  int npts, const int* __restrict__ stop, size_t plane) {
   if (stop && *stop) return;
   __shared__ float sA[2][TK][TM + PAD];
   __shared__ float sB[2][TK][TN + PAD];
@@ -229,7 +253,12 @@ __global__ __launch_bounds__(256) void k_do_to_sh(const float* __restrict__ D,
     mt = blockIdx.y;
     nt = blockIdx.x;
   }
-  const int m0 = mt * TM, n0 = nt * TN;
+  const int m0 = mt * TM;
+  const int ldb = QU ? 2 * ld : ld, kext = QU ? 2 * ldo : ldo;
+  const int nb = nt * TN;                  // column of B
+  const int pl = QU ? nb / ld : 0;         // output plane
+  const int n0 = nb - pl * ld;             // first SH term of the tile
+  R += pl * plane;
   // an SH tile at or past every radiance count of the point tile is zero:
   // written as such without the product (radiance truncation leaves most
   // clear-sky points a few low-order terms)
@@ -253,16 +282,16 @@ __global__ __launch_bounds__(256) void k_do_to_sh(const float* __restrict__ D,
     for (int j = 0; j < 2; j++) acc[i][j] = (float4_){0.f, 0.f, 0.f, 0.f};
   const int bk = tid >> 4, bc = (tid & 15) * 4;    // 16 k x 64 rows/cols per tile
   const float* acol = D + (size_t)bk * ldp + m0 + bc;
-  const float* bcol = Wc + (size_t)bk * ld + n0 + bc;
+  const float* bcol = Wc + (size_t)bk * ldb + nb + bc;
   float4 a = *reinterpret_cast<const float4*>(acol);
   float4 b = *reinterpret_cast<const float4*>(bcol);
-  for (int k0 = 0, buf = 0; k0 < ldo; k0 += TK, buf ^= 1) {
+  for (int k0 = 0, buf = 0; k0 < kext; k0 += TK, buf ^= 1) {
     *reinterpret_cast<float4*>(&sA[buf][bk][bc]) = a;
     *reinterpret_cast<float4*>(&sB[buf][bk][bc]) = b;
     __syncthreads();
-    if (k0 + TK < ldo) {
+    if (k0 + TK < kext) {
       a = *reinterpret_cast<const float4*>(acol + (size_t)(k0 + TK) * ldp);
-      b = *reinterpret_cast<const float4*>(bcol + (size_t)(k0 + TK) * ld);
+      b = *reinterpret_cast<const float4*>(bcol + (size_t)(k0 + TK) * ldb);
     }
     mfma_tile(sA[buf], sB[buf], wm, wn, lane, acc);
   }
@@ -302,13 +331,34 @@ void launch_sh_to_do_perm(hipStream_t st, const float* S, const float* Yc, const
                           float* dosrc, int mp, int ld, int ldp, int npts, const int* stop) {
   if (ntiles <= 0) return;
   dim3 grid(mp / TM, ntiles);
-  k_sh_to_do_perm<<<grid, 256, 0, st>>>(S, Yc, ext, ns, oct_perm, tiles, dosrc, ld, ldp, npts, stop);
+  k_sh_to_do_perm<false><<<grid, 256, 0, st>>>(S, Yc, ext, ns, oct_perm, tiles, dosrc, ld, ldp, npts, stop, 0);
+}
+
+void launch_sh_to_do_perm_qu(hipStream_t st, const float* S, size_t plane, const float* Yc, const float* ext,
+                             const int* ns, const int32_t* oct_perm, const int4* tiles, int ntiles,
+                             float* dosrc, int mp, int ld, int ldp, int npts, const int* stop) {
+  if (ntiles <= 0) return;
+  dim3 grid(mp / TM, ntiles);
+  k_sh_to_do_perm<true><<<grid, 256, 0, st>>>(S, Yc, ext, ns, oct_perm, tiles, dosrc, ld, ldp, npts, stop,
+                                              plane);
 }
 
 void launch_do_to_sh(hipStream_t st, const float* dorad, const float* Wc, const int* nr, float* R,
                      int mp, int ld, int ldo, int ldp, int npts, const int* stop) {
   dim3 grid(ld / TN, mp / TM);
-  k_do_to_sh<<<grid, 256, 0, st>>>(dorad, Wc, nr, R, ld, ldo, ldp, npts, stop);
+  k_do_to_sh<false><<<grid, 256, 0, st>>>(dorad, Wc, nr, R, ld, ldo, ldp, npts, stop, 0);
+}
+
+void launch_do_to_sh_qu(hipStream_t st, const float* dorad, const float* Wc, const int* nr, float* R, size_t plane,
+                        int mp, int ld, int ldo, int ldp, int npts, const int* stop) {
+  dim3 grid(2 * ld / TN, mp / TM);
+  k_do_to_sh<true><<<grid, 256, 0, st>>>(dorad, Wc, nr, R, ld, ldo, ldp, npts, stop, plane);
+}
+
+void launch_sh_dot_qu(hipStream_t st, const float* S, size_t plane, const float* Yd, float* raw, int mp,
+                      int ndp, int ld, int ldo, int npts) {
+  dim3 grid(ndp / TN, mp / TM);
+  k_sh_to_do<false, true><<<grid, 256, 0, st>>>(S, Yd, nullptr, raw, ld, ldo, npts, ndp, nullptr, plane);
 }
 
 }  // namespace at3d

@@ -3,7 +3,9 @@
 Mirrors the fields of the reference's scatterer xr.Dataset
 (at3d/solver.py:1689-1760 checks; medium.py): ``extinction`` and ``ssalb`` on
 an (x, y, z) grid, Legendre tables ``legcoef`` (legendre_index, table_index)
-including the (2l+1) factor, 1-based ``table_index`` and ``phase_weights``.
+including the (2l+1) factor, or (stokes_index, legendre_index, table_index)
+with the six phase-matrix elements a1, a2, a3, a4, b1, b2 for polarized
+solves, 1-based ``table_index`` and ``phase_weights``.
 """
 from __future__ import annotations
 
@@ -16,7 +18,7 @@ import numpy as np
 class Scatterer:
     extinction: np.ndarray                 # (nx, ny, nz) 1/km
     ssalb: np.ndarray                      # (nx, ny, nz)
-    legcoef: np.ndarray                    # (nleg+1, ntable) scalar I-I coefficients
+    legcoef: np.ndarray                    # (nleg+1, ntable) I-I, or (6, nleg+1, ntable)
     delx: float
     dely: float
     z: np.ndarray                          # (nz,) km
@@ -29,6 +31,8 @@ class Scatterer:
         leg = np.asarray(self.legcoef, np.float64)
         if leg.ndim == 1:
             leg = leg[:, None]
+        if leg.ndim == 3 and leg.shape[0] != 6:
+            raise ValueError("a polarized Legendre table has six elements (stokes_index)")
         self.legcoef = leg
         self.z = np.asarray(self.z, np.float32)
         shp = self.extinction.shape

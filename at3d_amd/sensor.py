@@ -25,9 +25,9 @@ def make_sensor_dataset(x, y, z, mu, phi, stokes, wavelength):
         raise ValueError("All altitudes (z) must be >= 0.0")
     if np.any(mu == 0.0):
         raise ValueError("'mu' values of 0.0 are not allowed.")
-    stokes = list(np.atleast_1d(stokes))
-    if stokes != ["I"]:
-        raise ValueError("only the 'I' Stokes component is supported (num_stokes=1)")
+    stokes = [str(v) for v in np.atleast_1d(stokes)]
+    if not stokes or any(v not in ("I", "Q", "U") for v in stokes) or len(set(stokes)) != len(stokes):
+        raise ValueError("stokes must name distinct components among 'I', 'Q', 'U' (NSTOKES <= 3)")
     s = {"wavelength": float(wavelength), "stokes": stokes,
          "cam_x": x.astype(np.float64), "cam_y": y.astype(np.float64),
          "cam_z": z.astype(np.float64), "cam_mu": mu.astype(np.float64),

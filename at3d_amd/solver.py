@@ -31,11 +31,11 @@ class RTE:
 
     def __init__(self, numerical_params, medium, source, surface, num_stokes=1, name=None,
                  atmosphere=None, device_setup=True):
-        if num_stokes != 1:
-            raise ValueError("only num_stokes=1 is supported by this engine")
+        if num_stokes not in (1, 3):
+            raise ValueError("num_stokes must be 1 or 3 (NSTOKES = 4 is not supported by this engine)")
         if atmosphere is not None:
             raise ValueError("atmosphere (thermal / gas absorption) is not supported")
-        self._nstokes = 1
+        self._nstokes = int(num_stokes)
         self.numerical_params = dict(get_config()) if numerical_params is None else dict(numerical_params)
         self.source = dict(source)
         self.surface = dict(surface)
@@ -80,17 +80,27 @@ class RTE:
         npar = self.numerical_params
         scs = list(self.medium.values())
         npx, npy, npz = scs[0].shape
-        # concatenated Legendre table (solver.py:2340-2395)
-        maxleg = max(s.legcoef.shape[0] for s in scs)
+        # concatenated Legendre table (solver.py:2340-2411): the I-I element
+        # for a scalar solve, all six elements (a scalar table is a1 alone)
+        # for a polarized one
+        nstleg = 1 if self._nstokes == 1 else 6
+
+        def elements(leg):
+            if leg.ndim == 2:
+                leg = np.concatenate([leg[None], np.zeros((5,) + leg.shape)], axis=0)
+            return leg[:nstleg]
+        legs = [elements(s.legcoef) for s in scs]
+        maxleg = max(t.shape[1] for t in legs)
         tables, offset = [], 0
         iphasep = []
-        for s in scs:
-            t = np.pad(s.legcoef, ((0, maxleg - s.legcoef.shape[0]), (0, 0)))
+        for s, t in zip(scs, legs):
+            t = np.pad(t, ((0, 0), (0, maxleg - t.shape[1]), (0, 0)))
             tables.append(t)
             iphasep.append(np.asarray(s.table_index, np.int32) + offset)
-            offset += t.shape[1]
-        table = np.concatenate(tables, axis=1)            # (nleg+1, numphase)
-        legenp = _lib.as_f32(table.T)                     # [numphase][nlegp+1]
+            offset += t.shape[2]
+        table3 = np.concatenate(tables, axis=2)           # (nstleg, nleg+1, numphase)
+        table = table3[0]                                 # (nleg+1, numphase)
+        legenp = _lib.as_f32(np.transpose(table3, (2, 1, 0)))   # [numphase][nlegp+1][nstleg]
         ext = _lib.as_f32(np.stack([s.extinction for s in scs]))
         alb = _lib.as_f32(np.stack([s.ssalb for s in scs]))
         iph = _lib.as_i32(np.stack(iphasep))
@@ -130,6 +140,7 @@ class RTE:
         d.transcut = float(npar["transcut"])
         d.maxig, d.maxic, d.maxiv, d.maxido = self._memory_params(npar, npx, npy, npz, table)
         d.device_medium = 1 if self._device_setup else 0
+        d.nstokes = self._nstokes
         self._keep = [ext, alb, iph, pw, zl, legenp]
         self._desc = d
         L = _lib.lib()
@@ -170,9 +181,11 @@ class RTE:
         big_arrays = 3 if bool(npar["acceleration_flag"]) else 2
         maxpg = npx * npy * npz
         numphase, maxleg = table.shape[1], table.shape[0]   # (the legendre_index size)
-        memword = nmu * (2 + 2 * nphi + 2 * nlm + 2 * 33 * 32) + 4.5 * maxpg + 2 * numphase * maxleg
+        nst = self._nstokes
+        nstleg = 1 if nst == 1 else 6
+        memword = nmu * (2 + 2 * nphi + 2 * nlm + 2 * 33 * 32) + 4.5 * maxpg + 2 * numphase * nstleg * maxleg
         ratio, shfac = float(npar["cell_to_point_ratio"]), float(npar["num_sh_term_factor"])
-        wantmem = adapt * nbpts * (28 + 16.5 * ratio + nphi0max + shfac * nlm * big_arrays)
+        wantmem = adapt * nbpts * (28 + 16.5 * ratio + nphi0max * nst + shfac * nst * nlm * big_arrays)
         reduce = min(1.0, ((max_total_mb * 1024 ** 2) / 4 - memword) / wantmem)
         adapt *= reduce
         if adapt < 1.0:
@@ -233,6 +246,14 @@ class RTE:
         "bcptr_bot": ("i4", lambda s: (s.plan_size("nbot"),)),
         "xgrid": ("f4", lambda s: (s.plan_size("nx1"),)), "ygrid": ("f4", lambda s: (s.plan_size("ny1"),)),
         "zgrid": ("f4", lambda s: (s.plan_size("nz"),)),
+        "ypol": ("f4", lambda s: (6, s._nang, s._nlm)), "wpol": ("f4", lambda s: (6, s._nang, s._nlm)),
+        "phasetab": ("f4", lambda s: (s.plan_size("numphase"), s.plan_size("nscatangle"))),
+        "phasetab2": ("f4", lambda s: (s.plan_size("numphase"), s.plan_size("nscatangle"))),
+        "legen": ("f4", lambda s: (s.plan_size("numphase"), s.plan_size("nleg") + 1)),
+        "legen2": ("f4", lambda s: (s.plan_size("numphase"), s.plan_size("nleg") + 1)),
+        "legen3": ("f4", lambda s: (s.plan_size("numphase"), s.plan_size("nleg") + 1)),
+        "legen5": ("f4", lambda s: (s.plan_size("numphase"), s.plan_size("nleg") + 1)),
+        "chip": ("f4", lambda s: (s._nbpts, 3, s._ml + 1)), "tms5": ("f4", lambda s: (s._nbpts, s._ml + 1)),
     }
 
     def plan_array(self, name: str) -> np.ndarray:
@@ -507,6 +528,10 @@ class RTE:
         for k in ("ray_x", "ray_y", "ray_z", "ray_mu", "ray_phi"):
             if k not in sensor:
                 raise KeyError(f"sensor is missing '{k}'")
+        requested = [str(v) for v in np.atleast_1d(sensor.get("stokes", ["I"]))]
+        if len(requested) > self._nstokes:
+            raise ValueError("'{}' Stokes components are requested by the sensor but the solver "
+                             "has nstokes={}".format(len(requested), self._nstokes))
         self.check_solved()
         self._ensure_state()
         x = _lib.as_f32(sensor["ray_x"])
@@ -517,13 +542,17 @@ class RTE:
         n = x.size
         if not all(a.size == n for a in (y, z, mu, phi)):
             raise ValueError("ray arrays must have the same length")
-        out = np.zeros(n, np.float32)
+        out = np.zeros((n, self._nstokes), np.float32)
         err = _lib.errbuf()
         _lib.check(_lib.lib().at3d_render(self._state, n, x.ctypes.data, y.ctypes.data,
                                           z.ctypes.data, mu.ctypes.data, phi.ctypes.data,
                                           int(single_scatter), int(nosurface), out.ctypes.data,
                                           err, len(err)), err)
-        sensor["I"] = out
+        sensor["I"] = np.ascontiguousarray(out[:, 0])
+        if self._nstokes > 1:
+            # Stokes parameters for linear polarization (solver.py:768-781)
+            sensor["Q"] = np.ascontiguousarray(out[:, 1])
+            sensor["U"] = np.ascontiguousarray(out[:, 2])
         return sensor
 
     def check_solved(self, verbose=True):
@@ -546,11 +575,13 @@ class RTE:
         return self._solcrit
 
     def state_array(self, name: str) -> np.ndarray:
-        """Copy a device array: source, radiance, delsource [npts, nlm]; fluxes [2, npts];
+        """Copy a device array: source, radiance, delsource [npts, nlm] (polarized:
+        [3, npts, nlm], the I, Q and U terms); fluxes [2, npts];
         dirflux [npts]; ns, nr [npts]; do_radiance, do_source [nang, npts]."""
         if self._state is None:
             raise RuntimeError("solve() has not been called")
         n, ml1 = self._npts, self._ml + 1
+        sh = (n, self._nlm) if self._nstokes == 1 else (self._nstokes, n, self._nlm)
         if name in ("point_ss_iph", "point_ss_coef"):
             off = self.state_array("point_ss_off")
             shp, dt = (int(off[-1]),), (np.int32 if name == "point_ss_iph" else np.float32)
@@ -563,9 +594,7 @@ class RTE:
                   "point_chi": ((n, ml1), np.float32), "point_tc": ((n, ml1), np.float32),
                   "point_tms": ((n, ml1), np.float32), "point_radiance0": ((n, 4), np.float32),
                   "point_ss_off": ((n + 1,), np.int32),
-                  "source": ((self._npts, self._nlm), np.float32),
-                  "radiance": ((self._npts, self._nlm), np.float32),
-                  "delsource": ((self._npts, self._nlm), np.float32),
+                  "source": (sh, np.float32), "radiance": (sh, np.float32), "delsource": (sh, np.float32),
                   "fluxes": ((2, self._npts), np.float32), "dirflux": ((self._npts,), np.float32),
                   "ns": ((self._npts,), np.int32), "nr": ((self._npts,), np.int32),
                   "do_radiance": ((self._nang, self._npts), np.float32),

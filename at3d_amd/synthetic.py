@@ -192,7 +192,7 @@ def make_rte(w: Workload, device_setup=True, **overrides):
     med = {f"species_{i}": Scatterer(e, a, leg, w.delx, w.dely, w.z)
            for i, (e, a, leg) in enumerate(w.species)}
     return RTE(cfg, med, source.solar(w.wavelength, w.solarmu, w.solaraz_deg),
-               surface.lambertian(w.gndalbedo), device_setup=device_setup)
+               surface.lambertian(w.gndalbedo), num_stokes=w.nstokes, device_setup=device_setup)
 
 
 # Serving stream: grids, (NMU, NPHI) angle sets (L = NMU - 1) and wavelengths

@@ -41,7 +41,10 @@ typedef struct at3d_scene_desc {
   const float* phasewtp;   /* [npart][npx][npy][npz] phase weights (num_micro=1) */
   int32_t numphase;        /* number of tabulated phase functions              */
   int32_t nlegp;           /* highest Legendre index in legenp                 */
-  const float* legenp;     /* [numphase][nlegp+1], coefficients incl. (2l+1)   */
+  const float* legenp;     /* [numphase][nlegp+1], coefficients incl. (2l+1);
+                              with nstokes 3: [numphase][nlegp+1][6], the six
+                              phase-matrix elements a1 a2 a3 a4 b1 b2 of each
+                              order (the reference's LEGENP(NSTLEG,0:NLEG,*)) */
   /* source / surface */
   float solarflux;         /* flux on a horizontal surface                     */
   float solarmu;           /* cosine of solar zenith, direction of propagation (<0) */
@@ -75,6 +78,11 @@ typedef struct at3d_scene_desc {
    * the plan's per-point arrays are then made on the host only when asked
    * for.  Same values either way. */
   int32_t device_medium;
+  /* Stokes components (num_stokes, solver.py:150): 1 (0 is taken as 1), or 3
+   * for the polarized solve and render (I, Q, U; six-element phase matrices,
+   * NSTLEG = 6).  A polarized solve runs on the base grid (splitacc 0), with
+   * the per-point medium made on the host, and has no Levis gradient. */
+  int32_t nstokes;
 } at3d_scene_desc;
 
 typedef struct at3d_plan at3d_plan;   /* host-side setup (no GPU needed)     */
@@ -119,7 +127,13 @@ int64_t at3d_plan_size(const at3d_plan* plan, const char* key);
  *  "chi" f32[npts][ml+1]       "alb" f32[npts]        "sweepord" i32[8][npts]
  *  "level" i32[8][npts]        "bcptr_top" i32[ntop]  "bcptr_bot" i32[nbot]
  * Sweep order entries are (cell<<3)|corner, 0-based, indexed by the octant
- * number IOCT-1 of the reference (not its JOCT slot). */
+ * number IOCT-1 of the reference (not its JOCT slot).
+ * Polarized plans (nstokes 3) also give "ypol" / "wpol" f32[6][nang][nlm]
+ * (the six-element synthesis and analysis tables of the ordinates),
+ * "phasetab" / "phasetab2" f32[numphase][nscatangle] (I-I and I-Q phase
+ * values), "legen" and "legen2" / "legen3" / "legen5" f32[numphase][nleg+1]
+ * (the prepared a1, a2, a3, b1 tables), "chip" f32[npts][3][ml+1] (the
+ * a2, a3, b1 source terms) and "tms5" f32[npts][ml+1]. */
 int at3d_plan_get(const at3d_plan* plan, const char* name, void* dst, size_t nbytes,
                   char* err, size_t errlen);
 
@@ -146,7 +160,9 @@ int at3d_solve(at3d_state* st, int32_t maxiter, int32_t verbose, float* solcrit_
                int32_t* iters_out, char* err, size_t errlen);
 
 /* Copy a named state array to host.  Names: "source" f32[npts][nlm],
- * "radiance" f32[npts][nlm], "delsource" f32[npts][nlm], "fluxes" f32[2][npts],
+ * "radiance" f32[npts][nlm], "delsource" f32[npts][nlm] (with nstokes 3
+ * f32[3][npts][nlm]: the I, Q and U terms in the polarized real generalised
+ * spherical harmonics of YLMALL, shdomsub2.f:4244), "fluxes" f32[2][npts],
  * "dirflux" f32[npts], "ns" i32[npts], "nr" i32[npts], "do_radiance" f32[nang][npts],
  * "do_source" f32[nang][npts] (max(0,S)*ext); and the state's current grid,
  * which adaptive cell splitting refines during at3d_solve (the base grid
@@ -172,7 +188,10 @@ int64_t at3d_state_size(const at3d_state* st, const char* key);
  * shdomsub4.f:93-286, with INTEGRATE_1RAY shdomsub2.f:2311-2743 and the TMS
  * single-scatter correction of COMPUTE_SOURCE_1CELL_UNPOL :3046-3192).
  * x,y,z: ray positions (km); mu, phi: cosine zenith and azimuth (radians) of the
- * direction TOWARDS the sensor (at3d/sensor.py conventions); out: I per ray.
+ * direction TOWARDS the sensor (at3d/sensor.py conventions); out: I per ray,
+ * or with nstokes 3 the Stokes vector (I, Q, U) per ray, f32[nrays][3] (the
+ * reference's STOKES(NSTOKES, NPIX); Q and U in the meridian plane of the
+ * ray, the solar single scatter rotated by ROTATE_POL_PLANE shdomsub2.f:3277).
  * Fails with the reference's errors: negative tabulated phase function
  * (PRECOMPUTE_PHASE_CHECK), ray below the domain. */
 int at3d_render(at3d_state* st, int64_t nrays, const float* x, const float* y, const float* z,

@@ -31,6 +31,19 @@ def cloud_l15(seed=2, n=16, nz=12):
                          nmu=16, nphi=32, solacc=1e-5)
 
 
+def pol_blob(nmu=8, nphi=16, bc="open", n=8, seed=0):
+    """Polarized (NSTOKES = 3) two-species scene: the 8^3 lognormal blob with
+    a six-element cloud phase matrix (64 orders) plus Rayleigh scattering
+    (depolarization 0.03, 0.05 / km) everywhere."""
+    w = S.lognormal_blob(n=n, radius=0.15, seed=seed, nleg=64, bc=bc, nmu=nmu, nphi=nphi)
+    e, a, _ = w.species[0]
+    rext = np.full(e.shape, 0.05, np.float32)
+    ralb = np.ones(e.shape, np.float32)
+    species = [(e, a, S.polarized_cloud_table(0.85, 64, seed=seed + 1)), (rext, ralb, S.rayleigh_phase_matrix(0.03))]
+    return S.Workload("pol_blob", w.delx, w.dely, w.z, species, bc=bc, nmu=nmu, nphi=nphi, solacc=1e-5,
+                      nstokes=3)
+
+
 def rel_l2(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
