@@ -610,10 +610,6 @@ int at3d_state_set_option(at3d_state* st, const char* key, int32_t value, char* 
     return AT3D_ERR_ARG;
   }
   const std::string k(key);
-  if (st->s.nst != 1 && ((k == "sweep_general" && value) || (k == "sweep_ring" && !value))) {
-    set_err(err, errlen, "a polarized state sweeps on the level-ordered layout only");
-    return AT3D_ERR_ARG;
-  }
   if (k == "sweep_general") {
     st->s.force_general = value ? 1 : 0;
     return AT3D_OK;

@@ -44,6 +44,11 @@ void launch_sh_to_do(hipStream_t st, const float* S, const float* Yc, const floa
 void launch_sh_to_do_perm(hipStream_t st, const float* S, const float* Yc, const float* ext,
                           const int* ns, const int32_t* oct_perm, const int4* tiles, int ntiles,
                           float* dosrc, int mp, int ld, int ldp, int npts, const int* stop = nullptr);
+// The polarized Q and U rows of the point-ordered SH->DO (no clamp): S the
+// Q plane, the U plane `plane` floats on; Yc [ncp][2 ld] with ncp = 2 ldo
+// rows, the Q rows then the U rows; dosrc the Q plane (U rows follow).
+void launch_sh_to_do_qu(hipStream_t st, const float* S, size_t plane, const float* Yc, const float* ext,
+                        float* dosrc, int mp, int ncp, int ld, int ldp, int npts, const int* stop = nullptr);
 // The polarized Q and U rows of the same (NSTOKES = 3): S holds the Q plane
 // and, `plane` floats after it, the U plane ([mp][ld] each); Yc rows are
 // 2 ld long (the Q-plane terms then the U-plane terms), one row per output

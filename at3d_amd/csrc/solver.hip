@@ -708,10 +708,6 @@ int state_create(const Plan& p, State& s, std::string& err) {
   TRY(alloc0(s, s.ctl, 1, err));
   s.nr_buf0 = s.nr;
   TRY(setup_ring(p, s, err));
-  if (p.nst == 3 && !s.ring_ok) {
-    err = "polarized solves need the level-ordered sweep, whose rings this grid does not fit";
-    return AT3D_ERR_ARG;
-  }
   for (int e = 0; e < State::NEV; e++) HIPCHK(hipEventCreate(&s.ev[e]));
   for (int e = 0; e < 4; e++) HIPCHK(hipEventCreate(&s.ev_sweep[e]));
   for (int e = 0; e < 2; e++) HIPCHK(hipEventCreate(&s.ev_ray[e]));
@@ -1066,13 +1062,18 @@ static void accel_solcrit(bool doaccel, float deljdot, float deljold, float delj
     solcrit = 0.0f;
 }
 
-// The Q and U rows of SH->DO and the Q and U planes of DO->SH of a polarized
-// state (its sweeps run on the level-ordered layout); nothing for NSTOKES = 1.
+// The Q and U rows of SH->DO (in the layout the I rows have: each octant's
+// level order for the ring sweep, point order otherwise) and the Q and U
+// planes of DO->SH of a polarized state; nothing for NSTOKES = 1.
 static void enqueue_sh_to_do_qu(State& s, const int* stop) {
   if (s.nst != 3) return;
   const Plan& p = *s.plan;
-  launch_sh_to_do_perm_qu(s.stream, s.S + s.psh, s.psh, s.Yqu, s.ext, s.ns, s.oct_perm, s.sh_tiles_qu,
-                          2 * s.n_sh_tiles, s.dosrc + s.pdo, s.nptsp, s.ld, s.nptsp, p.npts, stop);
+  if (ring_active(s))
+    launch_sh_to_do_perm_qu(s.stream, s.S + s.psh, s.psh, s.Yqu, s.ext, s.ns, s.oct_perm, s.sh_tiles_qu,
+                            2 * s.n_sh_tiles, s.dosrc + s.pdo, s.nptsp, s.ld, s.nptsp, p.npts, stop);
+  else
+    launch_sh_to_do_qu(s.stream, s.S + s.psh, s.psh, s.Yqu, s.ext, s.dosrc + s.pdo, s.nptsp, 2 * s.ldo, s.ld,
+                       s.nptsp, p.npts, stop);
 }
 
 static void enqueue_do_to_sh_qu(State& s, const int* stop) {

@@ -40,7 +40,7 @@ struct SimpleStep {
 };
 
 __device__ inline void sweep_simple_static(const SweepArgs& a, const OrdGeom& og, int col, int pt,
-                                           SimpleStep& st) {
+                                           SimpleStep& st, int kst = 0) {
 #pragma clang fp contract(off)
   const float* xg = a.xgrid;
   const float* yg = a.ygrid;
@@ -81,12 +81,12 @@ __device__ inline void sweep_simple_static(const SweepArgs& a, const OrdGeom& og
     v = (xe - (double)xg[x0]) / ((double)xg[x1] - (double)xg[x0]);
   }
   const float* ext = a.ext;
-  const float* se = a.srcext + (size_t)col * a.ldp;
+  const float* se = a.srcext + kst * a.pdo + (size_t)col * a.ldp;
   const double f1 = (1 - u) * (1 - v), f2 = (1 - u) * v, f3 = u * (1 - v), f4 = u * v;
   const double ext0 = f1 * ext[i1] + f2 * ext[i2] + f3 * ext[i3] + f4 * ext[i4];
   const double srcext0 = f1 * se[i1] + f2 * se[i2] + f3 * se[i3] + f4 * se[i4];
   double src, absc, trc;
-  cell_integral(ext0, (double)ext[pt], srcext0, (double)se[pt], so, src, absc, trc);
+  cell_integral(ext0, (double)ext[pt], srcext0, (double)se[pt], so, src, absc, trc, kst == 0);
   st.i[0] = i1;
   st.i[1] = i2;
   st.i[2] = i3;
@@ -107,10 +107,10 @@ __device__ inline float sweep_simple_finish(const SimpleStep& st, const float* r
   return (float)(st.radv + st.transmit * rad0);
 }
 
-__device__ inline float sweep_simple(const SweepArgs& a, const OrdGeom& og, int col, int pt) {
+__device__ inline float sweep_simple(const SweepArgs& a, const OrdGeom& og, int col, int pt, int kst = 0) {
   SimpleStep st;
-  sweep_simple_static(a, og, col, pt, st);
-  return sweep_simple_finish(st, a.rad + (size_t)col * a.ldp);
+  sweep_simple_static(a, og, col, pt, st, kst);
+  return sweep_simple_finish(st, a.rad + kst * a.pdo + (size_t)col * a.ldp);
 }
 
 // General characteristic: trace through cells until a face with known radiances.
@@ -179,12 +179,13 @@ constexpr int SWEEP_K = 4;   // items per thread whose static part is prepared a
 __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_columns(SweepArgs a, int col0) {
   if (a.stop && *a.stop) return;
   const int col = col0 + blockIdx.x;
+  const int kst = blockIdx.y;   // Stokes component (Q, U: no clamp of the cell source)
   const int o = a.col_oct[col];
   const OrdGeom og = a.ordg[col];
   const int nlev = a.nlev[o];
   const int32_t* off = a.lvl_off + a.lo_base[o];
-  float* rad = a.rad + (size_t)col * a.ldp;
-  const float* se_row = a.srcext + (size_t)col * a.ldp;
+  float* rad = a.rad + kst * a.pdo + (size_t)col * a.ldp;
+  const float* se_row = a.srcext + kst * a.pdo + (size_t)col * a.ldp;
   const int tid = threadIdx.x;
   const bool general = a.force_general != 0;
   SimpleStep st[SWEEP_K];
@@ -195,7 +196,7 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_columns(SweepArgs a, in
     for (int k = 0; k < SWEEP_K; k++) {
       const int j = beg + tid + k * SWEEP_THREADS;
       code[k] = (j < end) ? a.lvl_pts[j] : INT_MIN;
-      if (code[k] >= 0 && !general) sweep_simple_static(a, og, col, code[k], st[k]);
+      if (code[k] >= 0 && !general) sweep_simple_static(a, og, col, code[k], st[k], kst);
     }
   };
   if (nlev >= 1) prepare(1);
@@ -211,16 +212,16 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_sweep_columns(SweepArgs a, in
       } else {
         const int j = beg + tid + k * SWEEP_THREADS;
         const int pt = c >= 0 ? c : ~c;
-        rad[pt] = sweep_general(a, og, col, o, pt, a.lvl_cell[j], L, se_row, nullptr);
+        rad[pt] = sweep_general(a, og, col, o, pt, a.lvl_cell[j], L, se_row, nullptr, kst);
       }
     }
     for (int j = beg + tid + SWEEP_K * SWEEP_THREADS; j < end; j += SWEEP_THREADS) {
       const int c = a.lvl_pts[j];
       if (c >= 0 && !general) {
-        rad[c] = sweep_simple(a, og, col, c);
+        rad[c] = sweep_simple(a, og, col, c, kst);
       } else {
         const int pt = c >= 0 ? c : ~c;
-        rad[pt] = sweep_general(a, og, col, o, pt, a.lvl_cell[j], L, se_row, nullptr);
+        rad[pt] = sweep_general(a, og, col, o, pt, a.lvl_cell[j], L, se_row, nullptr, kst);
       }
     }
     if (L < nlev) prepare(L + 1);
@@ -746,16 +747,19 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
 // Boundary radiances for one hemisphere: sky radiance at the top points for
 // downwelling ordinates, surface radiance at the bottom points for upwelling
 // ones (PATH_INTEGRATION shdomsub1.f:2042-2086).
+// blockIdx.y: the Stokes component (Q and U boundary radiances are zero).
 __global__ void k_sweep_preset(SweepArgs a, int up, int col0, int ncol) {
   if (a.stop && *a.stop) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int np = up ? a.nbot : a.ntop;
   if (i >= np * ncol) return;
   const int b = i / ncol, c = col0 + (i - b * ncol);
+  const int kst = blockIdx.y;
+  float* rad = a.rad + kst * a.pdo;
   if (up)
-    a.rad[(size_t)c * a.ldp + a.bcbot[b]] = a.bcrad_bot[b];
+    rad[(size_t)c * a.ldp + a.bcbot[b]] = kst ? 0.0f : a.bcrad_bot[b];
   else
-    a.rad[(size_t)c * a.ldp + a.bctop[b]] = a.skyrad[c];
+    rad[(size_t)c * a.ldp + a.bctop[b]] = kst ? 0.0f : a.skyrad[c];
 }
 
 // FIXED_LAMBERTIAN_BOUNDARY (shdomsub1.f:2438-2480) with the downwelling
@@ -816,12 +820,12 @@ __global__ void k_fluxes(const float* __restrict__ rad, int ldp, const int32_t* 
 void launch_sweep_preset(hipStream_t st, const SweepArgs& a, bool up, int col0, int ncol) {
   const int n = (up ? a.nbot : a.ntop) * ncol;
   if (n <= 0) return;
-  k_sweep_preset<<<(n + 255) / 256, 256, 0, st>>>(a, up ? 1 : 0, col0, ncol);
+  k_sweep_preset<<<dim3((n + 255) / 256, a.nst), 256, 0, st>>>(a, up ? 1 : 0, col0, ncol);
 }
 
 void launch_sweep_columns(hipStream_t st, const SweepArgs& a, int col0, int ncol) {
   if (ncol <= 0) return;
-  k_sweep_columns<<<ncol, SWEEP_THREADS, 0, st>>>(a, col0);
+  k_sweep_columns<<<dim3(ncol, a.nst), SWEEP_THREADS, 0, st>>>(a, col0);
 }
 
 void launch_face_mask(hipStream_t st, const SweepArgs& a) {

@@ -101,10 +101,17 @@ __global__ __launch_bounds__(256) void k_sh_to_do(const float* __restrict__ S,
         const int c = n0 + wn * 32 + j * 16 + (lane & 15);
         if (c >= nang) continue;
         float4 o;
-        o.x = fmaxf(acc[i][j][0], 0.0f) * e.x;
-        o.y = fmaxf(acc[i][j][1], 0.0f) * e.y;
-        o.z = fmaxf(acc[i][j][2], 0.0f) * e.z;
-        o.w = fmaxf(acc[i][j][3], 0.0f) * e.w;
+        if (QU) {   // Q and U are not clamped
+          o.x = acc[i][j][0] * e.x;
+          o.y = acc[i][j][1] * e.y;
+          o.z = acc[i][j][2] * e.z;
+          o.w = acc[i][j][3] * e.w;
+        } else {
+          o.x = fmaxf(acc[i][j][0], 0.0f) * e.x;
+          o.y = fmaxf(acc[i][j][1], 0.0f) * e.y;
+          o.z = fmaxf(acc[i][j][2], 0.0f) * e.z;
+          o.w = fmaxf(acc[i][j][3], 0.0f) * e.w;
+        }
         *reinterpret_cast<float4*>(out + (size_t)c * ldo + p0) = o;
       }
     }
@@ -318,6 +325,12 @@ void launch_sh_to_do(hipStream_t st, const float* S, const float* Yc, const floa
                      const int* stop) {
   dim3 grid(ncp / TN, mp / TM);
   k_sh_to_do<true><<<grid, 256, 0, st>>>(S, Yc, ext, dosrc, ld, ldp, npts, nang, stop);
+}
+
+void launch_sh_to_do_qu(hipStream_t st, const float* S, size_t plane, const float* Yc, const float* ext,
+                        float* dosrc, int mp, int ncp, int ld, int ldp, int npts, const int* stop) {
+  dim3 grid(ncp / TN, mp / TM);
+  k_sh_to_do<true, true><<<grid, 256, 0, st>>>(S, Yc, ext, dosrc, ld, ldp, npts, ncp, stop, plane);
 }
 
 void launch_sh_dot(hipStream_t st, const float* S, const float* Yd, float* raw, int mp, int ndp,

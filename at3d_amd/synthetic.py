@@ -173,6 +173,8 @@ def workload(name: str) -> Workload:
         w = lognormal_blob(n=16, nleg=7)
     elif name == "cloud64":
         w = cloud_field(z0=0.35)
+    elif name == "polarized128":
+        w = polarized_cloud_field()
     else:
         raise KeyError(name)
     return w
@@ -327,6 +329,8 @@ BLOB16_VIEWS = ((0.0, 0.0), (45.0, 0.0), (45.0, 180.0))
 def workload_views(name: str, w: Workload):
     if name == "cloud64":
         return view_rays(w, CLOUD64_VIEWS, 64, 64)
+    if name == "polarized128":
+        return view_rays(w, CLOUD64_VIEWS, 128, 128)
     if name.startswith("blob16"):
         return view_rays(w, BLOB16_VIEWS, 32, 32)
     raise KeyError(name)

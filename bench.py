@@ -219,6 +219,75 @@ def split_leg(cpu_runs):
     return out
 
 
+POL_WAVELENGTHS = (0.47, 0.66, 0.86)
+
+
+def polarized_leg(cpu_runs):
+    """The polarized workload: a 128 x 128 x 64 open cloud (dx = dy = dz =
+    0.02 km; six-element cloud phase matrix plus Rayleigh with depolarization
+    0.03), NSTOKES 3, NMU 16 NPHI 32 (L = 15), solacc 1e-4, at 0.47, 0.66 and
+    0.86 um.  The three wavelengths are solved together (one host thread and
+    HIP stream each), timed whole from the plan to convergence; then 9 views
+    of 128 x 128 pixels are rendered for I, Q, U at each wavelength.  The
+    reference's polarized solve is timed on one host core on a bounded
+    sample: the 20 x 20 x 64 corner column block of the 0.66 um medium (same
+    angles and layers), to its own convergence."""
+    from concurrent.futures import ThreadPoolExecutor
+    from at3d_amd import synthetic as S
+    import torch
+    kw = dict(max_total_mb=60000.0, adapt_grid_factor=1.0)   # a fixed grid: no room for splitting
+    ws = [S.polarized_cloud_field(wavelength=wl) for wl in POL_WAVELENGTHS]
+
+    def solve(w):
+        r = S.make_rte(w, **kw)
+        r.solve(w.maxiter, verbose=False)
+        return r
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(len(ws)) as ex:
+        rtes = list(ex.map(solve, ws))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    rays = S.workload_views("polarized128", ws[0])
+    t0 = time.perf_counter()
+    outs = [r.integrate_to_sensor(dict(rays)) for r in rtes]
+    torch.cuda.synchronize()
+    render_s = time.perf_counter() - t0
+    out = {"workload": "128x128x64 open cloud, NSTOKES 3, NMU16 NPHI32 L15, solacc 1e-4, "
+                       "3 wavelengths (0.47 0.66 0.86 um), 9 views of 128x128 I/Q/U",
+           "npts": rtes[0].plan_size("npts"), "sweep": "level-ordered" if rtes[0].timing("sweep_ring") else
+           "point-ordered (the ring needs {:.0f} KB of LDS)".format(rtes[0].timing("sweep_ring_lds") / 1024),
+           "iterations": [r.num_iterations for r in rtes], "solve_wall_s": wall,
+           "solves_per_s": len(ws) / wall, "render_wall_s": render_s,
+           "rays": int(rays["ray_x"].size) * len(ws),
+           "device_GB_per_state": rtes[0].timing("device_bytes") / 1e9,
+           "kernel_ms_per_solve": {k: float(np.mean([r.timing(k) for r in rtes]))
+                                   for k in ("sh_to_do", "sweep", "do_to_sh", "source", "total")},
+           "mean_QoverI": float(np.mean([np.abs(o["Q"]).mean() / o["I"].mean() for o in outs]))}
+    for r in rtes:
+        r.release()
+    if cpu_runs > 0:
+        try:
+            from oracle import ref_shdom as ref
+        except Exception:
+            return out
+        if ref.available():
+            w = ws[1]
+            sub = [(e[:20, :20].copy(), a[:20, :20].copy(), t) for (e, a, t) in w.species]
+            ws_ = S.Workload("polarized_sample", w.delx, w.dely, w.z, sub, bc="open", nmu=w.nmu, nphi=w.nphi,
+                             solacc=w.solacc, wavelength=w.wavelength, nstokes=3)
+            t0 = time.perf_counter()
+            st = ref.solve(ref.scene_from_workload(ws_), maxiter=w.maxiter)
+            cpu_s = time.perf_counter() - t0
+            per_pt = cpu_s / st.npts
+            out["cpu_reference"] = {"sample": "20x20x64 corner block of the 0.66 um medium, to convergence",
+                                    "sample_points": st.npts, "sample_s": cpu_s,
+                                    "sample_iterations": st.iters, "cores": 1,
+                                    "est_s_per_solve_full_grid": per_pt * out["npts"],
+                                    "speedup_vs_cpu_est": per_pt * out["npts"] / (wall / len(ws))}
+    return out
+
+
 def cold_solves(n, maxiter):
     """`n` solves of new media on the headline geometry (the cloud field
     re-drawn with seeds 1..n): host plan of the new medium (the sweep levels
@@ -311,6 +380,7 @@ def main():
     ap.add_argument("--cold-steps", type=int, default=3, help="solves of new media timed whole")
     ap.add_argument("--render-reps", type=int, default=3)
     ap.add_argument("--gradient-reps", type=int, default=3)
+    ap.add_argument("--polarized", type=int, default=1, help="run the polarized (NSTOKES 3) leg")
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--stream-requests", type=int, default=48,
                     help="requests of the mixed-size serving stream (0: skip it)")
@@ -497,6 +567,8 @@ def main():
 
     split = split_leg(args.cpu_runs)
     print("bench: adaptive splitting done", file=sys.stderr, flush=True)
+    polarized = polarized_leg(args.cpu_runs) if args.polarized else None
+    print("bench: polarized done", file=sys.stderr, flush=True)
 
     stream = None
     if args.stream_requests > 0:
@@ -541,6 +613,7 @@ def main():
         "render": render,
         "gradient": grad,
         "adaptive_splitting": split,
+        "polarized": polarized,
         "serving_stream": stream,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": {"cold_solve": (1e3 / cold_ms / cpu["value"]) if (cpu and cold_ms) else None,

@@ -127,3 +127,27 @@ def test_a1_only_polarized_equals_scalar():
     o1 = r1.integrate_to_sensor(dict(rays))
     assert rel_l2(o3["I"], o1["I"]) < 1e-5
     assert np.all(o3["Q"] == 0.0) and np.all(o3["U"] == 0.0)
+
+
+@pytest.mark.parametrize("bc", ["open", "periodic"])
+def test_polarized_point_ordered_sweep(ref, bc):
+    """Grids whose rings do not fit in LDS sweep point by point: the same I,
+    Q, U (the one-cell path bit for bit against the general trace) and the
+    reference's converged fields."""
+    w = pol_blob(bc=bc)
+    a = S.make_rte(w)
+    a.set_option("sweep_ring", 0)
+    a.solve(3, verbose=False)
+    b = S.make_rte(w)
+    b.set_option("sweep_general", 1)
+    b.solve(3, verbose=False)
+    assert a.timing("sweep_ring") == 0.0
+    assert np.array_equal(a.state_array("source"), b.state_array("source"))
+    c = S.make_rte(w)
+    c.set_option("sweep_ring", 0)
+    c.solve(100, verbose=False)
+    st = _ref_solve(ref, w, 100)
+    assert abs(c.num_iterations - st.iters) <= 1
+    got, want = c.state_array("source"), _ref_fields(ref, st, "source")
+    for k in range(3):
+        assert rel_l2(got[k], want[k]) < 1e-4, k
