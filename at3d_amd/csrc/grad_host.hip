@@ -131,6 +131,10 @@ int state_levis_gradient(State& s, const at3d_deriv_desc& dd, int64_t nrays, con
     err = "the Levis gradient is for NSTOKES = 1 solves";
     return AT3D_ERR_ARG;
   }
+  if (p.nmicro != 1) {
+    err = "the Levis gradient is for one phase table per property point (num_micro = 1)";
+    return AT3D_ERR_ARG;
+  }
   if (!s.initialized) {
     err = "the solution is not initialised: solve before computing a gradient";
     return AT3D_ERR_ARG;

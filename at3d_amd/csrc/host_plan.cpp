@@ -370,6 +370,7 @@ void ylm_unpol(float mu, float phi, int ml, int mm, float* yr) {
 }
 
 void make_sh_tables_pol(Plan& p);
+void prepare_legen_pol(Plan& p);
 
 // Dense SH<->DO transform matrices.  SH_TO_DO_UNPOL (shdomsub1.f:2928-3036)
 // sums P_lm(mu) over l per m (CMU1 = PLMALL, MAKE_SH_DO_COEF shdomsub2.f:1149-1220)
@@ -490,11 +491,17 @@ void make_sh_tables_pol(Plan& p) {
 PropGrid plan_prop_grid(const Plan& p) {
   const at3d_scene_desc& d = p.d;
   return PropGrid{p.extinctp.data(), p.albedop.data(), p.iphasep.data(), p.phasewtp.data(), p.zlevels.data(),
-                  d.npx, d.npy, d.npz, d.delx, d.dely, p.extmin, p.scatmin};
+                  d.npx, d.npy, d.npz, d.delx, d.dely, p.extmin, p.scatmin, p.nmicro};
 }
 
 LegTable plan_leg_table(const Plan& p) {
-  return LegTable{p.legen.data(), p.nleg + 1, p.ml, p.legen.size()};
+  LegTable t{p.legen.data(), p.nleg + 1, p.ml, p.legen.size(), p.npw};
+  if (!p.legen6flat.empty()) {
+    t.flat = p.legen6flat.data();
+    t.nstleg = 6;
+    t.flat_size = p.legen6flat.size();
+  }
+  return t;
 }
 
 float sky_flux_weight(const Plan& p) {
@@ -540,8 +547,9 @@ void interp_optics(Plan& p) {
   medium_scalars(p);
   p.extinct.assign((size_t)npart * npts, 0.f);
   p.albedo.assign((size_t)npart * npts, 0.f);
-  p.iphase.assign((size_t)npart * npts * 8, 1);
-  p.phaseinterpwt.assign((size_t)npart * npts * 8, 0.f);
+  const int npw = p.npw;
+  p.iphase.assign((size_t)npart * npts * npw, 1);
+  p.phaseinterpwt.assign((size_t)npart * npts * npw, 0.f);
   p.total_ext.assign(npts, 0.f);
   for (int ipa = 0; ipa < npart; ipa++) {
     // points are independent; the species loop stays outside so each point's
@@ -550,7 +558,7 @@ void interp_optics(Plan& p) {
     parallel_for(nchunk, [&](int c) {
     const int ip0 = (int)((int64_t)npts * c / nchunk), ip1 = (int)((int64_t)npts * (c + 1) / nchunk);
     for (int ip = ip0; ip < ip1; ip++) {
-      const size_t base = ((size_t)ipa * npts + ip) * 8;
+      const size_t base = ((size_t)ipa * npts + ip) * npw;
       float extf, albf;
       point_optics(p, ipa, p.gridpos[ip * 3], p.gridpos[ip * 3 + 1], p.gridpos[ip * 3 + 2], extf, albf,
                    &p.iphase[base], &p.phaseinterpwt[base]);
@@ -561,6 +569,7 @@ void interp_optics(Plan& p) {
     });
   }
   prepare_legen(p);
+  if (p.nst == 3) prepare_legen_pol(p);
   const LegTable lt = plan_leg_table(p);
   if (p.d.deltam) {
     for (int ip = 0; ip < npts; ip++) {
@@ -571,7 +580,7 @@ void interp_optics(Plan& p) {
     }
     for (int ipa = 0; ipa < npart; ipa++)
       for (int ip = 0; ip < npts; ip++) {
-        const size_t base = ((size_t)ipa * npts + ip) * 8;
+        const size_t base = ((size_t)ipa * npts + ip) * npw;
         const float f = prepare_f(lt, &p.iphase[base], &p.phaseinterpwt[base]);
         float& ext = p.extinct[(size_t)ipa * npts + ip];
         deltam_scale(f, ext, p.albedo[(size_t)ipa * npts + ip]);
@@ -586,13 +595,13 @@ void interp_optics(Plan& p) {
 // COMPUTE_SOURCE, shdomsub1.f:1088-1112): returns coefficients 0..nleg.
 void mixed_legen(const Plan& p, int ipa, int ip, std::vector<float>& out) {
   const int nl = p.nleg + 1;
-  const size_t base = ((size_t)ipa * p.npts + ip) * 8;
+  const size_t base = ((size_t)ipa * p.npts + ip) * p.npw;
   out.assign(nl, 0.f);
   if (p.phaseinterpwt[base] >= 0.999f) {
     const float* L = &p.legen[(size_t)(p.iphase[base] - 1) * nl];
     for (int l = 0; l < nl; l++) out[l] = L[l];
   } else {
-    for (int q = 0; q < 8; q++) {
+    for (int q = 0; q < p.npw; q++) {
       const float w = p.phaseinterpwt[base + q];
       if (w <= 1e-5f) continue;
       const float* L = &p.legen[(size_t)(p.iphase[base + q] - 1) * nl];
@@ -617,32 +626,42 @@ void source_constants(Plan& p) {
     const int i0 = (int)((int64_t)npts * c / nchunk), i1 = (int)((int64_t)npts * (c + 1) / nchunk);
     for (int i = i0; i < i1; i++)
       point_source_constants(lt, npart, p.d.deltam != 0, p.total_ext[i], &p.extinct[i], &p.albedo[i],
-                             &p.iphase[(size_t)i * 8], &p.phaseinterpwt[(size_t)i * 8], (size_t)npts,
+                             &p.iphase[(size_t)i * p.npw], &p.phaseinterpwt[(size_t)i * p.npw], (size_t)npts,
                              &p.chi[(size_t)i * (ML + 1)], p.alb[i], &p.tc[(size_t)i * (ML + 1)]);
   });
 }
 
-// The prepared tables of phase-matrix elements a2, a3 and b1 (NSTLEG = 6):
-// divided by (2l+1) (TRILIN_INTERP_PROP's set-up); with delta-M, element 1's
-// f subtracted from l <= ML of a2 and a3, b1 left alone (PREPARE_PROP,
-// shdomsub2.f:526-543, 'N' interpolation).
+// The prepared tables of the phase-matrix elements (NSTLEG = 6): divided by
+// (2l+1) (TRILIN_INTERP_PROP's set-up); with delta-M, element 1's f
+// subtracted from l <= ML of a2, a3 and a4, b1 and b2 left alone
+// (PREPARE_PROP, shdomsub2.f:526-543, 'N' interpolation).  a2, a3 and b1 as
+// tables of their own, all six in the reference's interleaved order.
 void prepare_legen_pol(Plan& p) {
   const int nl = p.nleg + 1, ML = p.ml, np = p.numphase;
-  const int el[3] = {1, 2, 4};
-  std::vector<float>* out[3] = {&p.legen2, &p.legen3, &p.legen5};
-  for (int k = 0; k < 3; k++) {
-    std::vector<float>& t = *out[k];
+  std::vector<float> el[6];
+  for (int e = 0; e < 6; e++) {
+    std::vector<float>& t = el[e];
+    if (e == 0) {
+      t = p.legen;
+      continue;
+    }
     t.assign((size_t)np * nl, 0.f);
     for (int iph = 0; iph < np; iph++)
       for (int l = 0; l < nl; l++)
-        t[(size_t)iph * nl + l] =
-            p.legenp6[((size_t)el[k] * np + iph) * (p.nlegp + 1) + l] / (float)(2 * l + 1);
-    if (p.d.deltam && k < 2)
+        t[(size_t)iph * nl + l] = p.legenp6[((size_t)e * np + iph) * (p.nlegp + 1) + l] / (float)(2 * l + 1);
+    if (p.d.deltam && e < 4)
       for (int iph = 0; iph < np; iph++) {
         const float f = p.legen[(size_t)iph * nl + ML + 1];
         for (int l = 0; l <= ML; l++) t[(size_t)iph * nl + l] = t[(size_t)iph * nl + l] - f;
       }
   }
+  p.legen2 = el[1];
+  p.legen3 = el[2];
+  p.legen5 = el[4];
+  p.legen6flat.assign((size_t)np * nl * 6, 0.f);
+  for (int iph = 0; iph < np; iph++)
+    for (int l = 0; l < nl; l++)
+      for (int e = 0; e < 6; e++) p.legen6flat[((size_t)iph * nl + l) * 6 + e] = el[e][(size_t)iph * nl + l];
 }
 
 // Per point, the a2, a3 and b1 terms of the polarized source
@@ -650,13 +669,12 @@ void prepare_legen_pol(Plan& p) {
 // and the b1 single-scatter terms of rendering, per unit direct flux.
 void polarized_constants(Plan& p) {
   const int ML = p.ml, npts = p.npts, npart = p.npart, nl = p.nleg + 1;
-  prepare_legen_pol(p);
   p.chip.assign((size_t)npts * 3 * (ML + 1), 0.f);
   p.tms5.assign((size_t)npts * (ML + 1), 0.f);
   const LegTable t1 = plan_leg_table(p);
-  const LegTable te[3] = {LegTable{p.legen2.data(), nl, ML, p.legen2.size()},
-                          LegTable{p.legen3.data(), nl, ML, p.legen3.size()},
-                          LegTable{p.legen5.data(), nl, ML, p.legen5.size()}};
+  const LegTable te[3] = {LegTable{p.legen2.data(), nl, ML, p.legen2.size(), p.npw},
+                          LegTable{p.legen3.data(), nl, ML, p.legen3.size(), p.npw},
+                          LegTable{p.legen5.data(), nl, ML, p.legen5.size(), p.npw}};
   const float secmu0 = (float)(1.0 / fabs((double)p.d.solarmu));
   const int nchunk = 64;
   parallel_for(nchunk, [&](int c) {
@@ -664,11 +682,11 @@ void polarized_constants(Plan& p) {
     for (int i = i0; i < i1; i++) {
       for (int k = 0; k < 3; k++)
         point_element_constants(t1, te[k], npart, p.d.deltam != 0, &p.extinct[i], &p.albedo[i],
-                                &p.iphase[(size_t)i * 8], &p.phaseinterpwt[(size_t)i * 8], (size_t)npts,
+                                &p.iphase[(size_t)i * p.npw], &p.phaseinterpwt[(size_t)i * p.npw], (size_t)npts,
                                 &p.chip[((size_t)i * 3 + k) * (ML + 1)]);
       if (p.d.deltam)
         point_element_tms(t1, te[2], npart, p.total_ext[i], &p.extinct[i], &p.albedo[i],
-                          &p.iphase[(size_t)i * 8], &p.phaseinterpwt[(size_t)i * 8], (size_t)npts, secmu0,
+                          &p.iphase[(size_t)i * p.npw], &p.phaseinterpwt[(size_t)i * p.npw], (size_t)npts, secmu0,
                           &p.tms5[(size_t)i * (ML + 1)]);
     }
   });
@@ -691,11 +709,16 @@ void direct_beam_setup(const Plan& p, std::vector<float>& extdirp, DirectBeamGeo
       double ext = p.extinctp[(size_t)ipa * npg + ip];
       double alb = p.albedop[(size_t)ipa * npg + ip];
       if (d.deltam) {
+        // f over the point's phase tables, summed in double
         const int L = ML + 1;
-        int iph = p.iphasep[(size_t)ipa * npg + ip];
-        float term = p.phasewtp[(size_t)ipa * npg + ip] * p.legenp[(size_t)(iph - 1) * (p.nlegp + 1) + L];
-        term = term / (float)(2 * L + 1);
-        double F = 0.0 + (double)term;
+        double F = 0.0;
+        for (int q = 0; q < p.nmicro; q++) {
+          const size_t pq = ((size_t)ipa * npg + ip) * p.nmicro + q;
+          const int iph = p.iphasep[pq];
+          float term = p.phasewtp[pq] * p.legenp[(size_t)(iph - 1) * (p.nlegp + 1) + L];
+          term = term / (float)(2 * L + 1);
+          F = F + (double)term;
+        }
         ext = (1.0 - alb * F) * ext;
       }
       acc = (float)((double)acc + ext);
@@ -1232,7 +1255,7 @@ int render_point_constants(Plan& p) {
     const int i0 = (int)((int64_t)npts * c / nchunk), i1 = (int)((int64_t)npts * (c + 1) / nchunk);
     for (int ip = i0; ip < i1; ip++) {
       const int n = point_render_constants(lt, npart, p.total_ext[ip], &p.extinct[ip], &p.albedo[ip],
-                                           &p.iphase[(size_t)ip * 8], &p.phaseinterpwt[(size_t)ip * 8],
+                                           &p.iphase[(size_t)ip * p.npw], &p.phaseinterpwt[(size_t)ip * p.npw],
                                            (size_t)npts, secmu0, &p.tms[(size_t)ip * (ML + 1)], ib.data(),
                                            cb.data());
       c_iph[c].insert(c_iph[c].end(), ib.begin(), ib.begin() + n);
@@ -1405,6 +1428,17 @@ int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err) {
     return AT3D_ERR_ARG;
   }
   const int npg = d.npx * d.npy * d.npz;
+  // phase tables per property point (num_micro, MAXNMICRO; 0 is taken as 1)
+  p.nmicro = d.nmicro == 0 ? 1 : d.nmicro;
+  if (p.nmicro < 1 || p.nmicro > MAX_NMICRO) {
+    err = "nmicro must be 1 .. " + std::to_string(MAX_NMICRO);
+    return AT3D_ERR_ARG;
+  }
+  if (p.nmicro > 1 && d.splitacc > 0.0f) {
+    err = "more than one phase table per property point runs on the base grid: split_accuracy must be 0";
+    return AT3D_ERR_ARG;
+  }
+  p.npw = 8 * p.nmicro;
   p.zlevels.assign(d.zlevels, d.zlevels + d.npz);
   for (int i = 1; i < d.npz; i++)
     if (!(p.zlevels[i] > p.zlevels[i - 1])) {
@@ -1414,8 +1448,8 @@ int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err) {
   p.npart = d.npart;
   p.extinctp.assign(d.extinctp, d.extinctp + (size_t)npg * d.npart);
   p.albedop.assign(d.albedop, d.albedop + (size_t)npg * d.npart);
-  p.iphasep.assign(d.iphasep, d.iphasep + (size_t)npg * d.npart);
-  p.phasewtp.assign(d.phasewtp, d.phasewtp + (size_t)npg * d.npart);
+  p.iphasep.assign(d.iphasep, d.iphasep + (size_t)npg * d.npart * p.nmicro);
+  p.phasewtp.assign(d.phasewtp, d.phasewtp + (size_t)npg * d.npart * p.nmicro);
   for (int v : p.iphasep)
     if (v < 1 || v > d.numphase) {
       err = "Phase function indices are out of bounds.";

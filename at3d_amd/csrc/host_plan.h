@@ -89,8 +89,8 @@ struct Plan {
   std::vector<float> ymat, wmat, ylmsun;   // [nang][nlm], [nang][nlm], [nlm]
   // optics on the RTE grid
   std::vector<float> extinct, albedo, total_ext;  // [npart][npts], [npts]
-  std::vector<int32_t> iphase;                    // [npart][npts][8]
-  std::vector<float> phaseinterpwt;               // [npart][npts][8]
+  std::vector<int32_t> iphase;                    // [npart][npts][npw]
+  std::vector<float> phaseinterpwt;               // [npart][npts][npw]
   std::vector<float> legen;                       // [numphase][nleg+1] (post delta-M prep)
   std::vector<float> chi, alb, tc;                // [npts][ml+1], [npts], [npts][ml+1]
   float albmax = 0.f;
@@ -108,6 +108,13 @@ struct Plan {
   // spherical harmonic tables of the ordinates for the transforms [6][nang][nlm]
   int nst = 1;
   std::vector<float> legenp6, legen2, legen3, legen5, chip, tms5, phasetab2, ypol, wpol;
+  // the prepared six-element table in the reference's LEGEN(6, 0:NLEG,
+  // NUMPHASE) order ([numphase][nleg+1][6]), for RADIANCE_TRUNCATION's
+  // flat indexing
+  std::vector<float> legen6flat;
+  // phase tables per property point (num_micro, MAXNMICRO) and phase pairs
+  // per grid point and species (8 x nmicro)
+  int nmicro = 1, npw = 8;
   // rendering constants
   std::vector<float> phasetab;                    // [numphase][nscatangle]
   std::string phasetab_err;                       // negativity failure, raised by render

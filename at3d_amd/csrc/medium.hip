@@ -24,7 +24,7 @@ __global__ __launch_bounds__(256) void k_medium_optics(MediumArgs m) {
   float total = 0.f;
   for (int ipa = 0; ipa < m.npart; ipa++) {
     float e, a;
-    prop_point(m.g, ipa, X, Y, Z, e, a, m.iphase + (ipa * np + ip) * 8, m.wt + (ipa * np + ip) * 8);
+    prop_point(m.g, ipa, X, Y, Z, e, a, m.iphase + (ipa * np + ip) * m.t.npw, m.wt + (ipa * np + ip) * m.t.npw);
     m.extinct[ipa * np + ip] = e;
     m.albedo[ipa * np + ip] = a;
     total = total + e;
@@ -35,7 +35,7 @@ __global__ __launch_bounds__(256) void k_medium_optics(MediumArgs m) {
     const float t = total - s;
     total = t < 0.f ? 0.f : t;
     for (int ipa = 0; ipa < m.npart; ipa++) {
-      const float f = prepare_f(m.t, m.iphase + (ipa * np + ip) * 8, m.wt + (ipa * np + ip) * 8);
+      const float f = prepare_f(m.t, m.iphase + (ipa * np + ip) * m.t.npw, m.wt + (ipa * np + ip) * m.t.npw);
       float e = m.extinct[ipa * np + ip], a = m.albedo[ipa * np + ip];
       deltam_scale(f, e, a);
       m.extinct[ipa * np + ip] = e;
@@ -60,7 +60,8 @@ __global__ __launch_bounds__(256) void k_medium_constants(MediumArgs m) {
   float* chi = m.chi + (size_t)ip * ml1;
   float alb;
   point_source_constants(m.t, m.npart, m.deltam, m.total_ext[ip], m.extinct + ip, m.albedo + ip,
-                         m.iphase + (size_t)ip * 8, m.wt + (size_t)ip * 8, np, chi, alb, m.tc + (size_t)ip * ml1);
+                         m.iphase + (size_t)ip * m.t.npw, m.wt + (size_t)ip * m.t.npw, np, chi, alb,
+                         m.tc + (size_t)ip * ml1);
   m.alb[ip] = alb;
   int c = 0;
   for (int l = m.t.ml; l > 0; l--)
@@ -72,7 +73,7 @@ __global__ __launch_bounds__(256) void k_medium_constants(MediumArgs m) {
   int n = 0;
   if (m.deltam)
     n = point_render_constants(m.t, m.npart, m.total_ext[ip], m.extinct + ip, m.albedo + ip,
-                               m.iphase + (size_t)ip * 8, m.wt + (size_t)ip * 8, np, m.secmu0,
+                               m.iphase + (size_t)ip * m.t.npw, m.wt + (size_t)ip * m.t.npw, np, m.secmu0,
                                m.tms + (size_t)ip * ml1, nullptr, nullptr);
   m.ss_cnt[ip] = n;
 }
@@ -82,8 +83,9 @@ __global__ __launch_bounds__(256) void k_medium_pairs(MediumArgs m) {
   const int ip = blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= m.npts || !m.deltam) return;
   const int o = m.ss_off[ip];
-  point_render_constants(m.t, m.npart, m.total_ext[ip], m.extinct + ip, m.albedo + ip, m.iphase + (size_t)ip * 8,
-                         m.wt + (size_t)ip * 8, m.npts, m.secmu0, nullptr, m.ss_iph + o, m.ss_coef + o);
+  point_render_constants(m.t, m.npart, m.total_ext[ip], m.extinct + ip, m.albedo + ip,
+                         m.iphase + (size_t)ip * m.t.npw, m.wt + (size_t)ip * m.t.npw, m.npts, m.secmu0, nullptr,
+                         m.ss_iph + o, m.ss_coef + o);
 }
 
 // Exclusive prefix sum of the pair counts (npts + 1 offsets): sums of
@@ -173,10 +175,14 @@ __global__ __launch_bounds__(256) void k_medium_beam_ext(MediumArgs m) {
     const double alb = m.g.albedop[(size_t)ipa * npg + ip];
     if (m.deltam) {
       const int L = m.t.ml + 1;
-      const int iph = m.g.iphasep[(size_t)ipa * npg + ip];
-      float term = m.g.phasewtp[(size_t)ipa * npg + ip] * m.legenp[(size_t)(iph - 1) * (m.nlegp + 1) + L];
-      term = term / (float)(2 * L + 1);
-      const double F = 0.0 + (double)term;
+      double F = 0.0;
+      for (int q = 0; q < m.g.nmicro; q++) {
+        const size_t pq = ((size_t)ipa * npg + ip) * m.g.nmicro + q;
+        const int iph = m.g.iphasep[pq];
+        float term = m.g.phasewtp[pq] * m.legenp[(size_t)(iph - 1) * (m.nlegp + 1) + L];
+        term = term / (float)(2 * L + 1);
+        F = F + (double)term;
+      }
       ext = (1.0 - alb * F) * ext;
     }
     acc = (float)((double)acc + ext);

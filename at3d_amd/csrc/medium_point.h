@@ -7,7 +7,8 @@
 // and the device's math library).
 //
 // Species-major arrays: value of species ipa at point ip is at [ipa * stride
-// + ip], its eight phase pointers / weights at [(ipa * stride + ip) * 8 + q];
+// + ip], its NPW = 8 x NMICRO phase pointers / weights (eight corners times the
+// micro-physical tables of a property point) at [(ipa * stride + ip) * NPW + q];
 // phase pointers are 1-based into the prepared Legendre table.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -16,30 +17,138 @@
 
 namespace at3d {
 
+// Phase tables a property point mixes at most (MAXNMICRO).
+constexpr int MAX_NMICRO = 8;
+
 // The property grid (TRILIN_INTERP_PROP's inputs).
 struct PropGrid {
   const float* extinctp;   // [npart][npg]
   const float* albedop;    // [npart][npg]
-  const int32_t* iphasep;  // [npart][npg] 1-based
-  const float* phasewtp;   // [npart][npg]
+  const int32_t* iphasep;  // [npart][npg][nmicro] 1-based
+  const float* phasewtp;   // [npart][npg][nmicro]
   const float* zlevels;    // [npz]
   int npx, npy, npz;
   float delx, dely;
   double extmin, scatmin;
+  int nmicro = 1;
 };
 
 // The prepared Legendre table: coefficients / (2l+1), delta-M f subtracted
-// from l <= ML (PREPARE_PROP), nl = NLEG + 1 per phase function.
+// from l <= ML (PREPARE_PROP), nl = NLEG + 1 per phase function; npw phase
+// pairs per point and species.  `flat` is the whole prepared table as the
+// reference lays it out, LEGEN(NSTLEG, 0:NLEG, NUMPHASE) (null: `legen`,
+// NSTLEG = 1), read by RADIANCE_TRUNCATION's flat indexing.
 struct LegTable {
   const float* legen;   // [numphase][nl]
   int nl, ml;
   size_t table_size;    // numphase * nl
+  int npw = 8;
+  const float* flat = nullptr;
+  int nstleg = 1;
+  size_t flat_size = 0;
 };
+
+// SSORT (shdomsub2.f:4961-5217) with KFLAG = -2: x decreasing, y carried,
+// by Singleton's quicksort (R. C. Singleton, CACM Algorithm 347, 1969), the
+// reference's method, so that equal weights keep the reference's order.
+template <typename T>
+__host__ __device__ inline void ssort_decreasing(float* x, T* y, int n) {
+  if (n < 1) return;
+  float* X = x - 1;   // 1-based views; the decreasing sort is an increasing sort of -x
+  T* Y = y - 1;
+  for (int i = 1; i <= n; i++) X[i] = -X[i];
+  int il[32], iu[32];
+  int m = 1, i = 1, j = n, k = 0, l = 0, ij;
+  float r = 0.375f, t;
+  T ty;
+  int step = 0;          // 0 select, 1 split, 2 pop, 3 short segment
+  bool adjust = true;    // enter the selection through the r update
+  for (;;) {
+    if (step == 0) {
+      if (adjust) {
+        if (i == j) {
+          step = 2;
+          continue;
+        }
+        r = r <= 0.5898437f ? r + 3.90625e-2f : r - 0.21875f;
+      }
+      adjust = true;
+      k = i;
+      ij = i + (int)((float)(j - i) * r);
+      t = X[ij];
+      ty = Y[ij];
+      if (X[i] > t) {
+        X[ij] = X[i]; X[i] = t; t = X[ij];
+        Y[ij] = Y[i]; Y[i] = ty; ty = Y[ij];
+      }
+      l = j;
+      if (X[j] < t) {
+        X[ij] = X[j]; X[j] = t; t = X[ij];
+        Y[ij] = Y[j]; Y[j] = ty; ty = Y[ij];
+        if (X[i] > t) {
+          X[ij] = X[i]; X[i] = t; t = X[ij];
+          Y[ij] = Y[i]; Y[i] = ty; ty = Y[ij];
+        }
+      }
+      for (;;) {
+        do l--; while (X[l] > t);
+        do k++; while (X[k] < t);
+        if (k > l) break;
+        const float tx = X[l]; X[l] = X[k]; X[k] = tx;
+        const T tt = Y[l]; Y[l] = Y[k]; Y[k] = tt;
+      }
+      if (l - i > j - k) {
+        il[m] = i; iu[m] = l; i = k;
+      } else {
+        il[m] = k; iu[m] = j; j = l;
+      }
+      m++;
+      step = 1;
+    } else if (step == 2) {
+      m--;
+      if (m == 0) break;
+      i = il[m];
+      j = iu[m];
+      step = 1;
+    } else if (step == 1) {
+      if (j - i >= 1) {
+        step = 0;
+        adjust = false;   // partition again without the r update
+        continue;
+      }
+      if (i == 1) {
+        step = 0;
+        continue;
+      }
+      i--;
+      step = 3;
+    } else {   // straight insertion over the short segment
+      for (;;) {
+        i++;
+        if (i == j) break;
+        t = X[i + 1];
+        ty = Y[i + 1];
+        if (X[i] <= t) continue;
+        k = i;
+        do {
+          X[k + 1] = X[k];
+          Y[k + 1] = Y[k];
+          k--;
+        } while (t < X[k]);
+        X[k + 1] = t;
+        Y[k + 1] = ty;
+      }
+      step = 2;
+    }
+  }
+  for (int q = 1; q <= n; q++) X[q] = -X[q];
+}
 
 // TRILIN_INTERP_PROP (shdom90.f90:17-346), 'ON' method, one species at one
 // point: trilinear extinction, the scattering-weighted albedo and the eight
-// corners' phase pointers with weights, merged by table index and sorted by
-// weight (stable), before delta-M scaling.
+// corners' phase pointers with weights (NMICRO per corner: the property
+// point's tables times its phase weights), merged by table index and sorted
+// by weight as SSORT orders them, before delta-M scaling.
 __host__ __device__ inline void prop_point(const PropGrid& g, int ipa, float X, float Y, float Z, float& extf,
                                            float& albf, int32_t* iph_out, float* wt_out) {
 #pragma clang fp contract(off)
@@ -47,8 +156,8 @@ __host__ __device__ inline void prop_point(const PropGrid& g, int ipa, float X, 
   const size_t npg = (size_t)npx * npy * npz;
   const float* E = g.extinctp + (size_t)ipa * npg;
   const float* A = g.albedop + (size_t)ipa * npg;
-  const float* PW = g.phasewtp + (size_t)ipa * npg;
-  const int32_t* IP = g.iphasep + (size_t)ipa * npg;
+  const float* PW = g.phasewtp + (size_t)ipa * npg * g.nmicro;
+  const int32_t* IP = g.iphasep + (size_t)ipa * npg * g.nmicro;
   int il = 0, iu = npz;
   while (iu - il > 1) {
     const int im = (iu + il) / 2;
@@ -91,35 +200,21 @@ __host__ __device__ inline void prop_point(const PropGrid& g, int ipa, float X, 
   }
   extf = (float)ext;
   albf = ((double)extf > g.extmin) ? (float)(scatter / (double)extf) : (float)(scatter / g.extmin);
-  int iph[8];
-  float wt[8];
-  for (int q = 0; q < 8; q++) {
-    iph[q] = IP[I[q]];
-    wt[q] = (scatter >= g.scatmin) ? (float)((double)PW[I[q]] * scat[q] / scatter)
-                                   : (float)((double)PW[I[q]] * scat[q] / g.scatmin);
-  }
-  for (int q = 0; q < 8; q++)
-    for (int q2 = q + 1; q2 < 8; q2++)
-      if (iph[q] == iph[q2]) {
-        wt[q] += wt[q2];
-        wt[q2] = 0.f;
-      }
-  // stable sort by decreasing weight
-  int ord[8];
-  for (int q = 0; q < 8; q++) ord[q] = q;
-  for (int q = 1; q < 8; q++) {
-    const int key = ord[q];
-    int j = q - 1;
-    while (j >= 0 && wt[ord[j]] < wt[key]) {
-      ord[j + 1] = ord[j];
-      j--;
+  const int nm = g.nmicro, npw = 8 * nm;
+  for (int c = 0; c < 8; c++)
+    for (int m = 0; m < nm; m++) {
+      const size_t src = (size_t)I[c] * nm + m;
+      iph_out[c * nm + m] = IP[src];
+      wt_out[c * nm + m] = (scatter >= g.scatmin) ? (float)((double)PW[src] * scat[c] / scatter)
+                                                  : (float)((double)PW[src] * scat[c] / g.scatmin);
     }
-    ord[j + 1] = key;
-  }
-  for (int q = 0; q < 8; q++) {
-    iph_out[q] = iph[ord[q]];
-    wt_out[q] = wt[ord[q]];
-  }
+  for (int q = 0; q < npw; q++)
+    for (int q2 = q + 1; q2 < npw; q2++)
+      if (iph_out[q] == iph_out[q2]) {
+        wt_out[q] += wt_out[q2];
+        wt_out[q2] = 0.f;
+      }
+  ssort_decreasing(wt_out, iph_out, npw);
 }
 
 // Coefficient l of the species' mixed phase function (the reference's LEGENT
@@ -129,7 +224,7 @@ __host__ __device__ inline float mixed_leg(const LegTable& t, const int32_t* iph
 #pragma clang fp contract(off)
   if (wt[0] >= 0.999f) return t.legen[(size_t)(iph[0] - 1) * t.nl + l];
   float out = 0.f;
-  for (int q = 0; q < 8; q++) {
+  for (int q = 0; q < t.npw; q++) {
     const float w = wt[q];
     if (w <= 1e-5f) continue;
     out = out + t.legen[(size_t)(iph[q] - 1) * t.nl + l] * w;
@@ -143,7 +238,7 @@ __host__ __device__ inline float prepare_f(const LegTable& t, const int32_t* iph
 #pragma clang fp contract(off)
   if (wt[0] >= 0.999f) return t.legen[(size_t)(iph[0] - 1) * t.nl + t.ml + 1];
   float f = 0.f;
-  for (int q = 0; q < 8; q++) f = f + t.legen[(size_t)(iph[q] - 1) * t.nl + t.ml + 1] * wt[q];
+  for (int q = 0; q < t.npw; q++) f = f + t.legen[(size_t)(iph[q] - 1) * t.nl + t.ml + 1] * wt[q];
   return f;
 }
 
@@ -173,8 +268,8 @@ __host__ __device__ inline void point_source_constants(const LegTable& t, int np
     float legent = 0.f;
     for (int ipa = 0; ipa < npart; ipa++) {
       const double scat = (double)(ext[ipa * stride] * albs[ipa * stride]);
-      const int32_t* ip = iph + ipa * stride * 8;
-      const float* w = wt + ipa * stride * 8;
+      const int32_t* ip = iph + ipa * stride * t.npw;
+      const float* w = wt + ipa * stride * t.npw;
       float l1 = mixed_leg(t, ip, w, l);
       if (deltam) l1 = l1 / (1 - mixed_leg(t, ip, w, ML + 1));
       legent = (float)((double)legent + scat * (double)l1);
@@ -188,8 +283,8 @@ __host__ __device__ inline void point_source_constants(const LegTable& t, int np
     for (int ipa = 0; ipa < npart; ipa++) {
       const float W = (total_ext == 0.0f) ? 1.0f : ext[ipa * stride] / total_ext;
       if (W == 0.0f) continue;
-      const int32_t* ip = iph + ipa * stride * 8;
-      const float* w = wt + ipa * stride * 8;
+      const int32_t* ip = iph + ipa * stride * t.npw;
+      const float* w = wt + ipa * stride * t.npw;
       float legt = mixed_leg(t, ip, w, l);
       if (deltam) {
         float F;
@@ -197,13 +292,16 @@ __host__ __device__ inline void point_source_constants(const LegTable& t, int np
           F = t.legen[(size_t)(ip[0] - 1) * t.nl + ML + 1];
         } else {
           // The reference indexes LEGEN(Q,ML+1,IPHASE(Q)) here (first index Q,
-          // shdomsub1.f:1688-1689); with NSTLEG=1 that is coefficient ML+Q of
-          // the flattened table (zero past its end).
+          // shdomsub1.f:1688-1689): element Q of the flat LEGEN(NSTLEG,
+          // 0:NLEG, NUMPHASE) array, which with NSTLEG = 1 is coefficient
+          // ML+Q (zero past the table's end).
+          const float* fl = t.flat ? t.flat : t.legen;
+          const size_t fsz = t.flat ? t.flat_size : t.table_size;
           F = 0.f;
-          for (int q = 0; q < 8; q++) {
+          for (int q = 0; q < t.npw; q++) {
             if (w[q] <= 1e-5f) continue;
-            const size_t flat = (size_t)(ip[q] - 1) * t.nl + ML + 1 + q;
-            const float c = flat < t.table_size ? t.legen[flat] : 0.f;
+            const size_t flat = ((size_t)(ip[q] - 1) * t.nl + ML + 1) * t.nstleg + q;
+            const float c = flat < fsz ? fl[flat] : 0.f;
             F = F + c * w[q];
           }
         }
@@ -233,8 +331,8 @@ __host__ __device__ inline void point_element_constants(const LegTable& t1, cons
     float legent = 0.f;
     for (int ipa = 0; ipa < npart; ipa++) {
       const double scat = (double)(ext[ipa * stride] * albs[ipa * stride]);
-      const int32_t* ip = iph + ipa * stride * 8;
-      const float* w = wt + ipa * stride * 8;
+      const int32_t* ip = iph + ipa * stride * t1.npw;
+      const float* w = wt + ipa * stride * t1.npw;
       float l1 = mixed_leg(te, ip, w, l);
       if (deltam) l1 = l1 / (1 - mixed_leg(t1, ip, w, ML + 1));
       legent = (float)((double)legent + scat * (double)l1);
@@ -256,8 +354,8 @@ __host__ __device__ inline void point_element_tms(const LegTable& t1, const LegT
   for (int ipa = 0; ipa < npart; ipa++) {
     const float W = (total_ext == 0.0f) ? 1.0f : ext[ipa * stride] / total_ext;
     if (W == 0.0f) continue;
-    const int32_t* ip = iph + ipa * stride * 8;
-    const float* w = wt + ipa * stride * 8;
+    const int32_t* ip = iph + ipa * stride * t1.npw;
+    const float* w = wt + ipa * stride * t1.npw;
     const float f = mixed_leg(t1, ip, w, ML + 1);
     const float da = albs[ipa * stride] * secmu0 * W;
     for (int l = 0; l <= ML; l++) tms_e[l] += da * (mixed_leg(te, ip, w, l) / (1 - f));
@@ -282,8 +380,8 @@ __host__ __device__ inline int point_render_constants(const LegTable& t, int npa
   for (int ipa = 0; ipa < npart; ipa++) {
     const float W = (total_ext == 0.0f) ? 1.0f : ext[ipa * stride] / total_ext;
     if (W == 0.0f) continue;
-    const int32_t* ip = iph + ipa * stride * 8;
-    const float* w = wt + ipa * stride * 8;
+    const int32_t* ip = iph + ipa * stride * t.npw;
+    const float* w = wt + ipa * stride * t.npw;
     const float f = mixed_leg(t, ip, w, ML + 1);
     const float da = albs[ipa * stride] * secmu0 * W;
     if (tms)
@@ -295,7 +393,7 @@ __host__ __device__ inline int point_render_constants(const LegTable& t, int npa
       }
       n++;
     } else {
-      for (int q = 0; q < 8; q++) {
+      for (int q = 0; q < t.npw; q++) {
         if (w[q] <= 1e-5f) continue;
         if (ss_iph) {
           ss_iph[n] = ip[q] - 1;
@@ -332,17 +430,18 @@ __host__ __device__ inline void eddington_layers(const LegTable& t, int npart, b
     }
     od[L - 1] = (zgrid[iz] - zgrid[iz - 1]) * (ext0 + ext1) / 2;
     alb_out[L - 1] = (ext0 + ext1 > 0.f) ? (scat0 + scat1) / (ext0 + ext1) : 0.f;
-    const float w00 = wt[ip0 * 8], w10 = wt[ip1 * 8];   // species 1's leading weights
+    const int npw = t.npw;
+    const float w00 = wt[ip0 * npw], w10 = wt[ip1 * npw];   // species 1's leading weights
     float g0 = 0.f, g1 = 0.f;
     for (int ipa = 0; ipa < npart; ipa++) {
-      const int32_t* ph0 = iph + (ipa * stride + ip0) * 8;
-      const int32_t* ph1 = iph + (ipa * stride + ip1) * 8;
-      const float* wa0 = wt + (ipa * stride + ip0) * 8;
-      const float* wa1 = wt + (ipa * stride + ip1) * 8;
+      const int32_t* ph0 = iph + (ipa * stride + ip0) * npw;
+      const int32_t* ph1 = iph + (ipa * stride + ip1) * npw;
+      const float* wa0 = wt + (ipa * stride + ip0) * npw;
+      const float* wa1 = wt + (ipa * stride + ip1) * npw;
       auto coef = [&](const int32_t* ph, const float* wa, float lead, int l) {
         if (lead >= 0.999f) return t.legen[(size_t)(ph[0] - 1) * nl + l];
         float c = 0.f;
-        for (int q = 0; q < 8; q++) c = c + t.legen[(size_t)(ph[q] - 1) * nl + l] * wa[q];
+        for (int q = 0; q < npw; q++) c = c + t.legen[(size_t)(ph[q] - 1) * nl + l] * wa[q];
         return c;
       };
       float le0 = coef(ph0, wa0, w00, 1), le1 = coef(ph1, wa1, w10, 1);

@@ -374,7 +374,7 @@ size_t plan_footprint_bytes(const Plan& p) {
   if (p.host_medium)
     b += p.ss_off.size() * 4 + p.ss_iph.size() * 4 + p.ss_coef.size() * 4;
   else   // the device set-up: at most 8 pairs per species and point, its work arrays
-    b += npts * 4 * (1 + 16 * (size_t)p.npart) + npts * 4 * (4 + 3 * ml1 + 18 * (size_t)p.npart) +
+    b += npts * 4 * (1 + 2 * (size_t)p.npw * p.npart) + npts * 4 * (4 + 3 * ml1 + (2 + 2 * (size_t)p.npw) * p.npart) +
          (size_t)p.d.npx * p.d.npy * p.d.npz * 4 * (4 * (size_t)p.npart + 1) + (npts / p.nz) * (size_t)p.nz * 80;
   b += (size_t)p.d.npx * p.d.npy * p.d.npz * 4 + p.zlevels.size() * 4;   // direct-beam extinction
   return b;
@@ -397,8 +397,8 @@ static int device_medium(const Plan& p, State& s, std::string& err) {
   TRY(upload(s, zl, p.zlevels.data(), p.zlevels.size(), err));
   TRY(upload(s, leg, p.legen.data(), p.legen.size(), err));
   TRY(upload(s, legp, p.legenp.data(), p.legenp.size(), err));
-  m.g = PropGrid{extp, albp, iphp, wtp, zl, d.npx, d.npy, d.npz, d.delx, d.dely, p.extmin, p.scatmin};
-  m.t = LegTable{leg, p.nleg + 1, p.ml, p.legen.size()};
+  m.g = PropGrid{extp, albp, iphp, wtp, zl, d.npx, d.npy, d.npz, d.delx, d.dely, p.extmin, p.scatmin, p.nmicro};
+  m.t = LegTable{leg, p.nleg + 1, p.ml, p.legen.size(), p.npw};
   m.legenp = legp;
   m.nlegp = p.nlegp;
   m.npart = p.npart;
@@ -415,8 +415,8 @@ static int device_medium(const Plan& p, State& s, std::string& err) {
   m.zgrid = s.zgrid;
   TRY(alloc0(s, m.extinct, (size_t)p.npart * np, err));
   TRY(alloc0(s, m.albedo, (size_t)p.npart * np, err));
-  TRY(alloc0(s, m.iphase, (size_t)p.npart * np * 8, err));
-  TRY(alloc0(s, m.wt, (size_t)p.npart * np * 8, err));
+  TRY(alloc0(s, m.iphase, (size_t)p.npart * np * p.npw, err));
+  TRY(alloc0(s, m.wt, (size_t)p.npart * np * p.npw, err));
   TRY(alloc0(s, s.ext, (size_t)s.nptsp, err));
   m.total_ext = s.ext;
   TRY(alloc0(s, s.alb, np, err));

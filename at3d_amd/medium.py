@@ -22,8 +22,8 @@ class Scatterer:
     delx: float
     dely: float
     z: np.ndarray                          # (nz,) km
-    table_index: np.ndarray | None = None  # (nx, ny, nz) 1-based
-    phase_weights: np.ndarray | None = None
+    table_index: np.ndarray | None = None  # (nx, ny, nz) or (num_micro, nx, ny, nz), 1-based
+    phase_weights: np.ndarray | None = None  # the same shape: weights of the tables mixed at a point
 
     def __post_init__(self):
         self.extinction = np.asarray(self.extinction, np.float32)
@@ -41,10 +41,20 @@ class Scatterer:
         if self.table_index is None:
             self.table_index = np.ones(shp, np.int32)
         if self.phase_weights is None:
-            self.phase_weights = np.ones(shp, np.float32)
+            self.phase_weights = np.ones(np.shape(self.table_index), np.float32)
+        self.table_index = np.asarray(self.table_index, np.int32)
+        self.phase_weights = np.asarray(self.phase_weights, np.float32)
+        if self.table_index.shape != self.phase_weights.shape or \
+                self.table_index.shape[-3:] != shp or self.table_index.ndim not in (3, 4):
+            raise ValueError("table_index and phase_weights must be (x, y, z) or (num_micro, x, y, z) arrays")
         if np.any(self.extinction < 0) or np.any(self.ssalb < 0) or np.any(self.ssalb > 1):
             raise ValueError("extinction must be >= 0 and ssalb in [0, 1]")
 
     @property
     def shape(self):
         return self.extinction.shape
+
+    @property
+    def num_micro(self):
+        """Phase tables mixed at each grid point (medium.py's num_micro)."""
+        return 1 if self.table_index.ndim == 3 else self.table_index.shape[0]

@@ -93,18 +93,36 @@ class RTE:
         maxleg = max(t.shape[1] for t in legs)
         tables, offset = [], 0
         iphasep = []
+        # phase tables per grid point (num_micro): every species padded to the
+        # largest count with zero weights (solver.py:2340-2375)
+        nmicro = max(s.num_micro for s in scs)
+
+        def micro(a, fill):
+            a = np.asarray(a)
+            if a.ndim == 3:
+                a = a[None]
+            if a.shape[0] < nmicro:
+                a = np.concatenate([a, np.full((nmicro - a.shape[0],) + a.shape[1:], fill, a.dtype)])
+            return np.moveaxis(a, 0, -1)   # (x, y, z, num_micro)
+        # each species' pointers are offset by the largest pointer of the
+        # species before it (solver.py:2353-2356); pointers of 0 become 1
+        pws = []
         for s, t in zip(scs, legs):
             t = np.pad(t, ((0, 0), (0, maxleg - t.shape[1]), (0, 0)))
             tables.append(t)
-            iphasep.append(np.asarray(s.table_index, np.int32) + offset)
-            offset += t.shape[2]
+            ti = micro(np.asarray(s.table_index, np.int32), 1) + offset
+            iphasep.append(ti)
+            pws.append(micro(np.asarray(s.phase_weights, np.float32), 0.0))
+            offset = max(offset, int(ti.max()))
         table3 = np.concatenate(tables, axis=2)           # (nstleg, nleg+1, numphase)
         table = table3[0]                                 # (nleg+1, numphase)
         legenp = _lib.as_f32(np.transpose(table3, (2, 1, 0)))   # [numphase][nlegp+1][nstleg]
         ext = _lib.as_f32(np.stack([s.extinction for s in scs]))
         alb = _lib.as_f32(np.stack([s.ssalb for s in scs]))
-        iph = _lib.as_i32(np.stack(iphasep))
-        pw = _lib.as_f32(np.stack([s.phase_weights for s in scs]))
+        iph = np.stack(iphasep)
+        iph[iph == 0] = 1
+        iph = _lib.as_i32(iph)
+        pw = _lib.as_f32(np.stack(pws))
         zl = _lib.as_f32(scs[0].z)
         bcx, bcy = npar["x_boundary_condition"], npar["y_boundary_condition"]
         if bcx != bcy:
@@ -141,6 +159,7 @@ class RTE:
         d.maxig, d.maxic, d.maxiv, d.maxido = self._memory_params(npar, npx, npy, npz, table)
         d.device_medium = 1 if self._device_setup else 0
         d.nstokes = self._nstokes
+        d.nmicro = nmicro
         self._keep = [ext, alb, iph, pw, zl, legenp]
         self._desc = d
         L = _lib.lib()

@@ -16,7 +16,10 @@ class Workload:
     delx: float
     dely: float
     z: np.ndarray
-    species: list = field(default_factory=list)   # list of (ext, ssalb, legcoef[(6,) nleg+1, ntab])
+    # list of (ext, ssalb, legcoef[(6,) nleg+1, ntab]) or (ext, ssalb, legcoef,
+    # table_index, phase_weights) with 1-based table_index / phase_weights of
+    # shape ([num_micro,] nx, ny, nz)
+    species: list = field(default_factory=list)
     bc: str = "open"
     nmu: int = 16
     nphi: int = 32
@@ -191,8 +194,8 @@ def make_rte(w: Workload, device_setup=True, **overrides):
                                    solution_accuracy=w.solacc, deltam=w.deltam,
                                    split_accuracy=0.0)
     cfg.update(overrides)
-    med = {f"species_{i}": Scatterer(e, a, leg, w.delx, w.dely, w.z)
-           for i, (e, a, leg) in enumerate(w.species)}
+    med = {f"species_{i}": Scatterer(sp[0], sp[1], sp[2], w.delx, w.dely, w.z, *sp[3:5])
+           for i, sp in enumerate(w.species)}
     return RTE(cfg, med, source.solar(w.wavelength, w.solarmu, w.solaraz_deg),
                surface.lambertian(w.gndalbedo), num_stokes=w.nstokes, device_setup=device_setup)
 

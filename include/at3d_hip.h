@@ -37,8 +37,8 @@ typedef struct at3d_scene_desc {
   int32_t npart;           /* number of scattering species                     */
   const float* extinctp;   /* [npart][npx][npy][npz]  1/km                      */
   const float* albedop;    /* [npart][npx][npy][npz]                            */
-  const int32_t* iphasep;  /* [npart][npx][npy][npz] 1-based into legenp table   */
-  const float* phasewtp;   /* [npart][npx][npy][npz] phase weights (num_micro=1) */
+  const int32_t* iphasep;  /* [npart][npx][npy][npz][nmicro] 1-based into legenp */
+  const float* phasewtp;   /* [npart][npx][npy][npz][nmicro] phase weights       */
   int32_t numphase;        /* number of tabulated phase functions              */
   int32_t nlegp;           /* highest Legendre index in legenp                 */
   const float* legenp;     /* [numphase][nlegp+1], coefficients incl. (2l+1);
@@ -83,6 +83,12 @@ typedef struct at3d_scene_desc {
    * NSTLEG = 6).  A polarized solve runs on the base grid (splitacc 0), with
    * the per-point medium made on the host, and has no Levis gradient. */
   int32_t nstokes;
+  /* Phase tables mixed at each property point (num_micro, the reference's
+   * MAXNMICRO; 0 is taken as 1, at most 8): iphasep / phasewtp hold nmicro
+   * (table, weight) pairs per point and species, and every grid point mixes
+   * 8 x nmicro of them (TRILIN_INTERP_PROP, shdom90.f90:17-346).  With more
+   * than one, the solve runs on the base grid and has no Levis gradient. */
+  int32_t nmicro;
 } at3d_scene_desc;
 
 typedef struct at3d_plan at3d_plan;   /* host-side setup (no GPU needed)     */

@@ -130,8 +130,8 @@ class Species:
     legcoef: np.ndarray             # (nleg+1, ntable) scalar I-I coefficients incl. (2l+1),
                                     # or (6, nleg+1, ntable): the six phase-matrix elements
                                     # a1, a2, a3, a4, b1, b2 (medium.py stokes_index order)
-    table_index: np.ndarray | None = None   # (npx,npy,npz) 1-based, default 1
-    phase_weights: np.ndarray | None = None
+    table_index: np.ndarray | None = None   # (npx,npy,npz) or (num_micro,npx,npy,npz) 1-based, default 1
+    phase_weights: np.ndarray | None = None  # the same shape, default 1
 
 
 @dataclass
@@ -287,19 +287,29 @@ def _prepare_optical_properties(st: RefState):
     maxpg, npart = d["maxpg"], len(sc.species)
     extinctp = np.zeros((maxpg, npart), np.float32, order="F")
     albedop = np.zeros((maxpg, npart), np.float32, order="F")
-    maxnmicro = 1
+    # phase tables per property point (num_micro): table_index / phase_weights
+    # of shape (num_micro, npx, npy, npz), padded to the largest count with
+    # pointer 1 and weight 0 (solver.py:2339-2363)
+    def micro(a, fill, dtype):
+        a = np.asarray(a, dtype)
+        return a[None] if a.ndim == 3 else a
+    nmicros = [1 if sp.table_index is None else micro(sp.table_index, 1, np.int32).shape[0]
+               for sp in sc.species]
+    maxnmicro = max(nmicros)
     iphasep = np.zeros((maxnmicro, maxpg, npart), np.int32, order="F")
     phasewtp = np.zeros((maxnmicro, maxpg, npart), np.float32, order="F")
     tables = []
     for i, sp in enumerate(sc.species):
         extinctp[:, i] = np.asarray(sp.extinction, np.float32).ravel()
         albedop[:, i] = np.asarray(sp.ssalb, np.float32).ravel()
-        ti = np.ones(maxpg, np.int32) if sp.table_index is None else \
-            np.asarray(sp.table_index, np.int32).ravel()
-        iphasep[0, :, i] = ti + iphasep.max()
-        pw = np.ones(maxpg, np.float32) if sp.phase_weights is None else \
-            np.asarray(sp.phase_weights, np.float32).ravel()
-        phasewtp[0, :, i] = pw
+        ti = np.ones((1, maxpg), np.int32) if sp.table_index is None else \
+            micro(sp.table_index, 1, np.int32).reshape(nmicros[i], -1)
+        pw = np.ones((1, maxpg), np.float32) if sp.phase_weights is None else \
+            micro(sp.phase_weights, 0, np.float32).reshape(nmicros[i], -1)
+        ti = np.concatenate([ti, np.ones((maxnmicro - ti.shape[0], maxpg), np.int32)])
+        pw = np.concatenate([pw, np.zeros((maxnmicro - pw.shape[0], maxpg), np.float32)])
+        iphasep[:, :, i] = ti + iphasep.max()
+        phasewtp[:, :, i] = pw
         tables.append(_legcoef3(sp.legcoef, d["nstleg"]))
     iphasep[iphasep == 0] = 1
     maxleg = max(t.shape[1] for t in tables)
@@ -664,7 +674,7 @@ def dense_sh(ptr, arr, npts, nlm, k=0):
 
 def scene_from_workload(w, **kw) -> Scene:
     """Reference scene for an at3d_amd.synthetic.Workload (same arrays)."""
-    sp = [Species(e, a, leg) for (e, a, leg) in w.species]
+    sp = [Species(*s[:5]) for s in w.species]
     args = dict(delx=w.delx, dely=w.dely, zlevels=w.z, species=sp, bc=w.bc,
                 num_mu_bins=w.nmu, num_phi_bins=w.nphi, solacc=w.solacc,
                 solarmu=w.solarmu, solaraz=float(np.deg2rad(w.solaraz_deg)),

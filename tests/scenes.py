@@ -44,6 +44,25 @@ def pol_blob(nmu=8, nphi=16, bc="open", n=8, seed=0):
                       nstokes=3)
 
 
+def micro_blob(bc="open", nmicro=2, seed=0, nmu=8, nphi=16):
+    """Several phase tables per property point (num_micro): the 8^3 blob with
+    three HG tables (g = 0.8, 0.85, 0.9, 64 orders) mixed at every point by
+    `nmicro` seeded (table, weight) pairs, plus Rayleigh scattering with one
+    table (padded to `nmicro` with weight 0)."""
+    w = blob8(bc, seed=seed, nmu=nmu, nphi=nphi)
+    e, a, _ = w.species[0]
+    rng = np.random.default_rng(seed + 7)
+    tables = np.concatenate([S.hg_legendre(g, 64) for g in (0.8, 0.85, 0.9)], axis=1)
+    ti = rng.integers(1, 4, size=(nmicro,) + e.shape).astype(np.int32)
+    wt = rng.uniform(0.2, 1.0, size=(nmicro,) + e.shape)
+    wt = (wt / wt.sum(axis=0)).astype(np.float32)
+    rext = np.full(e.shape, 0.05, np.float32)
+    ralb = np.ones(e.shape, np.float32)
+    rleg = np.array([[1.0], [0.0], [0.5]])
+    return S.Workload("micro_blob", w.delx, w.dely, w.z, [(e, a, tables, ti, wt), (rext, ralb, rleg)], bc=bc,
+                      nmu=nmu, nphi=nphi, solacc=1e-5)
+
+
 def rel_l2(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
