@@ -622,6 +622,12 @@ int at3d_state_set_option(at3d_state* st, const char* key, int32_t value, char* 
     st->s.use_ring = value ? 1 : 0;
     return AT3D_OK;
   }
+  if (k == "sweep_ring4") {
+    std::string e;
+    const int rc = state_set_ring4(st->s, value != 0, e);
+    if (rc) set_err(err, errlen, e.c_str());
+    return rc;
+  }
   if (k == "iteration_graph") {
     st->s.use_iter_graph = value ? 1 : 0;
     return AT3D_OK;
@@ -652,6 +658,7 @@ double at3d_state_timing(const at3d_state* st, const char* key) {
   if (k == "sweep_ring") return (s.ring_ok && s.use_ring && !s.force_general) ? 1.0 : 0.0;
   if (k == "sweep_ring_lds") return (double)(s.ring_full ? s.ring_lds_full : s.ring_lds);
   if (k == "sweep_ring_slots") return s.slots;
+  if (k == "sweep_ring4") return s.ring4;
   if (k == "sweep_sectors") return s.nsectors;
   if (k == "sweep_general_points") return s.ngeneral;
   if (k == "iters") return s.iters_last;

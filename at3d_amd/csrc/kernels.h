@@ -133,6 +133,9 @@ struct SweepArgs {
   // boundary radiances and no clamp of the cell source at zero.
   int nst = 1;
   size_t pdo = 0;
+  // the ring holds radiances alone (4-byte slots) and the face sources are
+  // read from the ring-ordered source row
+  int ring4 = 0;
 };
 
 // The fixed part of one ring entry of one ordinate, 16 bytes: head = point |
@@ -157,14 +160,15 @@ constexpr int RING_KX = 4; // entries per thread loaded together past those (ful
 
 // Bytes of LDS k_sweep_ring needs: the step classes (16 bytes each; ncls = 0
 // for the full instantiation, which reads them from device memory), the ring
-// of (radiance, source) pairs, gld level offsets, gld + 1 sector offsets and
-// gld level flags.
-inline size_t ring_lds_bytes(int slots, int gld, int ncls) {
-  return (size_t)ncls * 16 + (size_t)slots * 8 + (size_t)gld * 9 + 4;
+// of (radiance, source) pairs (8 bytes a slot, or radiances alone, 4), gld
+// level offsets, gld + 1 sector offsets and gld level flags.
+inline size_t ring_lds_bytes(int slots, int gld, int ncls, int slot_bytes = 8) {
+  return (size_t)ncls * 16 + (size_t)slots * slot_bytes + (size_t)gld * 9 + 4;
 }
 // Allows k_sweep_ring launches with lds_fast / lds_full bytes of dynamic LDS
-// (lds_fast 0: the fast instantiation is not used); nonzero on failure.
-int sweep_ring_max_lds(size_t lds_fast, size_t lds_full);
+// (lds_fast 0: the fast instantiation is not used) for the ring of pairs or
+// (ring4) of radiances; nonzero on failure.
+int sweep_ring_max_lds(size_t lds_fast, size_t lds_full, bool ring4 = false);
 void launch_sweep_records(hipStream_t st, const SweepArgs& a);
 // face_ok of every point of the octants with ordinates.
 void launch_face_mask(hipStream_t st, const SweepArgs& a);

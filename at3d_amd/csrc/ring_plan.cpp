@@ -142,7 +142,7 @@ static bool build_octant_levels(const Plan& p, int o, bool used, const std::vect
 }
 
 bool build_ring_plan(const Plan& p, const bool used[8], size_t lds_budget, RingPlan& r, std::string& err,
-                     const std::vector<int32_t>* cls_in) {
+                     const std::vector<int32_t>* cls_in, int slot_bytes) {
   const int npts = p.npts;
   r.nlev_max = 0;
   std::vector<int32_t> cls;
@@ -178,7 +178,7 @@ bool build_ring_plan(const Plan& p, const bool used[8], size_t lds_budget, RingP
   };
   r.sector_points = 8;
   r.slots = ring_slots(8);
-  if (ring_plan_lds_bytes(r.slots, r.nlev_max, p.nz * 4) > lds_budget) {
+  if (ring_plan_lds_bytes(r.slots, r.nlev_max, p.nz * 4, slot_bytes) > lds_budget) {
     r.sector_points = 4;
     r.slots = ring_slots(4);
   }

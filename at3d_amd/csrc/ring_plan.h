@@ -38,10 +38,10 @@ struct RingPlan {
   int nlev_max = 0;
 };
 
-// Bytes of LDS the level-ordered sweep needs for a ring of `slots` entries,
-// `nlev` levels and `ncls` step classes.
-inline size_t ring_plan_lds_bytes(int slots, int nlev, int ncls) {
-  return (size_t)ncls * 16 + (size_t)slots * 8 + (size_t)(nlev + 2) * 9 + 4;
+// Bytes of LDS the level-ordered sweep needs for a ring of `slots` entries
+// of `slot_bytes` each, `nlev` levels and `ncls` step classes.
+inline size_t ring_plan_lds_bytes(int slots, int nlev, int ncls, int slot_bytes = 8) {
+  return (size_t)ncls * 16 + (size_t)slots * slot_bytes + (size_t)(nlev + 2) * 9 + 4;
 }
 
 // Builds the ring layout for the octants with ordinates (used[o] != 0), with
@@ -49,7 +49,8 @@ inline size_t ring_plan_lds_bytes(int slots, int nlev, int ncls) {
 // Returns false (err set) if the plan's level lists are incomplete.
 // cls: each point's source class (the plan's, from its source constants,
 // when null).
+// slot_bytes: 8 for a ring of (radiance, source) pairs, 4 for radiances alone.
 bool build_ring_plan(const Plan& p, const bool used[8], size_t lds_budget, RingPlan& r, std::string& err,
-                     const std::vector<int32_t>* cls = nullptr);
+                     const std::vector<int32_t>* cls = nullptr, int slot_bytes = 8);
 
 }  // namespace at3d

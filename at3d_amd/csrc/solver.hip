@@ -162,6 +162,7 @@ static SweepArgs sweep_args(const State& s) {
   a.stop = nullptr;
   a.nst = s.nst;
   a.pdo = s.pdo;
+  a.ring4 = s.ring4;
   return a;
 }
 
@@ -181,9 +182,17 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
   int dev = 0, maxlds = 0;
   HIPCHK(hipGetDevice(&dev));
   HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  // a ring of (radiance, source) pairs where it fits in LDS, else (or when
+  // asked for) of radiances alone
   RingPlan rp;
-  if (!build_ring_plan(p, used, (size_t)maxlds, rp, err, p.medium_on_device ? &s.medium_cls_h : nullptr))
-    return AT3D_ERR_NUMERIC;
+  const std::vector<int32_t>* cls = p.medium_on_device ? &s.medium_cls_h : nullptr;
+  s.ring4 = s.force_ring4 ? 1 : 0;
+  if (!build_ring_plan(p, used, (size_t)maxlds, rp, err, cls, s.ring4 ? 4 : 8)) return AT3D_ERR_NUMERIC;
+  if (!s.ring4 && ring_lds_bytes(rp.slots, rp.nlev_max + 2, 0, 8) > (size_t)maxlds) {
+    s.ring4 = 1;
+    rp = RingPlan();
+    if (!build_ring_plan(p, used, (size_t)maxlds, rp, err, cls, 4)) return AT3D_ERR_NUMERIC;
+  }
   const float* total_ext = p.medium_on_device ? s.medium_ext_h.data() : p.total_ext.data();
   s.sector_points = rp.sector_points;
   s.slots = rp.slots;
@@ -262,12 +271,13 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
   }
   // the fast instantiation keeps the step classes in LDS beside the ring;
   // where they do not fit, every column takes the full one
-  s.ring_lds = ring_lds_bytes(s.slots, s.gld, s.ncls);
-  s.ring_lds_full = ring_lds_bytes(s.slots, s.gld, 0);
+  const int slot_bytes = s.ring4 ? 4 : 8;
+  s.ring_lds = ring_lds_bytes(s.slots, s.gld, s.ncls, slot_bytes);
+  s.ring_lds_full = ring_lds_bytes(s.slots, s.gld, 0, slot_bytes);
   if (s.ring_lds > (size_t)maxlds) s.ring_full = 1;
   if (s.ring_full) s.ring_lds = 0;
   if (s.slots > 65536 || p.npts >= (1 << RECORD_PT_BITS) || p.nz >= 8192 ||
-      s.ring_lds_full > (size_t)maxlds || sweep_ring_max_lds(s.ring_lds, s.ring_lds_full) != 0)
+      s.ring_lds_full > (size_t)maxlds || sweep_ring_max_lds(s.ring_lds, s.ring_lds_full, s.ring4 != 0) != 0)
     return AT3D_OK;
   s.nz_magic = ((1ull << 40) + (unsigned long long)p.nz - 1) / (unsigned long long)p.nz;
   // Steps of one class differ only by the rounding of the float grid
@@ -752,6 +762,28 @@ void state_free(State& s) {
   s.stream = nullptr;
 }
 
+int state_set_ring4(State& s, bool on, std::string& err) {
+  if (s.force_ring4 == (on ? 1 : 0)) return AT3D_OK;
+  HIPCHK(hipStreamSynchronize(s.stream));
+  {
+    std::lock_guard<std::mutex> lock(device_mutex());
+    if (s.sweep_exec) (void)hipGraphExecDestroy(s.sweep_exec);
+    if (s.sweep_graph) (void)hipGraphDestroy(s.sweep_graph);
+    for (int k = 0; k < 2; k++) {
+      if (s.iter_exec[k]) (void)hipGraphExecDestroy(s.iter_exec[k]);
+      if (s.iter_graph[k]) (void)hipGraphDestroy(s.iter_graph[k]);
+      s.iter_exec[k] = nullptr;
+      s.iter_graph[k] = nullptr;
+    }
+  }
+  s.sweep_exec = nullptr;
+  s.sweep_graph = nullptr;
+  s.force_ring4 = on ? 1 : 0;
+  // the new layout's arrays are allocated afresh (the old ones are held
+  // until the state is freed)
+  return setup_ring(*s.plan, s, err);
+}
+
 // Replaces s by a new state of the refined grid `owned` (null: s's base
 // plan), keeping the options and the iteration state; the solution arrays
 // come from `f` (the refined grid's points; null: none, as before an
@@ -762,6 +794,7 @@ static int state_replace(State& s, std::shared_ptr<Plan> owned, SplitFields* f, 
   n.base_plan = s.base_plan;
   n.use_ring = s.use_ring;
   n.force_general = s.force_general;
+  n.force_ring4 = s.force_ring4;
   n.use_graph = s.use_graph;
   n.use_iter_graph = s.use_iter_graph;
   int rc = state_create(q, n, err);

@@ -109,6 +109,8 @@ struct State {
   int* col_general = nullptr;
   std::vector<int32_t> col_general_h;   // columns with multi-cell traces
   int ring_full = 0;                    // every column needs the full ring instantiation
+  int ring4 = 0;                        // the ring holds radiances alone (4-byte slots)
+  int force_ring4 = 0;                  // option: 4-byte slots even where pairs fit
   int* ngeneral_d = nullptr;
   int4* sh_tiles = nullptr;
   int n_sh_tiles = 0;
@@ -207,5 +209,9 @@ void state_free(State& s);
 int state_init_solution(State& s, std::string& err);
 int state_init_solution_from(State& s, State& src, std::string& err);
 int state_solve(State& s, int maxiter, int verbose, std::string& err);
+// Rebuilds the level-ordered sweep's layout with a ring of radiances alone
+// (on) or of (radiance, source) pairs where they fit (off); the captured
+// graphs are dropped.  The solution is kept.
+int state_set_ring4(State& s, bool on, std::string& err);
 
 }  // namespace at3d

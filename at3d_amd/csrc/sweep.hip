@@ -464,7 +464,12 @@ struct RingSet {
 // but the record prefetch, and runs the columns (and grids) that need none
 // of them.  Each launch covers a hemisphere's columns; a workgroup whose
 // column belongs to the other instantiation returns at once.
-template <bool FULL>
+//
+// R4: the ring holds radiances alone (4 bytes a slot, for grids whose
+// (radiance, source) ring does not fit in LDS); a face corner's source is
+// read from the column's ring-ordered source row at the corner's position,
+// which its slot gives (the ring holds the last C positions).
+template <bool FULL, bool R4>
 __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int col0) {
   if (a.stop && *a.stop) return;
   extern __shared__ float4 lds4[];
@@ -485,7 +490,8 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
   float4* geo_lds = lds4;                                   // [ncls] (fast only)
   const float4* geo = FULL ? a.geo + (size_t)col * a.ncls : geo_lds;
   float2* ring = reinterpret_cast<float2*>(lds4 + (FULL ? 0 : a.ncls));
-  int32_t* off = reinterpret_cast<int32_t*>(ring + C);
+  float* ring1 = reinterpret_cast<float*>(ring);   // R4
+  int32_t* off = R4 ? reinterpret_cast<int32_t*>(ring1 + C) : reinterpret_cast<int32_t*>(ring + C);
   int32_t* soffl = off + a.gld;                     // [gld + 1] sectors by completion level
   uint8_t* full = reinterpret_cast<uint8_t*>(soffl + a.gld + 1);
   const int32_t* goff = a.ring_off + (size_t)o * a.gld;
@@ -498,6 +504,19 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
   const int32_t* soff = a.sec_off + (size_t)o * (a.gld + 1);
   const int32_t* sp0 = a.sec_p0 + a.sec_base[o];
   const uint4* sslot = a.sec_slot + a.sec_base[o];
+  // a ring slot's (radiance, source): with R4 the source at the slot's
+  // position in [lo, lo + C), lo = the level's end - C, lom = lo mod C
+  auto gather = [&](uint32_t slot, int lo, int lom) -> float2 {
+    if (!R4) return ring[slot];
+    int d = (int)slot - lom;
+    if (d < 0) d += C;
+    return make_float2(ring1[slot], se[lo + d]);
+  };
+  auto rad_of = [&](uint32_t slot) { return R4 ? ring1[slot] : ring[slot].x; };
+  auto lo_mod = [&](int lo) {
+    int m = lo % C;
+    return m < 0 ? m + C : m;
+  };
   const bool wide = a.sector_points == 8;
   const int tid = threadIdx.x;
   const float sky = kst ? 0.0f : a.skyrad[col];
@@ -532,7 +551,10 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
   auto put = [&](int jl, int k0, float val, float s1) {
     int i = k0 + jl;
     if (i >= C) i -= C;
-    ring[i] = make_float2(val, s1);
+    if (R4)
+      ring1[i] = val;
+    else
+      ring[i] = make_float2(val, s1);
   };
   // Entries past the register slots (levels longer than RING_K * RING_THREADS,
   // full instantiation only), RING_KX per thread at a time: their records,
@@ -555,14 +577,15 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
     }
     float2 q[RING_KX][4];
     float4 gq[RING_KX];
+    const int lo = end - C, lom = lo_mod(lo);
 #pragma unroll
     for (int u = 0; u < RING_KX; u++) {
       const int kind = w[u].x >> 30;
       if (kind == 0 || kind == 3) {
-        q[u][0] = ring[w[u].y & 0xffffu];
-        q[u][1] = ring[w[u].y >> 16];
-        q[u][2] = ring[w[u].z & 0xffffu];
-        q[u][3] = ring[w[u].z >> 16];
+        q[u][0] = gather(w[u].y & 0xffffu, lo, lom);
+        q[u][1] = gather(w[u].y >> 16, lo, lom);
+        q[u][2] = gather(w[u].z & 0xffffu, lo, lom);
+        q[u][3] = gather(w[u].z >> 16, lo, lom);
       }
       if (kind == 0) gq[u] = geo[class_of(w[u].x)];
     }
@@ -613,16 +636,16 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
   };
   auto write_sector = [&](uint4 sl, int p0) {
     float4 v;
-    v.x = ring[sl.x & 0xffffu].x;
-    v.y = ring[sl.x >> 16].x;
-    v.z = ring[sl.y & 0xffffu].x;
-    v.w = ring[sl.y >> 16].x;
+    v.x = rad_of(sl.x & 0xffffu);
+    v.y = rad_of(sl.x >> 16);
+    v.z = rad_of(sl.y & 0xffffu);
+    v.w = rad_of(sl.y >> 16);
     *reinterpret_cast<float4*>(rad + p0) = v;
     if (wide) {
-      v.x = ring[sl.z & 0xffffu].x;
-      v.y = ring[sl.z >> 16].x;
-      v.z = ring[sl.w & 0xffffu].x;
-      v.w = ring[sl.w >> 16].x;
+      v.x = rad_of(sl.z & 0xffffu);
+      v.y = rad_of(sl.z >> 16);
+      v.z = rad_of(sl.w & 0xffffu);
+      v.w = rad_of(sl.w >> 16);
       *reinterpret_cast<float4*>(rad + p0 + 4) = v;
     }
   };
@@ -650,6 +673,7 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
     const int beg = __builtin_amdgcn_readfirstlane(off[L - 1]);
     const int end = __builtin_amdgcn_readfirstlane(off[L]);
     const int k0 = beg % C - beg;  // slot of position jl: k0 + jl, wrapped once
+    const int lo = end - C, lom = lo_mod(lo);
     float2 g[RING_K][4];
     float4 gc[RING_K];   // the steps' classes
 #pragma unroll
@@ -657,10 +681,10 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
       const int jl = beg + tid + k * RING_THREADS;
       const uint4 w0 = cur.w0[k];
       if (jl < end && (w0.x >> 30) != 1 && (w0.x >> 30) != 2) {
-        g[k][0] = ring[w0.y & 0xffffu];
-        g[k][1] = ring[w0.y >> 16];
-        g[k][2] = ring[w0.z & 0xffffu];
-        g[k][3] = ring[w0.z >> 16];
+        g[k][0] = gather(w0.y & 0xffffu, lo, lom);
+        g[k][1] = gather(w0.y >> 16, lo, lom);
+        g[k][2] = gather(w0.z & 0xffffu, lo, lom);
+        g[k][3] = gather(w0.z >> 16, lo, lom);
         gc[k] = geo[class_of(w0.x)];
       }
     }
@@ -668,15 +692,15 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
     float4 sv = make_float4(0.f, 0.f, 0.f, 0.f), sw = sv;
     if (dosec) {
       const uint4 sl = cur.sec;
-      sv.x = ring[sl.x & 0xffffu].x;
-      sv.y = ring[sl.x >> 16].x;
-      sv.z = ring[sl.y & 0xffffu].x;
-      sv.w = ring[sl.y >> 16].x;
+      sv.x = rad_of(sl.x & 0xffffu);
+      sv.y = rad_of(sl.x >> 16);
+      sv.z = rad_of(sl.y & 0xffffu);
+      sv.w = rad_of(sl.y >> 16);
       if (wide) {
-        sw.x = ring[sl.z & 0xffffu].x;
-        sw.y = ring[sl.z >> 16].x;
-        sw.z = ring[sl.w & 0xffffu].x;
-        sw.w = ring[sl.w >> 16].x;
+        sw.x = rad_of(sl.z & 0xffffu);
+        sw.y = rad_of(sl.z >> 16);
+        sw.z = rad_of(sl.w & 0xffffu);
+        sw.w = rad_of(sl.w >> 16);
       }
     }
     load(L + 1, fut);  // past the last level this loads nothing
@@ -845,17 +869,24 @@ void launch_sweep_ring(hipStream_t st, const SweepArgs& a, int col0, int ncol, s
                        size_t lds_full, bool fast, bool full) {
   if (ncol <= 0) return;
   const dim3 grid(ncol, a.nst);
-  if (fast) k_sweep_ring<false><<<grid, RING_THREADS, lds_fast, st>>>(a, col0);
-  if (full) k_sweep_ring<true><<<grid, RING_THREADS, lds_full, st>>>(a, col0);
+  if (a.ring4) {
+    if (fast) k_sweep_ring<false, true><<<grid, RING_THREADS, lds_fast, st>>>(a, col0);
+    if (full) k_sweep_ring<true, true><<<grid, RING_THREADS, lds_full, st>>>(a, col0);
+    return;
+  }
+  if (fast) k_sweep_ring<false, false><<<grid, RING_THREADS, lds_fast, st>>>(a, col0);
+  if (full) k_sweep_ring<true, false><<<grid, RING_THREADS, lds_full, st>>>(a, col0);
 }
 
-int sweep_ring_max_lds(size_t lds_fast, size_t lds_full) {
+int sweep_ring_max_lds(size_t lds_fast, size_t lds_full, bool ring4) {
+  const void* kf = ring4 ? reinterpret_cast<const void*>(k_sweep_ring<false, true>)
+                         : reinterpret_cast<const void*>(k_sweep_ring<false, false>);
+  const void* ku = ring4 ? reinterpret_cast<const void*>(k_sweep_ring<true, true>)
+                         : reinterpret_cast<const void*>(k_sweep_ring<true, false>);
   const bool ok =
       (lds_fast == 0 ||
-       hipFuncSetAttribute(reinterpret_cast<const void*>(k_sweep_ring<false>),
-                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_fast) == hipSuccess) &&
-      hipFuncSetAttribute(reinterpret_cast<const void*>(k_sweep_ring<true>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_full) == hipSuccess;
+       hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_fast) == hipSuccess) &&
+      hipFuncSetAttribute(ku, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_full) == hipSuccess;
   return ok ? 0 : 1;
 }
 

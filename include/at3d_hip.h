@@ -326,7 +326,11 @@ int at3d_levis_gradient_jacobian(at3d_state* st, const at3d_deriv_desc* deriv, i
  * sweep (bit-identical to its analytic one-cell path; for tests).
  * "sweep_ring" (0/1, default 1): use the level-ordered sweep with per-solve
  * step records and an LDS ring when the ring fits (else the point-ordered
- * sweep).  "sweep_graph" (0/1): replay the sweep launches from a HIP graph.
+ * sweep).  "sweep_ring4" (0/1, default 0): lay the ring out with 4-byte
+ * slots (radiances alone; the sources are read from the state's arrays) even
+ * where (radiance, source) pairs fit; grids whose pairs exceed LDS take it by
+ * themselves (for tests; rebuilds the layout and drops the captured graphs).
+ * "sweep_graph" (0/1): replay the sweep launches from a HIP graph.
  * "iteration_graph" (0/1, default 1): run a solve's iterations from captured
  * HIP graphs of four iterations each under device control (the convergence
  * test, FIXSH and the acceleration on the device; the host reads the control
@@ -345,7 +349,9 @@ int at3d_state_set_option(at3d_state* st, const char* key, int32_t value, char* 
  * device allocations); of the
  * last render call "render" (direction source and rays) and "render_rays"
  * (the ray kernel alone); of the last gradient call "gradient" and
- * "adjoint_rays" (the adjoint ray kernel alone).
+ * "adjoint_rays" (the adjoint ray kernel alone).  The sweep's layout: "sweep_ring"
+ * (1 when the level-ordered sweep runs), "sweep_ring_lds" (its LDS bytes),
+ * "sweep_ring_slots", "sweep_ring4" (1 for 4-byte slots), "sweep_sectors".
  * Returns a negative value for unknown keys. */
 double at3d_state_timing(const at3d_state* st, const char* key);
 
