@@ -35,6 +35,7 @@ class SceneDesc(C.Structure):
         ("transmin", C.c_float), ("tautol", C.c_double), ("transcut", C.c_double),
         ("maxig", C.c_int32), ("maxic", C.c_int32), ("maxiv", C.c_int32), ("maxido", C.c_int32),
         ("device_medium", C.c_int32), ("nstokes", C.c_int32), ("nmicro", C.c_int32),
+        ("interpmethod", C.c_int32),
     ]
 
 

@@ -622,6 +622,10 @@ int at3d_state_set_option(at3d_state* st, const char* key, int32_t value, char* 
     st->s.use_ring = value ? 1 : 0;
     return AT3D_OK;
   }
+  if (k == "render_correct_interpolate") {
+    st->s.render_correct_interp = value ? 1 : 0;
+    return AT3D_OK;
+  }
   if (k == "sweep_ring4") {
     std::string e;
     const int rc = state_set_ring4(st->s, value != 0, e);
@@ -659,6 +663,7 @@ double at3d_state_timing(const at3d_state* st, const char* key) {
   if (k == "sweep_ring_lds") return (double)(s.ring_full ? s.ring_lds_full : s.ring_lds);
   if (k == "sweep_ring_slots") return s.slots;
   if (k == "sweep_ring4") return s.ring4;
+  if (k == "sweep_ring_full") return s.ring_full;
   if (k == "sweep_sectors") return s.nsectors;
   if (k == "sweep_general_points") return s.ngeneral;
   if (k == "iters") return s.iters_last;

@@ -135,6 +135,10 @@ int state_levis_gradient(State& s, const at3d_deriv_desc& dd, int64_t nrays, con
     err = "the Levis gradient is for one phase table per property point (num_micro = 1)";
     return AT3D_ERR_ARG;
   }
+  if (p.interp_o) {
+    err = "the Levis gradient is for INTERPMETHOD 'ON' solves";
+    return AT3D_ERR_ARG;
+  }
   if (!s.initialized) {
     err = "the solution is not initialised: solve before computing a gradient";
     return AT3D_ERR_ARG;

@@ -491,11 +491,12 @@ void make_sh_tables_pol(Plan& p) {
 PropGrid plan_prop_grid(const Plan& p) {
   const at3d_scene_desc& d = p.d;
   return PropGrid{p.extinctp.data(), p.albedop.data(), p.iphasep.data(), p.phasewtp.data(), p.zlevels.data(),
-                  d.npx, d.npy, d.npz, d.delx, d.dely, p.extmin, p.scatmin, p.nmicro};
+                  d.npx, d.npy, d.npz, d.delx, d.dely, p.extmin, p.scatmin, p.nmicro, p.interp_o};
 }
 
 LegTable plan_leg_table(const Plan& p) {
   LegTable t{p.legen.data(), p.nleg + 1, p.ml, p.legen.size(), p.npw};
+  t.interp_o = p.interp_o;
   if (!p.legen6flat.empty()) {
     t.flat = p.legen6flat.data();
     t.nstleg = 6;
@@ -1439,6 +1440,15 @@ int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err) {
     return AT3D_ERR_ARG;
   }
   p.npw = 8 * p.nmicro;
+  if (d.interpmethod != 0 && d.interpmethod != 1) {
+    err = "interpmethod must be 0 ('ON') or 1 ('OO')";
+    return AT3D_ERR_ARG;
+  }
+  p.interp_o = d.interpmethod == 1;
+  if (p.interp_o && p.nmicro != 1) {
+    err = "INTERPMETHOD 'OO' takes one phase table per property point (nmicro = 1)";
+    return AT3D_ERR_ARG;
+  }
   p.zlevels.assign(d.zlevels, d.zlevels + d.npz);
   for (int i = 1; i < d.npz; i++)
     if (!(p.zlevels[i] > p.zlevels[i - 1])) {
@@ -1477,10 +1487,6 @@ int build_plan(const at3d_scene_desc& desc, Plan& p, std::string& err) {
   p.nst = d.nstokes == 0 ? 1 : d.nstokes;
   if (p.nst != 1 && p.nst != 3) {
     err = "nstokes must be 1 or 3";
-    return AT3D_ERR_ARG;
-  }
-  if (p.nst == 3 && d.splitacc > 0.0f) {
-    err = "polarized solves run on the base grid: split_accuracy must be 0";
     return AT3D_ERR_ARG;
   }
   const int nstleg = p.nst == 3 ? 6 : 1;
@@ -1600,6 +1606,7 @@ int ensure_host_medium(const Plan& pc, std::string& err) {
 
 int plan_refresh_points(Plan& p, std::string& err) {
   source_constants(p);
+  if (p.nst == 3) polarized_constants(p);
   // the two-stream start is a base-grid quantity; new points start from zero
   p.radiance0.resize((size_t)p.npts * 4, 0.f);
   boundary_points(p);

@@ -115,6 +115,9 @@ struct Plan {
   // phase tables per property point (num_micro, MAXNMICRO) and phase pairs
   // per grid point and species (8 x nmicro)
   int nmicro = 1, npw = 8;
+  // INTERPMETHOD 'OO': one table per point and species, the corner's with
+  // the most scattering
+  bool interp_o = false;
   // rendering constants
   std::vector<float> phasetab;                    // [numphase][nscatangle]
   std::string phasetab_err;                       // negativity failure, raised by render

@@ -315,6 +315,9 @@ struct RayArgs {
   int nrays;
   double tautol, transcut;
   int nosurface;
+  // CORRECTINTERPOLATE (shdomsub2.f:2562): 0 keeps the source and extinction
+  // a ray carries out of one cell as its start in the next
+  int correct_interp = 1;
   float* out;
   int* errflag;
 };

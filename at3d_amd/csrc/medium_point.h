@@ -31,6 +31,7 @@ struct PropGrid {
   float delx, dely;
   double extmin, scatmin;
   int nmicro = 1;
+  bool interp_o = false;   // INTERPMETHOD 'OO'
 };
 
 // The prepared Legendre table: coefficients / (2l+1), delta-M f subtracted
@@ -46,6 +47,10 @@ struct LegTable {
   const float* flat = nullptr;
   int nstleg = 1;
   size_t flat_size = 0;
+  // INTERPMETHOD 'OO': the reference scales the tables themselves by
+  // 1 / (1 - f) (PREPARE_PROP, shdomsub2.f:508-526), which changes the
+  // rounding of RADIANCE_TRUNCATION's weights
+  bool interp_o = false;
 };
 
 // SSORT (shdomsub2.f:4961-5217) with KFLAG = -2: x decreasing, y carried,
@@ -201,6 +206,24 @@ __host__ __device__ inline void prop_point(const PropGrid& g, int ipa, float X, 
   extf = (float)ext;
   albf = ((double)extf > g.extmin) ? (float)(scatter / (double)extf) : (float)(scatter / g.extmin);
   const int nm = g.nmicro, npw = 8 * nm;
+  if (g.interp_o) {
+    // 'OO' (shdom90.f90:198-235, one table per property point): the corners'
+    // tables in corner order, the first replaced by the table of the corner
+    // with the most scattering or the one the point sits on, weights 1, 0..
+    double maxscat = -1.0;
+    int best = IP[I[0]];
+    for (int c = 0; c < 8; c++) {
+      iph_out[c] = IP[I[c]];
+      wt_out[c] = 0.f;
+      if (scat[c] > maxscat || fabs(F[c] - 1.0) < 0.001) {
+        maxscat = scat[c];
+        best = IP[I[c]];
+      }
+    }
+    iph_out[0] = best;
+    wt_out[0] = 1.f;
+    return;
+  }
   for (int c = 0; c < 8; c++)
     for (int m = 0; m < nm; m++) {
       const size_t src = (size_t)I[c] * nm + m;
@@ -305,7 +328,12 @@ __host__ __device__ inline void point_source_constants(const LegTable& t, int np
             F = F + c * w[q];
           }
         }
-        legt = legt * (1.0f / (1 - F));
+        // 'OO' divides by 1 - f in the table (PREPARE_PROP), 'ON' multiplies
+        // by its reciprocal here (shdomsub1.f:1690-1735)
+        if (t.interp_o)
+          legt = legt / (1 - F);
+        else
+          legt = legt * (1.0f / (1 - F));
       }
       rad = rad + W * albs[ipa * stride] * legt;
     }

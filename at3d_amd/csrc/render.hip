@@ -151,6 +151,7 @@ __global__ __launch_bounds__(128) void k_render(RayArgs a) {
   int ngrid = 0;
   bool validrad = false;
   float e8[8], s8[NST][8];
+  float srcext1[NST], ext1 = 0.f;
   while (!validrad && icell >= 0) {
     ngrid++;
     const int* gp = a.g.gridptr + (size_t)icell * 8;
@@ -168,10 +169,13 @@ __global__ __launch_bounds__(128) void k_render(RayArgs a) {
     const double invdelz = 1.0 / delz;
     const double px1 = P[ipt1 * 3], py1 = P[ipt1 * 3 + 1], pz1 = P[ipt1 * 3 + 2];
     double u = (xe - px1) * invdelx, v = (ye - py1) * invdely, w = (ze - pz1) * invdelz;
-    float srcext1[NST];
-    for (int k = 0; k < NST; k++) srcext1[k] = trilin(s8[k], u, v, w);
-    srcext1[0] = fmaxf(0.0f, srcext1[0]);
-    float ext1 = trilin(e8, u, v, w);
+    // the entry values: interpolated in this cell (CORRECTINTERPOLATE, and
+    // always in the first), else those the ray left the last cell with
+    if (a.correct_interp || ngrid == 1) {
+      for (int k = 0; k < NST; k++) srcext1[k] = trilin(s8[k], u, v, w);
+      srcext1[0] = fmaxf(0.0f, srcext1[0]);
+      ext1 = trilin(e8, u, v, w);
+    }
     const int8_t fl = a.g.cellflags[icell];
     const bool ipinx = (fl & 1) && !(a.open_x && ((cx > 0 && xe < xm) || (cx < 0 && xe > xm)));
     const bool ipiny = (fl & 2) && !(a.open_y && ((cy > 0 && ye < ym) || (cy < 0 && ye > ym)));

@@ -299,6 +299,7 @@ static int render_impl(State& s, int64_t nrays, const float* x, const float* y, 
   a.tautol = p.d.tautol;
   a.transcut = p.d.transcut;
   a.nosurface = nosurface;
+  a.correct_interp = s.render_correct_interp;
   a.out = reinterpret_cast<float*>(B + o_out);
   a.errflag = s.errflag;
   (void)hipEventRecord(s.ev_ray[0], s.stream);

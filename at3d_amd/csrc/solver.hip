@@ -407,8 +407,10 @@ static int device_medium(const Plan& p, State& s, std::string& err) {
   TRY(upload(s, zl, p.zlevels.data(), p.zlevels.size(), err));
   TRY(upload(s, leg, p.legen.data(), p.legen.size(), err));
   TRY(upload(s, legp, p.legenp.data(), p.legenp.size(), err));
-  m.g = PropGrid{extp, albp, iphp, wtp, zl, d.npx, d.npy, d.npz, d.delx, d.dely, p.extmin, p.scatmin, p.nmicro};
+  m.g = PropGrid{extp, albp, iphp, wtp, zl, d.npx, d.npy, d.npz, d.delx, d.dely, p.extmin, p.scatmin, p.nmicro,
+                 p.interp_o};
   m.t = LegTable{leg, p.nleg + 1, p.ml, p.legen.size(), p.npw};
+  m.t.interp_o = p.interp_o;
   m.legenp = legp;
   m.nlegp = p.nlegp;
   m.npart = p.npart;
@@ -804,7 +806,8 @@ static int state_replace(State& s, std::shared_ptr<Plan> owned, SplitFields* f, 
   }
   if (f) {
     const size_t np = q.npts, ld = n.ld;
-    if ((int)ld != f->ld || f->S.size() < np * ld || f->ns.size() < np) {
+    if ((int)ld != f->ld || f->S.size() < np * ld || f->ns.size() < np || f->nst != n.nst ||
+        (n.nst == 3 && (f->Squ[1].size() < np * ld || f->Rqu[1].size() < np * ld || f->DSqu[1].size() < np * ld))) {
       state_free(n);
       err = "internal: refined grid fields do not match the new state";
       return AT3D_ERR_ARG;
@@ -823,6 +826,14 @@ static int state_replace(State& s, std::shared_ptr<Plan> owned, SplitFields* f, 
     HIPCHK(hipMemcpyAsync(n.ns, f->ns.data(), np * sizeof(int), hipMemcpyHostToDevice, n.stream));
     HIPCHK(hipMemcpyAsync(n.nr, f->nr.data(), np * sizeof(int), hipMemcpyHostToDevice, n.stream));
     HIPCHK(hipMemcpyAsync(n.nos, f->nos.data(), np * sizeof(int), hipMemcpyHostToDevice, n.stream));
+    if (n.nst == 3)
+      for (int k = 0; k < 2; k++) {
+        const size_t off = (size_t)(k + 1) * n.psh;
+        HIPCHK(hipMemcpyAsync(n.S + off, f->Squ[k].data(), np * ld * sizeof(float), hipMemcpyHostToDevice, n.stream));
+        HIPCHK(hipMemcpyAsync(n.R + off, f->Rqu[k].data(), np * ld * sizeof(float), hipMemcpyHostToDevice, n.stream));
+        HIPCHK(hipMemcpyAsync(n.DS + off, f->DSqu[k].data(), np * ld * sizeof(float), hipMemcpyHostToDevice,
+                              n.stream));
+      }
     HIPCHK(hipMemcpyAsync(n.fluxes, flux.data(), flux.size() * sizeof(float), hipMemcpyHostToDevice, n.stream));
     // this iteration's radiance truncation (RADIANCE_TRUNCATION follows
     // SPLIT_GRID, shdomsub1.f:723-731) with the FIXSH of the last one
@@ -1313,6 +1324,7 @@ static int split_step(State& s, bool dosplit, SplitSchedule& sc, std::string& er
   const size_t np = p.npts, ld = s.ld;
   SplitFields f;
   f.ld = s.ld;
+  f.nst = s.nst;
   f.S.resize(np * ld);
   f.ns.resize(np);
   HIPCHK(hipMemcpyAsync(f.S.data(), s.S, np * ld * sizeof(float), hipMemcpyDeviceToHost, s.stream));
@@ -1326,6 +1338,17 @@ static int split_step(State& s, bool dosplit, SplitSchedule& sc, std::string& er
     HIPCHK(hipMemcpyAsync(f.DS.data(), s.DS, np * ld * sizeof(float), hipMemcpyDeviceToHost, s.stream));
     HIPCHK(hipMemcpyAsync(f.nr.data(), s.nr, np * sizeof(int), hipMemcpyDeviceToHost, s.stream));
     HIPCHK(hipMemcpyAsync(f.nos.data(), s.nos, np * sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    if (s.nst == 3)
+      for (int k = 0; k < 2; k++) {
+        const size_t off = (size_t)(k + 1) * s.psh;   // the Q and U planes
+        f.Squ[k].resize(np * ld);
+        f.Rqu[k].resize(np * ld);
+        f.DSqu[k].resize(np * ld);
+        HIPCHK(hipMemcpyAsync(f.Squ[k].data(), s.S + off, np * ld * sizeof(float), hipMemcpyDeviceToHost, s.stream));
+        HIPCHK(hipMemcpyAsync(f.Rqu[k].data(), s.R + off, np * ld * sizeof(float), hipMemcpyDeviceToHost, s.stream));
+        HIPCHK(hipMemcpyAsync(f.DSqu[k].data(), s.DS + off, np * ld * sizeof(float), hipMemcpyDeviceToHost,
+                              s.stream));
+      }
   }
   HIPCHK(hipStreamSynchronize(s.stream));
   if (!dosplit) {

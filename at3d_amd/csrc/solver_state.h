@@ -127,6 +127,7 @@ struct State {
   IterCtl* ctl = nullptr;      // device
   IterCtl* ctl_h = nullptr;    // pinned host mirror
   int use_iter_graph = 1;      // option "iteration_graph"
+  int render_correct_interp = 1;   // option "render_correct_interpolate"
   int* nr_buf0 = nullptr;      // the first NR buffer (the graphs' parity reference)
   hipGraph_t iter_graph[2] = {};
   hipGraphExec_t iter_exec[2] = {};

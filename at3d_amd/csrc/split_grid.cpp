@@ -504,6 +504,48 @@ void new_point_phase(const Plan& p, const PointOptics& o, std::vector<float>& le
   albedo = (float)alb;
 }
 
+// CALC_SOURCE_PNT (shdomsub1.f:823-910) at a new point with NSTOKES 3: the
+// I, Q and U source terms 0..ns-1 from the radiance terms 0..nr-1 and the
+// solar direct beam, with the point's mixed, delta-M scaled elements a1
+// (legent, from new_point_phase), a2, a3 and b1 (made as COMPUTE_SOURCE
+// mixes them).  Q and U start at the fifth term, as the reference's loops do.
+void new_point_source_pol(const Plan& p, const PointOptics& o, const std::vector<float>& a1, float albedo,
+                          float flux0, int ns, int nr, const float* RI, const float* RQ, const float* RU, float* SI,
+                          float* SQ, float* SU) {
+#pragma clang fp contract(off)
+  const int ML = p.ml, nl = p.nleg + 1;
+  LegTable t1 = plan_leg_table(p);
+  t1.npw = 8;
+  const LegTable te[3] = {LegTable{p.legen2.data(), nl, ML, p.legen2.size(), 8},
+                          LegTable{p.legen3.data(), nl, ML, p.legen3.size(), 8},
+                          LegTable{p.legen5.data(), nl, ML, p.legen5.size(), 8}};
+  std::vector<float> a2(ML + 1), a3(ML + 1), b1(ML + 1);
+  float* el[3] = {a2.data(), a3.data(), b1.data()};
+  for (int e = 0; e < 3; e++)
+    point_element_constants(t1, te[e], p.npart, p.d.deltam != 0, o.ext.data(), o.alb.data(), o.iph.data(),
+                            o.wt.data(), 1, el[e]);
+  for (int j = 0; j < ns; j++) {
+    const int l = p.lofj[j];
+    float sI = 0.f, sQ = 0.f, sU = 0.f;
+    if (j < nr) {
+      sI = a1[l] * RI[j];
+      sI = sI + b1[l] * RQ[j];
+      if (j >= 4) {
+        sQ = b1[l] * RI[j] + a2[l] * RQ[j];
+        sU = a3[l] * RU[j];
+      }
+    }
+    sI = albedo * sI;
+    sQ = albedo * sQ;
+    sU = albedo * sU;
+    sI = sI + flux0 * albedo * a1[l] * p.ylmsun[j];
+    if (j >= 4) sQ = sQ + flux0 * albedo * b1[l] * p.ylmsun[j];
+    SI[j] = sI;
+    SQ[j] = sQ;
+    SU[j] = sU;
+  }
+}
+
 struct Splitter {
   Plan& p;
   SplitFields& f;
@@ -545,6 +587,18 @@ struct Splitter {
         R[j] = 0.5f * (r1 + r2);
       }
       f.nr.push_back(nr);
+      if (f.nst == 3)
+        for (int k = 0; k < 2; k++) {
+          std::vector<float>& Rk = f.Rqu[k];
+          Rk.resize((size_t)(ip + 1) * ld, 0.f);
+          for (int j = 0; j < nr; j++) {
+            const float r1 = j < nr1 ? Rk[(size_t)ip1 * ld + j] : 0.0f;
+            const float r2 = j < nr2 ? Rk[(size_t)ip2 * ld + j] : 0.0f;
+            Rk[(size_t)ip * ld + j] = 0.5f * (r1 + r2);
+          }
+          f.Squ[k].resize((size_t)(ip + 1) * ld, 0.f);
+          f.DSqu[k].resize((size_t)(ip + 1) * ld, 0.f);
+        }
       const int ns = std::max(f.ns[ip1], f.ns[ip2]);
       f.S.resize((size_t)(ip + 1) * ld, 0.f);
       f.DS.resize((size_t)(ip + 1) * ld, 0.f);
@@ -587,6 +641,11 @@ struct Splitter {
       const int ld = f.ld, ns = f.ns[ip], nr = f.nr[ip];
       const float* R = &f.R[(size_t)ip * ld];
       float* S = &f.S[(size_t)ip * ld];
+      if (f.nst == 3) {
+        new_point_source_pol(p, o, legent, albedo, flux0, ns, nr, R, &f.Rqu[0][(size_t)ip * ld],
+                             &f.Rqu[1][(size_t)ip * ld], S, &f.Squ[0][(size_t)ip * ld], &f.Squ[1][(size_t)ip * ld]);
+        return;
+      }
       for (int j = 0; j < ns; j++) {
         const int l = p.lofj[j];
         float s = flux0 * albedo * legent[l] * p.ylmsun[j];

@@ -14,8 +14,12 @@ namespace at3d {
 // reference's SHPTR / RSHPTR / OSHPTR extents).
 struct SplitFields {
   int ld = 0;
-  std::vector<float> S, R, DS;           // [npts][ld] source, radiance, delta-source
+  std::vector<float> S, R, DS;           // [npts][ld] source, radiance, delta-source (I)
   std::vector<int32_t> ns, nr, nos;      // [npts]
+  // polarized solves (NSTOKES 3): the Q and U components, each [npts][ld]
+  // with the counts above
+  int nst = 1;
+  std::vector<float> Squ[2], Rqu[2], DSqu[2];
 };
 
 // The splitting schedule of SOLUTION_ITERATIONS (shdomsub1.f:636-720) across

@@ -30,12 +30,21 @@ class RTE:
     """Radiative transfer solver (at3d/solver.py:143)."""
 
     def __init__(self, numerical_params, medium, source, surface, num_stokes=1, name=None,
-                 atmosphere=None, device_setup=True):
+                 atmosphere=None, device_setup=True, interpmethod="ON"):
         if num_stokes not in (1, 3):
             raise ValueError("num_stokes must be 1 or 3 (NSTOKES = 4 is not supported by this engine)")
         if atmosphere is not None:
             raise ValueError("atmosphere (thermal / gas absorption) is not supported")
         self._nstokes = int(num_stokes)
+        # INTERPMETHOD of the property transfer (solver.py:156): 'ON' mixes the
+        # eight property points' tables, 'OO' takes the one of the point with
+        # the most scattering (the reference's property-file verification)
+        if interpmethod not in ("ON", "OO"):
+            raise ValueError("interpmethod must be 'ON' or 'OO'")
+        self._interpmethod = interpmethod
+        self._phasemax = 0.999
+        # RENDER's CORRECTINTERPOLATE (solver.py:170)
+        self._correctinterpolate = True
         self.numerical_params = dict(get_config()) if numerical_params is None else dict(numerical_params)
         self.source = dict(source)
         self.surface = dict(surface)
@@ -160,6 +169,7 @@ class RTE:
         d.device_medium = 1 if self._device_setup else 0
         d.nstokes = self._nstokes
         d.nmicro = nmicro
+        d.interpmethod = 1 if self._interpmethod == "OO" else 0
         self._keep = [ext, alb, iph, pw, zl, legenp]
         self._desc = d
         L = _lib.lib()
@@ -562,6 +572,7 @@ class RTE:
         if not all(a.size == n for a in (y, z, mu, phi)):
             raise ValueError("ray arrays must have the same length")
         out = np.zeros((n, self._nstokes), np.float32)
+        self.set_option("render_correct_interpolate", 1 if self._correctinterpolate else 0)
         err = _lib.errbuf()
         _lib.check(_lib.lib().at3d_render(self._state, n, x.ctypes.data, y.ctypes.data,
                                           z.ctypes.data, mu.ctypes.data, phi.ctypes.data,

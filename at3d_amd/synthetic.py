@@ -31,6 +31,7 @@ class Workload:
     wavelength: float = 0.86
     deltam: bool = True
     nstokes: int = 1    # 1, or 3 for a polarized solve (legcoef then (6, nleg+1, ntab))
+    interpmethod: str = "ON"   # property transfer: 'ON' mixes tables, 'OO' takes one
 
     @property
     def shape(self):
@@ -197,7 +198,8 @@ def make_rte(w: Workload, device_setup=True, **overrides):
     med = {f"species_{i}": Scatterer(sp[0], sp[1], sp[2], w.delx, w.dely, w.z, *sp[3:5])
            for i, sp in enumerate(w.species)}
     return RTE(cfg, med, source.solar(w.wavelength, w.solarmu, w.solaraz_deg),
-               surface.lambertian(w.gndalbedo), num_stokes=w.nstokes, device_setup=device_setup)
+               surface.lambertian(w.gndalbedo), num_stokes=w.nstokes, device_setup=device_setup,
+               interpmethod=w.interpmethod)
 
 
 # Serving stream: grids, (NMU, NPHI) angle sets (L = NMU - 1) and wavelengths

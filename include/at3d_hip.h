@@ -89,6 +89,13 @@ typedef struct at3d_scene_desc {
    * 8 x nmicro of them (TRILIN_INTERP_PROP, shdom90.f90:17-346).  With more
    * than one, the solve runs on the base grid and has no Levis gradient. */
   int32_t nmicro;
+  /* INTERPMETHOD (TRILIN_INTERP_PROP, shdom90.f90:161-235): 0 'ON' mixes the
+   * eight corners' tables by scattering weight; 1 'OO' takes the table of the
+   * corner with the most scattering (a corner at the point itself wins), as
+   * the reference's verification against property files does
+   * (tests/test_shdom.py:62-170).  'OO' needs nmicro = 1 and has no Levis
+   * gradient. */
+  int32_t interpmethod;
 } at3d_scene_desc;
 
 typedef struct at3d_plan at3d_plan;   /* host-side setup (no GPU needed)     */
@@ -331,6 +338,9 @@ int at3d_levis_gradient_jacobian(at3d_state* st, const at3d_deriv_desc* deriv, i
  * where (radiance, source) pairs fit; grids whose pairs exceed LDS take it by
  * themselves (for tests; rebuilds the layout and drops the captured graphs).
  * "sweep_graph" (0/1): replay the sweep launches from a HIP graph.
+ * "render_correct_interpolate" (0/1, default 1): RENDER's CORRECTINTERPOLATE
+ * (shdomsub2.f:2562); 0 starts each cell after a ray's first with the source
+ * and extinction it left the previous cell with.
  * "iteration_graph" (0/1, default 1): run a solve's iterations from captured
  * HIP graphs of four iterations each under device control (the convergence
  * test, FIXSH and the acceleration on the device; the host reads the control
@@ -351,7 +361,8 @@ int at3d_state_set_option(at3d_state* st, const char* key, int32_t value, char* 
  * (the ray kernel alone); of the last gradient call "gradient" and
  * "adjoint_rays" (the adjoint ray kernel alone).  The sweep's layout: "sweep_ring"
  * (1 when the level-ordered sweep runs), "sweep_ring_lds" (its LDS bytes),
- * "sweep_ring_slots", "sweep_ring4" (1 for 4-byte slots), "sweep_sectors".
+ * "sweep_ring_slots", "sweep_ring4" (1 for 4-byte slots), "sweep_ring_full" (1 when
+ * every column takes the full instantiation), "sweep_sectors".
  * Returns a negative value for unknown keys. */
 double at3d_state_timing(const at3d_state* st, const char* key);
 

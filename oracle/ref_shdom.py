@@ -168,6 +168,11 @@ class Scene:
     transmin: float = 1.0
     ip_flag: int = 0
     nstokes: int = 1                # 1 or 3 (num_stokes)
+    # INTERPMETHOD: 'ON' mixes the phase functions of the eight property
+    # points by scattering weight; 'OO' takes the one with the most scattering
+    # (shdom90.f90:161-235; the reference's property-file verification uses it)
+    interpmethod: str = "ON"
+    correctinterpolate: bool = True   # RENDER's CORRECTINTERPOLATE (shdomsub2.f:2562)
 
     @property
     def shape(self):
@@ -238,12 +243,14 @@ def setup(sc: Scene) -> RefState:
         if max_total_mb * 1024 ** 2 > 0.9 * psutil.virtual_memory().total:
             max_total_mb = 0.9 * psutil.virtual_memory().total / 1024 ** 2
         big_arrays = 3 if sc.accelflag else 2
-        numphase = sum(sp.legcoef.shape[1] for sp in sc.species)
-        max_legendre = max(sp.legcoef.shape[0] for sp in sc.species)   # legendre_index size
+        # table_index and legendre_index sizes (the last two axes of a
+        # scalar or six-element table)
+        numphase = sum(np.shape(sp.legcoef)[-1] for sp in sc.species)
+        max_legendre = max(np.shape(sp.legcoef)[-2] for sp in sc.species)
         memword = nmu * (2 + 2 * nphi + 2 * nlm + 2 * 33 * 32) + 4.5 * maxpg + \
-            2 * numphase * max_legendre
-        wantmem = sc.adapt_grid_factor * nbpts * (28 + 16.5 * sc.cell_to_point_ratio + nphi0max +
-                                                  sc.num_sh_term_factor * nlm * big_arrays)
+            2 * numphase * nstleg * max_legendre
+        wantmem = sc.adapt_grid_factor * nbpts * (28 + 16.5 * sc.cell_to_point_ratio + nphi0max * nstokes +
+                                                  sc.num_sh_term_factor * nstokes * nlm * big_arrays)
         reduce = min(1.0, ((max_total_mb * 1024 ** 2) / 4 - memword) / wantmem)
         adapt = sc.adapt_grid_factor * reduce
         maxig = int(adapt * nbpts)
@@ -340,7 +347,7 @@ def _prepare_optical_properties(st: RefState):
           temp, planck, extinct, albedo, legen, iphase, i4(d["npts"]), d["gridpos"],
           i4(d["npx"]), i4(d["npy"]), i4(d["npz"]), r4(sc.delx), r4(sc.dely), r4(0.0), r4(0.0),
           d["zlevels"], tempp, extinctp, i4(maxpg), albedop, legenp, iphasep, i4(0), zckd, gasabs,
-          i4(npart), total_ext, extmin, scatmin, albmax, i4(d["nstleg"]), s("ON"), ierr, errmsg,
+          i4(npart), total_ext, extmin, scatmin, albmax, i4(d["nstleg"]), s(sc.interpmethod), ierr, errmsg,
           phaseinterpwt, r4(0.999), i4(nlegp), phasewtp, i4(maxnmicro))
     _check(ierr, errmsg, "TRANSFER_PA_TO_GRID")
     d.update(extinctp=extinctp, albedop=albedop, iphasep=iphasep, phasewtp=phasewtp,
@@ -457,7 +464,7 @@ def init_solution(st: RefState, restore=None):
           deljdot, deljold, deljnew, jnorm, i4(d["nbcells"]), ang["fftflag"], d["cmu1"],
           d["cmu2"], ang["wtmu"], d["cphi1"], d["cphi2"], d["wphisave"], i4(maxsfcpars),
           work, work1, work2, i4(nstleg), i4(nstokes), uniform_sfc_brdf, sfc_brdf_do, lg(inradflag), i4(ndel),
-          ierr, errmsg, i4(maxpg), i4(work2_size), d["phaseinterpwt"], r4(0.999), s("ON"),
+          ierr, errmsg, i4(maxpg), i4(work2_size), d["phaseinterpwt"], r4(0.999), s(sc.interpmethod),
           i4(d["nlegp"]), lg(False), i4(d["maxnmicro"]), d["phasewtp"], i4(sc.angle_set),
           lg(True), longest, r4(0.0), surface_rad, volsrc, volsrcgridptr, volsrcshptr, i4(1))
     _check(ierr, errmsg, "INIT_SOLUTION")
@@ -516,7 +523,7 @@ def solution_iterations(st: RefState, maxiter: int, verbose=False):
           r8(d["xdomain"]), r8(d["ydomain"]), r8(d["delxd"]), r8(d["delyd"]), r4(d["albmax"]),
           deljdot, deljold, deljnew, jnorm, d["fftflag"], d["cmu1"], d["cmu2"], d["wtmu"],
           d["cphi1"], d["cphi2"], d["wphisave"], d["work"], d["work1"], d["work2"],
-          lg(False), d["sfc_brdf_do"], i4(sc.iterfixsh), s("ON"), ierr, errmsg, i4(d["maxpg"]),
+          lg(False), d["sfc_brdf_do"], i4(sc.iterfixsh), s(sc.interpmethod), ierr, errmsg, i4(d["maxpg"]),
           d["phaseinterpwt"], r4(0.999), i4(d["nlegp"]), i4(d["maxnmicro"]), d["phasewtp"],
           lg(True), comptime, lg(True), r4(sc.transmin), splitcrit, d["surface_rad"],
           sfcgridrad, d["volsrc"], d["volsrcgridptr"], d["volsrcshptr"], i4(1))
@@ -558,7 +565,7 @@ def split_grid(st: RefState, dosplit: bool, cursplitacc: float, outofmem: bool =
           r8(d["cz"]), r8(d["cxinv"]), r8(d["cyinv"]), r8(d["czinv"]), i4(d["di"]), i4(d["dj"]),
           i4(d["dk"]), i4(d["ipdirect"]), r8(d["delxd"]), r8(d["delyd"]),
           r8(d["xdomain"]), r8(d["ydomain"]), r8(d["epss"]), r8(d["epsz"]), r8(d["uniformzlev"]),
-          i4(d["npart"]), i4(d["maxpg"]), d["total_ext"], s("ON"), ierr, errmsg,
+          i4(d["npart"]), i4(d["maxpg"]), d["total_ext"], s(sc.interpmethod), ierr, errmsg,
           d["phaseinterpwt"], r4(0.999), i4(d["nlegp"]), i4(d["maxnmicro"]),
           d["phasewtp"], i4(1), d["volsrcgridptr"], d["volsrcshptr"], d["volsrc"])
     _check(ierr, errmsg, "SPLIT_GRID")
@@ -616,9 +623,9 @@ def render(st: RefState, x, y, z, mu, phi, single_scatter=False, nosurface=False
           d["legen"], farr(d["iphase"][:, :npts], np.int32), d["dirflux"], d["fluxes"],
           d["shptr"], d["source"], camx, camy, camz, cammu, camphi, i4(npix), i4(d["npart"]),
           farr(d["total_ext"][:npts], np.float32), stokes, i4(d["nscatangle"]), d["ylmsun"],
-          d["phasetab"], i4(min(nstleg, 2)), ierr, errmsg, s("ON"),
+          d["phasetab"], i4(min(nstleg, 2)), ierr, errmsg, s(sc.interpmethod),
           farr(d["phaseinterpwt"][:, :npts], np.float32), r4(0.999), i4(d["maxnmicro"]),
-          r8(sc.tautol), lg(nosurface), lg(True), r8(sc.transcut), lg(single_scatter),
+          r8(sc.tautol), lg(nosurface), lg(sc.correctinterpolate), r8(sc.transcut), lg(single_scatter),
           d["sfcgridrad"], i4(d["nang"]))
     _check(ierr, errmsg, "RENDER")
     return stokes[0].copy() if nstokes == 1 else stokes.copy()
@@ -679,7 +686,7 @@ def scene_from_workload(w, **kw) -> Scene:
                 num_mu_bins=w.nmu, num_phi_bins=w.nphi, solacc=w.solacc,
                 solarmu=w.solarmu, solaraz=float(np.deg2rad(w.solaraz_deg)),
                 gndalbedo=w.gndalbedo, wavelength=w.wavelength, deltam=w.deltam,
-                nstokes=getattr(w, "nstokes", 1))
+                nstokes=getattr(w, "nstokes", 1), interpmethod=getattr(w, "interpmethod", "ON"))
     args.update(kw)
     return Scene(**args)
 
@@ -767,7 +774,7 @@ def prepare_derivatives(st: RefState, derivs):
           i4(d["npart"]), dalb, dext, i4(numder), partder, doexact, i4(d["ml"]),
           lg(sc.deltam), farr(d["albedo"][:npts], np.float32),
           farr(d["iphase"][:, :npts], np.int32), farr(d["phaseinterpwt"][:, :npts], np.float32),
-          r4(0.999), s("ON"), i4(1))
+          r4(0.999), s(sc.interpmethod), i4(1))
     _check(ierr, errmsg, "PREPARE_DERIV_INTERPS")
     longest = d["longest_path_pts"]
     dpath = np.zeros((longest, npts), np.float32, order="F")
@@ -838,7 +845,7 @@ def levis_gradient(st: RefState, x, y, z, mu, phi, measurements, uncertainties=N
           farr(unc, np.float64), jacobian, lg(make_jac), farr(jptr, np.int32), i4(njac),
           farr(rpp, np.int32), farr(rw, np.float64), farr(sw, np.float64),
           s(costfunc, 2), i4(1), i4(1), i4(1), r8(sc.tautol), lg(False), ierr, errmsg,
-          s("ON"), i4(d["maxnmicro"]), d["diphase"], d["iphasep"], d["phasewtp"], d["dphasewt"],
+          s(sc.interpmethod), i4(d["maxnmicro"]), d["diphase"], d["iphasep"], d["phasewtp"], d["dphasewt"],
           farr(d["phaseinterpwt"][:, :npts], np.float32), d["albedop"], d["extinctp"],
           d["optinterpwt"], d["interpptr"], r8(d["extmin"]), r8(d["scatmin"]), d["doexact"],
           farr(d["temp"][:npts], np.float32), r4(0.999), np.zeros((maxpg, numder), np.float32, order="F"),

@@ -18,12 +18,15 @@ bool put(const std::vector<T>& v, T* dst, int64_t cap) {
 }
 }  // namespace
 
-extern "C" int split_hook(const at3d_scene_desc* desc, int ld, const float* S, const int32_t* ns, const float* R,
-                          const int32_t* nr, int ncall, const float* cursplitacc, int outofmem_in,
-                          int64_t cap_pts, int64_t cap_cells, int32_t* sizes, float* gridpos, int32_t* gridptr,
-                          int32_t* neighptr, int32_t* treeptr, int8_t* cellflags, float* total_ext,
-                          float* extinct, float* albedo, float* S_out, int32_t* ns_out, float* R_out,
-                          int32_t* nr_out, float* splitcrit, int32_t* sweepord, char* err, size_t errlen) {
+// Polarized fields (NSTOKES 3): SQU / RQU hold the Q then the U component,
+// each [npts][ld] in and [cap_pts][ld] out; null for a scalar scene.
+extern "C" int split_hook_pol(const at3d_scene_desc* desc, int ld, const float* S, const int32_t* ns, const float* R,
+                              const int32_t* nr, const float* SQU, const float* RQU, int ncall,
+                              const float* cursplitacc, int outofmem_in, int64_t cap_pts, int64_t cap_cells,
+                              int32_t* sizes, float* gridpos, int32_t* gridptr, int32_t* neighptr, int32_t* treeptr,
+                              int8_t* cellflags, float* total_ext, float* extinct, float* albedo, float* S_out,
+                              int32_t* ns_out, float* R_out, int32_t* nr_out, float* SQU_out, float* RQU_out,
+                              float* splitcrit, int32_t* sweepord, char* err, size_t errlen) {
   std::string msg;
   at3d::Plan p;
   int rc = at3d::build_plan(*desc, p, msg);
@@ -38,6 +41,14 @@ extern "C" int split_hook(const at3d_scene_desc* desc, int ld, const float* S, c
     f.ns.assign(ns, ns + n);
     f.nr.assign(nr, nr + n);
     f.nos.assign(n, 0);
+    if (SQU) {
+      f.nst = 3;
+      for (int k = 0; k < 2; k++) {
+        f.Squ[k].assign(SQU + k * n * ld, SQU + (k + 1) * n * ld);
+        f.Rqu[k].assign(RQU + k * n * ld, RQU + (k + 1) * n * ld);
+        f.DSqu[k].assign(n * ld, 0.f);
+      }
+    }
   }
   bool outofmem = outofmem_in != 0;
   for (int k = 0; k < ncall && !rc; k++) rc = at3d::split_grid(p, f, true, outofmem, cursplitacc[k], splitcrit[k], msg);
@@ -51,6 +62,9 @@ extern "C" int split_hook(const at3d_scene_desc* desc, int ld, const float* S, c
               put(p.extinct, extinct, cap_pts * p.npart) && put(p.albedo, albedo, cap_pts * p.npart) &&
               put(f.S, S_out, cap_pts * ld) && put(f.ns, ns_out, cap_pts) && put(f.R, R_out, cap_pts * ld) &&
               put(f.nr, nr_out, cap_pts);
+    for (int k = 0; ok && SQU && k < 2; k++)
+      ok = put(f.Squ[k], SQU_out + k * cap_pts * ld, cap_pts * ld) &&
+           put(f.Rqu[k], RQU_out + k * cap_pts * ld, cap_pts * ld);
     if (!ok) {
       rc = 1;
       msg = "output capacity too small";
@@ -70,4 +84,15 @@ extern "C" int split_hook(const at3d_scene_desc* desc, int ld, const float* S, c
     err[errlen - 1] = 0;
   }
   return rc;
+}
+
+extern "C" int split_hook(const at3d_scene_desc* desc, int ld, const float* S, const int32_t* ns, const float* R,
+                          const int32_t* nr, int ncall, const float* cursplitacc, int outofmem_in,
+                          int64_t cap_pts, int64_t cap_cells, int32_t* sizes, float* gridpos, int32_t* gridptr,
+                          int32_t* neighptr, int32_t* treeptr, int8_t* cellflags, float* total_ext,
+                          float* extinct, float* albedo, float* S_out, int32_t* ns_out, float* R_out,
+                          int32_t* nr_out, float* splitcrit, int32_t* sweepord, char* err, size_t errlen) {
+  return split_hook_pol(desc, ld, S, ns, R, nr, nullptr, nullptr, ncall, cursplitacc, outofmem_in, cap_pts, cap_cells,
+                        sizes, gridpos, gridptr, neighptr, treeptr, cellflags, total_ext, extinct, albedo, S_out,
+                        ns_out, R_out, nr_out, nullptr, nullptr, splitcrit, sweepord, err, errlen);
 }

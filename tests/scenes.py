@@ -63,6 +63,24 @@ def micro_blob(bc="open", nmicro=2, seed=0, nmu=8, nphi=16):
                       nmu=nmu, nphi=nphi, solacc=1e-5)
 
 
+def oo_blob(bc="open", nstokes=1, seed=0, nmu=8, nphi=16):
+    """INTERPMETHOD 'OO': the 8^3 blob with one of three tables per property
+    point (seeded), HG g = 0.8 / 0.85 / 0.9 (64 orders) or, polarized, three
+    six-element cloud tables; every grid point takes the table of the
+    property point with the most scattering among its eight."""
+    w = blob8(bc, seed=seed, nmu=nmu, nphi=nphi)
+    e, a, _ = w.species[0]
+    rng = np.random.default_rng(seed + 11)
+    if nstokes == 1:
+        tables = np.concatenate([S.hg_legendre(g, 64) for g in (0.8, 0.85, 0.9)], axis=1)
+    else:
+        tables = np.concatenate([S.polarized_cloud_table(g, 64, seed=seed + k)
+                                 for k, g in enumerate((0.8, 0.85, 0.9))], axis=2)
+    ti = rng.integers(1, 4, size=e.shape).astype(np.int32)
+    return S.Workload("oo_blob", w.delx, w.dely, w.z, [(e, a, tables, ti)], bc=bc, nmu=nmu, nphi=nphi,
+                      solacc=1e-5, nstokes=nstokes, interpmethod="OO")
+
+
 def rel_l2(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)

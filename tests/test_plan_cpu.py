@@ -8,7 +8,7 @@ import pytest
 
 from at3d_amd import _lib
 from at3d_amd import synthetic as S
-from tests.scenes import blob8, cloud_small, micro_blob, pol_blob
+from tests.scenes import blob8, cloud_small, micro_blob, oo_blob, pol_blob
 
 
 def test_library_exports_every_declared_symbol():
@@ -34,9 +34,9 @@ def _pair(ref, w):
 
 @pytest.mark.parametrize("make", [lambda: blob8("periodic"), lambda: blob8("open"), cloud_small,
                                   lambda: micro_blob("open"), lambda: micro_blob("periodic", nmicro=3),
-                                  lambda: pol_blob()],
+                                  lambda: pol_blob(), lambda: oo_blob("open"), lambda: oo_blob("periodic", nstokes=3)],
                          ids=["blob8_periodic", "blob8_open", "cloud12_two_species", "micro2_open",
-                              "micro3_periodic", "polarized_blob8"])
+                              "micro3_periodic", "polarized_blob8", "interp_oo_open", "interp_oo_polarized"])
 def test_setup_bit_exact(ref, make):
     w = make()
     rte, st = _pair(ref, w)

@@ -94,7 +94,5 @@ def test_prepared_phase_matrix_elements():
 
 def test_polarized_options_checked():
     w = pol_blob()
-    with pytest.raises(Exception, match="split_accuracy"):
-        S.make_rte(w, split_accuracy=0.03)
     with pytest.raises(ValueError):
         S.make_rte(S.Workload("v", w.delx, w.dely, w.z, w.species, bc=w.bc, nstokes=4))

@@ -20,7 +20,7 @@ import numpy as np
 import pytest
 
 from at3d_amd import synthetic as S
-from tests.scenes import blob8, cloud_small
+from tests.scenes import blob8, cloud_small, oo_blob, pol_blob
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIBDIR = os.path.join(ROOT, "at3d_amd")
@@ -52,9 +52,10 @@ def _uniform_slab(n=6, nz=8):
                       nmu=8, nphi=16, solacc=1e-5)
 
 
-def _dense(src, ptr, n, ld):
+def _dense(src, ptr, n, ld, k=0):
+    """Stokes component k of a ragged (nstokes, maxiv) array as dense rows."""
     cnt = np.diff(np.asarray(ptr[:n + 1])).astype(np.int32)
-    flat = np.ravel(src, order="F")
+    flat = np.asarray(src)[k]
     out = np.zeros((n, ld), np.float32)
     for p in range(n):
         out[p, :cnt[p]] = flat[ptr[p]:ptr[p] + cnt[p]]
@@ -66,6 +67,9 @@ CASES = {
     "blob8_periodic": (lambda: blob8("periodic"), 0.03, [0.3, 0.05]),
     "cloud12_two_species": (cloud_small, 0.01, [0.2, 0.05, 0.01]),
     "uniform_slab_ties": (_uniform_slab, 0.01, [0.05, 0.01]),
+    "polarized_blob8_open": (lambda: pol_blob(), 0.03, [0.3, 0.1, 0.03]),
+    "polarized_blob8_periodic_oo": (lambda: oo_blob("periodic", nstokes=3), 0.03, [0.3, 0.05]),
+    "interp_oo_open": (lambda: oo_blob("open"), 0.03, [0.3, 0.1, 0.03]),
 }
 
 
@@ -84,8 +88,13 @@ def test_host_splitting_matches_reference(ref, hook, name):
     d = st.d
     n0, nlm = d["npts"], d["nlm"]
     ld = (nlm + 63) // 64 * 64
+    pol = d["nstokes"] == 3
     S0, ns0 = _dense(d["source"], d["shptr"], n0, ld)
     R0, nr0 = _dense(d["radiance"], d["rshptr"], n0, ld)
+    SQU0 = RQU0 = None
+    if pol:
+        SQU0 = np.stack([_dense(d["source"], d["shptr"], n0, ld, k)[0] for k in (1, 2)])
+        RQU0 = np.stack([_dense(d["radiance"], d["rshptr"], n0, ld, k)[0] for k in (1, 2)])
     crit_ref, oom = [], False
     for a in accs:
         c, oom = ref.split_grid(st, True, a, oom)
@@ -110,10 +119,14 @@ def test_host_splitting_matches_reference(ref, hook, name):
     A = np.asarray(accs, np.float32)
     err = C.create_string_buffer(512)
     ptr = lambda a: C.c_void_p(a.ctypes.data)
-    rc = hook.split_hook(C.byref(desc), C.c_int(ld), ptr(S0), ptr(ns0), ptr(R0), ptr(nr0), C.c_int(len(accs)),
-                         ptr(A), C.c_int(0), C.c_int64(cap_p), C.c_int64(cap_c), ptr(sizes), ptr(gp), ptr(gptr),
-                         ptr(nptr), ptr(tptr), ptr(cf), ptr(te), ptr(ex), ptr(al), ptr(So), ptr(nso), ptr(Ro),
-                         ptr(nro), ptr(crit), ptr(swo), err, C.c_size_t(512))
+    SQUo = np.zeros((2, cap_p, ld), np.float32)
+    RQUo = np.zeros((2, cap_p, ld), np.float32)
+    opt = lambda a: None if a is None else ptr(a)
+    rc = hook.split_hook_pol(C.byref(desc), C.c_int(ld), ptr(S0), ptr(ns0), ptr(R0), ptr(nr0), opt(SQU0), opt(RQU0),
+                             C.c_int(len(accs)), ptr(A), C.c_int(0), C.c_int64(cap_p), C.c_int64(cap_c), ptr(sizes),
+                             ptr(gp), ptr(gptr), ptr(nptr), ptr(tptr), ptr(cf), ptr(te), ptr(ex), ptr(al), ptr(So),
+                             ptr(nso), ptr(Ro), ptr(nro), ptr(SQUo) if pol else None, ptr(RQUo) if pol else None,
+                             ptr(crit), ptr(swo), err, C.c_size_t(512))
     assert rc == 0, err.value
     assert (sizes[0], sizes[1], bool(sizes[2])) == (N, NC, oom)
     assert np.array_equal(crit, np.float32(crit_ref))
@@ -131,6 +144,12 @@ def test_host_splitting_matches_reference(ref, hook, name):
     R1, nr1 = _dense(d["radiance"], d["rshptr"], N, ld)
     assert np.array_equal(nso[:N], ns1) and np.array_equal(nro[:N], nr1)
     assert np.array_equal(So[:N], S1) and np.array_equal(Ro[:N], R1)
+    if pol:
+        for k in (1, 2):
+            Sk, _ = _dense(d["source"], d["shptr"], N, ld, k)
+            Rk, _ = _dense(d["radiance"], d["rshptr"], N, ld, k)
+            assert np.array_equal(SQUo[k - 1, :N], Sk), k
+            assert np.array_equal(RQUo[k - 1, :N], Rk), k
     # the refined grid's sweep order
     ref.solution_iterations(st, st.iters + 1)
     assert st.npts == N

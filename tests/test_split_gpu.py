@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from at3d_amd import synthetic as S
-from tests.scenes import blob8, cloud_small, ortho_rays, interior_rays, rel_l2
+from tests.scenes import blob8, cloud_small, interior_rays, oo_blob, ortho_rays, pol_blob, rel_l2
 
 pytestmark = [pytest.mark.gpu, pytest.mark.filterwarnings("ignore:RTE is not solved")]
 
@@ -109,3 +109,38 @@ def test_refined_solution_saves_and_loads(ref):
     small = S.make_rte(w, split_accuracy=-1.0)      # no room beyond the base grid
     with pytest.raises(SHDOMError, match="Cannot load solution"):
         small.load_solution(ds)
+
+
+POL_SCENES = {"pol_blob8_open": (lambda: pol_blob(), 0.03),
+              "pol_blob8_periodic_oo": (lambda: oo_blob("periodic", nstokes=3), 0.03)}
+
+
+@pytest.mark.parametrize("name", list(POL_SCENES))
+def test_polarized_split_solve_and_render(ref, name):
+    """Polarized (NSTOKES 3) solves with splitting: the same refined grid as
+    the reference, converged I, Q, U sources and radiances (rel L2 <= 1e-4;
+    Q and U to 1e-4 of the I field's norm) and rendered I, Q, U."""
+    make, acc = POL_SCENES[name]
+    w = make()
+    st = ref.setup(ref.scene_from_workload(w, splitacc=acc))
+    ref.init_solution(st)
+    ref.solution_iterations(st, 100)
+    rte = S.make_rte(w, split_accuracy=acc)
+    rte.solve(100, verbose=False)
+    assert st.npts > st.nbpts
+    _same_grid(st, rte)
+    assert abs(rte.num_iterations - st.iters) <= 1
+    n, nlm = st.npts, st.nlm
+    for name_, ptr, arr in (("source", st.shptr, st.source), ("radiance", st.rshptr, st.radiance)):
+        got = rte.state_array(name_)
+        want = [ref.dense_sh(ptr, arr, n, nlm, k) for k in range(3)]
+        assert rel_l2(got[0], want[0]) < 1e-4, name_
+        for k in (1, 2):
+            assert np.linalg.norm(got[k] - want[k]) < 1e-4 * np.linalg.norm(want[0]), (name_, k)
+    rays = ortho_rays(w, [(0.0, 0.0), (45.0, 30.0)])
+    rays["stokes"] = ["I", "Q", "U"]
+    out = rte.integrate_to_sensor(dict(rays))
+    want = ref.render(st, rays["ray_x"], rays["ray_y"], rays["ray_z"], rays["ray_mu"], rays["ray_phi"])
+    assert rel_l2(out["I"], want[0]) < 1e-4
+    for k, key in ((1, "Q"), (2, "U")):
+        assert np.max(np.abs(out[key] - want[k])) <= 1e-4 * np.max(np.abs(want[0])) + 1e-3 * np.max(np.abs(want[k]))
