@@ -131,38 +131,38 @@ static int check_ptr(const int32_t* ptr, int npts, int nmax, int nmin, const cha
 int state_load(State& s, const int32_t* shptr, const float* source, const int32_t* rshptr,
                const float* radiance, const float* fluxes, std::string& err) {
   const Plan& p = *s.plan;
-  const int npts = p.npts;
-  if (s.nst != 1) {
-    err = "saved solutions and spherical-harmonic diagnostics are for NSTOKES = 1 solves";
-    return AT3D_ERR_ARG;
-  }
+  const int npts = p.npts, nst = s.nst;
   {
     int rc = check_ptr(shptr, npts, p.nlm, 0, "SHPTR", err);
     if (!rc) rc = check_ptr(rshptr, npts, p.nlm, 1, "RSHPTR", err);
     if (rc) return rc;
   }
+  // the Stokes components one after another: source [nst][SHPTR(npts)],
+  // radiance [nst][RSHPTR(npts)]
   const size_t ns_tot = (size_t)shptr[npts], nr_tot = (size_t)rshptr[npts];
-  const size_t bytes = (ns_tot + nr_tot + 2 * ((size_t)npts + 1)) * 4 + 1024;
+  const size_t bytes = ((ns_tot + nr_tot) * nst + 2 * ((size_t)npts + 1)) * 4 + 1024;
   void* tmp = nullptr;
   HIPCHK(dev_alloc(&tmp, bytes));
   char* B = static_cast<char*>(tmp);
   float* d_src = reinterpret_cast<float*>(B);
-  float* d_rad = d_src + ns_tot;
-  int32_t* d_sp = reinterpret_cast<int32_t*>(d_rad + nr_tot);
+  float* d_rad = d_src + ns_tot * nst;
+  int32_t* d_sp = reinterpret_cast<int32_t*>(d_rad + nr_tot * nst);
   int32_t* d_rp = d_sp + npts + 1;
   int rc = AT3D_OK;
   auto run = [&]() -> int {
     HIPCHK(hipStreamSynchronize(s.stream));
-    if (ns_tot) HIPCHK(hipMemcpyAsync(d_src, source, ns_tot * 4, hipMemcpyHostToDevice, s.stream));
-    HIPCHK(hipMemcpyAsync(d_rad, radiance, nr_tot * 4, hipMemcpyHostToDevice, s.stream));
+    if (ns_tot) HIPCHK(hipMemcpyAsync(d_src, source, ns_tot * nst * 4, hipMemcpyHostToDevice, s.stream));
+    HIPCHK(hipMemcpyAsync(d_rad, radiance, nr_tot * nst * 4, hipMemcpyHostToDevice, s.stream));
     HIPCHK(hipMemcpyAsync(d_sp, shptr, (npts + 1) * 4, hipMemcpyHostToDevice, s.stream));
     HIPCHK(hipMemcpyAsync(d_rp, rshptr, (npts + 1) * 4, hipMemcpyHostToDevice, s.stream));
     const size_t n = (size_t)s.nptsp * s.ld;
     const int nb = (int)((n + 255) / 256);
-    k_expand_sh<<<nb, 256, 0, s.stream>>>(d_src, d_sp, npts, s.nptsp, s.ld, s.S, s.ns);
-    k_expand_sh<<<nb, 256, 0, s.stream>>>(d_rad, d_rp, npts, s.nptsp, s.ld, s.R, s.nr);
+    for (int k = 0; k < nst; k++) {
+      k_expand_sh<<<nb, 256, 0, s.stream>>>(d_src + ns_tot * k, d_sp, npts, s.nptsp, s.ld, s.S + s.psh * k, s.ns);
+      k_expand_sh<<<nb, 256, 0, s.stream>>>(d_rad + nr_tot * k, d_rp, npts, s.nptsp, s.ld, s.R + s.psh * k, s.nr);
+    }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemsetAsync(s.DS, 0, n * sizeof(float), s.stream));
+    HIPCHK(hipMemsetAsync(s.DS, 0, s.psh * nst * sizeof(float), s.stream));
     HIPCHK(hipMemcpyAsync(s.nos, s.ns, npts * sizeof(int), hipMemcpyDeviceToDevice, s.stream));
     HIPCHK(hipMemcpyAsync(s.fluxes, fluxes, 2 * (size_t)npts * 4, hipMemcpyHostToDevice, s.stream));
     HIPCHK(hipStreamSynchronize(s.stream));
@@ -185,11 +185,7 @@ int state_load(State& s, const int32_t* shptr, const float* source, const int32_
 int state_save(State& s, int32_t* shptr, int32_t* rshptr, float* source, int64_t source_cap,
                float* radiance, int64_t radiance_cap, float* fluxes, std::string& err) {
   const Plan& p = *s.plan;
-  const int npts = p.npts;
-  if (s.nst != 1) {
-    err = "saved solutions and spherical-harmonic diagnostics are for NSTOKES = 1 solves";
-    return AT3D_ERR_ARG;
-  }
+  const int npts = p.npts, nst = s.nst;
   if (!s.initialized) {
     err = "no solution to save: initialise or solve first";
     return AT3D_ERR_ARG;
@@ -210,15 +206,8 @@ int state_save(State& s, int32_t* shptr, int32_t* rshptr, float* source, int64_t
     HIPCHK(hipMemcpyAsync(fluxes, s.fluxes, 2 * (size_t)npts * 4, hipMemcpyDeviceToHost, s.stream));
     HIPCHK(hipStreamSynchronize(s.stream));
   }
-  auto pack = [&](const float* dense, const int32_t* ptr, float* out, int64_t cap,
-                  const char* what) -> int {
-    if (!out) return AT3D_OK;
-    if (cap < ptr[npts]) {
-      err = std::string(what) + ": output holds " + std::to_string(cap) + " terms, the solution " +
-            std::to_string(ptr[npts]);
-      return AT3D_ERR_ARG;
-    }
-    // rows in blocks of 4096 points
+  // one component's dense rows into its ragged terms, in blocks of 4096 points
+  auto pack_plane = [&](const float* dense, const int32_t* ptr, float* out) -> int {
     const int blk = 4096;
     std::vector<float> rows((size_t)blk * s.ld);
     for (int i0 = 0; i0 < npts; i0 += blk) {
@@ -234,6 +223,21 @@ int state_save(State& s, int32_t* shptr, int32_t* rshptr, float* source, int64_t
     }
     return AT3D_OK;
   };
+  // the Stokes components one after another, each ptr[npts] terms
+  auto pack = [&](const float* dense, const int32_t* ptr, float* out, int64_t cap,
+                  const char* what) -> int {
+    if (!out) return AT3D_OK;
+    if (cap < (int64_t)ptr[npts] * nst) {
+      err = std::string(what) + ": output holds " + std::to_string(cap) + " terms, the solution " +
+            std::to_string((int64_t)ptr[npts] * nst);
+      return AT3D_ERR_ARG;
+    }
+    for (int k = 0; k < nst; k++) {
+      const int rc = pack_plane(dense + s.psh * k, ptr, out + (size_t)ptr[npts] * k);
+      if (rc) return rc;
+    }
+    return AT3D_OK;
+  };
   int rc = pack(s.S, shptr, source, source_cap, "SOURCE");
   if (!rc) rc = pack(s.R, rshptr, radiance, radiance_cap, "RADIANCE");
   return rc;
@@ -245,10 +249,8 @@ int state_save(State& s, int32_t* shptr, int32_t* rshptr, float* source, int64_t
 int state_sh_terms(State& s, int nshout, float* shterms, float* netfluxdiv, std::string& err) {
   const Plan& p = *s.plan;
   const int npts = p.npts;
-  if (s.nst != 1) {
-    err = "saved solutions and spherical-harmonic diagnostics are for NSTOKES = 1 solves";
-    return AT3D_ERR_ARG;
-  }
+  // (a polarized solve: the I terms, RADIANCE(1,*) of COMPUTE_SH and
+  // COMPUTE_NETFLUXDIV)
   if (!s.initialized) {
     err = "no solution: initialise or solve first";
     return AT3D_ERR_ARG;

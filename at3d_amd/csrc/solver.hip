@@ -868,10 +868,6 @@ int state_reset_grid(State& s, std::string& err) {
 int state_set_grid(State& s, int npts, int ncells, const float* gridpos, const int32_t* gridptr,
                    const int32_t* neighptr, const int32_t* treeptr, const int8_t* cellflags, std::string& err) {
   const Plan& b = *s.base_plan;
-  if (s.nst != 1) {
-    err = "a polarized solve stays on its base grid";
-    return AT3D_ERR_ARG;
-  }
   bool base = npts == b.npts && ncells == b.ncells;
   for (size_t i = 0; base && i < (size_t)ncells * 2; i++) base = treeptr[i] == -1;
   if (base) {

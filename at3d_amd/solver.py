@@ -371,8 +371,9 @@ class RTE:
             L, err = _lib.lib(), _lib.errbuf()
             _lib.check(L.at3d_state_save(self._state, shptr.ctypes.data, rshptr.ctypes.data, None, 0,
                                          None, 0, None, err, len(err)), err)
-            source = np.zeros((1, int(shptr[npts])), np.float32, order="F")
-            radiance = np.zeros((1, int(rshptr[npts])), np.float32, order="F")
+            # (nstokes, terms): the library writes the components one after another
+            source = np.zeros((self._nstokes, int(shptr[npts])), np.float32)
+            radiance = np.zeros((self._nstokes, int(rshptr[npts])), np.float32)
             _lib.check(L.at3d_state_save(self._state, shptr.ctypes.data, rshptr.ctypes.data,
                                          source.ctypes.data, source.size, radiance.ctypes.data,
                                          radiance.size, fluxes.ctypes.data, err, len(err)), err)
@@ -471,11 +472,15 @@ class RTE:
         rshptr = np.zeros(npts + 2, np.int32)
         rshptr[:npts + 1] = d["rshptr"][:npts + 1]
         rshptr[npts + 1] = rshptr[npts]
-        source = _lib.as_f32(np.ravel(d["source"])[:int(shptr[npts])])
-        radiance = _lib.as_f32(np.ravel(d["radiance"])[:int(rshptr[npts])])
-        fluxes = _lib.as_f32(np.asarray(d["fluxes"])[:, :npts])
-        if source.size < shptr[npts] or radiance.size < rshptr[npts]:
+        nst = self._nstokes
+        src = np.asarray(d["source"]).reshape(nst, -1)
+        rad = np.asarray(d["radiance"]).reshape(nst, -1)
+        if src.shape[1] < shptr[npts] or rad.shape[1] < rshptr[npts]:
             raise ValueError("loaded source / radiance hold fewer terms than their pointers give")
+        # the Stokes components one after another
+        source = _lib.as_f32(src[:, :int(shptr[npts])].ravel())
+        radiance = _lib.as_f32(rad[:, :int(rshptr[npts])].ravel())
+        fluxes = _lib.as_f32(np.asarray(d["fluxes"])[:, :npts])
         err = _lib.errbuf()
         _lib.check(_lib.lib().at3d_state_load(self._state, shptr.ctypes.data, source.ctypes.data,
                                               rshptr.ctypes.data, radiance.ctypes.data,

@@ -237,7 +237,10 @@ int at3d_state_set_grid(at3d_state* st, int32_t npts, int32_t ncells, const floa
  * source pass, DELSOURCE zero; the next at3d_solve continues from it.  Term
  * counts must be 0..nlm (source) and 1..nlm (radiance) per point.
  * at3d_state_save writes shptr and rshptr always, and source / radiance /
- * fluxes when non-null (capacities in floats; too small is AT3D_ERR_ARG). */
+ * fluxes when non-null (capacities in floats; too small is AT3D_ERR_ARG).
+ * A polarized state (NSTOKES 3) holds the I, Q and U terms one component
+ * after another: source[3][shptr[npts]], radiance[3][rshptr[npts]] (the
+ * reference's SOURCE(NSTOKES, *) transposed). */
 int at3d_state_load(at3d_state* st, const int32_t* shptr, const float* source,
                     const int32_t* rshptr, const float* radiance, const float* fluxes, char* err,
                     size_t errlen);

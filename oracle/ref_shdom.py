@@ -647,7 +647,7 @@ def compute_sh(st: RefState, nshout=4):
     d, sc = st.d, st.sc
     npts = d["npts"]
     shterms = np.zeros((nshout, npts), np.float32, order="F")
-    fcall("COMPUTE_SH", i4(nshout), i4(1), i4(npts), s("S"), r4(sc.solarmu), r4(sc.solaraz),
+    fcall("COMPUTE_SH", i4(nshout), i4(d["nstokes"]), i4(npts), s("S"), r4(sc.solarmu), r4(sc.solaraz),
           farr(d["dirflux"][:npts], np.float32), farr(d["rshptr"][:npts + 1], np.int32),
           i4(d["ml"]), i4(d["mm"]), d["radiance"], shterms)
     return shterms
@@ -658,7 +658,7 @@ def compute_netfluxdiv(st: RefState):
     d, sc = st.d, st.sc
     npts, npart = d["npts"], d["npart"]
     out = np.zeros(npts, np.float32)
-    fcall("COMPUTE_NETFLUXDIV", i4(1), i4(npts), farr(d["rshptr"][:npts + 1], np.int32), s("S"),
+    fcall("COMPUTE_NETFLUXDIV", i4(d["nstokes"]), i4(npts), farr(d["rshptr"][:npts + 1], np.int32), s("S"),
           r4(sc.solarmu), farr(d["extinct"][:npts], np.float32), farr(d["albedo"][:npts], np.float32),
           farr(d["planck"][:npts], np.float32), farr(d["dirflux"][:npts], np.float32), d["radiance"],
           out, i4(npart))

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from at3d_amd import synthetic as S
-from tests.scenes import blob8, cloud_l15, ortho_rays, rel_l2
+from tests.scenes import blob8, cloud_l15, ortho_rays, pol_blob, rel_l2
 
 pytestmark = [pytest.mark.gpu, pytest.mark.filterwarnings("ignore:RTE is not solved")]
 
@@ -124,3 +124,53 @@ def test_engine_saved_solution_matches_reference_and_round_trips(ref, tmp_path):
             assert np.array_equal(b.state_array(name), rte.state_array(name)), name
         assert np.array_equal(b.integrate_to_sensor(dict(rays))["I"], img)
         b.release()
+
+
+def test_polarized_diagnostics_and_saved_solutions(ref, tmp_path):
+    """NSTOKES 3: COMPUTE_SH and COMPUTE_NETFLUXDIV of the I terms against the
+    reference; the reference's saved I, Q, U solution loads and renders its
+    I, Q, U pixels; this engine's saved solution has the reference's pointers
+    and terms (I rel L2 <= 1e-4, Q and U to 1e-4 of the I norm) and
+    save -> load -> render is bit-identical."""
+    w = pol_blob()
+    st, rte = _converged(ref, w)
+    got = rte.spherical_harmonics
+    want = ref.compute_sh(st, 4)
+    for k, key in enumerate(("mean_intensity", "Fx", "Fy", "Fz")):
+        assert rel_l2(got[key].ravel(), want[k]) < 1e-4, key
+    nfd = rte.net_flux_divergence["net_flux_div"].ravel()
+    assert rel_l2(nfd, ref.compute_netfluxdiv(st)) < 1e-4
+    ds = rte.save_solution()
+    wsave = ref.save_solution(st)
+    assert np.array_equal(ds["shptr"], wsave["shptr"]) and np.array_equal(ds["rshptr"], wsave["rshptr"])
+    assert ds["source"].shape == wsave["source"].shape == (3, wsave["shptr"][-1])
+    for key in ("source", "radiance"):
+        assert rel_l2(ds[key][0], wsave[key][0]) < 1e-4, key
+        for k in (1, 2):
+            assert np.linalg.norm(ds[key][k] - wsave[key][k]) < 1e-4 * np.linalg.norm(wsave[key][0]), (key, k)
+    rays = _rays(w)
+    rays["stokes"] = ["I", "Q", "U"]
+    img = rte.integrate_to_sensor(dict(rays))
+    path = tmp_path / "pol.npz"
+    np.savez(path, **ds)
+    b = S.make_rte(w)
+    b.load_solution(str(path))
+    b.solve(0, verbose=False)
+    for name in ("source", "radiance", "fluxes"):
+        assert np.array_equal(b.state_array(name), rte.state_array(name)), name
+    out = b.integrate_to_sensor(dict(rays))
+    for key in ("I", "Q", "U"):
+        assert np.array_equal(out[key], img[key]), key
+    b.release()
+    # the reference's own saved solution renders its pixels
+    c = S.make_rte(w)
+    d2 = c.save_solution(save_radiances=False)
+    d2.update(wsave)
+    c.load_solution(d2)
+    c.solve(0, verbose=False)
+    ref_img = _ref_render(ref, st, rays)
+    out = c.integrate_to_sensor(dict(rays))
+    assert rel_l2(out["I"], ref_img[0]) < 1e-4
+    for k, key in ((1, "Q"), (2, "U")):
+        assert np.max(np.abs(out[key] - ref_img[k])) <= 1e-4 * np.max(np.abs(ref_img[0])) + \
+            1e-3 * np.max(np.abs(ref_img[k])), key

@@ -144,3 +144,14 @@ def test_polarized_split_solve_and_render(ref, name):
     assert rel_l2(out["I"], want[0]) < 1e-4
     for k, key in ((1, "Q"), (2, "U")):
         assert np.max(np.abs(out[key] - want[k])) <= 1e-4 * np.max(np.abs(want[0])) + 1e-3 * np.max(np.abs(want[k]))
+    # the refined polarized solution saves and loads (grid, I, Q, U) exactly
+    ds = rte.save_solution()
+    b = S.make_rte(w, split_accuracy=acc)
+    b.load_solution(ds)
+    b.solve(0, verbose=False)
+    assert b.state_size("npts") == n
+    for key in ("source", "radiance"):
+        assert np.array_equal(b.state_array(key), rte.state_array(key)), key
+    out2 = b.integrate_to_sensor(dict(rays))
+    for key in ("I", "Q", "U"):
+        assert np.array_equal(out2[key], out[key]), key
