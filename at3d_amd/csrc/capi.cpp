@@ -208,7 +208,7 @@ int at3d_plan_get(const at3d_plan* h, const char* name, void* dst, size_t nbytes
     for (const at3d::Ordinate& od : p.ords) used[od.g.ioct] = true;
     at3d::RingPlan rp;
     std::string msg;
-    if (!at3d::build_ring_plan(p, used, 160 * 1024, rp, msg)) {
+    if (!at3d::build_ring_plan(p, used, at3d::GFX950_LDS_BYTES, rp, msg)) {
       set_err(err, errlen, msg);
       return AT3D_ERR_NUMERIC;
     }

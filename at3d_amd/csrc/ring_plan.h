@@ -30,6 +30,10 @@ struct RingOctant {
 };
 
 constexpr int RING_WINDOW = 3;    // earlier levels a one-cell step may read through the ring
+// LDS a gfx950 workgroup may allocate (hipDeviceAttributeMaxSharedMemoryPerBlock
+// there): the budget of host-only layout queries made without a device; the
+// solver state takes the device's own attribute
+constexpr size_t GFX950_LDS_BYTES = 160 * 1024;
 
 struct RingPlan {
   RingOctant oct[8];

@@ -255,17 +255,26 @@ def polarized_leg(cpu_runs):
     render_s = time.perf_counter() - t0
     out = {"workload": "128x128x64 open cloud, NSTOKES 3, NMU16 NPHI32 L15, solacc 1e-4, "
                        "3 wavelengths (0.47 0.66 0.86 um), 9 views of 128x128 I/Q/U",
-           "npts": rtes[0].plan_size("npts"), "sweep": "level-ordered" if rtes[0].timing("sweep_ring") else
-           "point-ordered (the ring needs {:.0f} KB of LDS)".format(rtes[0].timing("sweep_ring_lds") / 1024),
+           "npts": rtes[0].plan_size("npts"),
+           "sweep": ("level-ordered, ring of radiances ({:.0f} KB of LDS)" if rtes[0].timing("sweep_ring4") else
+                     "level-ordered ({:.0f} KB of LDS)").format(rtes[0].timing("sweep_ring_lds") / 1024)
+           if rtes[0].timing("sweep_ring") else "point-ordered",
            "iterations": [r.num_iterations for r in rtes], "solve_wall_s": wall,
            "solves_per_s": len(ws) / wall, "render_wall_s": render_s,
            "rays": int(rays["ray_x"].size) * len(ws),
            "device_GB_per_state": rtes[0].timing("device_bytes") / 1e9,
-           "kernel_ms_per_solve": {k: float(np.mean([r.timing(k) for r in rtes]))
-                                   for k in ("sh_to_do", "sweep", "do_to_sh", "source", "total")},
+
            "mean_QoverI": float(np.mean([np.abs(o["Q"]).mean() / o["I"].mean() for o in outs]))}
     for r in rtes:
         r.release()
+    # kernel times of one more solve with its iterations launched one by one
+    # (graph-replayed iterations carry no timing events)
+    r = S.make_rte(ws[1], **kw)
+    r.set_option("iteration_graph", 0)
+    r.solve(ws[1].maxiter, verbose=False)
+    out["kernel_ms_per_iteration"] = {k: r.timing(k) / max(r.num_iterations, 1)
+                                      for k in ("sh_to_do", "sweep", "sweep_kernel", "do_to_sh", "source", "total")}
+    r.release()
     if cpu_runs > 0:
         try:
             from oracle import ref_shdom as ref
