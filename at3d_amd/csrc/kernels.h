@@ -136,6 +136,9 @@ struct SweepArgs {
   // the ring holds radiances alone (4-byte slots) and the face sources are
   // read from the ring-ordered source row
   int ring4 = 0;
+  // k_sweep_ring: columns from this one on run their Q and U workgroups only
+  // (blockIdx.y >= 1); the I workgroup returns at once
+  int qu_only_from = 1 << 30;
 };
 
 // The fixed part of one ring entry of one ordinate, 16 bytes: head = point |

@@ -1006,11 +1006,23 @@ static void enqueue_sweeps(State& s, const int* stop = nullptr) {
     }
   };
   bool fast = false, full = false;
+  // With NSTOKES 3 only the I sweep of the upwelling hemisphere waits for the
+  // surface (Q and U see a zero boundary), so the first launch also runs the
+  // upwelling Q and U columns: 5 of the 6 (hemisphere, component) sweeps in
+  // one grid, which fills the CUs better than two grids of three
+  const bool merge_qu = ring && s.nst == 3;
   if (ring) {
     // the first launch is timed from the event the caller records just
     // before this function, the second to the one just after it
-    needs(0, s.ndown, fast, full);
-    launch_sweep_ring(s.stream, a, 0, s.ndown, s.ring_lds, s.ring_lds_full, fast, full);
+    if (merge_qu) {
+      SweepArgs a1 = a;
+      a1.qu_only_from = s.ndown;
+      needs(0, p.nang, fast, full);
+      launch_sweep_ring(s.stream, a1, 0, p.nang, s.ring_lds, s.ring_lds_full, fast, full);
+    } else {
+      needs(0, s.ndown, fast, full);
+      launch_sweep_ring(s.stream, a, 0, s.ndown, s.ring_lds, s.ring_lds_full, fast, full);
+    }
     (void)hipEventRecord(s.ev_sweep[1], s.stream);
   } else {
     launch_sweep_preset(s.stream, a, false, 0, s.ndown);
@@ -1022,7 +1034,9 @@ static void enqueue_sweeps(State& s, const int* stop = nullptr) {
   if (ring) {
     needs(s.ndown, p.nang, fast, full);
     (void)hipEventRecord(s.ev_sweep[2], s.stream);
-    launch_sweep_ring(s.stream, a, s.ndown, p.nang - s.ndown, s.ring_lds, s.ring_lds_full, fast, full);
+    SweepArgs a2 = a;
+    if (merge_qu) a2.nst = 1;   // the upwelling I columns alone
+    launch_sweep_ring(s.stream, a2, s.ndown, p.nang - s.ndown, s.ring_lds, s.ring_lds_full, fast, full);
   } else {
     launch_sweep_preset(s.stream, a, true, s.ndown, p.nang - s.ndown);
     launch_sweep_columns(s.stream, a, s.ndown, p.nang - s.ndown);

@@ -475,6 +475,7 @@ __global__ __launch_bounds__(RING_THREADS) void k_sweep_ring(SweepArgs a, int co
   extern __shared__ float4 lds4[];
   const int col = col0 + blockIdx.x;
   if (FULL != (a.ring_full || a.col_general[col] != 0)) return;
+  if (blockIdx.y == 0 && col >= a.qu_only_from) return;
   // Stokes component of this workgroup: Q and U (kst 1, 2) see zero boundary
   // radiances and keep the sign of the cell source (BACK_INT_GRID3D clamps
   // only I, shdomsub1.f:3640)
