@@ -619,7 +619,7 @@ int at3d_state_set_option(at3d_state* st, const char* key, int32_t value, char* 
     return AT3D_OK;
   }
   if (k == "sweep_ring") {
-    st->s.use_ring = value ? 1 : 0;
+    st->s.use_ring = value < 0 ? 0 : (value > 2 ? 2 : value);
     return AT3D_OK;
   }
   if (k == "render_correct_interpolate") {
@@ -659,7 +659,8 @@ double at3d_state_timing(const at3d_state* st, const char* key) {
   if (k == "sweep_launches") return s.nsweep_launch;
   if (k == "sweep_kernel") return s.tsum[6];
   if (k == "device_bytes") return (double)s.device_bytes;
-  if (k == "sweep_ring") return (s.ring_ok && s.use_ring && !s.force_general) ? 1.0 : 0.0;
+  if (k == "sweep_ring")
+    return (s.ring_ok && s.use_ring && !s.force_general && (s.use_ring == 2 || !s.ring_chain)) ? 1.0 : 0.0;
   if (k == "sweep_ring_lds") return (double)(s.ring_full ? s.ring_lds_full : s.ring_lds);
   if (k == "sweep_ring_slots") return s.slots;
   if (k == "sweep_ring4") return s.ring4;

@@ -166,7 +166,13 @@ static SweepArgs sweep_args(const State& s) {
   return a;
 }
 
-static bool ring_active(const State& s) { return s.ring_ok && s.use_ring && !s.force_general; }
+// The level-ordered sweep, where its rings fit, unless its levels are chains
+// of a point or two (periodic grids, whose multi-cell traces depend on the
+// points swept just before): there the point-ordered sweep's lighter levels
+// are faster (16^3 periodic, NMU 8: 22 against 45 ms per iteration).
+static bool ring_active(const State& s) {
+  return s.ring_ok && s.use_ring && !s.force_general && (s.use_ring == 2 || !s.ring_chain);
+}
 
 // The level-ordered sweep: each octant's ring order (its levels, with the
 // hemisphere's boundary points as entries of their own), the SH->DO column
@@ -194,6 +200,9 @@ static int setup_ring(const Plan& p, State& s, std::string& err) {
     if (!build_ring_plan(p, used, (size_t)maxlds, rp, err, cls, 4)) return AT3D_ERR_NUMERIC;
   }
   const float* total_ext = p.medium_on_device ? s.medium_ext_h.data() : p.total_ext.data();
+  s.ring_chain = false;
+  for (int o = 0; o < 8; o++)
+    if (used[o] && rp.oct[o].nlev > 0 && (double)p.npts / rp.oct[o].nlev < 4.0) s.ring_chain = true;
   s.sector_points = rp.sector_points;
   s.slots = rp.slots;
   s.gld = rp.nlev_max + 2;

@@ -86,7 +86,8 @@ struct State {
   int8_t* col_oct = nullptr;
   // level-ordered sweep (k_sweep_ring)
   bool ring_ok = false;        // ring fits in LDS and records are built
-  int use_ring = 1;            // option: sweep with k_sweep_ring when ring_ok
+  int use_ring = 1;            // option: sweep with k_sweep_ring when ring_ok (1: unless ring_chain, 2: always)
+  bool ring_chain = false;     // the levels are chains (under 4 entries per level in some octant)
   int slots = 0, sector_points = 8, gld = 0, ngeneral = 0, nsectors = 0;
   int ring_nlev[8] = {}, sec_base[8] = {};
   size_t ring_lds = 0;         // dynamic LDS bytes of k_sweep_ring's fast instantiation (0: unused)

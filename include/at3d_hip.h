@@ -334,9 +334,11 @@ int at3d_levis_gradient_jacobian(at3d_state* st, const at3d_deriv_desc* deriv, i
 /* Engine options of a state.  "sweep_general" (0/1): route every grid point
  * through the general multi-cell characteristic trace of the point-ordered
  * sweep (bit-identical to its analytic one-cell path; for tests).
- * "sweep_ring" (0/1, default 1): use the level-ordered sweep with per-solve
+ * "sweep_ring" (0/1/2, default 1): use the level-ordered sweep with per-solve
  * step records and an LDS ring when the ring fits (else the point-ordered
- * sweep).  "sweep_ring4" (0/1, default 0): lay the ring out with 4-byte
+ * sweep); 1 leaves grids whose dependency levels are chains (under four
+ * points a level, as periodic grids have) to the point-ordered sweep, which
+ * is faster there; 2 takes the ring wherever it fits.  "sweep_ring4" (0/1, default 0): lay the ring out with 4-byte
  * slots (radiances alone; the sources are read from the state's arrays) even
  * where (radiance, source) pairs fit; grids whose pairs exceed LDS take it by
  * themselves (for tests; rebuilds the layout and drops the captured graphs).

@@ -159,8 +159,10 @@ def test_polarized_ring_of_radiances(bc):
     and U after three iterations (same cell integrals from the same values)."""
     w = pol_blob(bc=bc)
     a = S.make_rte(w)
+    a.set_option("sweep_ring", 2)
     a.solve(3, verbose=False)
     b = S.make_rte(w)
+    b.set_option("sweep_ring", 2)
     b.set_option("sweep_ring4", 1)
     b.solve(3, verbose=False)
     assert a.timing("sweep_ring") == 1.0 and b.timing("sweep_ring4") == 1.0

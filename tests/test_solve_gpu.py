@@ -150,6 +150,7 @@ def test_level_ordered_sweep_matches_point_ordered_sweep(make):
     96x96x48 cloud its step classes from device memory."""
     w = make()
     a = S.make_rte(w)
+    a.set_option("sweep_ring", 2)    # also where the levels are chains (periodic)
     a.solve(3, verbose=False)
     assert a.timing("sweep_ring") == 1.0
     if a.plan_size("npts") > 400000:
@@ -178,8 +179,10 @@ def test_ring_of_radiances_matches_ring_of_pairs(make):
     instantiation with its step classes in device memory."""
     w = make()
     a = S.make_rte(w)
+    a.set_option("sweep_ring", 2)
     a.solve(3, verbose=False)
     b = S.make_rte(w)
+    b.set_option("sweep_ring", 2)
     b.set_option("sweep_ring4", 1)
     b.solve(3, verbose=False)
     assert a.timing("sweep_ring4") == 0.0 and b.timing("sweep_ring4") == 1.0
@@ -187,6 +190,20 @@ def test_ring_of_radiances_matches_ring_of_pairs(make):
     assert b.timing("sweep_ring_lds") < a.timing("sweep_ring_lds")
     for name in ("do_radiance", "source", "radiance"):
         assert rel_l2(a.state_array(name), b.state_array(name)) < 1e-7, name
+
+
+def test_chain_levels_take_the_point_ordered_sweep():
+    """Periodic grids, whose dependency levels are chains of a point or two,
+    sweep point by point by default (faster there); open grids keep the
+    level-ordered ring; option 2 forces the ring."""
+    a = S.make_rte(blob8("periodic"))
+    a.solve(2, verbose=False)
+    assert a.timing("sweep_ring") == 0.0
+    a.set_option("sweep_ring", 2)
+    assert a.timing("sweep_ring") == 1.0
+    b = S.make_rte(blob8("open"))
+    b.solve(2, verbose=False)
+    assert b.timing("sweep_ring") == 1.0
 
 
 def test_sweep_graph_replay_bit_identical_to_direct_launches():
