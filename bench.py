@@ -391,8 +391,9 @@ def main():
     ap.add_argument("--gradient-reps", type=int, default=3)
     ap.add_argument("--polarized", type=int, default=1, help="run the polarized (NSTOKES 3) leg")
     ap.add_argument("--workers", type=int, default=4)
-    ap.add_argument("--stream-requests", type=int, default=48,
-                    help="requests of the mixed-size serving stream (0: skip it)")
+    ap.add_argument("--stream-requests", type=int, default=512,
+                    help="requests of the mixed-size serving stream (0: skip it; 512 is the "
+                         "stream's stated size)")
     ap.add_argument("--stream-rate", type=float, default=40.0, help="its Poisson arrival rate (1/s)")
     ap.add_argument("--stream-workers", type=int, default=8)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
