@@ -274,6 +274,14 @@ def polarized_leg(cpu_runs):
     r.solve(ws[1].maxiter, verbose=False)
     out["kernel_ms_per_iteration"] = {k: r.timing(k) / max(r.num_iterations, 1)
                                       for k in ("sh_to_do", "sweep", "sweep_kernel", "do_to_sh", "source", "total")}
+    # the sweep against the HBM roof: DO source and extinction read, radiance
+    # written, 12 B per point, ordinate and Stokes component, per iteration
+    sweep_bytes = float(r.plan_size("npts")) * r.plan_size("nang") * 3 * 12
+    sk = out["kernel_ms_per_iteration"]["sweep_kernel"]
+    if sk > 0:
+        ach = sweep_bytes / (sk * 1e-3) / 1e9
+        out["sweep_roofline"] = {"bound": "hbm", "achieved": ach, "peak": 8000.0, "unit": "GB/s",
+                                 "frac": ach / 8000.0, "bytes_per_iteration": sweep_bytes}
     r.release()
     if cpu_runs > 0:
         try:
