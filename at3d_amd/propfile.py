@@ -1,5 +1,6 @@
-"""SHDOM property files: the tabulated ('T') and polarized tabulated ('P')
-formats of READ_PROPERTY_SIZE / READ_PROPERTIES (src/polarized/shdomsub3.f:286-366,
+"""SHDOM property files: the four formats of READ_PROPERTY_SIZE /
+READ_PROPERTIES (the standard one, extinction only 'E', tabulated 'T' and
+polarized tabulated 'P') (src/polarized/shdomsub3.f:286-366,
 :367-560), which the reference reaches through ``at3d.core`` (the verification
 of tests/test_shdom.py:62-170 reads tests/data/rico32x36x26w672.prp this way).
 
@@ -92,13 +93,15 @@ def read_property_file(path: str, nleg: int = 0) -> PropertyFile:
     number of Legendre terms kept (the solver's NLEG)."""
     r = _Records(path)
     proptype = r.first_char()
-    if proptype not in ("T", "P"):
-        raise ValueError("only the tabulated ('T') and polarized tabulated ('P') property file "
-                         "formats are read; this file starts with {!r}".format(proptype))
+    if proptype not in ("T", "P", "E"):
+        r._i = 0            # the standard format has no format line
+        return _read_standard(r, nleg)
     npx, npy, npz = (int(v) for v in r.read(3))
     hdr = r.read(2 + npz)
     delx, dely = float(np.float32(hdr[0])), float(np.float32(hdr[1]))
     zlevels = np.array(hdr[2:], np.float32)
+    if proptype == "E":
+        return _read_extinction_only(r, npx, npy, npz, delx, dely, zlevels, nleg)
     numphase = int(r.read(1)[0])
     if numphase < 1:
         raise ValueError("property file has no phase functions")
@@ -152,3 +155,101 @@ def read_property_file(path: str, nleg: int = 0) -> PropertyFile:
         raise ValueError("property file leaves grid points without a valid phase function")
     return PropertyFile(proptype, npx, npy, npz, delx, dely, zlevels, tempp, extinctp, albedop, iphasep,
                         legenp, numphase, nl, maxasym)
+
+
+def _read_extinction_only(r, npx, npy, npz, delx, dely, zlevels, nleg):
+    """Format 'E': one temperature per level, then ALB NUML CHI(1..NUML) for
+    the one phase function, then IX IY IZ EXT per point (IX IZ EXT when
+    NPY = 1).  NLEG grows to NUML; as READ_PROPERTIES does
+    (shdomsub3.f:430-465), coefficients past NUML are zeroed from index NUML
+    on, so with NUML < NLEG the last coefficient read is zeroed too."""
+    tz = np.array(r.read(npz), np.float32)
+    head = r.read(2)
+    alb, numl = np.float32(head[0]), int(head[1])
+    r._i -= 1
+    chi = np.array(r.read(2 + numl)[2:], np.float32)
+    nl = max(int(nleg), numl)
+    leg = np.zeros(nl + 1, np.float32)
+    leg[0] = 1.0
+    leg[1:1 + min(numl, nl)] = chi[:min(numl, nl)]
+    for l in range(numl + 1, nl + 1):      # LEGENP(L) = 0 for L = NUML+1 .. NLEG (1-based)
+        leg[l - 1] = 0.0
+    maxasym = float(leg[1] / np.float32(3.0)) if nl >= 1 else -1.0
+    shape = (npx, npy, npz)
+    tempp = np.zeros(shape, np.float32)
+    extinctp = np.zeros(shape, np.float32)
+    albedop = np.zeros(shape, np.float32)
+    iphasep = np.zeros(shape, np.int32)
+    while True:
+        rec = r.read(3 if npy == 1 else 4)
+        if rec is None:
+            break
+        if npy == 1:
+            ix, iy, iz, ext = int(rec[0]), 1, int(rec[1]), rec[2]
+        else:
+            ix, iy, iz, ext = int(rec[0]), int(rec[1]), int(rec[2]), rec[3]
+        if 1 <= ix <= npx and 1 <= iy <= npy and 1 <= iz <= npz:
+            tempp[ix - 1, iy - 1, iz - 1] = tz[iz - 1]
+            extinctp[ix - 1, iy - 1, iz - 1] = np.float32(ext)
+            albedop[ix - 1, iy - 1, iz - 1] = alb
+            iphasep[ix - 1, iy - 1, iz - 1] = 1
+    if np.any(iphasep < 1):
+        raise ValueError("property file leaves grid points without properties")
+    return PropertyFile("E", npx, npy, npz, delx, dely, zlevels, tempp, extinctp, albedop, iphasep,
+                        leg[:, None], 1, nl, maxasym)
+
+
+def _read_standard(r, nleg):
+    """The standard format: IX IY IZ TEMP EXT ALB NUML CHI(1..NUML) per point,
+    each point its own phase function (READ_PROPERTIES' NUMPHASE = 0; here
+    one table per grid point, table_index = the point's index + 1, in the
+    reference's point order IZ + NPZ (IY-1) + NPZ NPY (IX-1)).  The rows are
+    the ones the reference's reader leaves: CHI(L) as term L - 1, term 0
+    then set to 1 (so CHI(1) is dropped)."""
+    npx, npy, npz = (int(v) for v in r.read(3))
+    hdr = r.read(2 + npz)
+    delx, dely = float(np.float32(hdr[0])), float(np.float32(hdr[1]))
+    zlevels = np.array(hdr[2:], np.float32)
+    recs = []
+    while True:
+        start = r._i
+        head = r.read(7)
+        if head is None:
+            break
+        numl = int(head[6])
+        r._i = start
+        recs.append(r.read(7 + numl))
+    nl = max([int(nleg)] + [int(x[6]) for x in recs])
+    shape = (npx, npy, npz)
+    n = npx * npy * npz
+    tempp = np.zeros(shape, np.float32)
+    extinctp = np.zeros(shape, np.float32)
+    albedop = np.zeros(shape, np.float32)
+    iphasep = np.zeros(shape, np.int32)
+    legenp = np.zeros((nl + 1, n), np.float32)
+    maxasym = -1.0
+    for x in recs:
+        ix, iy, iz = int(x[0]), int(x[1]), int(x[2])
+        if not (1 <= ix <= npx and 1 <= iy <= npy and 1 <= iz <= npz):
+            continue
+        k = iz + npz * (iy - 1) + npz * npy * (ix - 1) - 1
+        tempp[ix - 1, iy - 1, iz - 1] = np.float32(x[3])
+        extinctp[ix - 1, iy - 1, iz - 1] = np.float32(x[4])
+        albedop[ix - 1, iy - 1, iz - 1] = np.float32(x[5])
+        iphasep[ix - 1, iy - 1, iz - 1] = k + 1
+        numl = int(x[6])
+        chi = np.array(x[7:7 + min(numl, nl)], np.float32)
+        # READ_PROPERTIES stores CHI(L) at LEGENP(L + KL), whose slot L + KL
+        # is term L - 1 of the point's row, then sets term 0 to 1 and zeroes
+        # LEGENP(L + KL) for L = NUML+1 .. NLEG (shdomsub3.f:536-548); the
+        # same rows here, so a file reads as the reference reads it
+        legenp[:, k] = 0.0
+        legenp[:chi.size, k] = chi
+        for l in range(numl + 1, nl + 1):
+            legenp[l - 1, k] = 0.0
+        maxasym = max(maxasym, float(legenp[1, k] / np.float32(3.0)))
+        legenp[0, k] = 1.0
+    if np.any(iphasep < 1):
+        raise ValueError("property file leaves grid points without properties")
+    return PropertyFile("S", npx, npy, npz, delx, dely, zlevels, tempp, extinctp, albedop, iphasep,
+                        legenp, n, nl, maxasym)

@@ -84,3 +84,61 @@ def test_tabulated_property_file_matches_read_properties(tmp_path):
     if len(str(path)) > 80:
         pytest.skip("temporary path longer than the reference's CHARACTER*80")
     _compare(read_property_file(str(path), nleg=8), _reference_read(path, 1, 8), 1)
+
+
+def _write(path, lines):
+    path.write_text("\n".join(lines) + "\n")
+    if len(str(path)) > 80:
+        pytest.skip("temporary path longer than the reference's CHARACTER*80")
+
+
+def _compare_loose(pf, want):
+    """'E' and standard formats: grid arrays and the tables as the reference
+    leaves them."""
+    assert pf.nleg == want["nleg"]
+    assert pf.delx == want["delx"] and pf.dely == want["dely"]
+    assert np.array_equal(pf.zlevels, want["zlevels"])
+    for got, key in ((pf.tempp, "tempp"), (pf.extinctp, "ext"), (pf.albedop, "alb")):
+        assert np.array_equal(got, want[key]), key
+    assert np.float32(pf.maxasym) == np.float32(want["maxasym"])
+
+
+@pytest.mark.parametrize("numl", [3, 12])
+def test_extinction_only_property_file_matches_read_properties(tmp_path, numl):
+    rng = np.random.default_rng(numl)
+    chi = np.round(rng.uniform(0, 2, numl), 5)
+    lines = ["E", "3 2 4", "0.05 0.04 0.0 0.5 1.0 1.5", "290.0 280.0 270.0 260.0",
+             "0.99 {} ".format(numl) + " ".join(f"{v:.5f}" for v in chi)]
+    for ix in range(1, 4):
+        for iy in range(1, 3):
+            for iz in range(1, 5):
+                lines.append(f"{ix} {iy} {iz} {rng.uniform(0, 20):.4f}")
+    path = tmp_path / "e.prp"
+    _write(path, lines)
+    want = _reference_read(path, 1, 8)
+    pf = read_property_file(str(path), nleg=8)
+    _compare_loose(pf, want)
+    assert np.array_equal(pf.legenp[:, 0], want["legenp"][:pf.nleg + 1])
+    assert np.all(pf.iphasep == 1)
+
+
+def test_standard_property_file_matches_read_properties(tmp_path):
+    rng = np.random.default_rng(7)
+    lines = ["2 2 3", "0.05 0.05 0.0 0.5 1.0"]
+    numls = {}
+    for ix in range(1, 3):
+        for iy in range(1, 3):
+            for iz in range(1, 4):
+                numl = int(rng.integers(2, 12))
+                chi = np.round(rng.uniform(0, 2, numl), 5)
+                numls[(ix, iy, iz)] = numl
+                lines.append(f"{ix} {iy} {iz} 280.0 {rng.uniform(0, 20):.4f} {rng.uniform(0.5, 1):.5f} {numl} "
+                             + " ".join(f"{v:.5f}" for v in chi))
+    path = tmp_path / "s.prp"
+    _write(path, lines)
+    want = _reference_read(path, 1, 8)
+    pf = read_property_file(str(path), nleg=8)
+    _compare_loose(pf, want)
+    n = pf.npx * pf.npy * pf.npz
+    ref_leg = want["legenp"][:n * (want["nleg"] + 1)].reshape(n, want["nleg"] + 1)
+    assert np.array_equal(pf.legenp.T, ref_leg)
